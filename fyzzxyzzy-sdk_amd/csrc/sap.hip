@@ -1,0 +1,673 @@
+// SAP broadphase on gfx950: box bounds + 32-bit endpoint keys, stable
+// endpoint radix sort per axis, x-axis sweep for full rebuilds and an
+// endpoint-crossing scan for incremental found/lost pair reports.
+//
+// Semantics follow the reference NativeCpu SAP (paths relative to the
+// reference root):
+//   bounds and key encoding        src/native/scene.cpp:25-35, 1233-1316
+//   endpoint order                 src/native/scene.cpp:1341-1346
+//   pair filter and sweep          src/native/scene.cpp:2597-2612, 2642-2703
+//   found/lost publication         src/native/scene.cpp:3245-3296
+//
+// Device layout (HBM, per broadphase object of n boxes):
+//   planes[6][n]       current encoded keys (min x,y,z then max x,y,z)
+//   rec_cur[n]         32-B record: keys[6], body id, dynamic<<31 | stamp
+//   rec_prev[n]        the record before this frame's update (moved boxes)
+//   sorted[2][3][2n]   (key, payload) per axis, current and previous frame;
+//                      payload = slot << 1 | is_max
+//   pairs[2][cap]      sorted u64 (lo slot << 32 | hi slot)
+// Boxes are identified by slot (0..n-1, increasing shape id), so slot
+// order is shape-id order and every tie-break matches the reference.
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fyzz_hip.h"
+#include "context.hpp"
+#include "radix.hpp"
+#include "select.hpp"
+
+namespace fyzz_hip::sap {
+
+constexpr u64 kFoundBit = 1ull << 63;
+constexpr u64 kPairMask = ~kFoundBit;
+
+struct alignas(32) Rec {
+    u32 k[8];
+};
+
+__device__ inline u32 encode_float(float v) {
+    const u32 bits = __float_as_uint(v);
+    return (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u);
+}
+__device__ inline u32 encode_min(float v) { return ((encode_float(v) >> 4) - 1u) << 4; }
+__device__ inline u32 encode_max(float v) { return (((encode_float(v) >> 4) + 1u) << 4) | 4u; }
+
+// World AABB keys of a box (reference scene.cpp:1233-1316, box branch):
+// extent = sum |R e_i| h_i unless the orientation is exactly identity,
+// plus the contact offset. Same operation order; contraction is off.
+__device__ inline bool box_keys(const float* pos, const float* quat, const float* half,
+                                const float* offset, u32 i, u32 out[6]) {
+    const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    const float hx = half[3 * i], hy = half[3 * i + 1], hz = half[3 * i + 2];
+    const float qx = quat[4 * i], qy = quat[4 * i + 1], qz = quat[4 * i + 2], qw = quat[4 * i + 3];
+    float ex = hx, ey = hy, ez = hz;
+    if (qx != 0.0f || qy != 0.0f || qz != 0.0f || qw != 1.0f) {
+        // rotate(q, e_k): t = 2 (u x e_k); r = e_k + w t + u x t.
+        auto rot = [&](float vx, float vy, float vz, float& rx, float& ry, float& rz) {
+            const float cx = qy * vz - qz * vy, cy = qz * vx - qx * vz, cz = qx * vy - qy * vx;
+            const float tx = cx * 2.0f, ty = cy * 2.0f, tz = cz * 2.0f;
+            const float dx = qy * tz - qz * ty, dy = qz * tx - qx * tz, dz = qx * ty - qy * tx;
+            rx = vx + qw * tx + dx;
+            ry = vy + qw * ty + dy;
+            rz = vz + qw * tz + dz;
+        };
+        float xx, xy, xz, yx, yy, yz, zx, zy, zz;
+        rot(1.0f, 0.0f, 0.0f, xx, xy, xz);
+        rot(0.0f, 1.0f, 0.0f, yx, yy, yz);
+        rot(0.0f, 0.0f, 1.0f, zx, zy, zz);
+        ex = fabsf(xx) * hx + fabsf(yx) * hy + fabsf(zx) * hz;
+        ey = fabsf(xy) * hx + fabsf(yy) * hy + fabsf(zy) * hz;
+        ez = fabsf(xz) * hx + fabsf(yz) * hy + fabsf(zz) * hz;
+    }
+    const float c = offset[i];
+    ex += c;
+    ey += c;
+    ez += c;
+    const float b[6] = {px - ex, py - ey, pz - ez, px + ex, py + ey, pz + ez};
+    bool finite = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) finite &= isfinite(b[k]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        out[k] = encode_min(b[k]);
+        out[k + 3] = encode_max(b[k + 3]);
+    }
+    return finite;
+}
+
+// Counters of one broadphase object (device words).
+enum Counter : u32 {
+    kMoved = 0,
+    kDelta,
+    kUnique,
+    kFound,
+    kLost,
+    kPairs0,
+    kPairs1,
+    kKept,
+    kError,     // non-finite bounds seen
+    kOverflow,  // a capacity was exceeded
+    kSweep,
+    kZero,      // always 0 (count of an empty list)
+    kCounters
+};
+
+__global__ void k_fill(u32* p, u32 n, u32 v) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_bounds_planes(u32 n, const float* pos, const float* quat,
+                                                       const float* half, const float* off,
+                                                       u32* planes, u32* err) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u32 k[6];
+    if (!box_keys(pos, quat, half, off, i, k)) atomicOr(err, 1u);
+#pragma unroll
+    for (int a = 0; a < 6; ++a) planes[static_cast<std::size_t>(a) * n + i] = k[a];
+}
+
+// Full (re)build: every box gets its keys; no moved list.
+__global__ __launch_bounds__(256) void k_bounds_full(u32 n, const float* pos, const float* quat,
+                                                     const float* half, const float* off,
+                                                     u32* planes, Rec* cur, u32 stamp, u32* cnt) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u32 k[6];
+    if (!box_keys(pos, quat, half, off, i, k)) atomicOr(&cnt[kError], 1u);
+    Rec r = cur[i];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        r.k[a] = k[a];
+        planes[static_cast<std::size_t>(a) * n + i] = k[a];
+    }
+    r.k[7] = (r.k[7] & 0x80000000u) | stamp;
+    cur[i] = r;
+}
+
+// Incremental: recompute the listed boxes (or all when dirty == null);
+// boxes whose keys changed save their previous record, take this frame's
+// stamp and join the moved list. A slot listed twice is processed once.
+__global__ __launch_bounds__(256) void k_bounds_update(u32 count, const u32* dirty, u32 n,
+                                                       const float* pos, const float* quat,
+                                                       const float* half, const float* off,
+                                                       u32* planes, Rec* cur, Rec* prev,
+                                                       u32* seen, u32 stamp, u32* moved, u32* cnt) {
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const u32 i = dirty ? dirty[t] : t;
+    if (i >= n) {
+        atomicOr(&cnt[kError], 2u);
+        return;
+    }
+    if (dirty && atomicExch(&seen[i], stamp) == stamp) return;
+    u32 k[6];
+    if (!box_keys(pos, quat, half, off, i, k)) atomicOr(&cnt[kError], 1u);
+    Rec r = cur[i];
+    bool changed = false;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) changed |= r.k[a] != k[a];
+    if (!changed) return;
+    prev[i] = r;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        r.k[a] = k[a];
+        planes[static_cast<std::size_t>(a) * n + i] = k[a];
+    }
+    r.k[7] = (r.k[7] & 0x80000000u) | stamp;
+    cur[i] = r;
+    const u32 slot = atomicAdd(&cnt[kMoved], 1u);
+    if (slot < n) moved[slot] = i;
+}
+
+// Inclusive 3-axis overlap of encoded keys (scene.cpp:2606-2612).
+__device__ inline bool overlap3(const Rec& a, const Rec& b) {
+    return !(a.k[0] > b.k[3] || b.k[0] > a.k[3] || a.k[1] > b.k[4] || b.k[1] > a.k[4] ||
+             a.k[2] > b.k[5] || b.k[2] > a.k[5]);
+}
+__device__ inline bool pair_allowed(const Rec& a, const Rec& b) {
+    return a.k[6] != b.k[6] && ((a.k[7] | b.k[7]) & 0x80000000u);
+}
+__device__ inline u64 pair_key(u32 a, u32 b) {
+    const u32 lo = a < b ? a : b, hi = a < b ? b : a;
+    return (static_cast<u64>(lo) << 32) | hi;
+}
+__device__ inline void emit(u64 key, u64* out, u32* cnt, u32 counter, u32 cap) {
+    const u32 s = atomicAdd(&cnt[counter], 1u);
+    if (s < cap)
+        out[s] = key;
+    else
+        atomicOr(&cnt[kOverflow], 1u);
+}
+
+// x sweep over sorted endpoints (scene.cpp:2642-2680). One thread per
+// sorted position; a min endpoint of box A scans forward to A's own max
+// endpoint and tests the y/z keys of every box that starts inside.
+// Adjacent threads scan overlapping windows, so the payload reads of a
+// wave are coalesced.
+__global__ __launch_bounds__(256) void k_sweep(u32 n2, const u32* xpay, const Rec* cur,
+                                               u64* out, u32* cnt, u32 cap) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    const u32 pa = xpay[i];
+    if (pa & 1u) return;
+    const u32 a = pa >> 1;
+    const Rec ra = cur[a];
+    const u32 stop = pa | 1u;
+    for (u32 j = i + 1; j < n2; ++j) {
+        const u32 pb = xpay[j];
+        if (pb == stop) break;
+        if (pb & 1u) continue;
+        const u32 b = pb >> 1;
+        const Rec rb = cur[b];
+        if (!pair_allowed(ra, rb)) continue;
+        if (ra.k[1] > rb.k[4] || rb.k[1] > ra.k[4] || ra.k[2] > rb.k[5] || rb.k[2] > ra.k[5]) continue;
+        emit(pair_key(a, b), out, cnt, kSweep, cap);
+    }
+}
+
+// Lower bound of composite (key, payload) in a sorted endpoint array.
+__device__ inline u32 lower_bound_ep(const u32* key, const u32* pay, u32 n, u32 vk, u32 vp) {
+    u32 lo = 0, hi = n;
+    while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        const u32 k = key[mid];
+        if (k < vk || (k == vk && pay[mid] < vp))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Incremental crossings. A pair can change overlap status only if, on some
+// axis, a min endpoint of one box and a max endpoint of the other changed
+// their relative order. For every endpoint e that moved from v to v', the
+// endpoints that crossed it have old or new positions inside [v, v'] (in
+// the total order): scan that interval in the new array and in the old
+// array. One wave per (moved box, axis); lanes stride over each interval.
+// Every candidate pair is re-tested on all three axes with old and new
+// records; a status change emits the pair with kFoundBit for "found".
+__global__ __launch_bounds__(256) void k_cross(u32 n2, const u32* cnt_moved, const u32* moved,
+                                               const u32* new_key, const u32* new_pay,
+                                               const u32* old_key, const u32* old_pay,
+                                               const Rec* cur, const Rec* prev, u32 stamp,
+                                               u64* out, u32* cnt, u32 cap) {
+    const u32 lane = lane_id();
+    const u32 n_work = min(*cnt_moved, n2 >> 1) * 3u;
+    const u32 n_waves = gridDim.x * (blockDim.x / kWave);
+    for (u32 w = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; w < n_work; w += n_waves) {
+    const u32 a = moved[w / 3u];
+    const u32 axis = w % 3u;
+    const Rec an = cur[a];
+    const Rec ao = prev[a];
+    const u32* nk = new_key + static_cast<std::size_t>(axis) * n2;
+    const u32* np = new_pay + static_cast<std::size_t>(axis) * n2;
+    const u32* ok = old_key + static_cast<std::size_t>(axis) * n2;
+    const u32* op = old_pay + static_cast<std::size_t>(axis) * n2;
+    for (u32 kind = 0; kind < 2; ++kind) {
+        const u32 v_old = ao.k[axis + 3 * kind];
+        const u32 v_new = an.k[axis + 3 * kind];
+        if (v_old == v_new) continue;
+        const u32 pe = (a << 1) | kind;
+        const bool right = v_new > v_old;
+        for (u32 arr = 0; arr < 2; ++arr) {
+            const u32* K = arr ? ok : nk;
+            const u32* P = arr ? op : np;
+            // own position in this array, and the position of the other value
+            const u32 own = lower_bound_ep(K, P, n2, arr ? v_old : v_new, pe);
+            const u32 other = lower_bound_ep(K, P, n2, arr ? v_new : v_old, pe);
+            u32 lo, hi;
+            if (right) {
+                lo = arr ? own + 1 : other;
+                hi = arr ? other : own;
+            } else {
+                lo = arr ? other : own + 1;
+                hi = arr ? own : other;
+            }
+            for (u32 j = lo + lane; j < hi; j += kWave) {
+                const u32 pf = P[j];
+                if ((pf & 1u) == kind) continue;
+                const u32 b = pf >> 1;
+                if (b == a) continue;
+                const Rec bn = cur[b];
+                if (!pair_allowed(an, bn)) continue;
+                const Rec bo = ((bn.k[7] & 0x7fffffffu) == stamp) ? prev[b] : bn;
+                const bool ov_new = overlap3(an, bn);
+                const bool ov_old = overlap3(ao, bo);
+                if (ov_new != ov_old) emit(pair_key(a, b) | (ov_new ? kFoundBit : 0ull), out, cnt, kDelta, cap);
+            }
+        }
+    }
+    }
+}
+
+// ---- compaction predicates / writers ----
+struct Unique {
+    const u64* a;
+    __device__ bool operator()(u32 i) const { return i == 0 || a[i] != a[i - 1]; }
+};
+__device__ inline bool contains(const u64* a, u32 n, u64 x) {
+    u32 lo = 0, hi = n;
+    while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < n && a[lo] == x;
+}
+__device__ inline u32 lower_bound64(const u64* a, u32 n, u64 x) {
+    u32 lo = 0, hi = n;
+    while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+struct NotIn {
+    const u64* a;
+    const u64* other;
+    const u32* n_other;
+    __device__ bool operator()(u32 i) const { return !contains(other, *n_other, a[i]); }
+};
+struct Copy {
+    const u64* src;
+    u64* dst;
+    __device__ void operator()(u32 i, u32 o) const { dst[o] = src[i]; }
+};
+
+// Sorted unique delta = [lost..., found|bit63...] -> lost[], found[].
+__global__ void k_split(u32 cap, const u64* delta, const u32* d_n, u64* found, u64* lost, u32* cnt) {
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 n = min(*d_n, cap);
+    const u32 boundary = lower_bound64(delta, n, kFoundBit);
+    if (t == 0) {
+        cnt[kLost] = boundary;
+        cnt[kFound] = n - boundary;
+    }
+    for (u32 i = t; i < n; i += gridDim.x * blockDim.x) {
+        const u64 x = delta[i];
+        if (i < boundary)
+            lost[i] = x;
+        else
+            found[i - boundary] = x & kPairMask;
+    }
+}
+
+// (kept ∪ found), both sorted and disjoint -> out (sorted).
+__global__ void k_merge(u32 cap, const u64* kept, const u32* d_nk, const u64* found, const u32* d_nf,
+                        u64* out, u32* d_nout, u32* cnt) {
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 nk = *d_nk, nf = *d_nf;
+    if (t == 0) {
+        *d_nout = min(nk + nf, cap);
+        if (nk + nf > cap) atomicOr(&cnt[kOverflow], 2u);
+    }
+    for (u32 i = t; i < nk + nf; i += gridDim.x * blockDim.x) {
+        if (i < nk) {
+            const u64 x = kept[i];
+            const u32 pos = i + lower_bound64(found, nf, x);
+            if (pos < cap) out[pos] = x;
+        } else {
+            const u32 j = i - nk;
+            const u64 f = found[j];
+            const u32 pos = j + lower_bound64(kept, nk, f);
+            if (pos < cap) out[pos] = f;
+        }
+    }
+}
+
+struct Sap {
+    fyzz_hip_context* ctx = nullptr;
+    u32 n = 0;
+    u32 pair_cap = 0;
+    u32 delta_cap = 0;
+    u32 frame = 0;
+    int cur = 0;  // index of the current sorted arrays / pair list
+    bool built = false;
+    DevBuf<u32> planes, seen, moved, counters, sorted_key[2], sorted_val[2], tmp_key, tmp_val,
+        radix_counts, radix_totals, sel_tiles, dummy_a, dummy_b;
+    DevBuf<Rec> rec_cur, rec_prev;
+    DevBuf<u64> pairs[2], work_a, work_b, found, lost, kept;
+    std::vector<u32> host_counters;
+
+    u32 list_cap() const { return std::max(pair_cap, delta_cap); }
+};
+
+void sort_u64(Sap& s, hipStream_t st, const u32* d_count, u64* keys, u64* tmp) {
+    radix::Batch b{s.list_cap(), 1, s.list_cap(), 0, d_count, nullptr, 0};
+    const bool in_a = radix::sort<u64>(st, b, radix::Source::Arrays, keys, s.dummy_a.p, keys,
+                                       s.dummy_a.p, tmp, s.dummy_b.p, 64, s.radix_counts.p,
+                                       s.radix_totals.p);
+    if (!in_a) FYZZ_HIP_CHECK(hipMemcpyAsync(keys, tmp, sizeof(u64) * s.list_cap(), hipMemcpyDeviceToDevice, st));
+}
+
+void advance(Sap& s, const float* pos, const float* quat, const float* half, const float* off,
+             u32 n_dirty, const u32* dirty, bool force_full) {
+    hipStream_t st = s.ctx->stream;
+    const u32 n = s.n, n2 = 2 * n;
+    s.frame = (s.frame + 1) & 0x7fffffffu;
+    if (s.frame == 0) s.frame = 1;
+    const u32 stamp = s.frame;
+    const bool full = force_full || !s.built;
+    const int nxt = s.cur ^ 1;
+    u32* cnt = s.counters.p;
+    // reset the per-frame counters, keeping the current pair count
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt, static_cast<u32>(kPairs0), 0u);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt + kKept, static_cast<u32>(kCounters - kKept), 0u);
+    const u32 pairs_cur = s.cur == 0 ? kPairs0 : kPairs1;
+    const u32 pairs_nxt = s.cur == 0 ? kPairs1 : kPairs0;
+    if (!s.built) hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt + pairs_cur, 1u, 0u);
+
+    Profiler* prof = &s.ctx->prof;
+    {
+    ProfScope ps(prof, "sap.bounds", st);
+    if (full) {
+        if (n) hipLaunchKernelGGL(k_bounds_full, dim3(grid_for(n, 256)), dim3(256), 0, st, n, pos, quat, half, off,
+                                  s.planes.p, s.rec_cur.p, stamp, cnt);
+    } else {
+        const u32 count = dirty ? n_dirty : n;
+        if (count)
+            hipLaunchKernelGGL(k_bounds_update, dim3(grid_for(count, 256)), dim3(256), 0, st, count, dirty, n, pos,
+                               quat, half, off, s.planes.p, s.rec_cur.p, s.rec_prev.p, s.seen.p, stamp,
+                               s.moved.p, cnt);
+    }
+    FYZZ_HIP_CHECK(hipGetLastError());
+    }
+    // endpoint sort, 3 axes in one batch, from the key planes
+    if (n) {
+        ProfScope ps(prof, "sap.sort", st);
+        radix::Batch b{n2, 3, n2, n2, nullptr, s.planes.p, n};
+        const bool in_a = radix::sort<u32>(st, b, radix::Source::SapPlanes, nullptr, nullptr,
+                                           s.sorted_key[nxt].p, s.sorted_val[nxt].p, s.tmp_key.p,
+                                           s.tmp_val.p, 32, s.radix_counts.p, s.radix_totals.p,
+                                           prof, "sap.sort.scatter");
+        if (!in_a) {
+            FYZZ_HIP_CHECK(hipMemcpyAsync(s.sorted_key[nxt].p, s.tmp_key.p, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(s.sorted_val[nxt].p, s.tmp_val.p, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, st));
+        }
+    }
+    if (full) {
+        long pi = prof->begin("sap.pairs", st);
+        if (n) hipLaunchKernelGGL(k_sweep, dim3(grid_for(n2, 256)), dim3(256), 0, st, n2, s.sorted_val[nxt].p,
+                                  s.rec_cur.p, s.work_a.p, cnt, s.pair_cap);
+        prof->end("sap.pairs", pi, st);
+        ProfScope ps(prof, "sap.lists", st);
+        sort_u64(s, st, cnt + kSweep, s.work_a.p, s.work_b.p);
+        // unique -> pairs[nxt]
+        sel::select(st, s.pair_cap, cnt + kSweep, Unique{s.work_a.p}, Copy{s.work_a.p, s.pairs[nxt].p},
+                    s.sel_tiles.p, cnt + pairs_nxt);
+        sel::select(st, s.pair_cap, cnt + pairs_nxt, NotIn{s.pairs[nxt].p, s.pairs[s.cur].p, cnt + pairs_cur},
+                    Copy{s.pairs[nxt].p, s.found.p}, s.sel_tiles.p, cnt + kFound);
+        sel::select(st, s.pair_cap, cnt + pairs_cur, NotIn{s.pairs[s.cur].p, s.pairs[nxt].p, cnt + pairs_nxt},
+                    Copy{s.pairs[s.cur].p, s.lost.p}, s.sel_tiles.p, cnt + kLost);
+    } else {
+        long pi = prof->begin("sap.pairs", st);
+        hipLaunchKernelGGL(k_cross, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 3 * kWave, 256), 4096u)), dim3(256), 0, st, n2,
+                           cnt + kMoved, s.moved.p, s.sorted_key[nxt].p, s.sorted_val[nxt].p,
+                           s.sorted_key[s.cur].p, s.sorted_val[s.cur].p, s.rec_cur.p, s.rec_prev.p, stamp,
+                           s.work_a.p, cnt, s.delta_cap);
+        FYZZ_HIP_CHECK(hipGetLastError());
+        prof->end("sap.pairs", pi, st);
+        ProfScope ps(prof, "sap.lists", st);
+        sort_u64(s, st, cnt + kDelta, s.work_a.p, s.work_b.p);
+        sel::select(st, s.delta_cap, cnt + kDelta, Unique{s.work_a.p}, Copy{s.work_a.p, s.work_b.p},
+                    s.sel_tiles.p, cnt + kUnique);
+        hipLaunchKernelGGL(k_split, dim3(std::min<u32>(grid_for(s.delta_cap, 256), 1024u)), dim3(256), 0, st, s.delta_cap, s.work_b.p,
+                           cnt + kUnique, s.found.p, s.lost.p, cnt);
+        sel::select(st, s.pair_cap, cnt + pairs_cur, NotIn{s.pairs[s.cur].p, s.lost.p, cnt + kLost},
+                    Copy{s.pairs[s.cur].p, s.kept.p}, s.sel_tiles.p, cnt + kKept);
+        hipLaunchKernelGGL(k_merge, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(s.pair_cap) + s.delta_cap, 256), 2048u)), dim3(256), 0, st,
+                           s.pair_cap, s.kept.p, cnt + kKept, s.found.p, cnt + kFound, s.pairs[nxt].p,
+                           cnt + pairs_nxt, cnt);
+    }
+    FYZZ_HIP_CHECK(hipGetLastError());
+    s.cur = nxt;
+    s.built = true;
+}
+
+}  // namespace fyzz_hip::sap
+
+struct fyzz_hip_sap {
+    fyzz_hip::sap::Sap s;
+};
+
+using namespace fyzz_hip;
+
+extern "C" {
+
+int fyzz_hip_sap_box_bounds(fyzz_hip_context* ctx, uint32_t n, const float* pos, const float* quat,
+                            const float* half, const float* contact_offset, uint32_t* bounds,
+                            uint32_t* error_flag) {
+    return guarded(ctx, [&] {
+        FYZZ_REQUIRE(n == 0 || (pos && quat && half && contact_offset && bounds && error_flag),
+                     kInvalidArgument, "sap_box_bounds: null device pointer");
+        if (!n) return;
+        hipLaunchKernelGGL(sap::k_bounds_planes, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, pos, quat,
+                           half, contact_offset, bounds, error_flag);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int fyzz_hip_sap_sort_endpoints(fyzz_hip_context* ctx, uint32_t n, const uint32_t* bounds,
+                                uint32_t* values, uint32_t* payloads, uint32_t* scratch,
+                                uint64_t scratch_words) {
+    return guarded(ctx, [&] {
+        FYZZ_REQUIRE(n == 0 || (bounds && values && payloads && scratch), kInvalidArgument,
+                     "sap_sort_endpoints: null device pointer");
+        FYZZ_REQUIRE(n <= 0x7fffffffu, kInvalidArgument, "sap_sort_endpoints: n too large");
+        if (!n) return;
+        const u32 n2 = 2 * n;
+        radix::Batch b{n2, 3, n2, n2, nullptr, bounds, n};
+        const std::size_t need = 2ull * 3 * n2 + radix::count_words(b) + 3 * radix::kBins;
+        FYZZ_REQUIRE(scratch_words >= need, kInvalidArgument,
+                     "sap_sort_endpoints: scratch too small (need " + std::to_string(need) + " words)");
+        u32* tk = scratch;
+        u32* tv = tk + 3ull * n2;
+        u32* counts = tv + 3ull * n2;
+        u32* totals = counts + radix::count_words(b);
+        const bool in_a = radix::sort<u32>(ctx->stream, b, radix::Source::SapPlanes, nullptr, nullptr, values,
+                                           payloads, tk, tv, 32, counts, totals);
+        if (!in_a) {
+            FYZZ_HIP_CHECK(hipMemcpyAsync(values, tk, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, ctx->stream));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(payloads, tv, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    });
+}
+
+uint64_t fyzz_hip_sap_sort_scratch_words(uint32_t n) {
+    const u32 n2 = 2 * n;
+    radix::Batch b{n2, 3, n2, n2, nullptr, nullptr, n};
+    return 2ull * 3 * n2 + radix::count_words(b) + 3 * radix::kBins;
+}
+
+int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_ids,
+                        const uint8_t* dynamic, uint32_t pair_capacity, uint32_t delta_capacity,
+                        fyzz_hip_sap** out) {
+    if (!out) return kInvalidArgument;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        FYZZ_REQUIRE(n <= 0x3fffffffu, kInvalidArgument, "sap_create: too many boxes");
+        FYZZ_REQUIRE(n == 0 || (body_ids && dynamic), kInvalidArgument, "sap_create: null host array");
+        auto h = std::make_unique<fyzz_hip_sap>();
+        sap::Sap& s = h->s;
+        s.ctx = ctx;
+        s.n = n;
+        s.pair_cap = std::max<u32>(pair_capacity, 1024);
+        s.delta_cap = std::max<u32>(delta_capacity, 1024);
+        const u32 n2 = 2 * n;
+        s.planes.reserve(6ull * n);
+        s.seen.reserve(n);
+        s.moved.reserve(n);
+        s.counters.reserve(sap::kCounters);
+        for (int i = 0; i < 2; ++i) {
+            s.sorted_key[i].reserve(3ull * n2);
+            s.sorted_val[i].reserve(3ull * n2);
+            s.pairs[i].reserve(s.pair_cap);
+        }
+        s.tmp_key.reserve(3ull * n2);
+        s.tmp_val.reserve(3ull * n2);
+        const u32 lcap = s.list_cap();
+        radix::Batch be{n2, 3, n2, n2, nullptr, nullptr, n};
+        radix::Batch bl{lcap, 1, lcap, 0, nullptr, nullptr, 0};
+        s.radix_counts.reserve(std::max(radix::count_words(be), radix::count_words(bl)));
+        s.radix_totals.reserve(3 * radix::kBins);
+        s.sel_tiles.reserve(sel::tiles_for(lcap) + 1);
+        s.dummy_a.reserve(lcap);
+        s.dummy_b.reserve(lcap);
+        s.rec_cur.reserve(n);
+        s.rec_prev.reserve(n);
+        s.work_a.reserve(lcap);
+        s.work_b.reserve(lcap);
+        s.found.reserve(lcap);
+        s.lost.reserve(lcap);
+        s.kept.reserve(s.pair_cap);
+        std::vector<sap::Rec> init(n);
+        for (u32 i = 0; i < n; ++i) {
+            for (int k = 0; k < 6; ++k) init[i].k[k] = 0;
+            init[i].k[6] = body_ids[i];
+            init[i].k[7] = dynamic[i] ? 0x80000000u : 0u;
+        }
+        if (n) {
+            FYZZ_HIP_CHECK(hipMemcpy(s.rec_cur.p, init.data(), sizeof(sap::Rec) * n, hipMemcpyHostToDevice));
+            FYZZ_HIP_CHECK(hipMemcpy(s.rec_prev.p, init.data(), sizeof(sap::Rec) * n, hipMemcpyHostToDevice));
+            FYZZ_HIP_CHECK(hipMemset(s.seen.p, 0, sizeof(u32) * n));
+        }
+        FYZZ_HIP_CHECK(hipMemset(s.counters.p, 0, sizeof(u32) * sap::kCounters));
+        *out = h.release();
+    });
+}
+
+int fyzz_hip_sap_destroy(fyzz_hip_sap* sap) {
+    if (!sap) return kInvalidArgument;
+    fyzz_hip_context* ctx = sap->s.ctx;
+    return guarded(ctx, [&] { delete sap; });
+}
+
+int fyzz_hip_sap_advance(fyzz_hip_sap* sap, const float* pos, const float* quat, const float* half,
+                         const float* contact_offset, uint32_t n_dirty, const uint32_t* dirty_slots,
+                         int force_full) {
+    if (!sap) return kInvalidArgument;
+    return guarded(sap->s.ctx, [&] {
+        FYZZ_REQUIRE(sap->s.n == 0 || (pos && quat && half && contact_offset), kInvalidArgument,
+                     "sap_advance: null device pointer");
+        sap::advance(sap->s, pos, quat, half, contact_offset, n_dirty, dirty_slots, force_full != 0);
+    });
+}
+
+int fyzz_hip_sap_counts(fyzz_hip_sap* sap, uint64_t* n_pairs, uint64_t* n_found, uint64_t* n_lost,
+                        uint32_t* status_flags) {
+    if (!sap) return kInvalidArgument;
+    return guarded(sap->s.ctx, [&] {
+        sap::Sap& s = sap->s;
+        s.host_counters.resize(sap::kCounters);
+        FYZZ_HIP_CHECK(hipMemcpyAsync(s.host_counters.data(), s.counters.p, sizeof(u32) * sap::kCounters,
+                                      hipMemcpyDeviceToHost, s.ctx->stream));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+        const auto& c = s.host_counters;
+        if (n_pairs) *n_pairs = c[s.cur == 0 ? sap::kPairs0 : sap::kPairs1];
+        if (n_found) *n_found = c[sap::kFound];
+        if (n_lost) *n_lost = c[sap::kLost];
+        if (status_flags) *status_flags = (c[sap::kError] ? 1u : 0u) | (c[sap::kOverflow] ? 2u : 0u);
+    });
+}
+
+// which: 0 current pair list, 1 found, 2 lost. Slot pairs (lo, hi).
+int fyzz_hip_sap_fetch(fyzz_hip_sap* sap, int which, uint32_t* out_pairs, uint64_t capacity,
+                       uint64_t* written) {
+    if (!sap) return kInvalidArgument;
+    return guarded(sap->s.ctx, [&] {
+        FYZZ_REQUIRE(which >= 0 && which <= 2, kInvalidArgument, "sap_fetch: which must be 0, 1 or 2");
+        uint64_t np = 0, nf = 0, nl = 0;
+        uint32_t flags = 0;
+        int rc = fyzz_hip_sap_counts(sap, &np, &nf, &nl, &flags);
+        if (rc) throw Error(rc, sap->s.ctx->last_error);
+        FYZZ_REQUIRE(!(flags & 2u), kOverflow, "sap: pair capacity exceeded");
+        FYZZ_REQUIRE(!(flags & 1u), kOverflow, "sap: non-finite box bounds");
+        sap::Sap& s = sap->s;
+        const u64* src = which == 0 ? s.pairs[s.cur].p : which == 1 ? s.found.p : s.lost.p;
+        const uint64_t n = which == 0 ? np : which == 1 ? nf : nl;
+        const uint64_t m = std::min<uint64_t>(n, capacity);
+        std::vector<u64> tmp(m);
+        if (m) FYZZ_HIP_CHECK(hipMemcpyAsync(tmp.data(), src, sizeof(u64) * m, hipMemcpyDeviceToHost, s.ctx->stream));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(s.ctx->stream));
+        for (uint64_t i = 0; i < m; ++i) {
+            out_pairs[2 * i] = static_cast<u32>(tmp[i] >> 32);
+            out_pairs[2 * i + 1] = static_cast<u32>(tmp[i] & 0xffffffffu);
+        }
+        if (written) *written = n;
+    });
+}
+
+int fyzz_hip_sap_endpoints(fyzz_hip_sap* sap, uint32_t* values, uint32_t* payloads) {
+    if (!sap) return kInvalidArgument;
+    return guarded(sap->s.ctx, [&] {
+        sap::Sap& s = sap->s;
+        FYZZ_REQUIRE(s.built, kLogicError, "sap_endpoints: no frame advanced yet");
+        FYZZ_REQUIRE(values && payloads, kInvalidArgument, "sap_endpoints: null device pointer");
+        const std::size_t bytes = sizeof(u32) * 6ull * s.n;
+        if (!bytes) return;
+        FYZZ_HIP_CHECK(hipMemcpyAsync(values, s.sorted_key[s.cur].p, bytes, hipMemcpyDeviceToDevice, s.ctx->stream));
+        FYZZ_HIP_CHECK(hipMemcpyAsync(payloads, s.sorted_val[s.cur].p, bytes, hipMemcpyDeviceToDevice, s.ctx->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+"""ctypes binding of the C-ABI in include/fyzz_hip.h.
+
+The library is loaded from this package directory (libfyzz_hip.so, built
+in-tree by fyzzxyzzy-sdk_amd/build.py). There is no fallback: a missing or
+unloadable library raises ImportError-derived LibraryMissing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfyzz_hip.so"
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+vp = C.c_void_p
+
+# name -> (restype, argtypes). Mirrors include/fyzz_hip.h one to one.
+SIGNATURES = {
+    "fyzz_hip_version": (C.c_char_p, []),
+    "fyzz_hip_context_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "fyzz_hip_context_destroy": (C.c_int, [vp]),
+    "fyzz_hip_last_error": (C.c_char_p, [vp]),
+    "fyzz_hip_set_stream": (C.c_int, [vp, vp]),
+    "fyzz_hip_synchronize": (C.c_int, [vp]),
+    "fyzz_hip_profile_enable": (C.c_int, [vp, C.c_int]),
+    "fyzz_hip_profile_read": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_double), u64p]),
+    "fyzz_hip_sap_box_bounds": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp]),
+    "fyzz_hip_sap_sort_endpoints": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, C.c_uint64]),
+    "fyzz_hip_sap_sort_scratch_words": (C.c_uint64, [C.c_uint32]),
+    "fyzz_hip_sap_create": (C.c_int, [vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
+    "fyzz_hip_sap_destroy": (C.c_int, [vp]),
+    "fyzz_hip_sap_advance": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint32, vp, C.c_int]),
+    "fyzz_hip_sap_counts": (C.c_int, [vp, u64p, u64p, u64p, u32p]),
+    "fyzz_hip_sap_fetch": (C.c_int, [vp, C.c_int, vp, C.c_uint64, u64p]),
+    "fyzz_hip_sap_endpoints": (C.c_int, [vp, vp, vp]),
+}
+
+
+class LibraryMissing(ImportError):
+    pass
+
+
+class FyzzError(RuntimeError):
+    """A non-zero status from the C-ABI, with the context's message."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise LibraryMissing(
+                f"{LIB_PATH} is not built; run `python fyzzxyzzy-sdk_amd/build.py` "
+                "(there is no CPU fallback)")
+        try:
+            handle = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | getattr(os, "RTLD_LOCAL", 0))
+        except OSError as e:  # pragma: no cover - reported to the caller
+            raise LibraryMissing(f"cannot load {LIB_PATH}: {e}") from e
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(handle, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(ctx, rc: int):
+    if rc != 0:
+        msg = lib().fyzz_hip_last_error(ctx)
+        raise FyzzError(rc, msg.decode() if msg else "")
+    return rc
+
+
+class Context:
+    """One device context; optionally bound to a HIP stream handle."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        h = vp()
+        rc = lib().fyzz_hip_context_create(device, C.byref(h))
+        if rc != 0:
+            raise FyzzError(rc, f"fyzz_hip_context_create(device={device}) failed")
+        self.h = h
+        if stream is not None:
+            check(self.h, lib().fyzz_hip_set_stream(self.h, vp(stream)))
+
+    def close(self):
+        if self.h:
+            lib().fyzz_hip_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name: str, *args):
+        return check(self.h, getattr(lib(), name)(self.h, *args))
+
+    def synchronize(self):
+        self.call("fyzz_hip_synchronize")
+
+    def profile(self, enable: bool = True):
+        self.call("fyzz_hip_profile_enable", 1 if enable else 0)
+
+    def profile_read(self, name: str):
+        """(total_ms, count) of one named stage since the last read."""
+        t = C.c_double()
+        c = C.c_uint64()
+        self.call("fyzz_hip_profile_read", name.encode(), C.byref(t), C.byref(c))
+        return t.value, c.value

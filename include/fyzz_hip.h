@@ -1,0 +1,112 @@
+/*
+ * fyzz_hip.h - C-ABI of the MI355X (gfx950) FyzzXYZZY step pipeline.
+ *
+ * Plain pointers and sizes only. Every function returns 0 on success or a
+ * negative status (FYZZ_HIP_E_*); the message of the last failure on a
+ * context is available from fyzz_hip_last_error(). No C++ exception
+ * crosses this boundary. Pointers documented as "device" must be device
+ * allocations visible to the context's device; all work is enqueued on
+ * the context's stream (fyzz_hip_set_stream) and is asynchronous unless
+ * the function says it synchronizes.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   GpuBroadphaseProducer plug-in           src/native/scene.h:383-505
+ *   GpuBroadphaseReports {found,lost,...}   src/native/scene.h:78-91
+ *   Scene::update_bounds / update_pairs     src/native/scene.cpp:1336-2004, 2564-2704
+ * Each declaration below names the reference function it stands in for.
+ */
+#ifndef FYZZ_HIP_H
+#define FYZZ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FYZZ_HIP_OK 0
+#define FYZZ_HIP_E_INVALID_ARGUMENT (-1) /* std::invalid_argument */
+#define FYZZ_HIP_E_DEVICE (-2)           /* std::runtime_error (device) */
+#define FYZZ_HIP_E_OVERFLOW (-3)         /* std::overflow_error */
+#define FYZZ_HIP_E_LOGIC (-4)            /* std::logic_error */
+#define FYZZ_HIP_E_OUT_OF_MEMORY (-5)
+
+typedef struct fyzz_hip_context fyzz_hip_context;
+typedef struct fyzz_hip_sap fyzz_hip_sap;
+
+/* ---- context -------------------------------------------------------- */
+/* Stands in for the producer's owned compute::Context + stream
+ * (source_sc_owned_gpu_broadphase_native.h:66-162). */
+int fyzz_hip_context_create(int device, fyzz_hip_context** out);
+int fyzz_hip_context_destroy(fyzz_hip_context* ctx);
+const char* fyzz_hip_last_error(const fyzz_hip_context* ctx);
+/* hip_stream: a hipStream_t of the context's device, or NULL for the
+ * default stream. */
+int fyzz_hip_set_stream(fyzz_hip_context* ctx, void* hip_stream);
+int fyzz_hip_synchronize(fyzz_hip_context* ctx);
+/* Stage timing: when enabled, named launches are bracketed by HIP events
+ * on the context's stream (stands in for Context::profile_batch_gpu_nanoseconds,
+ * include/fyzzxyzzy/compute.h:299). profile_read synchronizes, returns the
+ * summed milliseconds and span count of one name, and resets it. Names:
+ * "sap.bounds", "sap.sort", "sap.sort.scatter", "sap.pairs", "sap.lists". */
+int fyzz_hip_profile_enable(fyzz_hip_context* ctx, int enable);
+int fyzz_hip_profile_read(fyzz_hip_context* ctx, const char* name, double* total_ms,
+                          uint64_t* count);
+/* Library build identification (static string). */
+const char* fyzz_hip_version(void);
+
+/* ---- SAP stage entries (device pointers) ----------------------------- */
+/* Scene::update_shape_bounds + encode_min/encode_max
+ * (scene.cpp:26-35, 1233-1316) for n boxes: pos[3n], quat[4n] (x,y,z,w),
+ * half[3n], contact_offset[n] -> bounds[6][n] (min x,y,z keys then max
+ * x,y,z keys). *error_flag |= 1 when a bound is not finite (the
+ * reference's std::overflow_error, scene.cpp:1298-1300). */
+int fyzz_hip_sap_box_bounds(fyzz_hip_context* ctx, uint32_t n, const float* pos,
+                            const float* quat, const float* half,
+                            const float* contact_offset, uint32_t* bounds,
+                            uint32_t* error_flag);
+
+/* The sorted endpoint arrays of Scene::update_bounds (scene.cpp:1336-1400,
+ * Endpoint scene.h:1295-1304) for all three axes: values[3][2n] and
+ * payloads[3][2n], payload = slot << 1 | is_max, ordered by (value,
+ * min-before-max, slot). scratch: fyzz_hip_sap_sort_scratch_words(n) u32. */
+int fyzz_hip_sap_sort_endpoints(fyzz_hip_context* ctx, uint32_t n, const uint32_t* bounds,
+                                uint32_t* values, uint32_t* payloads, uint32_t* scratch,
+                                uint64_t scratch_words);
+uint64_t fyzz_hip_sap_sort_scratch_words(uint32_t n);
+
+/* ---- incremental broadphase (GpuBroadphaseProducer::advance) -------- */
+/* n boxes in slot order (increasing shape id); body_ids[n] and dynamic[n]
+ * are HOST arrays (Bp filter: pairs need different bodies and at least one
+ * dynamic shape, scene.cpp:2606-2608). pair_capacity bounds the live pair
+ * list, delta_capacity the per-frame candidate changes. */
+int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_ids,
+                        const uint8_t* dynamic, uint32_t pair_capacity,
+                        uint32_t delta_capacity, fyzz_hip_sap** out);
+int fyzz_hip_sap_destroy(fyzz_hip_sap* sap);
+/* GpuBroadphaseProducer::advance (scene.h:394-396). Device pose arrays as
+ * in fyzz_hip_sap_box_bounds. dirty_slots (device, n_dirty entries) lists
+ * the boxes whose pose may have changed (NULL: all). The first call, and
+ * any call with force_full != 0, rebuilds from scratch
+ * (GpuBroadphaseReports::full_rebuild). Asynchronous. */
+int fyzz_hip_sap_advance(fyzz_hip_sap* sap, const float* pos, const float* quat,
+                         const float* half, const float* contact_offset, uint32_t n_dirty,
+                         const uint32_t* dirty_slots, int force_full);
+/* Synchronizes. status_flags: bit0 non-finite bounds, bit1 capacity
+ * overflow (GpuBroadphaseReports::overflow). */
+int fyzz_hip_sap_counts(fyzz_hip_sap* sap, uint64_t* n_pairs, uint64_t* n_found,
+                        uint64_t* n_lost, uint32_t* status_flags);
+/* Synchronizes. which: 0 live pairs (Scene::pairs), 1 found, 2 lost
+ * (GpuBroadphaseReports::found/lost), as sorted (lo, hi) SLOT pairs into a
+ * HOST array of 2*capacity u32; *written = total count. Overflow or
+ * non-finite bounds return FYZZ_HIP_E_OVERFLOW. */
+int fyzz_hip_sap_fetch(fyzz_hip_sap* sap, int which, uint32_t* out_pairs, uint64_t capacity,
+                       uint64_t* written);
+/* Copies the current sorted endpoints (as fyzz_hip_sap_sort_endpoints
+ * lays them out) into DEVICE arrays values[3][2n], payloads[3][2n]. */
+int fyzz_hip_sap_endpoints(fyzz_hip_sap* sap, uint32_t* values, uint32_t* payloads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FYZZ_HIP_H */
