@@ -285,7 +285,11 @@ __global__ __launch_bounds__(256) void k_cross(u32 n2, const u32* cnt_moved, con
                 if (b == a) continue;
                 const Rec bn = cur[b];
                 if (!pair_allowed(an, bn)) continue;
-                const Rec bo = ((bn.k[7] & 0x7fffffffu) == stamp) ? prev[b] : bn;
+                const bool b_moved = (bn.k[7] & 0x7fffffffu) == stamp;
+                // an unmoved endpoint sits at the same value in both arrays:
+                // the new-array scan already saw it
+                if (arr && !b_moved) continue;
+                const Rec bo = b_moved ? prev[b] : bn;
                 const bool ov_new = overlap3(an, bn);
                 const bool ov_old = overlap3(ao, bo);
                 if (ov_new != ov_old) emit(pair_key(a, b) | (ov_new ? kFoundBit : 0ull), out, cnt, kDelta, cap);
@@ -598,8 +602,15 @@ int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_
 
 int fyzz_hip_sap_destroy(fyzz_hip_sap* sap) {
     if (!sap) return kInvalidArgument;
-    fyzz_hip_context* ctx = sap->s.ctx;
-    return guarded(ctx, [&] { delete sap; });
+    // Device buffers are released on the device they were allocated on;
+    // the context must still be alive (destroy children first).
+    try {
+        (void)hipSetDevice(sap->s.ctx->device);
+        delete sap;
+    } catch (...) {
+        return kDeviceError;
+    }
+    return kOk;
 }
 
 int fyzz_hip_sap_advance(fyzz_hip_sap* sap, const float* pos, const float* quat, const float* half,

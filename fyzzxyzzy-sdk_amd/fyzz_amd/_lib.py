@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libfyzz_hip.so"
@@ -88,11 +89,18 @@ class Context:
         if rc != 0:
             raise FyzzError(rc, f"fyzz_hip_context_create(device={device}) failed")
         self.h = h
+        self._children = weakref.WeakSet()
         if stream is not None:
             check(self.h, lib().fyzz_hip_set_stream(self.h, vp(stream)))
 
+    def adopt(self, child):
+        """Register an object whose close() must run before this context's."""
+        self._children.add(child)
+
     def close(self):
         if self.h:
+            for c in list(self._children):
+                c.close()
             lib().fyzz_hip_context_destroy(self.h)
             self.h = None
 

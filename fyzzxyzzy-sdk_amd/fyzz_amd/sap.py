@@ -54,11 +54,12 @@ class SapBroadphase:
         b = np.ascontiguousarray(body_ids, np.uint32)
         d = np.ascontiguousarray(dynamic, np.uint8)
         pc = pair_capacity if pair_capacity is not None else max(8 * n, 4096)
-        dc = delta_capacity if delta_capacity is not None else max(2 * n, 4096)
+        dc = delta_capacity if delta_capacity is not None else max(8 * n, 4096)
         h = vp()
         ctx.call("fyzz_hip_sap_create", C.c_uint32(n), b.ctypes.data_as(vp), d.ctypes.data_as(vp),
                  C.c_uint32(pc), C.c_uint32(dc), C.byref(h))
         self.h = h
+        ctx.adopt(self)
 
     def close(self):
         if self.h:

@@ -83,6 +83,7 @@ uint64_t fyzz_hip_sap_sort_scratch_words(uint32_t n);
 int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_ids,
                         const uint8_t* dynamic, uint32_t pair_capacity,
                         uint32_t delta_capacity, fyzz_hip_sap** out);
+/* Destroy every broadphase object before its context. */
 int fyzz_hip_sap_destroy(fyzz_hip_sap* sap);
 /* GpuBroadphaseProducer::advance (scene.h:394-396). Device pose arrays as
  * in fyzz_hip_sap_box_bounds. dirty_slots (device, n_dirty entries) lists
