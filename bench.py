@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--workload", default="sap", choices=["sap"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-frames", type=int, default=1)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run one plain 256 MiB device copy first (PMC byte-counter calibration)")
     return p.parse_args()
 
 
@@ -127,6 +129,14 @@ def bench_sap(args):
     d_off = torch.from_numpy(off).to(dev)
     d_moves = [(torch.from_numpy(s.astype(np.int64)).to(dev), torch.from_numpy(s.astype(np.int32)).to(dev),
                 torch.from_numpy(p).to(dev)) for s, p in moves]
+    if args.calibrate_copy:
+        # one plain copy of known size: 256 MiB read + 256 MiB written
+        src = torch.empty(64 << 20, dtype=torch.int32, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     bp = SapBroadphase(ctx, np.arange(n, dtype=np.uint32), np.ones(n, np.uint8))
     bp.advance(d_pos, d_quat, d_half, d_off)  # frame 0: full build
     torch.cuda.synchronize()

@@ -267,32 +267,46 @@ __global__ __launch_bounds__(256) void k_cross(u32 n2, const u32* cnt_moved, con
         for (u32 arr = 0; arr < 2; ++arr) {
             const u32* K = arr ? ok : nk;
             const u32* P = arr ? op : np;
-            // own position in this array, and the position of the other value
+            // e sits at its new value in the new array and at its old value
+            // in the old one; the crossed entries lie between e and the
+            // composite (other value, pe), walking up or down from e.
             const u32 own = lower_bound_ep(K, P, n2, arr ? v_old : v_new, pe);
-            const u32 other = lower_bound_ep(K, P, n2, arr ? v_new : v_old, pe);
-            u32 lo, hi;
-            if (right) {
-                lo = arr ? own + 1 : other;
-                hi = arr ? other : own;
-            } else {
-                lo = arr ? other : own + 1;
-                hi = arr ? own : other;
-            }
-            for (u32 j = lo + lane; j < hi; j += kWave) {
-                const u32 pf = P[j];
-                if ((pf & 1u) == kind) continue;
-                const u32 b = pf >> 1;
-                if (b == a) continue;
-                const Rec bn = cur[b];
-                if (!pair_allowed(an, bn)) continue;
-                const bool b_moved = (bn.k[7] & 0x7fffffffu) == stamp;
-                // an unmoved endpoint sits at the same value in both arrays:
-                // the new-array scan already saw it
-                if (arr && !b_moved) continue;
-                const Rec bo = b_moved ? prev[b] : bn;
-                const bool ov_new = overlap3(an, bn);
-                const bool ov_old = overlap3(ao, bo);
-                if (ov_new != ov_old) emit(pair_key(a, b) | (ov_new ? kFoundBit : 0ull), out, cnt, kDelta, cap);
+            const u32 bound_v = arr ? v_new : v_old;
+            const bool up = (arr != 0) == right;  // old array: toward v_new
+            for (u32 base = 0;; base += kWave) {
+                const u32 step = base + lane + 1;
+                bool inside = false;
+                u32 j = 0;
+                if (up) {
+                    j = own + step;
+                    if (j < n2) {
+                        const u32 kj = K[j];
+                        inside = kj < bound_v || (kj == bound_v && P[j] < pe);
+                    }
+                } else if (step <= own) {
+                    j = own - step;
+                    const u32 kj = K[j];
+                    inside = kj > bound_v || (kj == bound_v && P[j] > pe);
+                }
+                if (inside) {
+                    const u32 pf = P[j];
+                    const u32 b = pf >> 1;
+                    if ((pf & 1u) != kind && b != a) {
+                        const Rec bn = cur[b];
+                        const bool b_moved = (bn.k[7] & 0x7fffffffu) == stamp;
+                        // an unmoved endpoint sits at the same value in both
+                        // arrays: the new-array scan already saw it
+                        if (pair_allowed(an, bn) && !(arr && !b_moved)) {
+                            const Rec bo = b_moved ? prev[b] : bn;
+                            const bool ov_new = overlap3(an, bn);
+                            const bool ov_old = overlap3(ao, bo);
+                            if (ov_new != ov_old)
+                                emit(pair_key(a, b) | (ov_new ? kFoundBit : 0ull), out, cnt, kDelta, cap);
+                        }
+                    }
+                }
+                // sorted entries: the in-range lanes form a prefix
+                if (__ballot(inside) != ~0ull) break;
             }
         }
     }
