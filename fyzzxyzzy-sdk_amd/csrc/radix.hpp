@@ -38,6 +38,7 @@ struct Batch {
     const u32* d_count;    // per-array element count on device, or null
     const u32* planes;     // Source::SapPlanes
     u32 nbox;              // Source::SapPlanes
+    const u32* d_gate;     // when set and *d_gate == 0 the launch does nothing
 };
 
 template <class K>
@@ -46,9 +47,11 @@ __device__ inline u32 digit_of(K k, int shift) {
 }
 
 __device__ inline u32 seg_count(const Batch& b, u32 seg) {
+    if (b.d_gate && *b.d_gate == 0) return 0;
     const u32 n = b.d_count ? b.d_count[seg] : b.n_fixed;
     return n < b.cap ? n : b.cap;
 }
+__device__ inline u32 tiles_needed(u32 n) { return (n + kTile - 1) / kTile; }
 
 template <class K, Source S>
 __device__ inline void load_item(const Batch& b, const K* keys, const u32* vals, u32 seg, u32 i,
@@ -79,37 +82,43 @@ template <class K, Source S>
 __global__ __launch_bounds__(kBlock) void k_upsweep(Batch b, const K* keys, const u32* vals,
                                                     int shift, u32 tiles, u32* counts) {
     __shared__ u32 hist[kBins];
-    const u32 seg = blockIdx.y, tile = blockIdx.x, tid = threadIdx.x;
-    hist[tid] = 0;
-    __syncthreads();
+    const u32 seg = blockIdx.y, tid = threadIdx.x;
     const u32 n = seg_count(b, seg);
-    const u32 base = tile * kTile;
-    for (int j = 0; j < kItems; ++j) {
-        const u32 i = base + j * kBlock + tid;
-        const bool valid = i < n;
-        K k = 0;
-        u32 v = 0;
-        if (valid) load_item<K, S>(b, keys, vals, seg, i, k, v);
-        const u32 d = digit_of(k, shift);
-        const u64 peers = match_digit(d, valid);
-        if (valid && lane_rank(peers) == 0) atomicAdd(&hist[d], static_cast<u32>(__popcll(peers)));
+    const u32 need = tiles_needed(n);
+    for (u32 tile = blockIdx.x; tile < need; tile += gridDim.x) {
+        hist[tid] = 0;
+        __syncthreads();
+        const u32 base = tile * kTile;
+        for (int j = 0; j < kItems; ++j) {
+            const u32 i = base + j * kBlock + tid;
+            const bool valid = i < n;
+            K k = 0;
+            u32 v = 0;
+            if (valid) load_item<K, S>(b, keys, vals, seg, i, k, v);
+            const u32 d = digit_of(k, shift);
+            const u64 peers = match_digit(d, valid);
+            if (valid && lane_rank(peers) == 0) atomicAdd(&hist[d], static_cast<u32>(__popcll(peers)));
+        }
+        __syncthreads();
+        counts[(static_cast<std::size_t>(seg) * kBins + tid) * tiles + tile] = hist[tid];
+        __syncthreads();
     }
-    __syncthreads();
-    counts[(static_cast<std::size_t>(seg) * kBins + tid) * tiles + tile] = hist[tid];
 }
 
 // Exclusive scan over tiles for each (seg, digit); totals[seg*256+digit].
-__global__ __launch_bounds__(kBlock) void k_tile_scan(u32 tiles, u32* counts, u32* totals) {
+__global__ __launch_bounds__(kBlock) void k_tile_scan(Batch b, u32 tiles, u32* counts, u32* totals) {
     __shared__ u32 scratch[kWaves + 1];
     const u32 seg = blockIdx.y, digit = blockIdx.x;
+    if (b.d_gate && *b.d_gate == 0) return;
+    const u32 need = tiles_needed(seg_count(b, seg));
     u32* row = counts + (static_cast<std::size_t>(seg) * kBins + digit) * tiles;
     u32 carry = 0;
-    for (u32 t0 = 0; t0 < tiles; t0 += kBlock) {
+    for (u32 t0 = 0; t0 < need; t0 += kBlock) {
         const u32 t = t0 + threadIdx.x;
-        const u32 v = t < tiles ? row[t] : 0u;
+        const u32 v = t < need ? row[t] : 0u;
         u32 total;
         const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
-        if (t < tiles) row[t] = carry + ex;
+        if (t < need) row[t] = carry + ex;
         carry += total;
     }
     if (threadIdx.x == 0) totals[seg * kBins + digit] = carry;
@@ -127,11 +136,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Batch b, const K* keys, cons
     __shared__ K s_keys[kTile];
     __shared__ u32 s_vals[kTile];
 
-    const u32 seg = blockIdx.y, tile = blockIdx.x, tid = threadIdx.x;
+    const u32 seg = blockIdx.y, tid = threadIdx.x;
     const u32 n = seg_count(b, seg);
-    const u32 base = tile * kTile;
-    if (base >= n) return;  // uniform across the block
+    const u32 need = tiles_needed(n);
     const u32 wave = tid / kWave, lane = lane_id();
+    for (u32 tile = blockIdx.x; tile < need; tile += gridDim.x) {
+    const u32 base = tile * kTile;
     for (u32 i = tid; i < kWaves * kBins; i += kBlock) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
 
@@ -194,6 +204,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Batch b, const K* keys, cons
         ko[g] = key;
         vo[g] = s_vals[i];
     }
+    __syncthreads();
+    }
 }
 
 // Scratch words needed for counts (per batch).
@@ -211,7 +223,10 @@ bool sort(hipStream_t st, const Batch& b, Source first, const K* src_keys, const
           Profiler* prof = nullptr, const char* scatter_name = nullptr) {
     const u32 tiles = (b.cap + kTile - 1) / kTile;
     if (tiles == 0 || b.segs == 0) return true;
-    const dim3 grid(tiles, b.segs), scan_grid(kBins, b.segs);
+    // Device-counted batches loop over the tiles they need inside a
+    // bounded grid, so a short list costs a short launch.
+    const u32 gx = b.d_count ? (tiles < 1024u ? tiles : 1024u) : tiles;
+    const dim3 grid(gx, b.segs), scan_grid(kBins, b.segs);
     const int passes = bits / 8;
     const K* in_k = src_keys;
     const u32* in_v = src_vals;
@@ -222,12 +237,12 @@ bool sort(hipStream_t st, const Batch& b, Source first, const K* src_keys, const
         const int shift = 8 * p;
         if (p == 0 && first == Source::SapPlanes) {
             hipLaunchKernelGGL((k_upsweep<K, Source::SapPlanes>), grid, dim3(kBlock), 0, st, b, in_k, in_v, shift, tiles, counts);
-            hipLaunchKernelGGL(k_tile_scan, scan_grid, dim3(kBlock), 0, st, tiles, counts, totals);
+            hipLaunchKernelGGL(k_tile_scan, scan_grid, dim3(kBlock), 0, st, b, tiles, counts, totals);
             ProfScope ps(scatter_name ? prof : nullptr, scatter_name ? scatter_name : "", st);
             hipLaunchKernelGGL((k_scatter<K, Source::SapPlanes>), grid, dim3(kBlock), 0, st, b, in_k, in_v, out_k, out_v, shift, tiles, counts, totals);
         } else {
             hipLaunchKernelGGL((k_upsweep<K, Source::Arrays>), grid, dim3(kBlock), 0, st, b, in_k, in_v, shift, tiles, counts);
-            hipLaunchKernelGGL(k_tile_scan, scan_grid, dim3(kBlock), 0, st, tiles, counts, totals);
+            hipLaunchKernelGGL(k_tile_scan, scan_grid, dim3(kBlock), 0, st, b, tiles, counts, totals);
             ProfScope ps(scatter_name ? prof : nullptr, scatter_name ? scatter_name : "", st);
             hipLaunchKernelGGL((k_scatter<K, Source::Arrays>), grid, dim3(kBlock), 0, st, b, in_k, in_v, out_k, out_v, shift, tiles, counts, totals);
         }

@@ -100,7 +100,7 @@ enum Counter : u32 {
     kError,     // non-finite bounds seen
     kOverflow,  // a capacity was exceeded
     kSweep,
-    kZero,      // always 0 (count of an empty list)
+    kLarge,     // this frame's candidate list exceeds the one-block path
     kCounters
 };
 
@@ -145,7 +145,8 @@ __global__ __launch_bounds__(256) void k_bounds_update(u32 count, const u32* dir
                                                        const float* pos, const float* quat,
                                                        const float* half, const float* off,
                                                        u32* planes, Rec* cur, Rec* prev,
-                                                       u32* seen, u32 stamp, u32* moved, u32* cnt) {
+                                                       u32* seen, u32 stamp, u32* moved, u32* moved_bits,
+                                                       u32* cnt) {
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const u32 i = dirty ? dirty[t] : t;
@@ -169,6 +170,7 @@ __global__ __launch_bounds__(256) void k_bounds_update(u32 count, const u32* dir
     }
     r.k[7] = (r.k[7] & 0x80000000u) | stamp;
     cur[i] = r;
+    atomicOr(&moved_bits[i >> 5], 1u << (i & 31u));
     const u32 slot = atomicAdd(&cnt[kMoved], 1u);
     if (slot < n) moved[slot] = i;
 }
@@ -219,20 +221,6 @@ __global__ __launch_bounds__(256) void k_sweep(u32 n2, const u32* xpay, const Re
     }
 }
 
-// Lower bound of composite (key, payload) in a sorted endpoint array.
-__device__ inline u32 lower_bound_ep(const u32* key, const u32* pay, u32 n, u32 vk, u32 vp) {
-    u32 lo = 0, hi = n;
-    while (lo < hi) {
-        const u32 mid = (lo + hi) >> 1;
-        const u32 k = key[mid];
-        if (k < vk || (k == vk && pay[mid] < vp))
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // Incremental crossings. A pair can change overlap status only if, on some
 // axis, a min endpoint of one box and a max endpoint of the other changed
 // their relative order. For every endpoint e that moved from v to v', the
@@ -244,7 +232,8 @@ __device__ inline u32 lower_bound_ep(const u32* key, const u32* pay, u32 n, u32 
 __global__ __launch_bounds__(256) void k_cross(u32 n2, const u32* cnt_moved, const u32* moved,
                                                const u32* new_key, const u32* new_pay,
                                                const u32* old_key, const u32* old_pay,
-                                               const Rec* cur, const Rec* prev, u32 stamp,
+                                               const u32* new_rank, const u32* old_rank,
+                                               const u32* moved_bits, const Rec* cur, const Rec* prev,
                                                u64* out, u32* cnt, u32 cap) {
     const u32 lane = lane_id();
     const u32 n_work = min(*cnt_moved, n2 >> 1) * 3u;
@@ -270,7 +259,7 @@ __global__ __launch_bounds__(256) void k_cross(u32 n2, const u32* cnt_moved, con
             // e sits at its new value in the new array and at its old value
             // in the old one; the crossed entries lie between e and the
             // composite (other value, pe), walking up or down from e.
-            const u32 own = lower_bound_ep(K, P, n2, arr ? v_old : v_new, pe);
+            const u32 own = (arr ? old_rank : new_rank)[static_cast<std::size_t>(axis) * n2 + pe];
             const u32 bound_v = arr ? v_new : v_old;
             const bool up = (arr != 0) == right;  // old array: toward v_new
             for (u32 base = 0;; base += kWave) {
@@ -291,12 +280,12 @@ __global__ __launch_bounds__(256) void k_cross(u32 n2, const u32* cnt_moved, con
                 if (inside) {
                     const u32 pf = P[j];
                     const u32 b = pf >> 1;
-                    if ((pf & 1u) != kind && b != a) {
+                    const bool b_moved = (moved_bits[b >> 5] >> (b & 31u)) & 1u;
+                    // an unmoved endpoint sits at the same value in both
+                    // arrays: the new-array scan already saw it
+                    if ((pf & 1u) != kind && b != a && (b_moved || !arr)) {
                         const Rec bn = cur[b];
-                        const bool b_moved = (bn.k[7] & 0x7fffffffu) == stamp;
-                        // an unmoved endpoint sits at the same value in both
-                        // arrays: the new-array scan already saw it
-                        if (pair_allowed(an, bn) && !(arr && !b_moved)) {
+                        if (pair_allowed(an, bn)) {
                             const Rec bo = b_moved ? prev[b] : bn;
                             const bool ov_new = overlap3(an, bn);
                             const bool ov_old = overlap3(ao, bo);
@@ -352,8 +341,86 @@ struct Copy {
     __device__ void operator()(u32 i, u32 o) const { dst[o] = src[i]; }
 };
 
-// Sorted unique delta = [lost..., found|bit63...] -> lost[], found[].
-__global__ void k_split(u32 cap, const u64* delta, const u32* d_n, u64* found, u64* lost, u32* cnt) {
+// rank[axis][payload] = sorted position (the SapBox1D min/max indices of
+// the reference GPU path, here for every endpoint of every axis).
+__global__ __launch_bounds__(256) void k_rank(u32 n2, const u32* pay, u32* rank) {
+    const std::size_t total = 3ull * n2;
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+        const std::size_t axis = i / n2;
+        const u32 pos = static_cast<u32>(i - axis * n2);
+        rank[axis * n2 + pay[i]] = pos;
+    }
+}
+
+// One-workgroup path for this frame's candidate changes when they fit in
+// LDS: bitonic sort, unique, split into lost (no kFoundBit) then found.
+// Larger lists raise kLarge and leave the work to the multi-block path.
+constexpr u32 kSmall = 8192;
+__global__ __launch_bounds__(1024) void k_lists_small(const u64* in, const u32* d_n, u32 cap, u64* found,
+                                                      u64* lost, u32* cnt) {
+    __shared__ u64 s[kSmall];
+    __shared__ u32 scratch[1024 / kWave + 1];
+    const u32 n = min(*d_n, cap);
+    const u32 tid = threadIdx.x;
+    if (n > kSmall) {
+        if (tid == 0) cnt[kLarge] = 1;
+        return;
+    }
+    u32 p2 = 64;
+    while (p2 < n) p2 <<= 1;
+    for (u32 i = tid; i < p2; i += 1024) s[i] = i < n ? in[i] : ~0ull;
+    __syncthreads();
+    for (u32 k = 2; k <= p2; k <<= 1) {
+        for (u32 j = k >> 1; j > 0; j >>= 1) {
+            for (u32 i = tid; i < p2; i += 1024) {
+                const u32 ixj = i ^ j;
+                if (ixj > i) {
+                    const u64 a = s[i], b = s[ixj];
+                    const bool asc = (i & k) == 0;
+                    if ((a > b) == asc) {
+                        s[i] = b;
+                        s[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    constexpr u32 per = kSmall / 1024;
+    const u32 first = tid * per;
+    u32 cu = 0, cl = 0;
+    for (u32 q = 0; q < per; ++q) {
+        const u32 i = first + q;
+        if (i < n && (i == 0 || s[i] != s[i - 1])) {
+            ++cu;
+            cl += (s[i] & kFoundBit) ? 0u : 1u;
+        }
+    }
+    u32 total_u, total_l;
+    u32 u = block_exclusive_scan<1024>(cu, scratch, &total_u);
+    (void)block_exclusive_scan<1024>(cl, scratch, &total_l);
+    for (u32 q = 0; q < per; ++q) {
+        const u32 i = first + q;
+        if (i < n && (i == 0 || s[i] != s[i - 1])) {
+            const u64 x = s[i];
+            if (u < total_l)
+                lost[u] = x;
+            else
+                found[u - total_l] = x & kPairMask;
+            ++u;
+        }
+    }
+    if (tid == 0) {
+        cnt[kUnique] = total_u;
+        cnt[kLost] = total_l;
+        cnt[kFound] = total_u - total_l;
+    }
+}
+
+// Unique entries of the sorted large list -> lost / found (gated on kLarge).
+__global__ void k_split_large(u32 cap, const u64* delta, const u32* d_n, u64* found, u64* lost, u32* cnt) {
+    if (cnt[kLarge] == 0) return;
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
     const u32 n = min(*d_n, cap);
     const u32 boundary = lower_bound64(delta, n, kFoundBit);
@@ -370,26 +437,31 @@ __global__ void k_split(u32 cap, const u64* delta, const u32* d_n, u64* found, u
     }
 }
 
-// (kept ∪ found), both sorted and disjoint -> out (sorted).
-__global__ void k_merge(u32 cap, const u64* kept, const u32* d_nk, const u64* found, const u32* d_nf,
-                        u64* out, u32* d_nout, u32* cnt) {
+// new list = (old - lost) + found in one pass: lost is a subset of old and
+// found is disjoint from it, so every output position is a rank sum.
+__global__ void k_apply_delta(u32 cap, const u64* old, const u32* d_no, const u64* found, const u32* d_nf,
+                              const u64* lost, const u32* d_nl, u64* out, u32* d_nout, u32* cnt) {
+    const u32 no = *d_no, nf = *d_nf, nl = *d_nl;
+    const u32 total = no - nl + nf;
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-    const u32 nk = *d_nk, nf = *d_nf;
     if (t == 0) {
-        *d_nout = min(nk + nf, cap);
-        if (nk + nf > cap) atomicOr(&cnt[kOverflow], 2u);
+        *d_nout = min(total, cap);
+        if (total > cap) atomicOr(&cnt[kOverflow], 2u);
     }
-    for (u32 i = t; i < nk + nf; i += gridDim.x * blockDim.x) {
-        if (i < nk) {
-            const u64 x = kept[i];
-            const u32 pos = i + lower_bound64(found, nf, x);
-            if (pos < cap) out[pos] = x;
+    for (u32 i = t; i < no + nf; i += gridDim.x * blockDim.x) {
+        u32 pos;
+        u64 x;
+        if (i < no) {
+            x = old[i];
+            const u32 l = lower_bound64(lost, nl, x);
+            if (l < nl && lost[l] == x) continue;
+            pos = i - l + lower_bound64(found, nf, x);
         } else {
-            const u32 j = i - nk;
-            const u64 f = found[j];
-            const u32 pos = j + lower_bound64(kept, nk, f);
-            if (pos < cap) out[pos] = f;
+            const u32 j = i - no;
+            x = found[j];
+            pos = lower_bound64(old, no, x) - lower_bound64(lost, nl, x) + j;
         }
+        if (pos < cap) out[pos] = x;
     }
 }
 
@@ -401,7 +473,7 @@ struct Sap {
     u32 frame = 0;
     int cur = 0;  // index of the current sorted arrays / pair list
     bool built = false;
-    DevBuf<u32> planes, seen, moved, counters, sorted_key[2], sorted_val[2], tmp_key, tmp_val,
+    DevBuf<u32> planes, seen, moved, moved_bits, counters, sorted_key[2], sorted_val[2], rank[2], tmp_key, tmp_val,
         radix_counts, radix_totals, sel_tiles, dummy_a, dummy_b;
     DevBuf<Rec> rec_cur, rec_prev;
     DevBuf<u64> pairs[2], work_a, work_b, found, lost, kept;
@@ -410,8 +482,8 @@ struct Sap {
     u32 list_cap() const { return std::max(pair_cap, delta_cap); }
 };
 
-void sort_u64(Sap& s, hipStream_t st, const u32* d_count, u64* keys, u64* tmp) {
-    radix::Batch b{s.list_cap(), 1, s.list_cap(), 0, d_count, nullptr, 0};
+void sort_u64(Sap& s, hipStream_t st, const u32* d_count, u64* keys, u64* tmp, const u32* gate = nullptr) {
+    radix::Batch b{s.list_cap(), 1, s.list_cap(), 0, d_count, nullptr, 0, gate};
     const bool in_a = radix::sort<u64>(st, b, radix::Source::Arrays, keys, s.dummy_a.p, keys,
                                        s.dummy_a.p, tmp, s.dummy_b.p, 64, s.radix_counts.p,
                                        s.radix_totals.p);
@@ -442,11 +514,13 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
         if (n) hipLaunchKernelGGL(k_bounds_full, dim3(grid_for(n, 256)), dim3(256), 0, st, n, pos, quat, half, off,
                                   s.planes.p, s.rec_cur.p, stamp, cnt);
     } else {
+        const u32 words = (n + 31) / 32;
+        if (words) hipLaunchKernelGGL(k_fill, dim3(grid_for(words, 256)), dim3(256), 0, st, s.moved_bits.p, words, 0u);
         const u32 count = dirty ? n_dirty : n;
         if (count)
             hipLaunchKernelGGL(k_bounds_update, dim3(grid_for(count, 256)), dim3(256), 0, st, count, dirty, n, pos,
                                quat, half, off, s.planes.p, s.rec_cur.p, s.rec_prev.p, s.seen.p, stamp,
-                               s.moved.p, cnt);
+                               s.moved.p, s.moved_bits.p, cnt);
     }
     FYZZ_HIP_CHECK(hipGetLastError());
     }
@@ -462,6 +536,8 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
             FYZZ_HIP_CHECK(hipMemcpyAsync(s.sorted_key[nxt].p, s.tmp_key.p, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, st));
             FYZZ_HIP_CHECK(hipMemcpyAsync(s.sorted_val[nxt].p, s.tmp_val.p, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, st));
         }
+        hipLaunchKernelGGL(k_rank, dim3(std::min<u32>(grid_for(3ull * n2, 256), 4096u)), dim3(256), 0, st, n2,
+                           s.sorted_val[nxt].p, s.rank[nxt].p);
     }
     if (full) {
         long pi = prof->begin("sap.pairs", st);
@@ -481,21 +557,22 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
         long pi = prof->begin("sap.pairs", st);
         hipLaunchKernelGGL(k_cross, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 3 * kWave, 256), 4096u)), dim3(256), 0, st, n2,
                            cnt + kMoved, s.moved.p, s.sorted_key[nxt].p, s.sorted_val[nxt].p,
-                           s.sorted_key[s.cur].p, s.sorted_val[s.cur].p, s.rec_cur.p, s.rec_prev.p, stamp,
-                           s.work_a.p, cnt, s.delta_cap);
+                           s.sorted_key[s.cur].p, s.sorted_val[s.cur].p, s.rank[nxt].p, s.rank[s.cur].p,
+                           s.moved_bits.p, s.rec_cur.p, s.rec_prev.p, s.work_a.p, cnt, s.delta_cap);
         FYZZ_HIP_CHECK(hipGetLastError());
         prof->end("sap.pairs", pi, st);
         ProfScope ps(prof, "sap.lists", st);
-        sort_u64(s, st, cnt + kDelta, s.work_a.p, s.work_b.p);
+        hipLaunchKernelGGL(k_lists_small, dim3(1), dim3(1024), 0, st, s.work_a.p, cnt + kDelta, s.delta_cap,
+                           s.found.p, s.lost.p, cnt);
+        // multi-block path, each launch gated on kLarge
+        sort_u64(s, st, cnt + kDelta, s.work_a.p, s.work_b.p, cnt + kLarge);
         sel::select(st, s.delta_cap, cnt + kDelta, Unique{s.work_a.p}, Copy{s.work_a.p, s.work_b.p},
-                    s.sel_tiles.p, cnt + kUnique);
-        hipLaunchKernelGGL(k_split, dim3(std::min<u32>(grid_for(s.delta_cap, 256), 1024u)), dim3(256), 0, st, s.delta_cap, s.work_b.p,
-                           cnt + kUnique, s.found.p, s.lost.p, cnt);
-        sel::select(st, s.pair_cap, cnt + pairs_cur, NotIn{s.pairs[s.cur].p, s.lost.p, cnt + kLost},
-                    Copy{s.pairs[s.cur].p, s.kept.p}, s.sel_tiles.p, cnt + kKept);
-        hipLaunchKernelGGL(k_merge, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(s.pair_cap) + s.delta_cap, 256), 2048u)), dim3(256), 0, st,
-                           s.pair_cap, s.kept.p, cnt + kKept, s.found.p, cnt + kFound, s.pairs[nxt].p,
-                           cnt + pairs_nxt, cnt);
+                    s.sel_tiles.p, cnt + kUnique, cnt + kLarge);
+        hipLaunchKernelGGL(k_split_large, dim3(std::min<u32>(grid_for(s.delta_cap, 256), 1024u)), dim3(256), 0, st,
+                           s.delta_cap, s.work_b.p, cnt + kUnique, s.found.p, s.lost.p, cnt);
+        hipLaunchKernelGGL(k_apply_delta, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(s.pair_cap) + s.delta_cap, 256), 2048u)), dim3(256), 0, st,
+                           s.pair_cap, s.pairs[s.cur].p, cnt + pairs_cur, s.found.p, cnt + kFound, s.lost.p,
+                           cnt + kLost, s.pairs[nxt].p, cnt + pairs_nxt, cnt);
     }
     FYZZ_HIP_CHECK(hipGetLastError());
     s.cur = nxt;
@@ -575,10 +652,12 @@ int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_
         s.planes.reserve(6ull * n);
         s.seen.reserve(n);
         s.moved.reserve(n);
+        s.moved_bits.reserve((n + 31) / 32 + 1);
         s.counters.reserve(sap::kCounters);
         for (int i = 0; i < 2; ++i) {
             s.sorted_key[i].reserve(3ull * n2);
             s.sorted_val[i].reserve(3ull * n2);
+            s.rank[i].reserve(3ull * n2);
             s.pairs[i].reserve(s.pair_cap);
         }
         s.tmp_key.reserve(3ull * n2);

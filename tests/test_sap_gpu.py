@@ -104,13 +104,14 @@ def test_incremental_sap_matches_reference_golden(ctx, torch_cuda, name):
     bp.close()
 
 
-def _run_frames(ctx, orc, torch, n, frames, seed, scale, move_scale, fraction, rotated, check_endpoints=True):
+def _run_frames(ctx, orc, torch, n, frames, seed, scale, move_scale, fraction, rotated, check_endpoints=True,
+                delta_capacity=None):
     from fyzz_amd import sap
     half, pos, quat, off = random_boxes(seed, n, extent_scale=scale, rotated=rotated)
     body = np.arange(n, dtype=np.uint32)
     body[1::11] = body[0::11][: len(body[1::11])]
     dyn = (scenes.uniform(seed, 31, n) < 0.8).astype(np.uint8)
-    bp = sap.SapBroadphase(ctx, body, dyn)
+    bp = sap.SapBroadphase(ctx, body, dyn, delta_capacity=delta_capacity)
     dq, dh, do = dev(torch, quat), dev(torch, half), dev(torch, off)
     prev = np.zeros((0, 2), np.uint32)
     for f in range(frames):
@@ -214,3 +215,10 @@ def bp_bounds(ctx, torch, pos, quat, half, off):
     b, _ = sap.box_bounds(ctx, dev(torch, pos), dev(torch, quat), dev(torch, half), dev(torch, off))
     torch.cuda.synchronize()
     return b
+
+
+def test_incremental_sap_large_change_lists(ctx, orc, torch_cuda):
+    # every box moves far: the per-frame candidate list exceeds the
+    # one-workgroup path and the multi-block sort/unique path runs
+    _run_frames(ctx, orc, torch_cuda, 30000, 3, 12, 1.5, 15.0, 1.0, False, check_endpoints=False,
+                delta_capacity=4_000_000)
