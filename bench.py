@@ -29,6 +29,11 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 METRIC = "body-steps/s at 100k-box pile + 4096-scene batch vs plain; SAP/radix HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes moved by one frame's endpoint radix sort (4 passes of tile
+# histogram + tile scan + scatter over 3 x 2n keys, n = 1e6), from the two
+# rocprofv3 PMC passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 after the copy
+# calibration) summarised in profiles/sap_1m_incremental.md.
+SAP_SORT_TRAFFIC_BYTES = 604_000_000
 
 
 def parse():
@@ -199,7 +204,7 @@ def bench_sap(args):
                    "endpoint_keys_per_s": keys * args.steps / (ms * 1e-3)},
         "roofline": {"kernel": "endpoint radix sort (3 axes, 4 passes x upsweep+tile-scan+scatter)",
                      "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": SAP_SORT_TRAFFIC_BYTES,
                      "algorithmic_bytes_per_launch": sort_bytes},
         "stages": stages,
     }

@@ -70,3 +70,105 @@ class Oracle:
         self.lib.orc_found_lost(C.c_size_t(len(old)), P(old), C.c_size_t(len(new)), P(new), P(f), C.byref(nf),
                                 P(l), C.byref(nl))
         return f[: 2 * nf.value].reshape(-1, 2), l[: 2 * nl.value].reshape(-1, 2)
+
+
+class OraclePcm:
+    """PCM entry points of the oracle (see oracle/orc_pcm.cpp)."""
+
+    CACHE_FLOATS = 56
+
+    def __init__(self, lib):
+        self.lib = lib
+        self.lib.orc_pcm_cache_floats.restype = C.c_uint32
+        assert self.lib.orc_pcm_cache_floats() == self.CACHE_FLOATS
+
+    def candidates(self, a, b, cd):
+        out = np.zeros(64 * 10, np.float32)
+        cnt = C.c_uint32()
+        ok = C.c_int()
+        rc = self.lib.orc_pcm_box_box_candidates(P(np.ascontiguousarray(a, np.float32)),
+                                                 P(np.ascontiguousarray(b, np.float32)), C.c_float(cd), P(out),
+                                                 C.c_uint32(64), C.byref(cnt), C.byref(ok))
+        assert rc == 0
+        return bool(ok.value), out.reshape(64, 10)[: cnt.value]
+
+    def batch_selection(self, cands, tol=1.0):
+        sel = np.zeros(4, np.uint8)
+        assert self.lib.orc_pcm_batch_selection(P(np.ascontiguousarray(cands, np.float32)), C.c_uint32(len(cands)),
+                                                C.c_float(tol), P(sel)) == 0
+        return sel
+
+    def gjk_epa(self, a, b, cd):
+        out = np.zeros(10, np.float32)
+        ok = C.c_int()
+        self.lib.orc_pcm_gjk_epa(P(np.ascontiguousarray(a, np.float32)), P(np.ascontiguousarray(b, np.float32)),
+                                 C.c_float(cd), P(out), C.byref(ok))
+        return bool(ok.value), out
+
+    def box_box(self, first, second, cd, cache):
+        pts = np.zeros(4 * 7, np.float32)
+        fl = C.c_uint32()
+        n = self.lib.orc_pcm_box_box(P(np.ascontiguousarray(first, np.float32)),
+                                     P(np.ascontiguousarray(second, np.float32)), C.c_float(cd), P(cache), P(pts),
+                                     C.byref(fl))
+        if n < 0:
+            raise OverflowError("PCM box/box manifold invalid")
+        return n, pts.reshape(4, 7)[:n], fl.value
+
+    def plane_box(self, plane_pos, box, cd, cache):
+        pts = np.zeros(4 * 7, np.float32)
+        fl = C.c_uint32()
+        n = self.lib.orc_pcm_plane_box(P(np.ascontiguousarray(plane_pos, np.float32)),
+                                       P(np.ascontiguousarray(box, np.float32)), C.c_float(cd), P(cache), P(pts),
+                                       C.byref(fl))
+        if n < 0:
+            raise OverflowError("PCM plane/box manifold invalid")
+        return n, pts.reshape(4, 7)[:n], fl.value
+
+    def new_cache(self):
+        return np.zeros(self.CACHE_FLOATS, np.float32)
+
+
+def replay_pcm_scene(orc, g, on_step=None):
+    """Replays the narrowphase of a recorded reference scene with the oracle.
+    Pairs come from the oracle SAP on each step's start poses (contact
+    offset 0.02 per shape, the floor as a plane pair); caches persist by
+    pair. Calls on_step(step, manifolds) where manifolds maps a shape pair
+    to (count, points[count, 7]) with published normals (plane pairs flip
+    the Gu plane-first normal, as the scene does before Dy)."""
+    pcm = OraclePcm(orc.lib)
+    half, ids = g["half"], g["ids"]
+    floor = int(g["floor_id"][0])
+    n = len(ids)
+    caches = {}
+    for s in range(int(g["steps"][0])):
+        pose = g["poses"][s]
+        pos, quat = pose[:, :3].copy(), pose[:, 3:].copy()
+        b = orc.box_bounds(pos, quat, half, np.full(n, 0.02, np.float32))
+        pairs = orc.sap_pairs(b, shape_id=ids, plane_ids=[floor])
+        slot = {int(v): i for i, v in enumerate(ids)}
+        new_caches = {}
+        manifolds = {}
+        for pa, pb in pairs:
+            key = (int(pa), int(pb))
+            cache = caches.get(key)
+            if cache is None:
+                cache = pcm.new_cache()
+            if pa == floor or pb == floor:
+                bx = slot[int(pb if pa == floor else pa)]
+                rec = np.concatenate([half[bx], pos[bx], quat[bx]])
+                cnt, pts, _ = pcm.plane_box(np.array([0.0, 0.0, 0.0], np.float32), rec, 0.04, cache)
+                pts = pts.copy()
+                pts[:, 3:6] *= -1.0
+            else:
+                # two dynamics: the Gu pair puts the higher body id first
+                first, second = slot[int(pb)], slot[int(pa)]
+                r1 = np.concatenate([half[first], pos[first], quat[first]])
+                r2 = np.concatenate([half[second], pos[second], quat[second]])
+                cnt, pts, _ = pcm.box_box(r1, r2, 0.04, cache)
+            new_caches[key] = cache
+            if cnt:
+                manifolds[key] = (cnt, pts)
+        caches = new_caches
+        if on_step:
+            on_step(s, pairs, manifolds)
