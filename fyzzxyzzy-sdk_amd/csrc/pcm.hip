@@ -1,0 +1,1388 @@
+// PCM narrowphase on gfx950: persistent box/box and plane/box contact
+// manifolds with the sparse GJK/EPA continuation.
+//
+// Semantics follow the reference NativeCpu Gu/Sc code (paths relative to
+// the reference root):
+//   manifold refresh                  src/native/scene.cpp:3432-3465
+//   box/box manifold + thresholds     src/native/scene.cpp:3466-3654
+//   five-point reduction              src/native/scene.cpp:3317-3431
+//   plane/box manifold                src/native/scene.cpp:3774-3955
+//   SAT, reference face, clipping     src/native/GuPCMContactBoxBox.cpp:20-436
+//   GJK/EPA continuation              src/native/GuPCMGjkEpa.cpp:18-566
+// Contacts must agree with the reference to 1e-5 and the manifold cache
+// makes discrete regeneration choices, so every expression keeps the
+// reference's floating-point operation order (contraction is off for the
+// whole library). The reference uses two quaternion rotations with
+// different association: scene.cpp computes (v + w t) + u x t (`srot`),
+// the Gu files v + (t w + u x t) (`grot`).
+//
+// Work split: k_pcm_box_box runs one thread per pair (refresh, regeneration
+// test, SAT + clipping into a 16-slot candidate array, batch reduction,
+// emission). Pairs whose SAT overlaps but clip yields no witness are
+// appended to a device list and finished by k_pcm_gjk (GJK + EPA with
+// 64-vertex/64-facet working sets), the sparse continuation.
+// k_pcm_plane_box runs one thread per dynamic box against the floor.
+#include <algorithm>
+#include <memory>
+#include <string>
+
+#include "../../include/fyzz_hip.h"
+#include "context.hpp"
+
+namespace fyzz_hip::pcm {
+
+struct V {
+    float x, y, z;
+};
+struct Q {
+    float x, y, z, w;
+};
+__device__ inline V vadd(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline V vmul(V a, float f) { return {a.x * f, a.y * f, a.z * f}; }
+__device__ inline V vneg(V a) { return {-a.x, -a.y, -a.z}; }
+__device__ inline float vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline V vcross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ inline float comp(V v, unsigned i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+// std::min / std::max / std::clamp semantics (first argument on ties).
+__device__ inline float smin(float a, float b) { return b < a ? b : a; }
+__device__ inline float smax(float a, float b) { return a < b ? b : a; }
+__device__ inline float sclamp(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+constexpr float kFltMax = 3.402823466e+38f;
+constexpr float kFltEps = 1.1920928955078125e-7f;
+
+// scene.cpp:57-66
+__device__ inline V srot(Q q, V v) {
+    const V u{q.x, q.y, q.z};
+    const V c = vcross(u, v);
+    const V t{c.x * 2.0f, c.y * 2.0f, c.z * 2.0f};
+    const V d = vcross(u, t);
+    return {v.x + q.w * t.x + d.x, v.y + q.w * t.y + d.y, v.z + q.w * t.z + d.z};
+}
+__device__ inline V srot_inv(Q q, V v) { return srot(Q{-q.x, -q.y, -q.z, q.w}, v); }
+// scene.cpp:101-110
+__device__ inline Q qmul(Q a, Q b) {
+    return {a.w * b.x + b.w * a.x + a.y * b.z - a.z * b.y, a.w * b.y + b.w * a.y + a.z * b.x - a.x * b.z,
+            a.w * b.z + b.w * a.z + a.x * b.y - a.y * b.x, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+// GuPCMContactBoxBox.cpp:29-37 / GuPCMGjkEpa.cpp:34-40
+__device__ inline V grot(Q q, V v) {
+    const V u{q.x, q.y, q.z};
+    const V t = vmul(vcross(u, v), 2.0f);
+    return vadd(v, vadd(vmul(t, q.w), vcross(u, t)));
+}
+__device__ inline V grot_inv(Q q, V v) {
+    const V u{-q.x, -q.y, -q.z};
+    const V t = vmul(vcross(u, v), 2.0f);
+    return vadd(v, vadd(vmul(t, q.w), vcross(u, t)));
+}
+
+struct Pt {  // persistent witness: local_a, local_b, normal_b, separation
+    V a, b, n;
+    float s;
+};
+
+// Cache record, 224 B: Scene::PcmManifoldCache (scene.h:1448-1459)
+// without the write-only GJK warm start.
+struct alignas(16) Cache {
+    Pt pts[4];
+    V rel_pos;
+    Q rel_rot, orient_a, orient_b;
+    u32 flags;  // count | initialized << 8
+};
+static_assert(sizeof(Cache) == 224, "cache layout");
+
+struct Box3 {
+    V half, pos;
+    Q rot;
+};
+__device__ inline Box3 load_box(const float* pos, const float* quat, const float* half, u32 s) {
+    return {{half[3 * s], half[3 * s + 1], half[3 * s + 2]},
+            {pos[3 * s], pos[3 * s + 1], pos[3 * s + 2]},
+            {quat[4 * s], quat[4 * s + 1], quat[4 * s + 2], quat[4 * s + 3]}};
+}
+
+// ---------------- GuPCMContactBoxBox.cpp ----------------
+struct GBox {
+    V half, pos;
+    Q rot;
+    V axis[3];
+};
+__device__ inline GBox make_gbox(const Box3& b) {
+    GBox g{b.half, b.pos, b.rot, {}};
+    g.axis[0] = grot(b.rot, {1, 0, 0});
+    g.axis[1] = grot(b.rot, {0, 1, 0});
+    g.axis[2] = grot(b.rot, {0, 0, 1});
+    return g;
+}
+struct Frame {
+    V x, y, z, p;
+};
+__device__ inline V f_project(const Frame& f, V w) {
+    const V d = vsub(w, f.p);
+    return {vdot(d, f.x), vdot(d, f.y), vdot(d, f.z)};
+}
+__device__ inline V f_unproject(const Frame& f, V v) {
+    return vadd(f.p, vadd(vadd(vmul(f.x, v.x), vmul(f.y, v.y)), vmul(f.z, v.z)));
+}
+// GuPCMContactBoxBox.cpp:63-83
+__device__ bool poly_contains(const V* poly, V p, V low, V high) {
+    if (p.x < low.x || p.y < low.y || p.x > high.x || p.y > high.y) return false;
+    unsigned hits = 0;
+    for (unsigned i = 0, j = 3; i < 4; j = i++) {
+        const V a = poly[j], b = poly[i];
+        if ((p.x == a.x && p.y == a.y) || (p.x == b.x && p.y == b.y)) return true;
+        if ((a.y > p.y) != (b.y > p.y)) {
+            const float dx = b.x - a.x, dy = b.y - a.y;
+            const float part = (p.y - a.y) * dx + (a.x + kFltEps) * dy;
+            const float target = p.x * dy;
+            const bool hit = dy > 0.0f ? part >= target : target >= part;
+            if (hit && ++hits == 2) return false;
+        }
+    }
+    return hits > 0;
+}
+// GuPCMContactBoxBox.cpp:87-104
+__device__ bool segment_slab(V p, V d, V high, V low, float& enter, float& exit) {
+    enter = 0.0f;
+    exit = 1.0f;
+    for (unsigned a = 0; a < 3; ++a) {
+        const float x = comp(p, a), dd = comp(d, a);
+        const float mn = comp(low, a), mx = comp(high, a);
+        if (fabsf(dd) < 1.0e-6f) {
+            if (x < mn || x > mx) return false;
+            continue;
+        }
+        const float one = (mn - x) / dd, two = (mx - x) / dd;
+        enter = smax(enter, smin(one, two));
+        exit = smin(exit, smax(one, two));
+    }
+    return enter <= exit && enter <= 1.0f;
+}
+
+constexpr int kMaxCand = 16;  // 4 inside corners + 4 projected + 8 edge cuts
+
+struct Gen {
+    const GBox* a;
+    const GBox* b;
+    Pt* out;
+    int count;
+    __device__ void emit(const Frame& f, V ref, V inc, V normal, float sep, bool flipped) {
+        if (count >= kMaxCand) return;  // unreachable: at most 16 witnesses
+        const V w0 = f_unproject(f, flipped ? inc : ref);
+        const V w1 = f_unproject(f, flipped ? ref : inc);
+        out[count++] = {grot_inv(a->rot, vsub(w0, a->pos)), grot_inv(b->rot, vsub(w1, b->pos)),
+                        grot_inv(b->rot, normal), sep};
+    }
+};
+
+// GuPCMContactBoxBox.cpp:119-178
+__device__ void clip_contacts(Gen& g, const Frame& f, const V* poly, V inc_n, float fx, float fy, V normal,
+                              float cd, bool flipped) {
+    fx *= 1.0001f;
+    fy *= 1.0001f;
+    bool pen[4], in[4];
+    V mn{kFltMax, kFltMax, kFltMax};
+    V mx = vneg(mn);
+    const V high{fx, fy, kFltMax};
+    for (unsigned i = 0; i < 4; ++i) {
+        const V p = poly[i];
+        mn.x = smin(mn.x, p.x);
+        mn.y = smin(mn.y, p.y);
+        mx.x = smax(mx.x, p.x);
+        mx.y = smax(mx.y, p.y);
+        pen[i] = false;
+        in[i] = false;
+        if (-p.z < cd) {
+            pen[i] = true;
+            in[i] = fabsf(p.x) <= fx && fabsf(p.y) <= fy;
+            if (in[i]) g.emit(f, {p.x, p.y, 0}, p, normal, -p.z, flipped);
+        }
+    }
+    if (g.count == 4) return;
+    if (inc_n.z != 0.0f) {
+        const V corners[4] = {{fx, fy, 0}, {fx, -fy, 0}, {-fx, fy, 0}, {-fx, -fy, 0}};
+        for (int k = 0; k < 4; ++k) {
+            const V c = corners[k];
+            if (!poly_contains(poly, c, mn, mx)) continue;
+            const float nom = vdot(inc_n, vsub(poly[0], c));
+            const float t = nom / inc_n.z;
+            const float sep = -t;
+            if (cd > sep) g.emit(f, c, {c.x, c.y, t}, normal, sep, flipped);
+        }
+    }
+    const V low{-fx, -fy, -cd - kFltEps};
+    for (unsigned s = 0, e = 3; s < 4; e = s++) {
+        if (!pen[s] && !pen[e]) continue;
+        const bool c0 = pen[s] && in[s];
+        const bool c1 = pen[e] && in[e];
+        if (c0 && c1) continue;
+        const V edge = vsub(poly[e], poly[s]);
+        float t0, t1;
+        if (!segment_slab(poly[s], edge, high, low, t0, t1)) continue;
+        if (!c0) {
+            const V p = vadd(poly[s], vmul(edge, t0));
+            g.emit(f, {p.x, p.y, 0}, p, normal, -p.z, flipped);
+        }
+        if (!c1) {
+            const V p = vadd(poly[s], vmul(edge, t1));
+            g.emit(f, {p.x, p.y, 0}, p, normal, -p.z, flipped);
+        }
+    }
+}
+
+// GuPCMContactBoxBox.cpp:301-436. Returns overlap; count = #witnesses.
+__device__ bool box_box_candidates(const Box3& ba, const Box3& bb, float cd, Pt* out, int& count) {
+    const GBox a = make_gbox(ba), b = make_gbox(bb);
+    Gen gen{&a, &b, out, 0};
+    const V delta = vsub(b.pos, a.pos);
+    float sgn[6], ov[6];
+    constexpr float se = 1.0e-6f;
+    for (unsigned i = 0; i < 6; ++i) {
+        const bool first = i < 3;
+        const GBox& ref = first ? a : b;
+        const GBox& inc = first ? b : a;
+        const unsigned ax = i % 3;
+        const V n = ref.axis[ax];
+        sgn[i] = vdot(delta, n);
+        const float r = (fabsf(vdot(n, inc.axis[0])) + se) * inc.half.x +
+                        (fabsf(vdot(n, inc.axis[1])) + se) * inc.half.y +
+                        (fabsf(vdot(n, inc.axis[2])) + se) * inc.half.z;
+        ov[i] = comp(ref.half, ax) + r - fabsf(sgn[i]) + cd;
+        if (ov[i] < 0.0f) {
+            count = 0;
+            return false;
+        }
+    }
+    for (unsigned i = 0; i < 3; ++i)
+        for (unsigned j = 0; j < 3; ++j) {
+            const V ca = vcross(a.axis[i], b.axis[j]);
+            float ra = 0.0f, rb = 0.0f;
+            for (unsigned k = 0; k < 3; ++k) {
+                if (k != i) ra += (fabsf(vdot(ca, a.axis[k])) + se) * comp(a.half, k);
+                if (k != j) rb += (fabsf(vdot(ca, b.axis[k])) + se) * comp(b.half, k);
+            }
+            if (fabsf(vdot(delta, ca)) > ra + rb + cd) {
+                count = 0;
+                return false;
+            }
+        }
+    unsigned feature = 0;
+    float mo = ov[0];
+    for (unsigned i = 1; i < 6; ++i)
+        if (mo > ov[i]) {
+            mo = ov[i];
+            feature = i;
+        }
+    const bool flipped = feature >= 3;
+    const GBox& inc = flipped ? a : b;
+    const bool np = sgn[feature] <= 0.0f;
+    Frame fr;
+    V normal;
+    float fx, fy;
+    switch (feature) {
+        case 0:
+            normal = np ? a.axis[0] : vneg(a.axis[0]);
+            fr = {np ? vneg(a.axis[2]) : a.axis[2], a.axis[1], normal,
+                  vadd(a.pos, vmul(a.axis[0], np ? -a.half.x : a.half.x))};
+            fx = a.half.z;
+            fy = a.half.y;
+            break;
+        case 1:
+            normal = np ? a.axis[1] : vneg(a.axis[1]);
+            fr = {a.axis[0], np ? vneg(a.axis[2]) : a.axis[2], normal,
+                  vadd(a.pos, vmul(a.axis[1], np ? -a.half.y : a.half.y))};
+            fx = a.half.x;
+            fy = a.half.z;
+            break;
+        case 2:
+            normal = np ? a.axis[2] : vneg(a.axis[2]);
+            fr = {a.axis[0], np ? a.axis[1] : vneg(a.axis[1]), normal,
+                  vadd(a.pos, vmul(a.axis[2], np ? -a.half.z : a.half.z))};
+            fx = a.half.x;
+            fy = a.half.y;
+            break;
+        case 3:
+            normal = np ? b.axis[0] : vneg(b.axis[0]);
+            fr = {np ? b.axis[2] : vneg(b.axis[2]), b.axis[1], vneg(normal),
+                  vadd(b.pos, vmul(b.axis[0], np ? b.half.x : -b.half.x))};
+            fx = b.half.z;
+            fy = b.half.y;
+            break;
+        case 4:
+            normal = np ? b.axis[1] : vneg(b.axis[1]);
+            fr = {b.axis[0], np ? b.axis[2] : vneg(b.axis[2]), vneg(normal),
+                  vadd(b.pos, vmul(b.axis[1], np ? b.half.y : -b.half.y))};
+            fx = b.half.x;
+            fy = b.half.z;
+            break;
+        default:
+            normal = np ? b.axis[2] : vneg(b.axis[2]);
+            fr = {b.axis[0], np ? vneg(b.axis[1]) : b.axis[1], vneg(normal),
+                  vadd(b.pos, vmul(b.axis[2], np ? b.half.z : -b.half.z))};
+            fx = b.half.x;
+            fy = b.half.y;
+            break;
+    }
+    const V search = vneg(fr.z);
+    const float dx = fabsf(vdot(search, inc.axis[0]));
+    const float dy = fabsf(vdot(search, inc.axis[1]));
+    const float dz = fabsf(vdot(search, inc.axis[2]));
+    const unsigned ia = dx >= dy && dx >= dz ? 0 : (dy >= dz ? 1 : 2);
+    const float facing = vdot(search, inc.axis[ia]);
+    const float sext = facing > 0.0f ? -comp(inc.half, ia) : comp(inc.half, ia);
+    const V fc = vadd(inc.pos, vmul(inc.axis[ia], sext));
+    const unsigned ua = ia == 0 ? 1 : 0;
+    const unsigned va = ia == 2 ? 1 : 2;
+    const V u = vmul(inc.axis[ua], comp(inc.half, ua));
+    const V v = vmul(inc.axis[va], comp(inc.half, va));
+    V poly[4];
+    poly[0] = f_project(fr, vadd(fc, vadd(u, v)));
+    poly[1] = f_project(fr, vadd(fc, vsub(u, v)));
+    poly[2] = f_project(fr, vsub(fc, vadd(u, v)));
+    poly[3] = f_project(fr, vsub(fc, vsub(u, v)));
+    const V fnorm = f_project(fr, vadd(fr.p, facing > 0.0f ? vneg(inc.axis[ia]) : inc.axis[ia]));
+    clip_contacts(gen, fr, poly, fnorm, fx, fy, normal, cd, flipped);
+    count = gen.count;
+    return true;
+}
+
+// GuPCMContactBoxBox.cpp:185-299
+__device__ void batch_selection(const Pt* c, int n, float tol, int sel[4]) {
+    int avail[kMaxCand];
+    for (int i = 0; i < n; ++i) avail[i] = i;
+    int rem = n;
+    float deepest_d = c[0].s, shallow = deepest_d;
+    int deepest = 0;
+    for (int i = 1; i < rem; ++i) {
+        const float d = c[i].s;
+        shallow = smax(shallow, d);
+        if (deepest_d > d) {
+            deepest_d = d;
+            deepest = i;
+        }
+    }
+    sel[0] = deepest;
+    avail[deepest] = avail[--rem];
+    int fp = 0;
+    float far2 = vdot(vsub(c[avail[0]].b, c[sel[0]].b), vsub(c[avail[0]].b, c[sel[0]].b));
+    for (int i = 1; i < rem; ++i) {
+        const V d = vsub(c[avail[i]].b, c[sel[0]].b);
+        const float v = vdot(d, d);
+        if (v > far2) {
+            far2 = v;
+            fp = i;
+        }
+    }
+    sel[1] = avail[fp];
+    avail[fp] = avail[--rem];
+    const V base = vsub(c[sel[1]].b, c[sel[0]].b);
+    V tr = vcross(base, c[sel[0]].n);
+    const float tr2 = vdot(tr, tr);
+    if (tr2 > 0.0f) {
+        const float inv = 1.0f / sqrtf(tr2);
+        tr = {tr.x * inv, tr.y * inv, tr.z * inv};
+    } else {
+        tr = c[sel[0]].n;
+    }
+    float greatest = -kFltMax, least = kFltMax;
+    int gp = 0, lp = 0;
+    for (int i = 0; i < rem; ++i) {
+        const float side = vdot(vsub(c[avail[i]].b, c[sel[0]].b), tr);
+        if (side > greatest) {
+            greatest = side;
+            gp = i;
+        }
+        if (side < least) {
+            least = side;
+            lp = i;
+        }
+    }
+    sel[2] = avail[gp];
+    avail[gp] = avail[--rem];
+    if (lp == rem) lp = gp;
+    if (least * greatest > 0.0f) {
+        greatest = -kFltMax;
+        for (int i = 0; i < rem; ++i) {
+            const float side = vdot(vsub(c[avail[i]].b, c[sel[0]].b), tr);
+            if (side > greatest) {
+                greatest = side;
+                lp = i;
+            }
+        }
+    }
+    sel[3] = avail[lp];
+    avail[lp] = avail[--rem];
+    const float th = tol * 0.02f;
+    if (deepest_d < th && shallow > th) {
+        for (int i = 0; i < 4; ++i) {
+            float depth = c[sel[i]].s;
+            if (depth > th) {
+                int rep = rem;
+                for (int j = 0; j < rem; ++j) {
+                    const float cdp = c[avail[j]].s;
+                    if (depth > cdp && th > cdp) {
+                        depth = cdp;
+                        rep = j;
+                    }
+                }
+                if (rep < rem) {
+                    const int t = sel[i];
+                    sel[i] = avail[rep];
+                    avail[rep] = t;
+                }
+            }
+        }
+    }
+}
+
+// ---------------- GuPCMGjkEpa.cpp (box/box polyhedral cores) ----------------
+__device__ inline float box_min_margin(V h) { return 0.05f * smin(smin(h.x, h.y), h.z); }
+struct Sup {
+    V a, b, q;
+    u32 ai, bi;
+};
+struct GPair {
+    V ha, hb, a_in_b;
+    Q rel;
+};
+__device__ inline V box_vertex(V e, V d, u32& idx) {
+    const bool x = d.x > 0.f, y = d.y > 0.f, z = d.z > 0.f;
+    idx = (x ? 1u : 0u) | (y ? 2u : 0u) | (z ? 4u : 0u);
+    return {x ? e.x : -e.x, y ? e.y : -e.y, z ? e.z : -e.z};
+}
+__device__ inline Sup support(const GPair& p, V dir) {
+    const V da = grot(Q{-p.rel.x, -p.rel.y, -p.rel.z, p.rel.w}, dir);
+    Sup s;
+    const V pa = box_vertex(p.ha, da, s.ai);
+    s.b = box_vertex(p.hb, vneg(dir), s.bi);
+    s.a = vadd(p.a_in_b, grot(p.rel, pa));
+    s.q = vsub(s.a, s.b);
+    return s;
+}
+__device__ inline V pair_local_a(const GPair& p, V pb) {
+    return grot(Q{-p.rel.x, -p.rel.y, -p.rel.z, p.rel.w}, vsub(pb, p.a_in_b));
+}
+// GuPCMGjkEpa.cpp:87-112 (explicit fused multiply-add, as written there)
+__device__ bool solve(int n, float a[3][4], float r[3]) {
+    float scale = 0.0f;
+    for (int i = 0; i < n; ++i) scale = smax(scale, fabsf(a[i][i]));
+    if (!(scale > 0.0f) || !isfinite(scale)) return false;
+    const float tol = kFltEps * scale;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int row = col + 1; row < n; ++row)
+            if (fabsf(a[row][col]) > fabsf(a[piv][col])) piv = row;
+        if (!(fabsf(a[piv][col]) > tol)) return false;
+        for (int j = col; j <= n; ++j) {
+            const float t = a[col][j];
+            a[col][j] = a[piv][j];
+            a[piv][j] = t;
+        }
+        const float div = a[col][col];
+        for (int j = col; j <= n; ++j) a[col][j] /= div;
+        for (int row = 0; row < n; ++row)
+            if (row != col) {
+                const float f = a[row][col];
+                for (int j = col; j <= n; ++j) a[row][j] = fmaf(-f, a[col][j], a[row][j]);
+            }
+    }
+    for (int i = 0; i < n; ++i) r[i] = a[i][n];
+    return true;
+}
+struct Closest {
+    V p, a, b;
+    float w[4];
+    unsigned mask;
+    float d2;
+};
+// GuPCMGjkEpa.cpp:119-161
+__device__ Closest closest_simplex(const Sup* sx, int count) {
+    Closest best;
+    best.p = best.a = best.b = V{0, 0, 0};
+    best.w[0] = best.w[1] = best.w[2] = best.w[3] = 0.0f;
+    best.mask = 0;
+    best.d2 = kFltMax;
+    for (unsigned mask = 1; mask < (1u << count); ++mask) {
+        int ids[4] = {0, 0, 0, 0}, n = 0;
+        for (int i = 0; i < count; ++i)
+            if (mask & (1u << i)) ids[n++] = i;
+        float w[4] = {0, 0, 0, 0};
+        w[ids[0]] = 1.0f;
+        if (n > 1) {
+            const V base = sx[ids[0]].q;
+            V dirs[3];
+            float m[3][4];
+            for (int i = 1; i < n; ++i) dirs[i - 1] = vsub(sx[ids[i]].q, base);
+            for (int i = 1; i < n; ++i) {
+                for (int j = 1; j < n; ++j) m[i - 1][j - 1] = vdot(dirs[i - 1], dirs[j - 1]);
+                m[i - 1][n - 1] = -vdot(dirs[i - 1], base);
+            }
+            float co[3];
+            if (!solve(n - 1, m, co)) continue;
+            for (int i = 1; i < n; ++i) {
+                w[ids[i]] = co[i - 1];
+                w[ids[0]] -= co[i - 1];
+            }
+        }
+        bool feasible = true;
+        for (int i = 0; i < n; ++i)
+            if (w[ids[i]] < -1.0e-5f) feasible = false;
+        if (!feasible) continue;
+        V q{0, 0, 0}, pa{0, 0, 0}, pb{0, 0, 0};
+        for (int i = 0; i < count; ++i) {
+            q = vadd(q, vmul(sx[i].q, w[i]));
+            pa = vadd(pa, vmul(sx[i].a, w[i]));
+            pb = vadd(pb, vmul(sx[i].b, w[i]));
+        }
+        const float d = vdot(q, q);
+        if (d < best.d2) {
+            best.p = q;
+            best.a = pa;
+            best.b = pb;
+            for (int k = 0; k < 4; ++k) best.w[k] = w[k];
+            best.mask = mask;
+            best.d2 = d;
+        }
+    }
+    return best;
+}
+__device__ bool contact_from_gjk(const GPair& pr, const Closest& c, Pt& out) {
+    const float dist = sqrtf(c.d2);
+    if (!(dist > 0.0f) || !isfinite(dist)) return false;
+    const V n = vmul(c.p, 1.0f / dist);
+    out = {pair_local_a(pr, c.a), c.b, n, dist};
+    return true;
+}
+struct Facet {
+    int i, j, k;
+    V n;
+    float d;
+    bool valid, in_heap;
+};
+__device__ Facet make_facet(const Sup* v, int i, int j, int k) {
+    Facet f;
+    f.valid = false;
+    f.in_heap = true;
+    const V a = v[i].q, b = v[j].q, c = v[k].q;
+    const V e0 = vsub(b, a), e1 = vsub(c, a);
+    V n = vcross(e0, e1);
+    const float n2 = vdot(n, n);
+    if (!(n2 > kFltEps)) return f;
+    n = vmul(n, 1.0f / sqrtf(n2));
+    f.i = i;
+    f.j = j;
+    f.k = k;
+    f.n = n;
+    f.d = vdot(n, a);
+    f.valid = true;
+    return f;
+}
+__device__ bool init_epa(const GPair& pr, Sup* v, int& count, Facet* fa, int& fc) {
+    if (count <= 0 || count > 4) return false;
+    if (count == 1) {
+        const Sup nx = support(pr, {1, 0, 0});
+        const V d = vsub(nx.q, v[0].q);
+        if (vdot(d, d) == 0.0f) return false;
+        v[count++] = nx;
+    }
+    if (count == 2) {
+        const V e = vsub(v[1].q, v[0].q);
+        const V ab{fabsf(e.x), fabsf(e.y), fabsf(e.z)};
+        V ax{1, 0, 0};
+        if (ab.x > ab.y && ab.z > ab.y)
+            ax = {0, 1, 0};
+        else if (ab.x > ab.z)
+            ax = {0, 0, 1};
+        V perp = vcross(ax, e);
+        const float sq = vdot(perp, perp);
+        if (!(sq > 0.0f)) return false;
+        perp = vmul(perp, 1.0f / sqrtf(sq));
+        v[count++] = support(pr, perp);
+    }
+    if (count == 3) {
+        const Facet f0 = make_facet(v, 0, 1, 2);
+        const Facet f1 = make_facet(v, 1, 0, 2);
+        if (!f0.valid || !f1.valid) return false;
+        fa[fc++] = f0;
+        fa[fc++] = f1;
+        return true;
+    }
+    const V p0 = v[0].q;
+    const V v1 = vsub(v[1].q, p0), v2 = vsub(v[2].q, p0);
+    if (vdot(vcross(v1, v2), vsub(v[3].q, p0)) > 0.0f) {
+        const Sup t = v[1];
+        v[1] = v[2];
+        v[2] = t;
+    }
+    const int faces[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+    for (int f = 0; f < 4; ++f) {
+        const Facet x = make_facet(v, faces[f][0], faces[f][1], faces[f][2]);
+        if (!x.valid) return false;
+        fa[fc++] = x;
+    }
+    return true;
+}
+__device__ bool contact_from_facet(const GPair& pr, const Sup* v, const Facet& f, Pt& out) {
+    const Sup& p = v[f.i];
+    const Sup& q = v[f.j];
+    const Sup& r = v[f.k];
+    const V e0 = vsub(q.q, p.q), e1 = vsub(r.q, p.q);
+    const float a = vdot(e0, e0), b = vdot(e0, e1), c = vdot(e1, e1);
+    const float den = a * c - b * b;
+    if (!(den > kFltEps * a * c)) return false;
+    const float x = -vdot(p.q, e0), y = -vdot(p.q, e1);
+    const float u = (x * c - y * b) / den;
+    const float vv = (y * a - x * b) / den;
+    const float w = 1.0f - u - vv;
+    const V pa = vadd(vadd(vmul(p.a, w), vmul(q.a, u)), vmul(r.a, vv));
+    const V pb = vadd(vadd(vmul(p.b, w), vmul(q.b, u)), vmul(r.b, vv));
+    if (!isfinite(pa.x) || !isfinite(pb.x) || !isfinite(f.d)) return false;
+    const float depth = fabsf(f.d);
+    out = {pair_local_a(pr, pa), pb, vneg(f.n), -depth};
+    const V diff = vsub(pa, pb);
+    return sqrtf(vdot(diff, diff)) <= depth + 1.0e-3f;
+}
+struct EpaWork {
+    Sup v[64];
+    Facet fa[64];
+    int bd_from[32], bd_to[32];
+};
+// GuPCMGjkEpa.cpp:302-395
+__device__ bool epa(const GPair& pr, const Sup* sx, int sc, Pt& out, EpaWork& w) {
+    Sup* v = w.v;
+    Facet* fa = w.fa;
+    int vc = sc;
+    for (int i = 0; i < sc; ++i) v[i] = sx[i];
+    int fc = 0;
+    if (!init_epa(pr, v, vc, fa, fc)) return false;
+    const float eps = 0.1f * smin(box_min_margin(pr.ha), box_min_margin(pr.hb));
+    float upper = kFltMax;
+    for (int it = 0; it < 60; ++it) {
+        int closest = -1;
+        float dist = kFltMax;
+        for (int i = 0; i < fc; ++i)
+            if (fa[i].valid && fa[i].in_heap && fa[i].d < dist) {
+                closest = i;
+                dist = fa[i].d;
+            }
+        if (closest < 0) return false;
+        const Facet sel = fa[closest];
+        const Sup np = support(pr, sel.n);
+        const float proj = vdot(np.q, sel.n);
+        if (fabsf(proj - dist) <= eps) return contact_from_facet(pr, v, sel, out);
+        if (vc == 64) return false;
+        upper = smin(upper, proj);
+        bool dup = false;
+        for (int i = 0; i < vc; ++i)
+            if (v[i].ai == np.ai && v[i].bi == np.bi) {
+                dup = true;
+                break;
+            }
+        if (dup) return contact_from_facet(pr, v, sel, out);
+        const int ni = vc++;
+        v[ni] = np;
+        int bc = 0;
+        for (int i = 0; i < fc; ++i) {
+            Facet& f = fa[i];
+            if (!f.valid) continue;
+            if (vdot(f.n, vsub(np.q, v[f.i].q)) < 0.0f) continue;
+            f.valid = false;
+            const int ef[3] = {f.i, f.j, f.k}, et[3] = {f.j, f.k, f.i};
+            for (int q = 0; q < 3; ++q) {
+                int rev = -1;
+                for (int j = 0; j < bc; ++j)
+                    if (w.bd_from[j] == et[q] && w.bd_to[j] == ef[q]) {
+                        rev = j;
+                        break;
+                    }
+                if (rev >= 0) {
+                    --bc;
+                    w.bd_from[rev] = w.bd_from[bc];
+                    w.bd_to[rev] = w.bd_to[bc];
+                } else {
+                    if (bc == 32) return false;
+                    w.bd_from[bc] = ef[q];
+                    w.bd_to[bc] = et[q];
+                    ++bc;
+                }
+            }
+        }
+        if (bc == 0) return false;
+        int avail = 64 - fc;
+        for (int i = 0; i < fc; ++i) avail += !fa[i].valid;
+        if (bc > avail) return false;
+        for (int i = 0; i < bc; ++i) {
+            Facet f = make_facet(v, w.bd_from[i], w.bd_to[i], ni);
+            if (!f.valid) return false;
+            f.in_heap = f.d <= upper;
+            int slot = -1;
+            for (int j = 0; j < fc; ++j)
+                if (!fa[j].valid) {
+                    slot = j;
+                    break;
+                }
+            if (slot < 0) slot = fc++;
+            fa[slot] = f;
+        }
+    }
+    return false;
+}
+// GuPCMGjkEpa.cpp:396-515 with the polyhedral (no sphere) branches.
+__device__ bool gjk_epa(const GPair& pr, float cd, Pt& out, EpaWork& w) {
+    const float eps = 0.1f * smin(box_min_margin(pr.ha), box_min_margin(pr.hb));
+    constexpr float progress = 1.0f - 0.000225f;
+    Sup sx[4];
+    int count = 0;
+    V closest = vdot(pr.a_in_b, pr.a_in_b) > 0.0f ? pr.a_in_b : V{1, 0, 0};
+    float previous = kFltMax;
+    for (int it = 0; it < 32; ++it) {
+        const float dist = sqrtf(vdot(closest, closest));
+        if (count > 0 && dist <= eps) return epa(pr, sx, count, out, w);
+        if (dist <= 0.0f) return false;
+        const V n = vmul(closest, 1.0f / dist);
+        const Sup s = support(pr, vneg(closest));
+        const float proj = vdot(n, s.q);
+        if (proj > cd) return false;
+        if (count > 0 && proj > dist * progress) {
+            const Closest cur = closest_simplex(sx, count);
+            return cur.mask && contact_from_gjk(pr, cur, out);
+        }
+        for (int i = 0; i < count; ++i)
+            if (s.ai == sx[i].ai && s.bi == sx[i].bi) {
+                if (proj > 0.f) {
+                    const Closest cur = closest_simplex(sx, count);
+                    return cur.mask && contact_from_gjk(pr, cur, out) && out.s <= cd;
+                }
+                return false;
+            }
+        if (count == 4) return epa(pr, sx, count, out, w);
+        sx[count++] = s;
+        const Closest nx = closest_simplex(sx, count);
+        if (!nx.mask) return false;
+        closest = nx.p;
+        const float nd = sqrtf(nx.d2);
+        int wr = 0;
+        for (int r = 0; r < count; ++r)
+            if ((nx.mask & (1u << r)) && nx.w[r] > 1.0e-6f) sx[wr++] = sx[r];
+        if (wr == 0 && count > 0) wr = 1;
+        count = wr;
+        if (nd >= previous) return false;
+        previous = nd;
+    }
+    return false;
+}
+// GuPCMGjkEpa.cpp:527-566, box/box entry.
+__device__ bool box_box_gjk_epa(const Box3& b0, const Box3& b1, float cd, Pt& out, EpaWork& w) {
+    const V rel = grot_inv(b1.rot, vsub(b0.pos, b1.pos));
+    const Q ib{-b1.rot.x, -b1.rot.y, -b1.rot.z, b1.rot.w};
+    const Q r0 = b0.rot;
+    const Q qa{ib.w * r0.x + r0.w * ib.x + ib.y * r0.z - ib.z * r0.y,
+               ib.w * r0.y + r0.w * ib.y + ib.z * r0.x - ib.x * r0.z,
+               ib.w * r0.z + r0.w * ib.z + ib.x * r0.y - ib.y * r0.x,
+               ib.w * r0.w - ib.x * r0.x - ib.y * r0.y - ib.z * r0.z};
+    const GPair pr{b0.half, b1.half, rel, qa};
+    if (!gjk_epa(pr, cd, out, w)) return false;
+    return out.s <= cd;
+}
+
+// ---------------- scene.cpp persistent manifolds ----------------
+__device__ inline u32 c_count(const Cache& c) { return c.flags & 0xffu; }
+__device__ inline bool c_init(const Cache& c) { return (c.flags >> 8) & 1u; }
+__device__ inline void c_set(Cache& c, u32 count, bool init) { c.flags = (count & 0xffu) | (init ? 0x100u : 0u); }
+
+// scene.cpp:3432-3465
+__device__ void refresh(Cache& c, V pos_ab, Q rot_ab, float brk) {
+    const float b2 = brk * brk;
+    u32 count = c_count(c);
+    for (u32 i = count; i > 0; --i) {
+        Pt& p = c.pts[i - 1];
+        const V r = srot(rot_ab, p.a);
+        const V ainb{pos_ab.x + r.x, pos_ab.y + r.y, pos_ab.z + r.z};
+        const V v{ainb.x - p.b.x, ainb.y - p.b.y, ainb.z - p.b.z};
+        const float d = v.x * p.n.x + v.y * p.n.y + v.z * p.n.z;
+        const V pd{v.x - p.n.x * d, v.y - p.n.y * d, v.z - p.n.z * d};
+        const float t2 = pd.x * pd.x + pd.y * pd.y + pd.z * pd.z;
+        if (t2 > b2)
+            c.pts[i - 1] = c.pts[--count];
+        else
+            p.s = d;
+    }
+    c_set(c, count, c_init(c));
+}
+
+__device__ float seg_sq(V p, V a, V b) {
+    const V ab = vsub(b, a), ap = vsub(p, a);
+    const float den = vdot(ab, ab);
+    const float t = den == 0.0f ? 0.0f : sclamp(vdot(ap, ab) / den, 0.0f, 1.0f);
+    const V d{ap.x - t * ab.x, ap.y - t * ab.y, ap.z - t * ab.z};
+    return vdot(d, d);
+}
+__device__ float tri_sq(V p, V a, V b, V c) {
+    const V ab = vsub(b, a), ac = vsub(c, a);
+    const V ap = vsub(p, a), bp = vsub(p, b), cp = vsub(p, c);
+    const float d1 = vdot(ab, ap), d2 = vdot(ac, ap);
+    const float d3 = vdot(ab, bp), d4 = vdot(ac, bp);
+    const float d5 = vdot(ab, cp), d6 = vdot(ac, cp);
+    if (d1 < 0.0f && d2 < 0.0f) return vdot(ap, ap);
+    if (d3 >= 0.0f && d3 >= d4) return vdot(bp, bp);
+    if (d6 >= 0.0f && d6 >= d5) return vdot(cp, cp);
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc < 0.0f && d1 >= 0.0f && d3 < 0.0f) return seg_sq(p, a, b);
+    const float va = d3 * d6 - d5 * d4;
+    if (va < 0.0f && d4 >= d3 && d5 >= d6) return seg_sq(p, b, c);
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb < 0.0f && d2 >= 0.0f && d6 < 0.0f) return seg_sq(p, a, c);
+    const V n{ab.y * ac.z - ab.z * ac.y, ab.z * ac.x - ab.x * ac.z, ab.x * ac.y - ab.y * ac.x};
+    const float den = vdot(n, n);
+    if (den == 0.0f) return 0.0f;
+    const float pr = vdot(n, ap);
+    return pr * pr / den;
+}
+// scene.cpp:3317-3431
+__device__ void reduce_five(Cache& cache, const Pt& nc) {
+    Pt cd[5];
+    for (int i = 0; i < 4; ++i) cd[i] = cache.pts[i];
+    cd[4] = nc;
+    bool chosen[5] = {false, false, false, false, false};
+    int sel[4] = {0, 0, 0, 0};
+    int deepest = 4;
+    for (int i = 0; i < 4; ++i)
+        if (cd[i].s < cd[deepest].s) deepest = i;
+    sel[0] = deepest;
+    chosen[deepest] = true;
+    int far = 0;
+    V d0 = vsub(cd[0].b, cd[deepest].b);
+    float largest = vdot(d0, d0);
+    for (int i = 1; i < 5; ++i)
+        if (!chosen[i]) {
+            const V d = vsub(cd[i].b, cd[deepest].b);
+            const float v = vdot(d, d);
+            if (v > largest) {
+                largest = v;
+                far = i;
+            }
+        }
+    sel[1] = far;
+    chosen[far] = true;
+    largest = -kFltMax;
+    int third = 0;
+    for (int i = 0; i < 5; ++i)
+        if (!chosen[i]) {
+            const float v = seg_sq(cd[i].b, cd[deepest].b, cd[far].b);
+            if (v > largest) {
+                largest = v;
+                third = i;
+            }
+        }
+    sel[2] = third;
+    chosen[third] = true;
+    largest = -kFltMax;
+    int fourth = third;
+    for (int i = 0; i < 5; ++i)
+        if (!chosen[i]) {
+            const float v = tri_sq(cd[i].b, cd[sel[0]].b, cd[sel[1]].b, cd[sel[2]].b);
+            if (v > largest) {
+                largest = v;
+                fourth = i;
+            }
+        }
+    if (chosen[fourth]) {
+        for (int i = 0; i < 3; ++i) cache.pts[i] = cd[sel[i]];
+        c_set(cache, 3, c_init(cache));
+        return;
+    }
+    sel[3] = fourth;
+    chosen[fourth] = true;
+    for (int i = 0; i < 4; ++i) cache.pts[i] = cd[sel[i]];
+    c_set(cache, 4, c_init(cache));
+    int uns = -1;
+    for (int i = 0; i < 5; ++i)
+        if (!chosen[i]) {
+            uns = i;
+            break;
+        }
+    if (uns < 0) return;
+    float closest = kFltMax;
+    int rep = 0;
+    for (int i = 0; i < 4; ++i) {
+        const V d = vsub(cache.pts[i].a, cd[uns].a);
+        const float dd = vdot(d, d);
+        if (dd < closest) {
+            closest = dd;
+            rep = i;
+        }
+    }
+    if (cd[uns].s < cache.pts[rep].s) cache.pts[rep] = cd[uns];
+}
+
+__device__ inline float pcm_margin(V h) { return smin(0.15f * smin(smin(h.x, h.y), h.z), 0.15f); }
+
+// Per-pair flags word written for every pair:
+enum : u32 {
+    kRegenerated = 1u,
+    kGjk = 2u,          // SAT overlapped with no clip witness: GJK/EPA continuation ran
+    kInvalid = 4u,      // reference would throw (invalid normal / non-finite contact)
+};
+
+// Manifold emission (scene.cpp:3598-3653). direct: contacts came from SAT
+// clipping this frame; normal source as in the reference.
+__device__ u32 emit_box_box(const Cache& cache, const Box3& second, float cd, bool direct, bool gjk, V gjk_n,
+                            float* out, u32& flags) {
+    const u32 count = c_count(cache);
+    if (count == 0) return 0;
+    V nb{0, 0, 0};
+    if (gjk)
+        nb = gjk_n;
+    else if (direct)
+        nb = cache.pts[0].n;
+    else
+        for (u32 i = 0; i < count; ++i) {
+            nb.x += cache.pts[i].n.x;
+            nb.y += cache.pts[i].n.y;
+            nb.z += cache.pts[i].n.z;
+        }
+    const float n2 = nb.x * nb.x + nb.y * nb.y + nb.z * nb.z;
+    if (!(n2 > 0.0f) || !isfinite(n2)) {
+        flags |= kInvalid;
+        return 0;
+    }
+    const float il = 1.0f / sqrtf(n2);
+    const V nn{nb.x * il, nb.y * il, nb.z * il};
+    const V normal = srot(second.rot, nn);
+    u32 e = 0;
+    for (u32 i = 0; i < count; ++i) {
+        const Pt& p = cache.pts[i];
+        if (!direct && p.s > cd) continue;
+        const V r = srot(second.rot, p.b);
+        float* o = out + 7 * e;
+        o[0] = r.x + second.pos.x;
+        o[1] = r.y + second.pos.y;
+        o[2] = r.z + second.pos.z;
+        o[3] = normal.x;
+        o[4] = normal.y;
+        o[5] = normal.z;
+        o[6] = p.s;
+        if (!isfinite(o[0]) || !isfinite(o[1]) || !isfinite(o[2]) || !isfinite(o[3]) || !isfinite(o[6]))
+            flags |= kInvalid;
+        ++e;
+    }
+    return e;
+}
+
+struct PairArgs {
+    u32 n_pairs;
+    const u32* first;   // slot of the Gu first shape
+    const u32* second;  // slot of the Gu second shape (normal frame)
+    const u32* cache_index;  // per pair, or null (= pair index)
+    const float* pos;
+    const float* quat;
+    const float* half;
+    float cd;
+    Cache* cache;
+    float* contacts;  // [n_pairs][4][7]
+    u32* counts;
+    u32* flags;
+    u32* gjk_list;    // pairs deferred to the continuation
+    u32* gjk_count;
+};
+
+// scene.cpp:3466-3654 up to the zero-witness continuation.
+__global__ __launch_bounds__(128) void k_pcm_box_box(PairArgs A) {
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.n_pairs) return;
+    const Box3 f = load_box(A.pos, A.quat, A.half, A.first[t]);
+    const Box3 s = load_box(A.pos, A.quat, A.half, A.second[t]);
+    Cache& cref = A.cache[A.cache_index ? A.cache_index[t] : t];
+    Cache cache = cref;
+    u32 flags = 0;
+    const V delta{f.pos.x - s.pos.x, f.pos.y - s.pos.y, f.pos.z - s.pos.z};
+    const V rel = srot_inv(s.rot, delta);
+    const Q rot = qmul(Q{-s.rot.x, -s.rot.y, -s.rot.z, s.rot.w}, f.rot);
+    const float mm = smin(pcm_margin(f.half), pcm_margin(s.half));
+    const u32 initial = c_count(cache);
+    refresh(cache, rel, rot, mm * 0.8f);
+    const u32 cnt = c_count(cache);
+    const bool lost = cnt != initial;
+    const float th[5] = {0.5f, 0.125f, 0.25f, 0.375f, 0.375f};
+    const float qth[5] = {0.9998f, 0.9999f, 0.9999f, 0.9999f, 0.9999f};
+    const bool init = c_init(cache);
+    const float td = init ? smax(smax(fabsf(rel.x - cache.rel_pos.x), fabsf(rel.y - cache.rel_pos.y)),
+                                 fabsf(rel.z - cache.rel_pos.z))
+                          : __builtin_inff();
+    const float qa = f.rot.x * cache.orient_a.x + f.rot.y * cache.orient_a.y + f.rot.z * cache.orient_a.z +
+                     f.rot.w * cache.orient_a.w;
+    const float qb = s.rot.x * cache.orient_b.x + s.rot.y * cache.orient_b.y + s.rot.z * cache.orient_b.z +
+                     s.rot.w * cache.orient_b.w;
+    const float tp = mm * th[cnt];
+    bool regen = !init || lost || td > tp || qa < qth[cnt] || qb < qth[cnt];
+    if (!regen) {
+        auto travel = [](float d, V e) {
+            const float ang = d < 1.0f ? acosf(smax(-1.0f, d)) : 0.0f;
+            return ang * sqrtf(e.x * e.x + e.y * e.y + e.z * e.z);
+        };
+        regen = travel(qa, f.half) > tp || travel(qb, s.half) > tp;
+    }
+    bool direct = false;
+    if (regen) {
+        flags |= kRegenerated;
+        cache.rel_pos = rel;
+        cache.rel_rot = rot;
+        cache.orient_a = f.rot;
+        cache.orient_b = s.rot;
+        c_set(cache, c_count(cache), true);
+        Pt cands[kMaxCand];
+        int count = 0;
+        const bool overlapping = box_box_candidates(f, s, A.cd, cands, count);
+        if (!overlapping) {
+            c_set(cache, 0, true);
+        } else if (count == 0) {
+            // finished by k_pcm_gjk, which owns this pair's outputs
+            flags |= kGjk;
+            cref = cache;
+            A.flags[t] = flags;
+            A.counts[t] = 0;
+            const u32 slot = atomicAdd(A.gjk_count, 1u);
+            A.gjk_list[slot] = t;
+            return;
+        } else if (count <= 4) {
+            for (int i = 0; i < count; ++i) cache.pts[i] = cands[i];
+            c_set(cache, static_cast<u32>(count), true);
+        } else {
+            int sel[4];
+            batch_selection(cands, count, 1.0f, sel);
+            for (int i = 0; i < 4; ++i) cache.pts[i] = cands[sel[i]];
+            c_set(cache, 4, true);
+        }
+        direct = count > 0;
+    }
+    const u32 e = emit_box_box(cache, s, A.cd, direct, false, V{0, 0, 0}, A.contacts + 28ull * t, flags);
+    cref = cache;
+    A.counts[t] = e;
+    A.flags[t] = flags;
+}
+
+// The zero-witness continuation (scene.cpp:3543-3593), one thread per
+// deferred pair; the EPA working set lives in per-thread scratch.
+__global__ __launch_bounds__(64) void k_pcm_gjk(PairArgs A) {
+    const u32 n = min(*A.gjk_count, A.n_pairs);
+    for (u32 w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+        const u32 t = A.gjk_list[w];
+        const Box3 f = load_box(A.pos, A.quat, A.half, A.first[t]);
+        const Box3 s = load_box(A.pos, A.quat, A.half, A.second[t]);
+        Cache& cref = A.cache[A.cache_index ? A.cache_index[t] : t];
+        Cache cache = cref;
+        u32 flags = A.flags[t];
+        const float mm = smin(pcm_margin(f.half), pcm_margin(s.half));
+        EpaWork work;
+        Pt wit;
+        if (!box_box_gjk_epa(f, s, A.cd, wit, work)) {
+            c_set(cache, 0, true);
+            cref = cache;
+            A.counts[t] = 0;
+            continue;
+        }
+        const float rb = mm * 0.05f;
+        const float rsq = rb * rb;
+        bool replaced = false;
+        const u32 cnt = c_count(cache);
+        for (u32 i = 0; i < cnt; ++i) {
+            const Pt& o = cache.pts[i];
+            const V da = vsub(o.a, wit.a), db = vsub(o.b, wit.b);
+            if (smin(vdot(da, da), vdot(db, db)) < rsq) {
+                cache.pts[i] = wit;
+                replaced = true;
+                break;
+            }
+        }
+        if (!replaced) {
+            if (cnt < 4) {
+                cache.pts[cnt] = wit;
+                c_set(cache, cnt + 1, true);
+            } else {
+                reduce_five(cache, wit);
+            }
+        }
+        const u32 e = emit_box_box(cache, s, A.cd, false, true, wit.n, A.contacts + 28ull * t, flags);
+        cref = cache;
+        A.counts[t] = e;
+        A.flags[t] = flags;
+    }
+}
+
+struct PlaneArgs {
+    u32 n;
+    const u32* box;  // slot per plane pair
+    const u32* cache_index;
+    const float* pos;
+    const float* quat;
+    const float* half;
+    V plane_pos;
+    float cd;
+    Cache* cache;
+    float* contacts;
+    u32* counts;
+    u32* flags;
+};
+
+// scene.cpp:3774-3955 (box geometry), followed by the scene's flip of the
+// Gu plane-first normal (scene.cpp:4652-4657): published normal is +y.
+__global__ __launch_bounds__(128) void k_pcm_plane_box(PlaneArgs A) {
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.n) return;
+    const Box3 bx = load_box(A.pos, A.quat, A.half, A.box[t]);
+    Cache& cref = A.cache[A.cache_index ? A.cache_index[t] : t];
+    Cache cache = cref;
+    const V rel{bx.pos.x - A.plane_pos.x, bx.pos.y - A.plane_pos.y, bx.pos.z - A.plane_pos.z};
+    const V h = bx.half;
+    const float margin = smin(0.15f * smin(smin(h.x, h.y), h.z), 0.15f);
+    const float brk = margin * 0.2f;
+    const u32 prev = c_count(cache);
+    refresh(cache, rel, bx.rot, brk);
+    const bool init = c_init(cache);
+    const float md = init ? smax(smax(fabsf(rel.x - cache.rel_pos.x), fabsf(rel.y - cache.rel_pos.y)),
+                                 fabsf(rel.z - cache.rel_pos.z))
+                          : __builtin_inff();
+    const Q q = bx.rot;
+    const Q old = cache.rel_rot;
+    const float rd = q.x * old.x + q.y * old.y + q.z * old.z + q.w * old.w;
+    const bool regen = !init || c_count(cache) != prev || md > brk || rd < 0.99996f;
+    u32 flags = regen ? kRegenerated : 0u;
+    if (regen) {
+        cache.rel_pos = rel;
+        cache.rel_rot = q;
+        Pt c[8];
+        int n = 0;
+        for (unsigned corner = 0; corner < 8u; ++corner) {
+            const V local{(corner & 4u) ? -h.x : h.x, (corner & 2u) ? -h.y : h.y, (corner & 1u) ? -h.z : h.z};
+            const V r = srot(q, local);
+            const V ip{rel.x + r.x, rel.y + r.y, rel.z + r.z};
+            if (ip.y >= A.cd) continue;
+            c[n++] = {local, {ip.x, 0.0f, ip.z}, {0.0f, 1.0f, 0.0f}, ip.y};
+        }
+        if (n <= 4) {
+            for (int i = 0; i < n; ++i) cache.pts[i] = c[i];
+            c_set(cache, static_cast<u32>(n), true);
+        } else {
+            bool chosen[8] = {false, false, false, false, false, false, false, false};
+            int idx[4];
+            int first = 0;
+            for (int i = 1; i < n; ++i)
+                if (c[i].s < c[first].s) first = i;
+            idx[0] = first;
+            chosen[first] = true;
+            int far = 0;
+            V d0 = vsub(c[0].b, c[first].b);
+            float maxd = vdot(d0, d0);
+            for (int i = 1; i < n; ++i) {
+                const V d = vsub(c[i].b, c[first].b);
+                const float dd = vdot(d, d);
+                if (dd > maxd) {
+                    maxd = dd;
+                    far = i;
+                }
+            }
+            idx[1] = far;
+            chosen[far] = true;
+            const V segv = vsub(c[far].b, c[first].b);
+            V side{-segv.z, 0.0f, segv.x};
+            const float ns = vdot(side, side);
+            if (ns > 0.0f) {
+                const float inv = 1.0f / sqrtf(ns);
+                side = {side.x * inv, 0.0f, side.z * inv};
+            } else {
+                side = {0.0f, 1.0f, 0.0f};
+            }
+            float largest = -kFltMax, smallest = kFltMax;
+            int third = 0, fourth = 0;
+            for (int i = 0; i < n; ++i) {
+                if (chosen[i]) continue;
+                const float d = vdot(vsub(c[i].b, c[first].b), side);
+                if (d > largest) {
+                    largest = d;
+                    third = i;
+                }
+                if (d < smallest) {
+                    smallest = d;
+                    fourth = i;
+                }
+            }
+            idx[2] = third;
+            chosen[third] = true;
+            if (smallest * largest > 0.0f) {
+                largest = -kFltMax;
+                for (int i = 0; i < n; ++i) {
+                    if (chosen[i]) continue;
+                    const float d = vdot(vsub(c[i].b, c[first].b), side);
+                    if (d > largest) {
+                        largest = d;
+                        fourth = i;
+                    }
+                }
+            }
+            idx[3] = fourth;
+            chosen[fourth] = true;
+            V centers[4];
+            for (int k = 0; k < 4; ++k) centers[k] = c[idx[k]].b;
+            for (int i = 0; i < n; ++i) {
+                if (chosen[i]) continue;
+                float nearest = kFltMax;
+                int slot = 0;
+                for (int j = 0; j < 4; ++j) {
+                    const V d = vsub(c[i].b, centers[j]);
+                    const float dd = vdot(d, d);
+                    if (dd < nearest) {
+                        nearest = dd;
+                        slot = j;
+                    }
+                }
+                if (c[i].s < c[idx[slot]].s) idx[slot] = i;
+            }
+            for (int k = 0; k < 4; ++k) cache.pts[k] = c[idx[k]];
+            c_set(cache, 4, true);
+        }
+    }
+    u32 e = 0;
+    float* out = A.contacts + 28ull * t;
+    const u32 count = c_count(cache);
+    for (u32 k = 0; k < count; ++k) {
+        const Pt& pt = cache.pts[k];
+        if (pt.s > A.cd) continue;
+        float* o = out + 7 * e;
+        o[0] = A.plane_pos.x + pt.b.x;
+        o[1] = A.plane_pos.y;
+        o[2] = A.plane_pos.z + pt.b.z;
+        o[3] = 0.0f;
+        o[4] = 1.0f;
+        o[5] = 0.0f;
+        o[6] = pt.s;
+        if (!isfinite(o[0]) || !isfinite(o[2]) || !isfinite(o[6])) flags |= kInvalid;
+        ++e;
+    }
+    cref = cache;
+    A.counts[t] = e;
+    A.flags[t] = flags;
+}
+
+__global__ void k_zero1(u32* p) { *p = 0u; }
+
+// Stateless parity entries (one thread per pose pair).
+__global__ void k_candidates(u32 n, const float* a, const float* b, float cd, float* out, u32* counts,
+                             u32* overlapping, u32* selection) {
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float* pa = a + 10ull * t;
+    const float* pb = b + 10ull * t;
+    const Box3 ba{{pa[0], pa[1], pa[2]}, {pa[3], pa[4], pa[5]}, {pa[6], pa[7], pa[8], pa[9]}};
+    const Box3 bb{{pb[0], pb[1], pb[2]}, {pb[3], pb[4], pb[5]}, {pb[6], pb[7], pb[8], pb[9]}};
+    Pt c[kMaxCand];
+    int cnt = 0;
+    const bool ov = box_box_candidates(ba, bb, cd, c, cnt);
+    overlapping[t] = ov ? 1u : 0u;
+    counts[t] = static_cast<u32>(cnt);
+    float* o = out + 160ull * t;
+    for (int i = 0; i < cnt; ++i) {
+        o[10 * i + 0] = c[i].a.x;
+        o[10 * i + 1] = c[i].a.y;
+        o[10 * i + 2] = c[i].a.z;
+        o[10 * i + 3] = c[i].b.x;
+        o[10 * i + 4] = c[i].b.y;
+        o[10 * i + 5] = c[i].b.z;
+        o[10 * i + 6] = c[i].n.x;
+        o[10 * i + 7] = c[i].n.y;
+        o[10 * i + 8] = c[i].n.z;
+        o[10 * i + 9] = c[i].s;
+    }
+    int sel[4] = {255, 255, 255, 255};
+    if (cnt > 4) batch_selection(c, cnt, 1.0f, sel);
+    for (int k = 0; k < 4; ++k) selection[4ull * t + k] = static_cast<u32>(sel[k]);
+}
+
+__global__ __launch_bounds__(64) void k_gjk_epa(u32 n, const float* a, const float* b, float cd, float* out,
+                                                u32* ok) {
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float* pa = a + 10ull * t;
+    const float* pb = b + 10ull * t;
+    const Box3 ba{{pa[0], pa[1], pa[2]}, {pa[3], pa[4], pa[5]}, {pa[6], pa[7], pa[8], pa[9]}};
+    const Box3 bb{{pb[0], pb[1], pb[2]}, {pb[3], pb[4], pb[5]}, {pb[6], pb[7], pb[8], pb[9]}};
+    EpaWork w;
+    Pt c{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, 0.0f};
+    const bool r = box_box_gjk_epa(ba, bb, cd, c, w);
+    ok[t] = r ? 1u : 0u;
+    float* o = out + 10ull * t;
+    o[0] = c.a.x; o[1] = c.a.y; o[2] = c.a.z;
+    o[3] = c.b.x; o[4] = c.b.y; o[5] = c.b.z;
+    o[6] = c.n.x; o[7] = c.n.y; o[8] = c.n.z;
+    o[9] = c.s;
+}
+
+}  // namespace fyzz_hip::pcm
+
+using namespace fyzz_hip;
+
+extern "C" {
+
+uint32_t fyzz_hip_pcm_cache_bytes(void) { return sizeof(pcm::Cache); }
+
+int fyzz_hip_pcm_box_box(fyzz_hip_context* ctx, uint32_t n_pairs, const uint32_t* first,
+                         const uint32_t* second, const uint32_t* cache_index, const float* pos,
+                         const float* quat, const float* half, float contact_distance, void* cache,
+                         float* contacts, uint32_t* counts, uint32_t* flags, uint32_t* scratch) {
+    return guarded(ctx, [&] {
+        if (!n_pairs) return;
+        FYZZ_REQUIRE(first && second && pos && quat && half && cache && contacts && counts && flags && scratch,
+                     kInvalidArgument, "pcm_box_box: null device pointer");
+        pcm::PairArgs A{n_pairs, first, second, cache_index, pos, quat, half, contact_distance,
+                        static_cast<pcm::Cache*>(cache), contacts, counts, flags, scratch + 1, scratch};
+        hipStream_t st = ctx->stream;
+        hipLaunchKernelGGL(pcm::k_zero1, dim3(1), dim3(1), 0, st, scratch);
+        ProfScope ps(&ctx->prof, "pcm.box_box", st);
+        hipLaunchKernelGGL(pcm::k_pcm_box_box, dim3(grid_for(n_pairs, 128)), dim3(128), 0, st, A);
+        hipLaunchKernelGGL(pcm::k_pcm_gjk, dim3(std::min<u32>(grid_for(n_pairs, 64), 1024u)), dim3(64), 0, st, A);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int fyzz_hip_pcm_plane_box(fyzz_hip_context* ctx, uint32_t n, const uint32_t* box_slot,
+                           const uint32_t* cache_index, const float* pos, const float* quat,
+                           const float* half, const float* plane_position, float contact_distance,
+                           void* cache, float* contacts, uint32_t* counts, uint32_t* flags) {
+    return guarded(ctx, [&] {
+        if (!n) return;
+        FYZZ_REQUIRE(box_slot && pos && quat && half && plane_position && cache && contacts && counts && flags,
+                     kInvalidArgument, "pcm_plane_box: null pointer");
+        pcm::PlaneArgs A{n, box_slot, cache_index, pos, quat, half,
+                         {plane_position[0], plane_position[1], plane_position[2]}, contact_distance,
+                         static_cast<pcm::Cache*>(cache), contacts, counts, flags};
+        ProfScope ps(&ctx->prof, "pcm.plane_box", ctx->stream);
+        hipLaunchKernelGGL(pcm::k_pcm_plane_box, dim3(grid_for(n, 128)), dim3(128), 0, ctx->stream, A);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int fyzz_hip_pcm_candidates(fyzz_hip_context* ctx, uint32_t n, const float* a, const float* b,
+                            float contact_distance, float* out, uint32_t* counts, uint32_t* overlapping,
+                            uint32_t* selection) {
+    return guarded(ctx, [&] {
+        if (!n) return;
+        FYZZ_REQUIRE(a && b && out && counts && overlapping && selection, kInvalidArgument,
+                     "pcm_candidates: null pointer");
+        hipLaunchKernelGGL(pcm::k_candidates, dim3(grid_for(n, 64)), dim3(64), 0, ctx->stream, n, a, b,
+                           contact_distance, out, counts, overlapping, selection);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int fyzz_hip_pcm_gjk_epa(fyzz_hip_context* ctx, uint32_t n, const float* a, const float* b,
+                         float contact_distance, float* out, uint32_t* ok) {
+    return guarded(ctx, [&] {
+        if (!n) return;
+        FYZZ_REQUIRE(a && b && out && ok, kInvalidArgument, "pcm_gjk_epa: null pointer");
+        hipLaunchKernelGGL(pcm::k_gjk_epa, dim3(grid_for(n, 64)), dim3(64), 0, ctx->stream, n, a, b,
+                           contact_distance, out, ok);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
+}
+
+}  // extern "C"

@@ -36,6 +36,11 @@ SIGNATURES = {
     "fyzz_hip_sap_counts": (C.c_int, [vp, u64p, u64p, u64p, u32p]),
     "fyzz_hip_sap_fetch": (C.c_int, [vp, C.c_int, vp, C.c_uint64, u64p]),
     "fyzz_hip_sap_endpoints": (C.c_int, [vp, vp, vp]),
+    "fyzz_hip_pcm_cache_bytes": (C.c_uint32, []),
+    "fyzz_hip_pcm_box_box": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp, vp, vp]),
+    "fyzz_hip_pcm_plane_box": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp, vp]),
+    "fyzz_hip_pcm_candidates": (C.c_int, [vp, C.c_uint32, vp, vp, C.c_float, vp, vp, vp, vp]),
+    "fyzz_hip_pcm_gjk_epa": (C.c_int, [vp, C.c_uint32, vp, vp, C.c_float, vp, vp]),
 }
 
 

@@ -107,6 +107,45 @@ int fyzz_hip_sap_fetch(fyzz_hip_sap* sap, int which, uint32_t* out_pairs, uint64
  * lays them out) into DEVICE arrays values[3][2n], payloads[3][2n]. */
 int fyzz_hip_sap_endpoints(fyzz_hip_sap* sap, uint32_t* values, uint32_t* payloads);
 
+/* ---- PCM narrowphase (persistent manifolds) ------------------------- */
+/* Bytes of one persistent manifold cache record (Scene::PcmManifoldCache,
+ * scene.h:1448-1459, without the write-only GJK warm start): 224. A
+ * zero-filled record is an empty, uninitialised cache. */
+uint32_t fyzz_hip_pcm_cache_bytes(void);
+
+/* Scene::generate_pcm_box_box (scene.cpp:3466-3654) for n_pairs pairs in
+ * Gu order (first, second are box slots; contact normals are expressed in
+ * second's frame). Poses: pos[3n], quat[4n], half[3n] per slot (device).
+ * cache_index (device, may be NULL = pair index) selects each pair's cache
+ * record in `cache`. Outputs (device): contacts[n_pairs][4][7] =
+ * (position, normal, separation) per published contact, counts[n_pairs],
+ * flags[n_pairs] (bit0 regenerated, bit1 GJK/EPA continuation, bit2 the
+ * reference would raise overflow_error). scratch: n_pairs + 1 u32. */
+int fyzz_hip_pcm_box_box(fyzz_hip_context* ctx, uint32_t n_pairs, const uint32_t* first,
+                         const uint32_t* second, const uint32_t* cache_index, const float* pos,
+                         const float* quat, const float* half, float contact_distance, void* cache,
+                         float* contacts, uint32_t* counts, uint32_t* flags, uint32_t* scratch);
+
+/* Scene::generate_pcm_plane_box (scene.cpp:3774-3955) for n dynamic boxes
+ * against the +y plane at HOST plane_position[3], with the scene's
+ * published (+y) normal (scene.cpp:4652-4657). Arguments as above. */
+int fyzz_hip_pcm_plane_box(fyzz_hip_context* ctx, uint32_t n, const uint32_t* box_slot,
+                           const uint32_t* cache_index, const float* pos, const float* quat,
+                           const float* half, const float* plane_position, float contact_distance,
+                           void* cache, float* contacts, uint32_t* counts, uint32_t* flags);
+
+/* Stateless parity entries over n pose pairs a[n][10], b[n][10] (half,
+ * position, quat): source_pcm_box_box_candidates + batch selection
+ * (GuPCMContactBoxBox.cpp:185-436) -> out[n][16][10] (local_a, local_b,
+ * normal_b, separation), counts[n], overlapping[n], selection[n][4] (255
+ * when <= 4 candidates); and source_pcm_box_box_gjk_epa
+ * (GuPCMGjkEpa.cpp:527-566) -> out[n][10], ok[n]. */
+int fyzz_hip_pcm_candidates(fyzz_hip_context* ctx, uint32_t n, const float* a, const float* b,
+                            float contact_distance, float* out, uint32_t* counts, uint32_t* overlapping,
+                            uint32_t* selection);
+int fyzz_hip_pcm_gjk_epa(fyzz_hip_context* ctx, uint32_t n, const float* a, const float* b,
+                         float contact_distance, float* out, uint32_t* ok);
+
 #ifdef __cplusplus
 }
 #endif
