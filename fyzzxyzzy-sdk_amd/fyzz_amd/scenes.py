@@ -223,3 +223,22 @@ def stack_scene(seed: int = SEED_BASE + 4, spacing: float = 1.05):
     half = np.full((64, 3), 0.5)
     quat = np.tile(np.array([0.0, 0.0, 0.0, 1.0]), (64, 1))
     return Scene(half, pos, quat, friction=0.5, restitution=0.0)
+
+
+def box_mass_properties(half: np.ndarray, density: float):
+    """Inverse mass and local inverse inertia of boxes, in the float32
+    operation order of Scene::add_dynamic_box (reference
+    src/native/scene.cpp:408-475): mass in double, then the unit-density
+    PxMassProperties tensor scaled by density in float32."""
+    h = np.asarray(half, np.float32).reshape(-1, 3)
+    hx, hy, hz = h[:, 0], h[:, 1], h[:, 2]
+    mass = 8.0 * hx.astype(np.float64) * hy * hz * float(density)
+    inv_mass = (1.0 / mass).astype(np.float32)
+    f = np.float32
+    unit_mass = (hx * hy * hz) * f(8.0)
+    ex, ey, ez = hx * hx, hy * hy, hz * hz
+    unit_scale = (unit_mass * f(1.0)) / f(3.0)
+    d = f(density)
+    inertia = np.stack([((ey + ez) * unit_scale) * d, ((ex + ez) * unit_scale) * d, ((ex + ey) * unit_scale) * d], 1)
+    inv_inertia = (f(1.0) / inertia).astype(np.float32)
+    return inv_mass, inv_inertia

@@ -34,6 +34,64 @@ extern "C" {
 typedef struct fyzz_hip_context fyzz_hip_context;
 typedef struct fyzz_hip_sap fyzz_hip_sap;
 
+/* ---- Dy records (plain data, shared by the device entries and hosts) --
+ * fyzz_dy_body: one dynamic rigid body (Scene::Actor's Dy fields,
+ * scene.h:1109-1165, with its single shape's material). flags: bit0
+ * sleeping, bit1 ready for sleep, bit2 gravity enabled. iterations:
+ * position | velocity << 8 (RigidBodyDynamics, world.h:115-140). */
+typedef struct fyzz_dy_body {
+    float position[3];
+    float orientation[4];
+    float linear_velocity[3];
+    float angular_velocity[3];
+    float inverse_mass;
+    float inverse_inertia[3];
+    float wake_counter;
+    float sleep_linear_acc[3];
+    float sleep_angular_acc[3];
+    uint32_t counted_interactions;
+    uint32_t flags;
+    float linear_damping, angular_damping, max_linear_velocity, max_angular_velocity;
+    float static_friction, dynamic_friction, restitution;
+    uint32_t iterations;
+    float pad[2];
+} fyzz_dy_body; /* 144 bytes */
+
+/* A static body (the floor): pose and material. */
+typedef struct fyzz_dy_static {
+    float position[3];
+    float orientation[4];
+    float static_friction, dynamic_friction, restitution;
+} fyzz_dy_static; /* 40 bytes */
+
+/* One published Gu contact (ContactPoint, scene.h:510-514). */
+typedef struct fyzz_dy_contact {
+    float position[3];
+    float normal[3];
+    float separation;
+} fyzz_dy_contact; /* 28 bytes */
+
+/* One touching contact manager in solver (descriptor) order: Gu-ordered
+ * bodies (body0 first; a negative value -1-k names static body k), its
+ * 1..4 published contacts, and the index of its previous-frame friction
+ * patch (0xffffffff: none). */
+typedef struct fyzz_dy_manager {
+    int32_t body0, body1;
+    uint32_t count;
+    uint32_t history;
+    fyzz_dy_contact contacts[4];
+} fyzz_dy_manager; /* 128 bytes */
+
+/* A persistent correlated friction patch (Scene::FrictionPatch,
+ * scene.h:1258-1280): body-local normals and anchors. flags: bit0 broken,
+ * bit1 disable strong friction. */
+typedef struct fyzz_dy_patch {
+    float body0_normal[3], body1_normal[3];
+    float body0_anchors[2][3], body1_anchors[2][3];
+    float static_friction, dynamic_friction, restitution, damping;
+    uint32_t anchor_count, flags;
+} fyzz_dy_patch; /* 96 bytes */
+
 /* ---- context -------------------------------------------------------- */
 /* Stands in for the producer's owned compute::Context + stream
  * (source_sc_owned_gpu_broadphase_native.h:66-162). */

@@ -59,6 +59,36 @@ struct SourceSapEndpointFixture {
         return p.size();
     }
 };
+// The reference also names this fixture a friend (scene.h:1083); the
+// driver reads the Dy descriptor stream, its conflict partitions and the
+// solver task boundaries of the last step through it.
+struct SolverIterationSourceFixture {
+    static std::size_t descriptors(const Scene& s, std::uint32_t* index, std::uint32_t* body0,
+                                   std::uint32_t* body1, std::uint64_t* partition,
+                                   std::size_t cap) {
+        const auto& d = s.constraint_descriptors_;
+        if (index)
+            for (std::size_t i = 0; i < d.size() && i < cap; ++i) {
+                index[i] = d[i].index;
+                body0[i] = d[i].body0;
+                body1[i] = d[i].body1;
+                partition[i] = i < s.constraint_partitions_.size()
+                                   ? static_cast<std::uint64_t>(s.constraint_partitions_[i])
+                                   : ~0ull;
+            }
+        return d.size();
+    }
+    static std::size_t tasks(const Scene& s, std::uint64_t* manager_end,
+                             std::uint64_t* partition_end, std::size_t cap) {
+        const auto& t = s.solver_tasks_;
+        if (manager_end)
+            for (std::size_t i = 0; i < t.size() && i < cap; ++i) {
+                manager_end[i] = t[i].manager_end;
+                partition_end[i] = t[i].partition_end;
+            }
+        return t.size();
+    }
+};
 }  // namespace fyzz::native
 
 namespace {
@@ -239,6 +269,24 @@ std::size_t ref_indexed_order(void* s, std::uint64_t* out, std::size_t cap) {
     if (out)
         for (std::size_t i = 0; i < o.size() && i < cap; ++i) out[i] = o[i];
     return o.size();
+}
+std::size_t ref_dy_descriptors(void* s, std::uint32_t* index, std::uint32_t* body0,
+                               std::uint32_t* body1, std::uint64_t* partition, std::size_t cap) {
+    return fyzz::native::SolverIterationSourceFixture::descriptors(
+        *static_cast<Scene*>(s), index, body0, body1, partition, cap);
+}
+std::size_t ref_dy_tasks(void* s, std::uint64_t* manager_end, std::uint64_t* partition_end,
+                         std::size_t cap) {
+    return fyzz::native::SolverIterationSourceFixture::tasks(*static_cast<Scene*>(s), manager_end,
+                                                             partition_end, cap);
+}
+// wake counter and ready-for-sleep of a body (scene.h:790-799)
+int ref_sleep_state(void* s, std::uint32_t id, float* wake_counter, int* ready) {
+    return guarded([&] {
+        const auto* sc = static_cast<Scene*>(s);
+        *wake_counter = sc->body_wake_counter(id);
+        *ready = sc->body_ready_for_sleep(id) ? 1 : 0;
+    });
 }
 std::size_t ref_partition_count(void* s) {
     return static_cast<Scene*>(s)->solver_partition_count();
