@@ -1,0 +1,1333 @@
+// Dy stage of the GPU step on gfx950: unconstrained pre-integration, solver
+// rows, exact first-fit conflict partitions, contact preparation with
+// correlated friction and anchor growth, partitioned projected Gauss-Seidel
+// (normal rows, then the patch's friction rows under the Coulomb clamp),
+// integration and the wake-counter sleep check.
+//
+// Reference (src/native/scene.cpp):
+//   prepare_unconstrained_velocities            10383-10507   k_pre
+//   prepare_solver_bodies                       6940-7112     k_rows
+//   partition_constraints                       7113-7489     k_partition
+//   prepare_constraints (contact managers)      7490-7515, 7763-8511   k_prep
+//   cached_friction_patch_valid                 3981-4010     history_valid
+//   solve_constraint_task                       8853-9148     k_pgs
+//   publish_solver_bodies, integrate_solver_task, integrate_quaternion
+//                                               9509-9549, 10508-10707, 111-146   k_integrate
+//
+// Admitted scene class: dynamic boxes and static planes/boxes, PCM
+// manifolds (one normal per manager, 1-4 contacts), non-compliant
+// materials, no joints, locks, slop or gyroscopic terms; one solver task.
+//
+// Exactness. Every FP32 expression keeps the reference's operation order
+// (the library builds with -ffp-contract=off). Partitions reproduce the
+// sequential first-fit over 32-bit masks exactly: a manager's partition
+// depends only on the masks its two bodies carry after the earlier managers
+// incident to them, so managers are coloured in dependency rounds (a
+// manager is ready once its predecessor on each body is coloured) inside
+// one cooperative launch. Within a partition no two managers share a
+// dynamic body and the static world's solver body keeps a zero state, so
+// solving a partition's managers concurrently gives the sequential sweep's
+// values bit for bit.
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <string>
+
+#include "../../include/fyzz_hip.h"
+#include "context.hpp"
+#include "vecmath.hpp"
+
+namespace cg = cooperative_groups;
+
+namespace fyzz_hip::dy {
+using namespace fyzz_hip::vm;
+
+constexpr u32 kNone = 0xffffffffu;
+constexpr u32 kPending = 0xfffffffeu;
+constexpr u32 kMaxParts = 4096;       // partitions per step
+constexpr u32 kMaxPending = 65536;    // managers left for overflow waves
+constexpr int kBlock = 256;
+
+// device counters
+enum : u32 { kFront0 = 0, kFront1, kFront2, kPend, kNParts, kPosIters, kVelIters, kRounds, kCtrWords = 16 };
+
+// one contact row (SolverContactPoint fields the solve reads), 64 B
+struct alignas(16) CRow {
+    float nx, ny, nz, vm;
+    float rax, ray, raz, biased;
+    float rbx, rby, rbz, unbiased;
+    float applied, max_impulse, pad0, pad1;
+};
+// one friction row (SolverContactFriction), 64 B
+struct alignas(16) FRow {
+    float tx, ty, tz, vm;
+    float rax, ray, raz, bias;
+    float rbx, rby, rbz, target;
+    float applied, pad0, pad1, pad2;
+};
+
+struct Ws {
+    // step inputs
+    float dt;
+    u32 nb, ns, nm, nh;
+    fyzz_dy_body* bodies;
+    const fyzz_dy_static* statics;
+    const fyzz_dy_manager* mg;
+    const fyzz_dy_patch* history;
+    fyzz_dy_patch* patches;
+    float* impulses;
+    u32* parts_out;
+    // per body
+    float4 *pv, *pw;          // pre-integrated velocities
+    float4 *lin, *ang;        // solver deltas: lin_delta|inv_mass, ang_state
+    float4 *mlin, *mang;      // motion snapshot after the last position iteration
+    float4 *rpos, *rq;        // mass pose of the solver row
+    float4* sii;              // 3 columns of the world sqrt inverse inertia
+    u32* has_row;
+    u32 *deg, *off, *fill, *max_dyn, *body_mask;
+    // per manager
+    u32 *inc, *pred, *succ, *cnt, *part, *mask_after, *front, *pend, *sorted, *order;
+    CRow* crow;
+    FRow* frow;
+    u32 *part_count, *part_off, *part_fill;
+    u32* blocksums;
+    u32* ctr;
+    u32* flags;
+};
+
+__device__ inline u32 ld(const u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void raise_flag(const Ws& W, u32 f) { atomicOr(W.flags, f); }
+__device__ inline void gsync(cg::grid_group& g) {
+    __threadfence();
+    g.sync();
+    __threadfence();
+}
+__device__ inline bool finite3(V v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+__device__ inline bool finite4(Q q) { return isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w); }
+__device__ inline V v3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ inline Q q4(const float* p) { return {p[0], p[1], p[2], p[3]}; }
+__device__ inline V f3(float4 f) { return {f.x, f.y, f.z}; }
+__device__ inline float len2(V v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+__device__ inline void put3(float* d, V s) {
+    d[0] = s.x;
+    d[1] = s.y;
+    d[2] = s.z;
+}
+// mass pose of a box: identity mass-local frame (scene.cpp:359-365)
+__device__ inline V mass_position(V p, Q q) { return vadd(p, srot(q, V{0.0f, 0.0f, 0.0f})); }
+__device__ inline Q mass_orientation(Q q) { return qmul(q, Q{0.0f, 0.0f, 0.0f, 1.0f}); }
+
+// ---------------------------------------------------------------- bodies
+
+__device__ inline bool manager_ok(const Ws& W, const fyzz_dy_manager& M) {
+    return !((M.body0 < 0 && M.body1 < 0) || M.body0 == M.body1 || M.body0 >= int(W.nb) || M.body1 >= int(W.nb) ||
+             (M.body0 < 0 && u32(-1 - M.body0) >= W.ns) || (M.body1 < 0 && u32(-1 - M.body1) >= W.ns) ||
+             M.count == 0 || M.count > 4);
+}
+
+// status flags and counters of the step (a fill kernel, so a captured
+// step records no memset node)
+__global__ void k_clear(Ws W) {
+    if (threadIdx.x < kCtrWords) W.ctr[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *W.flags = 0;
+}
+
+// scene.cpp:10383-10507 for default dynamics (no pending velocity mods)
+__global__ __launch_bounds__(kBlock) void k_pre(Ws W) {
+    const u32 i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= W.nb) return;
+    W.has_row[i] = 0;
+    const fyzz_dy_body& b = W.bodies[i];
+    if (b.flags & 1u) return;
+    const float dt = W.dt;
+    const float gravity = -9.81f;
+    float vx = b.linear_velocity[0] + 0.0f + 0.0f * dt;
+    float vy = b.linear_velocity[1] + 0.0f + 0.0f * dt;
+    float vz = b.linear_velocity[2] + 0.0f + 0.0f * dt;
+    vy = vy + ((b.flags & 4u) ? gravity * dt : 0.0f);
+    const float lm = smax(1.0f - b.linear_damping * dt, 0.0f);
+    vx *= lm;
+    vy *= lm;
+    vz *= lm;
+    V a{b.angular_velocity[0] + 0.0f + 0.0f * dt, b.angular_velocity[1] + 0.0f + 0.0f * dt,
+        b.angular_velocity[2] + 0.0f + 0.0f * dt};
+    const float am = smax(1.0f - b.angular_damping * dt, 0.0f);
+    a.x *= am;
+    a.y *= am;
+    a.z *= am;
+    const float s2 = vx * vx + vy * vy + vz * vz;
+    const float ml2 = b.max_linear_velocity * b.max_linear_velocity;
+    const float ma2 = b.max_angular_velocity * b.max_angular_velocity;
+    if (s2 > ml2) {
+        const float f = sqrtf(ml2 / s2);
+        vx *= f;
+        vy *= f;
+        vz *= f;
+    }
+    const float a2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    if (a2 > ma2) {
+        const float f = sqrtf(ma2 / a2);
+        a.x *= f;
+        a.y *= f;
+        a.z *= f;
+    }
+    if (!finite3(V{vx, vy, vz}) || !finite3(a)) raise_flag(W, FYZZ_DY_NONFINITE);
+    W.pv[i] = make_float4(vx, vy, vz, 0.0f);
+    W.pw[i] = make_float4(a.x, a.y, a.z, 0.0f);
+}
+
+// bodies of touching managers get a solver row (scene.cpp:6987-7013)
+__global__ __launch_bounds__(kBlock) void k_mark(Ws W) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= W.nm) return;
+    const fyzz_dy_manager M = W.mg[m];
+    if (!manager_ok(W, M)) {
+        raise_flag(W, FYZZ_DY_BAD_INPUT);
+        return;
+    }
+    if (M.body0 >= 0 && !(W.bodies[M.body0].flags & 1u)) W.has_row[M.body0] = 1;
+    if (M.body1 >= 0 && !(W.bodies[M.body1].flags & 1u)) W.has_row[M.body1] = 1;
+}
+
+// world sqrt inverse inertia R diag(sqrt I^-1) R^T (scene.cpp:7092-7110)
+__global__ __launch_bounds__(kBlock) void k_rows(Ws W) {
+    const u32 i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= W.nb) return;
+    float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0, c2 = c0;
+    W.ang[i] = c0;
+    W.mlin[i] = c0;
+    W.mang[i] = c0;
+    if (!W.has_row[i]) {
+        W.lin[i] = c0;
+        W.rpos[i] = c0;
+        W.rq[i] = make_float4(0.f, 0.f, 0.f, 1.f);
+        W.sii[3 * i] = c0;
+        W.sii[3 * i + 1] = c0;
+        W.sii[3 * i + 2] = c0;
+        return;
+    }
+    const fyzz_dy_body& b = W.bodies[i];
+    const Q q = mass_orientation(q4(b.orientation));
+    const V p = mass_position(v3(b.position), q4(b.orientation));
+    W.rpos[i] = make_float4(p.x, p.y, p.z, 0.f);
+    W.rq[i] = make_float4(q.x, q.y, q.z, q.w);
+    W.lin[i] = make_float4(0.f, 0.f, 0.f, b.inverse_mass);
+    const V l{sqrtf(b.inverse_inertia[0]), sqrtf(b.inverse_inertia[1]), sqrtf(b.inverse_inertia[2])};
+    if (q.x == 0.0f && q.y == 0.0f && q.z == 0.0f && q.w == 1.0f) {
+        c0.x = l.x;
+        c1.y = l.y;
+        c2.z = l.z;
+    } else {
+        const V rx = srot(q, V{1.f, 0.f, 0.f});
+        const V ry = srot(q, V{0.f, 1.f, 0.f});
+        const V rz = srot(q, V{0.f, 0.f, 1.f});
+        const float xx = l.x * rx.x * rx.x + l.y * ry.x * ry.x + l.z * rz.x * rz.x;
+        const float yy = l.x * rx.y * rx.y + l.y * ry.y * ry.y + l.z * rz.y * rz.y;
+        const float zz = l.x * rx.z * rx.z + l.y * ry.z * ry.z + l.z * rz.z * rz.z;
+        const float xy = l.x * rx.x * rx.y + l.y * ry.x * ry.y + l.z * rz.x * rz.y;
+        const float xz = l.x * rx.x * rx.z + l.y * ry.x * ry.z + l.z * rz.x * rz.z;
+        const float yz = l.x * rx.y * rx.z + l.y * ry.y * ry.z + l.z * rz.y * rz.z;
+        c0 = make_float4(xx, xy, xz, 0.f);
+        c1 = make_float4(xy, yy, yz, 0.f);
+        c2 = make_float4(xz, yz, zz, 0.f);
+    }
+    W.sii[3 * i] = c0;
+    W.sii[3 * i + 1] = c1;
+    W.sii[3 * i + 2] = c2;
+    atomicMax(&W.ctr[kPosIters], b.iterations & 0xffu);
+    atomicMax(&W.ctr[kVelIters], (b.iterations >> 8) & 0xffu);
+}
+
+// ------------------------------------------------------------ partitions
+
+__device__ inline bool dyn_dyn(const fyzz_dy_manager& M) { return M.body0 >= 0 && M.body1 >= 0; }
+
+// Exclusive scan of deg[0, nb) into off[0, nb] across the grid.
+__device__ void grid_scan_degrees(const Ws& W, cg::grid_group& g) {
+    __shared__ u32 scratch[kBlock / kWave + 1];
+    const u32 G = gridDim.x;
+    const u32 chunk = (W.nb + G - 1) / G;
+    const u32 lo = min(W.nb, blockIdx.x * chunk), hi = min(W.nb, lo + chunk);
+    u32 s = 0;
+    for (u32 i = lo + threadIdx.x; i < hi; i += kBlock) s += W.deg[i];
+    u32 total;
+    (void)block_exclusive_scan<kBlock>(s, scratch, &total);
+    if (threadIdx.x == 0) W.blocksums[blockIdx.x] = total;
+    gsync(g);
+    if (blockIdx.x == 0) {
+        u32 carry = 0;
+        for (u32 t0 = 0; t0 < G; t0 += kBlock) {
+            const u32 t = t0 + threadIdx.x;
+            const u32 v = t < G ? W.blocksums[t] : 0u;
+            const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
+            if (t < G) W.blocksums[G + t] = carry + ex;
+            carry += total;
+        }
+        if (threadIdx.x == 0) W.off[W.nb] = carry;
+    }
+    gsync(g);
+    u32 carry = W.blocksums[G + blockIdx.x];
+    for (u32 t0 = lo; t0 < hi; t0 += kBlock) {
+        const u32 i = t0 + threadIdx.x;
+        const u32 v = i < hi ? W.deg[i] : 0u;
+        const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
+        if (i < hi) W.off[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// First-fit partitions over 32-bit masks (scene.cpp:7352-7445) for one task
+// spanning the manager list; stable scatter by partition (7447-7489).
+__global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
+    cg::grid_group g = cg::this_grid();
+    const u32 tid = blockIdx.x * kBlock + threadIdx.x;
+    const u32 nthreads = gridDim.x * kBlock;
+    __shared__ u32 scratch[kBlock / kWave + 1];
+    // P0: clear per-body and per-partition state
+    for (u32 i = tid; i < W.nb; i += nthreads) {
+        W.deg[i] = 0;
+        W.fill[i] = 0;
+        W.max_dyn[i] = 0;
+    }
+    for (u32 i = tid; i < kMaxParts; i += nthreads) {
+        W.part_count[i] = 0;
+        W.part_fill[i] = 0;
+    }
+    if (tid <= kNParts) W.ctr[tid] = 0;  // frontier, pending and partition counters
+    gsync(g);
+    // P1: incidence degrees (dynamic sides)
+    for (u32 m = tid; m < W.nm; m += nthreads) {
+        const fyzz_dy_manager& M = W.mg[m];
+        W.part[m] = kNone;
+        if (!manager_ok(W, M)) continue;
+        if (M.body0 >= 0) atomicAdd(&W.deg[M.body0], 1u);
+        if (M.body1 >= 0) atomicAdd(&W.deg[M.body1], 1u);
+    }
+    gsync(g);
+    grid_scan_degrees(W, g);
+    gsync(g);
+    // P3: incidence lists, entry = manager << 1 | side
+    for (u32 m = tid; m < W.nm; m += nthreads) {
+        const fyzz_dy_manager& M = W.mg[m];
+        if (!manager_ok(W, M)) continue;
+        if (M.body0 >= 0) W.inc[W.off[M.body0] + atomicAdd(&W.fill[M.body0], 1u)] = m << 1;
+        if (M.body1 >= 0) W.inc[W.off[M.body1] + atomicAdd(&W.fill[M.body1], 1u)] = (m << 1) | 1u;
+    }
+    gsync(g);
+    // P4: order each body's list by manager; chain dynamic/dynamic entries
+    for (u32 b = tid; b < W.nb; b += nthreads) {
+        const u32 lo = W.off[b], hi = W.off[b + 1];
+        for (u32 i = lo + 1; i < hi; ++i) {
+            const u32 e = W.inc[i];
+            u32 j = i;
+            while (j > lo && W.inc[j - 1] > e) {
+                W.inc[j] = W.inc[j - 1];
+                --j;
+            }
+            W.inc[j] = e;
+        }
+        u32 prev = kNone;
+        for (u32 i = lo; i < hi; ++i) {
+            const u32 e = W.inc[i];
+            if (!dyn_dyn(W.mg[e >> 1])) continue;
+            W.pred[e] = prev;
+            W.succ[e] = kNone;
+            if (prev != kNone) W.succ[prev] = e;
+            prev = e;
+        }
+    }
+    gsync(g);
+    // P5: ready managers (no predecessor on either body)
+    for (u32 m = tid; m < W.nm; m += nthreads) {
+        if (!dyn_dyn(W.mg[m]) || !manager_ok(W, W.mg[m])) continue;
+        const u32 c = (W.pred[2 * m] != kNone ? 1u : 0u) + (W.pred[2 * m + 1] != kNone ? 1u : 0u);
+        W.cnt[m] = c;
+        if (c == 0) W.front[atomicAdd(&W.ctr[kFront0], 1u)] = m;
+    }
+    gsync(g);
+    // dependency rounds
+    u32 r = 0;
+    for (;; ++r) {
+        const u32 ncur = ld(&W.ctr[r % 3]);
+        if (ncur == 0) break;
+        if (r > W.nm) {
+            if (tid == 0) raise_flag(W, FYZZ_DY_LOGIC);
+            break;
+        }
+        const u32* cur = W.front + (r & 1u) * W.nm;
+        u32* nxt = W.front + ((r + 1) & 1u) * W.nm;
+        for (u32 i = tid; i < ncur; i += nthreads) {
+            const u32 m = cur[i];
+            const u32 p0 = W.pred[2 * m], p1 = W.pred[2 * m + 1];
+            const u32 before0 = p0 != kNone ? W.mask_after[p0] : 0u;
+            const u32 before1 = p1 != kNone ? W.mask_after[p1] : 0u;
+            const u32 avail = ~(before0 | before1);
+            if (avail) {
+                const u32 bit = __builtin_ctz(avail);
+                W.part[m] = bit;
+                W.mask_after[2 * m] = before0 | (1u << bit);
+                W.mask_after[2 * m + 1] = before1 | (1u << bit);
+            } else {
+                W.part[m] = kPending;
+                W.mask_after[2 * m] = before0;
+                W.mask_after[2 * m + 1] = before1;
+                const u32 k = atomicAdd(&W.ctr[kPend], 1u);
+                if (k < kMaxPending) W.pend[k] = m;
+            }
+            for (u32 s = 0; s < 2; ++s) {
+                const u32 e = W.succ[2 * m + s];
+                if (e == kNone) continue;
+                if (atomicSub(&W.cnt[e >> 1], 1u) == 1u) nxt[atomicAdd(&W.ctr[(r + 1) % 3], 1u)] = e >> 1;
+            }
+        }
+        if (tid == 0) W.ctr[(r + 2) % 3] = 0;
+        gsync(g);
+    }
+    if (tid == 0) W.ctr[kRounds] = r;
+    // overflow waves: the unassigned managers, in order, with fresh masks
+    if (tid == 0) {
+        u32 p = ld(&W.ctr[kPend]);
+        if (p > kMaxPending) {
+            raise_flag(W, FYZZ_DY_OVERFLOW);
+            p = 0;
+        }
+        for (u32 i = 1; i < p; ++i) {
+            const u32 e = W.pend[i];
+            u32 j = i;
+            while (j > 0 && W.pend[j - 1] > e) {
+                W.pend[j] = W.pend[j - 1];
+                --j;
+            }
+            W.pend[j] = e;
+        }
+        for (u32 wave = 32; p; wave += 32) {
+            for (u32 k = 0; k < p; ++k) {
+                const fyzz_dy_manager& M = W.mg[W.pend[k]];
+                W.body_mask[M.body0] = 0;
+                W.body_mask[M.body1] = 0;
+            }
+            u32 over = 0;
+            for (u32 k = 0; k < p; ++k) {
+                const u32 m = W.pend[k];
+                const fyzz_dy_manager& M = W.mg[m];
+                const u32 avail = ~(W.body_mask[M.body0] | W.body_mask[M.body1]);
+                if (!avail) {
+                    W.pend[over++] = m;
+                    continue;
+                }
+                const u32 bit = __builtin_ctz(avail);
+                W.body_mask[M.body0] |= 1u << bit;
+                W.body_mask[M.body1] |= 1u << bit;
+                W.part[m] = wave + bit;
+            }
+            if (over == p) {
+                raise_flag(W, FYZZ_DY_LOGIC);
+                break;
+            }
+            p = over;
+        }
+    }
+    gsync(g);
+    // P6: highest dynamic partition per body
+    for (u32 m = tid; m < W.nm; m += nthreads) {
+        const fyzz_dy_manager& M = W.mg[m];
+        if (!dyn_dyn(M)) continue;
+        const u32 pm = W.part[m];
+        if (pm >= kPending) continue;
+        atomicMax(&W.max_dyn[M.body0], pm + 1);
+        atomicMax(&W.max_dyn[M.body1], pm + 1);
+    }
+    gsync(g);
+    // P7: dynamic/static managers follow the body's dynamic partitions
+    for (u32 b = tid; b < W.nb; b += nthreads) {
+        u32 k = W.max_dyn[b];
+        for (u32 i = W.off[b]; i < W.off[b + 1]; ++i) {
+            const u32 m = W.inc[i] >> 1;
+            if (!dyn_dyn(W.mg[m])) W.part[m] = k++;
+        }
+    }
+    gsync(g);
+    // P8: partition sizes
+    for (u32 m = tid; m < W.nm; m += nthreads) {
+        const u32 pm = W.part[m];
+        if (pm == kNone) continue;  // malformed record, already flagged
+        if (pm >= kMaxParts) {
+            raise_flag(W, FYZZ_DY_OVERFLOW);
+            continue;
+        }
+        atomicAdd(&W.part_count[pm], 1u);
+        atomicMax(&W.ctr[kNParts], pm + 1);
+    }
+    gsync(g);
+    if (blockIdx.x == 0) {
+        const u32 np = ld(&W.ctr[kNParts]);
+        u32 carry = 0, total;
+        for (u32 t0 = 0; t0 < np; t0 += kBlock) {
+            const u32 t = t0 + threadIdx.x;
+            const u32 v = t < np ? W.part_count[t] : 0u;
+            const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
+            if (t < np) W.part_off[t] = carry + ex;
+            carry += total;
+        }
+        if (threadIdx.x == 0) W.part_off[np] = carry;
+    }
+    gsync(g);
+    // P10: managers grouped by partition (order inside a partition is free)
+    for (u32 m = tid; m < W.nm; m += nthreads) {
+        const u32 pm = W.part[m];
+        if (pm >= kMaxParts) continue;
+        W.sorted[W.part_off[pm] + atomicAdd(&W.part_fill[pm], 1u)] = m;
+    }
+}
+
+// ----------------------------------------------------- contact preparation
+
+struct Row {
+    V pos;
+    Q q;
+    V v, w;
+    float inv_mass;
+    V c0, c1, c2;
+};
+__device__ inline Row load_row(const Ws& W, int id) {
+    Row r{};
+    r.q = Q{0.f, 0.f, 0.f, 1.f};
+    if (id < 0 || !W.has_row[id]) return r;
+    r.pos = f3(W.rpos[id]);
+    const float4 q = W.rq[id];
+    r.q = Q{q.x, q.y, q.z, q.w};
+    r.v = f3(W.pv[id]);
+    r.w = f3(W.pw[id]);
+    r.inv_mass = W.lin[id].w;
+    r.c0 = f3(W.sii[3 * id]);
+    r.c1 = f3(W.sii[3 * id + 1]);
+    r.c2 = f3(W.sii[3 * id + 2]);
+    return r;
+}
+__device__ inline V transform(const Row& d, V v) {
+    return {d.c0.x * v.x + d.c1.x * v.y + d.c2.x * v.z, d.c0.y * v.x + d.c1.y * v.y + d.c2.y * v.z,
+            d.c0.z * v.x + d.c1.z * v.y + d.c2.z * v.z};
+}
+struct BodyView {
+    V pos;
+    Q q;
+};
+__device__ inline BodyView view_of(const Ws& W, int id) {
+    if (id >= 0) return {v3(W.bodies[id].position), q4(W.bodies[id].orientation)};
+    const fyzz_dy_static& s = W.statics[-1 - id];
+    return {v3(s.position), q4(s.orientation)};
+}
+__device__ inline V local_point(const BodyView& a, V world) {
+    const Q q = mass_orientation(a.q);
+    const V p = mass_position(a.pos, a.q);
+    return srot(Q{-q.x, -q.y, -q.z, q.w}, vsub(world, p));
+}
+__device__ inline V local_normal(const BodyView& a, V n) {
+    const Q q = mass_orientation(a.q);
+    return srot(Q{-q.x, -q.y, -q.z, q.w}, n);
+}
+__device__ inline V world_point(const BodyView& a, V local) {
+    const V r = srot(mass_orientation(a.q), local);
+    const V p = mass_position(a.pos, a.q);
+    return vadd(r, p);
+}
+
+struct Patch {
+    V normal, n0, n1;
+    float sf, df, rest, damping;
+    V a0[2], a1[2];
+    V bmin, bmax;
+    u32 contacts, anchors;
+    bool broken, disable_strong;
+};
+
+// scene.cpp:3981-4010
+__device__ bool history_valid(const Patch& old, const BodyView& a0, const BodyView& a1) {
+    if (old.broken || old.disable_strong || old.anchors == 0u) return false;
+    const float same_normal = 0.999f;
+    const float corr = 0.025f;
+    const Q q0 = mass_orientation(a0.q), q1 = mass_orientation(a1.q);
+    const V n10 = srot_inv(q0, srot(q1, old.n1));
+    if (vdot(old.n0, n10) <= same_normal) return false;
+    const V p0 = mass_position(a0.pos, a0.q), p1 = mass_position(a1.pos, a1.q);
+    for (u32 k = 0; k < old.anchors; ++k) {
+        const V l = srot(q1, old.a1[k]);
+        const V world = vadd(l, p1);
+        const V in0 = srot_inv(q0, vsub(world, p0));
+        const V d = vsub(old.a0[k], in0);
+        if (fabsf(vdot(d, old.n0)) >= corr) return false;
+    }
+    return true;
+}
+
+struct Run {
+    u32 start, count;
+    V normal, bmin, bmax;
+    int next;
+};
+
+// One thread per manager: materials, same-normal runs, friction-patch
+// correlation with the previous step, contact rows, anchor growth and
+// friction rows (scene.cpp:7794-8505).
+__global__ __launch_bounds__(128) void k_prep(Ws W) {
+    const u32 m = blockIdx.x * 128 + threadIdx.x;
+    if (m >= W.nm) return;
+    const fyzz_dy_manager& M = W.mg[m];
+    if (W.part[m] >= kMaxParts) return;
+    const float inv_dt = 1.0f / W.dt;
+    const BodyView a0 = view_of(W, M.body0), a1 = view_of(W, M.body1);
+    const Row a = load_row(W, M.body0);
+    const Row b = load_row(W, M.body1);
+    float sf0, df0, r0, sf1, df1, r1;
+    if (M.body0 >= 0) {
+        sf0 = W.bodies[M.body0].static_friction;
+        df0 = W.bodies[M.body0].dynamic_friction;
+        r0 = W.bodies[M.body0].restitution;
+    } else {
+        sf0 = W.statics[-1 - M.body0].static_friction;
+        df0 = W.statics[-1 - M.body0].dynamic_friction;
+        r0 = W.statics[-1 - M.body0].restitution;
+    }
+    if (M.body1 >= 0) {
+        sf1 = W.bodies[M.body1].static_friction;
+        df1 = W.bodies[M.body1].dynamic_friction;
+        r1 = W.bodies[M.body1].restitution;
+    } else {
+        sf1 = W.statics[-1 - M.body1].static_friction;
+        df1 = W.statics[-1 - M.body1].dynamic_friction;
+        r1 = W.statics[-1 - M.body1].restitution;
+    }
+    if (r0 < 0.0f || r1 < 0.0f) {  // compliant contacts are not admitted
+        raise_flag(W, FYZZ_DY_LOGIC);
+        return;
+    }
+    const float rigid_contact_bias = 0.8f;
+    const float bounce_threshold = -2.0f;
+    const float same_normal = 0.999f;
+    const float friction_offset_threshold = 0.04f;
+    const float restitution = 0.5f * (r0 + r1);
+    const float damping = 0.5f * (0.0f + 0.0f);
+    const float dynamic_friction = smax(0.0f, 0.5f * (df0 + df1));
+    const float static_friction = smax(dynamic_friction, 0.5f * (sf0 + sf1));
+    const float max_pen_bias = -1.0e32f;
+    const float rest_distance = 0.0f;
+    // contiguous same-normal runs (scene.cpp:7964-7997)
+    Run runs[4];
+    u32 nr = 0;
+    for (u32 j = 0; j < M.count; ++j) {
+        const V pn = v3(M.contacts[j].normal);
+        const V pp = v3(M.contacts[j].position);
+        if (nr == 0 || vdot(runs[nr - 1].normal, pn) < same_normal) {
+            runs[nr++] = Run{j, 1, pn, pp, pp, -1};
+        } else {
+            Run& r = runs[nr - 1];
+            ++r.count;
+            r.bmin = {smin(r.bmin.x, pp.x), smin(r.bmin.y, pp.y), smin(r.bmin.z, pp.z)};
+            r.bmax = {smax(r.bmax.x, pp.x), smax(r.bmax.y, pp.y), smax(r.bmax.z, pp.z)};
+        }
+    }
+    // correlation against the previous step's patch (scene.cpp:8007-8099)
+    Patch local[5];
+    u32 nl = 0;
+    if (M.history != kNone && M.history < W.nh && W.history[M.history].anchor_count > 2u)
+        raise_flag(W, FYZZ_DY_BAD_INPUT);
+    else if (M.history != kNone && M.history < W.nh) {
+        const fyzz_dy_patch& h = W.history[M.history];
+        Patch old{};
+        old.n0 = v3(h.body0_normal);
+        old.n1 = v3(h.body1_normal);
+        for (int k = 0; k < 2; ++k) {
+            old.a0[k] = v3(h.body0_anchors[k]);
+            old.a1[k] = v3(h.body1_anchors[k]);
+        }
+        old.sf = h.static_friction;
+        old.df = h.dynamic_friction;
+        old.rest = h.restitution;
+        old.damping = h.damping;
+        old.anchors = h.anchor_count;
+        old.broken = h.flags & 1u;
+        old.disable_strong = (h.flags >> 1) & 1u;
+        if (history_valid(old, a0, a1)) {
+            Patch c = old;
+            c.normal = srot(mass_orientation(a0.q), old.n0);
+            c.contacts = 0;
+            c.bmin = {kFltMax, kFltMax, kFltMax};
+            c.bmax = {-kFltMax, -kFltMax, -kFltMax};
+            local[nl++] = c;
+        }
+    }
+    int heads[5] = {-1, -1, -1, -1, -1};
+    for (u32 ri = 0; ri < nr; ++ri) {
+        Run& cp = runs[ri];
+        u32 lp = 0;
+        for (; lp < nl; ++lp) {
+            const Patch& p = local[lp];
+            if (vdot(p.normal, cp.normal) >= same_normal && p.rest == restitution && p.damping == damping &&
+                p.sf == static_friction && p.df == dynamic_friction)
+                break;
+        }
+        if (lp == nl) {
+            Patch p{};
+            p.normal = cp.normal;
+            p.sf = static_friction;
+            p.df = dynamic_friction;
+            p.rest = restitution;
+            p.damping = damping;
+            p.n0 = local_normal(a0, cp.normal);
+            p.n1 = local_normal(a1, cp.normal);
+            local[nl++] = p;
+        }
+        Patch& p = local[lp];
+        if (p.contacts == 0) {
+            p.bmin = cp.bmin;
+            p.bmax = cp.bmax;
+        } else {
+            p.bmin = {smin(p.bmin.x, cp.bmin.x), smin(p.bmin.y, cp.bmin.y), smin(p.bmin.z, cp.bmin.z)};
+            p.bmax = {smax(p.bmax.x, cp.bmax.x), smax(p.bmax.y, cp.bmax.y), smax(p.bmax.z, cp.bmax.z)};
+        }
+        p.contacts += cp.count;
+        cp.next = heads[lp];
+        heads[lp] = int(ri);
+    }
+    // exactly one patch keeps contacts: PCM managers publish one normal
+    int kept = -1, nkept = 0;
+    for (u32 j = 0; j < nl; ++j)
+        if (local[j].contacts) {
+            if (kept < 0) kept = int(j);
+            ++nkept;
+        }
+    if (nkept != 1) {
+        raise_flag(W, FYZZ_DY_LOGIC);
+        return;
+    }
+    Patch P = local[kept];
+    const V header = v3(M.contacts[runs[heads[kept]].start].normal);
+    u32 order[4];
+    u32 no = 0;
+    for (int cp = heads[kept]; cp >= 0; cp = runs[cp].next)
+        for (u32 c = runs[cp].start; c < runs[cp].start + runs[cp].count; ++c) order[no++] = c;
+    // contact rows in the correlation-list order (scene.cpp:8204-8225, 8237-8330)
+    bool bad = false;
+    for (u32 q = 0; q < no; ++q) {
+        const fyzz_dy_contact& C = M.contacts[order[q]];
+        const V n = header;
+        const V cpos = v3(C.position);
+        const V ra_x_n = vcross(vsub(cpos, a.pos), n);
+        const V rb_x_n = vcross(vsub(cpos, b.pos), n);
+        const float lin_vrel = vdot(n, vsub(a.v, b.v));
+        const V ra = transform(a, ra_x_n);
+        const V rb = transform(b, rb_x_n);
+        const float ang_vrel = vdot(ra_x_n, a.w) - vdot(rb_x_n, b.w);
+        const float vrel = lin_vrel + ang_vrel;
+        const float response = (a.inv_mass + vdot(ra, ra)) + (b.inv_mass + vdot(rb, rb));
+        const float vm = response > 0.0f ? 1.0f / response : 0.0f;
+        const float pen = C.separation - rest_distance;
+        const float pen_inv_dt = pen * inv_dt;
+        const bool separated = pen >= 0.0f;
+        const bool closing = -vrel > pen_inv_dt;
+        const bool bouncing = restitution > 0.0f && bounce_threshold > vrel && closing;
+        const float target_velocity = bouncing ? -vrel * restitution : 0.0f;
+        const float target = target_velocity - vrel;
+        const float pen_scaled = separated ? pen_inv_dt : pen * (inv_dt * rigid_contact_bias);
+        const float scaled_bias = bouncing ? 0.0f : vm * smax(max_pen_bias, pen_scaled);
+        CRow r;
+        r.nx = n.x;
+        r.ny = n.y;
+        r.nz = n.z;
+        r.vm = vm;
+        r.rax = ra.x;
+        r.ray = ra.y;
+        r.raz = ra.z;
+        r.biased = target * vm - scaled_bias;
+        r.rbx = rb.x;
+        r.rby = rb.y;
+        r.rbz = rb.z;
+        r.unbiased = target * vm - (bouncing ? 0.0f : smax(scaled_bias, 0.0f));
+        r.applied = 0.0f;
+        r.max_impulse = kFltMax;
+        r.pad0 = r.pad1 = 0.0f;
+        bad |= !isfinite(response) || !isfinite(r.biased) || !isfinite(r.unbiased);
+        W.crow[4 * m + q] = r;
+    }
+    if (bad) raise_flag(W, FYZZ_DY_NONFINITE);
+    // anchor growth (scene.cpp:8359-8433)
+    {
+        const V extent = vsub(P.bmax, P.bmin);
+        const float diag2 = len2(extent);
+        bool keep_pair = false;
+        if (P.anchors == 2) {
+            const V d = vsub(P.a0[0], P.a0[1]);
+            keep_pair = len2(d) * 4.0f >= diag2;
+            if (!keep_pair) P.anchors = 0;
+        }
+        V anchors[2] = {V{0.f, 0.f, 0.f}, V{0.f, 0.f, 0.f}};
+        bool changed[2] = {false, false};
+        u32 count = P.anchors;
+        if (!keep_pair) {
+            if (count == 1) anchors[0] = world_point(a0, P.a0[0]);
+            float pd2 = 0.0f;
+            for (u32 q = 0; q < no; ++q) {
+                const u32 j = order[q];
+                if (M.contacts[j].separation >= friction_offset_threshold + rest_distance) continue;
+                const V wp = v3(M.contacts[j].position);
+                if (count == 0) {
+                    anchors[0] = wp;
+                    changed[0] = true;
+                    ++count;
+                } else if (count == 1) {
+                    const float d = len2(vsub(wp, anchors[0]));
+                    if (d > 1.0e-8f) {
+                        anchors[1] = wp;
+                        changed[1] = true;
+                        pd2 = d;
+                        ++count;
+                    }
+                } else {
+                    const float d0 = len2(vsub(wp, anchors[0]));
+                    const float d1 = len2(vsub(wp, anchors[1]));
+                    if (d0 > d1 && d0 > pd2) {
+                        anchors[1] = wp;
+                        changed[1] = true;
+                        pd2 = d0;
+                    } else if (d1 >= d0 && d1 > pd2) {
+                        anchors[0] = wp;
+                        changed[0] = true;
+                        pd2 = d1;
+                    }
+                }
+            }
+            for (u32 k = 0; k < count; ++k) {
+                if (!changed[k]) continue;
+                P.a0[k] = local_point(a0, anchors[k]);
+                P.a1[k] = local_point(a1, anchors[k]);
+            }
+            if (count == 0) {
+                P.a0[0] = V{0.f, 0.f, 0.f};
+                P.a1[0] = V{0.f, 0.f, 0.f};
+            }
+        }
+        P.anchors = count;
+        if (count) {
+            const V n = header;
+            const V rel = vsub(a.v, b.v);
+            V t0 = vsub(rel, V{n.x * vdot(rel, n), n.y * vdot(rel, n), n.z * vdot(rel, n)});
+            if (len2(t0) <= 0.0001f) t0 = fabsf(n.x) < 0.70710678f ? V{0.0f, -n.z, n.y} : V{-n.y, n.x, 0.0f};
+            const float il = 1.0f / sqrtf(len2(t0));
+            t0 = {t0.x * il, t0.y * il, t0.z * il};
+            const V t1 = vcross(n, t0);
+            for (u32 k = 0; k < count; ++k) {
+                const V ra = srot(mass_orientation(a0.q), P.a0[k]);
+                const V rb = srot(mass_orientation(a1.q), P.a1[k]);
+                const V c0 = mass_position(a0.pos, a0.q), c1 = mass_position(a1.pos, a1.q);
+                const V err = vsub(vadd(ra, c0), vadd(rb, c1));
+                for (u32 ti = 0; ti < 2; ++ti) {
+                    const V tangent = ti ? t1 : t0;
+                    const V ax0 = vcross(ra, tangent);
+                    const V ax1 = vcross(rb, tangent);
+                    const V j0 = transform(a, ax0);
+                    const V j1 = transform(b, ax1);
+                    const float response = (a.inv_mass + vdot(j0, j0)) + (b.inv_mass + vdot(j1, j1));
+                    FRow fr;
+                    fr.tx = tangent.x;
+                    fr.ty = tangent.y;
+                    fr.tz = tangent.z;
+                    fr.vm = response > 0.0f ? rigid_contact_bias / response : 0.0f;
+                    fr.rax = j0.x;
+                    fr.ray = j0.y;
+                    fr.raz = j0.z;
+                    fr.bias = vdot(tangent, err) * inv_dt;
+                    fr.rbx = j1.x;
+                    fr.rby = j1.y;
+                    fr.rbz = j1.z;
+                    fr.target = -(vdot(tangent, rel) + vdot(ax0, a.w) - vdot(ax1, b.w));
+                    fr.applied = 0.0f;
+                    fr.pad0 = fr.pad1 = fr.pad2 = 0.0f;
+                    W.frow[4 * m + 2 * k + ti] = fr;
+                }
+            }
+        }
+    }
+    fyzz_dy_patch o;
+    put3(o.body0_normal, P.n0);
+    put3(o.body1_normal, P.n1);
+    for (int k = 0; k < 2; ++k) {
+        put3(o.body0_anchors[k], P.a0[k]);
+        put3(o.body1_anchors[k], P.a1[k]);
+    }
+    o.static_friction = P.sf;
+    o.dynamic_friction = P.df;
+    o.restitution = P.rest;
+    o.damping = P.damping;
+    o.anchor_count = P.anchors;
+    o.flags = (P.broken ? 1u : 0u) | (P.disable_strong ? 2u : 0u);
+    W.patches[m] = o;
+    W.order[m] = no | (order[0] << 8) | ((no > 1 ? order[1] : 0u) << 12) | ((no > 2 ? order[2] : 0u) << 16) |
+                 ((no > 3 ? order[3] : 0u) << 20);
+}
+
+// ------------------------------------------------------------------ solve
+
+// scene.cpp:8985-9010 (normal rows), 9050-9120 (friction rows)
+__device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 total_iters) {
+    const fyzz_dy_manager& M = W.mg[m];
+    const int i0 = M.body0, i1 = M.body1;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 la = i0 >= 0 ? W.lin[i0] : z4, lb = i1 >= 0 ? W.lin[i1] : z4;
+    const float4 aa4 = i0 >= 0 ? W.ang[i0] : z4, ab4 = i1 >= 0 ? W.ang[i1] : z4;
+    V va = f3(la), vb = f3(lb), wa = f3(aa4), wb = f3(ab4);
+    const float ima = la.w, imb = lb.w;
+    const u32 no = W.order[m] & 0xffu;
+    bool bad = false;
+    float normal_impulse = 0.0f;
+    for (u32 q = 0; q < no; ++q) {
+        CRow& r = W.crow[4 * m + q];
+        const V n{r.nx, r.ny, r.nz};
+        const V ra{r.rax, r.ray, r.raz}, rb{r.rbx, r.rby, r.rbz};
+        const float nv = vdot(n, va) - vdot(n, vb) + vdot(wa, ra) - vdot(wb, rb);
+        const float err = iter < pos_iters ? r.biased : r.unbiased;
+        const float dc = smax(-nv * r.vm + err, -r.applied);
+        const float next = smin(r.max_impulse, 1.0f * r.applied + dc);
+        const float delta = next - r.applied;
+        const float da = delta * ima;
+        const float db = delta * imb;
+        va = {va.x + n.x * da, va.y + n.y * da, va.z + n.z * da};
+        vb = {vb.x - n.x * db, vb.y - n.y * db, vb.z - n.z * db};
+        wa = {wa.x + ra.x * delta, wa.y + ra.y * delta, wa.z + ra.z * delta};
+        wb = {wb.x - rb.x * delta, wb.y - rb.y * delta, wb.z - rb.z * delta};
+        bad |= !finite3(va) || !finite3(vb) || !finite3(wa) || !finite3(wb);
+        r.applied = next;
+        normal_impulse += next;
+    }
+    fyzz_dy_patch& P = W.patches[m];
+    const u32 anchors = P.anchor_count;
+    if ((iter >= pos_iters || pos_iters - iter <= 3u) && anchors) {
+        const float coef = anchors == 2 ? 0.5f : 1.0f;
+        const float max_static = P.static_friction * coef * normal_impulse;
+        const float max_dynamic = P.dynamic_friction * coef * normal_impulse;
+        bool broken = false;
+        for (u32 k = 0; k < anchors * 2; ++k) {
+            FRow& fr = W.frow[4 * m + k];
+            const V t{fr.tx, fr.ty, fr.tz};
+            const V ra{fr.rax, fr.ray, fr.raz}, rb{fr.rbx, fr.rby, fr.rbz};
+            const float td = vdot(t, va) - vdot(t, vb) + vdot(wa, ra) - vdot(wb, rb);
+            const float bias = iter < pos_iters ? fr.bias : 0.0f;
+            const float cand = fr.applied + (fr.target - bias - td) * fr.vm;
+            const bool clamped = fabsf(cand) > max_static;
+            const float next = clamped ? smin(max_dynamic, smax(-max_dynamic, cand)) : cand;
+            const float delta = next - fr.applied;
+            const float da = delta * ima;
+            const float db = delta * imb;
+            va = {va.x + t.x * da, va.y + t.y * da, va.z + t.z * da};
+            vb = {vb.x - t.x * db, vb.y - t.y * db, vb.z - t.z * db};
+            wa = {wa.x + ra.x * delta, wa.y + ra.y * delta, wa.z + ra.z * delta};
+            wb = {wb.x - rb.x * delta, wb.y - rb.y * delta, wb.z - rb.z * delta};
+            bad |= !finite3(va) || !finite3(vb) || !finite3(wa) || !finite3(wb);
+            fr.applied = next;
+            broken |= clamped;
+        }
+        if (iter + 1 == total_iters) P.flags = (P.flags & ~1u) | (broken ? 1u : 0u);
+    }
+    if (bad) raise_flag(W, FYZZ_DY_NONFINITE);
+    if (i0 >= 0) {
+        W.lin[i0] = make_float4(va.x, va.y, va.z, ima);
+        W.ang[i0] = make_float4(wa.x, wa.y, wa.z, 0.f);
+    }
+    if (i1 >= 0) {
+        W.lin[i1] = make_float4(vb.x, vb.y, vb.z, imb);
+        W.ang[i1] = make_float4(wb.x, wb.y, wb.z, 0.f);
+    }
+}
+
+// Position then velocity iterations over the partitions in order; motion is
+// snapshotted after the last position iteration (scene.cpp:9136-9146).
+__global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
+    cg::grid_group g = cg::this_grid();
+    const u32 tid = blockIdx.x * kBlock + threadIdx.x;
+    const u32 nthreads = gridDim.x * kBlock;
+    const u32 np = ld(&W.ctr[kNParts]);
+    const u32 pos_iters = ld(&W.ctr[kPosIters]);
+    const u32 vel_iters = max(ld(&W.ctr[kVelIters]), 1u);
+    const u32 total = pos_iters + vel_iters;
+    for (u32 iter = 0; iter < total; ++iter) {
+        for (u32 p = 0; p < np; ++p) {
+            const u32 lo = W.part_off[p], hi = W.part_off[p + 1];
+            for (u32 i = lo + tid; i < hi; i += nthreads) solve_manager(W, W.sorted[i], iter, pos_iters, total);
+            gsync(g);
+        }
+        if (iter + 1 == pos_iters) {
+            for (u32 i = tid; i < W.nb; i += nthreads)
+                if (W.has_row[i]) {
+                    W.mlin[i] = W.lin[i];
+                    W.mang[i] = W.ang[i];
+                }
+            gsync(g);
+        }
+    }
+}
+
+// ------------------------------------------------------------- integrate
+
+// scene.cpp:111-146
+__device__ bool integrate_quaternion(Q pose, V w, float dt, Q& out) {
+    float ww = w.x * w.x + w.y * w.y + w.z * w.z;
+    if (ww != 0.0f) {
+        ww = sqrtf(ww);
+        const float max_w = 1.0e7f;
+        if (ww > max_w) {
+            const float s = max_w / ww;
+            w = {w.x * s, w.y * s, w.z * s};
+            ww = max_w;
+        }
+        const float v = dt * ww * 0.5f;
+        const float s = sinf(v) / ww;
+        const float c = cosf(v);
+        const Q dq = qmul(Q{w.x * s, w.y * s, w.z * s, 0.0f}, pose);
+        const Q nx{dq.x + pose.x * c, dq.y + pose.y * c, dq.z + pose.z * c, dq.w + pose.w * c};
+        const float len = sqrtf(nx.x * nx.x + nx.y * nx.y + nx.z * nx.z + nx.w * nx.w);
+        if (!(len > 0.0f) || !isfinite(len)) return false;
+        pose = {nx.x / len, nx.y / len, nx.z / len, nx.w / len};
+    }
+    out = pose;
+    return true;
+}
+
+__device__ inline V apply_sii(V w, float4 c0, float4 c1, float4 c2, V s) {
+    return {w.x + c0.x * s.x + c1.x * s.y + c2.x * s.z, w.y + c0.y * s.x + c1.y * s.y + c2.y * s.z,
+            w.z + c0.z * s.x + c1.z * s.y + c2.z * s.z};
+}
+
+// publish (scene.cpp:9509-9549), integrate and sleep check (10508-10707)
+__global__ __launch_bounds__(kBlock) void k_integrate(Ws W) {
+    const u32 i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= W.nb) return;
+    fyzz_dy_body& B = W.bodies[i];
+    if (B.flags & 1u) return;
+    const float dt = W.dt;
+    V vel = f3(W.pv[i]), ang = f3(W.pw[i]);
+    V motion_lin = vel, motion_ang = ang;
+    if (W.has_row[i]) {
+        const V rv = vel, rw = ang;
+        const float4 c0 = W.sii[3 * i], c1 = W.sii[3 * i + 1], c2 = W.sii[3 * i + 2];
+        const V ld = f3(W.lin[i]), as = f3(W.ang[i]);
+        const V ml = f3(W.mlin[i]), ma = f3(W.mang[i]);
+        vel = vadd(rv, ld);
+        ang = apply_sii(rw, c0, c1, c2, as);
+        motion_lin = vadd(rv, ml);
+        motion_ang = apply_sii(rw, c0, c1, c2, ma);
+        if (!finite3(vel) || !finite3(ang)) raise_flag(W, FYZZ_DY_NONFINITE);
+    }
+    const V p0 = v3(B.position);
+    const Q q0 = q4(B.orientation);
+    const V center = mass_position(p0, q0);
+    V p{center.x + motion_lin.x * dt, center.y + motion_lin.y * dt, center.z + motion_lin.z * dt};
+    Q q;
+    if (!integrate_quaternion(mass_orientation(q0), motion_ang, dt, q)) {
+        raise_flag(W, FYZZ_DY_NONFINITE);
+        q = q0;
+    }
+    q = qmul(q, Q{-0.0f, -0.0f, -0.0f, 1.0f});
+    const V off = srot(q, V{0.f, 0.f, 0.f});
+    p = vsub(p, off);
+    if (!finite3(p) || !finite4(q)) raise_flag(W, FYZZ_DY_NONFINITE);
+    // sleep check (scene.cpp:10640-10705)
+    const float wake_reset = 20.0f * 0.02f;
+    const float energy_threshold = 5.0e-5f * 10.0f * 10.0f;
+    float wc = B.wake_counter;
+    V lin_acc = v3(B.sleep_linear_acc), ang_acc = v3(B.sleep_angular_acc);
+    if (wc < wake_reset * 0.5f || wc < dt) {
+        const V la = srot_inv(q, motion_ang);
+        const V l2 = vadd(lin_acc, motion_lin);
+        const V a2 = vadd(ang_acc, la);
+        const V ii = v3(B.inverse_inertia);
+        const V inertia{ii.x > 0.0f ? 1.0f / ii.x : 1.0f, ii.y > 0.0f ? 1.0f / ii.y : 1.0f,
+                        ii.z > 0.0f ? 1.0f / ii.z : 1.0f};
+        const float ang_e =
+            (a2.x * a2.x * inertia.x + a2.y * a2.y * inertia.y + a2.z * a2.z * inertia.z) * B.inverse_mass;
+        const float lin_e = l2.x * l2.x + l2.y * l2.y + l2.z * l2.z;
+        const float ne = 0.5f * (ang_e + lin_e);
+        const float cluster = 1.0f + float(B.counted_interactions);
+        const float th = cluster * energy_threshold;
+        if (ne >= th) {
+            lin_acc = {0.f, 0.f, 0.f};
+            ang_acc = {0.f, 0.f, 0.f};
+            const float factor = th == 0.0f ? 2.0f : smin(ne / th, 2.0f);
+            wc = factor * 0.5f * wake_reset + dt * (cluster - 1.0f);
+        } else {
+            lin_acc = l2;
+            ang_acc = a2;
+            wc = smax(wc - dt, 0.0f);
+        }
+    } else {
+        wc = smax(wc - dt, 0.0f);
+    }
+    const bool ready = wc == 0.0f;
+    if (ready) {
+        lin_acc = {0.f, 0.f, 0.f};
+        ang_acc = {0.f, 0.f, 0.f};
+    }
+    put3(B.position, p);
+    B.orientation[0] = q.x;
+    B.orientation[1] = q.y;
+    B.orientation[2] = q.z;
+    B.orientation[3] = q.w;
+    put3(B.linear_velocity, vel);
+    put3(B.angular_velocity, ang);
+    B.wake_counter = wc;
+    put3(B.sleep_linear_acc, lin_acc);
+    put3(B.sleep_angular_acc, ang_acc);
+    B.flags = (B.flags & ~2u) | (ready ? 2u : 0u);
+}
+
+// impulses per manifold contact and the partition of each manager
+__global__ __launch_bounds__(kBlock) void k_outputs(Ws W) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= W.nm) return;
+    if (W.parts_out) W.parts_out[m] = W.part[m];
+    if (!W.impulses) return;
+    float out[4] = {0.f, 0.f, 0.f, 0.f};
+    if (W.part[m] < kMaxParts) {
+        const u32 o = W.order[m];
+        const u32 no = o & 0xffu;
+        for (u32 q = 0; q < no; ++q) out[(o >> (8 + 4 * q)) & 0xfu] = W.crow[4 * m + q].applied;
+    }
+    for (u32 q = 0; q < 4; ++q) W.impulses[4 * m + q] = out[q];
+}
+
+}  // namespace fyzz_hip::dy
+
+using namespace fyzz_hip;
+
+struct fyzz_hip_dy {
+    fyzz_hip_context* ctx = nullptr;
+    u32 body_cap = 0, mgr_cap = 0;
+    int coop_blocks_partition = 0, coop_blocks_pgs = 0;
+    DevBuf<float4> pv, pw, lin, ang, mlin, mang, rpos, rq, sii;
+    DevBuf<u32> has_row, deg, off, fill, max_dyn, body_mask;
+    DevBuf<u32> inc, pred, succ, cnt, part, mask_after, front, pend, sorted, order;
+    DevBuf<dy::CRow> crow;
+    DevBuf<dy::FRow> frow;
+    DevBuf<u32> part_count, part_off, part_fill, blocksums, ctr, flags;
+    dy::Ws ws{};
+    bool prepared = false;
+};
+
+namespace {
+
+int coop_blocks(const void* kernel, int device) {
+    int per_cu = 0, cus = 0;
+    FYZZ_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, dy::kBlock, 0));
+    FYZZ_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    FYZZ_REQUIRE(per_cu > 0 && cus > 0, kDeviceError, "dy: cooperative kernel does not fit a compute unit");
+    return std::min(per_cu, 2) * cus;
+}
+
+void launch_coop(const void* kernel, int blocks, dy::Ws& ws, hipStream_t st) {
+    void* args[] = {&ws};
+    FYZZ_HIP_CHECK(hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(dy::kBlock), args, 0, st));
+}
+
+void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uint32_t ns,
+                const fyzz_dy_static* statics, uint32_t nm, const fyzz_dy_manager* managers,
+                const fyzz_dy_patch* history, uint32_t nh, fyzz_dy_patch* patches, bool partition_only) {
+    FYZZ_REQUIRE(nb <= d->body_cap, kOverflow, "dy: body count exceeds the capacity given at creation");
+    FYZZ_REQUIRE(nm <= d->mgr_cap, kOverflow, "dy: manager count exceeds the capacity given at creation");
+    FYZZ_REQUIRE(dt > 0.0f, kInvalidArgument, "dy: dt must be positive");
+    FYZZ_REQUIRE(nb == 0 || bodies, kInvalidArgument, "dy: null bodies");
+    FYZZ_REQUIRE(nm == 0 || managers, kInvalidArgument, "dy: null managers");
+    FYZZ_REQUIRE(ns == 0 || statics, kInvalidArgument, "dy: null statics");
+    FYZZ_REQUIRE(nh == 0 || history, kInvalidArgument, "dy: null history");
+    FYZZ_REQUIRE(partition_only || nm == 0 || patches, kInvalidArgument, "dy: null patch output");
+    dy::Ws& w = d->ws;
+    w.dt = dt;
+    w.nb = nb;
+    w.ns = partition_only ? 0xffffffffu : ns;  // partitions never read statics
+    w.nm = nm;
+    w.nh = nh;
+    w.bodies = bodies;
+    w.statics = statics;
+    w.mg = managers;
+    w.history = history;
+    w.patches = patches;
+    hipStream_t st = d->ctx->stream;
+    auto& prof = d->ctx->prof;
+    {
+        ProfScope ps(&prof, "dy.bodies", st);
+        hipLaunchKernelGGL(dy::k_clear, dim3(1), dim3(64), 0, st, w);
+        if (nb) hipLaunchKernelGGL(dy::k_pre, dim3(grid_for(nb, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
+        if (nm) hipLaunchKernelGGL(dy::k_mark, dim3(grid_for(nm, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
+        if (nb) hipLaunchKernelGGL(dy::k_rows, dim3(grid_for(nb, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    }
+    {
+        ProfScope ps(&prof, "dy.partition", st);
+        launch_coop(reinterpret_cast<const void*>(&dy::k_partition), d->coop_blocks_partition, w, st);
+    }
+    if (!partition_only && nm) {
+        ProfScope ps(&prof, "dy.prep", st);
+        hipLaunchKernelGGL(dy::k_prep, dim3(grid_for(nm, 128)), dim3(128), 0, st, w);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    }
+    d->prepared = !partition_only;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t manager_capacity,
+                       fyzz_hip_dy** out) {
+    return guarded(ctx, [&] {
+        FYZZ_REQUIRE(out, kInvalidArgument, "dy_create: null out");
+        FYZZ_REQUIRE(manager_capacity < (1u << 30), kInvalidArgument, "dy_create: manager capacity too large");
+        auto d = std::make_unique<fyzz_hip_dy>();
+        d->ctx = ctx;
+        d->body_cap = body_capacity;
+        d->mgr_cap = manager_capacity;
+        const std::size_t nb = std::max<u32>(body_capacity, 1), nm = std::max<u32>(manager_capacity, 1);
+        for (auto* b : {&d->pv, &d->pw, &d->lin, &d->ang, &d->mlin, &d->mang, &d->rpos, &d->rq}) b->reserve(nb);
+        d->sii.reserve(3 * nb);
+        for (auto* b : {&d->has_row, &d->deg, &d->fill, &d->max_dyn, &d->body_mask}) b->reserve(nb);
+        d->off.reserve(nb + 1);
+        for (auto* b : {&d->inc, &d->pred, &d->succ, &d->mask_after, &d->front}) b->reserve(2 * nm);
+        for (auto* b : {&d->cnt, &d->part, &d->pend, &d->sorted, &d->order}) b->reserve(nm);
+        d->crow.reserve(4 * nm);
+        d->frow.reserve(4 * nm);
+        d->part_count.reserve(dy::kMaxParts + 1);
+        d->part_off.reserve(dy::kMaxParts + 1);
+        d->part_fill.reserve(dy::kMaxParts);
+        d->ctr.reserve(dy::kCtrWords);
+        d->flags.reserve(1);
+        d->coop_blocks_partition = coop_blocks(reinterpret_cast<const void*>(&dy::k_partition), ctx->device);
+        d->coop_blocks_pgs = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs), ctx->device);
+        d->blocksums.reserve(2 * std::max(d->coop_blocks_partition, d->coop_blocks_pgs) + 2);
+        FYZZ_HIP_CHECK(hipMemset(d->flags.get(), 0, sizeof(u32)));
+        FYZZ_HIP_CHECK(hipMemset(d->ctr.get(), 0, dy::kCtrWords * sizeof(u32)));
+        dy::Ws& w = d->ws;
+        w.pv = d->pv.get();
+        w.pw = d->pw.get();
+        w.lin = d->lin.get();
+        w.ang = d->ang.get();
+        w.mlin = d->mlin.get();
+        w.mang = d->mang.get();
+        w.rpos = d->rpos.get();
+        w.rq = d->rq.get();
+        w.sii = d->sii.get();
+        w.has_row = d->has_row.get();
+        w.deg = d->deg.get();
+        w.off = d->off.get();
+        w.fill = d->fill.get();
+        w.max_dyn = d->max_dyn.get();
+        w.body_mask = d->body_mask.get();
+        w.inc = d->inc.get();
+        w.pred = d->pred.get();
+        w.succ = d->succ.get();
+        w.cnt = d->cnt.get();
+        w.part = d->part.get();
+        w.mask_after = d->mask_after.get();
+        w.front = d->front.get();
+        w.pend = d->pend.get();
+        w.sorted = d->sorted.get();
+        w.order = d->order.get();
+        w.crow = d->crow.get();
+        w.frow = d->frow.get();
+        w.part_count = d->part_count.get();
+        w.part_off = d->part_off.get();
+        w.part_fill = d->part_fill.get();
+        w.blocksums = d->blocksums.get();
+        w.ctr = d->ctr.get();
+        w.flags = d->flags.get();
+        *out = d.release();
+    });
+}
+
+int fyzz_hip_dy_destroy(fyzz_hip_dy* d) {
+    if (!d) return kInvalidArgument;
+    (void)hipSetDevice(d->ctx->device);
+    (void)hipStreamSynchronize(d->ctx->stream);
+    delete d;
+    return kOk;
+}
+
+int fyzz_hip_dy_partition(fyzz_hip_dy* d, uint32_t n_bodies, fyzz_dy_body* bodies, uint32_t n_managers,
+                          const fyzz_dy_manager* managers, uint32_t* partitions) {
+    if (!d) return kInvalidArgument;
+    return guarded(d->ctx, [&] {
+        FYZZ_REQUIRE(n_managers == 0 || partitions, kInvalidArgument, "dy_partition: null output");
+        dy_prepare(d, 1.0f, n_bodies, bodies, 0, nullptr, n_managers, managers, nullptr, 0, nullptr, true);
+        d->ws.parts_out = partitions;
+        d->ws.impulses = nullptr;
+        if (n_managers)
+            hipLaunchKernelGGL(dy::k_outputs, dim3(grid_for(n_managers, dy::kBlock)), dim3(dy::kBlock), 0,
+                               d->ctx->stream, d->ws);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int fyzz_hip_dy_prepare(fyzz_hip_dy* d, float dt, uint32_t n_bodies, fyzz_dy_body* bodies, uint32_t n_statics,
+                        const fyzz_dy_static* statics, uint32_t n_managers, const fyzz_dy_manager* managers,
+                        const fyzz_dy_patch* history, uint32_t n_history, fyzz_dy_patch* patches) {
+    if (!d) return kInvalidArgument;
+    return guarded(d->ctx, [&] {
+        dy_prepare(d, dt, n_bodies, bodies, n_statics, statics, n_managers, managers, history, n_history, patches,
+                   false);
+    });
+}
+
+int fyzz_hip_dy_solve(fyzz_hip_dy* d) {
+    if (!d) return kInvalidArgument;
+    return guarded(d->ctx, [&] {
+        FYZZ_REQUIRE(d->prepared, kLogicError, "dy_solve: no prepared step");
+        if (!d->ws.nm) return;
+        ProfScope ps(&d->ctx->prof, "dy.pgs", d->ctx->stream);
+        launch_coop(reinterpret_cast<const void*>(&dy::k_pgs), d->coop_blocks_pgs, d->ws, d->ctx->stream);
+    });
+}
+
+int fyzz_hip_dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
+    if (!d) return kInvalidArgument;
+    return guarded(d->ctx, [&] {
+        FYZZ_REQUIRE(d->prepared, kLogicError, "dy_integrate: no prepared step");
+        hipStream_t st = d->ctx->stream;
+        d->ws.impulses = impulses;
+        d->ws.parts_out = partitions;
+        ProfScope ps(&d->ctx->prof, "dy.integrate", st);
+        if (d->ws.nb)
+            hipLaunchKernelGGL(dy::k_integrate, dim3(grid_for(d->ws.nb, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
+        if (d->ws.nm && (impulses || partitions))
+            hipLaunchKernelGGL(dy::k_outputs, dim3(grid_for(d->ws.nm, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
+        FYZZ_HIP_CHECK(hipGetLastError());
+        d->prepared = false;
+    });
+}
+
+int fyzz_hip_dy_step(fyzz_hip_dy* d, float dt, uint32_t n_bodies, fyzz_dy_body* bodies, uint32_t n_statics,
+                     const fyzz_dy_static* statics, uint32_t n_managers, const fyzz_dy_manager* managers,
+                     const fyzz_dy_patch* history, uint32_t n_history, fyzz_dy_patch* patches, float* impulses,
+                     uint32_t* partitions) {
+    int rc = fyzz_hip_dy_prepare(d, dt, n_bodies, bodies, n_statics, statics, n_managers, managers, history,
+                                 n_history, patches);
+    if (rc == kOk) rc = fyzz_hip_dy_solve(d);
+    if (rc == kOk) rc = fyzz_hip_dy_integrate(d, impulses, partitions);
+    return rc;
+}
+
+int fyzz_hip_dy_status(fyzz_hip_dy* d, uint32_t* flags, uint32_t* n_partitions, uint32_t* rounds) {
+    if (!d) return kInvalidArgument;
+    return guarded(d->ctx, [&] {
+        u32 f = 0, c[dy::kCtrWords];
+        FYZZ_HIP_CHECK(hipMemcpyAsync(&f, d->flags.get(), sizeof f, hipMemcpyDeviceToHost, d->ctx->stream));
+        FYZZ_HIP_CHECK(hipMemcpyAsync(c, d->ctr.get(), sizeof c, hipMemcpyDeviceToHost, d->ctx->stream));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(d->ctx->stream));
+        if (flags) *flags = f;
+        if (n_partitions) *n_partitions = c[dy::kNParts];
+        if (rounds) *rounds = c[dy::kRounds];
+        FYZZ_REQUIRE(!(f & FYZZ_DY_BAD_INPUT), kInvalidArgument, "dy: malformed manager record");
+        FYZZ_REQUIRE(!(f & FYZZ_DY_OVERFLOW), kOverflow, "dy: partition or overflow-wave capacity exceeded");
+        FYZZ_REQUIRE(!(f & FYZZ_DY_NONFINITE), kOverflow, "dy: non-finite solver value");
+        FYZZ_REQUIRE(!(f & FYZZ_DY_LOGIC), kLogicError, "dy: manager outside the admitted class");
+    });
+}
+
+}  // extern "C"

@@ -41,6 +41,16 @@ SIGNATURES = {
     "fyzz_hip_pcm_plane_box": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp, vp]),
     "fyzz_hip_pcm_candidates": (C.c_int, [vp, C.c_uint32, vp, vp, C.c_float, vp, vp, vp, vp]),
     "fyzz_hip_pcm_gjk_epa": (C.c_int, [vp, C.c_uint32, vp, vp, C.c_float, vp, vp]),
+    "fyzz_hip_dy_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
+    "fyzz_hip_dy_destroy": (C.c_int, [vp]),
+    "fyzz_hip_dy_partition": (C.c_int, [vp, C.c_uint32, vp, C.c_uint32, vp, vp]),
+    "fyzz_hip_dy_prepare": (C.c_int, [vp, C.c_float, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, vp,
+                                      C.c_uint32, vp]),
+    "fyzz_hip_dy_solve": (C.c_int, [vp]),
+    "fyzz_hip_dy_integrate": (C.c_int, [vp, vp, vp]),
+    "fyzz_hip_dy_step": (C.c_int, [vp, C.c_float, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_uint32,
+                                   vp, vp, vp]),
+    "fyzz_hip_dy_status": (C.c_int, [vp, u32p, u32p, u32p]),
 }
 
 

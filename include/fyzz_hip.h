@@ -13,6 +13,8 @@
  *   GpuBroadphaseProducer plug-in           src/native/scene.h:383-505
  *   GpuBroadphaseReports {found,lost,...}   src/native/scene.h:78-91
  *   Scene::update_bounds / update_pairs     src/native/scene.cpp:1336-2004, 2564-2704
+ *   Scene::generate_pcm_box_box / plane_box src/native/scene.cpp:3466-3654, 3774-3955
+ *   Dy prepare / partition / solve / integrate  src/native/scene.cpp:6940-8511, 8853-9148, 10383-10707
  * Each declaration below names the reference function it stands in for.
  */
 #ifndef FYZZ_HIP_H
@@ -33,6 +35,7 @@ extern "C" {
 
 typedef struct fyzz_hip_context fyzz_hip_context;
 typedef struct fyzz_hip_sap fyzz_hip_sap;
+typedef struct fyzz_hip_dy fyzz_hip_dy;
 
 /* ---- Dy records (plain data, shared by the device entries and hosts) --
  * fyzz_dy_body: one dynamic rigid body (Scene::Actor's Dy fields,
@@ -106,7 +109,9 @@ int fyzz_hip_synchronize(fyzz_hip_context* ctx);
  * on the context's stream (stands in for Context::profile_batch_gpu_nanoseconds,
  * include/fyzzxyzzy/compute.h:299). profile_read synchronizes, returns the
  * summed milliseconds and span count of one name, and resets it. Names:
- * "sap.bounds", "sap.sort", "sap.sort.scatter", "sap.pairs", "sap.lists". */
+ * "sap.bounds", "sap.sort", "sap.sort.scatter", "sap.pairs", "sap.lists",
+ * "pcm.box_box", "pcm.plane_box", "dy.bodies", "dy.partition", "dy.prep",
+ * "dy.pgs", "dy.integrate". */
 int fyzz_hip_profile_enable(fyzz_hip_context* ctx, int enable);
 int fyzz_hip_profile_read(fyzz_hip_context* ctx, const char* name, double* total_ms,
                           uint64_t* count);
@@ -203,6 +208,65 @@ int fyzz_hip_pcm_candidates(fyzz_hip_context* ctx, uint32_t n, const float* a, c
                             uint32_t* selection);
 int fyzz_hip_pcm_gjk_epa(fyzz_hip_context* ctx, uint32_t n, const float* a, const float* b,
                          float contact_distance, float* out, uint32_t* ok);
+
+/* ---- Dy stage (device records, one solver task) ---------------------- */
+/* Status bits raised by a Dy step on the device (fyzz_hip_dy_status). */
+#define FYZZ_DY_NONFINITE 1u /* overflow_error: a non-finite solver value */
+#define FYZZ_DY_LOGIC 2u     /* logic_error: manager outside the admitted class */
+#define FYZZ_DY_OVERFLOW 4u  /* overflow_error: partition capacity exceeded */
+#define FYZZ_DY_BAD_INPUT 8u /* invalid_argument: malformed manager record */
+
+/* Workspace for steps of up to body_capacity bodies and manager_capacity
+ * managers (the producer's Dy buffers, GpuBroadphaseProducer::
+ * solve_gpu_dy_distance's spans, scene.h:424-448). */
+int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t manager_capacity,
+                       fyzz_hip_dy** out);
+int fyzz_hip_dy_destroy(fyzz_hip_dy* dy);
+
+/* Scene::partition_constraints (scene.cpp:7113-7489) for one task spanning
+ * the managers in the given (descriptor) order: first-fit over 32-bit
+ * masks for dynamic/dynamic managers, overflow waves of 32, and
+ * dynamic/static managers after their body's highest dynamic partition.
+ * bodies, managers (device) as for fyzz_hip_dy_step; partitions[n] (device). */
+int fyzz_hip_dy_partition(fyzz_hip_dy* dy, uint32_t n_bodies, fyzz_dy_body* bodies, uint32_t n_managers,
+                          const fyzz_dy_manager* managers, uint32_t* partitions);
+
+/* One Dy step over device records, in three stages that fyzz_hip_dy_step
+ * runs back to back:
+ *   prepare   prepare_unconstrained_velocities (scene.cpp:10383-10507),
+ *             prepare_solver_bodies (6940-7112), partition_constraints
+ *             (7113-7489) and the contact part of prepare_constraints
+ *             (7490-7515, 7763-8511): material combine, friction-patch
+ *             correlation with history[n_history] (a manager's `history`
+ *             indexes it), contact and friction rows; writes patches[nm],
+ *             the patch each manager carries into the next step;
+ *   solve     solve_constraint_task (8853-9148): position then velocity
+ *             iterations over the partitions (counts: the maximum over the
+ *             constrained bodies' `iterations`);
+ *   integrate publish_solver_bodies (9509-9549) and integrate_solver_task
+ *             (10508-10707): bodies[] are updated in place (pose,
+ *             velocities, wake counter, sleep accumulators, ready flag);
+ *             impulses[nm][4] receive each manifold contact's normal
+ *             impulse and partitions[nm] each manager's partition (either
+ *             may be NULL).
+ * All pointers are device pointers; statics[k] is the static body that a
+ * manager names as -1-k. Records stay borrowed until integrate returns. */
+int fyzz_hip_dy_prepare(fyzz_hip_dy* dy, float dt, uint32_t n_bodies, fyzz_dy_body* bodies,
+                        uint32_t n_statics, const fyzz_dy_static* statics, uint32_t n_managers,
+                        const fyzz_dy_manager* managers, const fyzz_dy_patch* history,
+                        uint32_t n_history, fyzz_dy_patch* patches);
+int fyzz_hip_dy_solve(fyzz_hip_dy* dy);
+int fyzz_hip_dy_integrate(fyzz_hip_dy* dy, float* impulses, uint32_t* partitions);
+int fyzz_hip_dy_step(fyzz_hip_dy* dy, float dt, uint32_t n_bodies, fyzz_dy_body* bodies,
+                     uint32_t n_statics, const fyzz_dy_static* statics, uint32_t n_managers,
+                     const fyzz_dy_manager* managers, const fyzz_dy_patch* history, uint32_t n_history,
+                     fyzz_dy_patch* patches, float* impulses, uint32_t* partitions);
+
+/* Synchronizes the context's stream and reports the last Dy step:
+ * flags (FYZZ_DY_* bits), the partition count and the number of
+ * dependency rounds the partitioner ran. Returns the status matching the
+ * first raised flag (the exception the reference would throw). */
+int fyzz_hip_dy_status(fyzz_hip_dy* dy, uint32_t* flags, uint32_t* n_partitions, uint32_t* rounds);
 
 #ifdef __cplusplus
 }

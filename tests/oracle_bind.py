@@ -174,24 +174,7 @@ def replay_pcm_scene(orc, g, on_step=None):
             on_step(s, pairs, manifolds)
 
 
-# fyzz_dy_* record layouts (include/fyzz_hip.h)
-DY_BODY = np.dtype([("position", "<f4", 3), ("orientation", "<f4", 4), ("linear_velocity", "<f4", 3),
-                    ("angular_velocity", "<f4", 3), ("inverse_mass", "<f4"), ("inverse_inertia", "<f4", 3),
-                    ("wake_counter", "<f4"), ("sleep_linear_acc", "<f4", 3), ("sleep_angular_acc", "<f4", 3),
-                    ("counted_interactions", "<u4"), ("flags", "<u4"), ("linear_damping", "<f4"),
-                    ("angular_damping", "<f4"), ("max_linear_velocity", "<f4"), ("max_angular_velocity", "<f4"),
-                    ("static_friction", "<f4"), ("dynamic_friction", "<f4"), ("restitution", "<f4"),
-                    ("iterations", "<u4"), ("pad", "<f4", 2)])
-DY_STATIC = np.dtype([("position", "<f4", 3), ("orientation", "<f4", 4), ("static_friction", "<f4"),
-                      ("dynamic_friction", "<f4"), ("restitution", "<f4")])
-DY_CONTACT = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("separation", "<f4")])
-DY_MANAGER = np.dtype([("body0", "<i4"), ("body1", "<i4"), ("count", "<u4"), ("history", "<u4"),
-                       ("contacts", DY_CONTACT, 4)])
-DY_PATCH = np.dtype([("body0_normal", "<f4", 3), ("body1_normal", "<f4", 3), ("body0_anchors", "<f4", (2, 3)),
-                     ("body1_anchors", "<f4", (2, 3)), ("static_friction", "<f4"), ("dynamic_friction", "<f4"),
-                     ("restitution", "<f4"), ("damping", "<f4"), ("anchor_count", "<u4"), ("flags", "<u4")])
-assert DY_BODY.itemsize == 144 and DY_STATIC.itemsize == 40
-assert DY_MANAGER.itemsize == 128 and DY_PATCH.itemsize == 96
+from fyzz_amd.dy import DY_BODY, DY_CONTACT, DY_MANAGER, DY_PATCH, DY_STATIC  # noqa: E402,F401
 
 
 def dy_bodies(n):
@@ -245,101 +228,134 @@ def body_state_from(bodies):
                            bodies["angular_velocity"]], axis=1)
 
 
-def replay_dy_scene(orc, g, on_step, fp64=False):
-    """Replays the reference pile's Dy steps with the oracle from each
-    step's recorded start state (synchronized state): the oracle SAP and
-    PCM (pinned elsewhere) regenerate the step's pairs and manifolds, the
-    recorded descriptor stream fixes the solver order, and the oracle Dy
-    step runs with its own carried friction history and sleep
-    accumulators. Calls on_step(s, bodies_out, extra, partitions,
-    managers)."""
-    import sys as _sys
-    from pathlib import Path as _P
-    _sys.path.insert(0, str(_P(__file__).resolve().parent.parent / "fyzzxyzzy-sdk_amd"))
-    from fyzz_amd.scenes import box_mass_properties
-    pcm = OraclePcm(orc.lib)
-    dy = OracleDy(orc.lib)
-    half, ids = g["half"], g["ids"]
-    floor = int(g["floor_id"][0])
-    n = len(ids)
-    slot = {int(v): i for i, v in enumerate(ids)}
-    inv_mass, inv_inertia = box_mass_properties(half, 1000.0)
-    statics = np.zeros(1, DY_STATIC)
-    statics["orientation"][0, 3] = 1.0
-    statics["static_friction"] = 0.5
-    statics["dynamic_friction"] = 0.5
-    statics["restitution"] = 0.0
-    caches = {}
-    history = {}
-    acc_lin = np.zeros((n, 3), np.float32)
-    acc_ang = np.zeros((n, 3), np.float32)
-    for s in range(int(g["steps"][0])):
+class DyScene:
+    """Per-step Dy inputs of the reference pile golden (dy_scene.npz),
+    regenerated from each step's recorded start state: the oracle SAP and
+    PCM (pinned to the reference elsewhere) give the step's pairs and
+    manifolds, and the recorded descriptor stream fixes the solver order.
+    Friction history and sleep accumulators are carried by the caller (each
+    solver carries its own)."""
+
+    _cache = {}
+
+    def __init__(self, orc, g):
+        from fyzz_amd.scenes import box_mass_properties
+        key = id(g)
+        self.g = g
+        self.half, self.ids = g["half"], g["ids"]
+        self.n = len(self.ids)
+        self.steps = int(g["steps"][0])
+        self.inv_mass, self.inv_inertia = box_mass_properties(self.half, 1000.0)
+        st = np.zeros(1, DY_STATIC)
+        st["orientation"][0, 3] = 1.0
+        st["static_friction"] = 0.5
+        st["dynamic_friction"] = 0.5
+        self.statics = st
+        if key not in DyScene._cache:
+            DyScene._cache[key] = self._regenerate(orc)
+        self.managers, self.keys, self.counted = DyScene._cache[key]
+
+    def _regenerate(self, orc):
+        g, half, ids, n = self.g, self.half, self.ids, self.n
+        pcm = OraclePcm(orc.lib)
+        floor = int(g["floor_id"][0])
+        slot = {int(v): i for i, v in enumerate(ids)}
+        caches = {}
+        all_mg, all_keys, all_counted = [], [], []
+        for s in range(self.steps):
+            st = g["state"][s]
+            pos, quat = np.ascontiguousarray(st[:, 0:3]), np.ascontiguousarray(st[:, 6:10])
+            b = orc.box_bounds(pos, quat, half, np.full(n, 0.02, np.float32))
+            pairs = orc.sap_pairs(b, shape_id=ids, plane_ids=[floor])
+            manifolds, new_caches = {}, {}
+            for pa, pb in pairs:
+                key = (int(pa), int(pb))
+                cache = caches.get(key, pcm.new_cache())
+                if pa == floor or pb == floor:
+                    bx = slot[int(pb if pa == floor else pa)]
+                    rec = np.concatenate([half[bx], pos[bx], quat[bx]])
+                    cnt, pts, _ = pcm.plane_box(np.zeros(3, np.float32), rec, 0.04, cache)
+                    pts = pts.copy()
+                    pts[:, 3:6] *= -1.0  # published +y normal
+                    bodies_gu = (bx, -1)
+                else:
+                    # two dynamics: the Gu pair puts the higher body id first
+                    f, sc_ = slot[int(pb)], slot[int(pa)]
+                    r1 = np.concatenate([half[f], pos[f], quat[f]])
+                    r2 = np.concatenate([half[sc_], pos[sc_], quat[sc_]])
+                    cnt, pts, _ = pcm.box_box(r1, r2, 0.04, cache)
+                    bodies_gu = (f, sc_)
+                new_caches[key] = cache
+                if cnt:
+                    manifolds[key] = (cnt, pts, bodies_gu)
+            caches = new_caches
+            counted = np.zeros(n, np.uint32)
+            for pa, pb in pairs:
+                for x in (pa, pb):
+                    if int(x) != floor:
+                        counted[slot[int(x)]] += 1
+            desc = g[f"desc_{s}"]
+            mg = np.zeros(len(desc), DY_MANAGER)
+            keys = []
+            for i, (pa, pb) in enumerate(desc):
+                key = (int(pa), int(pb))
+                cnt, pts, (b0, b1) = manifolds[key]
+                mg[i]["body0"], mg[i]["body1"], mg[i]["count"] = b0, b1, cnt
+                for c in range(cnt):
+                    mg[i]["contacts"][c]["position"] = pts[c, 0:3]
+                    mg[i]["contacts"][c]["normal"] = pts[c, 3:6]
+                    mg[i]["contacts"][c]["separation"] = pts[c, 6]
+                keys.append(key)
+            all_mg.append(mg)
+            all_keys.append(keys)
+            all_counted.append(counted)
+        return all_mg, all_keys, all_counted
+
+    def bodies(self, s, acc_lin, acc_ang):
+        g = self.g
         st = g["state"][s]
-        pos, vel, quat, ang = st[:, 0:3], st[:, 3:6], st[:, 6:10], st[:, 10:13]
-        b = orc.box_bounds(np.ascontiguousarray(pos), np.ascontiguousarray(quat), half,
-                           np.full(n, 0.02, np.float32))
-        pairs = orc.sap_pairs(b, shape_id=ids, plane_ids=[floor])
-        manifolds = {}
-        new_caches = {}
-        for pa, pb in pairs:
-            key = (int(pa), int(pb))
-            cache = caches.get(key, pcm.new_cache())
-            if pa == floor or pb == floor:
-                bx = slot[int(pb if pa == floor else pa)]
-                rec = np.concatenate([half[bx], pos[bx], quat[bx]])
-                cnt, pts, _ = pcm.plane_box(np.zeros(3, np.float32), rec, 0.04, cache)
-                pts = pts.copy()
-                pts[:, 3:6] *= -1.0
-                bodies_gu = (bx, -1)
-            else:
-                f, sc_ = slot[int(pb)], slot[int(pa)]
-                r1 = np.concatenate([half[f], pos[f], quat[f]])
-                r2 = np.concatenate([half[sc_], pos[sc_], quat[sc_]])
-                cnt, pts, _ = pcm.box_box(r1, r2, 0.04, cache)
-                bodies_gu = (f, sc_)
-            new_caches[key] = cache
-            if cnt:
-                manifolds[key] = (cnt, pts, bodies_gu)
-        caches = new_caches
-        counted = np.zeros(n, np.uint32)
-        for pa, pb in pairs:
-            for x in (pa, pb):
-                if int(x) != floor:
-                    counted[slot[int(x)]] += 1
-        bodies = dy_bodies(n)
-        bodies["position"] = pos
-        bodies["orientation"] = quat
-        bodies["linear_velocity"] = vel
-        bodies["angular_velocity"] = ang
-        bodies["inverse_mass"] = inv_mass
-        bodies["inverse_inertia"] = inv_inertia
-        bodies["wake_counter"] = g["wake"][s]
-        bodies["flags"] = 4 | (g["ready"][s].astype(np.uint32) << 1)
-        bodies["sleep_linear_acc"] = acc_lin
-        bodies["sleep_angular_acc"] = acc_ang
-        bodies["counted_interactions"] = counted
-        bodies["restitution"] = 0.0
-        desc = g[f"desc_{s}"]
-        mg = np.zeros(len(desc), DY_MANAGER)
-        hist_list = []
-        keys = []
-        for i, (pa, pb) in enumerate(desc):
-            key = (int(pa), int(pb))
-            cnt, pts, (b0, b1) = manifolds[key]
-            mg[i]["body0"], mg[i]["body1"], mg[i]["count"] = b0, b1, cnt
-            for c in range(cnt):
-                mg[i]["contacts"][c]["position"] = pts[c, 0:3]
-                mg[i]["contacts"][c]["normal"] = pts[c, 3:6]
-                mg[i]["contacts"][c]["separation"] = pts[c, 6]
+        b = dy_bodies(self.n)
+        b["position"] = st[:, 0:3]
+        b["linear_velocity"] = st[:, 3:6]
+        b["orientation"] = st[:, 6:10]
+        b["angular_velocity"] = st[:, 10:13]
+        b["inverse_mass"] = self.inv_mass
+        b["inverse_inertia"] = self.inv_inertia
+        b["wake_counter"] = g["wake"][s]
+        b["flags"] = 4 | (g["ready"][s].astype(np.uint32) << 1)
+        b["sleep_linear_acc"] = acc_lin
+        b["sleep_angular_acc"] = acc_ang
+        b["counted_interactions"] = self.counted[s]
+        b["restitution"] = 0.0
+        return b
+
+    def step_managers(self, s, history):
+        """(managers with history indices, history records) for a
+        {pair key: patch} dict of the previous step's patches."""
+        mg = self.managers[s].copy()
+        hist = []
+        for i, key in enumerate(self.keys[s]):
             if key in history:
-                mg[i]["history"] = len(hist_list)
-                hist_list.append(history[key])
+                mg[i]["history"] = len(hist)
+                hist.append(history[key])
             else:
                 mg[i]["history"] = 0xFFFFFFFF
-            keys.append(key)
-        hist = np.array(hist_list, DY_PATCH) if hist_list else np.zeros(0, DY_PATCH)
-        out, patches, imp, extra = dy.step(1.0 / 120.0, bodies, statics, mg, hist, fp64=fp64)
-        history = {k: patches[i] for i, k in enumerate(keys)}
+        return mg, (np.array(hist, DY_PATCH) if hist else np.zeros(0, DY_PATCH))
+
+
+def replay_dy_scene(orc, g, on_step, fp64=False):
+    """Oracle Dy over the golden pile from synchronized start states; calls
+    on_step(s, bodies_out, partitions or FP64 state, managers)."""
+    sc = DyScene(orc, g)
+    dy = OracleDy(orc.lib)
+    history = {}
+    acc_lin = np.zeros((sc.n, 3), np.float32)
+    acc_ang = np.zeros((sc.n, 3), np.float32)
+    for s in range(sc.steps):
+        mg, hist = sc.step_managers(s, history)
+        out, patches, imp, extra = dy.step(1.0 / 120.0, sc.bodies(s, acc_lin, acc_ang), sc.statics, mg, hist,
+                                           fp64=fp64)
+        history = {k: patches[i] for i, k in enumerate(sc.keys[s])}
         acc_lin = out["sleep_linear_acc"].copy()
         acc_ang = out["sleep_angular_acc"].copy()
         on_step(s, out, extra, mg)
