@@ -51,6 +51,8 @@ def build_hip(verbose: bool = False) -> Path:
             jobs.append(ex.submit(_run, cmd))
         for j in jobs:
             j.result()
+    scenes_so = PKG / "fyzz_amd" / "libfyzz_scenes.so"
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-o", str(scenes_so), str(CSRC / "scenes_host.cpp")])
     tmp = OUT.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, OUT)

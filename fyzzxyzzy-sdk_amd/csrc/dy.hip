@@ -37,6 +37,7 @@
 
 #include "../../include/fyzz_hip.h"
 #include "context.hpp"
+#include "stages.hpp"
 #include "vecmath.hpp"
 
 namespace cg = cooperative_groups;
@@ -79,6 +80,7 @@ struct Ws {
     fyzz_dy_patch* patches;
     float* impulses;
     u32* parts_out;
+    const u32* d_nm;  // device manager count (at most nm), or null
     // per body
     float4 *pv, *pw;          // pre-integrated velocities
     float4 *lin, *ang;        // solver deltas: lin_delta|inv_mass, ang_state
@@ -99,6 +101,7 @@ struct Ws {
 
 __device__ inline u32 ld(const u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ inline void raise_flag(const Ws& W, u32 f) { atomicOr(W.flags, f); }
+__device__ inline u32 mcount(const Ws& W) { return W.d_nm ? min(ld(W.d_nm), W.nm) : W.nm; }
 __device__ inline void gsync(cg::grid_group& g) {
     __threadfence();
     g.sync();
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_pre(Ws W) {
 // bodies of touching managers get a solver row (scene.cpp:6987-7013)
 __global__ __launch_bounds__(kBlock) void k_mark(Ws W) {
     const u32 m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= W.nm) return;
+    if (m >= mcount(W)) return;
     const fyzz_dy_manager M = W.mg[m];
     if (!manager_ok(W, M)) {
         raise_flag(W, FYZZ_DY_BAD_INPUT);
@@ -284,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
     cg::grid_group g = cg::this_grid();
     const u32 tid = blockIdx.x * kBlock + threadIdx.x;
     const u32 nthreads = gridDim.x * kBlock;
+    const u32 nm = mcount(W);
     __shared__ u32 scratch[kBlock / kWave + 1];
     // P0: clear per-body and per-partition state
     for (u32 i = tid; i < W.nb; i += nthreads) {
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
     if (tid <= kNParts) W.ctr[tid] = 0;  // frontier, pending and partition counters
     gsync(g);
     // P1: incidence degrees (dynamic sides)
-    for (u32 m = tid; m < W.nm; m += nthreads) {
+    for (u32 m = tid; m < nm; m += nthreads) {
         const fyzz_dy_manager& M = W.mg[m];
         W.part[m] = kNone;
         if (!manager_ok(W, M)) continue;
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
     grid_scan_degrees(W, g);
     gsync(g);
     // P3: incidence lists, entry = manager << 1 | side
-    for (u32 m = tid; m < W.nm; m += nthreads) {
+    for (u32 m = tid; m < nm; m += nthreads) {
         const fyzz_dy_manager& M = W.mg[m];
         if (!manager_ok(W, M)) continue;
         if (M.body0 >= 0) W.inc[W.off[M.body0] + atomicAdd(&W.fill[M.body0], 1u)] = m << 1;
@@ -340,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
     }
     gsync(g);
     // P5: ready managers (no predecessor on either body)
-    for (u32 m = tid; m < W.nm; m += nthreads) {
+    for (u32 m = tid; m < nm; m += nthreads) {
         if (!dyn_dyn(W.mg[m]) || !manager_ok(W, W.mg[m])) continue;
         const u32 c = (W.pred[2 * m] != kNone ? 1u : 0u) + (W.pred[2 * m + 1] != kNone ? 1u : 0u);
         W.cnt[m] = c;
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
     }
     gsync(g);
     // P6: highest dynamic partition per body
-    for (u32 m = tid; m < W.nm; m += nthreads) {
+    for (u32 m = tid; m < nm; m += nthreads) {
         const fyzz_dy_manager& M = W.mg[m];
         if (!dyn_dyn(M)) continue;
         const u32 pm = W.part[m];
@@ -450,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
     }
     gsync(g);
     // P8: partition sizes
-    for (u32 m = tid; m < W.nm; m += nthreads) {
+    for (u32 m = tid; m < nm; m += nthreads) {
         const u32 pm = W.part[m];
         if (pm == kNone) continue;  // malformed record, already flagged
         if (pm >= kMaxParts) {
@@ -475,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
     }
     gsync(g);
     // P10: managers grouped by partition (order inside a partition is free)
-    for (u32 m = tid; m < W.nm; m += nthreads) {
+    for (u32 m = tid; m < nm; m += nthreads) {
         const u32 pm = W.part[m];
         if (pm >= kMaxParts) continue;
         W.sorted[W.part_off[pm] + atomicAdd(&W.part_fill[pm], 1u)] = m;
@@ -573,7 +577,7 @@ struct Run {
 // friction rows (scene.cpp:7794-8505).
 __global__ __launch_bounds__(128) void k_prep(Ws W) {
     const u32 m = blockIdx.x * 128 + threadIdx.x;
-    if (m >= W.nm) return;
+    if (m >= mcount(W)) return;
     const fyzz_dy_manager& M = W.mg[m];
     if (W.part[m] >= kMaxParts) return;
     const float inv_dt = 1.0f / W.dt;
@@ -981,8 +985,12 @@ __device__ bool integrate_quaternion(Q pose, V w, float dt, Q& out) {
             ww = max_w;
         }
         const float v = dt * ww * 0.5f;
+        // The reference calls glibc's sinf/cosf, which are not correctly
+        // rounded. For the small arguments of a step, the device sinf and
+        // the double-precision cos rounded to float agree with them on all
+        // but ~0.1% of arguments (one ulp apart there).
         const float s = sinf(v) / ww;
-        const float c = cosf(v);
+        const float c = static_cast<float>(cos(static_cast<double>(v)));
         const Q dq = qmul(Q{w.x * s, w.y * s, w.z * s, 0.0f}, pose);
         const Q nx{dq.x + pose.x * c, dq.y + pose.y * c, dq.z + pose.z * c, dq.w + pose.w * c};
         const float len = sqrtf(nx.x * nx.x + nx.y * nx.y + nx.z * nx.z + nx.w * nx.w);
@@ -1083,7 +1091,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Ws W) {
 // impulses per manifold contact and the partition of each manager
 __global__ __launch_bounds__(kBlock) void k_outputs(Ws W) {
     const u32 m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= W.nm) return;
+    if (m >= mcount(W)) return;
     if (W.parts_out) W.parts_out[m] = W.part[m];
     if (!W.impulses) return;
     float out[4] = {0.f, 0.f, 0.f, 0.f};
@@ -1130,7 +1138,8 @@ void launch_coop(const void* kernel, int blocks, dy::Ws& ws, hipStream_t st) {
 
 void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uint32_t ns,
                 const fyzz_dy_static* statics, uint32_t nm, const fyzz_dy_manager* managers,
-                const fyzz_dy_patch* history, uint32_t nh, fyzz_dy_patch* patches, bool partition_only) {
+                const fyzz_dy_patch* history, uint32_t nh, fyzz_dy_patch* patches, bool partition_only,
+                const u32* d_nm = nullptr) {
     FYZZ_REQUIRE(nb <= d->body_cap, kOverflow, "dy: body count exceeds the capacity given at creation");
     FYZZ_REQUIRE(nm <= d->mgr_cap, kOverflow, "dy: manager count exceeds the capacity given at creation");
     FYZZ_REQUIRE(dt > 0.0f, kInvalidArgument, "dy: dt must be positive");
@@ -1150,6 +1159,7 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
     w.mg = managers;
     w.history = history;
     w.patches = patches;
+    w.d_nm = d_nm;
     hipStream_t st = d->ctx->stream;
     auto& prof = d->ctx->prof;
     {
@@ -1172,7 +1182,38 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
     d->prepared = !partition_only;
 }
 
+void dy_solve(fyzz_hip_dy* d) {
+    FYZZ_REQUIRE(d->prepared, kLogicError, "dy_solve: no prepared step");
+    if (!d->ws.nm) return;
+    ProfScope ps(&d->ctx->prof, "dy.pgs", d->ctx->stream);
+    launch_coop(reinterpret_cast<const void*>(&dy::k_pgs), d->coop_blocks_pgs, d->ws, d->ctx->stream);
+}
+
+void dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
+    FYZZ_REQUIRE(d->prepared, kLogicError, "dy_integrate: no prepared step");
+    hipStream_t st = d->ctx->stream;
+    d->ws.impulses = impulses;
+    d->ws.parts_out = partitions;
+    ProfScope ps(&d->ctx->prof, "dy.integrate", st);
+    if (d->ws.nb)
+        hipLaunchKernelGGL(dy::k_integrate, dim3(grid_for(d->ws.nb, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
+    if (d->ws.nm && (impulses || partitions))
+        hipLaunchKernelGGL(dy::k_outputs, dim3(grid_for(d->ws.nm, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
+    FYZZ_HIP_CHECK(hipGetLastError());
+    d->prepared = false;
+}
+
 }  // namespace
+
+namespace fyzz_hip::dy {
+void step_device(fyzz_hip_dy* d, float dt, u32 nb, fyzz_dy_body* bodies, u32 ns, const fyzz_dy_static* statics,
+                 u32 nm_cap, const u32* d_nm, const fyzz_dy_manager* managers, const fyzz_dy_patch* history,
+                 u32 nh, fyzz_dy_patch* patches, float* impulses, u32* partitions) {
+    dy_prepare(d, dt, nb, bodies, ns, statics, nm_cap, managers, history, nh, patches, false, d_nm);
+    dy_solve(d);
+    dy_integrate(d, impulses, partitions);
+}
+}  // namespace fyzz_hip::dy
 
 extern "C" {
 
@@ -1277,29 +1318,12 @@ int fyzz_hip_dy_prepare(fyzz_hip_dy* d, float dt, uint32_t n_bodies, fyzz_dy_bod
 
 int fyzz_hip_dy_solve(fyzz_hip_dy* d) {
     if (!d) return kInvalidArgument;
-    return guarded(d->ctx, [&] {
-        FYZZ_REQUIRE(d->prepared, kLogicError, "dy_solve: no prepared step");
-        if (!d->ws.nm) return;
-        ProfScope ps(&d->ctx->prof, "dy.pgs", d->ctx->stream);
-        launch_coop(reinterpret_cast<const void*>(&dy::k_pgs), d->coop_blocks_pgs, d->ws, d->ctx->stream);
-    });
+    return guarded(d->ctx, [&] { dy_solve(d); });
 }
 
 int fyzz_hip_dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
     if (!d) return kInvalidArgument;
-    return guarded(d->ctx, [&] {
-        FYZZ_REQUIRE(d->prepared, kLogicError, "dy_integrate: no prepared step");
-        hipStream_t st = d->ctx->stream;
-        d->ws.impulses = impulses;
-        d->ws.parts_out = partitions;
-        ProfScope ps(&d->ctx->prof, "dy.integrate", st);
-        if (d->ws.nb)
-            hipLaunchKernelGGL(dy::k_integrate, dim3(grid_for(d->ws.nb, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
-        if (d->ws.nm && (impulses || partitions))
-            hipLaunchKernelGGL(dy::k_outputs, dim3(grid_for(d->ws.nm, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
-        FYZZ_HIP_CHECK(hipGetLastError());
-        d->prepared = false;
-    });
+    return guarded(d->ctx, [&] { dy_integrate(d, impulses, partitions); });
 }
 
 int fyzz_hip_dy_step(fyzz_hip_dy* d, float dt, uint32_t n_bodies, fyzz_dy_body* bodies, uint32_t n_statics,

@@ -28,6 +28,7 @@
 
 #include "../../include/fyzz_hip.h"
 #include "context.hpp"
+#include "stages.hpp"
 #include "vecmath.hpp"
 
 namespace fyzz_hip::pcm {
@@ -942,16 +943,24 @@ struct PairArgs {
     u32* flags;
     u32* gjk_list;    // pairs deferred to the continuation
     u32* gjk_count;
+    const u32* d_n = nullptr;          // device pair count (at most n_pairs), or null
+    const Cache* cache_src = nullptr;  // read caches from here (double-buffered), or null
+    const u32* src_index = nullptr;    // per pair: cache_src record, kNoSource = empty
 };
 
 // scene.cpp:3466-3654 up to the zero-witness continuation.
 __global__ __launch_bounds__(128) void k_pcm_box_box(PairArgs A) {
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.n_pairs) return;
+    const u32 n = A.d_n ? min(*A.d_n, A.n_pairs) : A.n_pairs;
+    if (t >= n) return;
     const Box3 f = load_box(A.pos, A.quat, A.half, A.first[t]);
     const Box3 s = load_box(A.pos, A.quat, A.half, A.second[t]);
     Cache& cref = A.cache[A.cache_index ? A.cache_index[t] : t];
-    Cache cache = cref;
+    Cache cache{};
+    if (!A.cache_src)
+        cache = cref;
+    else if (A.src_index[t] != kNoSource)
+        cache = A.cache_src[A.src_index[t]];
     u32 flags = 0;
     const V delta{f.pos.x - s.pos.x, f.pos.y - s.pos.y, f.pos.z - s.pos.z};
     const V rel = srot_inv(s.rot, delta);
@@ -1087,7 +1096,7 @@ struct PlaneArgs {
 __global__ __launch_bounds__(128) void k_pcm_plane_box(PlaneArgs A) {
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.n) return;
-    const Box3 bx = load_box(A.pos, A.quat, A.half, A.box[t]);
+    const Box3 bx = load_box(A.pos, A.quat, A.half, A.box ? A.box[t] : t);
     Cache& cref = A.cache[A.cache_index ? A.cache_index[t] : t];
     Cache cache = cref;
     const V rel{bx.pos.x - A.plane_pos.x, bx.pos.y - A.plane_pos.y, bx.pos.z - A.plane_pos.z};
@@ -1317,6 +1326,40 @@ int fyzz_hip_pcm_plane_box(fyzz_hip_context* ctx, uint32_t n, const uint32_t* bo
         FYZZ_HIP_CHECK(hipGetLastError());
     });
 }
+
+}  // extern "C"
+
+namespace fyzz_hip::pcm {
+
+void box_box_list(fyzz_hip_context* ctx, const ListPairs& L, const float* pos, const float* quat,
+                  const float* half, float cd, float* contacts, u32* counts, u32* flags, u32* scratch) {
+    if (!L.cap) return;
+    PairArgs A{L.cap, L.first, L.second, nullptr, pos, quat, half, cd, static_cast<Cache*>(L.cache_dst),
+               contacts, counts, flags, scratch + 1, scratch};
+    A.d_n = L.d_n;
+    A.cache_src = static_cast<const Cache*>(L.cache_src);
+    A.src_index = L.src_index;
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, scratch);
+    ProfScope ps(&ctx->prof, "pcm.box_box", st);
+    hipLaunchKernelGGL(k_pcm_box_box, dim3(grid_for(L.cap, 128)), dim3(128), 0, st, A);
+    hipLaunchKernelGGL(k_pcm_gjk, dim3(std::min<u32>(grid_for(L.cap, 64), 1024u)), dim3(64), 0, st, A);
+    FYZZ_HIP_CHECK(hipGetLastError());
+}
+
+void plane_all(fyzz_hip_context* ctx, u32 n, const float* pos, const float* quat, const float* half,
+               const float plane_position[3], float cd, void* cache, float* contacts, u32* counts, u32* flags) {
+    if (!n) return;
+    PlaneArgs A{n, nullptr, nullptr, pos, quat, half, {plane_position[0], plane_position[1], plane_position[2]}, cd,
+                static_cast<Cache*>(cache), contacts, counts, flags};
+    ProfScope ps(&ctx->prof, "pcm.plane_box", ctx->stream);
+    hipLaunchKernelGGL(k_pcm_plane_box, dim3(grid_for(n, 128)), dim3(128), 0, ctx->stream, A);
+    FYZZ_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace fyzz_hip::pcm
+
+extern "C" {
 
 int fyzz_hip_pcm_candidates(fyzz_hip_context* ctx, uint32_t n, const float* a, const float* b,
                             float contact_distance, float* out, uint32_t* counts, uint32_t* overlapping,

@@ -25,6 +25,7 @@
 
 #include "../../include/fyzz_hip.h"
 #include "context.hpp"
+#include "stages.hpp"
 #include "radix.hpp"
 #include "select.hpp"
 
@@ -584,6 +585,16 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
 struct fyzz_hip_sap {
     fyzz_hip::sap::Sap s;
 };
+
+namespace fyzz_hip::sap {
+Lists lists(fyzz_hip_sap* h) {
+    Sap& s = h->s;
+    const u32 cur = s.cur == 0 ? kPairs0 : kPairs1;
+    const u32 prev = s.cur == 0 ? kPairs1 : kPairs0;
+    return {s.pairs[s.cur].p, s.counters.p + cur, s.pairs[s.cur ^ 1].p, s.counters.p + prev,
+            s.pair_cap, s.counters.p + kError, s.counters.p + kOverflow};
+}
+}  // namespace fyzz_hip::sap
 
 using namespace fyzz_hip;
 

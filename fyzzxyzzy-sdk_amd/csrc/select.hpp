@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void k_count(u32 cap, const u32* d_n, P pre
 }
 
 // Exclusive scan of the needed tile sums in place (one block); *d_out = total.
-__global__ __launch_bounds__(1024) void k_scan_tiles(u32 cap, const u32* d_n, u32* tile_sums, u32* d_out,
+static __global__ __launch_bounds__(1024) void k_scan_tiles(u32 cap, const u32* d_n, u32* tile_sums, u32* d_out,
                                                      const u32* gate) {
     __shared__ u32 scratch[1024 / kWave + 1];
     if (gate && *gate == 0) return;

@@ -1,0 +1,55 @@
+// Internal hooks between the stage sources and the step pipeline: device
+// pair lists of the broadphase, list-driven narrowphase launches and a Dy
+// step whose manager count stays on the device. Nothing here crosses the
+// C-ABI.
+#pragma once
+#include "../../include/fyzz_hip.h"
+#include "common.hpp"
+#include "context.hpp"
+
+namespace fyzz_hip {
+
+namespace sap {
+// Current and previous sorted pair lists (u64 lo slot << 32 | hi slot) of a
+// broadphase after its last advance, with their device counts.
+struct Lists {
+    const u64* cur;
+    const u32* d_cur;
+    const u64* prev;
+    const u32* d_prev;
+    u32 cap;
+    const u32* d_error;     // non-zero: a box bound was not finite
+    const u32* d_overflow;  // non-zero: pair or delta capacity exceeded
+};
+Lists lists(fyzz_hip_sap* h);
+}  // namespace sap
+
+namespace pcm {
+constexpr u32 kNoSource = 0xffffffffu;
+// Box/box pairs of a device list: pair t (t < min(*d_n, cap)) reads its
+// cache from cache_src[src_index[t]] (a zero cache when kNoSource) and
+// writes it to cache_dst[t].
+struct ListPairs {
+    u32 cap;
+    const u32* d_n;
+    const u32* first;
+    const u32* second;
+    const void* cache_src;
+    const u32* src_index;
+    void* cache_dst;
+};
+void box_box_list(fyzz_hip_context* ctx, const ListPairs& L, const float* pos, const float* quat,
+                  const float* half, float cd, float* contacts, u32* counts, u32* flags, u32* scratch);
+// Every box slot t < n against one plane; caches in place, indexed by slot.
+void plane_all(fyzz_hip_context* ctx, u32 n, const float* pos, const float* quat, const float* half,
+               const float plane_position[3], float cd, void* cache, float* contacts, u32* counts, u32* flags);
+}  // namespace pcm
+
+namespace dy {
+// fyzz_hip_dy_step with the manager count read from d_nm (at most nm_cap).
+void step_device(fyzz_hip_dy* d, float dt, u32 nb, fyzz_dy_body* bodies, u32 ns, const fyzz_dy_static* statics,
+                 u32 nm_cap, const u32* d_nm, const fyzz_dy_manager* managers, const fyzz_dy_patch* history,
+                 u32 nh, fyzz_dy_patch* patches, float* impulses, u32* partitions);
+}  // namespace dy
+
+}  // namespace fyzz_hip

@@ -1,0 +1,599 @@
+// Device-resident step pipeline: body records, one broadphase over the
+// dynamic boxes, the plane x box pairs, persistent box/box and plane/box
+// manifolds, the touching-manager list with its friction history, and the
+// Dy step, enqueued on the context's stream with no host readback between
+// stages.
+//
+// Stands in for fyzz::World's step() over the native GPU backend
+// (reference include/fyzzxyzzy/world.h:400-470, src/native/scene.cpp
+// Scene::step 10940-11020) for the admitted class: one static floor plane
+// and dynamic boxes. Per step:
+//   poses -> SoA            (update_shape_bounds input, scene.cpp:1233)
+//   broadphase advance      (update_bounds/update_pairs, 1336-2704)
+//   plane x every box       (update_gpu_pairs, 2396-2418)
+//   box/box + plane/box PCM (update_contacts, 4012-4758)
+//   touching managers       (the descriptor stream; see below)
+//   Dy                      (6940-10707)
+// Manager (solver) order: touching box/box managers in broadphase pair
+// order (lo slot, hi slot), then touching plane managers in slot order.
+// The reference orders descriptors by its island contact lists
+// (scene.cpp:7174-7260), which this pipeline does not rebuild; every stage
+// reproduces the reference for a given order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fyzz_hip.h"
+#include "context.hpp"
+#include "select.hpp"
+#include "stages.hpp"
+
+namespace fyzz_hip::world {
+
+constexpr float kContactOffset = 0.02f;    // per shape
+constexpr float kContactDistance = 0.04f;  // sum of the pair's offsets
+constexpr u32 kNone = 0xffffffffu;
+constexpr u32 kCacheWords = 56;            // 224-byte manifold cache record
+
+struct alignas(16) CacheRec {
+    u32 w[kCacheWords];
+};
+
+__global__ void k_gather(u32 n, const fyzz_dy_body* bodies, float* pos, float* quat) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const fyzz_dy_body& b = bodies[i];
+    pos[3 * i] = b.position[0];
+    pos[3 * i + 1] = b.position[1];
+    pos[3 * i + 2] = b.position[2];
+    quat[4 * i] = b.orientation[0];
+    quat[4 * i + 1] = b.orientation[1];
+    quat[4 * i + 2] = b.orientation[2];
+    quat[4 * i + 3] = b.orientation[3];
+}
+
+__global__ void k_fill(u32* p, u32 n, u32 v) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__device__ inline u32 lower_bound64(const u64* a, u32 n, u64 x) {
+    u32 lo = 0, hi = n;
+    while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Gu order of a box/box pair (first = higher shape id, normals in the
+// second's frame), the pair's previous-list index (its cache and friction
+// history) and the broadphase pair count per box.
+__global__ void k_pairs(u32 cap, const u64* cur, const u32* d_cur, const u64* prev, const u32* d_prev, u32* first,
+                        u32* second, u32* src, u32* counted) {
+    const u32 n = min(*d_cur, cap), np = min(*d_prev, cap);
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const u64 x = cur[t];
+        const u32 lo = static_cast<u32>(x >> 32), hi = static_cast<u32>(x & 0xffffffffu);
+        first[t] = hi;
+        second[t] = lo;
+        const u32 j = lower_bound64(prev, np, x);
+        src[t] = (j < np && prev[j] == x) ? j : kNone;
+        atomicAdd(&counted[lo], 1u);
+        atomicAdd(&counted[hi], 1u);
+    }
+}
+
+__global__ void k_counted(u32 n, fyzz_dy_body* bodies, const u32* counted, u32 planes) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bodies[i].counted_interactions = counted[i] + planes;
+}
+
+__global__ void k_total(const u32* d_nbb, u32 cap, u32 extra, u32* d_total) {
+    *d_total = min(*d_nbb, cap) + extra;
+}
+
+struct MgrPred {
+    const u32* d_nbb;
+    u32 cap;
+    const u32* counts_bb;
+    const u32* counts_pl;
+    __device__ bool operator()(u32 i) const {
+        const u32 nbb = min(*d_nbb, cap);
+        return i < nbb ? counts_bb[i] != 0u : counts_pl[i - nbb] != 0u;
+    }
+};
+
+struct MgrWrite {
+    const u32* d_nbb;
+    u32 cap;
+    const u32* first;
+    const u32* second;
+    const u32* src;
+    const u32* counts_bb;
+    const float* contacts_bb;
+    const u32* counts_pl;
+    const float* contacts_pl;
+    const u32* stamp_prev;
+    u32 want_stamp;  // the previous step's stamp (kNone on the first step)
+    fyzz_dy_manager* managers;
+    u32* mgr_slot;
+    __device__ void operator()(u32 i, u32 o) const {
+        const u32 nbb = min(*d_nbb, cap);
+        fyzz_dy_manager M;
+        const float* c;
+        u32 slot, hist_slot;
+        if (i < nbb) {
+            M.body0 = static_cast<int>(first[i]);
+            M.body1 = static_cast<int>(second[i]);
+            M.count = counts_bb[i];
+            c = contacts_bb + 28ull * i;
+            slot = i;
+            hist_slot = src[i];
+        } else {
+            const u32 b = i - nbb;
+            M.body0 = static_cast<int>(b);
+            M.body1 = -1;
+            M.count = counts_pl[b];
+            c = contacts_pl + 28ull * b;
+            slot = cap + b;
+            hist_slot = slot;
+        }
+        M.history = (hist_slot != kNone && want_stamp != kNone && stamp_prev[hist_slot] == want_stamp) ? hist_slot
+                                                                                                       : kNone;
+        for (u32 k = 0; k < 4; ++k) {
+            fyzz_dy_contact& d = M.contacts[k];
+            const bool live = k < M.count;
+            d.position[0] = live ? c[7 * k] : 0.f;
+            d.position[1] = live ? c[7 * k + 1] : 0.f;
+            d.position[2] = live ? c[7 * k + 2] : 0.f;
+            d.normal[0] = live ? c[7 * k + 3] : 0.f;
+            d.normal[1] = live ? c[7 * k + 4] : 0.f;
+            d.normal[2] = live ? c[7 * k + 5] : 0.f;
+            d.separation = live ? c[7 * k + 6] : 0.f;
+        }
+        managers[o] = M;
+        mgr_slot[o] = slot;
+    }
+};
+
+// friction patches solved this step, keyed for the next step's lookup
+__global__ void k_scatter_patches(const u32* d_nm, u32 cap_m, const fyzz_dy_patch* patches, const u32* mgr_slot,
+                                  fyzz_dy_patch* hist_next, u32* stamp_next, u32 stamp) {
+    const u32 n = min(*d_nm, cap_m);
+    for (u32 m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+        const u32 s = mgr_slot[m];
+        hist_next[s] = patches[m];
+        stamp_next[s] = stamp;
+    }
+}
+
+// Scene::add_dynamic_box mass properties (scene.cpp:408-475): mass in
+// double, the unit-density inertia tensor scaled by density in float.
+void box_mass(const float h[3], float density, float& inv_mass, float inv_inertia[3]) {
+    const double mass = 8.0 * double(h[0]) * double(h[1]) * double(h[2]) * double(density);
+    inv_mass = static_cast<float>(1.0 / mass);
+    const float unit_mass = (h[0] * h[1] * h[2]) * 8.0f;
+    const float ex = h[0] * h[0], ey = h[1] * h[1], ez = h[2] * h[2];
+    const float unit_scale = (unit_mass * 1.0f) / 3.0f;
+    const float d = density;
+    const float ix = ((ey + ez) * unit_scale) * d, iy = ((ex + ez) * unit_scale) * d,
+                iz = ((ex + ey) * unit_scale) * d;
+    inv_inertia[0] = 1.0f / ix;
+    inv_inertia[1] = 1.0f / iy;
+    inv_inertia[2] = 1.0f / iz;
+}
+
+}  // namespace fyzz_hip::world
+
+using namespace fyzz_hip;
+
+struct fyzz_hip_world {
+    fyzz_hip_context* ctx = nullptr;
+    u32 max_bodies = 0;
+    // authoring (host) until the first step
+    std::vector<fyzz_dy_body> host_bodies;
+    std::vector<float> host_half;
+    std::vector<int> slot_of_id;  // body id -> box slot, -1 for the floor
+    bool has_floor = false;
+    u32 floor_id = 0;
+    fyzz_dy_static floor{};
+    bool uploaded = false;
+    u32 n = 0;
+    u32 pair_cap = 0, mgr_cap = 0;
+    u32 step_index = 0;
+    int ccur = 0, hcur = 0;
+    fyzz_hip_sap* sap = nullptr;
+    fyzz_hip_dy* dy = nullptr;
+    DevBuf<fyzz_dy_body> bodies;
+    DevBuf<fyzz_dy_static> statics;
+    DevBuf<float> pos, quat, half, offset, contacts_bb, contacts_pl, impulses;
+    DevBuf<u32> first, second, src, counted, counts_bb, flags_bb, counts_pl, flags_pl, pcm_scratch, tiles, d_total,
+        d_nm, mgr_slot, parts, stamp[2];
+    DevBuf<world::CacheRec> cache_bb[2], cache_pl;
+    DevBuf<fyzz_dy_manager> managers;
+    DevBuf<fyzz_dy_patch> patches, hist[2];
+
+    ~fyzz_hip_world() {
+        if (dy) fyzz_hip_dy_destroy(dy);
+        if (sap) fyzz_hip_sap_destroy(sap);
+    }
+    fyzz_dy_body& body(u32 id) {
+        FYZZ_REQUIRE(id < slot_of_id.size() && slot_of_id[id] >= 0, kInvalidArgument, "world: not a dynamic body id");
+        return host_bodies[static_cast<u32>(slot_of_id[id])];
+    }
+    void require_authoring() const {
+        FYZZ_REQUIRE(!uploaded, kLogicError, "world: authoring calls must precede the first step");
+    }
+};
+
+namespace {
+
+fyzz_dy_body default_body() {
+    fyzz_dy_body b{};
+    b.orientation[3] = 1.0f;
+    b.wake_counter = 0.4f;
+    b.flags = 4u;
+    b.angular_damping = 0.05f;
+    b.max_linear_velocity = 1.0e16f;
+    b.max_angular_velocity = 100.0f;
+    b.static_friction = 0.5f;
+    b.dynamic_friction = 0.5f;
+    b.restitution = 0.2f;
+    b.iterations = 4u | (1u << 8);
+    return b;
+}
+
+void upload(fyzz_hip_world* w) {
+    const u32 n = static_cast<u32>(w->host_bodies.size());
+    w->n = n;
+    w->pair_cap = std::max<u32>(8 * n, 4096);
+    w->mgr_cap = w->pair_cap + n;
+    const std::size_t nn = std::max<u32>(n, 1);
+    hipStream_t st = w->ctx->stream;
+    w->bodies.reserve(nn);
+    w->statics.reserve(1);
+    w->pos.reserve(3 * nn);
+    w->quat.reserve(4 * nn);
+    w->half.reserve(3 * nn);
+    w->offset.reserve(nn);
+    w->counted.reserve(nn);
+    w->counts_pl.reserve(nn);
+    w->flags_pl.reserve(nn);
+    w->contacts_pl.reserve(28 * nn);
+    w->cache_pl.reserve(nn);
+    for (auto* b : {&w->first, &w->second, &w->src, &w->counts_bb, &w->flags_bb}) b->reserve(w->pair_cap);
+    w->pcm_scratch.reserve(w->pair_cap + 1);
+    w->contacts_bb.reserve(28ull * w->pair_cap);
+    w->cache_bb[0].reserve(w->pair_cap);
+    w->cache_bb[1].reserve(w->pair_cap);
+    w->tiles.reserve(sel::tiles_for(w->mgr_cap) + 1);
+    w->d_total.reserve(1);
+    w->d_nm.reserve(1);
+    w->mgr_slot.reserve(w->mgr_cap);
+    w->parts.reserve(w->mgr_cap);
+    w->impulses.reserve(4ull * w->mgr_cap);
+    w->managers.reserve(w->mgr_cap);
+    w->patches.reserve(w->mgr_cap);
+    for (int k = 0; k < 2; ++k) {
+        w->hist[k].reserve(w->mgr_cap);
+        w->stamp[k].reserve(w->mgr_cap);
+        FYZZ_HIP_CHECK(hipMemsetAsync(w->stamp[k].p, 0xff, sizeof(u32) * w->mgr_cap, st));
+    }
+    FYZZ_HIP_CHECK(hipMemsetAsync(w->cache_pl.p, 0, sizeof(world::CacheRec) * nn, st));
+    if (n) {
+        FYZZ_HIP_CHECK(hipMemcpyAsync(w->bodies.p, w->host_bodies.data(), sizeof(fyzz_dy_body) * n,
+                                      hipMemcpyHostToDevice, st));
+        FYZZ_HIP_CHECK(hipMemcpyAsync(w->half.p, w->host_half.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice,
+                                      st));
+        std::vector<float> off(n, world::kContactOffset);
+        FYZZ_HIP_CHECK(hipMemcpyAsync(w->offset.p, off.data(), sizeof(float) * n, hipMemcpyHostToDevice, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    FYZZ_HIP_CHECK(hipMemcpyAsync(w->statics.p, &w->floor, sizeof(fyzz_dy_static), hipMemcpyHostToDevice, st));
+    std::vector<u32> ids(n);
+    std::vector<uint8_t> dyn(n, 1);
+    for (u32 i = 0; i < n; ++i) ids[i] = i;
+    int rc = fyzz_hip_sap_create(w->ctx, n, ids.data(), dyn.data(), w->pair_cap, w->pair_cap, &w->sap);
+    if (rc) throw Error(rc, w->ctx->last_error);
+    rc = fyzz_hip_dy_create(w->ctx, n, w->mgr_cap, &w->dy);
+    if (rc) throw Error(rc, w->ctx->last_error);
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    w->uploaded = true;
+}
+
+void step(fyzz_hip_world* w, float dt) {
+    using namespace fyzz_hip::world;
+    if (!w->uploaded) upload(w);
+    const u32 n = w->n;
+    hipStream_t st = w->ctx->stream;
+    ++w->step_index;
+    if (!n) return;
+    const u32 cap = w->pair_cap;
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, st, n, w->bodies.p, w->pos.p, w->quat.p);
+    int rc = fyzz_hip_sap_advance(w->sap, w->pos.p, w->quat.p, w->half.p, w->offset.p, n, nullptr, 0);
+    if (rc) throw Error(rc, w->ctx->last_error);
+    const sap::Lists L = sap::lists(w->sap);
+    hipLaunchKernelGGL(k_fill, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, w->counted.p, n, 0u);
+    hipLaunchKernelGGL(k_pairs, dim3(std::min<u32>(grid_for(cap, 256), 4096u)), dim3(256), 0, st, cap, L.cur, L.d_cur,
+                       L.prev, L.d_prev, w->first.p, w->second.p, w->src.p, w->counted.p);
+    const u32 planes = w->has_floor ? 1u : 0u;
+    hipLaunchKernelGGL(k_counted, dim3(grid_for(n, 256)), dim3(256), 0, st, n, w->bodies.p, w->counted.p, planes);
+    const int c = w->ccur, c2 = c ^ 1;
+    pcm::ListPairs lp{cap, L.d_cur, w->first.p, w->second.p, w->cache_bb[c].p, w->src.p, w->cache_bb[c2].p};
+    pcm::box_box_list(w->ctx, lp, w->pos.p, w->quat.p, w->half.p, kContactDistance, w->contacts_bb.p,
+                      w->counts_bb.p, w->flags_bb.p, w->pcm_scratch.p);
+    w->ccur = c2;
+    if (w->has_floor) {
+        pcm::plane_all(w->ctx, n, w->pos.p, w->quat.p, w->half.p, w->floor.position, kContactDistance, w->cache_pl.p,
+                       w->contacts_pl.p, w->counts_pl.p, w->flags_pl.p);
+    }
+    // touching managers, in order
+    const u32 extra = w->has_floor ? n : 0u;
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, st, L.d_cur, cap, extra, w->d_total.p);
+    const int h = w->hcur;
+    const u32 want = w->step_index > 1 ? w->step_index - 1 : kNone;
+    {
+        ProfScope ps(&w->ctx->prof, "world.managers", st);
+        MgrPred pred{L.d_cur, cap, w->counts_bb.p, w->counts_pl.p};
+        MgrWrite wr{L.d_cur,       cap,          w->first.p,         w->second.p,   w->src.p,
+                    w->counts_bb.p, w->contacts_bb.p, w->counts_pl.p, w->contacts_pl.p, w->stamp[h].p,
+                    want,          w->managers.p, w->mgr_slot.p};
+        sel::select(st, w->mgr_cap, w->d_total.p, pred, wr, w->tiles.p, w->d_nm.p);
+    }
+    dy::step_device(w->dy, dt, n, w->bodies.p, 1, w->statics.p, w->mgr_cap, w->d_nm.p, w->managers.p, w->hist[h].p,
+                    w->mgr_cap, w->patches.p, w->impulses.p, w->parts.p);
+    hipLaunchKernelGGL(k_scatter_patches, dim3(std::min<u32>(grid_for(w->mgr_cap, 256), 2048u)), dim3(256), 0, st,
+                       w->d_nm.p, w->mgr_cap, w->patches.p, w->mgr_slot.p, w->hist[h ^ 1].p, w->stamp[h ^ 1].p,
+                       w->step_index);
+    FYZZ_HIP_CHECK(hipGetLastError());
+    w->hcur = h ^ 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fyzz_hip_world_create(fyzz_hip_context* ctx, uint32_t max_bodies, fyzz_hip_world** out) {
+    if (!out) return kInvalidArgument;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        FYZZ_REQUIRE(max_bodies > 0 && max_bodies <= (1u << 26), kInvalidArgument, "world_create: bad max_bodies");
+        auto w = std::make_unique<fyzz_hip_world>();
+        w->ctx = ctx;
+        w->max_bodies = max_bodies;
+        w->floor.orientation[3] = 1.0f;
+        w->floor.static_friction = 0.5f;
+        w->floor.dynamic_friction = 0.5f;
+        w->floor.restitution = 0.2f;
+        *out = w.release();
+    });
+}
+
+int fyzz_hip_world_destroy(fyzz_hip_world* w) {
+    if (!w) return kInvalidArgument;
+    (void)hipSetDevice(w->ctx->device);
+    (void)hipStreamSynchronize(w->ctx->stream);
+    delete w;
+    return kOk;
+}
+
+int fyzz_hip_world_add_static_floor(fyzz_hip_world* w, float height, uint32_t* body_id) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(!w->has_floor, kLogicError, "world: one floor plane per world");
+        FYZZ_REQUIRE(std::isfinite(height), kInvalidArgument, "world: floor height must be finite");
+        w->has_floor = true;
+        w->floor_id = static_cast<u32>(w->slot_of_id.size());
+        w->slot_of_id.push_back(-1);
+        w->floor.position[1] = height;
+        if (body_id) *body_id = w->floor_id;
+    });
+}
+
+int fyzz_hip_world_add_dynamic_box(fyzz_hip_world* w, const float* half_extents, float density,
+                                   uint32_t* body_id) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(half_extents, kInvalidArgument, "world: null half extents");
+        FYZZ_REQUIRE(w->host_bodies.size() < w->max_bodies, kOverflow, "world: max_bodies reached");
+        for (int k = 0; k < 3; ++k)
+            FYZZ_REQUIRE(std::isfinite(half_extents[k]) && half_extents[k] > 0.0f, kInvalidArgument,
+                         "world: half extents must be positive and finite");
+        FYZZ_REQUIRE(std::isfinite(density) && density > 0.0f, kInvalidArgument, "world: density must be positive");
+        fyzz_dy_body b = default_body();
+        world::box_mass(half_extents, density, b.inverse_mass, b.inverse_inertia);
+        const u32 id = static_cast<u32>(w->slot_of_id.size());
+        w->slot_of_id.push_back(static_cast<int>(w->host_bodies.size()));
+        w->host_bodies.push_back(b);
+        w->host_half.insert(w->host_half.end(), half_extents, half_extents + 3);
+        if (body_id) *body_id = id;
+    });
+}
+
+int fyzz_hip_world_add_dynamic_boxes(fyzz_hip_world* w, uint32_t n, const float* half_extents, float density,
+                                     const float* positions, const float* orientations, const float* linear,
+                                     const float* angular, float static_friction, float dynamic_friction,
+                                     float restitution, uint32_t* first_id) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(n == 0 || (half_extents && positions && orientations), kInvalidArgument,
+                     "world: null box arrays");
+        FYZZ_REQUIRE(w->host_bodies.size() + n <= w->max_bodies, kOverflow, "world: max_bodies reached");
+        FYZZ_REQUIRE(std::isfinite(density) && density > 0.0f, kInvalidArgument, "world: density must be positive");
+        FYZZ_REQUIRE(static_friction >= 0.0f && dynamic_friction >= 0.0f && restitution >= 0.0f, kInvalidArgument,
+                     "world: material values must be non-negative");
+        if (first_id) *first_id = static_cast<u32>(w->slot_of_id.size());
+        for (u32 i = 0; i < n; ++i) {
+            const float* h = half_extents + 3ull * i;
+            for (int k = 0; k < 3; ++k)
+                FYZZ_REQUIRE(std::isfinite(h[k]) && h[k] > 0.0f, kInvalidArgument,
+                             "world: half extents must be positive and finite");
+            const float* q = orientations + 4ull * i;
+            const float len = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+            FYZZ_REQUIRE(std::isfinite(len) && len > 0.0f, kInvalidArgument, "world: orientation must be non-zero");
+            fyzz_dy_body b = default_body();
+            world::box_mass(h, density, b.inverse_mass, b.inverse_inertia);
+            for (int k = 0; k < 3; ++k) {
+                b.position[k] = positions[3ull * i + k];
+                if (linear) b.linear_velocity[k] = linear[3ull * i + k];
+                if (angular) b.angular_velocity[k] = angular[3ull * i + k];
+            }
+            for (int k = 0; k < 4; ++k) b.orientation[k] = q[k] / len;
+            b.static_friction = static_friction;
+            b.dynamic_friction = dynamic_friction;
+            b.restitution = restitution;
+            w->slot_of_id.push_back(static_cast<int>(w->host_bodies.size()));
+            w->host_bodies.push_back(b);
+            w->host_half.insert(w->host_half.end(), h, h + 3);
+        }
+    });
+}
+
+int fyzz_hip_world_set_body_pose(fyzz_hip_world* w, uint32_t body_id, const float* position,
+                                 const float* orientation) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(position && orientation, kInvalidArgument, "world: null pose");
+        fyzz_dy_body& b = w->body(body_id);
+        const float* q = orientation;
+        const float len = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        FYZZ_REQUIRE(std::isfinite(len) && len > 0.0f, kInvalidArgument, "world: orientation must be non-zero");
+        for (int k = 0; k < 3; ++k) b.position[k] = position[k];
+        for (int k = 0; k < 4; ++k) b.orientation[k] = q[k] / len;
+    });
+}
+
+int fyzz_hip_world_set_body_velocity(fyzz_hip_world* w, uint32_t body_id, const float* linear,
+                                     const float* angular) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        fyzz_dy_body& b = w->body(body_id);
+        for (int k = 0; k < 3; ++k) {
+            if (linear) b.linear_velocity[k] = linear[k];
+            if (angular) b.angular_velocity[k] = angular[k];
+        }
+    });
+}
+
+int fyzz_hip_world_set_body_material(fyzz_hip_world* w, uint32_t body_id, float static_friction,
+                                     float dynamic_friction, float restitution) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(static_friction >= 0.0f && dynamic_friction >= 0.0f && restitution >= 0.0f, kInvalidArgument,
+                     "world: material values must be non-negative");
+        if (w->has_floor && body_id == w->floor_id) {
+            w->floor.static_friction = static_friction;
+            w->floor.dynamic_friction = dynamic_friction;
+            w->floor.restitution = restitution;
+            return;
+        }
+        fyzz_dy_body& b = w->body(body_id);
+        b.static_friction = static_friction;
+        b.dynamic_friction = dynamic_friction;
+        b.restitution = restitution;
+    });
+}
+
+int fyzz_hip_world_step(fyzz_hip_world* w, float dt) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(std::isfinite(dt) && dt > 0.0f, kInvalidArgument, "world: dt must be positive");
+        step(w, dt);
+    });
+}
+
+int fyzz_hip_world_body_records(fyzz_hip_world* w, fyzz_dy_body* out, uint32_t capacity, uint32_t* count) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        const u32 n = static_cast<u32>(w->host_bodies.size());
+        if (count) *count = n;
+        const u32 m = std::min(n, capacity);
+        if (!m) return;
+        FYZZ_REQUIRE(out, kInvalidArgument, "world: null output");
+        if (!w->uploaded) {
+            std::copy(w->host_bodies.begin(), w->host_bodies.begin() + m, out);
+            return;
+        }
+        FYZZ_HIP_CHECK(hipMemcpyAsync(out, w->bodies.p, sizeof(fyzz_dy_body) * m, hipMemcpyDeviceToHost,
+                                      w->ctx->stream));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(w->ctx->stream));
+    });
+}
+
+int fyzz_hip_world_stats(fyzz_hip_world* w, uint32_t* n_pairs, uint32_t* n_managers, uint32_t* n_partitions,
+                         uint32_t* status_flags) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        u32 np = 0, nm = 0, parts = 0, dflags = 0, rounds = 0;
+        if (w->uploaded && w->n) {
+            const sap::Lists L = sap::lists(w->sap);
+            u32 err = 0, ovf = 0;
+            hipStream_t st = w->ctx->stream;
+            FYZZ_HIP_CHECK(hipMemcpyAsync(&np, L.d_cur, 4, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(&err, L.d_error, 4, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(&ovf, L.d_overflow, 4, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(&nm, w->d_nm.p, 4, hipMemcpyDeviceToHost, st));
+            (void)fyzz_hip_dy_status(w->dy, &dflags, &parts, &rounds);
+            dflags |= (err ? 0x100u : 0u) | (ovf ? 0x200u : 0u);
+        }
+        if (n_pairs) *n_pairs = np;
+        if (n_managers) *n_managers = nm;
+        if (n_partitions) *n_partitions = parts;
+        if (status_flags) *status_flags = dflags;
+    });
+}
+
+int fyzz_hip_world_fetch_managers(fyzz_hip_world* w, fyzz_dy_manager* managers, float* impulses,
+                                  uint32_t* partitions, uint32_t capacity, uint32_t* count) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        u32 nm = 0;
+        hipStream_t st = w->ctx->stream;
+        if (w->uploaded && w->n) {
+            FYZZ_HIP_CHECK(hipMemcpyAsync(&nm, w->d_nm.p, 4, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        if (count) *count = nm;
+        const u32 m = std::min(nm, capacity);
+        if (!m) return;
+        if (managers)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(managers, w->managers.p, sizeof(fyzz_dy_manager) * m, hipMemcpyDeviceToHost,
+                                          st));
+        if (impulses)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(impulses, w->impulses.p, sizeof(float) * 4 * m, hipMemcpyDeviceToHost, st));
+        if (partitions)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(partitions, w->parts.p, sizeof(u32) * m, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+// Device pointers of the step's manager list and Dy outputs (valid until
+// the next step): managers[*n_managers], impulses[.][4], partitions[.],
+// and the body records.
+int fyzz_hip_world_device_views(fyzz_hip_world* w, const fyzz_dy_manager** managers, const uint32_t** d_n_managers,
+                                const float** impulses, const uint32_t** partitions, fyzz_dy_body** bodies) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(w->uploaded, kLogicError, "world: no step taken");
+        if (managers) *managers = w->managers.p;
+        if (d_n_managers) *d_n_managers = w->d_nm.p;
+        if (impulses) *impulses = w->impulses.p;
+        if (partitions) *partitions = w->parts.p;
+        if (bodies) *bodies = w->bodies.p;
+    });
+}
+
+}  // extern "C"

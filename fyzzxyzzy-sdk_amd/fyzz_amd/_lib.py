@@ -51,6 +51,21 @@ SIGNATURES = {
     "fyzz_hip_dy_step": (C.c_int, [vp, C.c_float, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_uint32,
                                    vp, vp, vp]),
     "fyzz_hip_dy_status": (C.c_int, [vp, u32p, u32p, u32p]),
+    "fyzz_hip_world_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
+    "fyzz_hip_world_destroy": (C.c_int, [vp]),
+    "fyzz_hip_world_add_static_floor": (C.c_int, [vp, C.c_float, u32p]),
+    "fyzz_hip_world_add_dynamic_box": (C.c_int, [vp, vp, C.c_float, u32p]),
+    "fyzz_hip_world_add_dynamic_boxes": (C.c_int, [vp, C.c_uint32, vp, C.c_float, vp, vp, vp, vp, C.c_float,
+                                                   C.c_float, C.c_float, u32p]),
+    "fyzz_hip_world_set_body_pose": (C.c_int, [vp, C.c_uint32, vp, vp]),
+    "fyzz_hip_world_set_body_velocity": (C.c_int, [vp, C.c_uint32, vp, vp]),
+    "fyzz_hip_world_set_body_material": (C.c_int, [vp, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+    "fyzz_hip_world_step": (C.c_int, [vp, C.c_float]),
+    "fyzz_hip_world_body_records": (C.c_int, [vp, vp, C.c_uint32, u32p]),
+    "fyzz_hip_world_stats": (C.c_int, [vp, u32p, u32p, u32p, u32p]),
+    "fyzz_hip_world_fetch_managers": (C.c_int, [vp, vp, vp, vp, C.c_uint32, u32p]),
+    "fyzz_hip_world_device_views": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                              C.POINTER(vp)]),
 }
 
 

@@ -1,0 +1,127 @@
+"""World: the whole step pipeline on the device (C-ABI fyzz_hip_world_*).
+
+Mirrors the authoring and stepping surface of the reference's fyzz::World
+(include/fyzzxyzzy/world.h:400-470) for a floor plane and dynamic boxes:
+add_static_floor, add_dynamic_box, set_body_pose, set_body_velocity,
+set_body_angular_velocity, set_body_material, step, body_state.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import Context, FyzzError, lib
+from .dy import DY_BODY, DY_MANAGER
+
+vp = C.c_void_p
+
+
+def _f32(a, n):
+    a = np.ascontiguousarray(a, np.float32).reshape(n)
+    return a, a.ctypes.data_as(vp)
+
+
+class World:
+    def __init__(self, ctx: Context, max_bodies: int):
+        self.ctx = ctx
+        h = vp()
+        ctx.call("fyzz_hip_world_create", C.c_uint32(max_bodies), C.byref(h))
+        self.h = h
+        self.n_dynamic = 0
+        ctx.adopt(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fyzz_hip_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        rc = getattr(lib(), name)(self.h, *args)
+        if rc != 0:
+            msg = lib().fyzz_hip_last_error(self.ctx.h)
+            raise FyzzError(rc, msg.decode() if msg else "")
+
+    def add_static_floor(self, height: float = 0.0) -> int:
+        out = C.c_uint32()
+        self._call("fyzz_hip_world_add_static_floor", C.c_float(height), C.byref(out))
+        return out.value
+
+    def add_dynamic_box(self, half_extents, density: float = 1000.0) -> int:
+        h, hp = _f32(half_extents, 3)
+        out = C.c_uint32()
+        self._call("fyzz_hip_world_add_dynamic_box", hp, C.c_float(density), C.byref(out))
+        self.n_dynamic += 1
+        return out.value
+
+    def add_dynamic_boxes(self, half_extents, positions, orientations, linear=None, angular=None,
+                          density=1000.0, friction=0.5, restitution=0.0) -> int:
+        """Bulk add_dynamic_box + set_body_pose/velocity/material; returns
+        the first id (ids are consecutive)."""
+        n = len(half_extents)
+        h = np.ascontiguousarray(half_extents, np.float32).reshape(n, 3)
+        p = np.ascontiguousarray(positions, np.float32).reshape(n, 3)
+        q = np.ascontiguousarray(orientations, np.float32).reshape(n, 4)
+        lv = None if linear is None else np.ascontiguousarray(linear, np.float32).reshape(n, 3)
+        av = None if angular is None else np.ascontiguousarray(angular, np.float32).reshape(n, 3)
+        ptr = lambda a: a.ctypes.data_as(vp) if a is not None else None  # noqa: E731
+        out = C.c_uint32()
+        self._call("fyzz_hip_world_add_dynamic_boxes", C.c_uint32(n), ptr(h), C.c_float(density), ptr(p), ptr(q),
+                   ptr(lv), ptr(av), C.c_float(friction), C.c_float(friction), C.c_float(restitution),
+                   C.byref(out))
+        self.n_dynamic += n
+        return out.value
+
+    def set_body_pose(self, body, position, orientation):
+        p, pp = _f32(position, 3)
+        q, qp = _f32(orientation, 4)
+        self._call("fyzz_hip_world_set_body_pose", C.c_uint32(body), pp, qp)
+
+    def set_body_velocity(self, body, linear):
+        v, vpp = _f32(linear, 3)
+        self._call("fyzz_hip_world_set_body_velocity", C.c_uint32(body), vpp, None)
+
+    def set_body_angular_velocity(self, body, angular):
+        w, wp = _f32(angular, 3)
+        self._call("fyzz_hip_world_set_body_velocity", C.c_uint32(body), None, wp)
+
+    def set_body_material(self, body, static_friction, dynamic_friction, restitution):
+        self._call("fyzz_hip_world_set_body_material", C.c_uint32(body), C.c_float(static_friction),
+                   C.c_float(dynamic_friction), C.c_float(restitution))
+
+    def step(self, dt: float):
+        self._call("fyzz_hip_world_step", C.c_float(dt))
+
+    def body_records(self):
+        out = np.zeros(max(self.n_dynamic, 1), DY_BODY)
+        cnt = C.c_uint32()
+        self._call("fyzz_hip_world_body_records", out.ctypes.data_as(vp), C.c_uint32(len(out)), C.byref(cnt))
+        return out[:cnt.value]
+
+    def body_states(self):
+        """(n, 13): position, linear velocity, orientation, angular velocity."""
+        b = self.body_records()
+        return np.concatenate([b["position"], b["linear_velocity"], b["orientation"], b["angular_velocity"]], 1)
+
+    def stats(self):
+        a, b, c, d = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._call("fyzz_hip_world_stats", C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return {"pairs": a.value, "managers": b.value, "partitions": c.value, "flags": d.value}
+
+    def managers(self):
+        """(managers, impulses[n, 4], partitions[n]) of the last step, host copies."""
+        cnt = C.c_uint32()
+        self._call("fyzz_hip_world_fetch_managers", None, None, None, C.c_uint32(0), C.byref(cnt))
+        k = cnt.value
+        mg = np.zeros(max(k, 1), DY_MANAGER)
+        im = np.zeros((max(k, 1), 4), np.float32)
+        pa = np.zeros(max(k, 1), np.uint32)
+        self._call("fyzz_hip_world_fetch_managers", mg.ctypes.data_as(vp), im.ctypes.data_as(vp),
+                   pa.ctypes.data_as(vp), C.c_uint32(k), C.byref(cnt))
+        return mg[:k], im[:k], pa[:k]

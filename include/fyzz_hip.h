@@ -36,6 +36,7 @@ extern "C" {
 typedef struct fyzz_hip_context fyzz_hip_context;
 typedef struct fyzz_hip_sap fyzz_hip_sap;
 typedef struct fyzz_hip_dy fyzz_hip_dy;
+typedef struct fyzz_hip_world fyzz_hip_world;
 
 /* ---- Dy records (plain data, shared by the device entries and hosts) --
  * fyzz_dy_body: one dynamic rigid body (Scene::Actor's Dy fields,
@@ -267,6 +268,62 @@ int fyzz_hip_dy_step(fyzz_hip_dy* dy, float dt, uint32_t n_bodies, fyzz_dy_body*
  * dependency rounds the partitioner ran. Returns the status matching the
  * first raised flag (the exception the reference would throw). */
 int fyzz_hip_dy_status(fyzz_hip_dy* dy, uint32_t* flags, uint32_t* n_partitions, uint32_t* rounds);
+
+/* ---- Step pipeline (fyzz::World over the native GPU backend) --------- */
+/* A world of one static floor plane and dynamic boxes whose whole step
+ * (pose gather, broadphase advance, plane x box pairs, box/box and
+ * plane/box PCM with persistent manifolds, touching-manager list with
+ * friction history, Dy) is enqueued on the context's stream with no host
+ * readback between stages. Stands in for World + the native GPU backend
+ * (include/fyzzxyzzy/world.h:400-470; create_native_gpu_world,
+ * src/native/native_gpu_world.h:84-138) for this scene class. Body ids
+ * follow creation order as in the reference (the floor takes an id).
+ * Authoring calls precede the first step. Manager (solver) order: touching
+ * box/box managers in broadphase pair order, then touching plane managers
+ * in creation order (the reference's island order is not rebuilt). */
+int fyzz_hip_world_create(fyzz_hip_context* ctx, uint32_t max_bodies, fyzz_hip_world** out);
+int fyzz_hip_world_destroy(fyzz_hip_world* world);
+/* World::add_static_floor (world.h:414) */
+int fyzz_hip_world_add_static_floor(fyzz_hip_world* world, float height, uint32_t* body_id);
+/* World::add_dynamic_box (world.h:416): HOST half_extents[3], density */
+int fyzz_hip_world_add_dynamic_box(fyzz_hip_world* world, const float* half_extents, float density,
+                                   uint32_t* body_id);
+/* n add_dynamic_box calls followed by set_body_pose, set_body_velocity,
+ * set_body_angular_velocity and set_body_material on each (HOST arrays
+ * half_extents[3n], positions[3n], orientations[4n], linear[3n] and
+ * angular[3n], the last two may be NULL); ids are consecutive from
+ * *first_id. */
+int fyzz_hip_world_add_dynamic_boxes(fyzz_hip_world* world, uint32_t n, const float* half_extents, float density,
+                                     const float* positions, const float* orientations, const float* linear,
+                                     const float* angular, float static_friction, float dynamic_friction,
+                                     float restitution, uint32_t* first_id);
+/* World::set_body_pose / set_body_velocity + set_body_angular_velocity /
+ * set_body_material (world.h:420-440); HOST arrays, NULL leaves a part. */
+int fyzz_hip_world_set_body_pose(fyzz_hip_world* world, uint32_t body_id, const float* position,
+                                 const float* orientation);
+int fyzz_hip_world_set_body_velocity(fyzz_hip_world* world, uint32_t body_id, const float* linear,
+                                     const float* angular);
+int fyzz_hip_world_set_body_material(fyzz_hip_world* world, uint32_t body_id, float static_friction,
+                                     float dynamic_friction, float restitution);
+/* World::step (world.h:459); asynchronous. */
+int fyzz_hip_world_step(fyzz_hip_world* world, float dt);
+/* Synchronizes; copies the dynamic bodies' records (creation order) to
+ * HOST out[capacity]; *count = number of dynamic bodies. */
+int fyzz_hip_world_body_records(fyzz_hip_world* world, fyzz_dy_body* out, uint32_t capacity, uint32_t* count);
+/* Synchronizes; counts of the last step and its status bits (FYZZ_DY_*,
+ * 0x100 non-finite bound, 0x200 pair capacity exceeded). */
+int fyzz_hip_world_stats(fyzz_hip_world* world, uint32_t* n_pairs, uint32_t* n_managers,
+                         uint32_t* n_partitions, uint32_t* status_flags);
+/* Synchronizes; copies the last step's manager list (solver order), its
+ * contact impulses [.][4] and partitions to HOST arrays of `capacity`
+ * entries (any may be NULL); *count = number of managers. */
+int fyzz_hip_world_fetch_managers(fyzz_hip_world* world, fyzz_dy_manager* managers, float* impulses,
+                                  uint32_t* partitions, uint32_t capacity, uint32_t* count);
+/* Device pointers of the last step's manager list, contact impulses
+ * [n][4], partitions and body records; valid until the next step. */
+int fyzz_hip_world_device_views(fyzz_hip_world* world, const fyzz_dy_manager** managers,
+                                const uint32_t** d_n_managers, const float** impulses,
+                                const uint32_t** partitions, fyzz_dy_body** bodies);
 
 #ifdef __cplusplus
 }

@@ -359,3 +359,101 @@ def replay_dy_scene(orc, g, on_step, fp64=False):
         acc_lin = out["sleep_linear_acc"].copy()
         acc_ang = out["sleep_angular_acc"].copy()
         on_step(s, out, extra, mg)
+
+
+class OracleWorld:
+    """The whole step pipeline composed from the oracle's SAP, PCM and Dy,
+    in the device world's manager order (touching box/box managers in pair
+    order (lo slot, hi slot), then touching plane managers by slot); friction
+    history is kept for managers that touched on the previous step, box/box
+    caches while the pair stays in the broadphase list, plane caches per box."""
+
+    def __init__(self, orc, scene):
+        from fyzz_amd.scenes import box_mass_properties
+        self.orc = orc
+        self.pcm = OraclePcm(orc.lib)
+        self.dy = OracleDy(orc.lib)
+        n = scene.n
+        self.n = n
+        self.half = scene.half
+        inv_mass, inv_inertia = box_mass_properties(scene.half, scene.density)
+        b = dy_bodies(n)
+        b["position"] = scene.pos
+        q = scene.quat.astype(np.float32)
+        ln = np.sqrt(q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2] + q[:, 3] * q[:, 3])
+        b["orientation"] = q / ln[:, None]
+        b["linear_velocity"] = scene.vel
+        b["angular_velocity"] = scene.ang
+        b["inverse_mass"] = inv_mass
+        b["inverse_inertia"] = inv_inertia
+        b["static_friction"] = scene.friction
+        b["dynamic_friction"] = scene.friction
+        b["restitution"] = scene.restitution
+        self.bodies = b
+        self.floor = scene.floor
+        st = np.zeros(1, DY_STATIC)
+        st["orientation"][0, 3] = 1.0
+        st["position"][0, 1] = scene.floor_height
+        st["static_friction"] = scene.friction
+        st["dynamic_friction"] = scene.friction
+        st["restitution"] = scene.restitution
+        self.statics = st
+        self.bb_caches = {}
+        self.pl_caches = [self.pcm.new_cache() for _ in range(n)]
+        self.history = {}
+        self.last = None
+
+    def step(self, dt):
+        n, half, b = self.n, self.half, self.bodies
+        pos = np.ascontiguousarray(b["position"])
+        quat = np.ascontiguousarray(b["orientation"])
+        bounds = self.orc.box_bounds(pos, quat, half, np.full(n, 0.02, np.float32))
+        pairs = self.orc.sap_pairs(bounds)
+        order = np.lexsort((pairs[:, 1], pairs[:, 0]))
+        pairs = pairs[order]
+        counted = (np.bincount(pairs.ravel(), minlength=n) + (1 if self.floor else 0)).astype(np.uint32)
+        rec = lambda i: np.concatenate([half[i], pos[i], quat[i]])  # noqa: E731
+        mgrs, keys = [], []
+        caches = {}
+        for lo, hi in pairs:
+            key = ("bb", int(lo), int(hi))
+            cache = self.bb_caches.get(key, self.pcm.new_cache())
+            cnt, pts, _ = self.pcm.box_box(rec(hi), rec(lo), 0.04, cache)
+            caches[key] = cache
+            if cnt:
+                mgrs.append((int(hi), int(lo), cnt, pts))
+                keys.append(key)
+        self.bb_caches = caches
+        if self.floor:
+            fp = np.ascontiguousarray(self.statics["position"][0])
+            for i in range(n):
+                cnt, pts, _ = self.pcm.plane_box(fp, rec(i), 0.04, self.pl_caches[i])
+                if cnt:
+                    pts = pts.copy()
+                    pts[:, 3:6] *= -1.0
+                    mgrs.append((i, -1, cnt, pts))
+                    keys.append(("pl", i))
+        mg = np.zeros(len(mgrs), DY_MANAGER)
+        hist = []
+        for k, (b0, b1, cnt, pts) in enumerate(mgrs):
+            mg[k]["body0"], mg[k]["body1"], mg[k]["count"] = b0, b1, cnt
+            for c in range(cnt):
+                mg[k]["contacts"][c]["position"] = pts[c, 0:3]
+                mg[k]["contacts"][c]["normal"] = pts[c, 3:6]
+                mg[k]["contacts"][c]["separation"] = pts[c, 6]
+            if keys[k] in self.history:
+                mg[k]["history"] = len(hist)
+                hist.append(self.history[keys[k]])
+            else:
+                mg[k]["history"] = 0xFFFFFFFF
+        hist = np.array(hist, DY_PATCH) if hist else np.zeros(0, DY_PATCH)
+        b = b.copy()
+        b["counted_interactions"] = counted
+        out, patches, imp, parts = self.dy.step(dt, b, self.statics, mg, hist)
+        self.history = {k: patches[i] for i, k in enumerate(keys)}
+        self.bodies = out
+        self.last = (pairs, mg, imp, parts)
+        return out
+
+    def body_states(self):
+        return body_state_from(self.bodies)

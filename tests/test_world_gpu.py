@@ -1,0 +1,87 @@
+"""GPU: the device step pipeline (fyzz_hip_world_*) against the oracle's
+composed pipeline (oracle SAP -> PCM -> Dy in the world's manager order).
+
+Each step starts both from the device's body records (synchronized state);
+the oracle keeps its own manifold caches and friction history, which
+therefore see the same inputs as the device's. Bars: broadphase pair
+counts and the manager list (bodies, contact counts) exact; contacts 1e-5
+relative (1e-6 absolute); partitions exact; body state after the step 1e-4
+relative (1e-6 absolute). The quaternion update evaluates sin/cos, where
+the reference's glibc sinf/cosf and the device differ in the last bit for
+a fraction of arguments; the test reports how many steps are bit-identical.
+"""
+import numpy as np
+import pytest
+
+from fyzz_amd import scenes
+from oracle_bind import OracleWorld
+
+pytestmark = pytest.mark.gpu
+
+
+def make_world(ctx, sc):
+    from fyzz_amd.world import World
+    w = World(ctx, sc.n + 1)
+    if sc.floor:
+        fid = w.add_static_floor(sc.floor_height)
+        w.set_body_material(fid, sc.friction, sc.friction, sc.restitution)
+    for i in range(sc.n):
+        bid = w.add_dynamic_box(sc.half[i], sc.density)
+        w.set_body_pose(bid, sc.pos[i], sc.quat[i])
+        w.set_body_velocity(bid, sc.vel[i])
+        w.set_body_angular_velocity(bid, sc.ang[i])
+        w.set_body_material(bid, sc.friction, sc.friction, sc.restitution)
+    return w
+
+
+def small_pile(side=3, seed=scenes.SEED_BASE + 23):
+    n = side ** 3
+    idx = np.arange(n)
+    i, k, j = idx % side, (idx // side) % side, idx // (side * side)
+    half = np.stack([0.3 + 0.25 * scenes.uniform(seed, a, n) for a in range(3)], 1)
+    pos = np.stack([(i - (side - 1) / 2) * 1.15, 0.62 + j * 1.15, (k - (side - 1) / 2) * 1.15], 1)
+    quat = scenes.normalize_quat32(scenes.yaw_quat(scenes.uniform(seed, 3, n) * 2 * np.pi))
+    vel = np.stack([scenes.normal(seed, 6 + a, n) * 0.3 for a in range(3)], 1)
+    return scenes.Scene(half, pos, quat, vel=vel, friction=0.5, restitution=0.0)
+
+
+def run_compare(ctx, orc, sc, steps, dt=1 / 120):
+    w = make_world(ctx, sc)
+    ow = OracleWorld(orc, sc)
+    authored = w.body_records()
+    np.testing.assert_array_equal(authored.view(np.uint8), ow.bodies.view(np.uint8), err_msg="authored records")
+    exact_steps = 0
+    for s in range(steps):
+        ow.bodies = w.body_records().copy()
+        w.step(dt)
+        ow.step(dt)
+        st = w.stats()
+        assert st["flags"] == 0, st
+        pairs, mg_o, imp_o, parts_o = ow.last
+        assert st["pairs"] == len(pairs), f"step {s}"
+        mg, imp, parts = w.managers()
+        assert len(mg) == len(mg_o), f"step {s}: {len(mg)} vs {len(mg_o)} managers"
+        np.testing.assert_array_equal(mg["body0"], mg_o["body0"], err_msg=f"step {s}")
+        np.testing.assert_array_equal(mg["body1"], mg_o["body1"], err_msg=f"step {s}")
+        np.testing.assert_array_equal(mg["count"], mg_o["count"], err_msg=f"step {s}")
+        for f in ("position", "normal", "separation"):
+            np.testing.assert_allclose(mg["contacts"][f], mg_o["contacts"][f], rtol=1e-5, atol=1e-6,
+                                       err_msg=f"step {s} contact {f}")
+        np.testing.assert_array_equal(parts, parts_o, err_msg=f"step {s} partitions")
+        np.testing.assert_allclose(imp, imp_o, rtol=1e-4, atol=1e-5, err_msg=f"step {s} impulses")
+        gs, os_ = w.body_states(), ow.body_states()
+        np.testing.assert_allclose(gs, os_, rtol=1e-4, atol=1e-6, err_msg=f"step {s} bodies")
+        exact_steps += int(np.array_equal(gs, os_))
+    return exact_steps
+
+
+def test_small_pile_world_matches_oracle_pipeline(ctx, orc, torch_cuda):
+    exact = run_compare(ctx, orc, small_pile(), 120)
+    print(f"bit-identical steps: {exact}/120")
+
+
+def test_grid_1000_world_matches_oracle_pipeline(ctx, orc, torch_cuda):
+    """1,000 boxes on the 10x10x10 grid with seeded yaw and velocity over a
+    floor: 240 steps at 1/120 s (the boxes fall, land and stack)."""
+    exact = run_compare(ctx, orc, scenes.grid_scene(), 240)
+    print(f"bit-identical steps: {exact}/240")
