@@ -1,16 +1,25 @@
 """Throughput benchmark of the FyzzXYZZY step pipeline on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sap]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pile|sap]
 
-Workload `sap` (default): broadphase only, 1,000,000 axis-aligned boxes
-(centers uniform in a 1000 m cube, lognormal half extents median 0.5 m,
-sigma 0.5). Each frame a seeded 10% of the boxes move by N(0, 0.05 m) per
-axis; one step = apply that frame's moves + one incremental SAP frame
-(bounds, 3-axis endpoint radix sort, crossing scan, found/lost reports and
-the updated pair list). Frame 0 (the full build) is set-up, not timed.
+Workload `pile` (default): the full step on the 100,000-box pile. Half
+extents U[0.2, 1.0] m per axis, density 1000 kg/m^3, seeded
+non-overlapping centers in a 100 x 100 x 200 m column above the y = 0
+plane, seeded uniform orientations, friction 0.5, restitution 0,
+dt = 1/120 s, 4 position + 1 velocity iterations. One step = the device
+world's whole step (pose gather, SAP advance, plane x box pairs, box/box +
+plane/box PCM with GJK/EPA continuation, touching-manager list, Dy:
+partitions, contact prep, PGS, integration). Steps 0..W-1 are warm-up;
+steps W..W+K-1 are timed (the drop: boxes land and pile up). `value` is
+body-steps/s (bodies x timed steps / time; every body counts every step).
 
-Prints one JSON line. `value` is box-frames per second (body-steps/s of
-the broadphase), inputs resident in HBM before the timed region.
+Workload `sap`: broadphase only, 1,000,000 axis-aligned boxes (centers
+uniform in a 1000 m cube, lognormal half extents median 0.5 m, sigma 0.5).
+Each frame a seeded 10% of the boxes move by N(0, 0.05 m) per axis; one
+step = apply that frame's moves + one incremental SAP frame. Frame 0 (the
+full build) is set-up, not timed.
+
+Prints one JSON line; inputs are resident in HBM before the timed region.
 """
 from __future__ import annotations
 
@@ -39,9 +48,11 @@ SAP_SORT_TRAFFIC_BYTES = 604_000_000
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
-    p.add_argument("--warmup", type=int, default=9)
-    p.add_argument("--workload", default="sap", choices=["sap"])
+    p.add_argument("--steps", type=int, default=None)
+    p.add_argument("--warmup", type=int, default=None)
+    p.add_argument("--workload", default="pile", choices=["pile", "sap"])
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="wall-clock budget of the timed CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-frames", type=int, default=1)
     p.add_argument("--calibrate-copy", action="store_true",
@@ -211,6 +222,153 @@ def bench_sap(args):
     return result, (n, half, pos0, moves)
 
 
+PILE_STAGES = ("sap.bounds", "sap.sort", "sap.sort.scatter", "sap.pairs", "sap.lists", "pcm.box_box",
+               "pcm.plane_box", "world.managers", "dy.bodies", "dy.partition", "dy.prep", "dy.pgs",
+               "dy.integrate")
+
+
+def pile_algorithmic_bytes(stage, tot, n):
+    """Algorithmic HBM bytes of one stage summed over the counted steps
+    (tot: world totals over the same steps). Per-unit figures:
+      dy.pgs     per iteration: 68 B per contact row (64-B row read +
+                 applied impulse write), 60 B per friction row, 96 B per body
+                 (velocity deltas read + written)
+      pcm.box_box 644 B per broadphase pair (2 x 40 B poses, 224-B cache
+                 read + write, 112-B result + count)
+      dy.prep    per manager 128-B record + 96-B history read, per contact
+                 row 64 B and per friction row 64 B written, 96-B patch write
+      dy.partition per manager 128-B record read twice + 12 B of incidence
+                 and links + 8 B partition/order written."""
+    steps = max(tot["steps"], 1)
+    iters_per_step = tot["iterations"] / steps
+    if stage == "dy.pgs":
+        return iters_per_step * (tot["contact_rows"] * 68 + tot["friction_rows"] * 60) + \
+            tot["iterations"] * n * 96
+    if stage == "pcm.box_box":
+        return tot["pairs"] * 644
+    if stage == "dy.prep":
+        return tot["managers"] * (128 + 96 + 96) + (tot["contact_rows"] + tot["friction_rows"]) * 64
+    if stage == "dy.partition":
+        return tot["managers"] * (2 * 128 + 12 + 8)
+    return None
+
+
+def cpu_baseline_pile(records, half, budget_s):
+    """The compiled reference NativeCpu scene (oracle/_ref) loaded with the
+    device world's body states: 1 untimed step, then timed steps until the
+    budget is spent (at least one)."""
+    import ctypes as C
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    ref_path = ROOT / "oracle" / "_ref" / "libfyzzref.so"
+    n = len(records)
+    if not ref_path.exists():
+        return {"value": None, "unit": "body-steps/s", "cores": 0, "kind": "reference",
+                "sample": "unavailable: oracle/_ref not built"}
+    ref = C.CDLL(str(ref_path))
+    ref.ref_scene_create.restype = C.c_void_p
+    s = C.c_void_p(ref.ref_scene_create())
+    fid = C.c_uint32()
+    ref.ref_add_floor(s, C.c_float(0.0), C.byref(fid))
+    ref.ref_set_material(s, fid, C.c_float(0.5), C.c_float(0.5), C.c_float(0.0))
+    ids = np.zeros(n, np.uint32)
+    pos = np.ascontiguousarray(records["position"])
+    quat = np.ascontiguousarray(records["orientation"])
+    vel = np.ascontiguousarray(records["linear_velocity"])
+    ang = np.ascontiguousarray(records["angular_velocity"])
+    if ref.ref_add_boxes(s, C.c_uint32(n), P(np.ascontiguousarray(half, np.float32)), P(pos), P(quat), P(vel),
+                         P(ang), C.c_float(1000.0), C.c_float(0.5), C.c_float(0.5), C.c_float(0.0), P(ids)) != 0:
+        raise RuntimeError("reference rejected the scene")
+    if ref.ref_step(s, C.c_float(1.0 / 120.0)) != 0:
+        raise RuntimeError("reference step failed")
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        if ref.ref_step(s, C.c_float(1.0 / 120.0)) != 0:
+            raise RuntimeError("reference step failed")
+        steps += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or steps >= 20:
+            break
+    ref.ref_scene_destroy(s)
+    return {"value": n * steps / dt, "unit": "body-steps/s", "cores": 2, "kind": "reference",
+            "sample": (f"compiled reference NativeCpu Scene (caller + one solver worker thread) loaded with the "
+                       f"device world's {n} body states at the end of the timed region (floor + boxes, same "
+                       f"materials); 1 untimed step (fresh caches), then {steps} timed steps"),
+            "seconds": round(dt, 3), "steps": steps}
+
+
+def bench_pile(args):
+    import torch
+    from fyzz_amd import Context, scenes
+    from fyzz_amd.world import World
+
+    n = 100_000
+    dt = 1.0 / 120.0
+    sc = scenes.pile_scene(n)
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    ctx = Context(0, stream=stream.cuda_stream)
+    w = World(ctx, n + 1)
+    fid = w.add_static_floor(0.0)
+    w.set_body_material(fid, 0.5, 0.5, 0.0)
+    w.add_dynamic_boxes(sc.half, sc.pos, sc.quat, friction=0.5, restitution=0.0)
+    for _ in range(args.warmup):
+        w.step(dt)
+    torch.cuda.synchronize()
+    w.totals(reset=True)
+    ctx.profile(True)
+    for name in PILE_STAGES:
+        ctx.profile_read(name)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        w.step(dt)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    stages = {}
+    for name in PILE_STAGES:
+        t, c = ctx.profile_read(name)
+        stages[name] = {"ms_total": round(t, 3), "launches": c, "ms_avg": round(t / c, 5) if c else None}
+    ctx.profile(False)
+    tot = w.totals()
+    st = w.stats()
+    if st["flags"]:
+        raise RuntimeError(f"world status flags {st['flags']:#x}")
+    dominant = max((k for k in stages if pile_algorithmic_bytes(k, tot, n) is not None),
+                   key=lambda k: stages[k]["ms_total"])
+    alg = pile_algorithmic_bytes(dominant, tot, n)
+    achieved = alg / (stages[dominant]["ms_total"] * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": n * args.steps / (ms * 1e-3),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "100,000-box pile, full step (SAP, PCM box/box + plane/box with GJK/EPA, Dy "
+                               "partitions + PGS 4+1 + integration), dt 1/120 s",
+                   "boxes": n, "timed_steps": f"{args.warmup}..{args.warmup + args.steps - 1}",
+                   "pairs_at_end": st["pairs"], "managers_at_end": st["managers"],
+                   "partitions_at_end": st["partitions"],
+                   "contact_rows_per_step": tot["contact_rows"] / max(tot["steps"], 1),
+                   "friction_rows_per_step": tot["friction_rows"] / max(tot["steps"], 1)},
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes_per_launch": alg / max(stages[dominant]["launches"], 1)},
+        "stages": stages,
+    }
+    return result, (w, sc)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -220,11 +378,22 @@ def main():
     if os.environ.get("WORLD_SIZE", "1") not in ("1", ""):
         print("bench.py runs on one GPU; WORLD_SIZE>1 is not supported", file=sys.stderr)
         sys.exit(2)
-    result, (n, half, pos0, moves) = bench_sap(args)
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline_sap(n, min(args.cpu_frames, len(moves)), half, pos0, moves)
+    if args.workload == "sap":
+        args.steps = 50 if args.steps is None else args.steps
+        args.warmup = 9 if args.warmup is None else args.warmup
+        result, (n, half, pos0, moves) = bench_sap(args)
+        if not args.no_cpu_baseline:
+            result["cpu_baseline"] = cpu_baseline_sap(n, min(args.cpu_frames, len(moves)), half, pos0, moves)
+        else:
+            result["cpu_baseline"] = None
     else:
-        result["cpu_baseline"] = None
+        args.steps = 600 if args.steps is None else args.steps
+        args.warmup = 10 if args.warmup is None else args.warmup
+        result, (w, sc) = bench_pile(args)
+        if not args.no_cpu_baseline:
+            result["cpu_baseline"] = cpu_baseline_pile(w.body_records(), sc.half, args.cpu_seconds)
+        else:
+            result["cpu_baseline"] = None
     print(json.dumps(result))
 
 

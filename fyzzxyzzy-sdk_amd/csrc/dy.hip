@@ -28,7 +28,6 @@
 // dynamic body and the static world's solver body keeps a zero state, so
 // solving a partition's managers concurrently gives the sequential sweep's
 // values bit for bit.
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,8 +39,6 @@
 #include "stages.hpp"
 #include "vecmath.hpp"
 
-namespace cg = cooperative_groups;
-
 namespace fyzz_hip::dy {
 using namespace fyzz_hip::vm;
 
@@ -50,9 +47,11 @@ constexpr u32 kPending = 0xfffffffeu;
 constexpr u32 kMaxParts = 4096;       // partitions per step
 constexpr u32 kMaxPending = 65536;    // managers left for overflow waves
 constexpr int kBlock = 256;
+constexpr int kCoopBlocks = 64;       // workgroups of the cooperative launches
 
 // device counters
-enum : u32 { kFront0 = 0, kFront1, kFront2, kPend, kNParts, kPosIters, kVelIters, kRounds, kCtrWords = 16 };
+enum : u32 { kFront0 = 0, kFront1, kFront2, kPend, kNParts, kPosIters, kVelIters, kRounds, kContacts, kFrictionRows,
+             kCtrWords = 16 };
 
 // one contact row (SolverContactPoint fields the solve reads), 64 B
 struct alignas(16) CRow {
@@ -97,15 +96,45 @@ struct Ws {
     u32* blocksums;
     u32* ctr;
     u32* flags;
+    u32* bar;  // grid barrier: arrival count, generation
 };
 
 __device__ inline u32 ld(const u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ inline void raise_flag(const Ws& W, u32 f) { atomicOr(W.flags, f); }
 __device__ inline u32 mcount(const Ws& W) { return W.d_nm ? min(ld(W.d_nm), W.nm) : W.nm; }
-__device__ inline void gsync(cg::grid_group& g) {
-    __threadfence();
-    g.sync();
-    __threadfence();
+// Grid-wide barrier of a cooperative launch (every workgroup resident):
+// arrival count and generation word, relaxed agent-scope atomics, with an
+// agent-scope release before arriving and acquire after leaving (so the
+// grid's global-memory writes before the barrier are visible after it).
+// Thread 0 of each workgroup keeps the generation it waits on. The spin is
+// bounded; exceeding it (workgroups not co-resident) raises a flag.
+constexpr u32 kSpinLimit = 1u << 24;
+__device__ inline u32 barrier_start(const Ws& W) {
+    return threadIdx.x == 0 ? __hip_atomic_load(&W.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+}
+__device__ inline void grid_sync(const Ws& W, u32& gen) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const u32 g0 = gen;
+        const u32 arrived = __hip_atomic_fetch_add(&W.bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&W.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&W.bar[1], g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            u32 spins = 0;
+            while (__hip_atomic_load(&W.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                if (++spins > kSpinLimit) {
+                    atomicOr(W.flags, FYZZ_DY_LOGIC);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        gen = g0 + 1u;
+    }
+    __syncthreads();
 }
 __device__ inline bool finite3(V v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
 __device__ inline bool finite4(Q q) { return isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w); }
@@ -134,7 +163,10 @@ __device__ inline bool manager_ok(const Ws& W, const fyzz_dy_manager& M) {
 // step records no memset node)
 __global__ void k_clear(Ws W) {
     if (threadIdx.x < kCtrWords) W.ctr[threadIdx.x] = 0;
-    if (threadIdx.x == 0) *W.flags = 0;
+    if (threadIdx.x == 0) {
+        *W.flags = 0;
+        W.bar[0] = 0;
+    }
 }
 
 // scene.cpp:10383-10507 for default dynamics (no pending velocity mods)
@@ -248,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(Ws W) {
 __device__ inline bool dyn_dyn(const fyzz_dy_manager& M) { return M.body0 >= 0 && M.body1 >= 0; }
 
 // Exclusive scan of deg[0, nb) into off[0, nb] across the grid.
-__device__ void grid_scan_degrees(const Ws& W, cg::grid_group& g) {
+__device__ void grid_scan_degrees(const Ws& W, u32& gen) {
     __shared__ u32 scratch[kBlock / kWave + 1];
     const u32 G = gridDim.x;
     const u32 chunk = (W.nb + G - 1) / G;
@@ -258,7 +290,7 @@ __device__ void grid_scan_degrees(const Ws& W, cg::grid_group& g) {
     u32 total;
     (void)block_exclusive_scan<kBlock>(s, scratch, &total);
     if (threadIdx.x == 0) W.blocksums[blockIdx.x] = total;
-    gsync(g);
+    grid_sync(W, gen);
     if (blockIdx.x == 0) {
         u32 carry = 0;
         for (u32 t0 = 0; t0 < G; t0 += kBlock) {
@@ -270,7 +302,7 @@ __device__ void grid_scan_degrees(const Ws& W, cg::grid_group& g) {
         }
         if (threadIdx.x == 0) W.off[W.nb] = carry;
     }
-    gsync(g);
+    grid_sync(W, gen);
     u32 carry = W.blocksums[G + blockIdx.x];
     for (u32 t0 = lo; t0 < hi; t0 += kBlock) {
         const u32 i = t0 + threadIdx.x;
@@ -284,7 +316,7 @@ __device__ void grid_scan_degrees(const Ws& W, cg::grid_group& g) {
 // First-fit partitions over 32-bit masks (scene.cpp:7352-7445) for one task
 // spanning the manager list; stable scatter by partition (7447-7489).
 __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
-    cg::grid_group g = cg::this_grid();
+    u32 gen = barrier_start(W);
     const u32 tid = blockIdx.x * kBlock + threadIdx.x;
     const u32 nthreads = gridDim.x * kBlock;
     const u32 nm = mcount(W);
@@ -300,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
         W.part_fill[i] = 0;
     }
     if (tid <= kNParts) W.ctr[tid] = 0;  // frontier, pending and partition counters
-    gsync(g);
+    grid_sync(W, gen);
     // P1: incidence degrees (dynamic sides)
     for (u32 m = tid; m < nm; m += nthreads) {
         const fyzz_dy_manager& M = W.mg[m];
@@ -309,9 +341,9 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
         if (M.body0 >= 0) atomicAdd(&W.deg[M.body0], 1u);
         if (M.body1 >= 0) atomicAdd(&W.deg[M.body1], 1u);
     }
-    gsync(g);
-    grid_scan_degrees(W, g);
-    gsync(g);
+    grid_sync(W, gen);
+    grid_scan_degrees(W, gen);
+    grid_sync(W, gen);
     // P3: incidence lists, entry = manager << 1 | side
     for (u32 m = tid; m < nm; m += nthreads) {
         const fyzz_dy_manager& M = W.mg[m];
@@ -319,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
         if (M.body0 >= 0) W.inc[W.off[M.body0] + atomicAdd(&W.fill[M.body0], 1u)] = m << 1;
         if (M.body1 >= 0) W.inc[W.off[M.body1] + atomicAdd(&W.fill[M.body1], 1u)] = (m << 1) | 1u;
     }
-    gsync(g);
+    grid_sync(W, gen);
     // P4: order each body's list by manager; chain dynamic/dynamic entries
     for (u32 b = tid; b < W.nb; b += nthreads) {
         const u32 lo = W.off[b], hi = W.off[b + 1];
@@ -342,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
             prev = e;
         }
     }
-    gsync(g);
+    grid_sync(W, gen);
     // P5: ready managers (no predecessor on either body)
     for (u32 m = tid; m < nm; m += nthreads) {
         if (!dyn_dyn(W.mg[m]) || !manager_ok(W, W.mg[m])) continue;
@@ -350,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
         W.cnt[m] = c;
         if (c == 0) W.front[atomicAdd(&W.ctr[kFront0], 1u)] = m;
     }
-    gsync(g);
+    grid_sync(W, gen);
     // dependency rounds
     u32 r = 0;
     for (;; ++r) {
@@ -387,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
             }
         }
         if (tid == 0) W.ctr[(r + 2) % 3] = 0;
-        gsync(g);
+        grid_sync(W, gen);
     }
     if (tid == 0) W.ctr[kRounds] = r;
     // overflow waves: the unassigned managers, in order, with fresh masks
@@ -433,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
             p = over;
         }
     }
-    gsync(g);
+    grid_sync(W, gen);
     // P6: highest dynamic partition per body
     for (u32 m = tid; m < nm; m += nthreads) {
         const fyzz_dy_manager& M = W.mg[m];
@@ -443,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
         atomicMax(&W.max_dyn[M.body0], pm + 1);
         atomicMax(&W.max_dyn[M.body1], pm + 1);
     }
-    gsync(g);
+    grid_sync(W, gen);
     // P7: dynamic/static managers follow the body's dynamic partitions
     for (u32 b = tid; b < W.nb; b += nthreads) {
         u32 k = W.max_dyn[b];
@@ -452,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
             if (!dyn_dyn(W.mg[m])) W.part[m] = k++;
         }
     }
-    gsync(g);
+    grid_sync(W, gen);
     // P8: partition sizes
     for (u32 m = tid; m < nm; m += nthreads) {
         const u32 pm = W.part[m];
@@ -464,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
         atomicAdd(&W.part_count[pm], 1u);
         atomicMax(&W.ctr[kNParts], pm + 1);
     }
-    gsync(g);
+    grid_sync(W, gen);
     if (blockIdx.x == 0) {
         const u32 np = ld(&W.ctr[kNParts]);
         u32 carry = 0, total;
@@ -477,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
         }
         if (threadIdx.x == 0) W.part_off[np] = carry;
     }
-    gsync(g);
+    grid_sync(W, gen);
     // P10: managers grouped by partition (order inside a partition is free)
     for (u32 m = tid; m < nm; m += nthreads) {
         const u32 pm = W.part[m];
@@ -947,7 +979,7 @@ __device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 t
 // Position then velocity iterations over the partitions in order; motion is
 // snapshotted after the last position iteration (scene.cpp:9136-9146).
 __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
-    cg::grid_group g = cg::this_grid();
+    u32 gen = barrier_start(W);
     const u32 tid = blockIdx.x * kBlock + threadIdx.x;
     const u32 nthreads = gridDim.x * kBlock;
     const u32 np = ld(&W.ctr[kNParts]);
@@ -958,7 +990,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
         for (u32 p = 0; p < np; ++p) {
             const u32 lo = W.part_off[p], hi = W.part_off[p + 1];
             for (u32 i = lo + tid; i < hi; i += nthreads) solve_manager(W, W.sorted[i], iter, pos_iters, total);
-            gsync(g);
+            grid_sync(W, gen);
         }
         if (iter + 1 == pos_iters) {
             for (u32 i = tid; i < W.nb; i += nthreads)
@@ -966,7 +998,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                     W.mlin[i] = W.lin[i];
                     W.mang[i] = W.ang[i];
                 }
-            gsync(g);
+            grid_sync(W, gen);
         }
     }
 }
@@ -1088,19 +1120,32 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Ws W) {
     B.flags = (B.flags & ~2u) | (ready ? 2u : 0u);
 }
 
-// impulses per manifold contact and the partition of each manager
+// impulses per manifold contact and the partition of each manager; the
+// step's contact and friction row counts
 __global__ __launch_bounds__(kBlock) void k_outputs(Ws W) {
     const u32 m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= mcount(W)) return;
-    if (W.parts_out) W.parts_out[m] = W.part[m];
-    if (!W.impulses) return;
-    float out[4] = {0.f, 0.f, 0.f, 0.f};
-    if (W.part[m] < kMaxParts) {
-        const u32 o = W.order[m];
-        const u32 no = o & 0xffu;
-        for (u32 q = 0; q < no; ++q) out[(o >> (8 + 4 * q)) & 0xfu] = W.crow[4 * m + q].applied;
+    u32 rows = 0, frows = 0;
+    if (m < mcount(W)) {
+        if (W.parts_out) W.parts_out[m] = W.part[m];
+        float out[4] = {0.f, 0.f, 0.f, 0.f};
+        if (W.part[m] < kMaxParts) {
+            const u32 o = W.order[m];
+            const u32 no = o & 0xffu;
+            for (u32 q = 0; q < no; ++q) out[(o >> (8 + 4 * q)) & 0xfu] = W.crow[4 * m + q].applied;
+            rows = no;
+            frows = 2u * W.patches[m].anchor_count;
+        }
+        if (W.impulses)
+            for (u32 q = 0; q < 4; ++q) W.impulses[4 * m + q] = out[q];
     }
-    for (u32 q = 0; q < 4; ++q) W.impulses[4 * m + q] = out[q];
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        rows += __shfl_xor(rows, d, kWave);
+        frows += __shfl_xor(frows, d, kWave);
+    }
+    if (lane_id() == 0 && (rows || frows)) {
+        atomicAdd(&W.ctr[kContacts], rows);
+        atomicAdd(&W.ctr[kFrictionRows], frows);
+    }
 }
 
 }  // namespace fyzz_hip::dy
@@ -1116,7 +1161,7 @@ struct fyzz_hip_dy {
     DevBuf<u32> inc, pred, succ, cnt, part, mask_after, front, pend, sorted, order;
     DevBuf<dy::CRow> crow;
     DevBuf<dy::FRow> frow;
-    DevBuf<u32> part_count, part_off, part_fill, blocksums, ctr, flags;
+    DevBuf<u32> part_count, part_off, part_fill, blocksums, ctr, flags, bar;
     dy::Ws ws{};
     bool prepared = false;
 };
@@ -1128,7 +1173,9 @@ int coop_blocks(const void* kernel, int device) {
     FYZZ_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, dy::kBlock, 0));
     FYZZ_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     FYZZ_REQUIRE(per_cu > 0 && cus > 0, kDeviceError, "dy: cooperative kernel does not fit a compute unit");
-    return std::min(per_cu, 2) * cus;
+    // 64 resident workgroups: a grid barrier costs ~3 us there, against
+    // ~17 us at 512 (the barrier is the Dy kernels' critical path)
+    return std::min(per_cu * cus, dy::kCoopBlocks);
 }
 
 void launch_coop(const void* kernel, int blocks, dy::Ws& ws, hipStream_t st) {
@@ -1197,7 +1244,7 @@ void dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
     ProfScope ps(&d->ctx->prof, "dy.integrate", st);
     if (d->ws.nb)
         hipLaunchKernelGGL(dy::k_integrate, dim3(grid_for(d->ws.nb, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
-    if (d->ws.nm && (impulses || partitions))
+    if (d->ws.nm)
         hipLaunchKernelGGL(dy::k_outputs, dim3(grid_for(d->ws.nm, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
     FYZZ_HIP_CHECK(hipGetLastError());
     d->prepared = false;
@@ -1206,6 +1253,8 @@ void dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
 }  // namespace
 
 namespace fyzz_hip::dy {
+const u32* counters(fyzz_hip_dy* d) { return d->ctr.get(); }
+
 void step_device(fyzz_hip_dy* d, float dt, u32 nb, fyzz_dy_body* bodies, u32 ns, const fyzz_dy_static* statics,
                  u32 nm_cap, const u32* d_nm, const fyzz_dy_manager* managers, const fyzz_dy_patch* history,
                  u32 nh, fyzz_dy_patch* patches, float* impulses, u32* partitions) {
@@ -1240,6 +1289,8 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->part_fill.reserve(dy::kMaxParts);
         d->ctr.reserve(dy::kCtrWords);
         d->flags.reserve(1);
+        d->bar.reserve(4);
+        FYZZ_HIP_CHECK(hipMemset(d->bar.get(), 0, 4 * sizeof(u32)));
         d->coop_blocks_partition = coop_blocks(reinterpret_cast<const void*>(&dy::k_partition), ctx->device);
         d->coop_blocks_pgs = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs), ctx->device);
         d->blocksums.reserve(2 * std::max(d->coop_blocks_partition, d->coop_blocks_pgs) + 2);
@@ -1279,6 +1330,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         w.blocksums = d->blocksums.get();
         w.ctr = d->ctr.get();
         w.flags = d->flags.get();
+        w.bar = d->bar.get();
         *out = d.release();
     });
 }

@@ -80,6 +80,62 @@ __global__ __launch_bounds__(kBlock) void k_write(u32 cap, const u32* d_n, P pre
 
 inline u32 tiles_for(u32 cap) { return (cap + kTile - 1) / kTile; }
 
+// Exclusive scan of value(i) over i < min(*d_n, cap) into out[i]; the
+// total goes to *d_total. Same tiling as select (tile sums, one-block tile
+// scan, per-tile scan and write).
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_scan_sum(u32 cap, const u32* d_n, F value, u32* tile_sums) {
+    __shared__ u32 scratch[kBlock / kWave + 1];
+    const u32 n = count_of(cap, d_n);
+    const u32 need = (n + kTile - 1) / kTile;
+    for (u32 tile = blockIdx.x; tile < need; tile += gridDim.x) {
+        const u32 base = tile * kTile;
+        u32 c = 0;
+        for (int j = 0; j < kItems; ++j) {
+            const u32 i = base + j * kBlock + threadIdx.x;
+            if (i < n) c += value(i);
+        }
+        u32 total;
+        (void)block_exclusive_scan<kBlock>(c, scratch, &total);
+        if (threadIdx.x == 0) tile_sums[tile] = total;
+    }
+}
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_scan_write(u32 cap, const u32* d_n, F value, const u32* tile_sums,
+                                                       u32* out) {
+    __shared__ u32 scratch[kBlock / kWave + 1];
+    const u32 n = count_of(cap, d_n);
+    const u32 need = (n + kTile - 1) / kTile;
+    for (u32 tile = blockIdx.x; tile < need; tile += gridDim.x) {
+        const u32 base = tile * kTile;
+        const u32 first = base + threadIdx.x * kItems;
+        u32 c = 0;
+        for (int j = 0; j < kItems; ++j) {
+            const u32 i = first + j;
+            if (i < n) c += value(i);
+        }
+        u32 total;
+        u32 run = tile_sums[tile] + block_exclusive_scan<kBlock>(c, scratch, &total);
+        for (int j = 0; j < kItems; ++j) {
+            const u32 i = first + j;
+            if (i < n) {
+                out[i] = run;
+                run += value(i);
+            }
+        }
+    }
+}
+template <class F>
+void exclusive_scan(hipStream_t st, u32 cap, const u32* d_n, F value, u32* out, u32* tile_scratch, u32* d_total) {
+    const u32 tiles = tiles_for(cap) ? tiles_for(cap) : 1;
+    const u32 grid = tiles < 1024u ? tiles : 1024u;
+    hipLaunchKernelGGL((k_scan_sum<F>), dim3(grid), dim3(kBlock), 0, st, cap, d_n, value, tile_scratch);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, cap, d_n, tile_scratch, d_total,
+                       static_cast<const u32*>(nullptr));
+    hipLaunchKernelGGL((k_scan_write<F>), dim3(grid), dim3(kBlock), 0, st, cap, d_n, value, tile_scratch, out);
+    FYZZ_HIP_CHECK(hipGetLastError());
+}
+
 // Compacts indices [0, *d_n) (at most cap) where pred holds; write(i, o) is
 // called with consecutive o in index order; *d_out receives the count.
 // tile_scratch needs tiles_for(cap) words. With a gate, the three launches

@@ -20,6 +20,8 @@ struct Lists {
     u32 cap;
     const u32* d_error;     // non-zero: a box bound was not finite
     const u32* d_overflow;  // non-zero: pair or delta capacity exceeded
+    const u64* lost;        // pairs that left the list this frame (sorted)
+    const u32* d_lost;
 };
 Lists lists(fyzz_hip_sap* h);
 }  // namespace sap
@@ -46,6 +48,11 @@ void plane_all(fyzz_hip_context* ctx, u32 n, const float* pos, const float* quat
 }  // namespace pcm
 
 namespace dy {
+// Device counters of the last Dy step: [4] partitions, [5] position and
+// [6] velocity iterations, [7] partition rounds, [8] contact rows, [9]
+// friction rows.
+const u32* counters(fyzz_hip_dy* d);
+enum : u32 { kCtrParts = 4, kCtrPos = 5, kCtrVel = 6, kCtrRounds = 7, kCtrContacts = 8, kCtrFriction = 9 };
 // fyzz_hip_dy_step with the manager count read from d_nm (at most nm_cap).
 void step_device(fyzz_hip_dy* d, float dt, u32 nb, fyzz_dy_body* bodies, u32 ns, const fyzz_dy_static* statics,
                  u32 nm_cap, const u32* d_nm, const fyzz_dy_manager* managers, const fyzz_dy_patch* history,

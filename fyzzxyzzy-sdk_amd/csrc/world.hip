@@ -174,6 +174,222 @@ __global__ void k_scatter_patches(const u32* d_nm, u32 cap_m, const fyzz_dy_patc
     }
 }
 
+// ---- contact events (Scene::publish_contact_events, scene.cpp:9149-9259;
+// touch transitions of update_contacts, 4640-4731)
+
+__device__ inline u64 pair_key(u32 a, u32 b) {
+    return a < b ? (static_cast<u64>(a) << 32) | b : (static_cast<u64>(b) << 32) | a;
+}
+
+// manager index of each touching pair slot
+__global__ void k_slot_mgr(const u32* d_nm, u32 cap_m, const u32* mgr_slot, u32* slot_mgr) {
+    const u32 n = min(*d_nm, cap_m);
+    for (u32 m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) slot_mgr[mgr_slot[m]] = m;
+}
+
+struct EvArgs {
+    u32 cap;                  // pair capacity
+    const u32* d_nbb;         // current pair count
+    const u64* prev;          // previous pair list
+    const u32* d_prev;
+    const u64* lost;          // pairs that left the list (sorted)
+    const u32* d_lost;
+    const u32* src;           // previous index of each current pair
+    const u32* first;         // Gu first slot (higher) of each current pair
+    const u32* second;
+    const u32* counts_bb;
+    const u32* counts_pl;
+    const u32* touch_prev;    // touching flags aligned with the previous list
+    const u32* touch_pl;      // plane touching flags of the previous step
+    const u32* shape;         // shape (= body) id of each box slot
+    u32 floor_shape;
+    const u64* k1;            // compacted stream keys and sources
+    const u32* s1;
+    const u64* k2;
+    const u32* s2;
+    const u64* k3;
+    const u32* s3;
+    const u32* d_counts;      // n1, n2, n3, total
+};
+
+__device__ inline bool prev_touch_bb(const EvArgs& A, u32 i) {
+    return A.src[i] != kNone && A.touch_prev[A.src[i]] != 0u;
+}
+__device__ inline u32 lost_prev_index(const EvArgs& A, u32 j) {
+    const u32 np = min(*A.d_prev, A.cap);
+    return lower_bound64(A.prev, np, A.lost[j]);
+}
+
+struct EvPred1 {
+    EvArgs A;
+    __device__ bool operator()(u32 i) const { return A.counts_bb[i] != 0u || prev_touch_bb(A, i); }
+};
+struct EvWrite1 {
+    EvArgs A;
+    u64* k;
+    u32* s;
+    __device__ void operator()(u32 i, u32 o) const {
+        k[o] = pair_key(A.shape[A.second[i]], A.shape[A.first[i]]);
+        s[o] = i;
+    }
+};
+struct EvPred2 {
+    EvArgs A;
+    __device__ bool operator()(u32 b) const { return A.counts_pl[b] != 0u || A.touch_pl[b] != 0u; }
+};
+struct EvWrite2 {
+    EvArgs A;
+    u64* k;
+    u32* s;
+    __device__ void operator()(u32 b, u32 o) const {
+        k[o] = pair_key(A.floor_shape, A.shape[b]);
+        s[o] = b;
+    }
+};
+struct EvPred3 {
+    EvArgs A;
+    __device__ bool operator()(u32 j) const {
+        const u32 np = min(*A.d_prev, A.cap);
+        const u32 o = lost_prev_index(A, j);
+        return o < np && A.prev[o] == A.lost[j] && A.touch_prev[o] != 0u;
+    }
+};
+struct EvWrite3 {
+    EvArgs A;
+    u64* k;
+    u32* s;
+    __device__ void operator()(u32 j, u32 o) const {
+        const u64 x = A.lost[j];
+        k[o] = pair_key(A.shape[static_cast<u32>(x >> 32)], A.shape[static_cast<u32>(x & 0xffffffffu)]);
+        s[o] = j;
+    }
+};
+
+__global__ void k_ev_total(u32* d_counts) { d_counts[3] = d_counts[0] + d_counts[1] + d_counts[2]; }
+
+// Gu-order position of every stream item (keys are distinct across the
+// three streams), its event header and its point count.
+__global__ void k_ev_merge(EvArgs A, fyzz_contact_event* ev, u32* ev_src, u32* ev_pts) {
+    const u32 n1 = A.d_counts[0], n2 = A.d_counts[1], n3 = A.d_counts[2];
+    const u32 total = n1 + n2 + n3;
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        u32 stream, i, pos, count;
+        u64 key;
+        bool touching, before;
+        if (t < n1) {
+            stream = 0;
+            i = t;
+            key = A.k1[i];
+            pos = i + lower_bound64(A.k2, n2, key) + lower_bound64(A.k3, n3, key);
+            const u32 p = A.s1[i];
+            count = A.counts_bb[p];
+            touching = count != 0u;
+            before = prev_touch_bb(A, p);
+        } else if (t < n1 + n2) {
+            stream = 1;
+            i = t - n1;
+            key = A.k2[i];
+            pos = i + lower_bound64(A.k1, n1, key) + lower_bound64(A.k3, n3, key);
+            const u32 b = A.s2[i];
+            count = A.counts_pl[b];
+            touching = count != 0u;
+            before = A.touch_pl[b] != 0u;
+        } else {
+            stream = 2;
+            i = t - n1 - n2;
+            key = A.k3[i];
+            pos = i + lower_bound64(A.k1, n1, key) + lower_bound64(A.k2, n2, key);
+            count = 0;
+            touching = false;
+            before = true;
+        }
+        const u32 a = static_cast<u32>(key >> 32), b = static_cast<u32>(key & 0xffffffffu);
+        fyzz_contact_event e{};
+        e.body0 = a;  // one shape per body: body ids are the shape ids, a < b
+        e.body1 = b;
+        e.kind = touching ? (before ? 1 : 0) : 2;
+        e.contact_count = static_cast<uint8_t>(touching ? count : 0u);
+        e.shape0 = a;
+        e.shape1 = b;
+        ev[pos] = e;
+        ev_src[pos] = (stream << 30) | i;
+        ev_pts[pos] = touching ? count : 0u;
+    }
+}
+
+struct EvCount {
+    const u32* ev_pts;
+    __device__ u32 operator()(u32 i) const { return ev_pts[i]; }
+};
+
+// points of each event: published contacts, the outward normal flip
+// (gu_order_reversed != body(a) > body(b), scene.cpp:9238-9240), the
+// solver's impulses as double and their sum
+__global__ void k_ev_points(EvArgs A, const u32* ev_src, const u32* ev_off, fyzz_contact_event* ev,
+                            fyzz_contact_event_point* pts, const u32* slot_mgr, const fyzz_dy_manager* managers,
+                            const float* impulses, u32 pt_cap) {
+    const u32 total = A.d_counts[3];
+    for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const u32 off = ev_off[e];
+        ev[e].point_offset = off;
+        const u32 count = ev[e].contact_count;
+        if (!count) {
+            ev[e].normal_impulse = 0.0;
+            continue;
+        }
+        const u32 stream = ev_src[e] >> 30, i = ev_src[e] & 0x3fffffffu;
+        u32 slot;
+        bool flip;
+        if (stream == 0) {
+            slot = A.s1[i];
+            flip = true;  // both dynamic: Gu order puts the higher body first
+        } else {
+            const u32 b = A.s2[i];
+            slot = A.cap + b;
+            flip = A.floor_shape < A.shape[b];  // static first: reversed
+        }
+        const u32 m = slot_mgr[slot];
+        const fyzz_dy_manager& M = managers[m];
+        double sum = 0.0;
+        for (u32 k = 0; k < count; ++k) {
+            fyzz_contact_event_point q{};
+            const fyzz_dy_contact& c = M.contacts[k];
+            for (int d = 0; d < 3; ++d) {
+                q.position[d] = c.position[d];
+                q.normal[d] = flip ? -c.normal[d] : c.normal[d];
+            }
+            q.separation = c.separation;
+            q.normal_impulse = static_cast<double>(impulses[4ull * m + k]);
+            sum += q.normal_impulse;
+            if (off + k < pt_cap) pts[off + k] = q;
+        }
+        ev[e].normal_impulse = sum;
+    }
+}
+
+// this step's touch state for the next step's transitions
+__global__ void k_touch(u32 cap, const u32* d_nbb, const u32* counts_bb, u32* touch_cur, u32 n, const u32* counts_pl,
+                        u32* touch_pl) {
+    const u32 nbb = min(*d_nbb, cap);
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb || i < n; i += gridDim.x * blockDim.x) {
+        if (i < nbb) touch_cur[i] = counts_bb[i] != 0u ? 1u : 0u;
+        if (counts_pl && i < n) touch_pl[i] = counts_pl[i] != 0u ? 1u : 0u;
+    }
+}
+
+// running totals over steps (algorithmic byte accounting of the solve):
+// [0] steps, [1] broadphase pairs, [2] managers, [3] contact rows,
+// [4] friction rows, [5] partitions, [6] position+velocity iterations
+__global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const u32* dy_ctr, u64* totals) {
+    totals[0] += 1;
+    totals[1] += min(*d_pairs, cap);
+    totals[2] += *d_nm;
+    totals[3] += dy_ctr[dy::kCtrContacts];
+    totals[4] += dy_ctr[dy::kCtrFriction];
+    totals[5] += dy_ctr[dy::kCtrParts];
+    totals[6] += dy_ctr[dy::kCtrPos] + max(dy_ctr[dy::kCtrVel], 1u);
+}
+
 // Scene::add_dynamic_box mass properties (scene.cpp:408-475): mass in
 // double, the unit-density inertia tensor scaled by density in float.
 void box_mass(const float h[3], float density, float& inv_mass, float inv_inertia[3]) {
@@ -219,6 +435,14 @@ struct fyzz_hip_world {
     DevBuf<world::CacheRec> cache_bb[2], cache_pl;
     DevBuf<fyzz_dy_manager> managers;
     DevBuf<fyzz_dy_patch> patches, hist[2];
+    DevBuf<u64> totals;
+    // contact events
+    bool notify = false;
+    DevBuf<u32> shape, touch_bb[2], touch_pl, slot_mgr, s1, s2, s3, ev_counts, ev_src, ev_pts, ev_off, d_npts;
+    DevBuf<u64> k1, k2, k3;
+    DevBuf<fyzz_contact_event> events;
+    DevBuf<fyzz_contact_event_point> points;
+    u32 ev_cap = 0, pt_cap = 0;
 
     ~fyzz_hip_world() {
         if (dy) fyzz_hip_dy_destroy(dy);
@@ -277,6 +501,24 @@ void upload(fyzz_hip_world* w) {
     w->d_total.reserve(1);
     w->d_nm.reserve(1);
     w->mgr_slot.reserve(w->mgr_cap);
+    w->shape.reserve(nn);
+    w->touch_pl.reserve(nn);
+    FYZZ_HIP_CHECK(hipMemsetAsync(w->touch_pl.p, 0, sizeof(u32) * nn, st));
+    for (int k = 0; k < 2; ++k) {
+        w->touch_bb[k].reserve(w->pair_cap);
+        FYZZ_HIP_CHECK(hipMemsetAsync(w->touch_bb[k].p, 0, sizeof(u32) * w->pair_cap, st));
+    }
+    {
+        std::vector<u32> shape_of_slot(n);
+        for (std::size_t id = 0; id < w->slot_of_id.size(); ++id)
+            if (w->slot_of_id[id] >= 0) shape_of_slot[static_cast<u32>(w->slot_of_id[id])] = static_cast<u32>(id);
+        if (n)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(w->shape.p, shape_of_slot.data(), sizeof(u32) * n, hipMemcpyHostToDevice,
+                                          st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    w->totals.reserve(8);
+    FYZZ_HIP_CHECK(hipMemsetAsync(w->totals.p, 0, 8 * sizeof(u64), st));
     w->parts.reserve(w->mgr_cap);
     w->impulses.reserve(4ull * w->mgr_cap);
     w->managers.reserve(w->mgr_cap);
@@ -308,6 +550,56 @@ void upload(fyzz_hip_world* w) {
     w->uploaded = true;
 }
 
+void publish_events(fyzz_hip_world* w, const sap::Lists& L, int c_prev, int c_cur) {
+    using namespace fyzz_hip::world;
+    hipStream_t st = w->ctx->stream;
+    const u32 cap = w->pair_cap, n = w->n;
+    if (!w->ev_cap) {
+        w->ev_cap = 2 * cap + n;
+        w->pt_cap = 4 * (cap + n);
+        w->slot_mgr.reserve(w->mgr_cap);
+        w->s1.reserve(cap);
+        w->k1.reserve(cap);
+        w->s2.reserve(std::max<u32>(n, 1));
+        w->k2.reserve(std::max<u32>(n, 1));
+        w->s3.reserve(cap);
+        w->k3.reserve(cap);
+        w->ev_counts.reserve(4);
+        w->ev_src.reserve(w->ev_cap);
+        w->ev_pts.reserve(w->ev_cap);
+        w->ev_off.reserve(w->ev_cap);
+        w->d_npts.reserve(1);
+        w->events.reserve(w->ev_cap);
+        w->points.reserve(w->pt_cap);
+        w->tiles.reserve(sel::tiles_for(std::max(w->mgr_cap, w->ev_cap)) + 1);
+    }
+    ProfScope ps(&w->ctx->prof, "world.events", st);
+    EvArgs A{cap,          L.d_cur,         L.prev,         L.d_prev,        L.lost,       L.d_lost,
+             w->src.p,     w->first.p,      w->second.p,    w->counts_bb.p,  w->counts_pl.p,
+             w->touch_bb[c_prev].p, w->touch_pl.p, w->shape.p, w->floor_id,
+             w->k1.p,      w->s1.p,         w->k2.p,        w->s2.p,         w->k3.p,      w->s3.p,
+             w->ev_counts.p};
+    hipLaunchKernelGGL(k_slot_mgr, dim3(std::min<u32>(grid_for(w->mgr_cap, 256), 2048u)), dim3(256), 0, st,
+                       w->d_nm.p, w->mgr_cap, w->mgr_slot.p, w->slot_mgr.p);
+    sel::select(st, cap, L.d_cur, EvPred1{A}, EvWrite1{A, w->k1.p, w->s1.p}, w->tiles.p, w->ev_counts.p);
+    if (w->has_floor) {
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->d_npts.p, 1u, n);
+        sel::select(st, n, w->d_npts.p, EvPred2{A}, EvWrite2{A, w->k2.p, w->s2.p}, w->tiles.p, w->ev_counts.p + 1);
+    } else {
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->ev_counts.p + 1, 1u, 0u);
+    }
+    sel::select(st, cap, L.d_lost, EvPred3{A}, EvWrite3{A, w->k3.p, w->s3.p}, w->tiles.p, w->ev_counts.p + 2);
+    hipLaunchKernelGGL(k_ev_total, dim3(1), dim3(1), 0, st, w->ev_counts.p);
+    const u32 grid = std::min<u32>(grid_for(w->ev_cap, 256), 2048u);
+    hipLaunchKernelGGL(k_ev_merge, dim3(grid), dim3(256), 0, st, A, w->events.p, w->ev_src.p, w->ev_pts.p);
+    sel::exclusive_scan(st, w->ev_cap, w->ev_counts.p + 3, EvCount{w->ev_pts.p}, w->ev_off.p, w->tiles.p,
+                        w->d_npts.p);
+    hipLaunchKernelGGL(k_ev_points, dim3(grid), dim3(256), 0, st, A, w->ev_src.p, w->ev_off.p, w->events.p,
+                       w->points.p, w->slot_mgr.p, w->managers.p, w->impulses.p, w->pt_cap);
+    FYZZ_HIP_CHECK(hipGetLastError());
+    (void)c_cur;
+}
+
 void step(fyzz_hip_world* w, float dt) {
     using namespace fyzz_hip::world;
     if (!w->uploaded) upload(w);
@@ -325,7 +617,7 @@ void step(fyzz_hip_world* w, float dt) {
                        L.prev, L.d_prev, w->first.p, w->second.p, w->src.p, w->counted.p);
     const u32 planes = w->has_floor ? 1u : 0u;
     hipLaunchKernelGGL(k_counted, dim3(grid_for(n, 256)), dim3(256), 0, st, n, w->bodies.p, w->counted.p, planes);
-    const int c = w->ccur, c2 = c ^ 1;
+    const int c = w->ccur, c2 = c ^ 1;  // previous / current pair-aligned buffers
     pcm::ListPairs lp{cap, L.d_cur, w->first.p, w->second.p, w->cache_bb[c].p, w->src.p, w->cache_bb[c2].p};
     pcm::box_box_list(w->ctx, lp, w->pos.p, w->quat.p, w->half.p, kContactDistance, w->contacts_bb.p,
                       w->counts_bb.p, w->flags_bb.p, w->pcm_scratch.p);
@@ -352,6 +644,12 @@ void step(fyzz_hip_world* w, float dt) {
     hipLaunchKernelGGL(k_scatter_patches, dim3(std::min<u32>(grid_for(w->mgr_cap, 256), 2048u)), dim3(256), 0, st,
                        w->d_nm.p, w->mgr_cap, w->patches.p, w->mgr_slot.p, w->hist[h ^ 1].p, w->stamp[h ^ 1].p,
                        w->step_index);
+    hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(1), 0, st, L.d_cur, cap, w->d_nm.p, dy::counters(w->dy),
+                       w->totals.p);
+    if (w->notify) publish_events(w, L, c, c2);
+    hipLaunchKernelGGL(k_touch, dim3(std::min<u32>(grid_for(std::max(cap, n), 256), 2048u)), dim3(256), 0, st, cap,
+                       L.d_cur, w->counts_bb.p, w->touch_bb[c2].p, n, w->has_floor ? w->counts_pl.p : nullptr,
+                       w->touch_pl.p);
     FYZZ_HIP_CHECK(hipGetLastError());
     w->hcur = h ^ 1;
 }
@@ -576,6 +874,50 @@ int fyzz_hip_world_fetch_managers(fyzz_hip_world* w, fyzz_dy_manager* managers, 
             FYZZ_HIP_CHECK(hipMemcpyAsync(impulses, w->impulses.p, sizeof(float) * 4 * m, hipMemcpyDeviceToHost, st));
         if (partitions)
             FYZZ_HIP_CHECK(hipMemcpyAsync(partitions, w->parts.p, sizeof(u32) * m, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int fyzz_hip_world_totals(fyzz_hip_world* w, uint64_t* out, int reset) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(out, kInvalidArgument, "world: null output");
+        for (int k = 0; k < 8; ++k) out[k] = 0;
+        if (!w->uploaded) return;
+        hipStream_t st = w->ctx->stream;
+        FYZZ_HIP_CHECK(hipMemcpyAsync(out, w->totals.p, 8 * sizeof(u64), hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        if (reset) FYZZ_HIP_CHECK(hipMemsetAsync(w->totals.p, 0, 8 * sizeof(u64), st));
+    });
+}
+
+int fyzz_hip_world_set_contact_notifications(fyzz_hip_world* w, int enabled) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] { w->notify = enabled != 0; });
+}
+
+int fyzz_hip_world_contact_events(fyzz_hip_world* w, fyzz_contact_event* events, uint32_t capacity,
+                                  uint32_t* n_events, fyzz_contact_event_point* points, uint32_t point_capacity,
+                                  uint32_t* n_points) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        u32 counts[4] = {0, 0, 0, 0}, np = 0;
+        hipStream_t st = w->ctx->stream;
+        if (w->notify && w->ev_cap && w->uploaded) {
+            FYZZ_HIP_CHECK(hipMemcpyAsync(counts, w->ev_counts.p, sizeof counts, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(&np, w->d_npts.p, 4, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        FYZZ_REQUIRE(np <= w->pt_cap || !w->notify, kOverflow, "world: contact event points exceed capacity");
+        if (n_events) *n_events = counts[3];
+        if (n_points) *n_points = np;
+        const u32 ne = std::min(counts[3], capacity), mp = std::min(np, point_capacity);
+        if (events && ne)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(events, w->events.p, sizeof(fyzz_contact_event) * ne, hipMemcpyDeviceToHost,
+                                          st));
+        if (points && mp)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(points, w->points.p, sizeof(fyzz_contact_event_point) * mp,
+                                          hipMemcpyDeviceToHost, st));
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     });
 }

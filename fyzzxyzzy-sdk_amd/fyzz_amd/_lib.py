@@ -64,6 +64,9 @@ SIGNATURES = {
     "fyzz_hip_world_body_records": (C.c_int, [vp, vp, C.c_uint32, u32p]),
     "fyzz_hip_world_stats": (C.c_int, [vp, u32p, u32p, u32p, u32p]),
     "fyzz_hip_world_fetch_managers": (C.c_int, [vp, vp, vp, vp, C.c_uint32, u32p]),
+    "fyzz_hip_world_totals": (C.c_int, [vp, u64p, C.c_int]),
+    "fyzz_hip_world_set_contact_notifications": (C.c_int, [vp, C.c_int]),
+    "fyzz_hip_world_contact_events": (C.c_int, [vp, vp, C.c_uint32, u32p, vp, C.c_uint32, u32p]),
     "fyzz_hip_world_device_views": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                               C.POINTER(vp)]),
 }

@@ -14,6 +14,15 @@ import numpy as np
 from ._lib import Context, FyzzError, lib
 from .dy import DY_BODY, DY_MANAGER
 
+# fyzz_contact_event / fyzz_contact_event_point (include/fyzz_hip.h), the
+# layouts of fyzz::ContactEvent / ContactEventPoint
+CONTACT_EVENT = np.dtype([("body0", "<u4"), ("body1", "<u4"), ("kind", "u1"), ("contact_count", "u1"),
+                          ("pad", "u1", 6), ("point_offset", "<u8"), ("normal_impulse", "<f8"), ("shape0", "<u4"),
+                          ("shape1", "<u4")])
+CONTACT_EVENT_POINT = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("separation", "<f4"),
+                                ("pad", "<f4"), ("normal_impulse", "<f8")])
+assert CONTACT_EVENT.itemsize == 40 and CONTACT_EVENT_POINT.itemsize == 40
+
 vp = C.c_void_p
 
 
@@ -113,6 +122,28 @@ class World:
         a, b, c, d = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
         self._call("fyzz_hip_world_stats", C.byref(a), C.byref(b), C.byref(c), C.byref(d))
         return {"pairs": a.value, "managers": b.value, "partitions": c.value, "flags": d.value}
+
+    def set_contact_notifications(self, enabled=True):
+        self._call("fyzz_hip_world_set_contact_notifications", C.c_int(1 if enabled else 0))
+
+    def contact_events(self):
+        """(events, points) of the last step, host copies."""
+        ne, npt = C.c_uint32(), C.c_uint32()
+        self._call("fyzz_hip_world_contact_events", None, C.c_uint32(0), C.byref(ne), None, C.c_uint32(0),
+                   C.byref(npt))
+        ev = np.zeros(max(ne.value, 1), CONTACT_EVENT)
+        pt = np.zeros(max(npt.value, 1), CONTACT_EVENT_POINT)
+        self._call("fyzz_hip_world_contact_events", ev.ctypes.data_as(vp), C.c_uint32(ne.value), C.byref(ne),
+                   pt.ctypes.data_as(vp), C.c_uint32(npt.value), C.byref(npt))
+        return ev[:ne.value], pt[:npt.value]
+
+    def totals(self, reset=False):
+        """Running totals: steps, pairs, managers, contact rows, friction
+        rows, partitions, solver iterations (summed over steps)."""
+        out = (C.c_uint64 * 8)()
+        self._call("fyzz_hip_world_totals", out, C.c_int(1 if reset else 0))
+        keys = ("steps", "pairs", "managers", "contact_rows", "friction_rows", "partitions", "iterations")
+        return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def managers(self):
         """(managers, impulses[n, 4], partitions[n]) of the last step, host copies."""

@@ -96,6 +96,29 @@ typedef struct fyzz_dy_patch {
     uint32_t anchor_count, flags;
 } fyzz_dy_patch; /* 96 bytes */
 
+/* fyzz::ContactEvent (include/fyzzxyzzy/world.h:178-190), same layout:
+ * body0 < body1, kind (0 found, 1 persists, 2 lost), contact count, offset
+ * of the event's window in the point array, summed normal impulse, shapes
+ * in body order. */
+typedef struct fyzz_contact_event {
+    uint32_t body0, body1;
+    uint8_t kind, contact_count;
+    uint8_t pad[6];
+    uint64_t point_offset;
+    double normal_impulse;
+    uint32_t shape0, shape1;
+} fyzz_contact_event; /* 40 bytes */
+
+/* fyzz::ContactEventPoint (world.h:194-199), same layout: position,
+ * outward normal, separation, per-point normal impulse. */
+typedef struct fyzz_contact_event_point {
+    float position[3];
+    float normal[3];
+    float separation;
+    float pad;
+    double normal_impulse;
+} fyzz_contact_event_point; /* 40 bytes */
+
 /* ---- context -------------------------------------------------------- */
 /* Stands in for the producer's owned compute::Context + stream
  * (source_sc_owned_gpu_broadphase_native.h:66-162). */
@@ -319,6 +342,22 @@ int fyzz_hip_world_stats(fyzz_hip_world* world, uint32_t* n_pairs, uint32_t* n_m
  * entries (any may be NULL); *count = number of managers. */
 int fyzz_hip_world_fetch_managers(fyzz_hip_world* world, fyzz_dy_manager* managers, float* impulses,
                                   uint32_t* partitions, uint32_t capacity, uint32_t* count);
+/* Synchronizes; running totals since creation or the last reset: [0]
+ * steps, [1] broadphase pairs, [2] touching managers, [3] contact rows,
+ * [4] friction rows, [5] partitions, [6] solver iterations (out[8], HOST).
+ * reset != 0 zeroes them afterwards. */
+int fyzz_hip_world_totals(fyzz_hip_world* world, uint64_t* out, int reset);
+/* Scene::set_contact_notifications (scene.h:654): when enabled, every
+ * step also builds Scene::publish_contact_events' output on the device
+ * (scene.cpp:9149-9259): touching managers in Gu pair order as found or
+ * persists, merged with lost touches, each owning a window of points. */
+int fyzz_hip_world_set_contact_notifications(fyzz_hip_world* world, int enabled);
+/* Synchronizes; copies the last step's events and points to HOST arrays
+ * (either may be NULL to query the counts). World::contact_events /
+ * contact_event_points (world.h:467-470). */
+int fyzz_hip_world_contact_events(fyzz_hip_world* world, fyzz_contact_event* events, uint32_t capacity,
+                                  uint32_t* n_events, fyzz_contact_event_point* points,
+                                  uint32_t point_capacity, uint32_t* n_points);
 /* Device pointers of the last step's manager list, contact impulses
  * [n][4], partitions and body records; valid until the next step. */
 int fyzz_hip_world_device_views(fyzz_hip_world* world, const fyzz_dy_manager** managers,

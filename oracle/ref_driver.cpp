@@ -144,6 +144,66 @@ int ref_add_dynamic_box(void* s, const float* half, const float* pos,
             {half[0], half[1], half[2]}, {pos[0], pos[1], pos[2]}, density);
     });
 }
+// n dynamic boxes with pose, velocities and material (the bench's scene
+// load); ids[n] receives the body ids.
+int ref_add_boxes(void* s, std::uint32_t n, const float* half, const float* pos, const float* quat,
+                  const float* vel, const float* ang, float density, float sf, float df, float rest,
+                  std::uint32_t* ids) {
+    return guarded([&] {
+        auto* sc = static_cast<Scene*>(s);
+        fyzz::Material m;
+        m.static_friction = sf;
+        m.dynamic_friction = df;
+        m.restitution = rest;
+        for (std::uint32_t i = 0; i < n; ++i) {
+            const float* h = half + 3ull * i;
+            const float* p = pos + 3ull * i;
+            const float* q = quat + 4ull * i;
+            const std::uint32_t id = sc->add_dynamic_box({h[0], h[1], h[2]}, {p[0], p[1], p[2]}, density);
+            sc->set_pose(id, {p[0], p[1], p[2]}, {q[0], q[1], q[2], q[3]});
+            if (vel) sc->set_velocity(id, {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]});
+            if (ang) sc->set_angular_velocity(id, {ang[3 * i], ang[3 * i + 1], ang[3 * i + 2]});
+            sc->set_material(id, m);
+            ids[i] = id;
+        }
+    });
+}
+int ref_set_contact_notifications(void* s, int enabled) {
+    return guarded([&] { static_cast<Scene*>(s)->set_contact_notifications(enabled != 0); });
+}
+// The last step's contact events: ev[n][6] u32 (body0, body1, kind, count,
+// shape0, shape1), ev_offset[n] u64, ev_impulse[n] f64; points pt[m][7]
+// f32 (position, normal, separation), pt_impulse[m] f64. Returns the event
+// count; *n_points receives the point count. Null outputs query sizes.
+std::size_t ref_contact_events(void* s, std::uint32_t* ev, std::uint64_t* ev_offset, double* ev_impulse,
+                               std::size_t cap, float* pt, double* pt_impulse, std::size_t pt_cap,
+                               std::size_t* n_points) {
+    auto* sc = static_cast<Scene*>(s);
+    const auto events = sc->contact_events();
+    const auto points = sc->contact_event_points();
+    if (n_points) *n_points = points.size();
+    if (ev)
+        for (std::size_t i = 0; i < events.size() && i < cap; ++i) {
+            const auto& e = events[i];
+            ev[6 * i] = e.body0;
+            ev[6 * i + 1] = e.body1;
+            ev[6 * i + 2] = static_cast<std::uint32_t>(e.kind);
+            ev[6 * i + 3] = e.contact_count;
+            ev[6 * i + 4] = e.shape0;
+            ev[6 * i + 5] = e.shape1;
+            ev_offset[i] = e.point_offset;
+            ev_impulse[i] = e.normal_impulse;
+        }
+    if (pt)
+        for (std::size_t i = 0; i < points.size() && i < pt_cap; ++i) {
+            const auto& q = points[i];
+            const float v[7] = {q.position.x, q.position.y, q.position.z, q.normal.x, q.normal.y, q.normal.z,
+                                q.separation};
+            for (int k = 0; k < 7; ++k) pt[7 * i + k] = v[k];
+            pt_impulse[i] = q.normal_impulse;
+        }
+    return events.size();
+}
 int ref_add_static_box(void* s, const float* half, const float* pos,
                        std::uint32_t* id) {
     return guarded([&] {

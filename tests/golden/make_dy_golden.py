@@ -14,6 +14,13 @@ by the reference NativeCpu scene. Per step s:
 Contacts are not stored: the oracle's narrowphase reproduces the
 reference's manifolds bit-exactly (tests/test_pcm_oracle.py).
 
+dy_events.npz: the same run with contact notifications on (the simulation
+is unchanged; the script checks the states against dy_scene.npz). Per step
+s: ev_{s} [n, 6] u32 (body0, body1, kind 0 found / 1 persists / 2 lost,
+contact count, shape0, shape1), evoff_{s} u64 point offsets, evimp_{s} f64
+summed normal impulses, pt_{s} [m, 7] f32 (position, outward normal,
+separation), ptimp_{s} f64 per-point normal impulses.
+
     python tests/golden/make_dy_golden.py
 """
 from __future__ import annotations
@@ -31,11 +38,15 @@ from fyzz_amd import scenes  # noqa: E402
 from make_sap_golden import RefScene, load_ref, P  # noqa: E402
 
 
-def dy_scene(path: Path, side=3, steps=72, seed=scenes.SEED_BASE + 23, spacing=1.15, base=0.62):
+def dy_scene(path: Path, side=3, steps=72, seed=scenes.SEED_BASE + 23, spacing=1.15, base=0.62,
+             events_path: Path | None = None):
     ref = load_ref()
-    for f in ("ref_dy_descriptors", "ref_dy_tasks", "ref_manifolds"):
+    for f in ("ref_dy_descriptors", "ref_dy_tasks", "ref_manifolds", "ref_contact_events"):
         getattr(ref, f).restype = C.c_size_t
     sc = RefScene(ref)
+    if events_path is not None:
+        sc.ok(ref.ref_set_contact_notifications(sc.s, 1))
+    events = {}
     floor = sc.add_floor(0.0)
     sc.ok(ref.ref_set_material(sc.s, C.c_uint32(floor), C.c_float(0.5), C.c_float(0.5), C.c_float(0.0)))
     n = side ** 3
@@ -91,6 +102,20 @@ def dy_scene(path: Path, side=3, steps=72, seed=scenes.SEED_BASE + 23, spacing=1
         me = np.zeros(t, np.uint64)
         pe = np.zeros(t, np.uint64)
         ref.ref_dy_tasks(sc.s, P(me), P(pe), t)
+        if events_path is not None:
+            npts = C.c_size_t()
+            ne = ref.ref_contact_events(sc.s, None, None, None, 0, None, None, 0, C.byref(npts))
+            ev = np.zeros((max(ne, 1), 6), np.uint32)
+            off = np.zeros(max(ne, 1), np.uint64)
+            imp = np.zeros(max(ne, 1), np.float64)
+            pt = np.zeros((max(npts.value, 1), 7), np.float32)
+            pimp = np.zeros(max(npts.value, 1), np.float64)
+            ref.ref_contact_events(sc.s, P(ev), P(off), P(imp), ne, P(pt), P(pimp), npts.value, C.byref(npts))
+            events[f"ev_{s}"] = ev[:ne]
+            events[f"evoff_{s}"] = off[:ne]
+            events[f"evimp_{s}"] = imp[:ne]
+            events[f"pt_{s}"] = pt[:npts.value]
+            events[f"ptimp_{s}"] = pimp[:npts.value]
         rec[f"desc_{s}"] = mp[di]
         rec[f"body_{s}"] = np.stack([b0, b1], 1)
         rec[f"part_{s}"] = pt
@@ -103,8 +128,17 @@ def dy_scene(path: Path, side=3, steps=72, seed=scenes.SEED_BASE + 23, spacing=1
     print(path.name, "boxes", n, "steps", steps, "descriptors at last", len(rec[f"desc_{steps - 1}"]),
           "max partitions", max(int(rec[f"part_{s}"].max()) + 1 if len(rec[f"part_{s}"]) else 0
                                 for s in range(steps)))
+    if events_path is not None:
+        events["steps"] = np.array([steps])
+        np.savez_compressed(events_path, **events)
+        print(events_path.name, "events", sum(len(v) for k, v in events.items() if k.startswith("ev_")))
+        return rec
     np.savez_compressed(path, **rec)
+    return rec
 
 
 if __name__ == "__main__":
-    dy_scene(Path(__file__).resolve().parent / "dy_scene.npz")
+    here = Path(__file__).resolve().parent
+    plain = dy_scene(here / "dy_scene.npz")
+    with_events = dy_scene(here / "dy_scene.npz", events_path=here / "dy_events.npz")
+    assert np.array_equal(plain["state"], with_events["state"]), "notifications changed the simulation"

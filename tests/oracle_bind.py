@@ -453,7 +453,56 @@ class OracleWorld:
         self.history = {k: patches[i] for i, k in enumerate(keys)}
         self.bodies = out
         self.last = (pairs, mg, imp, parts)
+        self.last_keys = keys
         return out
+
+    def shape_pair(self, key):
+        """Canonical shape-id pair of a manager key (floor = body 0, box
+        slot s = body s + 1, as the world tests author them)."""
+        if key[0] == "bb":
+            return (key[1] + 1, key[2] + 1)
+        return (0, key[1] + 1)
 
     def body_states(self):
         return body_state_from(self.bodies)
+
+
+def oracle_contact_events(lib, managers, impulses, keys, reversed_flags, prev_pairs, body_of_shape=None):
+    """orc_contact_events over touching managers (any order; sorted here
+    into Gu pair order). managers: DY_MANAGER records with published
+    points; impulses [n, 4] per Gu point; keys: canonical (shape a, shape b)
+    per manager; reversed_flags per manager; prev_pairs: sorted (a, b)
+    touching last step. Returns (events [k, 6] u32, offsets u64, impulses
+    f64, points [m, 7] f32, point impulses f64)."""
+    n = len(keys)
+    order = sorted(range(n), key=lambda i: keys[i])
+    cur = np.zeros((max(n, 1), 36), np.uint32)
+    imp = np.zeros((max(n, 1), 4), np.float32)
+    for r, i in enumerate(order):
+        a, b = keys[i]
+        ba = a if body_of_shape is None else body_of_shape[a]
+        bb = b if body_of_shape is None else body_of_shape[b]
+        cur[r, :6] = [a, b, ba, bb, int(reversed_flags[i]), int(managers[i]["count"])]
+        pts = np.zeros((4, 7), np.float32)
+        for k in range(int(managers[i]["count"])):
+            c = managers[i]["contacts"][k]
+            pts[k, 0:3] = c["position"]
+            pts[k, 3:6] = c["normal"]
+            pts[k, 6] = c["separation"]
+        cur[r, 6:34] = pts.reshape(-1).view(np.uint32)
+        imp[r] = impulses[i]
+    prev = np.ascontiguousarray(np.asarray(prev_pairs, np.uint32).reshape(-1, 2)) if len(prev_pairs) else \
+        np.zeros((1, 2), np.uint32)
+    cap = n + len(prev_pairs) + 1
+    ev = np.zeros((cap, 6), np.uint32)
+    off = np.zeros(cap, np.uint64)
+    eimp = np.zeros(cap, np.float64)
+    pt = np.zeros((4 * n + 1, 7), np.float32)
+    pimp = np.zeros(4 * n + 1, np.float64)
+    ne, npt = C.c_uint32(), C.c_uint32()
+    rc = lib.orc_contact_events(C.c_uint32(n), P(cur), P(imp), C.c_uint32(len(prev_pairs)), P(prev), C.c_uint32(cap),
+                                P(ev), P(off), P(eimp), C.c_uint32(len(pt)), P(pt), P(pimp), C.byref(ne),
+                                C.byref(npt))
+    assert rc == 0
+    k, m = ne.value, npt.value
+    return ev[:k], off[:k], eimp[:k], pt[:m], pimp[:m]

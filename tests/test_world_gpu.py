@@ -45,9 +45,31 @@ def small_pile(side=3, seed=scenes.SEED_BASE + 23):
     return scenes.Scene(half, pos, quat, vel=vel, friction=0.5, restitution=0.0)
 
 
-def run_compare(ctx, orc, sc, steps, dt=1 / 120):
+def compare_events(orc, w, ow, prev, s):
+    """Device events == the event oracle over the oracle world's managers
+    and impulses; returns this step's touching shape pairs."""
+    from oracle_bind import oracle_contact_events
+    pairs, mg_o, imp_o, _ = ow.last
+    keys = [ow.shape_pair(k) for k in ow.last_keys]
+    ev_o, off_o, eimp_o, pt_o, pimp_o = oracle_contact_events(orc.lib, mg_o, imp_o, keys, [True] * len(keys), prev)
+    ev, pt = w.contact_events()
+    got = np.stack([ev["body0"], ev["body1"], ev["kind"], ev["contact_count"], ev["shape0"], ev["shape1"]], 1)
+    np.testing.assert_array_equal(got.astype(np.uint32), ev_o, err_msg=f"step {s} events")
+    np.testing.assert_array_equal(ev["point_offset"], off_o, err_msg=f"step {s} point offsets")
+    np.testing.assert_allclose(ev["normal_impulse"], eimp_o, rtol=1e-4, atol=1e-5, err_msg=f"step {s} impulses")
+    got_pt = np.concatenate([pt["position"], pt["normal"], pt["separation"][:, None]], 1)
+    np.testing.assert_allclose(got_pt, pt_o, rtol=1e-5, atol=1e-6, err_msg=f"step {s} points")
+    np.testing.assert_allclose(pt["normal_impulse"], pimp_o, rtol=1e-4, atol=1e-5, err_msg=f"step {s} point impulses")
+    return sorted(keys), np.bincount(ev_o[:, 2], minlength=3)
+
+
+def run_compare(ctx, orc, sc, steps, dt=1 / 120, events=False):
     w = make_world(ctx, sc)
+    if events:
+        w.set_contact_notifications(True)
     ow = OracleWorld(orc, sc)
+    prev_touch = []
+    kinds = np.zeros(3, np.int64)
     authored = w.body_records()
     np.testing.assert_array_equal(authored.view(np.uint8), ow.bodies.view(np.uint8), err_msg="authored records")
     exact_steps = 0
@@ -72,16 +94,22 @@ def run_compare(ctx, orc, sc, steps, dt=1 / 120):
         gs, os_ = w.body_states(), ow.body_states()
         np.testing.assert_allclose(gs, os_, rtol=1e-4, atol=1e-6, err_msg=f"step {s} bodies")
         exact_steps += int(np.array_equal(gs, os_))
+        if events:
+            prev_touch, k = compare_events(orc, w, ow, prev_touch, s)
+            kinds += k
+    if events:
+        print("event kinds (found, persists, lost):", kinds.tolist())
+        assert kinds.min() > 0
     return exact_steps
 
 
 def test_small_pile_world_matches_oracle_pipeline(ctx, orc, torch_cuda):
-    exact = run_compare(ctx, orc, small_pile(), 120)
+    exact = run_compare(ctx, orc, small_pile(), 120, events=True)
     print(f"bit-identical steps: {exact}/120")
 
 
 def test_grid_1000_world_matches_oracle_pipeline(ctx, orc, torch_cuda):
     """1,000 boxes on the 10x10x10 grid with seeded yaw and velocity over a
     floor: 240 steps at 1/120 s (the boxes fall, land and stack)."""
-    exact = run_compare(ctx, orc, scenes.grid_scene(), 240)
+    exact = run_compare(ctx, orc, scenes.grid_scene(), 240, events=True)
     print(f"bit-identical steps: {exact}/240")
