@@ -80,6 +80,7 @@ struct Ws {
     float* impulses;
     u32* parts_out;
     const u32* d_nm;  // device manager count (at most nm), or null
+    u32 with_rows;    // contact prep ran: order, rows and patches are valid
     // per body
     float4 *pv, *pw;          // pre-integrated velocities
     float4 *lin, *ang;        // solver deltas: lin_delta|inv_mass, ang_state
@@ -1128,9 +1129,9 @@ __global__ __launch_bounds__(kBlock) void k_outputs(Ws W) {
     if (m < mcount(W)) {
         if (W.parts_out) W.parts_out[m] = W.part[m];
         float out[4] = {0.f, 0.f, 0.f, 0.f};
-        if (W.part[m] < kMaxParts) {
+        if (W.with_rows && W.part[m] < kMaxParts) {
             const u32 o = W.order[m];
-            const u32 no = o & 0xffu;
+            const u32 no = min(o & 0xffu, 4u);
             for (u32 q = 0; q < no; ++q) out[(o >> (8 + 4 * q)) & 0xfu] = W.crow[4 * m + q].applied;
             rows = no;
             frows = 2u * W.patches[m].anchor_count;
@@ -1207,6 +1208,7 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
     w.history = history;
     w.patches = patches;
     w.d_nm = d_nm;
+    w.with_rows = partition_only ? 0u : 1u;
     hipStream_t st = d->ctx->stream;
     auto& prof = d->ctx->prof;
     {

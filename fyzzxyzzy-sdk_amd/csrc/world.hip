@@ -497,7 +497,9 @@ void upload(fyzz_hip_world* w) {
     w->contacts_bb.reserve(28ull * w->pair_cap);
     w->cache_bb[0].reserve(w->pair_cap);
     w->cache_bb[1].reserve(w->pair_cap);
-    w->tiles.reserve(sel::tiles_for(w->mgr_cap) + 1);
+    // sized for the largest stream that uses it (manager list, contact
+    // events): never grown after the first step, while kernels may hold it
+    w->tiles.reserve(sel::tiles_for(std::max(w->mgr_cap, 2 * w->pair_cap + n)) + 1);
     w->d_total.reserve(1);
     w->d_nm.reserve(1);
     w->mgr_slot.reserve(w->mgr_cap);
@@ -571,7 +573,6 @@ void publish_events(fyzz_hip_world* w, const sap::Lists& L, int c_prev, int c_cu
         w->d_npts.reserve(1);
         w->events.reserve(w->ev_cap);
         w->points.reserve(w->pt_cap);
-        w->tiles.reserve(sel::tiles_for(std::max(w->mgr_cap, w->ev_cap)) + 1);
     }
     ProfScope ps(&w->ctx->prof, "world.events", st);
     EvArgs A{cap,          L.d_cur,         L.prev,         L.d_prev,        L.lost,       L.d_lost,

@@ -306,9 +306,9 @@ int fyzz_hip_dy_status(fyzz_hip_dy* dy, uint32_t* flags, uint32_t* n_partitions,
  * in creation order (the reference's island order is not rebuilt). */
 int fyzz_hip_world_create(fyzz_hip_context* ctx, uint32_t max_bodies, fyzz_hip_world** out);
 int fyzz_hip_world_destroy(fyzz_hip_world* world);
-/* World::add_static_floor (world.h:414) */
+/* World::add_static_floor (world.h:424) */
 int fyzz_hip_world_add_static_floor(fyzz_hip_world* world, float height, uint32_t* body_id);
-/* World::add_dynamic_box (world.h:416): HOST half_extents[3], density */
+/* World::add_dynamic_box (world.h:425): HOST half_extents[3], density */
 int fyzz_hip_world_add_dynamic_box(fyzz_hip_world* world, const float* half_extents, float density,
                                    uint32_t* body_id);
 /* n add_dynamic_box calls followed by set_body_pose, set_body_velocity,
@@ -321,7 +321,7 @@ int fyzz_hip_world_add_dynamic_boxes(fyzz_hip_world* world, uint32_t n, const fl
                                      const float* angular, float static_friction, float dynamic_friction,
                                      float restitution, uint32_t* first_id);
 /* World::set_body_pose / set_body_velocity + set_body_angular_velocity /
- * set_body_material (world.h:420-440); HOST arrays, NULL leaves a part. */
+ * set_body_material (world.h:475-478, 491); HOST arrays, NULL leaves a part. */
 int fyzz_hip_world_set_body_pose(fyzz_hip_world* world, uint32_t body_id, const float* position,
                                  const float* orientation);
 int fyzz_hip_world_set_body_velocity(fyzz_hip_world* world, uint32_t body_id, const float* linear,
@@ -354,7 +354,7 @@ int fyzz_hip_world_totals(fyzz_hip_world* world, uint64_t* out, int reset);
 int fyzz_hip_world_set_contact_notifications(fyzz_hip_world* world, int enabled);
 /* Synchronizes; copies the last step's events and points to HOST arrays
  * (either may be NULL to query the counts). World::contact_events /
- * contact_event_points (world.h:467-470). */
+ * contact_event_points (world.h:467-468). */
 int fyzz_hip_world_contact_events(fyzz_hip_world* world, fyzz_contact_event* events, uint32_t capacity,
                                   uint32_t* n_events, fyzz_contact_event_point* points,
                                   uint32_t point_capacity, uint32_t* n_points);
