@@ -360,7 +360,9 @@ def bench_pile(args):
                    "pairs_at_end": st["pairs"], "managers_at_end": st["managers"],
                    "partitions_at_end": st["partitions"],
                    "contact_rows_per_step": tot["contact_rows"] / max(tot["steps"], 1),
-                   "friction_rows_per_step": tot["friction_rows"] / max(tot["steps"], 1)},
+                   "friction_rows_per_step": tot["friction_rows"] / max(tot["steps"], 1),
+                   "partitions_per_step": tot["partitions"] / max(tot["steps"], 1),
+                   "partition_rounds_per_step": tot["rounds"] / max(tot["steps"], 1)},
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "algorithmic_bytes_per_launch": alg / max(stages[dominant]["launches"], 1)},

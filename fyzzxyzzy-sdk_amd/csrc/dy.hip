@@ -103,13 +103,20 @@ struct Ws {
 __device__ inline u32 ld(const u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ inline void raise_flag(const Ws& W, u32 f) { atomicOr(W.flags, f); }
 __device__ inline u32 mcount(const Ws& W) { return W.d_nm ? min(ld(W.d_nm), W.nm) : W.nm; }
-// Grid-wide barrier of a cooperative launch (every workgroup resident):
-// arrival count and generation word, relaxed agent-scope atomics, with an
-// agent-scope release before arriving and acquire after leaving (so the
-// grid's global-memory writes before the barrier are visible after it).
-// Thread 0 of each workgroup keeps the generation it waits on. The spin is
-// bounded; exceeding it (workgroups not co-resident) raises a flag.
+// Grid-wide barrier of a cooperative launch (every workgroup resident),
+// two levels: workgroups arrive on their group's counter (groups of
+// kBarGroup), the last of each group arrives on the root counter, and the
+// last root arrival publishes the next generation. bar[0] root count,
+// bar[1] generation, bar[2 + g] group counts. Relaxed agent-scope atomics;
+// every workgroup releases its writes (agent scope) before arriving, the
+// releasing workgroup acquires before publishing, and every workgroup
+// acquires after leaving, so the grid's global-memory writes before the
+// barrier are visible after it. Thread 0 of each workgroup keeps the
+// generation it waits on. The spin is bounded; exceeding it (workgroups
+// not co-resident) raises a flag.
 constexpr u32 kSpinLimit = 1u << 24;
+constexpr u32 kBarGroup = 8;
+constexpr u32 kBarWords = 64;  // root, generation, up to 62 groups
 __device__ inline u32 barrier_start(const Ws& W) {
     return threadIdx.x == 0 ? __hip_atomic_load(&W.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
 }
@@ -118,9 +125,19 @@ __device__ inline void grid_sync(const Ws& W, u32& gen) {
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const u32 g0 = gen;
-        const u32 arrived = __hip_atomic_fetch_add(&W.bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(&W.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u32 grp = blockIdx.x / kBarGroup;
+        const u32 ngroups = (gridDim.x + kBarGroup - 1) / kBarGroup;
+        const u32 gsize = min(kBarGroup, gridDim.x - grp * kBarGroup);
+        bool last = false;
+        if (__hip_atomic_fetch_add(&W.bar[2 + grp], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
+            __hip_atomic_store(&W.bar[2 + grp], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(&W.bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1) {
+                __hip_atomic_store(&W.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                last = true;
+            }
+        }
+        if (last) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
             __hip_atomic_store(&W.bar[1], g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             u32 spins = 0;
@@ -162,12 +179,10 @@ __device__ inline bool manager_ok(const Ws& W, const fyzz_dy_manager& M) {
 
 // status flags and counters of the step (a fill kernel, so a captured
 // step records no memset node)
-__global__ void k_clear(Ws W) {
+__global__ __launch_bounds__(64) void k_clear(Ws W) {
     if (threadIdx.x < kCtrWords) W.ctr[threadIdx.x] = 0;
-    if (threadIdx.x == 0) {
-        *W.flags = 0;
-        W.bar[0] = 0;
-    }
+    if (threadIdx.x == 0) *W.flags = 0;
+    if (threadIdx.x < kBarWords && threadIdx.x != 1) W.bar[threadIdx.x] = 0;  // counts, not the generation
 }
 
 // scene.cpp:10383-10507 for default dynamics (no pending velocity mods)
@@ -1176,7 +1191,10 @@ int coop_blocks(const void* kernel, int device) {
     FYZZ_REQUIRE(per_cu > 0 && cus > 0, kDeviceError, "dy: cooperative kernel does not fit a compute unit");
     // 64 resident workgroups: a grid barrier costs ~3 us there, against
     // ~17 us at 512 (the barrier is the Dy kernels' critical path)
-    return std::min(per_cu * cus, dy::kCoopBlocks);
+    const int blocks = std::min(per_cu * cus, dy::kCoopBlocks);
+    FYZZ_REQUIRE(blocks <= static_cast<int>(dy::kBarGroup * (dy::kBarWords - 2)), kLogicError,
+                 "dy: too many workgroups for the grid barrier");
+    return blocks;
 }
 
 void launch_coop(const void* kernel, int blocks, dy::Ws& ws, hipStream_t st) {
@@ -1291,8 +1309,8 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->part_fill.reserve(dy::kMaxParts);
         d->ctr.reserve(dy::kCtrWords);
         d->flags.reserve(1);
-        d->bar.reserve(4);
-        FYZZ_HIP_CHECK(hipMemset(d->bar.get(), 0, 4 * sizeof(u32)));
+        d->bar.reserve(dy::kBarWords);
+        FYZZ_HIP_CHECK(hipMemset(d->bar.get(), 0, dy::kBarWords * sizeof(u32)));
         d->coop_blocks_partition = coop_blocks(reinterpret_cast<const void*>(&dy::k_partition), ctx->device);
         d->coop_blocks_pgs = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs), ctx->device);
         d->blocksums.reserve(2 * std::max(d->coop_blocks_partition, d->coop_blocks_pgs) + 2);

@@ -102,6 +102,7 @@ enum Counter : u32 {
     kOverflow,  // a capacity was exceeded
     kSweep,
     kLarge,     // this frame's candidate list exceeds the one-block path
+    kLongWalks, // crossing walks handed to k_cross_long
     kCounters
 };
 
@@ -230,76 +231,137 @@ __global__ __launch_bounds__(256) void k_sweep(u32 n2, const u32* xpay, const Re
 // array. One wave per (moved box, axis); lanes stride over each interval.
 // Every candidate pair is re-tested on all three axes with old and new
 // records; a status change emits the pair with kFoundBit for "found".
-__global__ __launch_bounds__(256) void k_cross(u32 n2, const u32* cnt_moved, const u32* moved,
-                                               const u32* new_key, const u32* new_pay,
-                                               const u32* old_key, const u32* old_pay,
-                                               const u32* new_rank, const u32* old_rank,
-                                               const u32* moved_bits, const Rec* cur, const Rec* prev,
-                                               u64* out, u32* cnt, u32 cap) {
-    const u32 lane = lane_id();
-    const u32 n_work = min(*cnt_moved, n2 >> 1) * 3u;
-    const u32 n_waves = gridDim.x * (blockDim.x / kWave);
-    for (u32 w = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; w < n_work; w += n_waves) {
-    const u32 a = moved[w / 3u];
-    const u32 axis = w % 3u;
-    const Rec an = cur[a];
-    const Rec ao = prev[a];
-    const u32* nk = new_key + static_cast<std::size_t>(axis) * n2;
-    const u32* np = new_pay + static_cast<std::size_t>(axis) * n2;
-    const u32* ok = old_key + static_cast<std::size_t>(axis) * n2;
-    const u32* op = old_pay + static_cast<std::size_t>(axis) * n2;
-    for (u32 kind = 0; kind < 2; ++kind) {
-        const u32 v_old = ao.k[axis + 3 * kind];
-        const u32 v_new = an.k[axis + 3 * kind];
+// One crossing walk: the endpoint (a, kind) of a moved box on one axis,
+// in the new array (arr 0) or the old one (arr 1).
+struct Walk {
+    u32 a, axis, kind, arr;
+};
+
+struct CrossArgs {
+    u32 n2;
+    const u32* cnt_moved;
+    const u32* moved;
+    const u32* new_key;
+    const u32* new_pay;
+    const u32* old_key;
+    const u32* old_pay;
+    const u32* new_rank;
+    const u32* old_rank;
+    const u32* moved_bits;
+    const Rec* cur;
+    const Rec* prev;
+    u64* out;
+    u32* cnt;
+    u32 cap;
+    u32* long_walks;   // walks still inside after kShortWalk entries
+    u32 long_cap;
+};
+
+// Entries a thread walks before handing the rest to a wave.
+constexpr u32 kShortWalk = 16;
+
+// Walk geometry: the endpoint's rank, the bound (its other value) and the
+// direction. e sits at its new value in the new array and at its old value
+// in the old one; the crossed entries lie between e and the composite
+// (other value, pe), walking up or down from e.
+struct WalkGeom {
+    const u32* K;
+    const u32* P;
+    u32 own, bound_v, pe;
+    bool up;
+};
+__device__ inline WalkGeom walk_geom(const CrossArgs& A, const Walk& w, u32 v_old, u32 v_new) {
+    WalkGeom g;
+    const std::size_t ax = static_cast<std::size_t>(w.axis) * A.n2;
+    g.K = (w.arr ? A.old_key : A.new_key) + ax;
+    g.P = (w.arr ? A.old_pay : A.new_pay) + ax;
+    g.pe = (w.a << 1) | w.kind;
+    g.own = (w.arr ? A.old_rank : A.new_rank)[ax + g.pe];
+    g.bound_v = w.arr ? v_new : v_old;
+    g.up = (w.arr != 0) == (v_new > v_old);  // old array: toward v_new
+    return g;
+}
+// Is the step-th entry from e inside the crossed range? (j receives its index)
+__device__ inline bool walk_inside(const CrossArgs& A, const WalkGeom& g, u32 step, u32& j) {
+    if (g.up) {
+        j = g.own + step;
+        if (j >= A.n2) return false;
+        const u32 kj = g.K[j];
+        return kj < g.bound_v || (kj == g.bound_v && g.P[j] < g.pe);
+    }
+    if (step > g.own) return false;
+    j = g.own - step;
+    const u32 kj = g.K[j];
+    return kj > g.bound_v || (kj == g.bound_v && g.P[j] > g.pe);
+}
+// A crossed entry: a candidate pair when it is the other kind of endpoint
+// of another box; re-tested on all three axes with old and new records.
+__device__ inline void walk_visit(const CrossArgs& A, const Walk& w, const WalkGeom& g, u32 j, const Rec& an,
+                                  const Rec& ao) {
+    const u32 pf = g.P[j];
+    const u32 b = pf >> 1;
+    const bool b_moved = (A.moved_bits[b >> 5] >> (b & 31u)) & 1u;
+    // an unmoved endpoint sits at the same value in both arrays: the
+    // new-array walk already saw it
+    if ((pf & 1u) != w.kind && b != w.a && (b_moved || !w.arr)) {
+        const Rec bn = A.cur[b];
+        if (pair_allowed(an, bn)) {
+            const Rec bo = b_moved ? A.prev[b] : bn;
+            const bool ov_new = overlap3(an, bn);
+            const bool ov_old = overlap3(ao, bo);
+            if (ov_new != ov_old) emit(pair_key(w.a, b) | (ov_new ? kFoundBit : 0ull), A.out, A.cnt, kDelta, A.cap);
+        }
+    }
+}
+
+// One thread per walk (moved box x axis x endpoint kind x array) for its
+// first kShortWalk entries; longer walks continue in k_cross_long.
+__global__ __launch_bounds__(256) void k_cross(CrossArgs A) {
+    const u32 n_work = min(*A.cnt_moved, A.n2 >> 1) * 12u;
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < n_work; t += gridDim.x * blockDim.x) {
+        const u32 wa = t >> 2;
+        const Walk w{A.moved[wa / 3u], wa % 3u, (t >> 1) & 1u, t & 1u};
+        const Rec an = A.cur[w.a];
+        const Rec ao = A.prev[w.a];
+        const u32 v_old = ao.k[w.axis + 3 * w.kind];
+        const u32 v_new = an.k[w.axis + 3 * w.kind];
         if (v_old == v_new) continue;
-        const u32 pe = (a << 1) | kind;
-        const bool right = v_new > v_old;
-        for (u32 arr = 0; arr < 2; ++arr) {
-            const u32* K = arr ? ok : nk;
-            const u32* P = arr ? op : np;
-            // e sits at its new value in the new array and at its old value
-            // in the old one; the crossed entries lie between e and the
-            // composite (other value, pe), walking up or down from e.
-            const u32 own = (arr ? old_rank : new_rank)[static_cast<std::size_t>(axis) * n2 + pe];
-            const u32 bound_v = arr ? v_new : v_old;
-            const bool up = (arr != 0) == right;  // old array: toward v_new
-            for (u32 base = 0;; base += kWave) {
-                const u32 step = base + lane + 1;
-                bool inside = false;
-                u32 j = 0;
-                if (up) {
-                    j = own + step;
-                    if (j < n2) {
-                        const u32 kj = K[j];
-                        inside = kj < bound_v || (kj == bound_v && P[j] < pe);
-                    }
-                } else if (step <= own) {
-                    j = own - step;
-                    const u32 kj = K[j];
-                    inside = kj > bound_v || (kj == bound_v && P[j] > pe);
-                }
-                if (inside) {
-                    const u32 pf = P[j];
-                    const u32 b = pf >> 1;
-                    const bool b_moved = (moved_bits[b >> 5] >> (b & 31u)) & 1u;
-                    // an unmoved endpoint sits at the same value in both
-                    // arrays: the new-array scan already saw it
-                    if ((pf & 1u) != kind && b != a && (b_moved || !arr)) {
-                        const Rec bn = cur[b];
-                        if (pair_allowed(an, bn)) {
-                            const Rec bo = b_moved ? prev[b] : bn;
-                            const bool ov_new = overlap3(an, bn);
-                            const bool ov_old = overlap3(ao, bo);
-                            if (ov_new != ov_old)
-                                emit(pair_key(a, b) | (ov_new ? kFoundBit : 0ull), out, cnt, kDelta, cap);
-                        }
-                    }
-                }
-                // sorted entries: the in-range lanes form a prefix
-                if (__ballot(inside) != ~0ull) break;
+        const WalkGeom g = walk_geom(A, w, v_old, v_new);
+        for (u32 step = 1;; ++step) {
+            u32 j;
+            if (!walk_inside(A, g, step, j)) break;
+            walk_visit(A, w, g, j, an, ao);
+            if (step == kShortWalk) {
+                const u32 slot = atomicAdd(&A.cnt[kLongWalks], 1u);
+                if (slot < A.long_cap)
+                    A.long_walks[slot] = t;
+                else
+                    atomicOr(&A.cnt[kOverflow], 4u);
+                break;
             }
         }
     }
+}
+
+// The rest of the long walks, one wave each, 64 entries per iteration
+// (sorted entries: the in-range lanes form a prefix).
+__global__ __launch_bounds__(256) void k_cross_long(CrossArgs A) {
+    const u32 lane = lane_id();
+    const u32 n_long = min(A.cnt[kLongWalks], A.long_cap);
+    const u32 n_waves = gridDim.x * (blockDim.x / kWave);
+    for (u32 i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < n_long; i += n_waves) {
+        const u32 t = A.long_walks[i];
+        const u32 wa = t >> 2;
+        const Walk w{A.moved[wa / 3u], wa % 3u, (t >> 1) & 1u, t & 1u};
+        const Rec an = A.cur[w.a];
+        const Rec ao = A.prev[w.a];
+        const WalkGeom g = walk_geom(A, w, ao.k[w.axis + 3 * w.kind], an.k[w.axis + 3 * w.kind]);
+        for (u32 base = kShortWalk;; base += kWave) {
+            u32 j = 0;
+            const bool inside = walk_inside(A, g, base + lane + 1, j);
+            if (inside) walk_visit(A, w, g, j, an, ao);
+            if (__ballot(inside) != ~0ull) break;
+        }
     }
 }
 
@@ -474,6 +536,7 @@ struct Sap {
     u32 frame = 0;
     int cur = 0;  // index of the current sorted arrays / pair list
     bool built = false;
+    DevBuf<u32> long_walks;
     DevBuf<u32> planes, seen, moved, moved_bits, counters, sorted_key[2], sorted_val[2], rank[2], tmp_key, tmp_val,
         radix_counts, radix_totals, sel_tiles, dummy_a, dummy_b;
     DevBuf<Rec> rec_cur, rec_prev;
@@ -556,10 +619,14 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
                     Copy{s.pairs[s.cur].p, s.lost.p}, s.sel_tiles.p, cnt + kLost);
     } else {
         long pi = prof->begin("sap.pairs", st);
-        hipLaunchKernelGGL(k_cross, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 3 * kWave, 256), 4096u)), dim3(256), 0, st, n2,
-                           cnt + kMoved, s.moved.p, s.sorted_key[nxt].p, s.sorted_val[nxt].p,
-                           s.sorted_key[s.cur].p, s.sorted_val[s.cur].p, s.rank[nxt].p, s.rank[s.cur].p,
-                           s.moved_bits.p, s.rec_cur.p, s.rec_prev.p, s.work_a.p, cnt, s.delta_cap);
+        CrossArgs A{n2, cnt + kMoved, s.moved.p, s.sorted_key[nxt].p, s.sorted_val[nxt].p, s.sorted_key[s.cur].p,
+                    s.sorted_val[s.cur].p, s.rank[nxt].p, s.rank[s.cur].p, s.moved_bits.p, s.rec_cur.p,
+                    s.rec_prev.p, s.work_a.p, cnt, s.delta_cap, s.long_walks.p, 12u * n};
+        hipLaunchKernelGGL(k_cross, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12, 256), 8192u)),
+                           dim3(256), 0, st, A);
+        hipLaunchKernelGGL(k_cross_long, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12 * kWave, 256),
+                                                            4096u)),
+                           dim3(256), 0, st, A);
         FYZZ_HIP_CHECK(hipGetLastError());
         prof->end("sap.pairs", pi, st);
         ProfScope ps(prof, "sap.lists", st);
@@ -663,6 +730,7 @@ int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_
         s.planes.reserve(6ull * n);
         s.seen.reserve(n);
         s.moved.reserve(n);
+        s.long_walks.reserve(12ull * std::max<u32>(n, 1));
         s.moved_bits.reserve((n + 31) / 32 + 1);
         s.counters.reserve(sap::kCounters);
         for (int i = 0; i < 2; ++i) {

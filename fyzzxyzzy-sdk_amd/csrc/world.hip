@@ -379,7 +379,8 @@ __global__ void k_touch(u32 cap, const u32* d_nbb, const u32* counts_bb, u32* to
 
 // running totals over steps (algorithmic byte accounting of the solve):
 // [0] steps, [1] broadphase pairs, [2] managers, [3] contact rows,
-// [4] friction rows, [5] partitions, [6] position+velocity iterations
+// [4] friction rows, [5] partitions, [6] position+velocity iterations,
+// [7] partition dependency rounds
 __global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const u32* dy_ctr, u64* totals) {
     totals[0] += 1;
     totals[1] += min(*d_pairs, cap);
@@ -388,6 +389,7 @@ __global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const
     totals[4] += dy_ctr[dy::kCtrFriction];
     totals[5] += dy_ctr[dy::kCtrParts];
     totals[6] += dy_ctr[dy::kCtrPos] + max(dy_ctr[dy::kCtrVel], 1u);
+    totals[7] += dy_ctr[dy::kCtrRounds];
 }
 
 // Scene::add_dynamic_box mass properties (scene.cpp:408-475): mass in

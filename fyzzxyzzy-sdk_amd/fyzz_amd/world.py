@@ -139,10 +139,11 @@ class World:
 
     def totals(self, reset=False):
         """Running totals: steps, pairs, managers, contact rows, friction
-        rows, partitions, solver iterations (summed over steps)."""
+        rows, partitions, solver iterations, partition rounds (summed over
+        steps)."""
         out = (C.c_uint64 * 8)()
         self._call("fyzz_hip_world_totals", out, C.c_int(1 if reset else 0))
-        keys = ("steps", "pairs", "managers", "contact_rows", "friction_rows", "partitions", "iterations")
+        keys = ("steps", "pairs", "managers", "contact_rows", "friction_rows", "partitions", "iterations", "rounds")
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def managers(self):

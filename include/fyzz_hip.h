@@ -344,7 +344,8 @@ int fyzz_hip_world_fetch_managers(fyzz_hip_world* world, fyzz_dy_manager* manage
                                   uint32_t* partitions, uint32_t capacity, uint32_t* count);
 /* Synchronizes; running totals since creation or the last reset: [0]
  * steps, [1] broadphase pairs, [2] touching managers, [3] contact rows,
- * [4] friction rows, [5] partitions, [6] solver iterations (out[8], HOST).
+ * [4] friction rows, [5] partitions, [6] solver iterations, [7] partition
+ * dependency rounds (out[8], HOST).
  * reset != 0 zeroes them afterwards. */
 int fyzz_hip_world_totals(fyzz_hip_world* world, uint64_t* out, int reset);
 /* Scene::set_contact_notifications (scene.h:654): when enabled, every
