@@ -29,6 +29,7 @@
 
 #include "../../include/fyzz_hip.h"
 #include "context.hpp"
+#include "box_keys.hpp"
 #include "select.hpp"
 #include "stages.hpp"
 
@@ -173,6 +174,89 @@ __global__ void k_scatter_patches(const u32* d_nm, u32 cap_m, const fyzz_dy_patc
         stamp_next[s] = stamp;
     }
 }
+
+// ---- per-scene broadphase (independent scenes in one world)
+//
+// Scene s owns box slots [first[s], first[s+1]). One workgroup per scene
+// computes the boxes' keys (the broadphase's encoding) into LDS and tests
+// every pair of the scene with the broadphase's pair filter (different
+// bodies, at least one dynamic, inclusive 3-axis overlap of the keys,
+// scene.cpp:2606-2612), so the pair set equals the SAP's for that scene.
+// Rows are counted, scanned and written in (lo, hi) order; scenes are
+// concatenated in slot order, so the whole list is sorted like the SAP's.
+constexpr u32 kSceneMax = 1024;  // boxes per scene in this path
+constexpr int kSceneBlock = 256;
+
+__global__ __launch_bounds__(kSceneBlock) void k_scene_pairs(int phase, u32 n_scenes, const u32* first,
+                                                             const float* pos, const float* quat, const float* half,
+                                                             const float* offset, u32* scene_cnt,
+                                                             const u32* scene_base, u64* out, u32 cap, u32* status) {
+    __shared__ u32 key[6][kSceneMax];
+    __shared__ u32 row[kSceneMax];
+    __shared__ u32 scratch[kSceneBlock / kWave + 1];
+    for (u32 sc = blockIdx.x; sc < n_scenes; sc += gridDim.x) {
+        const u32 b0 = first[sc], n = first[sc + 1] - b0;
+        if (n > kSceneMax) {
+            if (threadIdx.x == 0) {
+                atomicOr(&status[0], 4u);
+                if (phase == 0) scene_cnt[sc] = 0;
+            }
+            continue;
+        }
+        for (u32 i = threadIdx.x; i < n; i += kSceneBlock) {
+            u32 k[6];
+            if (!keys::box_keys(pos, quat, half, offset, b0 + i, k)) atomicOr(&status[0], 1u);
+            for (int a = 0; a < 6; ++a) key[a][i] = k[a];
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < n; i += kSceneBlock) {
+            u32 c = 0;
+            const u32 a0 = key[0][i], a1 = key[1][i], a2 = key[2][i], a3 = key[3][i], a4 = key[4][i],
+                      a5 = key[5][i];
+            for (u32 j = i + 1; j < n; ++j)
+                c += !(a0 > key[3][j] || key[0][j] > a3 || a1 > key[4][j] || key[1][j] > a4 || a2 > key[5][j] ||
+                       key[2][j] > a5);
+            row[i] = c;
+        }
+        __syncthreads();
+        // exclusive scan of the row counts in place
+        u32 carry = 0;
+        for (u32 t0 = 0; t0 < n; t0 += kSceneBlock) {
+            const u32 i = t0 + threadIdx.x;
+            const u32 v = i < n ? row[i] : 0u;
+            u32 total;
+            const u32 ex = block_exclusive_scan<kSceneBlock>(v, scratch, &total);
+            if (i < n) row[i] = carry + ex;
+            carry += total;
+        }
+        if (phase == 0) {
+            if (threadIdx.x == 0) scene_cnt[sc] = carry;
+        } else {
+            const u32 base = scene_base[sc];
+            if (threadIdx.x == 0 && base + carry > cap) atomicOr(&status[0], 2u);
+            for (u32 i = threadIdx.x; i < n; i += kSceneBlock) {
+                u32 o = base + row[i];
+                const u32 a0 = key[0][i], a1 = key[1][i], a2 = key[2][i], a3 = key[3][i], a4 = key[4][i],
+                          a5 = key[5][i];
+                for (u32 j = i + 1; j < n; ++j) {
+                    const bool ov = !(a0 > key[3][j] || key[0][j] > a3 || a1 > key[4][j] || key[1][j] > a4 ||
+                                      a2 > key[5][j] || key[2][j] > a5);
+                    if (!ov) continue;
+                    if (o < cap) out[o] = (static_cast<u64>(b0 + i) << 32) | (b0 + j);
+                    ++o;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+struct SceneCount {
+    const u32* cnt;
+    __device__ u32 operator()(u32 i) const { return cnt[i]; }
+};
+
+__global__ void k_clamp(u32* d_n, u32 cap) { *d_n = min(*d_n, cap); }
 
 // ---- contact events (Scene::publish_contact_events, scene.cpp:9149-9259;
 // touch transitions of update_contacts, 4640-4731)
@@ -438,6 +522,12 @@ struct fyzz_hip_world {
     DevBuf<fyzz_dy_manager> managers;
     DevBuf<fyzz_dy_patch> patches, hist[2];
     DevBuf<u64> totals;
+    // independent scenes (per-scene broadphase instead of one SAP)
+    std::vector<u32> scene_counts;
+    u32 n_scenes = 0;
+    int scur = 0;
+    DevBuf<u32> scene_first, scene_cnt, scene_base, scene_n, scene_status;
+    DevBuf<u64> scene_pairs[2];
     // contact events
     bool notify = false;
     DevBuf<u32> shape, touch_bb[2], touch_pl, slot_mgr, s1, s2, s3, ev_counts, ev_src, ev_pts, ev_off, d_npts;
@@ -543,15 +633,70 @@ void upload(fyzz_hip_world* w) {
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     }
     FYZZ_HIP_CHECK(hipMemcpyAsync(w->statics.p, &w->floor, sizeof(fyzz_dy_static), hipMemcpyHostToDevice, st));
-    std::vector<u32> ids(n);
-    std::vector<uint8_t> dyn(n, 1);
-    for (u32 i = 0; i < n; ++i) ids[i] = i;
-    int rc = fyzz_hip_sap_create(w->ctx, n, ids.data(), dyn.data(), w->pair_cap, w->pair_cap, &w->sap);
-    if (rc) throw Error(rc, w->ctx->last_error);
+    int rc = 0;
+    if (w->n_scenes) {
+        u64 total = 0;
+        std::vector<u32> first(w->n_scenes + 1);
+        for (u32 k = 0; k < w->n_scenes; ++k) {
+            first[k] = static_cast<u32>(total);
+            FYZZ_REQUIRE(w->scene_counts[k] <= world::kSceneMax, kInvalidArgument,
+                         "world: a scene holds more boxes than the per-scene broadphase takes (1024)");
+            total += w->scene_counts[k];
+        }
+        first[w->n_scenes] = static_cast<u32>(total);
+        FYZZ_REQUIRE(total == n, kInvalidArgument, "world: scene box counts do not add up to the boxes authored");
+        w->scene_first.reserve(w->n_scenes + 1);
+        w->scene_cnt.reserve(w->n_scenes);
+        w->scene_base.reserve(w->n_scenes);
+        w->scene_n.reserve(4);
+        w->scene_status.reserve(4);
+        w->scene_pairs[0].reserve(w->pair_cap);
+        w->scene_pairs[1].reserve(w->pair_cap);
+        FYZZ_HIP_CHECK(hipMemcpyAsync(w->scene_first.p, first.data(), sizeof(u32) * first.size(),
+                                      hipMemcpyHostToDevice, st));
+        FYZZ_HIP_CHECK(hipMemsetAsync(w->scene_n.p, 0, 4 * sizeof(u32), st));
+        FYZZ_HIP_CHECK(hipMemsetAsync(w->scene_status.p, 0, 4 * sizeof(u32), st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    } else {
+        std::vector<u32> ids(n);
+        std::vector<uint8_t> dyn(n, 1);
+        for (u32 i = 0; i < n; ++i) ids[i] = i;
+        rc = fyzz_hip_sap_create(w->ctx, n, ids.data(), dyn.data(), w->pair_cap, w->pair_cap, &w->sap);
+        if (rc) throw Error(rc, w->ctx->last_error);
+    }
     rc = fyzz_hip_dy_create(w->ctx, n, w->mgr_cap, &w->dy);
     if (rc) throw Error(rc, w->ctx->last_error);
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     w->uploaded = true;
+}
+
+sap::Lists sap_broadphase(fyzz_hip_world* w) {
+    const int rc = fyzz_hip_sap_advance(w->sap, w->pos.p, w->quat.p, w->half.p, w->offset.p, w->n, nullptr, 0);
+    if (rc) throw Error(rc, w->ctx->last_error);
+    return sap::lists(w->sap);
+}
+
+sap::Lists scene_broadphase(fyzz_hip_world* w) {
+    using namespace fyzz_hip::world;
+    hipStream_t st = w->ctx->stream;
+    const int cur = w->scur, nxt = cur ^ 1;
+    const u32 S = w->n_scenes, cap = w->pair_cap;
+    ProfScope ps(&w->ctx->prof, "world.scene_pairs", st);
+    const u32 grid = std::min<u32>(S, 4096u);
+    hipLaunchKernelGGL(k_scene_pairs, dim3(grid), dim3(kSceneBlock), 0, st, 0, S, w->scene_first.p, w->pos.p,
+                       w->quat.p, w->half.p, w->offset.p, w->scene_cnt.p, w->scene_base.p, w->scene_pairs[nxt].p,
+                       cap, w->scene_status.p);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->scene_n.p + 2, 1u, S);
+    sel::exclusive_scan(st, S, w->scene_n.p + 2, SceneCount{w->scene_cnt.p}, w->scene_base.p, w->tiles.p,
+                        w->scene_n.p + nxt);
+    hipLaunchKernelGGL(k_scene_pairs, dim3(grid), dim3(kSceneBlock), 0, st, 1, S, w->scene_first.p, w->pos.p,
+                       w->quat.p, w->half.p, w->offset.p, w->scene_cnt.p, w->scene_base.p, w->scene_pairs[nxt].p,
+                       cap, w->scene_status.p);
+    hipLaunchKernelGGL(k_clamp, dim3(1), dim3(1), 0, st, w->scene_n.p + nxt, cap);
+    FYZZ_HIP_CHECK(hipGetLastError());
+    w->scur = nxt;
+    return {w->scene_pairs[nxt].p, w->scene_n.p + nxt, w->scene_pairs[cur].p, w->scene_n.p + cur, cap,
+            w->scene_status.p, w->scene_status.p + 1, nullptr, nullptr};
 }
 
 void publish_events(fyzz_hip_world* w, const sap::Lists& L, int c_prev, int c_cur) {
@@ -612,9 +757,7 @@ void step(fyzz_hip_world* w, float dt) {
     if (!n) return;
     const u32 cap = w->pair_cap;
     hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, st, n, w->bodies.p, w->pos.p, w->quat.p);
-    int rc = fyzz_hip_sap_advance(w->sap, w->pos.p, w->quat.p, w->half.p, w->offset.p, n, nullptr, 0);
-    if (rc) throw Error(rc, w->ctx->last_error);
-    const sap::Lists L = sap::lists(w->sap);
+    const sap::Lists L = w->n_scenes ? scene_broadphase(w) : sap_broadphase(w);
     hipLaunchKernelGGL(k_fill, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, w->counted.p, n, 0u);
     hipLaunchKernelGGL(k_pairs, dim3(std::min<u32>(grid_for(cap, 256), 4096u)), dim3(256), 0, st, cap, L.cur, L.d_cur,
                        L.prev, L.d_prev, w->first.p, w->second.p, w->src.p, w->counted.p);
@@ -840,8 +983,15 @@ int fyzz_hip_world_stats(fyzz_hip_world* w, uint32_t* n_pairs, uint32_t* n_manag
     return guarded(w->ctx, [&] {
         u32 np = 0, nm = 0, parts = 0, dflags = 0, rounds = 0;
         if (w->uploaded && w->n) {
-            const sap::Lists L = sap::lists(w->sap);
             u32 err = 0, ovf = 0;
+            sap::Lists L{};
+            if (w->n_scenes) {
+                L.d_cur = w->scene_n.p + w->scur;
+                L.d_error = w->scene_status.p;
+                L.d_overflow = w->scene_status.p;
+            } else {
+                L = sap::lists(w->sap);
+            }
             hipStream_t st = w->ctx->stream;
             FYZZ_HIP_CHECK(hipMemcpyAsync(&np, L.d_cur, 4, hipMemcpyDeviceToHost, st));
             FYZZ_HIP_CHECK(hipMemcpyAsync(&err, L.d_error, 4, hipMemcpyDeviceToHost, st));
@@ -896,7 +1046,22 @@ int fyzz_hip_world_totals(fyzz_hip_world* w, uint64_t* out, int reset) {
 
 int fyzz_hip_world_set_contact_notifications(fyzz_hip_world* w, int enabled) {
     if (!w) return kInvalidArgument;
-    return guarded(w->ctx, [&] { w->notify = enabled != 0; });
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(!(enabled && w->n_scenes), kLogicError,
+                     "world: contact notifications are published for single-scene worlds");
+        w->notify = enabled != 0;
+    });
+}
+
+int fyzz_hip_world_set_scenes(fyzz_hip_world* w, uint32_t n_scenes, const uint32_t* box_counts) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(n_scenes == 0 || box_counts, kInvalidArgument, "world: null scene box counts");
+        FYZZ_REQUIRE(!w->notify, kLogicError, "world: contact notifications are published for single-scene worlds");
+        w->scene_counts.assign(box_counts, box_counts + n_scenes);
+        w->n_scenes = n_scenes;
+    });
 }
 
 int fyzz_hip_world_contact_events(fyzz_hip_world* w, fyzz_contact_event* events, uint32_t capacity,

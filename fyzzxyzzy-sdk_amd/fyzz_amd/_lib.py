@@ -60,6 +60,7 @@ SIGNATURES = {
     "fyzz_hip_world_set_body_pose": (C.c_int, [vp, C.c_uint32, vp, vp]),
     "fyzz_hip_world_set_body_velocity": (C.c_int, [vp, C.c_uint32, vp, vp]),
     "fyzz_hip_world_set_body_material": (C.c_int, [vp, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+    "fyzz_hip_world_set_scenes": (C.c_int, [vp, C.c_uint32, vp]),
     "fyzz_hip_world_step": (C.c_int, [vp, C.c_float]),
     "fyzz_hip_world_body_records": (C.c_int, [vp, vp, C.c_uint32, u32p]),
     "fyzz_hip_world_stats": (C.c_int, [vp, u32p, u32p, u32p, u32p]),

@@ -215,7 +215,7 @@ def _pile_centers_native(seed, radius, footprint, height):
     used = C.c_uint64()
     rc = lib.fyzz_scene_pile_centers(C.c_uint64(seed), C.c_uint32(n), r.ctypes.data_as(C.c_void_p),
                                      C.c_double(footprint), C.c_double(height), pos.ctypes.data_as(C.c_void_p),
-                                     C.c_uint64(1 << 40), C.byref(used))
+                                     C.c_uint64(1 << 31), C.byref(used))
     if rc != 0:
         raise RuntimeError("pile placement did not finish")
     return pos
@@ -277,3 +277,22 @@ def box_mass_properties(half: np.ndarray, density: float):
     inertia = np.stack([((ey + ez) * unit_scale) * d, ((ex + ez) * unit_scale) * d, ((ex + ey) * unit_scale) * d], 1)
     inv_inertia = (f(1.0) / inertia).astype(np.float32)
     return inv_mass, inv_inertia
+
+
+def batch_scenes(n_scenes: int = 4096, seed: int = SEED_BASE + 3, min_boxes: int = 8, max_boxes: int = 512,
+                 height: float = 200.0, footprint_per_100k: float = 100.0):
+    """Independent piles for the batched form: scene k has
+    n_k = min_boxes + floor(u * (max_boxes - min_boxes + 1)) boxes (stream 0
+    of `seed`, index k), each a pile_scene of seed (seed << 20) + k with the
+    footprint scaled by sqrt(n_k / 100,000) and the column height kept.
+    Returns (counts, half, pos, quat) concatenated in scene order."""
+    u = uniform(seed, 0, n_scenes)
+    counts = (min_boxes + np.floor(u * (max_boxes - min_boxes + 1))).astype(np.int64)
+    halves, poss, quats = [], [], []
+    for k, n in enumerate(counts):
+        fp = footprint_per_100k * math.sqrt(n / 100000.0)
+        sc = pile_scene(int(n), seed=(seed << 20) + k, footprint=fp, height=height)
+        halves.append(sc.half)
+        poss.append(sc.pos)
+        quats.append(sc.quat)
+    return (counts.astype(np.uint32), np.concatenate(halves), np.concatenate(poss), np.concatenate(quats))

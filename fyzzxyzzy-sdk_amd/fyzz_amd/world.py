@@ -104,6 +104,11 @@ class World:
         self._call("fyzz_hip_world_set_body_material", C.c_uint32(body), C.c_float(static_friction),
                    C.c_float(dynamic_friction), C.c_float(restitution))
 
+    def set_scenes(self, box_counts):
+        """Independent scenes: scene k owns the next box_counts[k] boxes."""
+        c = np.ascontiguousarray(box_counts, np.uint32)
+        self._call("fyzz_hip_world_set_scenes", C.c_uint32(len(c)), c.ctypes.data_as(vp))
+
     def step(self, dt: float):
         self._call("fyzz_hip_world_step", C.c_float(dt))
 

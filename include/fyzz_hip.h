@@ -328,6 +328,15 @@ int fyzz_hip_world_set_body_velocity(fyzz_hip_world* world, uint32_t body_id, co
                                      const float* angular);
 int fyzz_hip_world_set_body_material(fyzz_hip_world* world, uint32_t body_id, float static_friction,
                                      float dynamic_friction, float restitution);
+/* Independent scenes in one world (the batched form): scene k owns the
+ * next box_counts[k] dynamic boxes in creation order (at most 1024 each);
+ * boxes of different scenes never pair. The scenes share the floor plane
+ * and its material. The broadphase then runs per scene (one workgroup per
+ * scene, every pair of the scene tested with the broadphase's filter, so
+ * each scene's pair set equals the SAP's), and every later stage runs once
+ * over all scenes. Each scene steps exactly as a world holding only that
+ * scene would. Before the first step; no contact notifications. */
+int fyzz_hip_world_set_scenes(fyzz_hip_world* world, uint32_t n_scenes, const uint32_t* box_counts);
 /* World::step (world.h:459); asynchronous. */
 int fyzz_hip_world_step(fyzz_hip_world* world, float dt);
 /* Synchronizes; copies the dynamic bodies' records (creation order) to

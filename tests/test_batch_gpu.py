@@ -1,0 +1,56 @@
+"""GPU: the batched form (independent scenes in one device world, per-scene
+broadphase, one launch per stage over all scenes) against worlds that hold
+one scene each. Scenes never interact, the per-scene broadphase uses the
+broadphase's own key encoding and pair filter, and the manager order keeps
+each scene's relative order, so every scene must step bit-identically to
+its own world (the bar is 1e-4 relative on velocity; the test asks for
+equality)."""
+import numpy as np
+import pytest
+
+from fyzz_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def make(ctx, half, pos, quat, counts=None):
+    from fyzz_amd.world import World
+    w = World(ctx, len(half) + 1)
+    fid = w.add_static_floor(0.0)
+    w.set_body_material(fid, 0.5, 0.5, 0.0)
+    w.add_dynamic_boxes(half, pos, quat, friction=0.5, restitution=0.0)
+    if counts is not None:
+        w.set_scenes(counts)
+    return w
+
+
+def test_batched_scenes_match_single_scene_worlds(ctx, torch_cuda):
+    counts, half, pos, quat = scenes.batch_scenes(6, seed=scenes.SEED_BASE + 50, min_boxes=3, max_boxes=120,
+                                                  height=3.0, footprint_per_100k=1500.0)
+    first = np.concatenate([[0], np.cumsum(counts)])
+    batched = make(ctx, half, pos, quat, counts)
+    singles = [make(ctx, half[a:b], pos[a:b], quat[a:b]) for a, b in zip(first[:-1], first[1:])]
+    for s in range(90):
+        batched.step(1 / 120)
+        for w in singles:
+            w.step(1 / 120)
+        st = batched.stats()
+        assert st["flags"] == 0, st
+        single = [w.stats() for w in singles]
+        assert st["pairs"] == sum(x["pairs"] for x in single), f"step {s}"
+        assert st["managers"] == sum(x["managers"] for x in single), f"step {s}"
+        got = batched.body_states()
+        for k, w in enumerate(singles):
+            np.testing.assert_array_equal(got[first[k]:first[k + 1]], w.body_states(),
+                                          err_msg=f"step {s} scene {k} ({counts[k]} boxes)")
+    assert batched.stats()["managers"] > 0
+    print("scenes", counts.tolist(), "managers at the end", batched.stats()["managers"])
+
+
+def test_scene_too_large_is_rejected(ctx, torch_cuda):
+    from fyzz_amd import FyzzError
+    counts, half, pos, quat = scenes.batch_scenes(1, min_boxes=1100, max_boxes=1100, height=50.0)
+    w = make(ctx, half, pos, quat, counts)
+    with pytest.raises(FyzzError) as e:
+        w.step(1 / 120)
+    assert e.value.code == -1
