@@ -114,7 +114,7 @@ __device__ inline u32 mcount(const Ws& W) { return W.d_nm ? min(ld(W.d_nm), W.nm
 // barrier are visible after it. Thread 0 of each workgroup keeps the
 // generation it waits on. The spin is bounded; exceeding it (workgroups
 // not co-resident) raises a flag.
-constexpr u32 kSpinLimit = 1u << 24;
+constexpr u32 kSpinLimit = 1u << 20;
 constexpr u32 kBarGroup = 8;
 constexpr u32 kBarWords = 64;  // root, generation, up to 62 groups
 __device__ inline u32 barrier_start(const Ws& W) {
@@ -1172,6 +1172,7 @@ struct fyzz_hip_dy {
     fyzz_hip_context* ctx = nullptr;
     u32 body_cap = 0, mgr_cap = 0;
     int coop_blocks_partition = 0, coop_blocks_pgs = 0;
+    bool plain_launch = false;
     DevBuf<float4> pv, pw, lin, ang, mlin, mang, rpos, rq, sii;
     DevBuf<u32> has_row, deg, off, fill, max_dyn, body_mask;
     DevBuf<u32> inc, pred, succ, cnt, part, mask_after, front, pend, sorted, order;
@@ -1184,22 +1185,27 @@ struct fyzz_hip_dy {
 
 namespace {
 
-int coop_blocks(const void* kernel, int device) {
+int coop_blocks(const void* kernel, int device, int wanted = dy::kCoopBlocks) {
     int per_cu = 0, cus = 0;
     FYZZ_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, dy::kBlock, 0));
     FYZZ_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     FYZZ_REQUIRE(per_cu > 0 && cus > 0, kDeviceError, "dy: cooperative kernel does not fit a compute unit");
     // 64 resident workgroups: a grid barrier costs ~3 us there, against
     // ~17 us at 512 (the barrier is the Dy kernels' critical path)
-    const int blocks = std::min(per_cu * cus, dy::kCoopBlocks);
-    FYZZ_REQUIRE(blocks <= static_cast<int>(dy::kBarGroup * (dy::kBarWords - 2)), kLogicError,
-                 "dy: too many workgroups for the grid barrier");
-    return blocks;
+    // resident capacity, and the grid barrier's group counters
+    return std::min({per_cu * cus, wanted, static_cast<int>(dy::kBarGroup * (dy::kBarWords - 2))});
 }
 
-void launch_coop(const void* kernel, int blocks, dy::Ws& ws, hipStream_t st) {
+// Cooperative launch (the runtime checks that every workgroup is resident).
+// Inside a captured graph the kernels are launched as plain kernels with
+// the same small grid (far below the device's resident capacity); the grid
+// barrier's bounded spin turns a non-resident grid into a status flag.
+void launch_coop(const void* kernel, int blocks, dy::Ws& ws, hipStream_t st, bool plain) {
     void* args[] = {&ws};
-    FYZZ_HIP_CHECK(hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(dy::kBlock), args, 0, st));
+    if (plain)
+        FYZZ_HIP_CHECK(hipLaunchKernel(kernel, dim3(blocks), dim3(dy::kBlock), args, 0, st));
+    else
+        FYZZ_HIP_CHECK(hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(dy::kBlock), args, 0, st));
 }
 
 void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uint32_t ns,
@@ -1239,7 +1245,7 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
     }
     {
         ProfScope ps(&prof, "dy.partition", st);
-        launch_coop(reinterpret_cast<const void*>(&dy::k_partition), d->coop_blocks_partition, w, st);
+        launch_coop(reinterpret_cast<const void*>(&dy::k_partition), d->coop_blocks_partition, w, st, d->plain_launch);
     }
     if (!partition_only && nm) {
         ProfScope ps(&prof, "dy.prep", st);
@@ -1253,7 +1259,8 @@ void dy_solve(fyzz_hip_dy* d) {
     FYZZ_REQUIRE(d->prepared, kLogicError, "dy_solve: no prepared step");
     if (!d->ws.nm) return;
     ProfScope ps(&d->ctx->prof, "dy.pgs", d->ctx->stream);
-    launch_coop(reinterpret_cast<const void*>(&dy::k_pgs), d->coop_blocks_pgs, d->ws, d->ctx->stream);
+    launch_coop(reinterpret_cast<const void*>(&dy::k_pgs), d->coop_blocks_pgs, d->ws, d->ctx->stream,
+                d->plain_launch);
 }
 
 void dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
@@ -1274,6 +1281,21 @@ void dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
 
 namespace fyzz_hip::dy {
 const u32* counters(fyzz_hip_dy* d) { return d->ctr.get(); }
+void set_plain_launch(fyzz_hip_dy* d, bool plain) { d->plain_launch = plain; }
+void set_coop_blocks(fyzz_hip_dy* d, int partition_blocks, int pgs_blocks) {
+    const int cap_p = coop_blocks(reinterpret_cast<const void*>(&k_partition), d->ctx->device, 1 << 30);
+    const int cap_s = coop_blocks(reinterpret_cast<const void*>(&k_pgs), d->ctx->device, 1 << 30);
+    FYZZ_REQUIRE(partition_blocks > 0 && pgs_blocks > 0, kInvalidArgument, "dy: workgroup counts must be positive");
+    d->coop_blocks_partition = std::min(partition_blocks, cap_p);
+    d->coop_blocks_pgs = std::min(pgs_blocks, cap_s);
+    FYZZ_REQUIRE(2 * static_cast<std::size_t>(std::max(d->coop_blocks_partition, d->coop_blocks_pgs)) + 2 <=
+                     d->blocksums.cap,
+                 kInvalidArgument, "dy: workgroup count beyond the workspace");
+}
+void coop_blocks_of(fyzz_hip_dy* d, int* partition_blocks, int* pgs_blocks) {
+    *partition_blocks = d->coop_blocks_partition;
+    *pgs_blocks = d->coop_blocks_pgs;
+}
 
 void step_device(fyzz_hip_dy* d, float dt, u32 nb, fyzz_dy_body* bodies, u32 ns, const fyzz_dy_static* statics,
                  u32 nm_cap, const u32* d_nm, const fyzz_dy_manager* managers, const fyzz_dy_patch* history,
@@ -1313,7 +1335,8 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         FYZZ_HIP_CHECK(hipMemset(d->bar.get(), 0, dy::kBarWords * sizeof(u32)));
         d->coop_blocks_partition = coop_blocks(reinterpret_cast<const void*>(&dy::k_partition), ctx->device);
         d->coop_blocks_pgs = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs), ctx->device);
-        d->blocksums.reserve(2 * std::max(d->coop_blocks_partition, d->coop_blocks_pgs) + 2);
+        // room for the largest grid the autotuner may pick
+        d->blocksums.reserve(2 * 512 + 2);
         FYZZ_HIP_CHECK(hipMemset(d->flags.get(), 0, sizeof(u32)));
         FYZZ_HIP_CHECK(hipMemset(d->ctr.get(), 0, dy::kCtrWords * sizeof(u32)));
         dy::Ws& w = d->ws;

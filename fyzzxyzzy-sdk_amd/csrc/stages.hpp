@@ -53,6 +53,11 @@ namespace dy {
 // friction rows.
 const u32* counters(fyzz_hip_dy* d);
 enum : u32 { kCtrParts = 4, kCtrPos = 5, kCtrVel = 6, kCtrRounds = 7, kCtrContacts = 8, kCtrFriction = 9 };
+// Launch the grid-barrier kernels as plain kernels (for graph capture).
+void set_plain_launch(fyzz_hip_dy* d, bool plain);
+// Workgroups of the partition and PGS launches (clamped to residency).
+void set_coop_blocks(fyzz_hip_dy* d, int partition_blocks, int pgs_blocks);
+void coop_blocks_of(fyzz_hip_dy* d, int* partition_blocks, int* pgs_blocks);
 // fyzz_hip_dy_step with the manager count read from d_nm (at most nm_cap).
 void step_device(fyzz_hip_dy* d, float dt, u32 nb, fyzz_dy_body* bodies, u32 ns, const fyzz_dy_static* statics,
                  u32 nm_cap, const u32* d_nm, const fyzz_dy_manager* managers, const fyzz_dy_patch* history,

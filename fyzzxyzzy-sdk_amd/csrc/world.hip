@@ -96,6 +96,8 @@ __global__ void k_counted(u32 n, fyzz_dy_body* bodies, const u32* counted, u32 p
     if (i < n) bodies[i].counted_interactions = counted[i] + planes;
 }
 
+__global__ void k_step_begin(u32* d_step) { *d_step += 1u; }
+
 __global__ void k_total(const u32* d_nbb, u32 cap, u32 extra, u32* d_total) {
     *d_total = min(*d_nbb, cap) + extra;
 }
@@ -122,11 +124,13 @@ struct MgrWrite {
     const u32* counts_pl;
     const float* contacts_pl;
     const u32* stamp_prev;
-    u32 want_stamp;  // the previous step's stamp (kNone on the first step)
+    const u32* d_step;  // this step's number (1, 2, ...); history must carry the previous one
     fyzz_dy_manager* managers;
     u32* mgr_slot;
     __device__ void operator()(u32 i, u32 o) const {
         const u32 nbb = min(*d_nbb, cap);
+        const u32 step = *d_step;
+        const u32 want_stamp = step > 1u ? step - 1u : kNone;
         fyzz_dy_manager M;
         const float* c;
         u32 slot, hist_slot;
@@ -166,8 +170,9 @@ struct MgrWrite {
 
 // friction patches solved this step, keyed for the next step's lookup
 __global__ void k_scatter_patches(const u32* d_nm, u32 cap_m, const fyzz_dy_patch* patches, const u32* mgr_slot,
-                                  fyzz_dy_patch* hist_next, u32* stamp_next, u32 stamp) {
+                                  fyzz_dy_patch* hist_next, u32* stamp_next, const u32* d_step) {
     const u32 n = min(*d_nm, cap_m);
+    const u32 stamp = *d_step;
     for (u32 m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
         const u32 s = mgr_slot[m];
         hist_next[s] = patches[m];
@@ -522,6 +527,12 @@ struct fyzz_hip_world {
     DevBuf<fyzz_dy_manager> managers;
     DevBuf<fyzz_dy_patch> patches, hist[2];
     DevBuf<u64> totals;
+    DevBuf<u32> d_step;
+    // graph stepping: one graph per parity of the double-buffer toggles
+    hipStream_t gstream = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    float gdt[2] = {0.0f, 0.0f};
+    hipEvent_t gev_in = nullptr, gev_out = nullptr;
     // independent scenes (per-scene broadphase instead of one SAP)
     std::vector<u32> scene_counts;
     u32 n_scenes = 0;
@@ -537,6 +548,11 @@ struct fyzz_hip_world {
     u32 ev_cap = 0, pt_cap = 0;
 
     ~fyzz_hip_world() {
+        for (auto& g : gexec)
+            if (g) (void)hipGraphExecDestroy(g);
+        if (gev_in) (void)hipEventDestroy(gev_in);
+        if (gev_out) (void)hipEventDestroy(gev_out);
+        if (gstream) (void)hipStreamDestroy(gstream);
         if (dy) fyzz_hip_dy_destroy(dy);
         if (sap) fyzz_hip_sap_destroy(sap);
     }
@@ -613,6 +629,8 @@ void upload(fyzz_hip_world* w) {
     }
     w->totals.reserve(8);
     FYZZ_HIP_CHECK(hipMemsetAsync(w->totals.p, 0, 8 * sizeof(u64), st));
+    w->d_step.reserve(1);
+    FYZZ_HIP_CHECK(hipMemsetAsync(w->d_step.p, 0, sizeof(u32), st));
     w->parts.reserve(w->mgr_cap);
     w->impulses.reserve(4ull * w->mgr_cap);
     w->managers.reserve(w->mgr_cap);
@@ -756,6 +774,7 @@ void step(fyzz_hip_world* w, float dt) {
     ++w->step_index;
     if (!n) return;
     const u32 cap = w->pair_cap;
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, w->d_step.p);
     hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, st, n, w->bodies.p, w->pos.p, w->quat.p);
     const sap::Lists L = w->n_scenes ? scene_broadphase(w) : sap_broadphase(w);
     hipLaunchKernelGGL(k_fill, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, w->counted.p, n, 0u);
@@ -776,20 +795,19 @@ void step(fyzz_hip_world* w, float dt) {
     const u32 extra = w->has_floor ? n : 0u;
     hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, st, L.d_cur, cap, extra, w->d_total.p);
     const int h = w->hcur;
-    const u32 want = w->step_index > 1 ? w->step_index - 1 : kNone;
     {
         ProfScope ps(&w->ctx->prof, "world.managers", st);
         MgrPred pred{L.d_cur, cap, w->counts_bb.p, w->counts_pl.p};
         MgrWrite wr{L.d_cur,       cap,          w->first.p,         w->second.p,   w->src.p,
                     w->counts_bb.p, w->contacts_bb.p, w->counts_pl.p, w->contacts_pl.p, w->stamp[h].p,
-                    want,          w->managers.p, w->mgr_slot.p};
+                    w->d_step.p,   w->managers.p, w->mgr_slot.p};
         sel::select(st, w->mgr_cap, w->d_total.p, pred, wr, w->tiles.p, w->d_nm.p);
     }
     dy::step_device(w->dy, dt, n, w->bodies.p, 1, w->statics.p, w->mgr_cap, w->d_nm.p, w->managers.p, w->hist[h].p,
                     w->mgr_cap, w->patches.p, w->impulses.p, w->parts.p);
     hipLaunchKernelGGL(k_scatter_patches, dim3(std::min<u32>(grid_for(w->mgr_cap, 256), 2048u)), dim3(256), 0, st,
                        w->d_nm.p, w->mgr_cap, w->patches.p, w->mgr_slot.p, w->hist[h ^ 1].p, w->stamp[h ^ 1].p,
-                       w->step_index);
+                       w->d_step.p);
     hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(1), 0, st, L.d_cur, cap, w->d_nm.p, dy::counters(w->dy),
                        w->totals.p);
     if (w->notify) publish_events(w, L, c, c2);
@@ -800,9 +818,142 @@ void step(fyzz_hip_world* w, float dt) {
     w->hcur = h ^ 1;
 }
 
+// One step through a captured graph. The step's host-side toggles (pair
+// list, caches, friction history) flip every step, so a graph per parity
+// is captured on first use (or when dt changes) and replayed afterwards;
+// the history stamps come from the device step counter. The graphs run on
+// the world's own stream, ordered after and before the context stream's
+// work by events.
+void step_graph(fyzz_hip_world* w, float dt) {
+    FYZZ_REQUIRE(w->n_scenes, kLogicError,
+                 "world: graph stepping runs the per-scene broadphase (set_scenes); the incremental SAP keeps "
+                 "host-side frame state");
+    if (!w->uploaded) upload(w);
+    if (!w->n) {
+        ++w->step_index;
+        return;
+    }
+    if (!w->gstream) {
+        FYZZ_HIP_CHECK(hipStreamCreateWithFlags(&w->gstream, hipStreamNonBlocking));
+        FYZZ_HIP_CHECK(hipEventCreateWithFlags(&w->gev_in, hipEventDisableTiming));
+        FYZZ_HIP_CHECK(hipEventCreateWithFlags(&w->gev_out, hipEventDisableTiming));
+    }
+    const int parity = static_cast<int>(w->step_index & 1u);
+    hipStream_t outer = w->ctx->stream;
+    FYZZ_HIP_CHECK(hipEventRecord(w->gev_in, outer));
+    FYZZ_HIP_CHECK(hipStreamWaitEvent(w->gstream, w->gev_in, 0));
+    if (!w->gexec[parity] || w->gdt[parity] != dt) {
+        if (w->gexec[parity]) {
+            FYZZ_HIP_CHECK(hipGraphExecDestroy(w->gexec[parity]));
+            w->gexec[parity] = nullptr;
+        }
+        const bool prof = w->ctx->prof.enabled;
+        w->ctx->prof.enabled = false;
+        w->ctx->stream = w->gstream;
+        dy::set_plain_launch(w->dy, true);
+        hipGraph_t graph = nullptr;
+        auto restore = [&] {
+            w->ctx->stream = outer;
+            w->ctx->prof.enabled = prof;
+            dy::set_plain_launch(w->dy, false);
+        };
+        FYZZ_HIP_CHECK(hipStreamBeginCapture(w->gstream, hipStreamCaptureModeThreadLocal));
+        try {
+            step(w, dt);
+        } catch (...) {
+            (void)hipStreamEndCapture(w->gstream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            restore();
+            throw;
+        }
+        const hipError_t e = hipStreamEndCapture(w->gstream, &graph);
+        restore();
+        FYZZ_HIP_CHECK(e);
+        const hipError_t ie = hipGraphInstantiate(&w->gexec[parity], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        FYZZ_HIP_CHECK(ie);
+        w->gdt[parity] = dt;
+    } else {
+        // the toggles a recorded step flips
+        ++w->step_index;
+        w->ccur ^= 1;
+        w->hcur ^= 1;
+        w->scur ^= 1;
+    }
+    FYZZ_HIP_CHECK(hipGraphLaunch(w->gexec[parity], w->gstream));
+    FYZZ_HIP_CHECK(hipEventRecord(w->gev_out, w->gstream));
+    FYZZ_HIP_CHECK(hipStreamWaitEvent(outer, w->gev_out, 0));
+}
+
 }  // namespace
 
 extern "C" {
+
+int fyzz_hip_world_step_graph(fyzz_hip_world* w, float dt) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(std::isfinite(dt) && dt > 0.0f, kInvalidArgument, "world: dt must be positive");
+        step_graph(w, dt);
+    });
+}
+
+// Autotuner: picks the Dy partition and PGS workgroup counts (the grid
+// barrier's cost against the work per sweep) by timing `trial_steps` real
+// steps per candidate; candidates are tried one parameter at a time. The
+// steps advance the simulation (use it in place of warm-up steps); the
+// counts do not change results. Writes the chosen counts.
+int fyzz_hip_world_autotune(fyzz_hip_world* w, float dt, uint32_t trial_steps, int* partition_blocks,
+                            int* pgs_blocks) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(trial_steps > 0, kInvalidArgument, "world: autotune needs at least one step per trial");
+        if (!w->uploaded) upload(w);
+        const bool graph = w->n_scenes != 0;
+        hipStream_t st = w->ctx->stream;
+        hipEvent_t a, b;
+        FYZZ_HIP_CHECK(hipEventCreate(&a));
+        FYZZ_HIP_CHECK(hipEventCreate(&b));
+        auto drop_graphs = [&] {
+            for (auto& g : w->gexec)
+                if (g) {
+                    FYZZ_HIP_CHECK(hipGraphExecDestroy(g));
+                    g = nullptr;
+                }
+        };
+        auto trial = [&](int pb, int sb) {
+            dy::set_coop_blocks(w->dy, pb, sb);
+            drop_graphs();  // graphs hold their launch shapes
+            for (u32 k = 0; k < 2; ++k) graph ? step_graph(w, dt) : step(w, dt);  // settle both graphs
+            FYZZ_HIP_CHECK(hipEventRecord(a, st));
+            for (u32 k = 0; k < trial_steps; ++k) graph ? step_graph(w, dt) : step(w, dt);
+            FYZZ_HIP_CHECK(hipEventRecord(b, st));
+            FYZZ_HIP_CHECK(hipEventSynchronize(b));
+            float ms = 0.f;
+            FYZZ_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+            return ms;
+        };
+        const int cand[] = {16, 32, 64, 128, 256};
+        int best_p = 64, best_s = 64;
+        float best = 3.4e38f;
+        for (int c : cand) {
+            const float t = trial(c, best_s);
+            if (t < best) best = t, best_p = c;
+        }
+        best = 3.4e38f;
+        for (int c : cand) {
+            const float t = trial(best_p, c);
+            if (t < best) best = t, best_s = c;
+        }
+        dy::set_coop_blocks(w->dy, best_p, best_s);
+        drop_graphs();
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        int pb = 0, sb = 0;
+        dy::coop_blocks_of(w->dy, &pb, &sb);
+        if (partition_blocks) *partition_blocks = pb;
+        if (pgs_blocks) *pgs_blocks = sb;
+    });
+}
 
 int fyzz_hip_world_create(fyzz_hip_context* ctx, uint32_t max_bodies, fyzz_hip_world** out) {
     if (!out) return kInvalidArgument;

@@ -112,6 +112,17 @@ class World:
     def step(self, dt: float):
         self._call("fyzz_hip_world_step", C.c_float(dt))
 
+    def step_graph(self, dt: float):
+        """A step through the captured graphs (worlds with scenes)."""
+        self._call("fyzz_hip_world_step_graph", C.c_float(dt))
+
+    def autotune(self, dt: float, trial_steps: int = 3):
+        """Times real steps per launch shape and keeps the fastest; returns
+        (partition workgroups, PGS workgroups)."""
+        pb, sb = C.c_int(), C.c_int()
+        self._call("fyzz_hip_world_autotune", C.c_float(dt), C.c_uint32(trial_steps), C.byref(pb), C.byref(sb))
+        return pb.value, sb.value
+
     def body_records(self):
         out = np.zeros(max(self.n_dynamic, 1), DY_BODY)
         cnt = C.c_uint32()

@@ -339,6 +339,17 @@ int fyzz_hip_world_set_body_material(fyzz_hip_world* world, uint32_t body_id, fl
 int fyzz_hip_world_set_scenes(fyzz_hip_world* world, uint32_t n_scenes, const uint32_t* box_counts);
 /* World::step (world.h:459); asynchronous. */
 int fyzz_hip_world_step(fyzz_hip_world* world, float dt);
+/* One World::step through a captured HIP graph (worlds with scenes,
+ * fyzz_hip_world_set_scenes): the first two calls capture one graph each
+ * (the step's double buffers alternate), later calls replay them; a new dt
+ * recaptures. Results equal fyzz_hip_world_step's. Asynchronous. */
+int fyzz_hip_world_step_graph(fyzz_hip_world* world, float dt);
+/* Autotuner of the launch shapes (no counterpart in the reference): times
+ * trial_steps real steps per candidate workgroup count of the partition
+ * and PGS launches and keeps the fastest; the steps advance the world.
+ * Launch shapes do not change results. Writes the chosen counts. */
+int fyzz_hip_world_autotune(fyzz_hip_world* world, float dt, uint32_t trial_steps, int* partition_blocks,
+                            int* pgs_blocks);
 /* Synchronizes; copies the dynamic bodies' records (creation order) to
  * HOST out[capacity]; *count = number of dynamic bodies. */
 int fyzz_hip_world_body_records(fyzz_hip_world* world, fyzz_dy_body* out, uint32_t capacity, uint32_t* count);

@@ -27,7 +27,7 @@ def make(ctx, half, pos, quat, counts=None):
 def test_batched_scenes_match_single_scene_worlds(ctx, torch_cuda):
     counts, half, pos, quat = scenes.batch_scenes(6, seed=scenes.SEED_BASE + 50, min_boxes=3, max_boxes=120,
                                                   height=3.0, footprint_per_100k=1500.0)
-    first = np.concatenate([[0], np.cumsum(counts)])
+    first = np.concatenate([[0], np.cumsum(counts.astype(np.int64))]).astype(np.int64)
     batched = make(ctx, half, pos, quat, counts)
     singles = [make(ctx, half[a:b], pos[a:b], quat[a:b]) for a, b in zip(first[:-1], first[1:])]
     for s in range(90):
@@ -49,8 +49,44 @@ def test_batched_scenes_match_single_scene_worlds(ctx, torch_cuda):
 
 def test_scene_too_large_is_rejected(ctx, torch_cuda):
     from fyzz_amd import FyzzError
-    counts, half, pos, quat = scenes.batch_scenes(1, min_boxes=1100, max_boxes=1100, height=50.0)
+    counts, half, pos, quat = scenes.batch_scenes(1, min_boxes=1100, max_boxes=1100, height=50.0,
+                                                  footprint_per_100k=3000.0)
     w = make(ctx, half, pos, quat, counts)
     with pytest.raises(FyzzError) as e:
         w.step(1 / 120)
     assert e.value.code == -1
+
+
+def test_graph_steps_match_eager_steps(ctx, torch_cuda):
+    """Graph-captured steps (two alternating graphs, replayed) against
+    eager steps of an identical world, with eager work and synchronization
+    between replays; every replay is checked."""
+    counts, half, pos, quat = scenes.batch_scenes(5, seed=scenes.SEED_BASE + 51, min_boxes=3, max_boxes=80,
+                                                  height=3.0, footprint_per_100k=1500.0)
+    eager = make(ctx, half, pos, quat, counts)
+    graphed = make(ctx, half, pos, quat, counts)
+    for s in range(40):
+        eager.step(1 / 120)
+        graphed.step_graph(1 / 120)
+        a, b = eager.body_states(), graphed.body_states()
+        np.testing.assert_array_equal(b, a, err_msg=f"step {s}")
+        assert graphed.stats() == eager.stats(), f"step {s}"
+    assert eager.totals()["managers"] > 0
+
+
+def test_autotune_keeps_results(ctx, torch_cuda):
+    """Launch shapes picked by the autotuner do not change results: a tuned
+    world and an untuned one stepped the same number of times agree."""
+    counts, half, pos, quat = scenes.batch_scenes(4, seed=scenes.SEED_BASE + 52, min_boxes=10, max_boxes=60,
+                                                  height=3.0, footprint_per_100k=1500.0)
+    tuned = make(ctx, half, pos, quat, counts)
+    plain = make(ctx, half, pos, quat, counts)
+    pb, sb = tuned.autotune(1 / 120, trial_steps=2)
+    assert pb > 0 and sb > 0
+    steps = tuned.totals()["steps"]
+    for _ in range(steps):
+        plain.step(1 / 120)
+    for _ in range(10):
+        tuned.step_graph(1 / 120)
+        plain.step(1 / 120)
+    np.testing.assert_array_equal(tuned.body_states(), plain.body_states())
