@@ -98,6 +98,23 @@ struct Ws {
     u32* ctr;
     u32* flags;
     u32* bar;  // grid barrier: arrival count, generation
+    // adjoint support (tape mode)
+    u32 tape;             // record branches and raw row vectors
+    u32 tape_iters;       // iterations the tape holds
+    uint8_t* tape_c;      // [iter][4 nm] normal rows: 1 free update, 0 clamped to zero, 2 at the maximum
+    uint8_t* tape_f;      // [iter][4 nm] friction rows: 0 free, 1 clamped inside, 2 at +max, 3 at -max
+    float* tape_ni;       // [iter][nm] the manager's normal impulse sum of the iteration
+    float4* craw;         // [4 nm][2] contact rows: (ra x n, kv), (rb x n, 0); d err / d vrel = kv vm
+    float4* fraw;         // [4 nm][2] friction rows: (ra x t, 0), (rb x t, 0) unscaled
+    // adjoint inputs / outputs and state
+    const float* adj_v_out;
+    const float* adj_w_out;
+    const float* adj_p_out;
+    float* adj_v_in;
+    float* adj_w_in;
+    float* adj_friction;  // per body: d loss / d dynamic friction
+    float4 *a_lin, *a_ang, *a_vpre, *a_wpre;
+    float *a_capp, *a_fapp, *a_err, *a_tgt, *a_df;
 };
 
 __device__ inline u32 ld(const u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
@@ -802,6 +819,11 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
         r.pad0 = r.pad1 = 0.0f;
         bad |= !isfinite(response) || !isfinite(r.biased) || !isfinite(r.unbiased);
         W.crow[4 * m + q] = r;
+        if (W.tape) {
+            // err = target vm - bias with target = -(1 + e) vrel when bouncing, -vrel otherwise
+            W.craw[2 * (4 * m + q)] = make_float4(ra_x_n.x, ra_x_n.y, ra_x_n.z, bouncing ? -(1.0f + restitution) : -1.0f);
+            W.craw[2 * (4 * m + q) + 1] = make_float4(rb_x_n.x, rb_x_n.y, rb_x_n.z, 0.0f);
+        }
     }
     if (bad) raise_flag(W, FYZZ_DY_NONFINITE);
     // anchor growth (scene.cpp:8359-8433)
@@ -897,6 +919,10 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
                     fr.applied = 0.0f;
                     fr.pad0 = fr.pad1 = fr.pad2 = 0.0f;
                     W.frow[4 * m + 2 * k + ti] = fr;
+                    if (W.tape) {
+                        W.fraw[2 * (4 * m + 2 * k + ti)] = make_float4(ax0.x, ax0.y, ax0.z, 0.0f);
+                        W.fraw[2 * (4 * m + 2 * k + ti) + 1] = make_float4(ax1.x, ax1.y, ax1.z, 0.0f);
+                    }
                 }
             }
         }
@@ -939,8 +965,12 @@ __device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 t
         const V ra{r.rax, r.ray, r.raz}, rb{r.rbx, r.rby, r.rbz};
         const float nv = vdot(n, va) - vdot(n, vb) + vdot(wa, ra) - vdot(wb, rb);
         const float err = iter < pos_iters ? r.biased : r.unbiased;
-        const float dc = smax(-nv * r.vm + err, -r.applied);
+        const float x = -nv * r.vm + err;
+        const float dc = smax(x, -r.applied);
         const float next = smin(r.max_impulse, 1.0f * r.applied + dc);
+        if (W.tape && iter < W.tape_iters)
+            W.tape_c[static_cast<std::size_t>(iter) * 4 * W.nm + 4 * m + q] =
+                r.max_impulse < 1.0f * r.applied + dc ? 2 : (x < -r.applied ? 0 : 1);
         const float delta = next - r.applied;
         const float da = delta * ima;
         const float db = delta * imb;
@@ -954,6 +984,7 @@ __device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 t
     }
     fyzz_dy_patch& P = W.patches[m];
     const u32 anchors = P.anchor_count;
+    if (W.tape && iter < W.tape_iters) W.tape_ni[static_cast<std::size_t>(iter) * W.nm + m] = normal_impulse;
     if ((iter >= pos_iters || pos_iters - iter <= 3u) && anchors) {
         const float coef = anchors == 2 ? 0.5f : 1.0f;
         const float max_static = P.static_friction * coef * normal_impulse;
@@ -968,6 +999,9 @@ __device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 t
             const float cand = fr.applied + (fr.target - bias - td) * fr.vm;
             const bool clamped = fabsf(cand) > max_static;
             const float next = clamped ? smin(max_dynamic, smax(-max_dynamic, cand)) : cand;
+            if (W.tape && iter < W.tape_iters)
+                W.tape_f[static_cast<std::size_t>(iter) * 4 * W.nm + 4 * m + k] =
+                    !clamped ? 0 : (max_dynamic < cand ? 2 : (cand < -max_dynamic ? 3 : 1));
             const float delta = next - fr.applied;
             const float da = delta * ima;
             const float db = delta * imb;
@@ -1017,6 +1051,229 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
             grid_sync(W, gen);
         }
     }
+}
+
+// --------------------------------------------------------------- adjoint
+//
+// Reverse mode through one step's PGS and integration with the step's
+// contact rows frozen (normals, lever arms, effective masses, biases and
+// friction frames are constants): every row update is then piecewise
+// linear in the velocities, with the branch recorded in the tape. Inputs:
+// adjoints of the step's outputs (velocity, angular velocity, position);
+// outputs: adjoints of its input velocities and of each body's dynamic
+// friction. Orientation is not differentiated (it reaches the outputs only
+// through the frozen rows).
+
+__device__ inline V ld3(const float* p, u32 i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ inline void atomic_add3(float4* p, V v) {
+    atomicAdd(&p->x, v.x);
+    atomicAdd(&p->y, v.y);
+    atomicAdd(&p->z, v.z);
+}
+
+// adjoint state at the end of the sweeps
+__global__ __launch_bounds__(kBlock) void k_adj_init(Ws W) {
+    const u32 t = blockIdx.x * kBlock + threadIdx.x;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t < W.nb) {
+        const V av = ld3(W.adj_v_out, t), aw = ld3(W.adj_w_out, t);
+        const float4 c0 = W.sii[3 * t], c1 = W.sii[3 * t + 1], c2 = W.sii[3 * t + 2];
+        // v_out = v_pre + lin; w_out = w_pre + S ang (S symmetric)
+        W.a_lin[t] = make_float4(av.x, av.y, av.z, 0.f);
+        W.a_ang[t] = make_float4(c0.x * aw.x + c0.y * aw.y + c0.z * aw.z, c1.x * aw.x + c1.y * aw.y + c1.z * aw.z,
+                                 c2.x * aw.x + c2.y * aw.y + c2.z * aw.z, 0.f);
+        W.a_vpre[t] = z;
+        W.a_wpre[t] = z;
+        W.adj_friction[t] = 0.0f;
+    }
+    const u32 nm = mcount(W);
+    if (t < 4 * nm) {
+        W.a_capp[t] = 0.f;
+        W.a_fapp[t] = 0.f;
+        W.a_err[t] = 0.f;
+        W.a_tgt[t] = 0.f;
+    }
+    if (t < nm) W.a_df[t] = 0.f;
+}
+
+__device__ void adjoint_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters) {
+    const fyzz_dy_manager& M = W.mg[m];
+    const int i0 = M.body0, i1 = M.body1;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float ima = i0 >= 0 ? W.lin[i0].w : 0.f, imb = i1 >= 0 ? W.lin[i1].w : 0.f;
+    V ava = f3(i0 >= 0 ? W.a_lin[i0] : z4), avb = f3(i1 >= 0 ? W.a_lin[i1] : z4);
+    V awa = f3(i0 >= 0 ? W.a_ang[i0] : z4), awb = f3(i1 >= 0 ? W.a_ang[i1] : z4);
+    const fyzz_dy_patch& P = W.patches[m];
+    const u32 anchors = P.anchor_count;
+    const std::size_t row_base = static_cast<std::size_t>(iter) * 4 * W.nm + 4 * m;
+    float a_ni = 0.0f;
+    if ((iter >= pos_iters || pos_iters - iter <= 3u) && anchors) {
+        const float coef = anchors == 2 ? 0.5f : 1.0f;
+        const float ni = W.tape_ni[static_cast<std::size_t>(iter) * W.nm + m];
+        const float df = P.dynamic_friction;
+        float a_maxd = 0.0f;
+        for (int k = static_cast<int>(anchors * 2) - 1; k >= 0; --k) {
+            const FRow& fr = W.frow[4 * m + k];
+            const V t{fr.tx, fr.ty, fr.tz};
+            const V ra{fr.rax, fr.ray, fr.raz}, rb{fr.rbx, fr.rby, fr.rbz};
+            const float a_delta = vdot(t, ava) * ima - vdot(t, avb) * imb + vdot(ra, awa) - vdot(rb, awb);
+            const float a_next = W.a_fapp[4 * m + k] + a_delta;
+            float a_applied = -a_delta;
+            const uint8_t br = W.tape_f[row_base + k];
+            float a_cand = 0.0f;
+            if (br <= 1)
+                a_cand = a_next;
+            else if (br == 2)
+                a_maxd += a_next;
+            else
+                a_maxd -= a_next;
+            a_applied += a_cand;
+            W.a_tgt[4 * m + k] += fr.vm * a_cand;
+            const float a_td = -fr.vm * a_cand;
+            ava = vadd(ava, vmul(t, a_td));
+            avb = vsub(avb, vmul(t, a_td));
+            awa = vadd(awa, vmul(ra, a_td));
+            awb = vsub(awb, vmul(rb, a_td));
+            W.a_fapp[4 * m + k] = a_applied;
+        }
+        // max_dynamic = df * coef * ni
+        W.a_df[m] += coef * ni * a_maxd;
+        a_ni = df * coef * a_maxd;
+    }
+    const u32 no = W.order[m] & 0xffu;
+    for (int q = static_cast<int>(no) - 1; q >= 0; --q) {
+        const CRow& r = W.crow[4 * m + q];
+        const V n{r.nx, r.ny, r.nz};
+        const V ra{r.rax, r.ray, r.raz}, rb{r.rbx, r.rby, r.rbz};
+        const float a_delta = vdot(n, ava) * ima - vdot(n, avb) * imb + vdot(ra, awa) - vdot(rb, awb);
+        const float a_next = W.a_capp[4 * m + q] + a_delta + a_ni;
+        float a_applied = -a_delta;
+        if (W.tape_c[row_base + q] == 1) {
+            a_applied += a_next;
+            const float a_nv = -r.vm * a_next;
+            W.a_err[4 * m + q] += a_next;
+            ava = vadd(ava, vmul(n, a_nv));
+            avb = vsub(avb, vmul(n, a_nv));
+            awa = vadd(awa, vmul(ra, a_nv));
+            awb = vsub(awb, vmul(rb, a_nv));
+        }
+        W.a_capp[4 * m + q] = a_applied;
+    }
+    if (i0 >= 0) {
+        W.a_lin[i0] = make_float4(ava.x, ava.y, ava.z, 0.f);
+        W.a_ang[i0] = make_float4(awa.x, awa.y, awa.z, 0.f);
+    }
+    if (i1 >= 0) {
+        W.a_lin[i1] = make_float4(avb.x, avb.y, avb.z, 0.f);
+        W.a_ang[i1] = make_float4(awb.x, awb.y, awb.z, 0.f);
+    }
+}
+
+// The sweeps in reverse: iterations and partitions backwards. The motion
+// snapshot taken after the last position iteration feeds the position
+// update, so its adjoint (dt x adjoint of the position) joins the
+// velocity-delta adjoint when the reverse reaches that point.
+__global__ __launch_bounds__(kBlock) void k_pgs_adjoint(Ws W) {
+    u32 gen = barrier_start(W);
+    const u32 tid = blockIdx.x * kBlock + threadIdx.x;
+    const u32 nthreads = gridDim.x * kBlock;
+    const u32 np = ld(&W.ctr[kNParts]);
+    const u32 pos_iters = ld(&W.ctr[kPosIters]);
+    const u32 vel_iters = max(ld(&W.ctr[kVelIters]), 1u);
+    const u32 total = pos_iters + vel_iters;
+    if (total > W.tape_iters) {  // the tape does not hold every iteration
+        if (tid == 0) raise_flag(W, FYZZ_DY_OVERFLOW);
+        return;
+    }
+    for (u32 it = total; it-- > 0;) {
+        if (it + 1 == pos_iters) {
+            for (u32 i = tid; i < W.nb; i += nthreads)
+                if (W.has_row[i]) {
+                    const V ap = ld3(W.adj_p_out, i);
+                    float4 a = W.a_lin[i];
+                    a.x += W.dt * ap.x;
+                    a.y += W.dt * ap.y;
+                    a.z += W.dt * ap.z;
+                    W.a_lin[i] = a;
+                }
+            grid_sync(W, gen);
+        }
+        for (u32 p = np; p-- > 0;) {
+            const u32 lo = W.part_off[p], hi = W.part_off[p + 1];
+            for (u32 i = lo + tid; i < hi; i += nthreads) adjoint_manager(W, W.sorted[i], it, pos_iters);
+            grid_sync(W, gen);
+        }
+    }
+}
+
+// Rows' velocity terms back to the bodies' pre-integrated velocities; the
+// combined dynamic friction back to each body's (max(0, (df0 + df1) / 2)).
+__global__ __launch_bounds__(kBlock) void k_adj_rows(Ws W) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= mcount(W) || W.part[m] >= kMaxParts) return;
+    const fyzz_dy_manager& M = W.mg[m];
+    const int i0 = M.body0, i1 = M.body1;
+    V ava{0.f, 0.f, 0.f}, avb{0.f, 0.f, 0.f}, awa{0.f, 0.f, 0.f}, awb{0.f, 0.f, 0.f};
+    const u32 no = W.order[m] & 0xffu;
+    for (u32 q = 0; q < no; ++q) {
+        const CRow& r = W.crow[4 * m + q];
+        const float4 x0 = W.craw[2 * (4 * m + q)], x1 = W.craw[2 * (4 * m + q) + 1];
+        const float a_vrel = x0.w * r.vm * W.a_err[4 * m + q];
+        const V n{r.nx, r.ny, r.nz};
+        ava = vadd(ava, vmul(n, a_vrel));
+        avb = vsub(avb, vmul(n, a_vrel));
+        awa = vadd(awa, vmul(f3(x0), a_vrel));
+        awb = vsub(awb, vmul(f3(x1), a_vrel));
+    }
+    const fyzz_dy_patch& P = W.patches[m];
+    for (u32 k = 0; k < 2 * P.anchor_count; ++k) {
+        const FRow& fr = W.frow[4 * m + k];
+        const float a_t = W.a_tgt[4 * m + k];
+        const V t{fr.tx, fr.ty, fr.tz};
+        // target = -(t . (va - vb) + (ra x t) . wa - (rb x t) . wb)
+        ava = vsub(ava, vmul(t, a_t));
+        avb = vadd(avb, vmul(t, a_t));
+        awa = vsub(awa, vmul(f3(W.fraw[2 * (4 * m + k)]), a_t));
+        awb = vadd(awb, vmul(f3(W.fraw[2 * (4 * m + k) + 1]), a_t));
+    }
+    const float a_df = P.dynamic_friction > 0.0f ? 0.5f * W.a_df[m] : 0.0f;
+    if (i0 >= 0 && W.has_row[i0]) {
+        atomic_add3(&W.a_vpre[i0], ava);
+        atomic_add3(&W.a_wpre[i0], awa);
+        atomicAdd(&W.adj_friction[i0], a_df);
+    }
+    if (i1 >= 0 && W.has_row[i1]) {
+        atomic_add3(&W.a_vpre[i1], avb);
+        atomic_add3(&W.a_wpre[i1], awb);
+        atomicAdd(&W.adj_friction[i1], a_df);
+    }
+}
+
+// Pre-integration backwards: v_pre = lm (v_in + g dt), w_pre = am w_in, and
+// v_pre reaches the outputs directly (v_out, and p_out through dt).
+__global__ __launch_bounds__(kBlock) void k_adj_bodies(Ws W) {
+    const u32 i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= W.nb) return;
+    const fyzz_dy_body& b = W.bodies[i];
+    const V av = ld3(W.adj_v_out, i), aw = ld3(W.adj_w_out, i), ap = ld3(W.adj_p_out, i);
+    if (b.flags & 1u) {  // asleep: the step leaves it unchanged
+        for (int k = 0; k < 3; ++k) {
+            W.adj_v_in[3 * i + k] = (&av.x)[k];
+            W.adj_w_in[3 * i + k] = (&aw.x)[k];
+        }
+        return;
+    }
+    const V a_rows_v = f3(W.a_vpre[i]), a_rows_w = f3(W.a_wpre[i]);
+    const V a_vpre = vadd(vadd(av, vmul(ap, W.dt)), a_rows_v);
+    const V a_wpre = vadd(aw, a_rows_w);
+    const float lm = smax(1.0f - b.linear_damping * W.dt, 0.0f);
+    const float am = smax(1.0f - b.angular_damping * W.dt, 0.0f);
+    W.adj_v_in[3 * i] = lm * a_vpre.x;
+    W.adj_v_in[3 * i + 1] = lm * a_vpre.y;
+    W.adj_v_in[3 * i + 2] = lm * a_vpre.z;
+    W.adj_w_in[3 * i] = am * a_wpre.x;
+    W.adj_w_in[3 * i + 1] = am * a_wpre.y;
+    W.adj_w_in[3 * i + 2] = am * a_wpre.z;
 }
 
 // ------------------------------------------------------------- integrate
@@ -1179,6 +1436,12 @@ struct fyzz_hip_dy {
     DevBuf<dy::CRow> crow;
     DevBuf<dy::FRow> frow;
     DevBuf<u32> part_count, part_off, part_fill, blocksums, ctr, flags, bar;
+    // tape mode
+    bool tape = false;
+    bool taped_step = false;  // the last solved step recorded a tape
+    DevBuf<uint8_t> tape_c, tape_f;
+    DevBuf<float> tape_ni, a_capp, a_fapp, a_err, a_tgt, a_df;
+    DevBuf<float4> craw, fraw, a_lin, a_ang, a_vpre, a_wpre;
     dy::Ws ws{};
     bool prepared = false;
 };
@@ -1233,6 +1496,8 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
     w.patches = patches;
     w.d_nm = d_nm;
     w.with_rows = partition_only ? 0u : 1u;
+    w.tape = (d->tape && !partition_only) ? 1u : 0u;
+    d->taped_step = false;
     hipStream_t st = d->ctx->stream;
     auto& prof = d->ctx->prof;
     {
@@ -1257,6 +1522,7 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
 
 void dy_solve(fyzz_hip_dy* d) {
     FYZZ_REQUIRE(d->prepared, kLogicError, "dy_solve: no prepared step");
+    d->taped_step = d->ws.tape != 0;
     if (!d->ws.nm) return;
     ProfScope ps(&d->ctx->prof, "dy.pgs", d->ctx->stream);
     launch_coop(reinterpret_cast<const void*>(&dy::k_pgs), d->coop_blocks_pgs, d->ws, d->ctx->stream,
@@ -1275,6 +1541,63 @@ void dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
         hipLaunchKernelGGL(dy::k_outputs, dim3(grid_for(d->ws.nm, dy::kBlock)), dim3(dy::kBlock), 0, st, d->ws);
     FYZZ_HIP_CHECK(hipGetLastError());
     d->prepared = false;
+}
+
+void dy_set_tape(fyzz_hip_dy* d, bool enabled) {
+    if (enabled && !d->tape_c.p) {
+        const std::size_t nm = std::max<u32>(d->mgr_cap, 1), nb = std::max<u32>(d->body_cap, 1);
+        constexpr u32 kTapeIters = 8;
+        d->tape_c.reserve(kTapeIters * 4 * nm);
+        d->tape_f.reserve(kTapeIters * 4 * nm);
+        d->tape_ni.reserve(kTapeIters * nm);
+        for (auto* b : {&d->a_capp, &d->a_fapp, &d->a_err, &d->a_tgt}) b->reserve(4 * nm);
+        d->a_df.reserve(nm);
+        d->craw.reserve(8 * nm);
+        d->fraw.reserve(8 * nm);
+        for (auto* b : {&d->a_lin, &d->a_ang, &d->a_vpre, &d->a_wpre}) b->reserve(nb);
+        dy::Ws& w = d->ws;
+        w.tape_iters = kTapeIters;
+        w.tape_c = d->tape_c.p;
+        w.tape_f = d->tape_f.p;
+        w.tape_ni = d->tape_ni.p;
+        w.craw = d->craw.p;
+        w.fraw = d->fraw.p;
+        w.a_capp = d->a_capp.p;
+        w.a_fapp = d->a_fapp.p;
+        w.a_err = d->a_err.p;
+        w.a_tgt = d->a_tgt.p;
+        w.a_df = d->a_df.p;
+        w.a_lin = d->a_lin.p;
+        w.a_ang = d->a_ang.p;
+        w.a_vpre = d->a_vpre.p;
+        w.a_wpre = d->a_wpre.p;
+    }
+    d->tape = enabled;
+}
+
+void dy_adjoint(fyzz_hip_dy* d, const float* adj_v_out, const float* adj_w_out, const float* adj_p_out,
+                float* adj_v_in, float* adj_w_in, float* adj_friction) {
+    FYZZ_REQUIRE(d->taped_step, kLogicError, "dy_adjoint: the last step was not solved with the tape on");
+    FYZZ_REQUIRE(!d->prepared, kLogicError, "dy_adjoint: the last step has not been integrated");
+    dy::Ws& w = d->ws;
+    FYZZ_REQUIRE(w.nb == 0 || (adj_v_out && adj_w_out && adj_p_out && adj_v_in && adj_w_in && adj_friction),
+                 kInvalidArgument, "dy_adjoint: null device pointer");
+    w.adj_v_out = adj_v_out;
+    w.adj_w_out = adj_w_out;
+    w.adj_p_out = adj_p_out;
+    w.adj_v_in = adj_v_in;
+    w.adj_w_in = adj_w_in;
+    w.adj_friction = adj_friction;
+    hipStream_t st = d->ctx->stream;
+    ProfScope ps(&d->ctx->prof, "dy.adjoint", st);
+    const u32 span = std::max<u32>(w.nb, 4 * w.nm);
+    if (span) hipLaunchKernelGGL(dy::k_adj_init, dim3(grid_for(span, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
+    if (w.nm) {
+        launch_coop(reinterpret_cast<const void*>(&dy::k_pgs_adjoint), d->coop_blocks_pgs, w, st, d->plain_launch);
+        hipLaunchKernelGGL(dy::k_adj_rows, dim3(grid_for(w.nm, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
+    }
+    if (w.nb) hipLaunchKernelGGL(dy::k_adj_bodies, dim3(grid_for(w.nb, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
+    FYZZ_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
@@ -1430,6 +1753,17 @@ int fyzz_hip_dy_step(fyzz_hip_dy* d, float dt, uint32_t n_bodies, fyzz_dy_body* 
     if (rc == kOk) rc = fyzz_hip_dy_solve(d);
     if (rc == kOk) rc = fyzz_hip_dy_integrate(d, impulses, partitions);
     return rc;
+}
+
+int fyzz_hip_dy_set_tape(fyzz_hip_dy* d, int enabled) {
+    if (!d) return kInvalidArgument;
+    return guarded(d->ctx, [&] { dy_set_tape(d, enabled != 0); });
+}
+
+int fyzz_hip_dy_adjoint(fyzz_hip_dy* d, const float* adj_v_out, const float* adj_w_out, const float* adj_p_out,
+                        float* adj_v_in, float* adj_w_in, float* adj_friction) {
+    if (!d) return kInvalidArgument;
+    return guarded(d->ctx, [&] { dy_adjoint(d, adj_v_out, adj_w_out, adj_p_out, adj_v_in, adj_w_in, adj_friction); });
 }
 
 int fyzz_hip_dy_status(fyzz_hip_dy* d, uint32_t* flags, uint32_t* n_partitions, uint32_t* rounds) {

@@ -51,6 +51,8 @@ SIGNATURES = {
     "fyzz_hip_dy_step": (C.c_int, [vp, C.c_float, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_uint32,
                                    vp, vp, vp]),
     "fyzz_hip_dy_status": (C.c_int, [vp, u32p, u32p, u32p]),
+    "fyzz_hip_dy_set_tape": (C.c_int, [vp, C.c_int]),
+    "fyzz_hip_dy_adjoint": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "fyzz_hip_world_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
     "fyzz_hip_world_destroy": (C.c_int, [vp]),
     "fyzz_hip_world_add_static_floor": (C.c_int, [vp, C.c_float, u32p]),

@@ -135,7 +135,8 @@ int fyzz_hip_synchronize(fyzz_hip_context* ctx);
  * summed milliseconds and span count of one name, and resets it. Names:
  * "sap.bounds", "sap.sort", "sap.sort.scatter", "sap.pairs", "sap.lists",
  * "pcm.box_box", "pcm.plane_box", "dy.bodies", "dy.partition", "dy.prep",
- * "dy.pgs", "dy.integrate". */
+ * "dy.pgs", "dy.integrate", "dy.adjoint", "world.managers", "world.events",
+ * "world.scene_pairs". */
 int fyzz_hip_profile_enable(fyzz_hip_context* ctx, int enable);
 int fyzz_hip_profile_read(fyzz_hip_context* ctx, const char* name, double* total_ms,
                           uint64_t* count);
@@ -285,6 +286,21 @@ int fyzz_hip_dy_step(fyzz_hip_dy* dy, float dt, uint32_t n_bodies, fyzz_dy_body*
                      uint32_t n_statics, const fyzz_dy_static* statics, uint32_t n_managers,
                      const fyzz_dy_manager* managers, const fyzz_dy_patch* history, uint32_t n_history,
                      fyzz_dy_patch* patches, float* impulses, uint32_t* partitions);
+
+/* Adjoint of a Dy step (no counterpart in the reference: the
+ * differentiable form). With the tape on, a step records each row's
+ * branch per iteration and the rows' unscaled lever arms.
+ * fyzz_hip_dy_adjoint then runs reverse mode through that step's PGS and
+ * integration with its contact rows frozen (normals, lever arms, effective
+ * masses, biases and friction frames constant): inputs are the adjoints of
+ * the step's output linear velocity, angular velocity and position
+ * (device [nb][3] each); outputs the adjoints of its input linear and
+ * angular velocity ([nb][3]) and of each body's dynamic friction ([nb]).
+ * The position adjoint passes through unchanged. Orientation is not
+ * differentiated. Call it after the step's integrate. */
+int fyzz_hip_dy_set_tape(fyzz_hip_dy* dy, int enabled);
+int fyzz_hip_dy_adjoint(fyzz_hip_dy* dy, const float* adj_v_out, const float* adj_w_out, const float* adj_p_out,
+                        float* adj_v_in, float* adj_w_in, float* adj_friction);
 
 /* Synchronizes the context's stream and reports the last Dy step:
  * flags (FYZZ_DY_* bits), the partition count and the number of

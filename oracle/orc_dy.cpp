@@ -66,10 +66,15 @@ struct Dy {
     struct ContactRow {  // SolverContactPoint
         V n, ra, rb;
         T vm, biased, unbiased, applied, max_impulse;
+        // velocity dependence, kept for the frozen-row rollout (not used by
+        // the step): err = kv * vrel * vm - c, vrel = n.(va - vb) + raxn.wa - rbxn.wb
+        V raxn{}, rbxn{};
+        T kv = 0, cb = 0, cu = 0;
     };
     struct FrictionRow {  // SolverContactFriction
         V t, ra, rb;
         T vm, bias, target, applied;
+        V ax0{}, ax1{};  // unscaled lever arms: target = -(t.(va - vb) + ax0.wa - ax1.wb)
     };
     struct Patch {  // FrictionPatch (fields the admitted class needs)
         V normal{}, n0{}, n1{};
@@ -457,6 +462,11 @@ struct Dy {
                 r.biased = target * vm - scaled_bias;
                 r.unbiased = target * vm - (bouncing ? T(0) : std::max(scaled_bias, T(0)));
                 r.applied = T(0);
+                r.raxn = ra_x_n;
+                r.rbxn = rb_x_n;
+                r.kv = bouncing ? -(T(1) + restitution) : T(-1);
+                r.cb = scaled_bias;
+                r.cu = bouncing ? T(0) : std::max(scaled_bias, T(0));
                 if (!std::isfinite(response) || !std::isfinite(r.biased) || !std::isfinite(r.unbiased)) return -3;
                 crow[m][j] = r;
             }
@@ -547,6 +557,8 @@ struct Dy {
                             fr.bias = dot(tangent, err) * inv_dt;
                             fr.target = -(dot(tangent, rel) + dot(ax0, a.w) - dot(ax1, b.w));
                             fr.applied = T(0);
+                            fr.ax0 = ax0;
+                            fr.ax1 = ax1;
                             frows.push_back(fr);
                         }
                     }
@@ -582,6 +594,41 @@ struct Dy {
                 vel_iters = std::max(vel_iters, (bodies[i].iterations >> 8) & 0xffu);
             }
         vel_iters = std::max(vel_iters, 1u);
+        if (capture_) {
+            Frozen& F = *capture_;
+            F.dt = dt;
+            F.pos_iters = pos_iters;
+            F.vel_iters = vel_iters;
+            F.bodies.assign(nb, FrozenBody{});
+            for (std::uint32_t i = 0; i < nb; ++i) {
+                FrozenBody& fb = F.bodies[i];
+                const fyzz_dy_body& b = bodies[i];
+                fb.awake = !(b.flags & 1u);
+                fb.has_row = has_row[i] != 0;
+                fb.inv_mass = has_row[i] ? rows[i].inv_mass : T(0);
+                for (int k = 0; k < 3; ++k) fb.c[k] = has_row[i] ? rows[i].sii[k] : V{0, 0, 0};
+                fb.lm = std::max(T(1) - T(b.linear_damping) * dt, T(0));
+                fb.am = std::max(T(1) - T(b.angular_damping) * dt, T(0));
+                fb.gravity = (b.flags & 4u) != 0;
+            }
+            F.mgrs.clear();
+            for (std::uint32_t m : sorted) {
+                FrozenMgr fm;
+                fm.b0 = mgr[m].body0;
+                fm.b1 = mgr[m].body1;
+                fm.c = crow[m];
+                const Patch& P = patches[m];
+                fm.anchors = P.anchors;
+                fm.f.assign(frows.begin() + static_cast<std::ptrdiff_t>(P.row_offset),
+                            frows.begin() + static_cast<std::ptrdiff_t>(P.row_offset + 2 * P.anchors));
+                for (int k = 0; k < 2; ++k) {
+                    const int32_t id = k ? mgr[m].body1 : mgr[m].body0;
+                    fm.static_sf[k] = id < 0 ? T(statics[-1 - id].static_friction) : T(0);
+                    fm.static_df[k] = id < 0 ? T(statics[-1 - id].dynamic_friction) : T(0);
+                }
+                F.mgrs.push_back(std::move(fm));
+            }
+        }
         // the task-local copy of the static-world body (scene.cpp:8866)
         Row wcopy = world;
         for (unsigned iter = 0; iter < pos_iters + vel_iters; ++iter) {
@@ -776,11 +823,400 @@ struct Dy {
         return 0;
     }
     static inline thread_local std::vector<std::array<std::uint32_t, 4>> patch_order_;
+
+    // ---- frozen-row rollout (the differentiable form's reference)
+    struct FrozenBody {
+        bool awake = true, has_row = false, gravity = true;
+        T inv_mass = 0, lm = 1, am = 1;
+        V c[3]{};
+    };
+    struct FrozenMgr {
+        int32_t b0 = 0, b1 = 0;
+        std::vector<ContactRow> c;
+        std::vector<FrictionRow> f;
+        std::uint32_t anchors = 0;
+        T static_sf[2]{}, static_df[2]{};
+    };
+    struct Frozen {
+        T dt = 0;
+        unsigned pos_iters = 0, vel_iters = 1;
+        std::vector<FrozenBody> bodies;
+        std::vector<FrozenMgr> mgrs;  // solve order
+    };
+    static inline thread_local Frozen* capture_ = nullptr;
+};
+
+// One step of the frozen-row function: rows (geometry, effective masses,
+// biases, friction frames, branches of the bounce test) fixed at the
+// nominal step; velocities, the PGS, and positions follow the inputs.
+// Records each row update's branch for the reverse pass.
+struct FrozenTape {
+    std::vector<std::vector<unsigned char>> c, f;  // per iteration, per solve-ordered row
+    std::vector<std::vector<double>> ni;            // per iteration, per manager
+    std::vector<double> vpre, wpre;                 // [nb][3]
+};
+using D = Dy<double>;
+
+void frozen_friction(const D::Frozen& F, const D::FrozenMgr& fm, const double* mu, double& sf, double& df) {
+    double sfb[2], dfb[2];
+    for (int k = 0; k < 2; ++k) {
+        const int32_t id = k ? fm.b1 : fm.b0;
+        sfb[k] = id >= 0 ? mu[id] : fm.static_sf[k];
+        dfb[k] = id >= 0 ? mu[id] : fm.static_df[k];
+    }
+    df = std::max(0.0, 0.5 * (dfb[0] + dfb[1]));
+    sf = std::max(df, 0.5 * (sfb[0] + sfb[1]));
+    (void)F;
+}
+
+void frozen_step(const D::Frozen& F, std::vector<double>& v, std::vector<double>& w, std::vector<double>& p,
+                 const double* mu, FrozenTape* tape) {
+    using V = D::V;
+    const std::size_t nb = F.bodies.size();
+    const double dt = F.dt;
+    std::vector<double> vpre(3 * nb), wpre(3 * nb), lin(3 * nb, 0.0), ang(3 * nb, 0.0), mlin(3 * nb, 0.0);
+    for (std::size_t i = 0; i < nb; ++i) {
+        const auto& fb = F.bodies[i];
+        for (int k = 0; k < 3; ++k) {
+            vpre[3 * i + k] = v[3 * i + k];
+            wpre[3 * i + k] = w[3 * i + k];
+        }
+        if (!fb.awake) continue;
+        if (fb.gravity) vpre[3 * i + 1] += -9.81 * dt;
+        for (int k = 0; k < 3; ++k) {
+            vpre[3 * i + k] *= fb.lm;
+            wpre[3 * i + k] *= fb.am;
+        }
+    }
+    auto get = [&](const std::vector<double>& a, int32_t id) -> V {
+        if (id < 0) return {0, 0, 0};
+        return {a[3 * id], a[3 * id + 1], a[3 * id + 2]};
+    };
+    auto add = [&](std::vector<double>& a, int32_t id, V d, double s) {
+        if (id < 0) return;
+        a[3 * id] += d.x * s;
+        a[3 * id + 1] += d.y * s;
+        a[3 * id + 2] += d.z * s;
+    };
+    auto dot = [](V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    // velocity-dependent row terms
+    std::vector<std::vector<double>> biased(F.mgrs.size()), unbiased(F.mgrs.size()), target(F.mgrs.size());
+    std::vector<std::vector<double>> capp(F.mgrs.size()), fapp(F.mgrs.size());
+    for (std::size_t m = 0; m < F.mgrs.size(); ++m) {
+        const auto& fm = F.mgrs[m];
+        const V va = get(vpre, fm.b0), vb = get(vpre, fm.b1), wa = get(wpre, fm.b0), wb = get(wpre, fm.b1);
+        for (const auto& r : fm.c) {
+            const double vrel = dot(r.n, {va.x - vb.x, va.y - vb.y, va.z - vb.z}) + dot(r.raxn, wa) - dot(r.rbxn, wb);
+            biased[m].push_back(r.kv * vrel * r.vm - r.cb);
+            unbiased[m].push_back(r.kv * vrel * r.vm - r.cu);
+        }
+        for (const auto& fr : fm.f)
+            target[m].push_back(-(dot(fr.t, {va.x - vb.x, va.y - vb.y, va.z - vb.z}) + dot(fr.ax0, wa) - dot(fr.ax1, wb)));
+        capp[m].assign(fm.c.size(), 0.0);
+        fapp[m].assign(fm.f.size(), 0.0);
+    }
+    const unsigned total = F.pos_iters + F.vel_iters;
+    if (tape) {
+        tape->c.assign(total, {});
+        tape->f.assign(total, {});
+        tape->ni.assign(total, std::vector<double>(F.mgrs.size(), 0.0));
+        tape->vpre = vpre;
+        tape->wpre = wpre;
+    }
+    for (unsigned it = 0; it < total; ++it) {
+        for (std::size_t m = 0; m < F.mgrs.size(); ++m) {
+            const auto& fm = F.mgrs[m];
+            const double ima = fm.b0 >= 0 ? F.bodies[fm.b0].inv_mass : 0.0;
+            const double imb = fm.b1 >= 0 ? F.bodies[fm.b1].inv_mass : 0.0;
+            double ni = 0.0;
+            for (std::size_t q = 0; q < fm.c.size(); ++q) {
+                const auto& r = fm.c[q];
+                const V va = get(lin, fm.b0), vb = get(lin, fm.b1), wa = get(ang, fm.b0), wb = get(ang, fm.b1);
+                const double nv = dot(r.n, va) - dot(r.n, vb) + dot(wa, r.ra) - dot(wb, r.rb);
+                const double err = it < F.pos_iters ? biased[m][q] : unbiased[m][q];
+                const double x = -nv * r.vm + err;
+                const bool free = !(x < -capp[m][q]);
+                const double next = free ? capp[m][q] + x : 0.0;
+                const double delta = next - capp[m][q];
+                add(lin, fm.b0, r.n, delta * ima);
+                add(lin, fm.b1, r.n, -delta * imb);
+                add(ang, fm.b0, r.ra, delta);
+                add(ang, fm.b1, r.rb, -delta);
+                capp[m][q] = next;
+                ni += next;
+                if (tape) tape->c[it].push_back(free ? 1 : 0);
+            }
+            if (tape) tape->ni[it][m] = ni;
+            if ((it >= F.pos_iters || F.pos_iters - it <= 3) && fm.anchors) {
+                double sf, df;
+                frozen_friction(F, fm, mu, sf, df);
+                const double coef = fm.anchors == 2 ? 0.5 : 1.0;
+                const double max_static = sf * coef * ni, max_dynamic = df * coef * ni;
+                for (std::size_t k = 0; k < fm.f.size(); ++k) {
+                    const auto& fr = fm.f[k];
+                    const V va = get(lin, fm.b0), vb = get(lin, fm.b1), wa = get(ang, fm.b0), wb = get(ang, fm.b1);
+                    const double td = dot(fr.t, va) - dot(fr.t, vb) + dot(wa, fr.ra) - dot(wb, fr.rb);
+                    const double bias = it < F.pos_iters ? fr.bias : 0.0;
+                    const double cand = fapp[m][k] + (target[m][k] - bias - td) * fr.vm;
+                    const bool clamped = std::abs(cand) > max_static;
+                    unsigned char br = 0;
+                    double next = cand;
+                    if (clamped) {
+                        br = 1;
+                        if (cand > max_dynamic) next = max_dynamic, br = 2;
+                        else if (cand < -max_dynamic) next = -max_dynamic, br = 3;
+                    }
+                    const double delta = next - fapp[m][k];
+                    add(lin, fm.b0, fr.t, delta * ima);
+                    add(lin, fm.b1, fr.t, -delta * imb);
+                    add(ang, fm.b0, fr.ra, delta);
+                    add(ang, fm.b1, fr.rb, -delta);
+                    fapp[m][k] = next;
+                    if (tape) tape->f[it].push_back(br);
+                }
+            }
+        }
+        if (it + 1 == F.pos_iters) mlin = lin;
+    }
+    for (std::size_t i = 0; i < nb; ++i) {
+        const auto& fb = F.bodies[i];
+        if (!fb.awake) continue;
+        for (int k = 0; k < 3; ++k) {
+            const double motion = vpre[3 * i + k] + (fb.has_row ? mlin[3 * i + k] : 0.0);
+            p[3 * i + k] += motion * dt;
+            v[3 * i + k] = vpre[3 * i + k] + lin[3 * i + k];
+            double s = 0.0;
+            for (int j = 0; j < 3; ++j) s += (&fb.c[j].x)[k] * ang[3 * i + j];
+            w[3 * i + k] = wpre[3 * i + k] + s;
+        }
+    }
+}
+
+// Reverse of frozen_step: adjoints of (v, w, p) after the step in, of
+// (v, w, p) before it out; per-body dynamic friction adjoints accumulate.
+void frozen_step_adjoint(const D::Frozen& F, const FrozenTape& tape, const double* mu, std::vector<double>& av,
+                         std::vector<double>& aw, const std::vector<double>& ap, std::vector<double>& amu) {
+    using V = D::V;
+    const std::size_t nb = F.bodies.size();
+    const double dt = F.dt;
+    auto dot = [](V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    std::vector<double> alin(3 * nb, 0.0), aang(3 * nb, 0.0), avpre(3 * nb, 0.0), awpre(3 * nb, 0.0);
+    for (std::size_t i = 0; i < nb; ++i) {
+        const auto& fb = F.bodies[i];
+        if (!fb.awake) continue;
+        for (int k = 0; k < 3; ++k) {
+            alin[3 * i + k] = av[3 * i + k];
+            avpre[3 * i + k] = av[3 * i + k] + dt * ap[3 * i + k];
+            awpre[3 * i + k] = aw[3 * i + k];
+        }
+        for (int j = 0; j < 3; ++j)
+            aang[3 * i + j] = fb.c[j].x * aw[3 * i] + fb.c[j].y * aw[3 * i + 1] + fb.c[j].z * aw[3 * i + 2];
+    }
+    auto get = [&](const std::vector<double>& a, int32_t id) -> V {
+        if (id < 0) return {0, 0, 0};
+        return {a[3 * id], a[3 * id + 1], a[3 * id + 2]};
+    };
+    auto put = [&](std::vector<double>& a, int32_t id, V x) {
+        if (id < 0) return;
+        a[3 * id] = x.x;
+        a[3 * id + 1] = x.y;
+        a[3 * id + 2] = x.z;
+    };
+    auto axpy = [](V a, V d, double s) { return V{a.x + d.x * s, a.y + d.y * s, a.z + d.z * s}; };
+    std::vector<std::vector<double>> acapp(F.mgrs.size()), afapp(F.mgrs.size()), aerr(F.mgrs.size()),
+        atgt(F.mgrs.size());
+    std::vector<double> adf(F.mgrs.size(), 0.0);
+    for (std::size_t m = 0; m < F.mgrs.size(); ++m) {
+        acapp[m].assign(F.mgrs[m].c.size(), 0.0);
+        aerr[m].assign(F.mgrs[m].c.size(), 0.0);
+        afapp[m].assign(F.mgrs[m].f.size(), 0.0);
+        atgt[m].assign(F.mgrs[m].f.size(), 0.0);
+    }
+    // row offsets into the tape, per iteration
+    const unsigned total = F.pos_iters + F.vel_iters;
+    for (unsigned it = total; it-- > 0;) {
+        if (it + 1 == F.pos_iters)
+            for (std::size_t i = 0; i < nb; ++i)
+                if (F.bodies[i].awake && F.bodies[i].has_row)
+                    for (int k = 0; k < 3; ++k) alin[3 * i + k] += dt * ap[3 * i + k];
+        std::vector<std::size_t> coff(F.mgrs.size() + 1, 0), foff(F.mgrs.size() + 1, 0);
+        for (std::size_t m = 0; m < F.mgrs.size(); ++m) {
+            coff[m + 1] = coff[m] + F.mgrs[m].c.size();
+            const bool fr_on = (it >= F.pos_iters || F.pos_iters - it <= 3) && F.mgrs[m].anchors;
+            foff[m + 1] = foff[m] + (fr_on ? F.mgrs[m].f.size() : 0);
+        }
+        for (std::size_t m = F.mgrs.size(); m-- > 0;) {
+            const auto& fm = F.mgrs[m];
+            const double ima = fm.b0 >= 0 ? F.bodies[fm.b0].inv_mass : 0.0;
+            const double imb = fm.b1 >= 0 ? F.bodies[fm.b1].inv_mass : 0.0;
+            V ava = get(alin, fm.b0), avb = get(alin, fm.b1), awa = get(aang, fm.b0), awb = get(aang, fm.b1);
+            double a_ni = 0.0;
+            if (foff[m + 1] > foff[m]) {
+                double sf, df;
+                frozen_friction(F, fm, mu, sf, df);
+                const double coef = fm.anchors == 2 ? 0.5 : 1.0;
+                const double ni = tape.ni[it][m];
+                double a_maxd = 0.0;
+                for (std::size_t k = fm.f.size(); k-- > 0;) {
+                    const auto& fr = fm.f[k];
+                    const double a_delta = dot(fr.t, ava) * ima - dot(fr.t, avb) * imb + dot(fr.ra, awa) - dot(fr.rb, awb);
+                    const double a_next = afapp[m][k] + a_delta;
+                    double a_applied = -a_delta, a_cand = 0.0;
+                    const unsigned char br = tape.f[it][foff[m] + k];
+                    if (br <= 1) a_cand = a_next;
+                    else if (br == 2) a_maxd += a_next;
+                    else a_maxd -= a_next;
+                    a_applied += a_cand;
+                    atgt[m][k] += fr.vm * a_cand;
+                    const double a_td = -fr.vm * a_cand;
+                    ava = axpy(ava, fr.t, a_td);
+                    avb = axpy(avb, fr.t, -a_td);
+                    awa = axpy(awa, fr.ra, a_td);
+                    awb = axpy(awb, fr.rb, -a_td);
+                    afapp[m][k] = a_applied;
+                }
+                adf[m] += coef * ni * a_maxd;
+                a_ni = df * coef * a_maxd;
+            }
+            for (std::size_t q = fm.c.size(); q-- > 0;) {
+                const auto& r = fm.c[q];
+                const double a_delta = dot(r.n, ava) * ima - dot(r.n, avb) * imb + dot(r.ra, awa) - dot(r.rb, awb);
+                const double a_next = acapp[m][q] + a_delta + a_ni;
+                double a_applied = -a_delta;
+                if (tape.c[it][coff[m] + q]) {
+                    a_applied += a_next;
+                    const double a_nv = -r.vm * a_next;
+                    aerr[m][q] += a_next;
+                    ava = axpy(ava, r.n, a_nv);
+                    avb = axpy(avb, r.n, -a_nv);
+                    awa = axpy(awa, r.ra, a_nv);
+                    awb = axpy(awb, r.rb, -a_nv);
+                }
+                acapp[m][q] = a_applied;
+            }
+            put(alin, fm.b0, ava);
+            put(alin, fm.b1, avb);
+            put(aang, fm.b0, awa);
+            put(aang, fm.b1, awb);
+        }
+    }
+    for (std::size_t m = 0; m < F.mgrs.size(); ++m) {
+        const auto& fm = F.mgrs[m];
+        V ava{0, 0, 0}, avb{0, 0, 0}, awa{0, 0, 0}, awb{0, 0, 0};
+        for (std::size_t q = 0; q < fm.c.size(); ++q) {
+            const auto& r = fm.c[q];
+            const double a_vrel = r.kv * r.vm * aerr[m][q];
+            ava = axpy(ava, r.n, a_vrel);
+            avb = axpy(avb, r.n, -a_vrel);
+            awa = axpy(awa, r.raxn, a_vrel);
+            awb = axpy(awb, r.rbxn, -a_vrel);
+        }
+        for (std::size_t k = 0; k < fm.f.size(); ++k) {
+            const auto& fr = fm.f[k];
+            ava = axpy(ava, fr.t, -atgt[m][k]);
+            avb = axpy(avb, fr.t, atgt[m][k]);
+            awa = axpy(awa, fr.ax0, -atgt[m][k]);
+            awb = axpy(awb, fr.ax1, atgt[m][k]);
+        }
+        double sf, df;
+        frozen_friction(F, fm, mu, sf, df);
+        const double a_df = df > 0.0 ? 0.5 * adf[m] : 0.0;
+        for (int k = 0; k < 2; ++k) {
+            const int32_t id = k ? fm.b1 : fm.b0;
+            if (id < 0 || !F.bodies[id].has_row) continue;
+            const V dv = k ? avb : ava, dw = k ? awb : awa;
+            avpre[3 * id] += dv.x;
+            avpre[3 * id + 1] += dv.y;
+            avpre[3 * id + 2] += dv.z;
+            awpre[3 * id] += dw.x;
+            awpre[3 * id + 1] += dw.y;
+            awpre[3 * id + 2] += dw.z;
+            amu[id] += a_df;
+        }
+    }
+    for (std::size_t i = 0; i < nb; ++i) {
+        const auto& fb = F.bodies[i];
+        if (!fb.awake) continue;  // unchanged by the step: adjoints pass through
+        for (int k = 0; k < 3; ++k) {
+            av[3 * i + k] = fb.lm * avpre[3 * i + k];
+            aw[3 * i + k] = fb.am * awpre[3 * i + k];
+        }
+    }
+}
+
+struct DiffRollout {
+    std::vector<D::Frozen> steps;
+    std::vector<double> p0;
 };
 
 }  // namespace
 
 extern "C" {
+
+// Frozen-row rollout of the differentiable form (FP64). create: runs the
+// nominal FP64 Dy steps from bodies0 with each step's recorded managers
+// and history (managers concatenated, nm[s] per step; history likewise,
+// manager history indices local to the step) and keeps every step's rows.
+void* orc_diff_create(std::uint32_t nb, const fyzz_dy_body* bodies0, std::uint32_t ns, const fyzz_dy_static* statics,
+                      std::uint32_t steps, const std::uint32_t* nm, const fyzz_dy_manager* managers,
+                      const std::uint32_t* nh, const fyzz_dy_patch* history, float dt) {
+    auto* R = new DiffRollout();
+    std::vector<fyzz_dy_body> b(bodies0, bodies0 + nb);
+    R->p0.resize(3 * nb);
+    for (std::uint32_t i = 0; i < nb; ++i)
+        for (int k = 0; k < 3; ++k) R->p0[3 * i + k] = b[i].position[k];
+    std::size_t mo = 0, ho = 0;
+    R->steps.resize(steps);
+    for (std::uint32_t s = 0; s < steps; ++s) {
+        std::vector<fyzz_dy_patch> patches(std::max<std::uint32_t>(nm[s], 1));
+        std::vector<float> imp(4 * std::max<std::uint32_t>(nm[s], 1));
+        std::vector<double> st(13ull * nb);
+        D::patch_order_.assign(nm[s], {});
+        D::capture_ = &R->steps[s];
+        const int rc = D::step(double(dt), nb, b.data(), ns, statics, nm[s], managers + mo, history + ho, nh[s],
+                               patches.data(), imp.data(), nullptr, st.data());
+        D::capture_ = nullptr;
+        if (rc != 0) {
+            delete R;
+            return nullptr;
+        }
+        mo += nm[s];
+        ho += nh[s];
+    }
+    return R;
+}
+
+void orc_diff_destroy(void* h) { delete static_cast<DiffRollout*>(h); }
+
+// Final positions [nb][3] of the frozen-row rollout for initial linear and
+// angular velocities v0, w0 [nb][3] and per-body friction mu [nb].
+int orc_diff_eval(void* h, const double* v0, const double* w0, const double* mu, double* p_final) {
+    auto* R = static_cast<DiffRollout*>(h);
+    const std::size_t nb = R->p0.size() / 3;
+    std::vector<double> v(v0, v0 + 3 * nb), w(w0, w0 + 3 * nb), p = R->p0;
+    for (const auto& F : R->steps) frozen_step(F, v, w, p, mu, nullptr);
+    std::copy(p.begin(), p.end(), p_final);
+    return 0;
+}
+
+// Reverse mode: adjoint of the final positions in, gradients of v0, w0 and
+// mu out (and the final positions).
+int orc_diff_grad(void* h, const double* v0, const double* w0, const double* mu, const double* adj_p_final,
+                  double* g_v0, double* g_w0, double* g_mu, double* p_final) {
+    auto* R = static_cast<DiffRollout*>(h);
+    const std::size_t nb = R->p0.size() / 3;
+    std::vector<double> v(v0, v0 + 3 * nb), w(w0, w0 + 3 * nb), p = R->p0;
+    std::vector<FrozenTape> tapes(R->steps.size());
+    std::vector<std::vector<double>> vs, ws;
+    for (std::size_t s = 0; s < R->steps.size(); ++s) frozen_step(R->steps[s], v, w, p, mu, &tapes[s]);
+    std::copy(p.begin(), p.end(), p_final);
+    std::vector<double> av(3 * nb, 0.0), aw(3 * nb, 0.0), ap(adj_p_final, adj_p_final + 3 * nb), amu(nb, 0.0);
+    for (std::size_t s = R->steps.size(); s-- > 0;) frozen_step_adjoint(R->steps[s], tapes[s], mu, av, aw, ap, amu);
+    std::copy(av.begin(), av.end(), g_v0);
+    std::copy(aw.begin(), aw.end(), g_w0);
+    std::copy(amu.begin(), amu.end(), g_mu);
+    return 0;
+}
 
 // One Dy step of the FP32 restatement. bodies are updated in place;
 // patches_out[nm], impulses_out[nm][4] (per manifold contact), parts_out[nm]

@@ -450,6 +450,7 @@ class OracleWorld:
         b = b.copy()
         b["counted_interactions"] = counted
         out, patches, imp, parts = self.dy.step(dt, b, self.statics, mg, hist)
+        self.last_dy_inputs = (b, mg, hist)
         self.history = {k: patches[i] for i, k in enumerate(keys)}
         self.bodies = out
         self.last = (pairs, mg, imp, parts)
@@ -506,3 +507,45 @@ def oracle_contact_events(lib, managers, impulses, keys, reversed_flags, prev_pa
     assert rc == 0
     k, m = ne.value, npt.value
     return ev[:k], off[:k], eimp[:k], pt[:m], pimp[:m]
+
+
+class OracleDiff:
+    """The frozen-row rollout of the differentiable form in FP64
+    (oracle/orc_dy.cpp orc_diff_*): built from the initial body records and
+    each step's recorded managers and history."""
+
+    def __init__(self, lib, bodies0, statics, managers_per_step, history_per_step, dt):
+        self.lib = lib
+        lib.orc_diff_create.restype = C.c_void_p
+        self.nb = len(bodies0)
+        nm = np.array([len(m) for m in managers_per_step], np.uint32)
+        nh = np.array([len(h) for h in history_per_step], np.uint32)
+        mg = np.concatenate([np.asarray(m, DY_MANAGER) for m in managers_per_step]) if nm.sum() else \
+            np.zeros(1, DY_MANAGER)
+        hs = [np.asarray(h, DY_PATCH) for h in history_per_step if len(h)]
+        hist = np.concatenate(hs) if hs else np.zeros(1, DY_PATCH)
+        b0 = np.ascontiguousarray(bodies0)
+        st = np.ascontiguousarray(statics)
+        self.h = lib.orc_diff_create(C.c_uint32(self.nb), P(b0), C.c_uint32(len(st)), P(st), C.c_uint32(len(nm)),
+                                     P(nm), P(np.ascontiguousarray(mg)), P(nh), P(np.ascontiguousarray(hist)),
+                                     C.c_float(dt))
+        if not self.h:
+            raise RuntimeError("oracle nominal rollout failed")
+        self.h = C.c_void_p(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_diff_destroy(self.h)
+
+    def eval(self, v0, w0, mu):
+        out = np.zeros((self.nb, 3))
+        self.lib.orc_diff_eval(self.h, P(np.ascontiguousarray(v0, np.float64)), P(np.ascontiguousarray(w0, np.float64)),
+                               P(np.ascontiguousarray(mu, np.float64)), P(out))
+        return out
+
+    def grad(self, v0, w0, mu, adj_p):
+        gv, gw, gm, pf = np.zeros((self.nb, 3)), np.zeros((self.nb, 3)), np.zeros(self.nb), np.zeros((self.nb, 3))
+        self.lib.orc_diff_grad(self.h, P(np.ascontiguousarray(v0, np.float64)), P(np.ascontiguousarray(w0, np.float64)),
+                               P(np.ascontiguousarray(mu, np.float64)), P(np.ascontiguousarray(adj_p, np.float64)),
+                               P(gv), P(gw), P(gm), P(pf))
+        return gv, gw, gm, pf
