@@ -168,6 +168,23 @@ struct MgrWrite {
     }
 };
 
+// the step's Dy inputs with each manager's history moved to its own index
+// (the differentiable form records the contact sets this way)
+__global__ void k_record_dy(const u32* d_nm, u32 cap_m, const fyzz_dy_manager* managers,
+                            const fyzz_dy_patch* hist, fyzz_dy_manager* out_m, fyzz_dy_patch* out_h) {
+    const u32 n = min(*d_nm, cap_m);
+    for (u32 m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+        fyzz_dy_manager M = managers[m];
+        if (M.history != kNone) {
+            out_h[m] = hist[M.history];
+            M.history = m;
+        } else {
+            out_h[m] = fyzz_dy_patch{};
+        }
+        out_m[m] = M;
+    }
+}
+
 // friction patches solved this step, keyed for the next step's lookup
 __global__ void k_scatter_patches(const u32* d_nm, u32 cap_m, const fyzz_dy_patch* patches, const u32* mgr_slot,
                                   fyzz_dy_patch* hist_next, u32* stamp_next, const u32* d_step) {
@@ -1238,6 +1255,55 @@ int fyzz_hip_world_contact_events(fyzz_hip_world* w, fyzz_contact_event* events,
             FYZZ_HIP_CHECK(hipMemcpyAsync(points, w->points.p, sizeof(fyzz_contact_event_point) * mp,
                                           hipMemcpyDeviceToHost, st));
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int fyzz_hip_world_set_materials(fyzz_hip_world* w, uint32_t first_id, uint32_t n, const float* static_friction,
+                                 const float* dynamic_friction, const float* restitution) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(n == 0 || (static_friction && dynamic_friction && restitution), kInvalidArgument,
+                     "world: null material arrays");
+        for (u32 k = 0; k < n; ++k) {
+            FYZZ_REQUIRE(static_friction[k] >= 0.0f && dynamic_friction[k] >= 0.0f && restitution[k] >= 0.0f,
+                         kInvalidArgument, "world: material values must be non-negative");
+            fyzz_dy_body& b = w->body(first_id + k);
+            b.static_friction = static_friction[k];
+            b.dynamic_friction = dynamic_friction[k];
+            b.restitution = restitution[k];
+        }
+    });
+}
+
+int fyzz_hip_world_record_dy_inputs(fyzz_hip_world* w, fyzz_dy_manager* managers, fyzz_dy_patch* history,
+                                    uint32_t capacity, uint32_t* n_managers) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(w->uploaded, kLogicError, "world: no step taken");
+        hipStream_t st = w->ctx->stream;
+        u32 nm = 0;
+        if (w->n) {
+            FYZZ_HIP_CHECK(hipMemcpyAsync(&nm, w->d_nm.p, 4, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        if (n_managers) *n_managers = nm;
+        if (!managers || !history || !nm) return;
+        FYZZ_REQUIRE(capacity >= nm, kOverflow, "world: record capacity below the manager count");
+        hipLaunchKernelGGL(world::k_record_dy, dim3(std::min<u32>(grid_for(nm, 256), 2048u)), dim3(256), 0, st,
+                           w->d_nm.p, w->mgr_cap, w->managers.p, w->hist[w->hcur ^ 1].p, managers, history);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int fyzz_hip_world_copy_bodies(fyzz_hip_world* w, fyzz_dy_body* dst) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        if (!w->uploaded) upload(w);
+        FYZZ_REQUIRE(dst || !w->n, kInvalidArgument, "world: null destination");
+        if (w->n)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(dst, w->bodies.p, sizeof(fyzz_dy_body) * w->n, hipMemcpyDeviceToDevice,
+                                          w->ctx->stream));
     });
 }
 

@@ -63,6 +63,9 @@ SIGNATURES = {
     "fyzz_hip_world_set_body_velocity": (C.c_int, [vp, C.c_uint32, vp, vp]),
     "fyzz_hip_world_set_body_material": (C.c_int, [vp, C.c_uint32, C.c_float, C.c_float, C.c_float]),
     "fyzz_hip_world_set_scenes": (C.c_int, [vp, C.c_uint32, vp]),
+    "fyzz_hip_world_set_materials": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp]),
+    "fyzz_hip_world_record_dy_inputs": (C.c_int, [vp, vp, vp, C.c_uint32, u32p]),
+    "fyzz_hip_world_copy_bodies": (C.c_int, [vp, vp]),
     "fyzz_hip_world_step": (C.c_int, [vp, C.c_float]),
     "fyzz_hip_world_step_graph": (C.c_int, [vp, C.c_float]),
     "fyzz_hip_world_autotune": (C.c_int, [vp, C.c_float, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

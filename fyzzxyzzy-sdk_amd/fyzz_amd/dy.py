@@ -116,6 +116,22 @@ class DySolver:
                    C.c_uint32(nh), _p(patches), _p(impulses), _p(partitions))
         return patches[:nm], impulses[:nm], partitions[:nm]
 
+    def set_tape(self, enabled=True):
+        self._call("fyzz_hip_dy_set_tape", C.c_int(1 if enabled else 0))
+
+    def adjoint(self, adj_v_out, adj_w_out, adj_p_out):
+        """Reverse of the last (taped) step: returns (adj_v_in, adj_w_in,
+        adj_dynamic_friction) device tensors."""
+        import torch
+        nb = adj_v_out.shape[0]
+        av = torch.empty((nb, 3), dtype=torch.float32, device=adj_v_out.device)
+        aw = torch.empty_like(av)
+        af = torch.empty(nb, dtype=torch.float32, device=adj_v_out.device)
+        c = lambda t: t.contiguous()  # noqa: E731
+        a1, a2, a3 = c(adj_v_out), c(adj_w_out), c(adj_p_out)
+        self._call("fyzz_hip_dy_adjoint", _p(a1), _p(a2), _p(a3), _p(av), _p(aw), _p(af))
+        return av, aw, af
+
     def partition(self, bodies, managers):
         import torch
         nm = managers.shape[0]

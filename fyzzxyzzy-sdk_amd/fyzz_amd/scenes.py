@@ -296,3 +296,26 @@ def batch_scenes(n_scenes: int = 4096, seed: int = SEED_BASE + 3, min_boxes: int
         poss.append(sc.pos)
         quats.append(sc.quat)
     return (counts.astype(np.uint32), np.concatenate(halves), np.concatenate(poss), np.concatenate(quats))
+
+
+def diff_scenes(n_scenes: int = 1024, seed: int = SEED_BASE + 4):
+    """The differentiable workload: n_scenes independent 4x4x4 stacks of 64
+    boxes (h = 0.5, spacing 1.05, bottom layer on the plane; scenes never
+    interact, so every stack sits at the origin). Per body v0, w0 ~
+    N(0, 0.5) per axis (streams 10-12 and 13-15, index = global body);
+    per scene friction mu ~ U(0.2, 0.8) (stream 16) and target (x, z) ~
+    U(-2, 2) (streams 17, 18). Returns (counts, half, pos, quat, v0, w0,
+    mu, target[n_scenes, 3] with y = 0)."""
+    st = stack_scene(seed)
+    n = n_scenes * st.n
+    counts = np.full(n_scenes, st.n, np.uint32)
+    half = np.tile(st.half, (n_scenes, 1))
+    pos = np.tile(st.pos, (n_scenes, 1))
+    quat = np.tile(st.quat, (n_scenes, 1))
+    v0 = f32(np.stack([normal(seed, 10 + a, n) * 0.5 for a in range(3)], 1))
+    w0 = f32(np.stack([normal(seed, 13 + a, n) * 0.5 for a in range(3)], 1))
+    mu = f32(0.2 + 0.6 * uniform(seed, 16, n_scenes))
+    target = np.zeros((n_scenes, 3))
+    target[:, 0] = (uniform(seed, 17, n_scenes) - 0.5) * 4.0
+    target[:, 2] = (uniform(seed, 18, n_scenes) - 0.5) * 4.0
+    return counts, half, pos, quat, v0, w0, mu, target

@@ -87,6 +87,34 @@ class World:
         self.n_dynamic += n
         return out.value
 
+    def set_materials(self, first_id, static_friction, dynamic_friction, restitution):
+        sf = np.ascontiguousarray(static_friction, np.float32)
+        df = np.ascontiguousarray(dynamic_friction, np.float32)
+        rs = np.ascontiguousarray(restitution, np.float32)
+        self._call("fyzz_hip_world_set_materials", C.c_uint32(first_id), C.c_uint32(len(sf)), sf.ctypes.data_as(vp),
+                   df.ctypes.data_as(vp), rs.ctypes.data_as(vp))
+
+    def record_dy_inputs(self):
+        """The last step's (managers, history) as device uint8 tensors of
+        DY_MANAGER / DY_PATCH records (history indexed by manager)."""
+        import torch
+        from .dy import DY_PATCH
+        cnt = C.c_uint32()
+        self._call("fyzz_hip_world_record_dy_inputs", None, None, C.c_uint32(0), C.byref(cnt))
+        n = cnt.value
+        mg = torch.empty((max(n, 1), DY_MANAGER.itemsize), dtype=torch.uint8, device="cuda")
+        hs = torch.empty((max(n, 1), DY_PATCH.itemsize), dtype=torch.uint8, device="cuda")
+        self._call("fyzz_hip_world_record_dy_inputs", vp(mg.data_ptr()), vp(hs.data_ptr()), C.c_uint32(max(n, 1)),
+                   C.byref(cnt))
+        return mg[:n], hs[:n]
+
+    def copy_bodies(self):
+        """Device copy of the body records, (n, 144) uint8 tensor."""
+        import torch
+        out = torch.empty((max(self.n_dynamic, 1), DY_BODY.itemsize), dtype=torch.uint8, device="cuda")
+        self._call("fyzz_hip_world_copy_bodies", vp(out.data_ptr()))
+        return out[:self.n_dynamic]
+
     def set_body_pose(self, body, position, orientation):
         p, pp = _f32(position, 3)
         q, qp = _f32(orientation, 4)

@@ -353,6 +353,10 @@ int fyzz_hip_world_set_body_material(fyzz_hip_world* world, uint32_t body_id, fl
  * over all scenes. Each scene steps exactly as a world holding only that
  * scene would. Before the first step; no contact notifications. */
 int fyzz_hip_world_set_scenes(fyzz_hip_world* world, uint32_t n_scenes, const uint32_t* box_counts);
+/* Per-body materials for n consecutive body ids from first_id (HOST
+ * arrays): set_body_material on each. */
+int fyzz_hip_world_set_materials(fyzz_hip_world* world, uint32_t first_id, uint32_t n, const float* static_friction,
+                                 const float* dynamic_friction, const float* restitution);
 /* World::step (world.h:459); asynchronous. */
 int fyzz_hip_world_step(fyzz_hip_world* world, float dt);
 /* One World::step through a captured HIP graph (worlds with scenes,
@@ -395,6 +399,15 @@ int fyzz_hip_world_set_contact_notifications(fyzz_hip_world* world, int enabled)
 int fyzz_hip_world_contact_events(fyzz_hip_world* world, fyzz_contact_event* events, uint32_t capacity,
                                   uint32_t* n_events, fyzz_contact_event_point* points,
                                   uint32_t point_capacity, uint32_t* n_points);
+/* The last step's Dy inputs for the differentiable form (no counterpart
+ * in the reference): its managers in solver order and the friction patch
+ * each one read, with the manager's history index rewritten to its own
+ * index (DEVICE arrays of `capacity`; NULL queries the count).
+ * Synchronizes to read the count. */
+int fyzz_hip_world_record_dy_inputs(fyzz_hip_world* world, fyzz_dy_manager* managers, fyzz_dy_patch* history,
+                                    uint32_t capacity, uint32_t* n_managers);
+/* Device-to-device copy of the body records (creation order). */
+int fyzz_hip_world_copy_bodies(fyzz_hip_world* world, fyzz_dy_body* dst);
 /* Device pointers of the last step's manager list, contact impulses
  * [n][4], partitions and body records; valid until the next step. */
 int fyzz_hip_world_device_views(fyzz_hip_world* world, const fyzz_dy_manager** managers,

@@ -1,0 +1,65 @@
+"""GPU: the differentiable form. The device rollout (world forward with
+recorded contact sets, checkpoints every 16 steps, recomputed segments and
+the Dy adjoint in reverse) behind a torch.autograd.Function, against the
+FP64 CPU reference gradient of the same frozen-contact function (built
+from the device's recorded contact sets), whose own adjoint is checked
+against central differences on the CPU.
+
+Bar: 1e-3 relative on the gradient vector (v0, w0 of every body and the
+per-scene friction). Both sides freeze each step's rows at their own
+nominal trajectory (FP32 on the device, FP64 on the CPU), so a row whose
+clamp decision sits within rounding of its threshold may take the other
+branch; the vector norm absorbs such isolated rows."""
+import numpy as np
+import pytest
+
+from fyzz_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rollout_gradient_matches_fp64_reference(ctx, orc, torch_cuda):
+    import torch
+    from fyzz_amd.diff import Rollout, rollout_function
+    from fyzz_amd.dy import DY_BODY, DY_MANAGER, DY_PATCH, from_device
+    from oracle_bind import OracleDiff
+    counts, half, pos, quat, v0, w0, mu, target = scenes.diff_scenes(3)
+    steps = 20
+    ro = Rollout(ctx, half, pos, quat, counts, steps, checkpoint_every=16)
+    fn = rollout_function()
+    dev = torch.device("cuda")
+    tv = torch.tensor(v0, device=dev, requires_grad=True)
+    tw = torch.tensor(w0, device=dev, requires_grad=True)
+    tm = torch.tensor(mu, device=dev, requires_grad=True)
+    tgt = torch.tensor(target, dtype=torch.float32, device=dev)
+    p = fn.apply(tv, tw, tm, ro)
+    scene = torch.as_tensor(ro.scene_of_body, device=dev)
+    com = torch.zeros((len(counts), 3), device=dev).index_add_(0, scene, p) / 64.0
+    d = (com - tgt)[:, [0, 2]]
+    loss = (d * d).sum()
+    loss.backward()
+    gv, gw, gm = tv.grad.cpu().numpy(), tw.grad.cpu().numpy(), tm.grad.cpu().numpy()
+    # the FP64 reference over the same recorded contact sets
+    b0 = from_device(ro.checkpoints[0], DY_BODY)
+    mgs = [from_device(m, DY_MANAGER) if len(m) else np.zeros(0, DY_MANAGER) for m, _ in ro.records]
+    hs = [from_device(h, DY_PATCH) if len(h) else np.zeros(0, DY_PATCH) for _, h in ro.records]
+    st = np.zeros(1, [("position", "<f4", 3), ("orientation", "<f4", 4), ("static_friction", "<f4"),
+                      ("dynamic_friction", "<f4"), ("restitution", "<f4")])
+    st["orientation"][0, 3] = 1.0
+    st["static_friction"] = st["dynamic_friction"] = 0.5
+    ref = OracleDiff(orc.lib, b0, st, mgs, hs, 1 / 120)
+    mus = np.repeat(mu.astype(np.float64), counts)
+    pf = ref.eval(v0.astype(np.float64), w0.astype(np.float64), mus)
+    np.testing.assert_allclose(p.detach().cpu().numpy(), pf, rtol=0, atol=1e-4)
+    com_r = np.stack([pf[ro.scene_of_body == k].mean(0) for k in range(len(counts))])
+    dr = com_r - target
+    dr[:, 1] = 0.0
+    adj = 2.0 * dr[ro.scene_of_body] / 64.0
+    rv, rw, rm, _ = ref.grad(v0.astype(np.float64), w0.astype(np.float64), mus, adj)
+    rm_scene = np.array([rm[ro.scene_of_body == k].sum() for k in range(len(counts))])
+    g = np.concatenate([gv.ravel(), gw.ravel(), gm])
+    r = np.concatenate([rv.ravel(), rw.ravel(), rm_scene])
+    rel = np.linalg.norm(g - r) / np.linalg.norm(r)
+    print(f"loss {loss.item():.5g}; gradient norm {np.linalg.norm(r):.4g}; device vs FP64 relative {rel:.3g}; "
+          f"d/dmu {gm} vs {rm_scene}")
+    assert rel < 1e-3
