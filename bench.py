@@ -20,6 +20,11 @@ captured HIP graph; launch shapes picked by the autotuner during warm-up.
 Compared with the same scenes stepped as plain per-scene worlds (a sample
 of 32 scenes, per-scene launches) and with the reference NativeCpu.
 
+Workload `diff`: the differentiable form, 1,024 stacks of 64 boxes, 120
+steps; one step = forward (device world, contact sets recorded,
+checkpoints every 16 steps) plus backward (segments recomputed, Dy adjoint)
+of one rollout step on average; value = body-steps/s of forward + backward.
+
 Workload `sap`: broadphase only, 1,000,000 axis-aligned boxes (centers
 uniform in a 1000 m cube, lognormal half extents median 0.5 m, sigma 0.5).
 Each frame a seeded 10% of the boxes move by N(0, 0.05 m) per axis; one
@@ -57,7 +62,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", default="pile", choices=["pile", "sap", "batch"])
+    p.add_argument("--workload", default="pile", choices=["pile", "sap", "batch", "diff"])
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="wall-clock budget of the timed CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -539,6 +544,64 @@ def bench_batch(args):
     return result, (sample_recs, sample_half)
 
 
+def bench_diff(args):
+    import torch
+    from fyzz_amd import Context, scenes
+    from fyzz_amd.diff import Rollout, rollout_function
+
+    n_scenes, steps = 1024, 120
+    counts, half, pos, quat, v0, w0, mu, target = scenes.diff_scenes(n_scenes)
+    n = int(counts.sum())
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    ctx = Context(0, stream=stream.cuda_stream)
+    ro = Rollout(ctx, half, pos, quat, counts, steps, checkpoint_every=16)
+    fn = rollout_function()
+    scene = torch.as_tensor(ro.scene_of_body, device=dev)
+    tgt = torch.tensor(target, dtype=torch.float32, device=dev)
+
+    def once():
+        tv = torch.tensor(v0, device=dev, requires_grad=True)
+        tw = torch.tensor(w0, device=dev, requires_grad=True)
+        tm = torch.tensor(mu, device=dev, requires_grad=True)
+        t0 = time.perf_counter()
+        p = fn.apply(tv, tw, tm, ro)
+        com = torch.zeros((n_scenes, 3), device=dev).index_add_(0, scene, p) / 64.0
+        d = (com - tgt)[:, [0, 2]]
+        loss = (d * d).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return loss.item(), t1 - t0, t2 - t1, tv.grad, tm.grad
+
+    once()  # warm-up rollout (allocations, first-use paths)
+    loss, tf, tb, gv, gm = once()
+    total = tf + tb
+    result = {
+        "metric": METRIC,
+        "value": n * steps / total,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": steps,
+        "ms_per_step": total * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "differentiable: 1,024 stacks of 64 boxes, 120 steps, loss on the stacks' final "
+                               "COM x/z; gradients w.r.t. v0, w0 per body and friction per scene (frozen contacts, "
+                               "checkpoint every 16 steps)",
+                   "bodies": n, "forward_s": round(tf, 4), "backward_s": round(tb, 4), "loss": loss,
+                   "grad_v0_norm": float(gv.norm()), "grad_mu_norm": float(gm.norm())},
+        "roofline": None,
+    }
+    return result, (ro, counts, v0, w0, mu)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -556,6 +619,9 @@ def main():
             result["cpu_baseline"] = cpu_baseline_sap(n, min(args.cpu_frames, len(moves)), half, pos0, moves)
         else:
             result["cpu_baseline"] = None
+    elif args.workload == "diff":
+        result, _ = bench_diff(args)
+        result["cpu_baseline"] = None
     elif args.workload == "batch":
         args.steps = 240 if args.steps is None else args.steps
         args.warmup = 40 if args.warmup is None else args.warmup

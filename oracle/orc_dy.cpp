@@ -1149,17 +1149,86 @@ struct DiffRollout {
     std::vector<double> p0;
 };
 
+// Widens rows frozen along an FP32 nominal trajectory to the FP64 form.
+D::V wide(const Dy<float>::V& v) { return {double(v.x), double(v.y), double(v.z)}; }
+D::Frozen wide(const Dy<float>::Frozen& f) {
+    D::Frozen o;
+    o.dt = f.dt;
+    o.pos_iters = f.pos_iters;
+    o.vel_iters = f.vel_iters;
+    o.bodies.resize(f.bodies.size());
+    for (std::size_t i = 0; i < f.bodies.size(); ++i) {
+        const auto& a = f.bodies[i];
+        auto& b = o.bodies[i];
+        b.awake = a.awake;
+        b.has_row = a.has_row;
+        b.gravity = a.gravity;
+        b.inv_mass = a.inv_mass;
+        b.lm = a.lm;
+        b.am = a.am;
+        for (int k = 0; k < 3; ++k) b.c[k] = wide(a.c[k]);
+    }
+    o.mgrs.resize(f.mgrs.size());
+    for (std::size_t m = 0; m < f.mgrs.size(); ++m) {
+        const auto& a = f.mgrs[m];
+        auto& b = o.mgrs[m];
+        b.b0 = a.b0;
+        b.b1 = a.b1;
+        b.anchors = a.anchors;
+        for (int k = 0; k < 2; ++k) {
+            b.static_sf[k] = a.static_sf[k];
+            b.static_df[k] = a.static_df[k];
+        }
+        for (const auto& r : a.c) {
+            D::ContactRow c;
+            c.n = wide(r.n);
+            c.ra = wide(r.ra);
+            c.rb = wide(r.rb);
+            c.vm = r.vm;
+            c.biased = r.biased;
+            c.unbiased = r.unbiased;
+            c.applied = r.applied;
+            c.max_impulse = r.max_impulse;
+            c.raxn = wide(r.raxn);
+            c.rbxn = wide(r.rbxn);
+            c.kv = r.kv;
+            c.cb = r.cb;
+            c.cu = r.cu;
+            b.c.push_back(c);
+        }
+        for (const auto& r : a.f) {
+            D::FrictionRow c;
+            c.t = wide(r.t);
+            c.ra = wide(r.ra);
+            c.rb = wide(r.rb);
+            c.vm = r.vm;
+            c.bias = r.bias;
+            c.target = r.target;
+            c.applied = r.applied;
+            c.ax0 = wide(r.ax0);
+            c.ax1 = wide(r.ax1);
+            b.f.push_back(c);
+        }
+    }
+    return o;
+}
+
 }  // namespace
 
 extern "C" {
 
 // Frozen-row rollout of the differentiable form (FP64). create: runs the
-// nominal FP64 Dy steps from bodies0 with each step's recorded managers
-// and history (managers concatenated, nm[s] per step; history likewise,
-// manager history indices local to the step) and keeps every step's rows.
+// nominal Dy steps from bodies0 with each step's recorded managers and
+// history (managers concatenated, nm[s] per step; history likewise, manager
+// history indices local to the step) and keeps every step's rows. With
+// nominal_fp32 the nominal trajectory is the FP32 restatement's (rows
+// widened to FP64), else the FP64 twin's. The Dy step has discontinuous
+// decisions on the poses (friction anchors kept or reset), so two nominal
+// trajectories of different precision can freeze different rows; the FP32
+// nominal freezes the rows an FP32 implementation freezes.
 void* orc_diff_create(std::uint32_t nb, const fyzz_dy_body* bodies0, std::uint32_t ns, const fyzz_dy_static* statics,
                       std::uint32_t steps, const std::uint32_t* nm, const fyzz_dy_manager* managers,
-                      const std::uint32_t* nh, const fyzz_dy_patch* history, float dt) {
+                      const std::uint32_t* nh, const fyzz_dy_patch* history, float dt, int nominal_fp32) {
     auto* R = new DiffRollout();
     std::vector<fyzz_dy_body> b(bodies0, bodies0 + nb);
     R->p0.resize(3 * nb);
@@ -1171,11 +1240,22 @@ void* orc_diff_create(std::uint32_t nb, const fyzz_dy_body* bodies0, std::uint32
         std::vector<fyzz_dy_patch> patches(std::max<std::uint32_t>(nm[s], 1));
         std::vector<float> imp(4 * std::max<std::uint32_t>(nm[s], 1));
         std::vector<double> st(13ull * nb);
-        D::patch_order_.assign(nm[s], {});
-        D::capture_ = &R->steps[s];
-        const int rc = D::step(double(dt), nb, b.data(), ns, statics, nm[s], managers + mo, history + ho, nh[s],
-                               patches.data(), imp.data(), nullptr, st.data());
-        D::capture_ = nullptr;
+        int rc;
+        if (nominal_fp32) {
+            Dy<float>::Frozen f;
+            Dy<float>::patch_order_.assign(nm[s], {});
+            Dy<float>::capture_ = &f;
+            rc = Dy<float>::step(dt, nb, b.data(), ns, statics, nm[s], managers + mo, history + ho, nh[s],
+                                 patches.data(), imp.data(), nullptr, nullptr);
+            Dy<float>::capture_ = nullptr;
+            R->steps[s] = wide(f);
+        } else {
+            D::patch_order_.assign(nm[s], {});
+            D::capture_ = &R->steps[s];
+            rc = D::step(double(dt), nb, b.data(), ns, statics, nm[s], managers + mo, history + ho, nh[s],
+                         patches.data(), imp.data(), nullptr, st.data());
+            D::capture_ = nullptr;
+        }
         if (rc != 0) {
             delete R;
             return nullptr;

@@ -512,9 +512,11 @@ def oracle_contact_events(lib, managers, impulses, keys, reversed_flags, prev_pa
 class OracleDiff:
     """The frozen-row rollout of the differentiable form in FP64
     (oracle/orc_dy.cpp orc_diff_*): built from the initial body records and
-    each step's recorded managers and history."""
+    each step's recorded managers and history. nominal_fp32 freezes the
+    rows along the FP32 restatement's trajectory (the one an FP32
+    implementation freezes), else along the FP64 twin's."""
 
-    def __init__(self, lib, bodies0, statics, managers_per_step, history_per_step, dt):
+    def __init__(self, lib, bodies0, statics, managers_per_step, history_per_step, dt, nominal_fp32=False):
         self.lib = lib
         lib.orc_diff_create.restype = C.c_void_p
         self.nb = len(bodies0)
@@ -528,7 +530,7 @@ class OracleDiff:
         st = np.ascontiguousarray(statics)
         self.h = lib.orc_diff_create(C.c_uint32(self.nb), P(b0), C.c_uint32(len(st)), P(st), C.c_uint32(len(nm)),
                                      P(nm), P(np.ascontiguousarray(mg)), P(nh), P(np.ascontiguousarray(hist)),
-                                     C.c_float(dt))
+                                     C.c_float(dt), C.c_int(1 if nominal_fp32 else 0))
         if not self.h:
             raise RuntimeError("oracle nominal rollout failed")
         self.h = C.c_void_p(self.h)

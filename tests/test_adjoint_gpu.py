@@ -5,11 +5,19 @@ FP64 CPU reference gradient of the same frozen-contact function (built
 from the device's recorded contact sets), whose own adjoint is checked
 against central differences on the CPU.
 
-Bar: 1e-3 relative on the gradient vector (v0, w0 of every body and the
-per-scene friction). Both sides freeze each step's rows at their own
-nominal trajectory (FP32 on the device, FP64 on the CPU), so a row whose
-clamp decision sits within rounding of its threshold may take the other
-branch; the vector norm absorbs such isolated rows."""
+The reference freezes each step's rows along the FP32 restatement's
+trajectory from the same initial records, which is the device's trajectory
+bit for bit (the recorded inputs replay the device world exactly), so both
+sides differentiate the same function. An FP64 nominal trajectory is not a
+usable reference here: the Dy step keeps or resets friction anchors on
+pose-dependent tests, and on these three stacks one such decision goes the
+other way in FP64 within 20 steps (final positions then differ by 1e-3).
+
+Bars: final positions within 1e-4 of the FP64 evaluation (FP32 PGS and
+integration rounding over 20 steps); gradient vector (v0, w0 of every body
+and the per-scene friction) within 1e-3 relative, a row whose clamp sits
+within rounding of its threshold may take the other branch in FP64 and the
+vector norm absorbs such isolated rows."""
 import numpy as np
 import pytest
 
@@ -47,7 +55,7 @@ def test_rollout_gradient_matches_fp64_reference(ctx, orc, torch_cuda):
                       ("dynamic_friction", "<f4"), ("restitution", "<f4")])
     st["orientation"][0, 3] = 1.0
     st["static_friction"] = st["dynamic_friction"] = 0.5
-    ref = OracleDiff(orc.lib, b0, st, mgs, hs, 1 / 120)
+    ref = OracleDiff(orc.lib, b0, st, mgs, hs, 1 / 120, nominal_fp32=True)
     mus = np.repeat(mu.astype(np.float64), counts)
     pf = ref.eval(v0.astype(np.float64), w0.astype(np.float64), mus)
     np.testing.assert_allclose(p.detach().cpu().numpy(), pf, rtol=0, atol=1e-4)

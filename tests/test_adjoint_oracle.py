@@ -90,3 +90,34 @@ def test_adjoint_matches_central_differences(orc):
           f"{comp.max():.3g} over {int(big.sum())}; d/dmu {gm.sum():.6g} vs {f6[-1]:.6g}")
     assert rel < 1e-3
     assert comp.max() < 1e-4
+
+
+def test_frozen_rollout_reproduces_its_nominal_trajectory(orc):
+    """At the recorded inputs the frozen-row function is the nominal
+    trajectory itself: rows frozen along the FP32 restatement's run give
+    that run's final positions up to FP64-vs-FP32 arithmetic in the PGS and
+    integration (a dropped row term, a wrong lever arm or a widened field
+    left out moves them by far more than the 1e-5 bar); rows frozen along
+    the FP64 twin's run give the same to the same bar here (this scene
+    takes no pose-dependent anchor decision differently in the two
+    precisions)."""
+    sc, mu, _, steps = stack_case()
+    ow = OracleWorld(orc, sc)
+    ow.bodies["static_friction"] = mu
+    ow.bodies["dynamic_friction"] = mu
+    b0 = ow.bodies.copy()
+    mgs, hists = [], []
+    for _ in range(steps):
+        ow.step(1 / 120)
+        _, mg, hist = ow.last_dy_inputs
+        mgs.append(mg)
+        hists.append(hist)
+    p32 = ow.bodies["position"].astype(np.float64)
+    b0["counted_interactions"] = 0
+    args = (b0["linear_velocity"].astype(np.float64), b0["angular_velocity"].astype(np.float64),
+            np.full(sc.n, float(mu)))
+    for fp32 in (True, False):
+        diff = OracleDiff(orc.lib, b0, ow.statics, mgs, hists, 1 / 120, nominal_fp32=fp32)
+        gap = np.abs(diff.eval(*args) - p32).max()
+        print(f"nominal {'FP32' if fp32 else 'FP64'}: final positions within {gap:.3g} of the FP32 run")
+        assert gap < 1e-5
