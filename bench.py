@@ -67,6 +67,13 @@ def parse():
                    help="wall-clock budget of the timed CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-frames", type=int, default=1)
+    p.add_argument("--save-state", default=None,
+                   help="pile: save the body states after the timed region (.npz)")
+    p.add_argument("--load-state", default=None,
+                   help="pile: start from saved body states instead of the drop (counter passes over a few "
+                        "steps of the formed pile)")
+    p.add_argument("--plain-launch", action="store_true",
+                   help="plain instead of cooperative launches of the Dy grid-barrier kernels (counter profilers)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run one plain 256 MiB device copy first (PMC byte-counter calibration)")
     return p.parse_args()
@@ -158,13 +165,7 @@ def bench_sap(args):
     d_moves = [(torch.from_numpy(s.astype(np.int64)).to(dev), torch.from_numpy(s.astype(np.int32)).to(dev),
                 torch.from_numpy(p).to(dev)) for s, p in moves]
     if args.calibrate_copy:
-        # one plain copy of known size: 256 MiB read + 256 MiB written
-        src = torch.empty(64 << 20, dtype=torch.int32, device=dev).fill_(1)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy(dev)
     bp = SapBroadphase(ctx, np.arange(n, dtype=np.uint32), np.ones(n, np.uint8))
     bp.advance(d_pos, d_quat, d_half, d_off)  # frame 0: full build
     torch.cuda.synchronize()
@@ -309,6 +310,18 @@ def cpu_baseline_pile(records, half, budget_s):
             "seconds": round(dt, 3), "steps": steps}
 
 
+def calibrate_copy(dev):
+    """One plain copy of known size (256 MiB read + 256 MiB written), the
+    reference dispatch for scaling the PMC byte counters."""
+    import torch
+    src = torch.empty(64 << 20, dtype=torch.int32, device=dev).fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def bench_pile(args):
     import torch
     from fyzz_amd import Context, scenes
@@ -320,10 +333,19 @@ def bench_pile(args):
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream(dev)
     ctx = Context(0, stream=stream.cuda_stream)
+    if args.plain_launch:
+        ctx.set_cooperative(False)
+    if args.calibrate_copy:
+        calibrate_copy(dev)
     w = World(ctx, n + 1)
     fid = w.add_static_floor(0.0)
     w.set_body_material(fid, 0.5, 0.5, 0.0)
-    w.add_dynamic_boxes(sc.half, sc.pos, sc.quat, friction=0.5, restitution=0.0)
+    if args.load_state:
+        s0 = np.load(args.load_state)
+        w.add_dynamic_boxes(sc.half, s0["position"], s0["orientation"], linear=s0["linear_velocity"],
+                            angular=s0["angular_velocity"], friction=0.5, restitution=0.0)
+    else:
+        w.add_dynamic_boxes(sc.half, sc.pos, sc.quat, friction=0.5, restitution=0.0)
     for _ in range(args.warmup):
         w.step(dt)
     torch.cuda.synchronize()
@@ -349,6 +371,10 @@ def bench_pile(args):
     st = w.stats()
     if st["flags"]:
         raise RuntimeError(f"world status flags {st['flags']:#x}")
+    if args.save_state:
+        r = w.body_records()
+        np.savez(args.save_state, **{f: np.ascontiguousarray(r[f]) for f in
+                                     ("position", "orientation", "linear_velocity", "angular_velocity")})
     dominant = max((k for k in stages if pile_algorithmic_bytes(k, tot, n) is not None),
                    key=lambda k: stages[k]["ms_total"])
     alg = pile_algorithmic_bytes(dominant, tot, n)
@@ -383,10 +409,11 @@ def bench_pile(args):
     return result, (w, sc)
 
 
-def cpu_baseline_scenes(records_list, halves, budget_s):
+def cpu_baseline_scenes(records_list, halves, budget_s, max_rounds):
     """The compiled reference NativeCpu on a sample of scenes, each loaded
-    with its device end state: 1 untimed step, then timed steps until the
-    budget is spent."""
+    with its device end state: 1 untimed step, then rounds of one step per
+    scene until the budget is spent or max_rounds (the device's timed step
+    count) are done."""
     import ctypes as C
     P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
     ref_path = ROOT / "oracle" / "_ref" / "libfyzzref.so"
@@ -420,7 +447,7 @@ def cpu_baseline_scenes(records_list, halves, budget_s):
             body_steps += nb
         rounds += 1
         dt = time.perf_counter() - t0
-        if dt >= budget_s or rounds >= 50:
+        if dt >= budget_s or rounds >= max_rounds:
             break
     for sc, _ in handles:
         ref.ref_scene_destroy(sc)
@@ -539,8 +566,9 @@ def bench_batch(args):
         "stages": stages,
     }
     recs = w.body_records()
-    sample_recs = [recs[first[k]:first[k + 1]] for k in sample[:8]]
-    sample_half = [half[first[k]:first[k + 1]] for k in sample[:8]]
+    cpu_sample = np.linspace(0, n_scenes - 1, 128).astype(np.int64)
+    sample_recs = [recs[first[k]:first[k + 1]] for k in cpu_sample]
+    sample_half = [half[first[k]:first[k + 1]] for k in cpu_sample]
     return result, (sample_recs, sample_half)
 
 
@@ -627,7 +655,7 @@ def main():
         args.warmup = 40 if args.warmup is None else args.warmup
         result, (recs, halves) = bench_batch(args)
         if not args.no_cpu_baseline:
-            result["cpu_baseline"] = cpu_baseline_scenes(recs, halves, args.cpu_seconds)
+            result["cpu_baseline"] = cpu_baseline_scenes(recs, halves, args.cpu_seconds, args.steps)
         else:
             result["cpu_baseline"] = None
     else:

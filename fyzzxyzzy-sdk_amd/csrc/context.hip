@@ -38,6 +38,10 @@ int fyzz_hip_set_stream(fyzz_hip_context* ctx, void* hip_stream) {
     return guarded(ctx, [&] { ctx->stream = static_cast<hipStream_t>(hip_stream); });
 }
 
+int fyzz_hip_set_cooperative(fyzz_hip_context* ctx, int cooperative) {
+    return guarded(ctx, [&] { ctx->cooperative = cooperative != 0; });
+}
+
 int fyzz_hip_synchronize(fyzz_hip_context* ctx) {
     return guarded(ctx, [&] { FYZZ_HIP_CHECK(hipStreamSynchronize(ctx->stream)); });
 }

@@ -91,6 +91,7 @@ struct ProfScope {
 struct fyzz_hip_context {
     int device = 0;
     hipStream_t stream = nullptr;  // null = the device's default stream
+    bool cooperative = true;       // grid-barrier kernels: cooperative or plain launches
     std::string last_error;
     fyzz_hip::Profiler prof;
 };

@@ -1510,7 +1510,8 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
     }
     {
         ProfScope ps(&prof, "dy.partition", st);
-        launch_coop(reinterpret_cast<const void*>(&dy::k_partition), d->coop_blocks_partition, w, st, d->plain_launch);
+        launch_coop(reinterpret_cast<const void*>(&dy::k_partition), d->coop_blocks_partition, w, st,
+                    d->plain_launch || !d->ctx->cooperative);
     }
     if (!partition_only && nm) {
         ProfScope ps(&prof, "dy.prep", st);
@@ -1526,7 +1527,7 @@ void dy_solve(fyzz_hip_dy* d) {
     if (!d->ws.nm) return;
     ProfScope ps(&d->ctx->prof, "dy.pgs", d->ctx->stream);
     launch_coop(reinterpret_cast<const void*>(&dy::k_pgs), d->coop_blocks_pgs, d->ws, d->ctx->stream,
-                d->plain_launch);
+                d->plain_launch || !d->ctx->cooperative);
 }
 
 void dy_integrate(fyzz_hip_dy* d, float* impulses, uint32_t* partitions) {
@@ -1593,7 +1594,8 @@ void dy_adjoint(fyzz_hip_dy* d, const float* adj_v_out, const float* adj_w_out, 
     const u32 span = std::max<u32>(w.nb, 4 * w.nm);
     if (span) hipLaunchKernelGGL(dy::k_adj_init, dim3(grid_for(span, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
     if (w.nm) {
-        launch_coop(reinterpret_cast<const void*>(&dy::k_pgs_adjoint), d->coop_blocks_pgs, w, st, d->plain_launch);
+        launch_coop(reinterpret_cast<const void*>(&dy::k_pgs_adjoint), d->coop_blocks_pgs, w, st,
+                    d->plain_launch || !d->ctx->cooperative);
         hipLaunchKernelGGL(dy::k_adj_rows, dim3(grid_for(w.nm, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
     }
     if (w.nb) hipLaunchKernelGGL(dy::k_adj_bodies, dim3(grid_for(w.nb, dy::kBlock)), dim3(dy::kBlock), 0, st, w);

@@ -13,6 +13,7 @@
 //   planes[6][n]       current encoded keys (min x,y,z then max x,y,z)
 //   rec_cur[n]         32-B record: keys[6], body id, dynamic<<31 | stamp
 //   rec_prev[n]        the record before this frame's update (moved boxes)
+//   coarse[n]          16-B truncated bounds covering old and new keys
 //   sorted[2][3][2n]   (key, payload) per axis, current and previous frame;
 //                      payload = slot << 1 | is_max
 //   pairs[2][cap]      sorted u64 (lo slot << 32 | hi slot)
@@ -75,10 +76,25 @@ __global__ __launch_bounds__(256) void k_bounds_planes(u32 n, const float* pos, 
     for (int a = 0; a < 6; ++a) planes[static_cast<std::size_t>(a) * n + i] = k[a];
 }
 
+// Coarse bounds: per axis (key_lo >> 16) << 16 | (key_hi >> 16). The
+// encoded keys are monotone in the coordinate, so truncating both ends
+// keeps every overlap: two boxes whose key intervals overlap on an axis
+// have overlapping coarse intervals there. 16 B per box (the fourth word
+// is unused), small enough to stay in L2 while the crossing walks test
+// candidates against it before loading full records.
+__device__ inline u32 coarse_axis(u32 lo, u32 hi) { return (lo & 0xffff0000u) | (hi >> 16); }
+__device__ inline bool coarse_overlap(const uint4& a, const uint4& b) {
+    const u32 av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z};
+    bool ov = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ov &= !((av[k] >> 16) > (bv[k] & 0xffffu) || (bv[k] >> 16) > (av[k] & 0xffffu));
+    return ov;
+}
+
 // Full (re)build: every box gets its keys; no moved list.
 __global__ __launch_bounds__(256) void k_bounds_full(u32 n, const float* pos, const float* quat,
                                                      const float* half, const float* off,
-                                                     u32* planes, Rec* cur, u32 stamp, u32* cnt) {
+                                                     u32* planes, Rec* cur, uint4* coarse, u32 stamp, u32* cnt) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     u32 k[6];
@@ -91,15 +107,18 @@ __global__ __launch_bounds__(256) void k_bounds_full(u32 n, const float* pos, co
     }
     r.k[7] = (r.k[7] & 0x80000000u) | stamp;
     cur[i] = r;
+    coarse[i] = make_uint4(coarse_axis(k[0], k[3]), coarse_axis(k[1], k[4]), coarse_axis(k[2], k[5]), 0u);
 }
 
 // Incremental: recompute the listed boxes (or all when dirty == null);
 // boxes whose keys changed save their previous record, take this frame's
 // stamp and join the moved list. A slot listed twice is processed once.
+// A moved box's coarse bounds cover its previous and new keys; a box that
+// then rests keeps them, which still covers its (unchanged) keys.
 __global__ __launch_bounds__(256) void k_bounds_update(u32 count, const u32* dirty, u32 n,
                                                        const float* pos, const float* quat,
                                                        const float* half, const float* off,
-                                                       u32* planes, Rec* cur, Rec* prev,
+                                                       u32* planes, Rec* cur, Rec* prev, uint4* coarse,
                                                        u32* seen, u32 stamp, u32* moved, u32* moved_bits,
                                                        u32* cnt) {
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,6 +137,9 @@ __global__ __launch_bounds__(256) void k_bounds_update(u32 count, const u32* dir
     for (int a = 0; a < 6; ++a) changed |= r.k[a] != k[a];
     if (!changed) return;
     prev[i] = r;
+    coarse[i] = make_uint4(coarse_axis(min(r.k[0], k[0]), max(r.k[3], k[3])),
+                           coarse_axis(min(r.k[1], k[1]), max(r.k[4], k[4])),
+                           coarse_axis(min(r.k[2], k[2]), max(r.k[5], k[5])), 0u);
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
         r.k[a] = k[a];
@@ -203,6 +225,7 @@ struct CrossArgs {
     const u32* moved_bits;
     const Rec* cur;
     const Rec* prev;
+    const uint4* coarse;
     u64* out;
     u32* cnt;
     u32 cap;
@@ -249,14 +272,18 @@ __device__ inline bool walk_inside(const CrossArgs& A, const WalkGeom& g, u32 st
 }
 // A crossed entry: a candidate pair when it is the other kind of endpoint
 // of another box; re-tested on all three axes with old and new records.
+// ca: the walking box's coarse bounds (its old and new keys). A candidate
+// whose coarse bounds miss them on some axis overlaps it neither before
+// nor after, so its status cannot change.
 __device__ inline void walk_visit(const CrossArgs& A, const Walk& w, const WalkGeom& g, u32 j, const Rec& an,
-                                  const Rec& ao) {
+                                  const Rec& ao, const uint4& ca) {
     const u32 pf = g.P[j];
     const u32 b = pf >> 1;
+    if ((pf & 1u) == w.kind || b == w.a) return;
     const bool b_moved = (A.moved_bits[b >> 5] >> (b & 31u)) & 1u;
     // an unmoved endpoint sits at the same value in both arrays: the
     // new-array walk already saw it
-    if ((pf & 1u) != w.kind && b != w.a && (b_moved || !w.arr)) {
+    if ((b_moved || !w.arr) && coarse_overlap(ca, A.coarse[b])) {
         const Rec bn = A.cur[b];
         if (pair_allowed(an, bn)) {
             const Rec bo = b_moved ? A.prev[b] : bn;
@@ -280,10 +307,11 @@ __global__ __launch_bounds__(256) void k_cross(CrossArgs A) {
         const u32 v_new = an.k[w.axis + 3 * w.kind];
         if (v_old == v_new) continue;
         const WalkGeom g = walk_geom(A, w, v_old, v_new);
+        const uint4 ca = A.coarse[w.a];
         for (u32 step = 1;; ++step) {
             u32 j;
             if (!walk_inside(A, g, step, j)) break;
-            walk_visit(A, w, g, j, an, ao);
+            walk_visit(A, w, g, j, an, ao, ca);
             if (step == kShortWalk) {
                 const u32 slot = atomicAdd(&A.cnt[kLongWalks], 1u);
                 if (slot < A.long_cap)
@@ -309,10 +337,11 @@ __global__ __launch_bounds__(256) void k_cross_long(CrossArgs A) {
         const Rec an = A.cur[w.a];
         const Rec ao = A.prev[w.a];
         const WalkGeom g = walk_geom(A, w, ao.k[w.axis + 3 * w.kind], an.k[w.axis + 3 * w.kind]);
+        const uint4 ca = A.coarse[w.a];
         for (u32 base = kShortWalk;; base += kWave) {
             u32 j = 0;
             const bool inside = walk_inside(A, g, base + lane + 1, j);
-            if (inside) walk_visit(A, w, g, j, an, ao);
+            if (inside) walk_visit(A, w, g, j, an, ao, ca);
             if (__ballot(inside) != ~0ull) break;
         }
     }
@@ -493,6 +522,7 @@ struct Sap {
     DevBuf<u32> planes, seen, moved, moved_bits, counters, sorted_key[2], sorted_val[2], rank[2], tmp_key, tmp_val,
         radix_counts, radix_totals, sel_tiles, dummy_a, dummy_b;
     DevBuf<Rec> rec_cur, rec_prev;
+    DevBuf<uint4> coarse;
     DevBuf<u64> pairs[2], work_a, work_b, found, lost, kept;
     std::vector<u32> host_counters;
 
@@ -529,14 +559,14 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
     ProfScope ps(prof, "sap.bounds", st);
     if (full) {
         if (n) hipLaunchKernelGGL(k_bounds_full, dim3(grid_for(n, 256)), dim3(256), 0, st, n, pos, quat, half, off,
-                                  s.planes.p, s.rec_cur.p, stamp, cnt);
+                                  s.planes.p, s.rec_cur.p, s.coarse.p, stamp, cnt);
     } else {
         const u32 words = (n + 31) / 32;
         if (words) hipLaunchKernelGGL(k_fill, dim3(grid_for(words, 256)), dim3(256), 0, st, s.moved_bits.p, words, 0u);
         const u32 count = dirty ? n_dirty : n;
         if (count)
             hipLaunchKernelGGL(k_bounds_update, dim3(grid_for(count, 256)), dim3(256), 0, st, count, dirty, n, pos,
-                               quat, half, off, s.planes.p, s.rec_cur.p, s.rec_prev.p, s.seen.p, stamp,
+                               quat, half, off, s.planes.p, s.rec_cur.p, s.rec_prev.p, s.coarse.p, s.seen.p, stamp,
                                s.moved.p, s.moved_bits.p, cnt);
     }
     FYZZ_HIP_CHECK(hipGetLastError());
@@ -574,7 +604,7 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
         long pi = prof->begin("sap.pairs", st);
         CrossArgs A{n2, cnt + kMoved, s.moved.p, s.sorted_key[nxt].p, s.sorted_val[nxt].p, s.sorted_key[s.cur].p,
                     s.sorted_val[s.cur].p, s.rank[nxt].p, s.rank[s.cur].p, s.moved_bits.p, s.rec_cur.p,
-                    s.rec_prev.p, s.work_a.p, cnt, s.delta_cap, s.long_walks.p, 12u * n};
+                    s.rec_prev.p, s.coarse.p, s.work_a.p, cnt, s.delta_cap, s.long_walks.p, 12u * n};
         hipLaunchKernelGGL(k_cross, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12, 256), 8192u)),
                            dim3(256), 0, st, A);
         hipLaunchKernelGGL(k_cross_long, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12 * kWave, 256),
@@ -704,6 +734,7 @@ int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_
         s.dummy_b.reserve(lcap);
         s.rec_cur.reserve(n);
         s.rec_prev.reserve(n);
+        s.coarse.reserve(n);
         s.work_a.reserve(lcap);
         s.work_b.reserve(lcap);
         s.found.reserve(lcap);

@@ -25,6 +25,7 @@ SIGNATURES = {
     "fyzz_hip_last_error": (C.c_char_p, [vp]),
     "fyzz_hip_set_stream": (C.c_int, [vp, vp]),
     "fyzz_hip_synchronize": (C.c_int, [vp]),
+    "fyzz_hip_set_cooperative": (C.c_int, [vp, C.c_int]),
     "fyzz_hip_profile_enable": (C.c_int, [vp, C.c_int]),
     "fyzz_hip_profile_read": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_double), u64p]),
     "fyzz_hip_sap_box_bounds": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp]),
@@ -156,6 +157,11 @@ class Context:
 
     def synchronize(self):
         self.call("fyzz_hip_synchronize")
+
+    def set_cooperative(self, cooperative: bool = True):
+        """Cooperative (default) or plain launches of the Dy grid-barrier
+        kernels; results are identical."""
+        self.call("fyzz_hip_set_cooperative", 1 if cooperative else 0)
 
     def profile(self, enable: bool = True):
         self.call("fyzz_hip_profile_enable", 1 if enable else 0)

@@ -129,6 +129,14 @@ const char* fyzz_hip_last_error(const fyzz_hip_context* ctx);
  * default stream. */
 int fyzz_hip_set_stream(fyzz_hip_context* ctx, void* hip_stream);
 int fyzz_hip_synchronize(fyzz_hip_context* ctx);
+/* Launch mode of the Dy grid-barrier kernels (no counterpart in the
+ * reference): cooperative != 0 (default) uses hipLaunchCooperativeKernel,
+ * which checks that every workgroup is resident; 0 launches the same
+ * resident-sized grid as a plain kernel, as inside captured graphs (for
+ * tools that do not intercept cooperative launches, such as counter
+ * profilers). Results are identical; a non-resident grid sets the
+ * FYZZ_DY_LOGIC status bit instead of waiting forever. */
+int fyzz_hip_set_cooperative(fyzz_hip_context* ctx, int cooperative);
 /* Stage timing: when enabled, named launches are bracketed by HIP events
  * on the context's stream (stands in for Context::profile_batch_gpu_nanoseconds,
  * include/fyzzxyzzy/compute.h:299). profile_read synchronizes, returns the
