@@ -7,7 +7,7 @@
 // Reference (src/native/scene.cpp):
 //   prepare_unconstrained_velocities            10383-10507   k_pre
 //   prepare_solver_bodies                       6940-7112     k_rows
-//   partition_constraints                       7113-7489     k_partition
+//   partition_constraints                       7113-7489     k_part_*
 //   prepare_constraints (contact managers)      7490-7515, 7763-8511   k_prep
 //   cached_friction_patch_valid                 3981-4010     history_valid
 //   solve_constraint_task                       8853-9148     k_pgs
@@ -22,9 +22,9 @@
 // (the library builds with -ffp-contract=off). Partitions reproduce the
 // sequential first-fit over 32-bit masks exactly: a manager's partition
 // depends only on the masks its two bodies carry after the earlier managers
-// incident to them, so managers are coloured in dependency rounds (a
-// manager is ready once its predecessor on each body is coloured) inside
-// one cooperative launch. Within a partition no two managers share a
+// incident to them, so a manager is coloured as soon as its predecessor on
+// each body is (one cooperative launch, no global rounds). Within a
+// partition no two managers share a
 // dynamic body and the static world's solver body keeps a zero state, so
 // solving a partition's managers concurrently gives the sequential sweep's
 // values bit for bit.
@@ -36,6 +36,7 @@
 
 #include "../../include/fyzz_hip.h"
 #include "context.hpp"
+#include "select.hpp"
 #include "stages.hpp"
 #include "vecmath.hpp"
 
@@ -51,7 +52,7 @@ constexpr int kCoopBlocks = 64;       // workgroups of the cooperative launches
 
 // device counters
 enum : u32 { kFront0 = 0, kFront1, kFront2, kPend, kNParts, kPosIters, kVelIters, kRounds, kContacts, kFrictionRows,
-             kCtrWords = 16 };
+             kNbWord, kCtrWords = 16 };
 
 // one contact row (SolverContactPoint fields the solve reads), 64 B
 struct alignas(16) CRow {
@@ -90,7 +91,8 @@ struct Ws {
     u32* has_row;
     u32 *deg, *off, *fill, *max_dyn, *body_mask;
     // per manager
-    u32 *inc, *pred, *succ, *cnt, *part, *mask_after, *front, *pend, *sorted, *order;
+    u32 *inc, *pred, *part, *pend, *sorted, *order;
+    u64* entry;  // [2 nm] partition colouring: ready | level << 32 | mask after the manager, per side
     CRow* crow;
     FRow* frow;
     u32 *part_count, *part_off, *part_fill;
@@ -312,151 +314,154 @@ __global__ __launch_bounds__(kBlock) void k_rows(Ws W) {
 
 __device__ inline bool dyn_dyn(const fyzz_dy_manager& M) { return M.body0 >= 0 && M.body1 >= 0; }
 
-// Exclusive scan of deg[0, nb) into off[0, nb] across the grid.
-__device__ void grid_scan_degrees(const Ws& W, u32& gen) {
-    __shared__ u32 scratch[kBlock / kWave + 1];
-    const u32 G = gridDim.x;
-    const u32 chunk = (W.nb + G - 1) / G;
-    const u32 lo = min(W.nb, blockIdx.x * chunk), hi = min(W.nb, lo + chunk);
-    u32 s = 0;
-    for (u32 i = lo + threadIdx.x; i < hi; i += kBlock) s += W.deg[i];
-    u32 total;
-    (void)block_exclusive_scan<kBlock>(s, scratch, &total);
-    if (threadIdx.x == 0) W.blocksums[blockIdx.x] = total;
-    grid_sync(W, gen);
-    if (blockIdx.x == 0) {
-        u32 carry = 0;
-        for (u32 t0 = 0; t0 < G; t0 += kBlock) {
-            const u32 t = t0 + threadIdx.x;
-            const u32 v = t < G ? W.blocksums[t] : 0u;
-            const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
-            if (t < G) W.blocksums[G + t] = carry + ex;
-            carry += total;
-        }
-        if (threadIdx.x == 0) W.off[W.nb] = carry;
-    }
-    grid_sync(W, gen);
-    u32 carry = W.blocksums[G + blockIdx.x];
-    for (u32 t0 = lo; t0 < hi; t0 += kBlock) {
-        const u32 i = t0 + threadIdx.x;
-        const u32 v = i < hi ? W.deg[i] : 0u;
-        const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
-        if (i < hi) W.off[i] = carry + ex;
-        carry += total;
-    }
-}
-
 // First-fit partitions over 32-bit masks (scene.cpp:7352-7445) for one task
 // spanning the manager list; stable scatter by partition (7447-7489).
-__global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
-    u32 gen = barrier_start(W);
-    const u32 tid = blockIdx.x * kBlock + threadIdx.x;
-    const u32 nthreads = gridDim.x * kBlock;
-    const u32 nm = mcount(W);
-    __shared__ u32 scratch[kBlock / kWave + 1];
-    // P0: clear per-body and per-partition state
-    for (u32 i = tid; i < W.nb; i += nthreads) {
+//
+// The data-parallel phases are plain kernels over bodies or managers:
+// degrees, their scan, incidence lists (CSR by body), each body's list in
+// manager order with the predecessor of every dynamic/dynamic entry,
+// static managers after the body's dynamic partitions, sizes, offsets and
+// the scatter. The colouring is one cooperative kernel: a manager's
+// partition depends only on the masks its two bodies carry after their
+// earlier managers, so each thread walks its managers in increasing order
+// (m = tid, tid + threads, ...) and colours one as soon as the entries of
+// its two predecessors are published. An entry is one 64-bit word (ready
+// bit, chain level, mask after the manager) written and read as a single
+// relaxed agent-scope atomic, cleared by the degree kernel every step. The
+// smallest uncoloured manager is always ready and its thread is resident,
+// so the colouring always advances; its waits are bounded regardless.
+
+constexpr u64 kEntryReady = 1ull << 63;
+
+// per body: degree, CSR fill cursor, highest dynamic partition
+__global__ __launch_bounds__(kBlock) void k_part_clear(Ws W) {
+    const u32 i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < W.nb) {
         W.deg[i] = 0;
         W.fill[i] = 0;
         W.max_dyn[i] = 0;
     }
-    for (u32 i = tid; i < kMaxParts; i += nthreads) {
+    if (i < kMaxParts) {
         W.part_count[i] = 0;
         W.part_fill[i] = 0;
     }
-    if (tid <= kNParts) W.ctr[tid] = 0;  // frontier, pending and partition counters
-    grid_sync(W, gen);
-    // P1: incidence degrees (dynamic sides)
-    for (u32 m = tid; m < nm; m += nthreads) {
-        const fyzz_dy_manager& M = W.mg[m];
-        W.part[m] = kNone;
-        if (!manager_ok(W, M)) continue;
-        if (M.body0 >= 0) atomicAdd(&W.deg[M.body0], 1u);
-        if (M.body1 >= 0) atomicAdd(&W.deg[M.body1], 1u);
-    }
-    grid_sync(W, gen);
-    grid_scan_degrees(W, gen);
-    grid_sync(W, gen);
-    // P3: incidence lists, entry = manager << 1 | side
-    for (u32 m = tid; m < nm; m += nthreads) {
-        const fyzz_dy_manager& M = W.mg[m];
-        if (!manager_ok(W, M)) continue;
-        if (M.body0 >= 0) W.inc[W.off[M.body0] + atomicAdd(&W.fill[M.body0], 1u)] = m << 1;
-        if (M.body1 >= 0) W.inc[W.off[M.body1] + atomicAdd(&W.fill[M.body1], 1u)] = (m << 1) | 1u;
-    }
-    grid_sync(W, gen);
-    // P4: order each body's list by manager; chain dynamic/dynamic entries
-    for (u32 b = tid; b < W.nb; b += nthreads) {
-        const u32 lo = W.off[b], hi = W.off[b + 1];
-        for (u32 i = lo + 1; i < hi; ++i) {
-            const u32 e = W.inc[i];
-            u32 j = i;
-            while (j > lo && W.inc[j - 1] > e) {
-                W.inc[j] = W.inc[j - 1];
-                --j;
-            }
-            W.inc[j] = e;
+    if (i == 0) W.ctr[kNbWord] = W.nb;
+}
+
+// incidence degrees (dynamic sides); clears the manager's entries
+__global__ __launch_bounds__(kBlock) void k_part_degrees(Ws W) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= mcount(W)) return;
+    const fyzz_dy_manager& M = W.mg[m];
+    W.part[m] = kNone;
+    W.entry[2 * m] = 0ull;
+    W.entry[2 * m + 1] = 0ull;
+    if (!manager_ok(W, M)) return;
+    if (M.body0 >= 0) atomicAdd(&W.deg[M.body0], 1u);
+    if (M.body1 >= 0) atomicAdd(&W.deg[M.body1], 1u);
+}
+
+struct DegreeOf {
+    const u32* deg;
+    __device__ u32 operator()(u32 i) const { return deg[i]; }
+};
+
+// incidence lists, entry = manager << 1 | side
+__global__ __launch_bounds__(kBlock) void k_part_fill(Ws W) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= mcount(W)) return;
+    const fyzz_dy_manager& M = W.mg[m];
+    if (!manager_ok(W, M)) return;
+    if (M.body0 >= 0) W.inc[W.off[M.body0] + atomicAdd(&W.fill[M.body0], 1u)] = m << 1;
+    if (M.body1 >= 0) W.inc[W.off[M.body1] + atomicAdd(&W.fill[M.body1], 1u)] = (m << 1) | 1u;
+}
+
+// each body's list in manager order; predecessor links of the
+// dynamic/dynamic entries
+__global__ __launch_bounds__(kBlock) void k_part_chain(Ws W) {
+    const u32 b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= W.nb) return;
+    const u32 lo = W.off[b], hi = W.off[b + 1];
+    for (u32 i = lo + 1; i < hi; ++i) {
+        const u32 e = W.inc[i];
+        u32 j = i;
+        while (j > lo && W.inc[j - 1] > e) {
+            W.inc[j] = W.inc[j - 1];
+            --j;
         }
-        u32 prev = kNone;
-        for (u32 i = lo; i < hi; ++i) {
-            const u32 e = W.inc[i];
-            if (!dyn_dyn(W.mg[e >> 1])) continue;
-            W.pred[e] = prev;
-            W.succ[e] = kNone;
-            if (prev != kNone) W.succ[prev] = e;
-            prev = e;
-        }
+        W.inc[j] = e;
     }
-    grid_sync(W, gen);
-    // P5: ready managers (no predecessor on either body)
-    for (u32 m = tid; m < nm; m += nthreads) {
-        if (!dyn_dyn(W.mg[m]) || !manager_ok(W, W.mg[m])) continue;
-        const u32 c = (W.pred[2 * m] != kNone ? 1u : 0u) + (W.pred[2 * m + 1] != kNone ? 1u : 0u);
-        W.cnt[m] = c;
-        if (c == 0) W.front[atomicAdd(&W.ctr[kFront0], 1u)] = m;
+    u32 prev = kNone;
+    for (u32 i = lo; i < hi; ++i) {
+        const u32 e = W.inc[i];
+        if (!dyn_dyn(W.mg[e >> 1])) continue;
+        W.pred[e] = prev;
+        prev = e;
     }
-    grid_sync(W, gen);
-    // dependency rounds
-    u32 r = 0;
-    for (;; ++r) {
-        const u32 ncur = ld(&W.ctr[r % 3]);
-        if (ncur == 0) break;
-        if (r > W.nm) {
-            if (tid == 0) raise_flag(W, FYZZ_DY_LOGIC);
-            break;
-        }
-        const u32* cur = W.front + (r & 1u) * W.nm;
-        u32* nxt = W.front + ((r + 1) & 1u) * W.nm;
-        for (u32 i = tid; i < ncur; i += nthreads) {
-            const u32 m = cur[i];
-            const u32 p0 = W.pred[2 * m], p1 = W.pred[2 * m + 1];
-            const u32 before0 = p0 != kNone ? W.mask_after[p0] : 0u;
-            const u32 before1 = p1 != kNone ? W.mask_after[p1] : 0u;
-            const u32 avail = ~(before0 | before1);
-            if (avail) {
-                const u32 bit = __builtin_ctz(avail);
-                W.part[m] = bit;
-                W.mask_after[2 * m] = before0 | (1u << bit);
-                W.mask_after[2 * m + 1] = before1 | (1u << bit);
+}
+
+// The colouring (cooperative) and the overflow waves.
+__global__ __launch_bounds__(kBlock) void k_part_color(Ws W) {
+    u32 gen = barrier_start(W);
+    const u32 nthreads = gridDim.x * kBlock;
+    const u32 nm = mcount(W);
+    u32 m = blockIdx.x * kBlock + threadIdx.x;
+    bool have = false;
+    u32 p0 = kNone, p1 = kNone, spins = 0, depth = 0;
+    for (;;) {
+        bool progress = false;
+        if (m < nm && !have) {
+            const fyzz_dy_manager& M = W.mg[m];
+            if (!dyn_dyn(M) || !manager_ok(W, M)) {
+                m += nthreads;
+                progress = true;
             } else {
-                W.part[m] = kPending;
-                W.mask_after[2 * m] = before0;
-                W.mask_after[2 * m + 1] = before1;
-                const u32 k = atomicAdd(&W.ctr[kPend], 1u);
-                if (k < kMaxPending) W.pend[k] = m;
-            }
-            for (u32 s = 0; s < 2; ++s) {
-                const u32 e = W.succ[2 * m + s];
-                if (e == kNone) continue;
-                if (atomicSub(&W.cnt[e >> 1], 1u) == 1u) nxt[atomicAdd(&W.ctr[(r + 1) % 3], 1u)] = e >> 1;
+                p0 = W.pred[2 * m];
+                p1 = W.pred[2 * m + 1];
+                have = true;
             }
         }
-        if (tid == 0) W.ctr[(r + 2) % 3] = 0;
-        grid_sync(W, gen);
+        if (m < nm && have) {
+            const u64 a0 = p0 != kNone ? __hip_atomic_load(&W.entry[p0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                       : kEntryReady;
+            const u64 a1 = p1 != kNone ? __hip_atomic_load(&W.entry[p1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                       : kEntryReady;
+            if ((a0 & kEntryReady) && (a1 & kEntryReady)) {
+                const u32 before0 = static_cast<u32>(a0), before1 = static_cast<u32>(a1);
+                const u32 level = max(static_cast<u32>(a0 >> 32) & 0x7fffffffu,
+                                      static_cast<u32>(a1 >> 32) & 0x7fffffffu) + 1u;
+                depth = max(depth, level);
+                const u32 avail = ~(before0 | before1);
+                u32 after0 = before0, after1 = before1;
+                if (avail) {
+                    const u32 bit = __builtin_ctz(avail);
+                    W.part[m] = bit;
+                    after0 |= 1u << bit;
+                    after1 |= 1u << bit;
+                } else {
+                    W.part[m] = kPending;
+                    const u32 k = atomicAdd(&W.ctr[kPend], 1u);
+                    if (k < kMaxPending) W.pend[k] = m;
+                }
+                const u64 hi = kEntryReady | (static_cast<u64>(level) << 32);
+                __hip_atomic_store(&W.entry[2 * m], hi | after0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&W.entry[2 * m + 1], hi | after1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                have = false;
+                m += nthreads;
+                progress = true;
+            } else if (++spins > kSpinLimit) {
+                raise_flag(W, FYZZ_DY_LOGIC);
+                m = nm;  // give up on this thread's managers
+            }
+        }
+        const bool busy = m < nm;
+        if (!__ballot(busy)) break;
+        if (!__ballot(progress)) __builtin_amdgcn_s_sleep(1);
     }
-    if (tid == 0) W.ctr[kRounds] = r;
+    if (depth) atomicMax(&W.ctr[kRounds], depth);
+    grid_sync(W, gen);
     // overflow waves: the unassigned managers, in order, with fresh masks
-    if (tid == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         u32 p = ld(&W.ctr[kPend]);
         if (p > kMaxPending) {
             raise_flag(W, FYZZ_DY_OVERFLOW);
@@ -479,17 +484,17 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
             }
             u32 over = 0;
             for (u32 k = 0; k < p; ++k) {
-                const u32 m = W.pend[k];
-                const fyzz_dy_manager& M = W.mg[m];
+                const u32 mm = W.pend[k];
+                const fyzz_dy_manager& M = W.mg[mm];
                 const u32 avail = ~(W.body_mask[M.body0] | W.body_mask[M.body1]);
                 if (!avail) {
-                    W.pend[over++] = m;
+                    W.pend[over++] = mm;
                     continue;
                 }
                 const u32 bit = __builtin_ctz(avail);
                 W.body_mask[M.body0] |= 1u << bit;
                 W.body_mask[M.body1] |= 1u << bit;
-                W.part[m] = wave + bit;
+                W.part[mm] = wave + bit;
             }
             if (over == p) {
                 raise_flag(W, FYZZ_DY_LOGIC);
@@ -498,57 +503,67 @@ __global__ __launch_bounds__(kBlock) void k_partition(Ws W) {
             p = over;
         }
     }
-    grid_sync(W, gen);
-    // P6: highest dynamic partition per body
-    for (u32 m = tid; m < nm; m += nthreads) {
-        const fyzz_dy_manager& M = W.mg[m];
-        if (!dyn_dyn(M)) continue;
-        const u32 pm = W.part[m];
-        if (pm >= kPending) continue;
-        atomicMax(&W.max_dyn[M.body0], pm + 1);
-        atomicMax(&W.max_dyn[M.body1], pm + 1);
+}
+
+// highest dynamic partition per body
+__global__ __launch_bounds__(kBlock) void k_part_maxdyn(Ws W) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= mcount(W)) return;
+    const fyzz_dy_manager& M = W.mg[m];
+    if (!dyn_dyn(M)) return;
+    const u32 pm = W.part[m];
+    if (pm >= kPending) return;
+    atomicMax(&W.max_dyn[M.body0], pm + 1);
+    atomicMax(&W.max_dyn[M.body1], pm + 1);
+}
+
+// dynamic/static managers follow the body's dynamic partitions
+__global__ __launch_bounds__(kBlock) void k_part_static(Ws W) {
+    const u32 b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= W.nb) return;
+    u32 k = W.max_dyn[b];
+    for (u32 i = W.off[b]; i < W.off[b + 1]; ++i) {
+        const u32 m = W.inc[i] >> 1;
+        if (!dyn_dyn(W.mg[m])) W.part[m] = k++;
     }
-    grid_sync(W, gen);
-    // P7: dynamic/static managers follow the body's dynamic partitions
-    for (u32 b = tid; b < W.nb; b += nthreads) {
-        u32 k = W.max_dyn[b];
-        for (u32 i = W.off[b]; i < W.off[b + 1]; ++i) {
-            const u32 m = W.inc[i] >> 1;
-            if (!dyn_dyn(W.mg[m])) W.part[m] = k++;
-        }
+}
+
+// partition sizes and count
+__global__ __launch_bounds__(kBlock) void k_part_sizes(Ws W) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= mcount(W)) return;
+    const u32 pm = W.part[m];
+    if (pm == kNone) return;  // malformed record, already flagged
+    if (pm >= kMaxParts) {
+        raise_flag(W, FYZZ_DY_OVERFLOW);
+        return;
     }
-    grid_sync(W, gen);
-    // P8: partition sizes
-    for (u32 m = tid; m < nm; m += nthreads) {
-        const u32 pm = W.part[m];
-        if (pm == kNone) continue;  // malformed record, already flagged
-        if (pm >= kMaxParts) {
-            raise_flag(W, FYZZ_DY_OVERFLOW);
-            continue;
-        }
-        atomicAdd(&W.part_count[pm], 1u);
-        atomicMax(&W.ctr[kNParts], pm + 1);
+    atomicAdd(&W.part_count[pm], 1u);
+    atomicMax(&W.ctr[kNParts], pm + 1);
+}
+
+// partition offsets (one workgroup)
+__global__ __launch_bounds__(kBlock) void k_part_offsets(Ws W) {
+    __shared__ u32 scratch[kBlock / kWave + 1];
+    const u32 np = W.ctr[kNParts];
+    u32 carry = 0, total;
+    for (u32 t0 = 0; t0 < np; t0 += kBlock) {
+        const u32 t = t0 + threadIdx.x;
+        const u32 v = t < np ? W.part_count[t] : 0u;
+        const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
+        if (t < np) W.part_off[t] = carry + ex;
+        carry += total;
     }
-    grid_sync(W, gen);
-    if (blockIdx.x == 0) {
-        const u32 np = ld(&W.ctr[kNParts]);
-        u32 carry = 0, total;
-        for (u32 t0 = 0; t0 < np; t0 += kBlock) {
-            const u32 t = t0 + threadIdx.x;
-            const u32 v = t < np ? W.part_count[t] : 0u;
-            const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
-            if (t < np) W.part_off[t] = carry + ex;
-            carry += total;
-        }
-        if (threadIdx.x == 0) W.part_off[np] = carry;
-    }
-    grid_sync(W, gen);
-    // P10: managers grouped by partition (order inside a partition is free)
-    for (u32 m = tid; m < nm; m += nthreads) {
-        const u32 pm = W.part[m];
-        if (pm >= kMaxParts) continue;
-        W.sorted[W.part_off[pm] + atomicAdd(&W.part_fill[pm], 1u)] = m;
-    }
+    if (threadIdx.x == 0) W.part_off[np] = carry;
+}
+
+// managers grouped by partition (order inside a partition is free)
+__global__ __launch_bounds__(kBlock) void k_part_scatter(Ws W) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= mcount(W)) return;
+    const u32 pm = W.part[m];
+    if (pm >= kMaxParts) return;
+    W.sorted[W.part_off[pm] + atomicAdd(&W.part_fill[pm], 1u)] = m;
 }
 
 // ----------------------------------------------------- contact preparation
@@ -1432,7 +1447,8 @@ struct fyzz_hip_dy {
     bool plain_launch = false;
     DevBuf<float4> pv, pw, lin, ang, mlin, mang, rpos, rq, sii;
     DevBuf<u32> has_row, deg, off, fill, max_dyn, body_mask;
-    DevBuf<u32> inc, pred, succ, cnt, part, mask_after, front, pend, sorted, order;
+    DevBuf<u32> inc, pred, part, pend, sorted, order;
+    DevBuf<u64> entry;
     DevBuf<dy::CRow> crow;
     DevBuf<dy::FRow> frow;
     DevBuf<u32> part_count, part_off, part_fill, blocksums, ctr, flags, bar;
@@ -1510,8 +1526,26 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
     }
     {
         ProfScope ps(&prof, "dy.partition", st);
-        launch_coop(reinterpret_cast<const void*>(&dy::k_partition), d->coop_blocks_partition, w, st,
-                    d->plain_launch || !d->ctx->cooperative);
+        const dim3 blk(dy::kBlock);
+        const dim3 gb(grid_for(std::max<u32>(nb, dy::kMaxParts), dy::kBlock)), gm(grid_for(nm, dy::kBlock));
+        hipLaunchKernelGGL(dy::k_part_clear, gb, blk, 0, st, w);
+        if (nm) {
+            hipLaunchKernelGGL(dy::k_part_degrees, gm, blk, 0, st, w);
+            if (nb)
+                sel::exclusive_scan(st, nb, d->ctr.get() + dy::kNbWord, dy::DegreeOf{d->deg.get()}, d->off.get(),
+                                    d->blocksums.get(), d->off.get() + nb);
+            hipLaunchKernelGGL(dy::k_part_fill, gm, blk, 0, st, w);
+            if (nb) hipLaunchKernelGGL(dy::k_part_chain, dim3(grid_for(nb, dy::kBlock)), blk, 0, st, w);
+            FYZZ_HIP_CHECK(hipGetLastError());
+            launch_coop(reinterpret_cast<const void*>(&dy::k_part_color), d->coop_blocks_partition, w, st,
+                        d->plain_launch || !d->ctx->cooperative);
+            hipLaunchKernelGGL(dy::k_part_maxdyn, gm, blk, 0, st, w);
+            if (nb) hipLaunchKernelGGL(dy::k_part_static, dim3(grid_for(nb, dy::kBlock)), blk, 0, st, w);
+            hipLaunchKernelGGL(dy::k_part_sizes, gm, blk, 0, st, w);
+        }
+        hipLaunchKernelGGL(dy::k_part_offsets, dim3(1), blk, 0, st, w);
+        if (nm) hipLaunchKernelGGL(dy::k_part_scatter, gm, blk, 0, st, w);
+        FYZZ_HIP_CHECK(hipGetLastError());
     }
     if (!partition_only && nm) {
         ProfScope ps(&prof, "dy.prep", st);
@@ -1608,7 +1642,7 @@ namespace fyzz_hip::dy {
 const u32* counters(fyzz_hip_dy* d) { return d->ctr.get(); }
 void set_plain_launch(fyzz_hip_dy* d, bool plain) { d->plain_launch = plain; }
 void set_coop_blocks(fyzz_hip_dy* d, int partition_blocks, int pgs_blocks) {
-    const int cap_p = coop_blocks(reinterpret_cast<const void*>(&k_partition), d->ctx->device, 1 << 30);
+    const int cap_p = coop_blocks(reinterpret_cast<const void*>(&k_part_color), d->ctx->device, 1 << 30);
     const int cap_s = coop_blocks(reinterpret_cast<const void*>(&k_pgs), d->ctx->device, 1 << 30);
     FYZZ_REQUIRE(partition_blocks > 0 && pgs_blocks > 0, kInvalidArgument, "dy: workgroup counts must be positive");
     d->coop_blocks_partition = std::min(partition_blocks, cap_p);
@@ -1647,8 +1681,9 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->sii.reserve(3 * nb);
         for (auto* b : {&d->has_row, &d->deg, &d->fill, &d->max_dyn, &d->body_mask}) b->reserve(nb);
         d->off.reserve(nb + 1);
-        for (auto* b : {&d->inc, &d->pred, &d->succ, &d->mask_after, &d->front}) b->reserve(2 * nm);
-        for (auto* b : {&d->cnt, &d->part, &d->pend, &d->sorted, &d->order}) b->reserve(nm);
+        for (auto* b : {&d->inc, &d->pred}) b->reserve(2 * nm);
+        d->entry.reserve(2 * nm);
+        for (auto* b : {&d->part, &d->pend, &d->sorted, &d->order}) b->reserve(nm);
         d->crow.reserve(4 * nm);
         d->frow.reserve(4 * nm);
         d->part_count.reserve(dy::kMaxParts + 1);
@@ -1658,10 +1693,11 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->flags.reserve(1);
         d->bar.reserve(dy::kBarWords);
         FYZZ_HIP_CHECK(hipMemset(d->bar.get(), 0, dy::kBarWords * sizeof(u32)));
-        d->coop_blocks_partition = coop_blocks(reinterpret_cast<const void*>(&dy::k_partition), ctx->device);
+        d->coop_blocks_partition = coop_blocks(reinterpret_cast<const void*>(&dy::k_part_color), ctx->device);
         d->coop_blocks_pgs = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs), ctx->device);
-        // room for the largest grid the autotuner may pick
-        d->blocksums.reserve(2 * 512 + 2);
+        // room for the largest grid the autotuner may pick, and for the
+        // tile sums of the degree scan
+        d->blocksums.reserve(std::max<std::size_t>(2 * 512 + 2, sel::tiles_for(static_cast<u32>(nb)) + 1));
         FYZZ_HIP_CHECK(hipMemset(d->flags.get(), 0, sizeof(u32)));
         FYZZ_HIP_CHECK(hipMemset(d->ctr.get(), 0, dy::kCtrWords * sizeof(u32)));
         dy::Ws& w = d->ws;
@@ -1682,11 +1718,8 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         w.body_mask = d->body_mask.get();
         w.inc = d->inc.get();
         w.pred = d->pred.get();
-        w.succ = d->succ.get();
-        w.cnt = d->cnt.get();
         w.part = d->part.get();
-        w.mask_after = d->mask_after.get();
-        w.front = d->front.get();
+        w.entry = d->entry.get();
         w.pend = d->pend.get();
         w.sorted = d->sorted.get();
         w.order = d->order.get();

@@ -386,18 +386,6 @@ struct Copy {
     __device__ void operator()(u32 i, u32 o) const { dst[o] = src[i]; }
 };
 
-// rank[axis][payload] = sorted position (the SapBox1D min/max indices of
-// the reference GPU path, here for every endpoint of every axis).
-__global__ __launch_bounds__(256) void k_rank(u32 n2, const u32* pay, u32* rank) {
-    const std::size_t total = 3ull * n2;
-    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < total;
-         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-        const std::size_t axis = i / n2;
-        const u32 pos = static_cast<u32>(i - axis * n2);
-        rank[axis * n2 + pay[i]] = pos;
-    }
-}
-
 // One-workgroup path for this frame's candidate changes when they fit in
 // LDS: bitonic sort, unique, split into lost (no kFoundBit) then found.
 // Larger lists raise kLarge and leave the work to the multi-block path.
@@ -575,16 +563,17 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
     if (n) {
         ProfScope ps(prof, "sap.sort", st);
         radix::Batch b{n2, 3, n2, n2, nullptr, s.planes.p, n};
+        // the last pass also writes rank[axis][pe] = sorted position (the
+        // SapBox1D min/max indices of the reference GPU path, here for every
+        // endpoint of every axis)
         const bool in_a = radix::sort<u32>(st, b, radix::Source::SapPlanes, nullptr, nullptr,
                                            s.sorted_key[nxt].p, s.sorted_val[nxt].p, s.tmp_key.p,
                                            s.tmp_val.p, 32, s.radix_counts.p, s.radix_totals.p,
-                                           prof, "sap.sort.scatter");
+                                           prof, "sap.sort.scatter", s.rank[nxt].p);
         if (!in_a) {
             FYZZ_HIP_CHECK(hipMemcpyAsync(s.sorted_key[nxt].p, s.tmp_key.p, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, st));
             FYZZ_HIP_CHECK(hipMemcpyAsync(s.sorted_val[nxt].p, s.tmp_val.p, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, st));
         }
-        hipLaunchKernelGGL(k_rank, dim3(std::min<u32>(grid_for(3ull * n2, 256), 4096u)), dim3(256), 0, st, n2,
-                           s.sorted_val[nxt].p, s.rank[nxt].p);
     }
     if (full) {
         long pi = prof->begin("sap.pairs", st);

@@ -486,7 +486,7 @@ __global__ void k_touch(u32 cap, const u32* d_nbb, const u32* counts_bb, u32* to
 // running totals over steps (algorithmic byte accounting of the solve):
 // [0] steps, [1] broadphase pairs, [2] managers, [3] contact rows,
 // [4] friction rows, [5] partitions, [6] position+velocity iterations,
-// [7] partition dependency rounds
+// [7] partition dependency depth
 __global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const u32* dy_ctr, u64* totals) {
     totals[0] += 1;
     totals[1] += min(*d_pairs, cap);

@@ -311,8 +311,10 @@ int fyzz_hip_dy_adjoint(fyzz_hip_dy* dy, const float* adj_v_out, const float* ad
                         float* adj_v_in, float* adj_w_in, float* adj_friction);
 
 /* Synchronizes the context's stream and reports the last Dy step:
- * flags (FYZZ_DY_* bits), the partition count and the number of
- * dependency rounds the partitioner ran. Returns the status matching the
+ * flags (FYZZ_DY_* bits), the partition count and the dependency depth
+ * of the partitioner (the longest chain of managers where each shares a
+ * dynamic body with the previous, earlier in list order; a sequential
+ * round-by-round colouring needs this many rounds). Returns the status matching the
  * first raised flag (the exception the reference would throw). */
 int fyzz_hip_dy_status(fyzz_hip_dy* dy, uint32_t* flags, uint32_t* n_partitions, uint32_t* rounds);
 
@@ -393,7 +395,7 @@ int fyzz_hip_world_fetch_managers(fyzz_hip_world* world, fyzz_dy_manager* manage
 /* Synchronizes; running totals since creation or the last reset: [0]
  * steps, [1] broadphase pairs, [2] touching managers, [3] contact rows,
  * [4] friction rows, [5] partitions, [6] solver iterations, [7] partition
- * dependency rounds (out[8], HOST).
+ * dependency depth (out[8], HOST).
  * reset != 0 zeroes them afterwards. */
 int fyzz_hip_world_totals(fyzz_hip_world* world, uint64_t* out, int reset);
 /* Scene::set_contact_notifications (scene.h:654): when enabled, every
