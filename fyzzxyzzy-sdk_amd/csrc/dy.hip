@@ -263,7 +263,22 @@ __global__ __launch_bounds__(kBlock) void k_mark(Ws W) {
 
 // world sqrt inverse inertia R diag(sqrt I^-1) R^T (scene.cpp:7092-7110)
 __global__ __launch_bounds__(kBlock) void k_rows(Ws W) {
+    // iteration counts (maximum over bodies with rows): one atomic per
+    // workgroup and counter
+    __shared__ u32 pos_it, vel_it;
+    if (threadIdx.x == 0) pos_it = vel_it = 0;
+    __syncthreads();
     const u32 i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < W.nb && W.has_row[i]) {
+        const u32 it = W.bodies[i].iterations;
+        atomicMax(&pos_it, it & 0xffu);
+        atomicMax(&vel_it, (it >> 8) & 0xffu);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (pos_it) atomicMax(&W.ctr[kPosIters], pos_it);
+        if (vel_it) atomicMax(&W.ctr[kVelIters], vel_it);
+    }
     if (i >= W.nb) return;
     float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0, c2 = c0;
     W.ang[i] = c0;
@@ -306,8 +321,6 @@ __global__ __launch_bounds__(kBlock) void k_rows(Ws W) {
     W.sii[3 * i] = c0;
     W.sii[3 * i + 1] = c1;
     W.sii[3 * i + 2] = c2;
-    atomicMax(&W.ctr[kPosIters], b.iterations & 0xffu);
-    atomicMax(&W.ctr[kVelIters], (b.iterations >> 8) & 0xffu);
 }
 
 // ------------------------------------------------------------ partitions
@@ -528,18 +541,31 @@ __global__ __launch_bounds__(kBlock) void k_part_static(Ws W) {
     }
 }
 
-// partition sizes and count
+// Partition sizes and count. A pile has a few tens of partitions, so one
+// global counter per manager would serialize ~10^4 atomics per address:
+// each workgroup counts in LDS (its managers' partition range) and adds
+// one value per partition it holds.
 __global__ __launch_bounds__(kBlock) void k_part_sizes(Ws W) {
+    __shared__ u32 cnt[kMaxParts];
+    __shared__ u32 top;
     const u32 m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= mcount(W)) return;
-    const u32 pm = W.part[m];
-    if (pm == kNone) return;  // malformed record, already flagged
-    if (pm >= kMaxParts) {
-        raise_flag(W, FYZZ_DY_OVERFLOW);
-        return;
+    for (u32 i = threadIdx.x; i < kMaxParts; i += kBlock) cnt[i] = 0;
+    if (threadIdx.x == 0) top = 0;
+    __syncthreads();
+    const u32 pm = m < mcount(W) ? W.part[m] : kNone;
+    if (pm != kNone) {  // kNone: malformed record, already flagged
+        if (pm >= kMaxParts) {
+            raise_flag(W, FYZZ_DY_OVERFLOW);
+        } else {
+            atomicAdd(&cnt[pm], 1u);
+            atomicMax(&top, pm + 1);
+        }
     }
-    atomicAdd(&W.part_count[pm], 1u);
-    atomicMax(&W.ctr[kNParts], pm + 1);
+    __syncthreads();
+    const u32 np = top;
+    for (u32 i = threadIdx.x; i < np; i += kBlock)
+        if (cnt[i]) atomicAdd(&W.part_count[i], cnt[i]);
+    if (threadIdx.x == 0 && np) atomicMax(&W.ctr[kNParts], np);
 }
 
 // partition offsets (one workgroup)
@@ -557,13 +583,30 @@ __global__ __launch_bounds__(kBlock) void k_part_offsets(Ws W) {
     if (threadIdx.x == 0) W.part_off[np] = carry;
 }
 
-// managers grouped by partition (order inside a partition is free)
+// Managers grouped by partition (order inside a partition is free): each
+// workgroup ranks its managers per partition in LDS, reserves one range
+// per partition it holds, then writes.
 __global__ __launch_bounds__(kBlock) void k_part_scatter(Ws W) {
+    __shared__ u32 cnt[kMaxParts];
+    __shared__ u32 base[kMaxParts];
+    __shared__ u32 top;
     const u32 m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= mcount(W)) return;
-    const u32 pm = W.part[m];
-    if (pm >= kMaxParts) return;
-    W.sorted[W.part_off[pm] + atomicAdd(&W.part_fill[pm], 1u)] = m;
+    for (u32 i = threadIdx.x; i < kMaxParts; i += kBlock) cnt[i] = 0;
+    if (threadIdx.x == 0) top = 0;
+    __syncthreads();
+    const u32 pm = m < mcount(W) ? W.part[m] : kNone;
+    const bool live = pm < kMaxParts;
+    u32 local = 0;
+    if (live) {
+        local = atomicAdd(&cnt[pm], 1u);
+        atomicMax(&top, pm + 1);
+    }
+    __syncthreads();
+    const u32 np = top;
+    for (u32 i = threadIdx.x; i < np; i += kBlock)
+        if (cnt[i]) base[i] = W.part_off[i] + atomicAdd(&W.part_fill[i], cnt[i]);
+    __syncthreads();
+    if (live) W.sorted[base[pm] + local] = m;
 }
 
 // ----------------------------------------------------- contact preparation
