@@ -49,6 +49,7 @@ constexpr u32 kMaxParts = 4096;       // partitions per step
 constexpr u32 kMaxPending = 65536;    // managers left for overflow waves
 constexpr int kBlock = 256;
 constexpr int kCoopBlocks = 64;       // workgroups of the cooperative launches
+constexpr int kPgsBlocks = 256;       // workgroups of the sweep (no grid barrier there)
 
 // device counters
 enum : u32 { kFront0 = 0, kFront1, kFront2, kPend, kNParts, kPosIters, kVelIters, kRounds, kContacts, kFrictionRows,
@@ -93,6 +94,10 @@ struct Ws {
     // per manager
     u32 *inc, *pred, *part, *pend, *sorted, *order;
     u64* entry;  // [2 nm] partition colouring: ready | level << 32 | mask after the manager, per side
+    int4* sinfo; // [nm] per partition-sorted position: body0, body1, manager
+    uint4* srank;   // [nm] per position: rank of the manager among body0's / body1's (by partition), their degrees
+    u32* erank;     // [2 nm] per incidence entry: rank among the body's managers by partition
+    u32* ver;       // [nb] solver updates applied to the body in this sweep
     CRow* crow;
     FRow* frow;
     u32 *part_count, *part_off, *part_fill;
@@ -172,6 +177,22 @@ __device__ inline void grid_sync(const Ws& W, u32& gen) {
         gen = g0 + 1u;
     }
     __syncthreads();
+}
+// Waits until this wave's outstanding vector memory operations have
+// completed (s_waitcnt vmcnt(0); gfx9 encoding with expcnt and lgkmcnt
+// left at their maxima), so the body stores above are performed at agent
+// scope before the version that publishes them.
+__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0f70); }
+// A wave whose lanes all wait backs off (64, 256, then 1024 cycles per
+// poll), so polling leaves the memory system to the waves doing work.
+__device__ inline void wait_backoff(u32 idle) {
+    if (idle == 0) return;
+    if (idle < 8)
+        __builtin_amdgcn_s_sleep(1);
+    else if (idle < 64)
+        __builtin_amdgcn_s_sleep(4);
+    else
+        __builtin_amdgcn_s_sleep(16);
 }
 __device__ inline bool finite3(V v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
 __device__ inline bool finite4(Q q) { return isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w); }
@@ -353,6 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_part_clear(Ws W) {
         W.deg[i] = 0;
         W.fill[i] = 0;
         W.max_dyn[i] = 0;
+        W.ver[i] = 0;
     }
     if (i < kMaxParts) {
         W.part_count[i] = 0;
@@ -420,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_part_color(Ws W) {
     const u32 nm = mcount(W);
     u32 m = blockIdx.x * kBlock + threadIdx.x;
     bool have = false;
-    u32 p0 = kNone, p1 = kNone, spins = 0, depth = 0;
+    u32 p0 = kNone, p1 = kNone, spins = 0, depth = 0, idle = 0;
     for (;;) {
         bool progress = false;
         if (m < nm && !have) {
@@ -469,7 +491,8 @@ __global__ __launch_bounds__(kBlock) void k_part_color(Ws W) {
         }
         const bool busy = m < nm;
         if (!__ballot(busy)) break;
-        if (!__ballot(progress)) __builtin_amdgcn_s_sleep(1);
+        idle = __ballot(progress) ? 0u : idle + 1u;
+        wait_backoff(idle);
     }
     if (depth) atomicMax(&W.ctr[kRounds], depth);
     grid_sync(W, gen);
@@ -541,6 +564,21 @@ __global__ __launch_bounds__(kBlock) void k_part_static(Ws W) {
     }
 }
 
+// Each body's managers ranked by partition (the order in which a sweep
+// reaches them; the partitions of a body's managers are distinct).
+__global__ __launch_bounds__(kBlock) void k_part_rank(Ws W) {
+    const u32 b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= W.nb) return;
+    const u32 lo = W.off[b], hi = W.off[b + 1];
+    for (u32 i = lo; i < hi; ++i) {
+        const u32 e = W.inc[i];
+        const u32 pe = W.part[e >> 1];
+        u32 r = 0;
+        for (u32 j = lo; j < hi; ++j) r += W.part[W.inc[j] >> 1] < pe ? 1u : 0u;
+        W.erank[e] = r;
+    }
+}
+
 // Partition sizes and count. A pile has a few tens of partitions, so one
 // global counter per manager would serialize ~10^4 atomics per address:
 // each workgroup counts in LDS (its managers' partition range) and adds
@@ -606,7 +644,14 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(Ws W) {
     for (u32 i = threadIdx.x; i < np; i += kBlock)
         if (cnt[i]) base[i] = W.part_off[i] + atomicAdd(&W.part_fill[i], cnt[i]);
     __syncthreads();
-    if (live) W.sorted[base[pm] + local] = m;
+    if (live) {
+        const fyzz_dy_manager& M = W.mg[m];
+        const u32 pos = base[pm] + local;
+        W.sorted[pos] = m;
+        W.sinfo[pos] = make_int4(M.body0, M.body1, static_cast<int>(m), 0);
+        W.srank[pos] = make_uint4(M.body0 >= 0 ? W.erank[2 * m] : 0u, M.body1 >= 0 ? W.erank[2 * m + 1] : 0u,
+                                  M.body0 >= 0 ? W.deg[M.body0] : 0u, M.body1 >= 0 ? W.deg[M.body1] : 0u);
+    }
 }
 
 // ----------------------------------------------------- contact preparation
@@ -1006,19 +1051,50 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
 // ------------------------------------------------------------------ solve
 
 // scene.cpp:8985-9010 (normal rows), 9050-9120 (friction rows)
-__device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 total_iters) {
-    const fyzz_dy_manager& M = W.mg[m];
-    const int i0 = M.body0, i1 = M.body1;
+// Coherent body state: the sweep's body rows move between workgroups on
+// different XCDs through agent-scope relaxed atomics (64-bit halves), so
+// no L2 write-back or invalidation is needed around them.
+__device__ inline float4 load_body(const float4* p) {
+    const u64* q = reinterpret_cast<const u64*>(p);
+    const u64 a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const u64 b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(__uint_as_float(static_cast<u32>(a)), __uint_as_float(static_cast<u32>(a >> 32)),
+                       __uint_as_float(static_cast<u32>(b)), __uint_as_float(static_cast<u32>(b >> 32)));
+}
+__device__ inline void store_body(float4* p, float4 v) {
+    u64* q = reinterpret_cast<u64*>(p);
+    const u64 a = static_cast<u64>(__float_as_uint(v.x)) | (static_cast<u64>(__float_as_uint(v.y)) << 32);
+    const u64 b = static_cast<u64>(__float_as_uint(v.z)) | (static_cast<u64>(__float_as_uint(v.w)) << 32);
+    __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// One manager's rows of one iteration (scene.cpp:8853-9148). info: (body0,
+// body1, manager) of a partition-sorted position. Rows and patches are read
+// with plain loads: the same thread solves the same manager in every
+// iteration, and nothing else writes them during the sweep.
+__device__ void solve_manager(const Ws& W, int4 info, bool snap0, bool snap1, u32 iter, u32 pos_iters,
+                              u32 total_iters) {
+    const int i0 = info.x, i1 = info.y;
+    const u32 m = static_cast<u32>(info.z);
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 la = i0 >= 0 ? W.lin[i0] : z4, lb = i1 >= 0 ? W.lin[i1] : z4;
-    const float4 aa4 = i0 >= 0 ? W.ang[i0] : z4, ab4 = i1 >= 0 ? W.ang[i1] : z4;
+    const float4 la = i0 >= 0 ? load_body(&W.lin[i0]) : z4, lb = i1 >= 0 ? load_body(&W.lin[i1]) : z4;
+    const float4 aa4 = i0 >= 0 ? load_body(&W.ang[i0]) : z4, ab4 = i1 >= 0 ? load_body(&W.ang[i1]) : z4;
+    CRow rows[4];
+    FRow frows[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        rows[q] = W.crow[4 * m + q];
+        frows[q] = W.frow[4 * m + q];
+    }
     V va = f3(la), vb = f3(lb), wa = f3(aa4), wb = f3(ab4);
     const float ima = la.w, imb = lb.w;
     const u32 no = W.order[m] & 0xffu;
     bool bad = false;
     float normal_impulse = 0.0f;
-    for (u32 q = 0; q < no; ++q) {
-        CRow& r = W.crow[4 * m + q];
+#pragma unroll
+    for (u32 q = 0; q < 4; ++q) {
+        if (q >= no) break;
+        CRow& r = rows[q];
         const V n{r.nx, r.ny, r.nz};
         const V ra{r.rax, r.ray, r.raz}, rb{r.rbx, r.rby, r.rbz};
         const float nv = vdot(n, va) - vdot(n, vb) + vdot(wa, ra) - vdot(wb, rb);
@@ -1038,6 +1114,7 @@ __device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 t
         wb = {wb.x - rb.x * delta, wb.y - rb.y * delta, wb.z - rb.z * delta};
         bad |= !finite3(va) || !finite3(vb) || !finite3(wa) || !finite3(wb);
         r.applied = next;
+        W.crow[4 * m + q].applied = next;
         normal_impulse += next;
     }
     fyzz_dy_patch& P = W.patches[m];
@@ -1048,8 +1125,10 @@ __device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 t
         const float max_static = P.static_friction * coef * normal_impulse;
         const float max_dynamic = P.dynamic_friction * coef * normal_impulse;
         bool broken = false;
-        for (u32 k = 0; k < anchors * 2; ++k) {
-            FRow& fr = W.frow[4 * m + k];
+#pragma unroll
+        for (u32 k = 0; k < 4; ++k) {
+            if (k >= anchors * 2) break;
+            FRow& fr = frows[k];
             const V t{fr.tx, fr.ty, fr.tz};
             const V ra{fr.rax, fr.ray, fr.raz}, rb{fr.rbx, fr.rby, fr.rbz};
             const float td = vdot(t, va) - vdot(t, vb) + vdot(wa, ra) - vdot(wb, rb);
@@ -1069,45 +1148,92 @@ __device__ void solve_manager(const Ws& W, u32 m, u32 iter, u32 pos_iters, u32 t
             wb = {wb.x - rb.x * delta, wb.y - rb.y * delta, wb.z - rb.z * delta};
             bad |= !finite3(va) || !finite3(vb) || !finite3(wa) || !finite3(wb);
             fr.applied = next;
+            W.frow[4 * m + k].applied = next;
             broken |= clamped;
         }
         if (iter + 1 == total_iters) P.flags = (P.flags & ~1u) | (broken ? 1u : 0u);
     }
     if (bad) raise_flag(W, FYZZ_DY_NONFINITE);
     if (i0 >= 0) {
-        W.lin[i0] = make_float4(va.x, va.y, va.z, ima);
-        W.ang[i0] = make_float4(wa.x, wa.y, wa.z, 0.f);
+        const float4 l = make_float4(va.x, va.y, va.z, ima), a = make_float4(wa.x, wa.y, wa.z, 0.f);
+        store_body(&W.lin[i0], l);
+        store_body(&W.ang[i0], a);
+        if (snap0) {
+            W.mlin[i0] = l;
+            W.mang[i0] = a;
+        }
     }
     if (i1 >= 0) {
-        W.lin[i1] = make_float4(vb.x, vb.y, vb.z, imb);
-        W.ang[i1] = make_float4(wb.x, wb.y, wb.z, 0.f);
+        const float4 l = make_float4(vb.x, vb.y, vb.z, imb), a = make_float4(wb.x, wb.y, wb.z, 0.f);
+        store_body(&W.lin[i1], l);
+        store_body(&W.ang[i1], a);
+        if (snap1) {
+            W.mlin[i1] = l;
+            W.mang[i1] = a;
+        }
     }
 }
 
-// Position then velocity iterations over the partitions in order; motion is
-// snapshotted after the last position iteration (scene.cpp:9136-9146).
+// Position then velocity iterations over the partitions in order, motion
+// snapshotted after the last position iteration (scene.cpp:9136-9146),
+// without grid barriers. The sweep is the sequence (iteration, sorted
+// position); within a partition no two managers share a dynamic body, so
+// the Gauss-Seidel result depends only on each body seeing its managers'
+// updates in that order. A body's version counts the updates applied to
+// it; the manager at rank r among the body's d managers (by partition)
+// runs in iteration t once the version reads t*d + r, and bumps it after
+// its body stores have completed. Thread k solves positions k, k + T, ...
+// in every iteration (T threads), in sequence order, each as soon as it is
+// ready (a lane waits without holding back the rest of its wave). The
+// earliest unsolved (iteration, position) always has its predecessors done
+// and its thread resident (cooperative launch), so the sweep always
+// advances; waits are bounded regardless (a status flag, not a hang).
 __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
-    u32 gen = barrier_start(W);
     const u32 tid = blockIdx.x * kBlock + threadIdx.x;
     const u32 nthreads = gridDim.x * kBlock;
-    const u32 np = ld(&W.ctr[kNParts]);
+    const u32 np = min(ld(&W.ctr[kNParts]), kMaxParts);
     const u32 pos_iters = ld(&W.ctr[kPosIters]);
     const u32 vel_iters = max(ld(&W.ctr[kVelIters]), 1u);
     const u32 total = pos_iters + vel_iters;
-    for (u32 iter = 0; iter < total; ++iter) {
-        for (u32 p = 0; p < np; ++p) {
-            const u32 lo = W.part_off[p], hi = W.part_off[p + 1];
-            for (u32 i = lo + tid; i < hi; i += nthreads) solve_manager(W, W.sorted[i], iter, pos_iters, total);
-            grid_sync(W, gen);
+    const u32 n = np ? W.part_off[np] : 0u;
+    u32 iter = 0, i = tid, spins = 0, idle = 0;
+    bool have = false;
+    int4 info = make_int4(-1, -1, 0, 0);
+    uint4 rk = make_uint4(0u, 0u, 0u, 0u);
+    for (;;) {
+        bool progress = false;
+        while (iter < total && i >= n) {
+            ++iter;
+            i = tid;
         }
-        if (iter + 1 == pos_iters) {
-            for (u32 i = tid; i < W.nb; i += nthreads)
-                if (W.has_row[i]) {
-                    W.mlin[i] = W.lin[i];
-                    W.mang[i] = W.ang[i];
-                }
-            grid_sync(W, gen);
+        if (iter < total) {
+            if (!have) {
+                info = W.sinfo[i];
+                rk = W.srank[i];
+                have = true;
+            }
+            const u32 need0 = iter * rk.z + rk.x, need1 = iter * rk.w + rk.y;
+            const bool ok0 = info.x < 0 || __hip_atomic_load(&W.ver[info.x], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT) == need0;
+            const bool ok1 = info.y < 0 || __hip_atomic_load(&W.ver[info.y], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT) == need1;
+            if (ok0 && ok1) {
+                const bool snap = iter + 1 == pos_iters;
+                solve_manager(W, info, snap && rk.x + 1 == rk.z, snap && rk.y + 1 == rk.w, iter, pos_iters, total);
+                wait_stores();
+                if (info.x >= 0) __hip_atomic_store(&W.ver[info.x], need0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (info.y >= 0) __hip_atomic_store(&W.ver[info.y], need1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                have = false;
+                i += nthreads;
+                progress = true;
+            } else if (++spins > kSpinLimit) {
+                raise_flag(W, FYZZ_DY_LOGIC);
+                iter = total;  // give up on this thread's work
+            }
         }
+        if (!__ballot(iter < total)) break;
+        idle = __ballot(progress) ? 0u : idle + 1u;
+        wait_backoff(idle);
     }
 }
 
@@ -1486,12 +1612,15 @@ using namespace fyzz_hip;
 struct fyzz_hip_dy {
     fyzz_hip_context* ctx = nullptr;
     u32 body_cap = 0, mgr_cap = 0;
-    int coop_blocks_partition = 0, coop_blocks_pgs = 0;
+    int coop_blocks_partition = 0, coop_blocks_pgs = 0, coop_blocks_adjoint = 0;
     bool plain_launch = false;
     DevBuf<float4> pv, pw, lin, ang, mlin, mang, rpos, rq, sii;
     DevBuf<u32> has_row, deg, off, fill, max_dyn, body_mask;
     DevBuf<u32> inc, pred, part, pend, sorted, order;
     DevBuf<u64> entry;
+    DevBuf<int4> sinfo;
+    DevBuf<uint4> srank;
+    DevBuf<u32> erank, ver;
     DevBuf<dy::CRow> crow;
     DevBuf<dy::FRow> frow;
     DevBuf<u32> part_count, part_off, part_fill, blocksums, ctr, flags, bar;
@@ -1584,6 +1713,7 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
                         d->plain_launch || !d->ctx->cooperative);
             hipLaunchKernelGGL(dy::k_part_maxdyn, gm, blk, 0, st, w);
             if (nb) hipLaunchKernelGGL(dy::k_part_static, dim3(grid_for(nb, dy::kBlock)), blk, 0, st, w);
+            if (nb) hipLaunchKernelGGL(dy::k_part_rank, dim3(grid_for(nb, dy::kBlock)), blk, 0, st, w);
             hipLaunchKernelGGL(dy::k_part_sizes, gm, blk, 0, st, w);
         }
         hipLaunchKernelGGL(dy::k_part_offsets, dim3(1), blk, 0, st, w);
@@ -1671,7 +1801,7 @@ void dy_adjoint(fyzz_hip_dy* d, const float* adj_v_out, const float* adj_w_out, 
     const u32 span = std::max<u32>(w.nb, 4 * w.nm);
     if (span) hipLaunchKernelGGL(dy::k_adj_init, dim3(grid_for(span, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
     if (w.nm) {
-        launch_coop(reinterpret_cast<const void*>(&dy::k_pgs_adjoint), d->coop_blocks_pgs, w, st,
+        launch_coop(reinterpret_cast<const void*>(&dy::k_pgs_adjoint), d->coop_blocks_adjoint, w, st,
                     d->plain_launch || !d->ctx->cooperative);
         hipLaunchKernelGGL(dy::k_adj_rows, dim3(grid_for(w.nm, dy::kBlock)), dim3(dy::kBlock), 0, st, w);
     }
@@ -1726,6 +1856,10 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->off.reserve(nb + 1);
         for (auto* b : {&d->inc, &d->pred}) b->reserve(2 * nm);
         d->entry.reserve(2 * nm);
+        d->sinfo.reserve(nm);
+        d->srank.reserve(nm);
+        d->erank.reserve(2 * nm);
+        d->ver.reserve(nb);
         for (auto* b : {&d->part, &d->pend, &d->sorted, &d->order}) b->reserve(nm);
         d->crow.reserve(4 * nm);
         d->frow.reserve(4 * nm);
@@ -1737,7 +1871,9 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->bar.reserve(dy::kBarWords);
         FYZZ_HIP_CHECK(hipMemset(d->bar.get(), 0, dy::kBarWords * sizeof(u32)));
         d->coop_blocks_partition = coop_blocks(reinterpret_cast<const void*>(&dy::k_part_color), ctx->device);
-        d->coop_blocks_pgs = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs), ctx->device);
+        d->coop_blocks_pgs = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs), ctx->device, dy::kPgsBlocks);
+        // the reverse sweep keeps grid barriers between partitions
+        d->coop_blocks_adjoint = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs_adjoint), ctx->device);
         // room for the largest grid the autotuner may pick, and for the
         // tile sums of the degree scan
         d->blocksums.reserve(std::max<std::size_t>(2 * 512 + 2, sel::tiles_for(static_cast<u32>(nb)) + 1));
@@ -1763,6 +1899,10 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         w.pred = d->pred.get();
         w.part = d->part.get();
         w.entry = d->entry.get();
+        w.sinfo = d->sinfo.get();
+        w.srank = d->srank.get();
+        w.erank = d->erank.get();
+        w.ver = d->ver.get();
         w.pend = d->pend.get();
         w.sorted = d->sorted.get();
         w.order = d->order.get();
@@ -1775,6 +1915,12 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         w.ctr = d->ctr.get();
         w.flags = d->flags.get();
         w.bar = d->bar.get();
+        // every workspace array the step kernels touch must exist
+        const void* need[] = {w.pv, w.pw, w.lin, w.ang, w.mlin, w.mang, w.rpos, w.rq, w.sii, w.has_row,
+                              w.deg, w.off, w.fill, w.max_dyn, w.body_mask, w.inc, w.pred, w.part, w.pend,
+                              w.sorted, w.order, w.entry, w.sinfo, w.srank, w.erank, w.ver, w.crow, w.frow,
+                              w.part_count, w.part_off, w.part_fill, w.blocksums, w.ctr, w.flags, w.bar};
+        for (const void* q : need) FYZZ_REQUIRE(q != nullptr, kDeviceError, "dy_create: workspace array missing");
         *out = d.release();
     });
 }
