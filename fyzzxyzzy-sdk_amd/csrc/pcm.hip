@@ -518,11 +518,10 @@ struct Facet {
     float d;
     bool valid, in_heap;
 };
-__device__ Facet make_facet(const Sup* v, int i, int j, int k) {
+__device__ Facet make_facet_q(V a, V b, V c, int i, int j, int k) {
     Facet f;
     f.valid = false;
     f.in_heap = true;
-    const V a = v[i].q, b = v[j].q, c = v[k].q;
     const V e0 = vsub(b, a), e1 = vsub(c, a);
     V n = vcross(e0, e1);
     const float n2 = vdot(n, n);
@@ -535,6 +534,9 @@ __device__ Facet make_facet(const Sup* v, int i, int j, int k) {
     f.d = vdot(n, a);
     f.valid = true;
     return f;
+}
+__device__ inline Facet make_facet(const Sup* v, int i, int j, int k) {
+    return make_facet_q(v[i].q, v[j].q, v[k].q, i, j, k);
 }
 __device__ bool init_epa(const GPair& pr, Sup* v, int& count, Facet* fa, int& fc) {
     if (count <= 0 || count > 4) return false;
@@ -581,10 +583,8 @@ __device__ bool init_epa(const GPair& pr, Sup* v, int& count, Facet* fa, int& fc
     }
     return true;
 }
-__device__ bool contact_from_facet(const GPair& pr, const Sup* v, const Facet& f, Pt& out) {
-    const Sup& p = v[f.i];
-    const Sup& q = v[f.j];
-    const Sup& r = v[f.k];
+__device__ bool contact_from_facet(const GPair& pr, const Sup& p, const Sup& q, const Sup& r, const Facet& f,
+                                   Pt& out) {
     const V e0 = vsub(q.q, p.q), e1 = vsub(r.q, p.q);
     const float a = vdot(e0, e0), b = vdot(e0, e1), c = vdot(e1, e1);
     const float den = a * c - b * b;
@@ -601,93 +601,146 @@ __device__ bool contact_from_facet(const GPair& pr, const Sup* v, const Facet& f
     const V diff = vsub(pa, pb);
     return sqrtf(vdot(diff, diff)) <= depth + 1.0e-3f;
 }
-struct EpaWork {
-    Sup v[64];
-    Facet fa[64];
-    int bd_from[32], bd_to[32];
-};
-// GuPCMGjkEpa.cpp:302-395
-__device__ bool epa(const GPair& pr, const Sup* sx, int sc, Pt& out, EpaWork& w) {
-    Sup* v = w.v;
-    Facet* fa = w.fa;
-    int vc = sc;
-    for (int i = 0; i < sc; ++i) v[i] = sx[i];
-    int fc = 0;
-    if (!init_epa(pr, v, vc, fa, fc)) return false;
+// ---- EPA, one wave per pair (GuPCMGjkEpa.cpp:302-395) ----
+// Lane t holds vertex t and facet t of the working sets (at most 64 of
+// each) and horizon edge t (at most 32). Every step keeps the sequential
+// order of the reference: the closest facet is the lowest (distance,
+// index) among live facets; the horizon is built over the visible facets
+// in index order, three edges each, a repeated edge cancelling its
+// reverse (the last entry moves into its place); new facets fill the
+// freed slots in increasing order, then append. All lanes run the same
+// control flow and hold the same uniform values.
+__device__ inline V shfl_v(V x, int src) {
+    return {__shfl(x.x, src), __shfl(x.y, src), __shfl(x.z, src)};
+}
+__device__ inline Sup shfl_sup(const Sup& x, int src) {
+    Sup r;
+    r.a = shfl_v(x.a, src);
+    r.b = shfl_v(x.b, src);
+    r.q = shfl_v(x.q, src);
+    r.ai = static_cast<u32>(__shfl(static_cast<int>(x.ai), src));
+    r.bi = static_cast<u32>(__shfl(static_cast<int>(x.bi), src));
+    return r;
+}
+__device__ inline Facet shfl_facet(const Facet& x, int src) {
+    Facet r;
+    r.i = __shfl(x.i, src);
+    r.j = __shfl(x.j, src);
+    r.k = __shfl(x.k, src);
+    r.n = shfl_v(x.n, src);
+    r.d = __shfl(x.d, src);
+    r.valid = __shfl(x.valid ? 1 : 0, src) != 0;
+    r.in_heap = __shfl(x.in_heap ? 1 : 0, src) != 0;
+    return r;
+}
+__device__ bool epa_wave(const GPair& pr, const Sup* sx, int sc, Pt& out) {
+    const int lane = static_cast<int>(lane_id());
+    Sup v4[4];
+    Facet f4[4];
+    int vc = sc, fc = 0;
+    for (int i = 0; i < sc; ++i) v4[i] = sx[i];
+    if (!init_epa(pr, v4, vc, f4, fc)) return false;
+    Sup myv{};
+    Facet myf{};
+    myf.valid = false;
+    myf.in_heap = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (lane == q) {
+            if (q < vc) myv = v4[q];
+            if (q < fc) myf = f4[q];
+        }
+    int bfrom = 0, bto = 0;
     const float eps = 0.1f * smin(box_min_margin(pr.ha), box_min_margin(pr.hb));
     float upper = kFltMax;
+    const u64 below = (1ull << lane) - 1ull;
     for (int it = 0; it < 60; ++it) {
-        int closest = -1;
-        float dist = kFltMax;
-        for (int i = 0; i < fc; ++i)
-            if (fa[i].valid && fa[i].in_heap && fa[i].d < dist) {
-                closest = i;
-                dist = fa[i].d;
+        // closest live facet: lowest distance below FLT_MAX, lowest index on ties
+        float key = (lane < fc && myf.valid && myf.in_heap && myf.d < kFltMax) ? myf.d : __builtin_inff();
+        int idx = lane;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ok = __shfl_xor(key, off);
+            const int oi = __shfl_xor(idx, off);
+            if (ok < key || (ok == key && oi < idx)) {
+                key = ok;
+                idx = oi;
             }
-        if (closest < 0) return false;
-        const Facet sel = fa[closest];
+        }
+        if (!(key < __builtin_inff())) return false;
+        const Facet sel = shfl_facet(myf, idx);
+        const float dist = key;
         const Sup np = support(pr, sel.n);
         const float proj = vdot(np.q, sel.n);
-        if (fabsf(proj - dist) <= eps) return contact_from_facet(pr, v, sel, out);
+        if (fabsf(proj - dist) <= eps)
+            return contact_from_facet(pr, shfl_sup(myv, sel.i & 63), shfl_sup(myv, sel.j & 63),
+                                      shfl_sup(myv, sel.k & 63), sel, out);
         if (vc == 64) return false;
         upper = smin(upper, proj);
-        bool dup = false;
-        for (int i = 0; i < vc; ++i)
-            if (v[i].ai == np.ai && v[i].bi == np.bi) {
-                dup = true;
-                break;
-            }
-        if (dup) return contact_from_facet(pr, v, sel, out);
+        if (__ballot(lane < vc && myv.ai == np.ai && myv.bi == np.bi))
+            return contact_from_facet(pr, shfl_sup(myv, sel.i & 63), shfl_sup(myv, sel.j & 63),
+                                      shfl_sup(myv, sel.k & 63), sel, out);
         const int ni = vc++;
-        v[ni] = np;
+        if (lane == ni) myv = np;
+        // visible facets (each lane its own), then the horizon in index order
+        const V fiq = shfl_v(myv.q, myf.i & 63);
+        const bool vis = lane < fc && myf.valid && !(vdot(myf.n, vsub(np.q, fiq)) < 0.0f);
+        u64 vmask = __ballot(vis);
+        if (vis) myf.valid = false;
         int bc = 0;
-        for (int i = 0; i < fc; ++i) {
-            Facet& f = fa[i];
-            if (!f.valid) continue;
-            if (vdot(f.n, vsub(np.q, v[f.i].q)) < 0.0f) continue;
-            f.valid = false;
-            const int ef[3] = {f.i, f.j, f.k}, et[3] = {f.j, f.k, f.i};
+        while (vmask) {
+            const int f = __builtin_ctzll(vmask);
+            vmask &= vmask - 1ull;
+            const int fi = __shfl(myf.i, f), fj = __shfl(myf.j, f), fk = __shfl(myf.k, f);
+            const int ef[3] = {fi, fj, fk}, et[3] = {fj, fk, fi};
+#pragma unroll
             for (int q = 0; q < 3; ++q) {
-                int rev = -1;
-                for (int j = 0; j < bc; ++j)
-                    if (w.bd_from[j] == et[q] && w.bd_to[j] == ef[q]) {
-                        rev = j;
-                        break;
-                    }
-                if (rev >= 0) {
+                const u64 m = __ballot(lane < bc && bfrom == et[q] && bto == ef[q]);
+                if (m) {
+                    const int rev = __builtin_ctzll(m);
                     --bc;
-                    w.bd_from[rev] = w.bd_from[bc];
-                    w.bd_to[rev] = w.bd_to[bc];
+                    const int lf = __shfl(bfrom, bc), lt = __shfl(bto, bc);
+                    if (lane == rev) {
+                        bfrom = lf;
+                        bto = lt;
+                    }
                 } else {
                     if (bc == 32) return false;
-                    w.bd_from[bc] = ef[q];
-                    w.bd_to[bc] = et[q];
+                    if (lane == bc) {
+                        bfrom = ef[q];
+                        bto = et[q];
+                    }
                     ++bc;
                 }
             }
         }
         if (bc == 0) return false;
-        int avail = 64 - fc;
-        for (int i = 0; i < fc; ++i) avail += !fa[i].valid;
-        if (bc > avail) return false;
-        for (int i = 0; i < bc; ++i) {
-            Facet f = make_facet(v, w.bd_from[i], w.bd_to[i], ni);
-            if (!f.valid) return false;
-            f.in_heap = f.d <= upper;
-            int slot = -1;
-            for (int j = 0; j < fc; ++j)
-                if (!fa[j].valid) {
-                    slot = j;
-                    break;
-                }
-            if (slot < 0) slot = fc++;
-            fa[slot] = f;
+        const u64 inval = __ballot(lane < fc && !myf.valid);
+        const int ninv = __popcll(inval);
+        if (bc > 64 - fc + ninv) return false;
+        // lane t < bc builds the facet of horizon edge t
+        Facet nf = make_facet_q(shfl_v(myv.q, bfrom & 63), shfl_v(myv.q, bto & 63), shfl_v(myv.q, ni), bfrom, bto,
+                                ni);
+        if (__ballot(lane < bc && !nf.valid)) return false;
+        nf.in_heap = nf.d <= upper;
+        // edge t goes to the t-th freed slot, or after the last facet
+        int t = -1;
+        if (lane < fc && !myf.valid) {
+            const int k = __popcll(inval & below);
+            if (k < bc) t = k;
+        } else if (lane >= fc) {
+            const int k = ninv + (lane - fc);
+            if (k < bc) t = k;
         }
+        const Facet got = shfl_facet(nf, t >= 0 ? t : 0);
+        if (t >= 0) myf = got;
+        fc += bc > ninv ? bc - ninv : 0;
     }
     return false;
 }
-// GuPCMGjkEpa.cpp:396-515 with the polyhedral (no sphere) branches.
-__device__ bool gjk_epa(const GPair& pr, float cd, Pt& out, EpaWork& w) {
+// GuPCMGjkEpa.cpp:396-515 with the polyhedral (no sphere) branches; every
+// lane of the wave runs it with the same values.
+__device__ bool gjk_epa(const GPair& pr, float cd, Pt& out) {
     const float eps = 0.1f * smin(box_min_margin(pr.ha), box_min_margin(pr.hb));
     constexpr float progress = 1.0f - 0.000225f;
     Sup sx[4];
@@ -696,7 +749,7 @@ __device__ bool gjk_epa(const GPair& pr, float cd, Pt& out, EpaWork& w) {
     float previous = kFltMax;
     for (int it = 0; it < 32; ++it) {
         const float dist = sqrtf(vdot(closest, closest));
-        if (count > 0 && dist <= eps) return epa(pr, sx, count, out, w);
+        if (count > 0 && dist <= eps) return epa_wave(pr, sx, count, out);
         if (dist <= 0.0f) return false;
         const V n = vmul(closest, 1.0f / dist);
         const Sup s = support(pr, vneg(closest));
@@ -714,7 +767,7 @@ __device__ bool gjk_epa(const GPair& pr, float cd, Pt& out, EpaWork& w) {
                 }
                 return false;
             }
-        if (count == 4) return epa(pr, sx, count, out, w);
+        if (count == 4) return epa_wave(pr, sx, count, out);
         sx[count++] = s;
         const Closest nx = closest_simplex(sx, count);
         if (!nx.mask) return false;
@@ -731,7 +784,7 @@ __device__ bool gjk_epa(const GPair& pr, float cd, Pt& out, EpaWork& w) {
     return false;
 }
 // GuPCMGjkEpa.cpp:527-566, box/box entry.
-__device__ bool box_box_gjk_epa(const Box3& b0, const Box3& b1, float cd, Pt& out, EpaWork& w) {
+__device__ bool box_box_gjk_epa(const Box3& b0, const Box3& b1, float cd, Pt& out) {
     const V rel = grot_inv(b1.rot, vsub(b0.pos, b1.pos));
     const Q ib{-b1.rot.x, -b1.rot.y, -b1.rot.z, b1.rot.w};
     const Q r0 = b0.rot;
@@ -740,7 +793,7 @@ __device__ bool box_box_gjk_epa(const Box3& b0, const Box3& b1, float cd, Pt& ou
                ib.w * r0.z + r0.w * ib.z + ib.x * r0.y - ib.y * r0.x,
                ib.w * r0.w - ib.x * r0.x - ib.y * r0.y - ib.z * r0.z};
     const GPair pr{b0.half, b1.half, rel, qa};
-    if (!gjk_epa(pr, cd, out, w)) return false;
+    if (!gjk_epa(pr, cd, out)) return false;
     return out.s <= cd;
 }
 
@@ -1028,11 +1081,13 @@ __global__ __launch_bounds__(128) void k_pcm_box_box(PairArgs A) {
     A.flags[t] = flags;
 }
 
-// The zero-witness continuation (scene.cpp:3543-3593), one thread per
-// deferred pair; the EPA working set lives in per-thread scratch.
+// The zero-witness continuation (scene.cpp:3543-3593), one wave per
+// deferred pair (one wave per workgroup): every lane runs GJK and the
+// wave-parallel EPA with the same values; lane 0 writes the outputs.
 __global__ __launch_bounds__(64) void k_pcm_gjk(PairArgs A) {
     const u32 n = min(*A.gjk_count, A.n_pairs);
-    for (u32 w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+    const bool writer = threadIdx.x == 0;
+    for (u32 w = blockIdx.x; w < n; w += gridDim.x) {
         const u32 t = A.gjk_list[w];
         const Box3 f = load_box(A.pos, A.quat, A.half, A.first[t]);
         const Box3 s = load_box(A.pos, A.quat, A.half, A.second[t]);
@@ -1040,9 +1095,10 @@ __global__ __launch_bounds__(64) void k_pcm_gjk(PairArgs A) {
         Cache cache = cref;
         u32 flags = A.flags[t];
         const float mm = smin(pcm_margin(f.half), pcm_margin(s.half));
-        EpaWork work;
         Pt wit;
-        if (!box_box_gjk_epa(f, s, A.cd, wit, work)) {
+        const bool hit = box_box_gjk_epa(f, s, A.cd, wit);
+        if (!writer) continue;
+        if (!hit) {
             c_set(cache, 0, true);
             cref = cache;
             A.counts[t] = 0;
@@ -1264,17 +1320,18 @@ __global__ void k_candidates(u32 n, const float* a, const float* b, float cd, fl
     for (int k = 0; k < 4; ++k) selection[4ull * t + k] = static_cast<u32>(sel[k]);
 }
 
+// One wave per pair (one wave per workgroup), as in the pipeline.
 __global__ __launch_bounds__(64) void k_gjk_epa(u32 n, const float* a, const float* b, float cd, float* out,
                                                 u32* ok) {
-    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 t = blockIdx.x;
     if (t >= n) return;
     const float* pa = a + 10ull * t;
     const float* pb = b + 10ull * t;
     const Box3 ba{{pa[0], pa[1], pa[2]}, {pa[3], pa[4], pa[5]}, {pa[6], pa[7], pa[8], pa[9]}};
     const Box3 bb{{pb[0], pb[1], pb[2]}, {pb[3], pb[4], pb[5]}, {pb[6], pb[7], pb[8], pb[9]}};
-    EpaWork w;
     Pt c{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, 0.0f};
-    const bool r = box_box_gjk_epa(ba, bb, cd, c, w);
+    const bool r = box_box_gjk_epa(ba, bb, cd, c);
+    if (threadIdx.x != 0) return;
     ok[t] = r ? 1u : 0u;
     float* o = out + 10ull * t;
     o[0] = c.a.x; o[1] = c.a.y; o[2] = c.a.z;
@@ -1305,7 +1362,7 @@ int fyzz_hip_pcm_box_box(fyzz_hip_context* ctx, uint32_t n_pairs, const uint32_t
         hipLaunchKernelGGL(pcm::k_zero1, dim3(1), dim3(1), 0, st, scratch);
         ProfScope ps(&ctx->prof, "pcm.box_box", st);
         hipLaunchKernelGGL(pcm::k_pcm_box_box, dim3(grid_for(n_pairs, 128)), dim3(128), 0, st, A);
-        hipLaunchKernelGGL(pcm::k_pcm_gjk, dim3(std::min<u32>(grid_for(n_pairs, 64), 1024u)), dim3(64), 0, st, A);
+        hipLaunchKernelGGL(pcm::k_pcm_gjk, dim3(std::min<u32>(std::max<u32>(n_pairs, 1u), 8192u)), dim3(64), 0, st, A);
         FYZZ_HIP_CHECK(hipGetLastError());
     });
 }
@@ -1343,7 +1400,7 @@ void box_box_list(fyzz_hip_context* ctx, const ListPairs& L, const float* pos, c
     hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, scratch);
     ProfScope ps(&ctx->prof, "pcm.box_box", st);
     hipLaunchKernelGGL(k_pcm_box_box, dim3(grid_for(L.cap, 128)), dim3(128), 0, st, A);
-    hipLaunchKernelGGL(k_pcm_gjk, dim3(std::min<u32>(grid_for(L.cap, 64), 1024u)), dim3(64), 0, st, A);
+    hipLaunchKernelGGL(k_pcm_gjk, dim3(std::min<u32>(std::max<u32>(L.cap, 1u), 8192u)), dim3(64), 0, st, A);
     FYZZ_HIP_CHECK(hipGetLastError());
 }
 
@@ -1379,7 +1436,7 @@ int fyzz_hip_pcm_gjk_epa(fyzz_hip_context* ctx, uint32_t n, const float* a, cons
     return guarded(ctx, [&] {
         if (!n) return;
         FYZZ_REQUIRE(a && b && out && ok, kInvalidArgument, "pcm_gjk_epa: null pointer");
-        hipLaunchKernelGGL(pcm::k_gjk_epa, dim3(grid_for(n, 64)), dim3(64), 0, ctx->stream, n, a, b,
+        hipLaunchKernelGGL(pcm::k_gjk_epa, dim3(n), dim3(64), 0, ctx->stream, n, a, b,
                            contact_distance, out, ok);
         FYZZ_HIP_CHECK(hipGetLastError());
     });
