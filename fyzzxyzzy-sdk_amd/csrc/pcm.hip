@@ -633,23 +633,34 @@ __device__ inline Facet shfl_facet(const Facet& x, int src) {
     r.in_heap = __shfl(x.in_heap ? 1 : 0, src) != 0;
     return r;
 }
-__device__ bool epa_wave(const GPair& pr, const Sup* sx, int sc, Pt& out) {
+// sx: the simplex (sc vertices) held by every lane.
+__device__ bool epa_wave(const GPair& pr, const Sup (&sx)[4], int sc, Pt& out) {
     const int lane = static_cast<int>(lane_id());
+    // the initial polytope, built by lane 0 and spread one entry per lane
     Sup v4[4];
     Facet f4[4];
-    int vc = sc, fc = 0;
-    for (int i = 0; i < sc; ++i) v4[i] = sx[i];
-    if (!init_epa(pr, v4, vc, f4, fc)) return false;
+    int vc = sc, fc = 0, ok = 0;
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v4[i] = sx[i];
+        ok = init_epa(pr, v4, vc, f4, fc) ? 1 : 0;
+    }
+    if (!__shfl(ok, 0)) return false;
+    vc = __shfl(vc, 0);
+    fc = __shfl(fc, 0);
     Sup myv{};
     Facet myf{};
     myf.valid = false;
     myf.in_heap = false;
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 4; ++q) {
+        const Sup vq = shfl_sup(v4[q], 0);
+        const Facet fq = shfl_facet(f4[q], 0);
         if (lane == q) {
-            if (q < vc) myv = v4[q];
-            if (q < fc) myf = f4[q];
+            if (q < vc) myv = vq;
+            if (q < fc) myf = fq;
         }
+    }
     int bfrom = 0, bto = 0;
     const float eps = 0.1f * smin(box_min_margin(pr.ha), box_min_margin(pr.hb));
     float upper = kFltMax;
@@ -738,39 +749,39 @@ __device__ bool epa_wave(const GPair& pr, const Sup* sx, int sc, Pt& out) {
     }
     return false;
 }
-// GuPCMGjkEpa.cpp:396-515 with the polyhedral (no sphere) branches; every
-// lane of the wave runs it with the same values.
-__device__ bool gjk_epa(const GPair& pr, float cd, Pt& out) {
+// GuPCMGjkEpa.cpp:396-515 with the polyhedral (no sphere) branches, on one
+// lane. Returns 0 (no contact), 1 (contact in out) or 2 (EPA from the
+// simplex sx[0, count)).
+__device__ int gjk_lane(const GPair& pr, float cd, Pt& out, Sup (&sx)[4], int& count) {
     const float eps = 0.1f * smin(box_min_margin(pr.ha), box_min_margin(pr.hb));
     constexpr float progress = 1.0f - 0.000225f;
-    Sup sx[4];
-    int count = 0;
+    count = 0;
     V closest = vdot(pr.a_in_b, pr.a_in_b) > 0.0f ? pr.a_in_b : V{1, 0, 0};
     float previous = kFltMax;
     for (int it = 0; it < 32; ++it) {
         const float dist = sqrtf(vdot(closest, closest));
-        if (count > 0 && dist <= eps) return epa_wave(pr, sx, count, out);
-        if (dist <= 0.0f) return false;
+        if (count > 0 && dist <= eps) return 2;
+        if (dist <= 0.0f) return 0;
         const V n = vmul(closest, 1.0f / dist);
         const Sup s = support(pr, vneg(closest));
         const float proj = vdot(n, s.q);
-        if (proj > cd) return false;
+        if (proj > cd) return 0;
         if (count > 0 && proj > dist * progress) {
             const Closest cur = closest_simplex(sx, count);
-            return cur.mask && contact_from_gjk(pr, cur, out);
+            return cur.mask && contact_from_gjk(pr, cur, out) ? 1 : 0;
         }
         for (int i = 0; i < count; ++i)
             if (s.ai == sx[i].ai && s.bi == sx[i].bi) {
                 if (proj > 0.f) {
                     const Closest cur = closest_simplex(sx, count);
-                    return cur.mask && contact_from_gjk(pr, cur, out) && out.s <= cd;
+                    return cur.mask && contact_from_gjk(pr, cur, out) && out.s <= cd ? 1 : 0;
                 }
-                return false;
+                return 0;
             }
-        if (count == 4) return epa_wave(pr, sx, count, out);
+        if (count == 4) return 2;
         sx[count++] = s;
         const Closest nx = closest_simplex(sx, count);
-        if (!nx.mask) return false;
+        if (!nx.mask) return 0;
         closest = nx.p;
         const float nd = sqrtf(nx.d2);
         int wr = 0;
@@ -778,10 +789,27 @@ __device__ bool gjk_epa(const GPair& pr, float cd, Pt& out) {
             if ((nx.mask & (1u << r)) && nx.w[r] > 1.0e-6f) sx[wr++] = sx[r];
         if (wr == 0 && count > 0) wr = 1;
         count = wr;
-        if (nd >= previous) return false;
+        if (nd >= previous) return 0;
         previous = nd;
     }
-    return false;
+    return 0;
+}
+__device__ inline Pt shfl_pt(const Pt& x, int src) { return {shfl_v(x.a, src), shfl_v(x.b, src), shfl_v(x.n, src), __shfl(x.s, src)}; }
+// GJK on lane 0; its result, or the EPA it hands over, shared by the wave.
+__device__ bool gjk_epa(const GPair& pr, float cd, Pt& out) {
+    Sup sx[4];
+    int count = 0, r = 0;
+    if (lane_id() == 0) r = gjk_lane(pr, cd, out, sx, count);
+    r = __shfl(r, 0);
+    if (r == 2) {
+        count = __shfl(count, 0);
+        Sup all[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) all[i] = shfl_sup(sx[i], 0);
+        return epa_wave(pr, all, count, out);
+    }
+    out = shfl_pt(out, 0);
+    return r == 1;
 }
 // GuPCMGjkEpa.cpp:527-566, box/box entry.
 __device__ bool box_box_gjk_epa(const Box3& b0, const Box3& b1, float cd, Pt& out) {
@@ -1091,13 +1119,13 @@ __global__ __launch_bounds__(64) void k_pcm_gjk(PairArgs A) {
         const u32 t = A.gjk_list[w];
         const Box3 f = load_box(A.pos, A.quat, A.half, A.first[t]);
         const Box3 s = load_box(A.pos, A.quat, A.half, A.second[t]);
+        Pt wit;
+        const bool hit = box_box_gjk_epa(f, s, A.cd, wit);
+        if (!writer) continue;
         Cache& cref = A.cache[A.cache_index ? A.cache_index[t] : t];
         Cache cache = cref;
         u32 flags = A.flags[t];
         const float mm = smin(pcm_margin(f.half), pcm_margin(s.half));
-        Pt wit;
-        const bool hit = box_box_gjk_epa(f, s, A.cd, wit);
-        if (!writer) continue;
         if (!hit) {
             c_set(cache, 0, true);
             cref = cache;
