@@ -50,11 +50,18 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 METRIC = "body-steps/s at 100k-box pile + 4096-scene batch vs plain; SAP/radix HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes moved by one frame's endpoint radix sort (histogram + 4
-# one-sweep passes over 3 x 2n keys, n = 1e6), from the two rocprofv3 PMC
-# passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 after the copy calibration)
-# summarised in profiles/sap_1m_incremental.md; None until measured.
-SAP_SORT_TRAFFIC_BYTES = None
+# HBM bytes moved by one frame's endpoint radix sort (4 passes of tile
+# histogram + tile scan + scatter over 3 x 2n keys, n = 1e6, ranks from the
+# last scatter), from the two rocprofv3 PMC passes (FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0 after the copy calibration) summarised in
+# profiles/sap_1m_incremental.md.
+SAP_SORT_TRAFFIC_BYTES = 834_500_000
+# HBM bytes of one k_pgs launch on the formed pile (3 steps after the state
+# at step 610 is loaded: 224k managers, 601k contact rows, 789k friction
+# rows, 5 iterations), same passes, profiles/pile_100k_full_step.md; the
+# algorithmic bytes of those launches by the byte model below.
+PILE_PGS_TRAFFIC = {"bytes": 2_694_000_000, "algorithmic_bytes": 488_600_000,
+                    "measured_on": "formed pile: 3 steps after loading the state at step 610 (224k managers)"}
 
 
 def parse():
@@ -403,10 +410,14 @@ def bench_pile(args):
                    "partitions_per_step": tot["partitions"] / max(tot["steps"], 1),
                    "partition_rounds_per_step": tot["rounds"] / max(tot["steps"], 1)},
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": PILE_PGS_TRAFFIC["bytes"] if dominant == "dy.pgs" else None,
                      "algorithmic_bytes_per_launch": alg / max(stages[dominant]["launches"], 1)},
         "stages": stages,
     }
+    if dominant == "dy.pgs":
+        result["roofline"]["traffic_measured_on"] = PILE_PGS_TRAFFIC["measured_on"]
+        result["roofline"]["traffic_algorithmic_bytes"] = PILE_PGS_TRAFFIC["algorithmic_bytes"]
     return result, (w, sc)
 
 
