@@ -517,12 +517,41 @@ struct Sap {
     u32 list_cap() const { return std::max(pair_cap, delta_cap); }
 };
 
+// Pair keys (found << 63 | lo << 32 | hi, slots below 2^B) packed into
+// their 2B + 1 significant bits in the same order, so the list sort runs
+// ceil((2B + 1) / 8) passes instead of 8; unpacked on the way back.
+__global__ void k_pack_pairs(const u32* d_n, u32 cap, const u32* gate, u64* keys, u32 bits) {
+    if (gate && *gate == 0) return;
+    const u32 n = min(*d_n, cap);
+    const u64 mask = (1ull << bits) - 1ull;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const u64 k = keys[i];
+        keys[i] = ((k >> 63) << (2 * bits)) | (((k >> 32) & mask) << bits) | (k & mask);
+    }
+}
+__global__ void k_unpack_pairs(const u32* d_n, u32 cap, const u32* gate, const u64* src, u64* keys, u32 bits) {
+    if (gate && *gate == 0) return;
+    const u32 n = min(*d_n, cap);
+    const u64 mask = (1ull << bits) - 1ull;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const u64 k = src[i];
+        keys[i] = ((k >> (2 * bits)) << 63) | (((k >> bits) & mask) << 32) | (k & mask);
+    }
+}
+
 void sort_u64(Sap& s, hipStream_t st, const u32* d_count, u64* keys, u64* tmp, const u32* gate = nullptr) {
     radix::Batch b{s.list_cap(), 1, s.list_cap(), 0, d_count, nullptr, 0, gate};
+    u32 bits = 1;
+    while (bits < 31 && (1ull << bits) < s.n) ++bits;  // slots < 2^bits
+    const int key_bits = static_cast<int>((2 * bits + 1 + 7) / 8 * 8);
+    const dim3 grid(std::min<u32>(grid_for(s.list_cap(), 256), 4096u));
+    hipLaunchKernelGGL(k_pack_pairs, grid, dim3(256), 0, st, d_count, s.list_cap(), gate, keys, bits);
     const bool in_a = radix::sort<u64>(st, b, radix::Source::Arrays, keys, s.dummy_a.p, keys,
-                                       s.dummy_a.p, tmp, s.dummy_b.p, 64, s.radix_counts.p,
+                                       s.dummy_a.p, tmp, s.dummy_b.p, key_bits, s.radix_counts.p,
                                        s.radix_totals.p);
-    if (!in_a) FYZZ_HIP_CHECK(hipMemcpyAsync(keys, tmp, sizeof(u64) * s.list_cap(), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_unpack_pairs, grid, dim3(256), 0, st, d_count, s.list_cap(), gate, in_a ? keys : tmp, keys,
+                       bits);
+    FYZZ_HIP_CHECK(hipGetLastError());
 }
 
 void advance(Sap& s, const float* pos, const float* quat, const float* half, const float* off,
