@@ -553,23 +553,20 @@ __global__ __launch_bounds__(kBlock) void k_part_maxdyn(Ws W) {
     atomicMax(&W.max_dyn[M.body1], pm + 1);
 }
 
-// dynamic/static managers follow the body's dynamic partitions
+// Dynamic/static managers follow the body's dynamic partitions; then each
+// of the body's managers is ranked by partition (the order in which a
+// sweep reaches them; a body's managers have distinct partitions). A
+// static manager touches one dynamic body, so this thread owns its
+// partition; the dynamic ones are final after the colouring.
 __global__ __launch_bounds__(kBlock) void k_part_static(Ws W) {
     const u32 b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= W.nb) return;
+    const u32 lo = W.off[b], hi = W.off[b + 1];
     u32 k = W.max_dyn[b];
-    for (u32 i = W.off[b]; i < W.off[b + 1]; ++i) {
+    for (u32 i = lo; i < hi; ++i) {
         const u32 m = W.inc[i] >> 1;
         if (!dyn_dyn(W.mg[m])) W.part[m] = k++;
     }
-}
-
-// Each body's managers ranked by partition (the order in which a sweep
-// reaches them; the partitions of a body's managers are distinct).
-__global__ __launch_bounds__(kBlock) void k_part_rank(Ws W) {
-    const u32 b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= W.nb) return;
-    const u32 lo = W.off[b], hi = W.off[b + 1];
     for (u32 i = lo; i < hi; ++i) {
         const u32 e = W.inc[i];
         const u32 pe = W.part[e >> 1];
@@ -1713,7 +1710,6 @@ void dy_prepare(fyzz_hip_dy* d, float dt, uint32_t nb, fyzz_dy_body* bodies, uin
                         d->plain_launch || !d->ctx->cooperative);
             hipLaunchKernelGGL(dy::k_part_maxdyn, gm, blk, 0, st, w);
             if (nb) hipLaunchKernelGGL(dy::k_part_static, dim3(grid_for(nb, dy::kBlock)), blk, 0, st, w);
-            if (nb) hipLaunchKernelGGL(dy::k_part_rank, dim3(grid_for(nb, dy::kBlock)), blk, 0, st, w);
             hipLaunchKernelGGL(dy::k_part_sizes, gm, blk, 0, st, w);
         }
         hipLaunchKernelGGL(dy::k_part_offsets, dim3(1), blk, 0, st, w);

@@ -257,33 +257,38 @@ __device__ inline WalkGeom walk_geom(const CrossArgs& A, const Walk& w, u32 v_ol
     g.up = (w.arr != 0) == (v_new > v_old);  // old array: toward v_new
     return g;
 }
-// Is the step-th entry from e inside the crossed range? (j receives its index)
-__device__ inline bool walk_inside(const CrossArgs& A, const WalkGeom& g, u32 step, u32& j) {
+// Is the step-th entry from e inside the crossed range? (j receives its
+// index and pj its payload, both loaded together)
+__device__ inline bool walk_inside(const CrossArgs& A, const WalkGeom& g, u32 step, u32& j, u32& pj) {
     if (g.up) {
         j = g.own + step;
         if (j >= A.n2) return false;
         const u32 kj = g.K[j];
-        return kj < g.bound_v || (kj == g.bound_v && g.P[j] < g.pe);
+        pj = g.P[j];
+        return kj < g.bound_v || (kj == g.bound_v && pj < g.pe);
     }
     if (step > g.own) return false;
     j = g.own - step;
     const u32 kj = g.K[j];
-    return kj > g.bound_v || (kj == g.bound_v && g.P[j] > g.pe);
+    pj = g.P[j];
+    return kj > g.bound_v || (kj == g.bound_v && pj > g.pe);
 }
-// A crossed entry: a candidate pair when it is the other kind of endpoint
-// of another box; re-tested on all three axes with old and new records.
-// ca: the walking box's coarse bounds (its old and new keys). A candidate
-// whose coarse bounds miss them on some axis overlaps it neither before
-// nor after, so its status cannot change.
-__device__ inline void walk_visit(const CrossArgs& A, const Walk& w, const WalkGeom& g, u32 j, const Rec& an,
-                                  const Rec& ao, const uint4& ca) {
-    const u32 pf = g.P[j];
+// A crossed entry (payload pf): a candidate pair when it is the other kind
+// of endpoint of another box; re-tested on all three axes with old and new
+// records. ca: the walking box's coarse bounds (its old and new keys). A
+// candidate whose coarse bounds miss them on some axis overlaps it neither
+// before nor after, so its status cannot change. The moved bit and the
+// coarse bounds are loaded together.
+__device__ inline void walk_visit(const CrossArgs& A, const Walk& w, u32 pf, const Rec& an, const Rec& ao,
+                                  const uint4& ca) {
     const u32 b = pf >> 1;
     if ((pf & 1u) == w.kind || b == w.a) return;
-    const bool b_moved = (A.moved_bits[b >> 5] >> (b & 31u)) & 1u;
+    const u32 bits = A.moved_bits[b >> 5];
+    const uint4 cb = A.coarse[b];
+    const bool b_moved = (bits >> (b & 31u)) & 1u;
     // an unmoved endpoint sits at the same value in both arrays: the
     // new-array walk already saw it
-    if ((b_moved || !w.arr) && coarse_overlap(ca, A.coarse[b])) {
+    if ((b_moved || !w.arr) && coarse_overlap(ca, cb)) {
         const Rec bn = A.cur[b];
         if (pair_allowed(an, bn)) {
             const Rec bo = b_moved ? A.prev[b] : bn;
@@ -309,9 +314,9 @@ __global__ __launch_bounds__(256) void k_cross(CrossArgs A) {
         const WalkGeom g = walk_geom(A, w, v_old, v_new);
         const uint4 ca = A.coarse[w.a];
         for (u32 step = 1;; ++step) {
-            u32 j;
-            if (!walk_inside(A, g, step, j)) break;
-            walk_visit(A, w, g, j, an, ao, ca);
+            u32 j, pj = 0;
+            if (!walk_inside(A, g, step, j, pj)) break;
+            walk_visit(A, w, pj, an, ao, ca);
             if (step == kShortWalk) {
                 const u32 slot = atomicAdd(&A.cnt[kLongWalks], 1u);
                 if (slot < A.long_cap)
@@ -339,9 +344,9 @@ __global__ __launch_bounds__(256) void k_cross_long(CrossArgs A) {
         const WalkGeom g = walk_geom(A, w, ao.k[w.axis + 3 * w.kind], an.k[w.axis + 3 * w.kind]);
         const uint4 ca = A.coarse[w.a];
         for (u32 base = kShortWalk;; base += kWave) {
-            u32 j = 0;
-            const bool inside = walk_inside(A, g, base + lane + 1, j);
-            if (inside) walk_visit(A, w, g, j, an, ao, ca);
+            u32 j = 0, pj = 0;
+            const bool inside = walk_inside(A, g, base + lane + 1, j, pj);
+            if (inside) walk_visit(A, w, pj, an, ao, ca);
             if (__ballot(inside) != ~0ull) break;
         }
     }
