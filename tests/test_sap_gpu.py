@@ -217,8 +217,12 @@ def bp_bounds(ctx, torch, pos, quat, half, off):
     return b
 
 
-def test_incremental_sap_large_change_lists(ctx, orc, torch_cuda):
+@pytest.mark.parametrize("n", [30000, 65537])
+def test_incremental_sap_large_change_lists(ctx, orc, torch_cuda, n):
     # every box moves far: the per-frame candidate list exceeds the
-    # one-workgroup path and the multi-block sort/unique path runs
-    _run_frames(ctx, orc, torch_cuda, 30000, 3, 12, 1.5, 15.0, 1.0, False, check_endpoints=False,
-                delta_capacity=4_000_000)
+    # one-workgroup path and the multi-block sort/unique path runs. The list
+    # sort packs pair keys to 2B + 1 bits (slots < 2^B): 30000 boxes sort 32
+    # bits in 4 passes, 65537 boxes 40 bits in 5 (an odd pass count, the
+    # result coming back from the second buffer).
+    _run_frames(ctx, orc, torch_cuda, n, 3, 12, 1.5, 15.0, 1.0, False, check_endpoints=False,
+                delta_capacity=4_000_000 * max(1, n // 30000))
