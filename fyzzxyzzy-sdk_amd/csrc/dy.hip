@@ -94,7 +94,8 @@ struct Ws {
     // per manager
     u32 *inc, *pred, *part, *pend, *sorted, *order;
     u64* entry;  // [2 nm] partition colouring: ready | level << 32 | mask after the manager, per side
-    int4* sinfo; // [nm] per partition-sorted position: body0, body1, manager
+    int4* sinfo; // [nm] per partition-sorted position: body0, body1, manager, contact rows | anchors << 8
+    u32* pos_of;    // [nm] partition-sorted position of each manager in a partition
     uint4* srank;   // [nm] per position: rank of the manager among body0's / body1's (by partition), their degrees
     u32* erank;     // [2 nm] per incidence entry: rank among the body's managers by partition
     u32* ver;       // [nb] solver updates applied to the body in this sweep
@@ -645,6 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(Ws W) {
         const fyzz_dy_manager& M = W.mg[m];
         const u32 pos = base[pm] + local;
         W.sorted[pos] = m;
+        W.pos_of[m] = pos;
         W.sinfo[pos] = make_int4(M.body0, M.body1, static_cast<int>(m), 0);
         W.srank[pos] = make_uint4(M.body0 >= 0 ? W.erank[2 * m] : 0u, M.body1 >= 0 ? W.erank[2 * m + 1] : 0u,
                                   M.body0 >= 0 ? W.deg[M.body0] : 0u, M.body1 >= 0 ? W.deg[M.body1] : 0u);
@@ -1043,6 +1045,8 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
     W.patches[m] = o;
     W.order[m] = no | (order[0] << 8) | ((no > 1 ? order[1] : 0u) << 12) | ((no > 2 ? order[2] : 0u) << 16) |
                  ((no > 3 ? order[3] : 0u) << 20);
+    // the sweep reads the row counts with the position record
+    W.sinfo[W.pos_of[m]].w = static_cast<int>(no | (P.anchors << 8));
 }
 
 // ------------------------------------------------------------------ solve
@@ -1076,16 +1080,19 @@ __device__ void solve_manager(const Ws& W, int4 info, bool snap0, bool snap1, u3
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 la = i0 >= 0 ? load_body(&W.lin[i0]) : z4, lb = i1 >= 0 ? load_body(&W.lin[i1]) : z4;
     const float4 aa4 = i0 >= 0 ? load_body(&W.ang[i0]) : z4, ab4 = i1 >= 0 ? load_body(&W.ang[i1]) : z4;
+    // row counts come with the position record, so only the rows in use
+    // are loaded, all at once
+    const u32 no = static_cast<u32>(info.w) & 0xffu;
+    const u32 anchors = (static_cast<u32>(info.w) >> 8) & 0xffu;
     CRow rows[4];
     FRow frows[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        rows[q] = W.crow[4 * m + q];
-        frows[q] = W.frow[4 * m + q];
+    for (u32 q = 0; q < 4; ++q) {
+        if (q < no) rows[q] = W.crow[4 * m + q];
+        if (q < 2 * anchors) frows[q] = W.frow[4 * m + q];
     }
     V va = f3(la), vb = f3(lb), wa = f3(aa4), wb = f3(ab4);
     const float ima = la.w, imb = lb.w;
-    const u32 no = W.order[m] & 0xffu;
     bool bad = false;
     float normal_impulse = 0.0f;
 #pragma unroll
@@ -1115,7 +1122,6 @@ __device__ void solve_manager(const Ws& W, int4 info, bool snap0, bool snap1, u3
         normal_impulse += next;
     }
     fyzz_dy_patch& P = W.patches[m];
-    const u32 anchors = P.anchor_count;
     if (W.tape && iter < W.tape_iters) W.tape_ni[static_cast<std::size_t>(iter) * W.nm + m] = normal_impulse;
     if ((iter >= pos_iters || pos_iters - iter <= 3u) && anchors) {
         const float coef = anchors == 2 ? 0.5f : 1.0f;
@@ -1194,7 +1200,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
     const u32 total = pos_iters + vel_iters;
     const u32 n = np ? W.part_off[np] : 0u;
     u32 iter = 0, i = tid, spins = 0, idle = 0;
-    bool have = false;
+    bool have = false, ready0 = false, ready1 = false;
     int4 info = make_int4(-1, -1, 0, 0);
     uint4 rk = make_uint4(0u, 0u, 0u, 0u);
     for (;;) {
@@ -1208,13 +1214,17 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 info = W.sinfo[i];
                 rk = W.srank[i];
                 have = true;
+                ready0 = info.x < 0;
+                ready1 = info.y < 0;
             }
+            // a body whose version reached the awaited value stays there
+            // until this manager runs: it is not polled again
             const u32 need0 = iter * rk.z + rk.x, need1 = iter * rk.w + rk.y;
-            const bool ok0 = info.x < 0 || __hip_atomic_load(&W.ver[info.x], __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_AGENT) == need0;
-            const bool ok1 = info.y < 0 || __hip_atomic_load(&W.ver[info.y], __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_AGENT) == need1;
-            if (ok0 && ok1) {
+            if (!ready0)
+                ready0 = __hip_atomic_load(&W.ver[info.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == need0;
+            if (!ready1)
+                ready1 = __hip_atomic_load(&W.ver[info.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == need1;
+            if (ready0 && ready1) {
                 const bool snap = iter + 1 == pos_iters;
                 solve_manager(W, info, snap && rk.x + 1 == rk.z, snap && rk.y + 1 == rk.w, iter, pos_iters, total);
                 wait_stores();
@@ -1617,7 +1627,7 @@ struct fyzz_hip_dy {
     DevBuf<u64> entry;
     DevBuf<int4> sinfo;
     DevBuf<uint4> srank;
-    DevBuf<u32> erank, ver;
+    DevBuf<u32> erank, ver, pos_of;
     DevBuf<dy::CRow> crow;
     DevBuf<dy::FRow> frow;
     DevBuf<u32> part_count, part_off, part_fill, blocksums, ctr, flags, bar;
@@ -1856,6 +1866,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->srank.reserve(nm);
         d->erank.reserve(2 * nm);
         d->ver.reserve(nb);
+        d->pos_of.reserve(nm);
         for (auto* b : {&d->part, &d->pend, &d->sorted, &d->order}) b->reserve(nm);
         d->crow.reserve(4 * nm);
         d->frow.reserve(4 * nm);
@@ -1899,6 +1910,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         w.srank = d->srank.get();
         w.erank = d->erank.get();
         w.ver = d->ver.get();
+        w.pos_of = d->pos_of.get();
         w.pend = d->pend.get();
         w.sorted = d->sorted.get();
         w.order = d->order.get();
@@ -1914,7 +1926,8 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         // every workspace array the step kernels touch must exist
         const void* need[] = {w.pv, w.pw, w.lin, w.ang, w.mlin, w.mang, w.rpos, w.rq, w.sii, w.has_row,
                               w.deg, w.off, w.fill, w.max_dyn, w.body_mask, w.inc, w.pred, w.part, w.pend,
-                              w.sorted, w.order, w.entry, w.sinfo, w.srank, w.erank, w.ver, w.crow, w.frow,
+                              w.sorted, w.order, w.entry, w.sinfo, w.srank, w.erank, w.ver, w.pos_of, w.crow,
+                              w.frow,
                               w.part_count, w.part_off, w.part_fill, w.blocksums, w.ctr, w.flags, w.bar};
         for (const void* q : need) FYZZ_REQUIRE(q != nullptr, kDeviceError, "dy_create: workspace array missing");
         *out = d.release();
