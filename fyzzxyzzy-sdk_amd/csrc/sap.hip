@@ -660,6 +660,13 @@ struct fyzz_hip_sap {
 };
 
 namespace fyzz_hip::sap {
+bool built(fyzz_hip_sap* h) { return h->s.built; }
+void replay_advance(fyzz_hip_sap* h) {
+    Sap& s = h->s;
+    s.frame = (s.frame + 1) & 0x7fffffffu;
+    if (s.frame == 0) s.frame = 1;
+    s.cur ^= 1;
+}
 Lists lists(fyzz_hip_sap* h) {
     Sap& s = h->s;
     const u32 cur = s.cur == 0 ? kPairs0 : kPairs1;

@@ -24,6 +24,11 @@ struct Lists {
     const u32* d_lost;
 };
 Lists lists(fyzz_hip_sap* h);
+// Whether a frame has been advanced (the next one is incremental).
+bool built(fyzz_hip_sap* h);
+// The host-side bookkeeping of one incremental advance (frame stamp,
+// current buffer), for a replayed graph that recorded such an advance.
+void replay_advance(fyzz_hip_sap* h);
 }  // namespace sap
 
 namespace pcm {

@@ -842,12 +842,16 @@ void step(fyzz_hip_world* w, float dt) {
 // the world's own stream, ordered after and before the context stream's
 // work by events.
 void step_graph(fyzz_hip_world* w, float dt) {
-    FYZZ_REQUIRE(w->n_scenes, kLogicError,
-                 "world: graph stepping runs the per-scene broadphase (set_scenes); the incremental SAP keeps "
-                 "host-side frame state");
     if (!w->uploaded) upload(w);
     if (!w->n) {
         ++w->step_index;
+        return;
+    }
+    // The incremental SAP builds its endpoint arrays on its first frame;
+    // the graphs record the incremental frame (whose stamps only label
+    // records, as the world lists no dirty boxes).
+    if (!w->n_scenes && !sap::built(w->sap)) {
+        step(w, dt);
         return;
     }
     if (!w->gstream) {
@@ -896,6 +900,7 @@ void step_graph(fyzz_hip_world* w, float dt) {
         w->ccur ^= 1;
         w->hcur ^= 1;
         w->scur ^= 1;
+        if (!w->n_scenes) sap::replay_advance(w->sap);
     }
     FYZZ_HIP_CHECK(hipGraphLaunch(w->gexec[parity], w->gstream));
     FYZZ_HIP_CHECK(hipEventRecord(w->gev_out, w->gstream));

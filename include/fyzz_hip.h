@@ -369,10 +369,11 @@ int fyzz_hip_world_set_materials(fyzz_hip_world* world, uint32_t first_id, uint3
                                  const float* dynamic_friction, const float* restitution);
 /* World::step (world.h:459); asynchronous. */
 int fyzz_hip_world_step(fyzz_hip_world* world, float dt);
-/* One World::step through a captured HIP graph (worlds with scenes,
- * fyzz_hip_world_set_scenes): the first two calls capture one graph each
- * (the step's double buffers alternate), later calls replay them; a new dt
- * recaptures. Results equal fyzz_hip_world_step's. Asynchronous. */
+/* One World::step through a captured HIP graph: the first two calls
+ * capture one graph each (the step's double buffers alternate), later
+ * calls replay them; a new dt recaptures. A world without scenes runs its
+ * first step (the broadphase's full build) directly. Results equal
+ * fyzz_hip_world_step's. Asynchronous. */
 int fyzz_hip_world_step_graph(fyzz_hip_world* world, float dt);
 /* Autotuner of the launch shapes (no counterpart in the reference): times
  * trial_steps real steps per candidate workgroup count of the partition
