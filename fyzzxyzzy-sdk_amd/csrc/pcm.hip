@@ -421,31 +421,53 @@ __device__ inline Sup support(const GPair& p, V dir) {
 __device__ inline V pair_local_a(const GPair& p, V pb) {
     return grot(Q{-p.rel.x, -p.rel.y, -p.rel.z, p.rel.w}, vsub(pb, p.a_in_b));
 }
-// GuPCMGjkEpa.cpp:87-112 (explicit fused multiply-add, as written there)
-__device__ bool solve(int n, float a[3][4], float r[3]) {
+// GuPCMGjkEpa.cpp:87-112 (explicit fused multiply-add, as written there),
+// for an N x N system with its right-hand side in column N. Every index is
+// a compile-time constant (the pivot row swap is a select over the rows),
+// so the system stays in registers.
+template <int N>
+__device__ inline bool solve_n(float (&a)[3][4], float (&r)[3]) {
     float scale = 0.0f;
-    for (int i = 0; i < n; ++i) scale = smax(scale, fabsf(a[i][i]));
+#pragma unroll
+    for (int i = 0; i < N; ++i) scale = smax(scale, fabsf(a[i][i]));
     if (!(scale > 0.0f) || !isfinite(scale)) return false;
     const float tol = kFltEps * scale;
-    for (int col = 0; col < n; ++col) {
+#pragma unroll
+    for (int col = 0; col < N; ++col) {
         int piv = col;
-        for (int row = col + 1; row < n; ++row)
-            if (fabsf(a[row][col]) > fabsf(a[piv][col])) piv = row;
-        if (!(fabsf(a[piv][col]) > tol)) return false;
-        for (int j = col; j <= n; ++j) {
-            const float t = a[col][j];
-            a[col][j] = a[piv][j];
-            a[piv][j] = t;
+        float best = fabsf(a[col][col]);
+#pragma unroll
+        for (int row = col + 1; row < N; ++row) {
+            const float v = fabsf(a[row][col]);
+            if (v > best) {
+                piv = row;
+                best = v;
+            }
         }
+        if (!(best > tol)) return false;
+#pragma unroll
+        for (int row = col + 1; row < N; ++row)
+            if (piv == row) {
+#pragma unroll
+                for (int j = col; j <= N; ++j) {
+                    const float t = a[col][j];
+                    a[col][j] = a[row][j];
+                    a[row][j] = t;
+                }
+            }
         const float div = a[col][col];
-        for (int j = col; j <= n; ++j) a[col][j] /= div;
-        for (int row = 0; row < n; ++row)
+#pragma unroll
+        for (int j = col; j <= N; ++j) a[col][j] /= div;
+#pragma unroll
+        for (int row = 0; row < N; ++row)
             if (row != col) {
                 const float f = a[row][col];
-                for (int j = col; j <= n; ++j) a[row][j] = fmaf(-f, a[col][j], a[row][j]);
+#pragma unroll
+                for (int j = col; j <= N; ++j) a[row][j] = fmaf(-f, a[col][j], a[row][j]);
             }
     }
-    for (int i = 0; i < n; ++i) r[i] = a[i][n];
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] = a[i][N];
     return true;
 }
 struct Closest {
@@ -454,41 +476,57 @@ struct Closest {
     unsigned mask;
     float d2;
 };
-// GuPCMGjkEpa.cpp:119-161
-__device__ Closest closest_simplex(const Sup* sx, int count) {
+// GuPCMGjkEpa.cpp:119-161 for a simplex of COUNT vertices: every subset in
+// increasing mask order, barycentric weights from the subset's normal
+// equations, the feasible point nearest the origin (first one on ties).
+// Fully unrolled, so the subsets' index lists are constants.
+template <int COUNT>
+__device__ Closest closest_simplex_n(const Sup (&sx)[4]) {
     Closest best;
     best.p = best.a = best.b = V{0, 0, 0};
     best.w[0] = best.w[1] = best.w[2] = best.w[3] = 0.0f;
     best.mask = 0;
     best.d2 = kFltMax;
-    for (unsigned mask = 1; mask < (1u << count); ++mask) {
+#pragma unroll
+    for (unsigned mask = 1; mask < (1u << COUNT); ++mask) {
         int ids[4] = {0, 0, 0, 0}, n = 0;
-        for (int i = 0; i < count; ++i)
+#pragma unroll
+        for (int i = 0; i < COUNT; ++i)
             if (mask & (1u << i)) ids[n++] = i;
         float w[4] = {0, 0, 0, 0};
         w[ids[0]] = 1.0f;
+        bool solved = true;
         if (n > 1) {
             const V base = sx[ids[0]].q;
             V dirs[3];
             float m[3][4];
+#pragma unroll
             for (int i = 1; i < n; ++i) dirs[i - 1] = vsub(sx[ids[i]].q, base);
+#pragma unroll
             for (int i = 1; i < n; ++i) {
+#pragma unroll
                 for (int j = 1; j < n; ++j) m[i - 1][j - 1] = vdot(dirs[i - 1], dirs[j - 1]);
                 m[i - 1][n - 1] = -vdot(dirs[i - 1], base);
             }
             float co[3];
-            if (!solve(n - 1, m, co)) continue;
-            for (int i = 1; i < n; ++i) {
-                w[ids[i]] = co[i - 1];
-                w[ids[0]] -= co[i - 1];
+            solved = n == 2 ? solve_n<1>(m, co) : n == 3 ? solve_n<2>(m, co) : solve_n<3>(m, co);
+            if (solved) {
+#pragma unroll
+                for (int i = 1; i < n; ++i) {
+                    w[ids[i]] = co[i - 1];
+                    w[ids[0]] -= co[i - 1];
+                }
             }
         }
+        if (!solved) continue;
         bool feasible = true;
+#pragma unroll
         for (int i = 0; i < n; ++i)
             if (w[ids[i]] < -1.0e-5f) feasible = false;
         if (!feasible) continue;
         V q{0, 0, 0}, pa{0, 0, 0}, pb{0, 0, 0};
-        for (int i = 0; i < count; ++i) {
+#pragma unroll
+        for (int i = 0; i < COUNT; ++i) {
             q = vadd(q, vmul(sx[i].q, w[i]));
             pa = vadd(pa, vmul(sx[i].a, w[i]));
             pb = vadd(pb, vmul(sx[i].b, w[i]));
@@ -498,12 +536,25 @@ __device__ Closest closest_simplex(const Sup* sx, int count) {
             best.p = q;
             best.a = pa;
             best.b = pb;
+#pragma unroll
             for (int k = 0; k < 4; ++k) best.w[k] = w[k];
             best.mask = mask;
             best.d2 = d;
         }
     }
     return best;
+}
+__device__ inline Closest closest_simplex(const Sup (&sx)[4], int count) {
+    switch (count) {
+    case 1:
+        return closest_simplex_n<1>(sx);
+    case 2:
+        return closest_simplex_n<2>(sx);
+    case 3:
+        return closest_simplex_n<3>(sx);
+    default:
+        return closest_simplex_n<4>(sx);
+    }
 }
 __device__ bool contact_from_gjk(const GPair& pr, const Closest& c, Pt& out) {
     const float dist = sqrtf(c.d2);
@@ -770,24 +821,45 @@ __device__ int gjk_lane(const GPair& pr, float cd, Pt& out, Sup (&sx)[4], int& c
             const Closest cur = closest_simplex(sx, count);
             return cur.mask && contact_from_gjk(pr, cur, out) ? 1 : 0;
         }
-        for (int i = 0; i < count; ++i)
-            if (s.ai == sx[i].ai && s.bi == sx[i].bi) {
-                if (proj > 0.f) {
-                    const Closest cur = closest_simplex(sx, count);
-                    return cur.mask && contact_from_gjk(pr, cur, out) && out.s <= cd ? 1 : 0;
-                }
-                return 0;
+        // (the simplex array is indexed with constants only: it stays in
+        // registers)
+        bool dup = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dup |= i < count && s.ai == sx[i].ai && s.bi == sx[i].bi;
+        if (dup) {
+            if (proj > 0.f) {
+                const Closest cur = closest_simplex(sx, count);
+                return cur.mask && contact_from_gjk(pr, cur, out) && out.s <= cd ? 1 : 0;
             }
+            return 0;
+        }
         if (count == 4) return 2;
-        sx[count++] = s;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k == count) sx[k] = s;
+        ++count;
         const Closest nx = closest_simplex(sx, count);
         if (!nx.mask) return 0;
         closest = nx.p;
         const float nd = sqrtf(nx.d2);
+        // keep the vertices with weight, in order
+        Sup kept[4];
         int wr = 0;
-        for (int r = 0; r < count; ++r)
-            if ((nx.mask & (1u << r)) && nx.w[r] > 1.0e-6f) sx[wr++] = sx[r];
-        if (wr == 0 && count > 0) wr = 1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (r < count && (nx.mask & (1u << r)) && nx.w[r] > 1.0e-6f) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (k == wr) kept[k] = sx[r];
+                ++wr;
+            }
+        if (wr == 0 && count > 0) {
+            wr = 1;
+            kept[0] = sx[0];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < wr) sx[k] = kept[k];
         count = wr;
         if (nd >= previous) return 0;
         previous = nd;
