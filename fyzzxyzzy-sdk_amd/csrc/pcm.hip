@@ -119,24 +119,50 @@ __device__ bool segment_slab(V p, V d, V high, V low, float& enter, float& exit)
 }
 
 constexpr int kMaxCand = 16;  // 4 inside corners + 4 projected + 8 edge cuts
+constexpr u32 kWitnessThreads = 128;  // largest workgroup that keeps witnesses in LDS
 
-struct Gen {
-    const GBox* a;
-    const GBox* b;
-    Pt* out;
+// The clip witnesses of one pair. Every witness the clipper emits has its
+// reference-face point at the incident point's (x, y, 0) in the clip frame
+// and separation -z, so the incident point alone is stored, in three LDS
+// planes indexed [plane][witness][thread] (conflict-free); the witness
+// (point on A, point on B, normal in B, separation) is rebuilt from it with
+// the arithmetic that generated it.
+struct Witnesses {
+    float* lds;  // this thread's entry of plane 0, witness 0
+    u32 stride;  // threads per workgroup
     int count;
-    __device__ void emit(const Frame& f, V ref, V inc, V normal, float sep, bool flipped) {
+    Frame f;
+    Q arot, brot;
+    V apos, bpos;
+    V nb;  // contact normal in B's frame (the same for every witness)
+    bool flipped;
+    __device__ float& at(int plane, int k) const { return lds[(plane * kMaxCand + k) * stride]; }
+    __device__ void emit(V inc) {
         if (count >= kMaxCand) return;  // unreachable: at most 16 witnesses
-        const V w0 = f_unproject(f, flipped ? inc : ref);
-        const V w1 = f_unproject(f, flipped ? ref : inc);
-        out[count++] = {grot_inv(a->rot, vsub(w0, a->pos)), grot_inv(b->rot, vsub(w1, b->pos)),
-                        grot_inv(b->rot, normal), sep};
+        at(0, count) = inc.x;
+        at(1, count) = inc.y;
+        at(2, count) = inc.z;
+        ++count;
+    }
+    __device__ V inc(int k) const { return {at(0, k), at(1, k), at(2, k)}; }
+    __device__ float s(int k) const { return -at(2, k); }
+    __device__ V n(int) const { return nb; }
+    __device__ V b(int k) const {
+        const V in = inc(k);
+        const V w1 = f_unproject(f, flipped ? V{in.x, in.y, 0} : in);
+        return grot_inv(brot, vsub(w1, bpos));
+    }
+    __device__ Pt pt(int k) const {
+        const V in = inc(k);
+        const V ref{in.x, in.y, 0};
+        const V w0 = f_unproject(f, flipped ? in : ref);
+        const V w1 = f_unproject(f, flipped ? ref : in);
+        return {grot_inv(arot, vsub(w0, apos)), grot_inv(brot, vsub(w1, bpos)), nb, -in.z};
     }
 };
 
 // GuPCMContactBoxBox.cpp:119-178
-__device__ void clip_contacts(Gen& g, const Frame& f, const V* poly, V inc_n, float fx, float fy, V normal,
-                              float cd, bool flipped) {
+__device__ void clip_contacts(Witnesses& g, const V* poly, V inc_n, float fx, float fy, float cd) {
     fx *= 1.0001f;
     fy *= 1.0001f;
     bool pen[4], in[4];
@@ -154,7 +180,7 @@ __device__ void clip_contacts(Gen& g, const Frame& f, const V* poly, V inc_n, fl
         if (-p.z < cd) {
             pen[i] = true;
             in[i] = fabsf(p.x) <= fx && fabsf(p.y) <= fy;
-            if (in[i]) g.emit(f, {p.x, p.y, 0}, p, normal, -p.z, flipped);
+            if (in[i]) g.emit(p);
         }
     }
     if (g.count == 4) return;
@@ -166,7 +192,7 @@ __device__ void clip_contacts(Gen& g, const Frame& f, const V* poly, V inc_n, fl
             const float nom = vdot(inc_n, vsub(poly[0], c));
             const float t = nom / inc_n.z;
             const float sep = -t;
-            if (cd > sep) g.emit(f, c, {c.x, c.y, t}, normal, sep, flipped);
+            if (cd > sep) g.emit({c.x, c.y, t});
         }
     }
     const V low{-fx, -fy, -cd - kFltEps};
@@ -180,19 +206,20 @@ __device__ void clip_contacts(Gen& g, const Frame& f, const V* poly, V inc_n, fl
         if (!segment_slab(poly[s], edge, high, low, t0, t1)) continue;
         if (!c0) {
             const V p = vadd(poly[s], vmul(edge, t0));
-            g.emit(f, {p.x, p.y, 0}, p, normal, -p.z, flipped);
+            g.emit(p);
         }
         if (!c1) {
             const V p = vadd(poly[s], vmul(edge, t1));
-            g.emit(f, {p.x, p.y, 0}, p, normal, -p.z, flipped);
+            g.emit(p);
         }
     }
 }
 
-// GuPCMContactBoxBox.cpp:301-436. Returns overlap; count = #witnesses.
-__device__ bool box_box_candidates(const Box3& ba, const Box3& bb, float cd, Pt* out, int& count) {
+// GuPCMContactBoxBox.cpp:301-436. Returns overlap; gen.count = #witnesses
+// (gen.lds and gen.stride set by the caller).
+__device__ bool box_box_candidates(const Box3& ba, const Box3& bb, float cd, Witnesses& gen) {
     const GBox a = make_gbox(ba), b = make_gbox(bb);
-    Gen gen{&a, &b, out, 0};
+    gen.count = 0;
     const V delta = vsub(b.pos, a.pos);
     float sgn[6], ov[6];
     constexpr float se = 1.0e-6f;
@@ -207,10 +234,7 @@ __device__ bool box_box_candidates(const Box3& ba, const Box3& bb, float cd, Pt*
                         (fabsf(vdot(n, inc.axis[1])) + se) * inc.half.y +
                         (fabsf(vdot(n, inc.axis[2])) + se) * inc.half.z;
         ov[i] = comp(ref.half, ax) + r - fabsf(sgn[i]) + cd;
-        if (ov[i] < 0.0f) {
-            count = 0;
-            return false;
-        }
+        if (ov[i] < 0.0f) return false;
     }
     for (unsigned i = 0; i < 3; ++i)
         for (unsigned j = 0; j < 3; ++j) {
@@ -220,21 +244,20 @@ __device__ bool box_box_candidates(const Box3& ba, const Box3& bb, float cd, Pt*
                 if (k != i) ra += (fabsf(vdot(ca, a.axis[k])) + se) * comp(a.half, k);
                 if (k != j) rb += (fabsf(vdot(ca, b.axis[k])) + se) * comp(b.half, k);
             }
-            if (fabsf(vdot(delta, ca)) > ra + rb + cd) {
-                count = 0;
-                return false;
-            }
+            if (fabsf(vdot(delta, ca)) > ra + rb + cd) return false;
         }
     unsigned feature = 0;
-    float mo = ov[0];
+    float mo = ov[0], msgn = sgn[0];
+#pragma unroll
     for (unsigned i = 1; i < 6; ++i)
         if (mo > ov[i]) {
             mo = ov[i];
+            msgn = sgn[i];
             feature = i;
         }
     const bool flipped = feature >= 3;
-    const GBox& inc = flipped ? a : b;
-    const bool np = sgn[feature] <= 0.0f;
+    const GBox inc = flipped ? a : b;
+    const bool np = msgn <= 0.0f;
     Frame fr;
     V normal;
     float fx, fy;
@@ -287,33 +310,48 @@ __device__ bool box_box_candidates(const Box3& ba, const Box3& bb, float cd, Pt*
     const float dy = fabsf(vdot(search, inc.axis[1]));
     const float dz = fabsf(vdot(search, inc.axis[2]));
     const unsigned ia = dx >= dy && dx >= dz ? 0 : (dy >= dz ? 1 : 2);
-    const float facing = vdot(search, inc.axis[ia]);
-    const float sext = facing > 0.0f ? -comp(inc.half, ia) : comp(inc.half, ia);
-    const V fc = vadd(inc.pos, vmul(inc.axis[ia], sext));
     const unsigned ua = ia == 0 ? 1 : 0;
     const unsigned va = ia == 2 ? 1 : 2;
-    const V u = vmul(inc.axis[ua], comp(inc.half, ua));
-    const V v = vmul(inc.axis[va], comp(inc.half, va));
+    const V axis_i = ia == 0 ? inc.axis[0] : (ia == 1 ? inc.axis[1] : inc.axis[2]);
+    const V axis_u = ua == 0 ? inc.axis[0] : inc.axis[1];
+    const V axis_v = va == 1 ? inc.axis[1] : inc.axis[2];
+    const float facing = vdot(search, axis_i);
+    const float sext = facing > 0.0f ? -comp(inc.half, ia) : comp(inc.half, ia);
+    const V fc = vadd(inc.pos, vmul(axis_i, sext));
+    const V u = vmul(axis_u, comp(inc.half, ua));
+    const V v = vmul(axis_v, comp(inc.half, va));
     V poly[4];
     poly[0] = f_project(fr, vadd(fc, vadd(u, v)));
     poly[1] = f_project(fr, vadd(fc, vsub(u, v)));
     poly[2] = f_project(fr, vsub(fc, vadd(u, v)));
     poly[3] = f_project(fr, vsub(fc, vsub(u, v)));
-    const V fnorm = f_project(fr, vadd(fr.p, facing > 0.0f ? vneg(inc.axis[ia]) : inc.axis[ia]));
-    clip_contacts(gen, fr, poly, fnorm, fx, fy, normal, cd, flipped);
-    count = gen.count;
+    const V fnorm = f_project(fr, vadd(fr.p, facing > 0.0f ? vneg(axis_i) : axis_i));
+    gen.f = fr;
+    gen.arot = a.rot;
+    gen.apos = a.pos;
+    gen.brot = b.rot;
+    gen.bpos = b.pos;
+    gen.nb = grot_inv(b.rot, normal);
+    gen.flipped = flipped;
+    clip_contacts(gen, poly, fnorm, fx, fy, cd);
     return true;
 }
 
+// A list of at most 16 witness numbers, 4 bits each (registers only).
+struct Avail {
+    u64 v;
+    __device__ int get(int i) const { return static_cast<int>((v >> (4 * i)) & 15u); }
+    __device__ void set(int i, int x) { v = (v & ~(15ull << (4 * i))) | (static_cast<u64>(x) << (4 * i)); }
+};
+
 // GuPCMContactBoxBox.cpp:185-299
-__device__ void batch_selection(const Pt* c, int n, float tol, int sel[4]) {
-    int avail[kMaxCand];
-    for (int i = 0; i < n; ++i) avail[i] = i;
+__device__ void batch_selection(const Witnesses& c, int n, float tol, int sel[4]) {
+    Avail avail{0xfedcba9876543210ull};
     int rem = n;
-    float deepest_d = c[0].s, shallow = deepest_d;
+    float deepest_d = c.s(0), shallow = deepest_d;
     int deepest = 0;
     for (int i = 1; i < rem; ++i) {
-        const float d = c[i].s;
+        const float d = c.s(i);
         shallow = smax(shallow, d);
         if (deepest_d > d) {
             deepest_d = d;
@@ -321,32 +359,37 @@ __device__ void batch_selection(const Pt* c, int n, float tol, int sel[4]) {
         }
     }
     sel[0] = deepest;
-    avail[deepest] = avail[--rem];
+    avail.set(deepest, avail.get(--rem));
+    const V b0 = c.b(sel[0]);
     int fp = 0;
-    float far2 = vdot(vsub(c[avail[0]].b, c[sel[0]].b), vsub(c[avail[0]].b, c[sel[0]].b));
+    float far2;
+    {
+        const V d = vsub(c.b(avail.get(0)), b0);
+        far2 = vdot(d, d);
+    }
     for (int i = 1; i < rem; ++i) {
-        const V d = vsub(c[avail[i]].b, c[sel[0]].b);
+        const V d = vsub(c.b(avail.get(i)), b0);
         const float v = vdot(d, d);
         if (v > far2) {
             far2 = v;
             fp = i;
         }
     }
-    sel[1] = avail[fp];
-    avail[fp] = avail[--rem];
-    const V base = vsub(c[sel[1]].b, c[sel[0]].b);
-    V tr = vcross(base, c[sel[0]].n);
+    sel[1] = avail.get(fp);
+    avail.set(fp, avail.get(--rem));
+    const V base = vsub(c.b(sel[1]), b0);
+    V tr = vcross(base, c.n(sel[0]));
     const float tr2 = vdot(tr, tr);
     if (tr2 > 0.0f) {
         const float inv = 1.0f / sqrtf(tr2);
         tr = {tr.x * inv, tr.y * inv, tr.z * inv};
     } else {
-        tr = c[sel[0]].n;
+        tr = c.n(sel[0]);
     }
     float greatest = -kFltMax, least = kFltMax;
     int gp = 0, lp = 0;
     for (int i = 0; i < rem; ++i) {
-        const float side = vdot(vsub(c[avail[i]].b, c[sel[0]].b), tr);
+        const float side = vdot(vsub(c.b(avail.get(i)), b0), tr);
         if (side > greatest) {
             greatest = side;
             gp = i;
@@ -356,29 +399,30 @@ __device__ void batch_selection(const Pt* c, int n, float tol, int sel[4]) {
             lp = i;
         }
     }
-    sel[2] = avail[gp];
-    avail[gp] = avail[--rem];
+    sel[2] = avail.get(gp);
+    avail.set(gp, avail.get(--rem));
     if (lp == rem) lp = gp;
     if (least * greatest > 0.0f) {
         greatest = -kFltMax;
         for (int i = 0; i < rem; ++i) {
-            const float side = vdot(vsub(c[avail[i]].b, c[sel[0]].b), tr);
+            const float side = vdot(vsub(c.b(avail.get(i)), b0), tr);
             if (side > greatest) {
                 greatest = side;
                 lp = i;
             }
         }
     }
-    sel[3] = avail[lp];
-    avail[lp] = avail[--rem];
+    sel[3] = avail.get(lp);
+    avail.set(lp, avail.get(--rem));
     const float th = tol * 0.02f;
     if (deepest_d < th && shallow > th) {
+#pragma unroll
         for (int i = 0; i < 4; ++i) {
-            float depth = c[sel[i]].s;
+            float depth = c.s(sel[i]);
             if (depth > th) {
                 int rep = rem;
                 for (int j = 0; j < rem; ++j) {
-                    const float cdp = c[avail[j]].s;
+                    const float cdp = c.s(avail.get(j));
                     if (depth > cdp && th > cdp) {
                         depth = cdp;
                         rep = j;
@@ -386,8 +430,8 @@ __device__ void batch_selection(const Pt* c, int n, float tol, int sel[4]) {
                 }
                 if (rep < rem) {
                     const int t = sel[i];
-                    sel[i] = avail[rep];
-                    avail[rep] = t;
+                    sel[i] = avail.get(rep);
+                    avail.set(rep, t);
                 }
             }
         }
@@ -400,6 +444,12 @@ struct Sup {
     V a, b, q;
     u32 ai, bi;
 };
+__device__ inline V vsel3(bool c, V x, V y) { return {c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z}; }
+// (field-wise select: keeps a simplex slot written at a runtime position in
+// registers)
+__device__ inline Sup ssel(bool c, const Sup& x, const Sup& y) {
+    return {vsel3(c, x.a, y.a), vsel3(c, x.b, y.b), vsel3(c, x.q, y.q), c ? x.ai : y.ai, c ? x.bi : y.bi};
+}
 struct GPair {
     V ha, hb, a_in_b;
     Q rel;
@@ -589,13 +639,15 @@ __device__ Facet make_facet_q(V a, V b, V c, int i, int j, int k) {
 __device__ inline Facet make_facet(const Sup* v, int i, int j, int k) {
     return make_facet_q(v[i].q, v[j].q, v[k].q, i, j, k);
 }
-__device__ bool init_epa(const GPair& pr, Sup* v, int& count, Facet* fa, int& fc) {
+// (all slots indexed with constants: the polytope stays in registers)
+__device__ bool init_epa(const GPair& pr, Sup (&v)[4], int& count, Facet (&fa)[4], int& fc) {
     if (count <= 0 || count > 4) return false;
     if (count == 1) {
         const Sup nx = support(pr, {1, 0, 0});
         const V d = vsub(nx.q, v[0].q);
         if (vdot(d, d) == 0.0f) return false;
-        v[count++] = nx;
+        v[1] = nx;
+        count = 2;
     }
     if (count == 2) {
         const V e = vsub(v[1].q, v[0].q);
@@ -609,14 +661,16 @@ __device__ bool init_epa(const GPair& pr, Sup* v, int& count, Facet* fa, int& fc
         const float sq = vdot(perp, perp);
         if (!(sq > 0.0f)) return false;
         perp = vmul(perp, 1.0f / sqrtf(sq));
-        v[count++] = support(pr, perp);
+        v[2] = support(pr, perp);
+        count = 3;
     }
     if (count == 3) {
         const Facet f0 = make_facet(v, 0, 1, 2);
         const Facet f1 = make_facet(v, 1, 0, 2);
         if (!f0.valid || !f1.valid) return false;
-        fa[fc++] = f0;
-        fa[fc++] = f1;
+        fa[0] = f0;
+        fa[1] = f1;
+        fc = 2;
         return true;
     }
     const V p0 = v[0].q;
@@ -627,12 +681,14 @@ __device__ bool init_epa(const GPair& pr, Sup* v, int& count, Facet* fa, int& fc
         v[2] = t;
     }
     const int faces[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+    bool valid = true;
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
-        const Facet x = make_facet(v, faces[f][0], faces[f][1], faces[f][2]);
-        if (!x.valid) return false;
-        fa[fc++] = x;
+        fa[f] = make_facet(v, faces[f][0], faces[f][1], faces[f][2]);
+        valid = valid && fa[f].valid;
     }
-    return true;
+    fc = 4;
+    return valid;
 }
 __device__ bool contact_from_facet(const GPair& pr, const Sup& p, const Sup& q, const Sup& r, const Facet& f,
                                    Pt& out) {
@@ -835,31 +891,28 @@ __device__ int gjk_lane(const GPair& pr, float cd, Pt& out, Sup (&sx)[4], int& c
         }
         if (count == 4) return 2;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (k == count) sx[k] = s;
+        for (int k = 0; k < 4; ++k) sx[k] = ssel(k == count, s, sx[k]);
         ++count;
         const Closest nx = closest_simplex(sx, count);
         if (!nx.mask) return 0;
         closest = nx.p;
         const float nd = sqrtf(nx.d2);
         // keep the vertices with weight, in order
-        Sup kept[4];
+        Sup kept[4] = {sx[0], sx[1], sx[2], sx[3]};
         int wr = 0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            if (r < count && (nx.mask & (1u << r)) && nx.w[r] > 1.0e-6f) {
+        for (int r = 0; r < 4; ++r) {
+            const bool keep = r < count && (nx.mask & (1u << r)) && nx.w[r] > 1.0e-6f;
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (k == wr) kept[k] = sx[r];
-                ++wr;
-            }
+            for (int k = 0; k <= r; ++k) kept[k] = ssel(keep && k == wr, sx[r], kept[k]);
+            wr += keep ? 1 : 0;
+        }
         if (wr == 0 && count > 0) {
             wr = 1;
             kept[0] = sx[0];
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (k < wr) sx[k] = kept[k];
+        for (int k = 0; k < 4; ++k) sx[k] = ssel(k < wr, kept[k], sx[k]);
         count = wr;
         if (nd >= previous) return 0;
         previous = nd;
@@ -902,12 +955,34 @@ __device__ inline u32 c_count(const Cache& c) { return c.flags & 0xffu; }
 __device__ inline bool c_init(const Cache& c) { return (c.flags >> 8) & 1u; }
 __device__ inline void c_set(Cache& c, u32 count, bool init) { c.flags = (count & 0xffu) | (init ? 0x100u : 0u); }
 
-// scene.cpp:3432-3465
+// The manifold's four points are indexed with constants only (loops
+// unrolled over the four slots, runtime slot numbers resolved by selects),
+// so a cache copy stays in registers.
+__device__ inline V vsel(bool c, V x, V y) { return {c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z}; }
+__device__ inline Pt psel(bool c, const Pt& x, const Pt& y) {
+    return {vsel(c, x.a, y.a), vsel(c, x.b, y.b), vsel(c, x.n, y.n), c ? x.s : y.s};
+}
+// (selects on the scalar fields: a whole-struct copy from a selected slot
+// would become a pointer select, which keeps the slots in memory)
+__device__ inline Pt pick4(const Pt (&p)[4], u32 i) {
+    return psel(i == 3u, p[3], psel(i == 2u, p[2], psel(i == 1u, p[1], p[0])));
+}
+// (every slot written through a select: conditional stores to the slots
+// would be merged into one store through a selected pointer)
+__device__ inline void put4(Pt (&p)[4], u32 i, const Pt& v) {
+#pragma unroll
+    for (u32 k = 0; k < 4; ++k) p[k] = psel(i == k, v, p[k]);
+}
+
+// scene.cpp:3432-3465 (newest slot first; a dropped point takes the last
+// slot's point, which has been refreshed already)
 __device__ void refresh(Cache& c, V pos_ab, Q rot_ab, float brk) {
     const float b2 = brk * brk;
     u32 count = c_count(c);
-    for (u32 i = count; i > 0; --i) {
-        Pt& p = c.pts[i - 1];
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        if (static_cast<u32>(i) >= count) continue;
+        Pt& p = c.pts[i];
         const V r = srot(rot_ab, p.a);
         const V ainb{pos_ab.x + r.x, pos_ab.y + r.y, pos_ab.z + r.z};
         const V v{ainb.x - p.b.x, ainb.y - p.b.y, ainb.z - p.b.z};
@@ -915,7 +990,7 @@ __device__ void refresh(Cache& c, V pos_ab, Q rot_ab, float brk) {
         const V pd{v.x - p.n.x * d, v.y - p.n.y * d, v.z - p.n.z * d};
         const float t2 = pd.x * pd.x + pd.y * pd.y + pd.z * pd.z;
         if (t2 > b2)
-            c.pts[i - 1] = c.pts[--count];
+            c.pts[i] = pick4(c.pts, --count);
         else
             p.s = d;
     }
@@ -950,81 +1025,92 @@ __device__ float tri_sq(V p, V a, V b, V c) {
     const float pr = vdot(n, ap);
     return pr * pr / den;
 }
-// scene.cpp:3317-3431
+// scene.cpp:3317-3431 (the five points indexed with constants; runtime
+// picks resolved by selects)
+__device__ inline Pt pick5(const Pt (&p)[5], int i) {
+    return psel(i == 4, p[4], psel(i == 3, p[3], psel(i == 2, p[2], psel(i == 1, p[1], p[0]))));
+}
 __device__ void reduce_five(Cache& cache, const Pt& nc) {
-    Pt cd[5];
-    for (int i = 0; i < 4; ++i) cd[i] = cache.pts[i];
-    cd[4] = nc;
-    bool chosen[5] = {false, false, false, false, false};
-    int sel[4] = {0, 0, 0, 0};
+    const Pt cd[5] = {cache.pts[0], cache.pts[1], cache.pts[2], cache.pts[3], nc};
+    unsigned chosen = 0;
     int deepest = 4;
+    float ds = cd[4].s;
+#pragma unroll
     for (int i = 0; i < 4; ++i)
-        if (cd[i].s < cd[deepest].s) deepest = i;
-    sel[0] = deepest;
-    chosen[deepest] = true;
+        if (cd[i].s < ds) {
+            deepest = i;
+            ds = cd[i].s;
+        }
+    chosen |= 1u << deepest;
+    const V bd = pick5(cd, deepest).b;
     int far = 0;
-    V d0 = vsub(cd[0].b, cd[deepest].b);
+    V d0 = vsub(cd[0].b, bd);
     float largest = vdot(d0, d0);
+#pragma unroll
     for (int i = 1; i < 5; ++i)
-        if (!chosen[i]) {
-            const V d = vsub(cd[i].b, cd[deepest].b);
+        if (!(chosen & (1u << i))) {
+            const V d = vsub(cd[i].b, bd);
             const float v = vdot(d, d);
             if (v > largest) {
                 largest = v;
                 far = i;
             }
         }
-    sel[1] = far;
-    chosen[far] = true;
+    chosen |= 1u << far;
+    const V bf = pick5(cd, far).b;
     largest = -kFltMax;
     int third = 0;
+#pragma unroll
     for (int i = 0; i < 5; ++i)
-        if (!chosen[i]) {
-            const float v = seg_sq(cd[i].b, cd[deepest].b, cd[far].b);
+        if (!(chosen & (1u << i))) {
+            const float v = seg_sq(cd[i].b, bd, bf);
             if (v > largest) {
                 largest = v;
                 third = i;
             }
         }
-    sel[2] = third;
-    chosen[third] = true;
+    chosen |= 1u << third;
+    const V bt = pick5(cd, third).b;
     largest = -kFltMax;
     int fourth = third;
+#pragma unroll
     for (int i = 0; i < 5; ++i)
-        if (!chosen[i]) {
-            const float v = tri_sq(cd[i].b, cd[sel[0]].b, cd[sel[1]].b, cd[sel[2]].b);
+        if (!(chosen & (1u << i))) {
+            const float v = tri_sq(cd[i].b, bd, bf, bt);
             if (v > largest) {
                 largest = v;
                 fourth = i;
             }
         }
-    if (chosen[fourth]) {
-        for (int i = 0; i < 3; ++i) cache.pts[i] = cd[sel[i]];
+    if (chosen & (1u << fourth)) {
+        cache.pts[0] = pick5(cd, deepest);
+        cache.pts[1] = pick5(cd, far);
+        cache.pts[2] = pick5(cd, third);
         c_set(cache, 3, c_init(cache));
         return;
     }
-    sel[3] = fourth;
-    chosen[fourth] = true;
-    for (int i = 0; i < 4; ++i) cache.pts[i] = cd[sel[i]];
+    chosen |= 1u << fourth;
+    cache.pts[0] = pick5(cd, deepest);
+    cache.pts[1] = pick5(cd, far);
+    cache.pts[2] = pick5(cd, third);
+    cache.pts[3] = pick5(cd, fourth);
     c_set(cache, 4, c_init(cache));
-    int uns = -1;
-    for (int i = 0; i < 5; ++i)
-        if (!chosen[i]) {
-            uns = i;
-            break;
-        }
-    if (uns < 0) return;
+    const unsigned rest = ~chosen & 0x1fu;
+    if (!rest) return;
+    const int uns = __ffs(rest) - 1;
+    const Pt pu = pick5(cd, uns);
     float closest = kFltMax;
     int rep = 0;
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const V d = vsub(cache.pts[i].a, cd[uns].a);
+        const V d = vsub(cache.pts[i].a, pu.a);
         const float dd = vdot(d, d);
         if (dd < closest) {
             closest = dd;
             rep = i;
         }
     }
-    if (cd[uns].s < cache.pts[rep].s) cache.pts[rep] = cd[uns];
+    if (pu.s < pick4(cache.pts, static_cast<u32>(rep)).s) put4(cache.pts, static_cast<u32>(rep), pu);
 }
 
 __device__ inline float pcm_margin(V h) { return smin(0.15f * smin(smin(h.x, h.y), h.z), 0.15f); }
@@ -1048,11 +1134,13 @@ __device__ u32 emit_box_box(const Cache& cache, const Box3& second, float cd, bo
     else if (direct)
         nb = cache.pts[0].n;
     else
-        for (u32 i = 0; i < count; ++i) {
-            nb.x += cache.pts[i].n.x;
-            nb.y += cache.pts[i].n.y;
-            nb.z += cache.pts[i].n.z;
-        }
+#pragma unroll
+        for (u32 i = 0; i < 4; ++i)
+            if (i < count) {
+                nb.x += cache.pts[i].n.x;
+                nb.y += cache.pts[i].n.y;
+                nb.z += cache.pts[i].n.z;
+            }
     const float n2 = nb.x * nb.x + nb.y * nb.y + nb.z * nb.z;
     if (!(n2 > 0.0f) || !isfinite(n2)) {
         flags |= kInvalid;
@@ -1062,7 +1150,9 @@ __device__ u32 emit_box_box(const Cache& cache, const Box3& second, float cd, bo
     const V nn{nb.x * il, nb.y * il, nb.z * il};
     const V normal = srot(second.rot, nn);
     u32 e = 0;
-    for (u32 i = 0; i < count; ++i) {
+#pragma unroll
+    for (u32 i = 0; i < 4; ++i) {
+        if (i >= count) continue;
         const Pt& p = cache.pts[i];
         if (!direct && p.s > cd) continue;
         const V r = srot(second.rot, p.b);
@@ -1102,7 +1192,8 @@ struct PairArgs {
 };
 
 // scene.cpp:3466-3654 up to the zero-witness continuation.
-__global__ __launch_bounds__(128) void k_pcm_box_box(PairArgs A) {
+__global__ __launch_bounds__(kWitnessThreads) void k_pcm_box_box(PairArgs A) {
+    __shared__ float wit_lds[3 * kMaxCand * kWitnessThreads];
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
     const u32 n = A.d_n ? min(*A.d_n, A.n_pairs) : A.n_pairs;
     if (t >= n) return;
@@ -1123,8 +1214,9 @@ __global__ __launch_bounds__(128) void k_pcm_box_box(PairArgs A) {
     refresh(cache, rel, rot, mm * 0.8f);
     const u32 cnt = c_count(cache);
     const bool lost = cnt != initial;
-    const float th[5] = {0.5f, 0.125f, 0.25f, 0.375f, 0.375f};
-    const float qth[5] = {0.9998f, 0.9999f, 0.9999f, 0.9999f, 0.9999f};
+    // position / orientation thresholds by point count (0..4)
+    const float thc = cnt == 0u ? 0.5f : cnt == 1u ? 0.125f : cnt == 2u ? 0.25f : 0.375f;
+    const float qthc = cnt == 0u ? 0.9998f : 0.9999f;
     const bool init = c_init(cache);
     const float td = init ? smax(smax(fabsf(rel.x - cache.rel_pos.x), fabsf(rel.y - cache.rel_pos.y)),
                                  fabsf(rel.z - cache.rel_pos.z))
@@ -1133,8 +1225,8 @@ __global__ __launch_bounds__(128) void k_pcm_box_box(PairArgs A) {
                      f.rot.w * cache.orient_a.w;
     const float qb = s.rot.x * cache.orient_b.x + s.rot.y * cache.orient_b.y + s.rot.z * cache.orient_b.z +
                      s.rot.w * cache.orient_b.w;
-    const float tp = mm * th[cnt];
-    bool regen = !init || lost || td > tp || qa < qth[cnt] || qb < qth[cnt];
+    const float tp = mm * thc;
+    bool regen = !init || lost || td > tp || qa < qthc || qb < qthc;
     if (!regen) {
         auto travel = [](float d, V e) {
             const float ang = d < 1.0f ? acosf(smax(-1.0f, d)) : 0.0f;
@@ -1150,9 +1242,11 @@ __global__ __launch_bounds__(128) void k_pcm_box_box(PairArgs A) {
         cache.orient_a = f.rot;
         cache.orient_b = s.rot;
         c_set(cache, c_count(cache), true);
-        Pt cands[kMaxCand];
-        int count = 0;
-        const bool overlapping = box_box_candidates(f, s, A.cd, cands, count);
+        Witnesses cands;
+        cands.lds = wit_lds + threadIdx.x;
+        cands.stride = blockDim.x;
+        const bool overlapping = box_box_candidates(f, s, A.cd, cands);
+        const int count = overlapping ? cands.count : 0;
         if (!overlapping) {
             c_set(cache, 0, true);
         } else if (count == 0) {
@@ -1165,12 +1259,15 @@ __global__ __launch_bounds__(128) void k_pcm_box_box(PairArgs A) {
             A.gjk_list[slot] = t;
             return;
         } else if (count <= 4) {
-            for (int i = 0; i < count; ++i) cache.pts[i] = cands[i];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (i < count) cache.pts[i] = cands.pt(i);
             c_set(cache, static_cast<u32>(count), true);
         } else {
             int sel[4];
             batch_selection(cands, count, 1.0f, sel);
-            for (int i = 0; i < 4; ++i) cache.pts[i] = cands[sel[i]];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cache.pts[i] = cands.pt(sel[i]);
             c_set(cache, 4, true);
         }
         direct = count > 0;
@@ -1208,18 +1305,17 @@ __global__ __launch_bounds__(64) void k_pcm_gjk(PairArgs A) {
         const float rsq = rb * rb;
         bool replaced = false;
         const u32 cnt = c_count(cache);
-        for (u32 i = 0; i < cnt; ++i) {
+#pragma unroll
+        for (u32 i = 0; i < 4; ++i) {
             const Pt& o = cache.pts[i];
             const V da = vsub(o.a, wit.a), db = vsub(o.b, wit.b);
-            if (smin(vdot(da, da), vdot(db, db)) < rsq) {
-                cache.pts[i] = wit;
-                replaced = true;
-                break;
-            }
+            const bool hit = !replaced && i < cnt && smin(vdot(da, da), vdot(db, db)) < rsq;
+            cache.pts[i] = psel(hit, wit, cache.pts[i]);
+            replaced |= hit;
         }
         if (!replaced) {
             if (cnt < 4) {
-                cache.pts[cnt] = wit;
+                put4(cache.pts, cnt, wit);
                 c_set(cache, cnt + 1, true);
             } else {
                 reduce_five(cache, wit);
@@ -1273,40 +1369,64 @@ __global__ __launch_bounds__(128) void k_pcm_plane_box(PlaneArgs A) {
     if (regen) {
         cache.rel_pos = rel;
         cache.rel_rot = q;
-        Pt c[8];
-        int n = 0;
+        // The corners below the contact distance, in corner order, are the
+        // candidates; every selection loop below visits them in that order
+        // (unrolled over the eight corners, so nothing leaves registers).
+        V loc[8], ip[8];
+        unsigned below = 0;
+#pragma unroll
         for (unsigned corner = 0; corner < 8u; ++corner) {
-            const V local{(corner & 4u) ? -h.x : h.x, (corner & 2u) ? -h.y : h.y, (corner & 1u) ? -h.z : h.z};
-            const V r = srot(q, local);
-            const V ip{rel.x + r.x, rel.y + r.y, rel.z + r.z};
-            if (ip.y >= A.cd) continue;
-            c[n++] = {local, {ip.x, 0.0f, ip.z}, {0.0f, 1.0f, 0.0f}, ip.y};
+            loc[corner] = {(corner & 4u) ? -h.x : h.x, (corner & 2u) ? -h.y : h.y, (corner & 1u) ? -h.z : h.z};
+            const V r = srot(q, loc[corner]);
+            ip[corner] = {rel.x + r.x, rel.y + r.y, rel.z + r.z};
+            if (!(ip[corner].y >= A.cd)) below |= 1u << corner;
         }
+        auto cand = [&](unsigned corner) -> Pt {
+            return {loc[corner], {ip[corner].x, 0.0f, ip[corner].z}, {0.0f, 1.0f, 0.0f}, ip[corner].y};
+        };
+        auto cand_of = [&](unsigned id) -> Pt {
+            Pt r = cand(0);
+#pragma unroll
+            for (unsigned corner = 1; corner < 8u; ++corner) r = psel(id == corner, cand(corner), r);
+            return r;
+        };
+        const int n = __popc(below);
         if (n <= 4) {
-            for (int i = 0; i < n; ++i) cache.pts[i] = c[i];
+            u32 k = 0;
+#pragma unroll
+            for (unsigned corner = 0; corner < 8u; ++corner)
+                if (below & (1u << corner)) put4(cache.pts, k++, cand(corner));
             c_set(cache, static_cast<u32>(n), true);
         } else {
-            bool chosen[8] = {false, false, false, false, false, false, false, false};
-            int idx[4];
-            int first = 0;
-            for (int i = 1; i < n; ++i)
-                if (c[i].s < c[first].s) first = i;
-            idx[0] = first;
-            chosen[first] = true;
-            int far = 0;
-            V d0 = vsub(c[0].b, c[first].b);
-            float maxd = vdot(d0, d0);
-            for (int i = 1; i < n; ++i) {
-                const V d = vsub(c[i].b, c[first].b);
-                const float dd = vdot(d, d);
-                if (dd > maxd) {
-                    maxd = dd;
-                    far = i;
+            // deepest (first on ties)
+            unsigned first = 8;
+            float sf = 0.0f;
+            V bf{0, 0, 0};
+#pragma unroll
+            for (unsigned corner = 0; corner < 8u; ++corner)
+                if ((below & (1u << corner)) && (first == 8 || ip[corner].y < sf)) {
+                    first = corner;
+                    sf = ip[corner].y;
+                    bf = {ip[corner].x, 0.0f, ip[corner].z};
                 }
-            }
-            idx[1] = far;
-            chosen[far] = true;
-            const V segv = vsub(c[far].b, c[first].b);
+            // farthest from it (the first candidate counts, even if it is the deepest)
+            unsigned far = 8;
+            float maxd = 0.0f;
+            V bfar{0, 0, 0};
+#pragma unroll
+            for (unsigned corner = 0; corner < 8u; ++corner)
+                if (below & (1u << corner)) {
+                    const V b{ip[corner].x, 0.0f, ip[corner].z};
+                    const V d = vsub(b, bf);
+                    const float dd = vdot(d, d);
+                    if (far == 8 || dd > maxd) {
+                        maxd = dd;
+                        far = corner;
+                        bfar = b;
+                    }
+                }
+            unsigned chosen = (1u << first) | (1u << far);
+            const V segv = vsub(bfar, bf);
             V side{-segv.z, 0.0f, segv.x};
             const float ns = vdot(side, side);
             if (ns > 0.0f) {
@@ -1315,59 +1435,82 @@ __global__ __launch_bounds__(128) void k_pcm_plane_box(PlaneArgs A) {
             } else {
                 side = {0.0f, 1.0f, 0.0f};
             }
+            // the lowest candidate stands in when no side value compares
+            const unsigned lowest = static_cast<unsigned>(__ffs(below) - 1);
             float largest = -kFltMax, smallest = kFltMax;
-            int third = 0, fourth = 0;
-            for (int i = 0; i < n; ++i) {
-                if (chosen[i]) continue;
-                const float d = vdot(vsub(c[i].b, c[first].b), side);
+            unsigned third = lowest, fourth = lowest;
+#pragma unroll
+            for (unsigned corner = 0; corner < 8u; ++corner) {
+                if (!(below & (1u << corner)) || (chosen & (1u << corner))) continue;
+                const V b{ip[corner].x, 0.0f, ip[corner].z};
+                const float d = vdot(vsub(b, bf), side);
                 if (d > largest) {
                     largest = d;
-                    third = i;
+                    third = corner;
                 }
                 if (d < smallest) {
                     smallest = d;
-                    fourth = i;
+                    fourth = corner;
                 }
             }
-            idx[2] = third;
-            chosen[third] = true;
+            chosen |= 1u << third;
             if (smallest * largest > 0.0f) {
                 largest = -kFltMax;
-                for (int i = 0; i < n; ++i) {
-                    if (chosen[i]) continue;
-                    const float d = vdot(vsub(c[i].b, c[first].b), side);
+#pragma unroll
+                for (unsigned corner = 0; corner < 8u; ++corner) {
+                    if (!(below & (1u << corner)) || (chosen & (1u << corner))) continue;
+                    const V b{ip[corner].x, 0.0f, ip[corner].z};
+                    const float d = vdot(vsub(b, bf), side);
                     if (d > largest) {
                         largest = d;
-                        fourth = i;
+                        fourth = corner;
                     }
                 }
             }
-            idx[3] = fourth;
-            chosen[fourth] = true;
+            chosen |= 1u << fourth;
+            unsigned idx[4] = {first, far, third, fourth};
             V centers[4];
-            for (int k = 0; k < 4; ++k) centers[k] = c[idx[k]].b;
-            for (int i = 0; i < n; ++i) {
-                if (chosen[i]) continue;
+            float sidx[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const Pt ck = cand_of(idx[k]);
+                centers[k] = ck.b;
+                sidx[k] = ck.s;
+            }
+            // every other candidate replaces the nearest selected one if deeper
+#pragma unroll
+            for (unsigned corner = 0; corner < 8u; ++corner) {
+                if (!(below & (1u << corner)) || (chosen & (1u << corner))) continue;
+                const V b{ip[corner].x, 0.0f, ip[corner].z};
                 float nearest = kFltMax;
                 int slot = 0;
+#pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const V d = vsub(c[i].b, centers[j]);
+                    const V d = vsub(b, centers[j]);
                     const float dd = vdot(d, d);
                     if (dd < nearest) {
                         nearest = dd;
                         slot = j;
                     }
                 }
-                if (c[i].s < c[idx[slot]].s) idx[slot] = i;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j == slot && ip[corner].y < sidx[j]) {
+                        idx[j] = corner;
+                        sidx[j] = ip[corner].y;
+                    }
             }
-            for (int k = 0; k < 4; ++k) cache.pts[k] = c[idx[k]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) cache.pts[k] = cand_of(idx[k]);
             c_set(cache, 4, true);
         }
     }
     u32 e = 0;
     float* out = A.contacts + 28ull * t;
     const u32 count = c_count(cache);
-    for (u32 k = 0; k < count; ++k) {
+#pragma unroll
+    for (u32 k = 0; k < 4; ++k) {
+        if (k >= count) continue;
         const Pt& pt = cache.pts[k];
         if (pt.s > A.cd) continue;
         float* o = out + 7 * e;
@@ -1389,31 +1532,36 @@ __global__ __launch_bounds__(128) void k_pcm_plane_box(PlaneArgs A) {
 __global__ void k_zero1(u32* p) { *p = 0u; }
 
 // Stateless parity entries (one thread per pose pair).
-__global__ void k_candidates(u32 n, const float* a, const float* b, float cd, float* out, u32* counts,
-                             u32* overlapping, u32* selection) {
+__global__ __launch_bounds__(kWitnessThreads) void k_candidates(u32 n, const float* a, const float* b, float cd,
+                                                                float* out, u32* counts, u32* overlapping,
+                                                                u32* selection) {
+    __shared__ float wit_lds[3 * kMaxCand * kWitnessThreads];
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const float* pa = a + 10ull * t;
     const float* pb = b + 10ull * t;
     const Box3 ba{{pa[0], pa[1], pa[2]}, {pa[3], pa[4], pa[5]}, {pa[6], pa[7], pa[8], pa[9]}};
     const Box3 bb{{pb[0], pb[1], pb[2]}, {pb[3], pb[4], pb[5]}, {pb[6], pb[7], pb[8], pb[9]}};
-    Pt c[kMaxCand];
-    int cnt = 0;
-    const bool ov = box_box_candidates(ba, bb, cd, c, cnt);
+    Witnesses c;
+    c.lds = wit_lds + threadIdx.x;
+    c.stride = blockDim.x;
+    const bool ov = box_box_candidates(ba, bb, cd, c);
+    const int cnt = ov ? c.count : 0;
     overlapping[t] = ov ? 1u : 0u;
     counts[t] = static_cast<u32>(cnt);
     float* o = out + 160ull * t;
     for (int i = 0; i < cnt; ++i) {
-        o[10 * i + 0] = c[i].a.x;
-        o[10 * i + 1] = c[i].a.y;
-        o[10 * i + 2] = c[i].a.z;
-        o[10 * i + 3] = c[i].b.x;
-        o[10 * i + 4] = c[i].b.y;
-        o[10 * i + 5] = c[i].b.z;
-        o[10 * i + 6] = c[i].n.x;
-        o[10 * i + 7] = c[i].n.y;
-        o[10 * i + 8] = c[i].n.z;
-        o[10 * i + 9] = c[i].s;
+        const Pt w = c.pt(i);
+        o[10 * i + 0] = w.a.x;
+        o[10 * i + 1] = w.a.y;
+        o[10 * i + 2] = w.a.z;
+        o[10 * i + 3] = w.b.x;
+        o[10 * i + 4] = w.b.y;
+        o[10 * i + 5] = w.b.z;
+        o[10 * i + 6] = w.n.x;
+        o[10 * i + 7] = w.n.y;
+        o[10 * i + 8] = w.n.z;
+        o[10 * i + 9] = w.s;
     }
     int sel[4] = {255, 255, 255, 255};
     if (cnt > 4) batch_selection(c, cnt, 1.0f, sel);
