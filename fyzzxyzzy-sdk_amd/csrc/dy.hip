@@ -564,6 +564,39 @@ __global__ __launch_bounds__(kBlock) void k_part_static(Ws W) {
     if (b >= W.nb) return;
     const u32 lo = W.off[b], hi = W.off[b + 1];
     u32 k = W.max_dyn[b];
+    // Bodies with at most kCache managers (all of a pile's) keep the
+    // partitions in registers: one load per manager instead of one per
+    // pair of managers. A static manager has this body as its only dynamic
+    // body, so this thread alone writes its partition.
+    constexpr u32 kCache = 16;
+    if (hi - lo <= kCache) {
+        u32 pc[kCache], ec[kCache];
+#pragma unroll
+        for (u32 q = 0; q < kCache; ++q) {
+            pc[q] = kNone;  // absent entries never count as lower
+            ec[q] = 0;
+            if (lo + q < hi) {
+                const u32 e = W.inc[lo + q];
+                const u32 m = e >> 1;
+                ec[q] = e;
+                if (!dyn_dyn(W.mg[m])) {
+                    W.part[m] = k;
+                    pc[q] = k++;
+                } else {
+                    pc[q] = W.part[m];
+                }
+            }
+        }
+#pragma unroll
+        for (u32 q = 0; q < kCache; ++q) {
+            if (lo + q >= hi) continue;
+            u32 r = 0;
+#pragma unroll
+            for (u32 t = 0; t < kCache; ++t) r += pc[t] < pc[q] ? 1u : 0u;
+            W.erank[ec[q]] = r;
+        }
+        return;
+    }
     for (u32 i = lo; i < hi; ++i) {
         const u32 m = W.inc[i] >> 1;
         if (!dyn_dyn(W.mg[m])) W.part[m] = k++;

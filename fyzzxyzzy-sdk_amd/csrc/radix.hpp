@@ -94,7 +94,8 @@ __device__ inline u64 match_digit(u32 d, bool valid) {
     return peers;
 }
 
-// counts[(seg*256 + digit)*tiles + tile] = #items of the tile with digit.
+// counts[(seg*tiles + tile)*256 + digit] = #items of the tile with digit
+// (tile-major: each tile writes and later reads one contiguous 1 KB row).
 template <class K, Source S>
 __global__ __launch_bounds__(kBlock) void k_upsweep(Batch b, const K* keys, const u32* vals,
                                                     int shift, u32 tiles, u32* counts) {
@@ -106,17 +107,22 @@ __global__ __launch_bounds__(kBlock) void k_upsweep(Batch b, const K* keys, cons
         hist[tid] = 0;
         __syncthreads();
         const u32 base = tile * kTile;
+        // all of the tile's loads in flight before any is used
+        K k[kItems];
+#pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const u32 i = base + j * kBlock + tid;
-            const bool valid = i < n;
-            K k = 0;
-            if (valid) k = load_key<K, S>(b, keys, seg, i);
-            const u32 d = digit_of(k, shift);
+            k[j] = i < n ? load_key<K, S>(b, keys, seg, i) : K(0);
+        }
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const bool valid = base + j * kBlock + tid < n;
+            const u32 d = digit_of(k[j], shift);
             const u64 peers = match_digit(d, valid);
             if (valid && lane_rank(peers) == 0) atomicAdd(&hist[d], static_cast<u32>(__popcll(peers)));
         }
         __syncthreads();
-        counts[(static_cast<std::size_t>(seg) * kBins + tid) * tiles + tile] = hist[tid];
+        counts[(static_cast<std::size_t>(seg) * tiles + tile) * kBins + tid] = hist[tid];
         __syncthreads();
     }
 }
@@ -127,14 +133,14 @@ __global__ __launch_bounds__(kBlock) void k_tile_scan(Batch b, u32 tiles, u32* c
     const u32 seg = blockIdx.y, digit = blockIdx.x;
     if (b.d_gate && *b.d_gate == 0) return;
     const u32 need = tiles_needed(seg_count(b, seg));
-    u32* row = counts + (static_cast<std::size_t>(seg) * kBins + digit) * tiles;
+    u32* col = counts + static_cast<std::size_t>(seg) * tiles * kBins + digit;  // stride kBins
     u32 carry = 0;
     for (u32 t0 = 0; t0 < need; t0 += kBlock) {
         const u32 t = t0 + threadIdx.x;
-        const u32 v = t < need ? row[t] : 0u;
+        const u32 v = t < need ? col[static_cast<std::size_t>(t) * kBins] : 0u;
         u32 total;
         const u32 ex = block_exclusive_scan<kBlock>(v, scratch, &total);
-        if (t < need) row[t] = carry + ex;
+        if (t < need) col[static_cast<std::size_t>(t) * kBins] = carry + ex;
         carry += total;
     }
     if (threadIdx.x == 0) totals[seg * kBins + digit] = carry;
@@ -165,13 +171,17 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Batch b, const K* keys, cons
     u32 v[kItems];
     u32 r[kItems];
     const u32 strip = base + wave * (kItems * kWave);
+    // all of the tile's loads in flight before any is used
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const u32 i = strip + j * kWave + lane;
-        const bool valid = i < n;
         k[j] = 0;
         v[j] = 0;
-        if (valid) load_item<K, S>(b, keys, vals, seg, i, k[j], v[j]);
+        if (i < n) load_item<K, S>(b, keys, vals, seg, i, k[j], v[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const bool valid = strip + j * kWave + lane < n;
         const u32 d = digit_of(k[j], shift);
         const u64 peers = match_digit(d, valid);
         r[j] = 0xffffffffu;
@@ -197,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Batch b, const K* keys, cons
         s_lbase[d] = lb;
         const u32 gtot = totals[seg * kBins + d];
         const u32 gb = block_exclusive_scan<kBlock>(gtot, scratch, &total);
-        s_gbase[d] = gb + counts[(static_cast<std::size_t>(seg) * kBins + d) * tiles + tile];
+        s_gbase[d] = gb + counts[(static_cast<std::size_t>(seg) * tiles + tile) * kBins + d];
     }
     __syncthreads();
 #pragma unroll

@@ -53,28 +53,46 @@ static __global__ __launch_bounds__(1024) void k_scan_tiles(u32 cap, const u32* 
     if (threadIdx.x == 0) *d_out = carry;
 }
 
+// The ordered write visits items in the count pass's layout (item j of
+// thread t is index base + j*256 + t), so consecutive lanes handle
+// consecutive indices and write consecutive outputs: the reads behind the
+// predicate and the writer's records stay coalesced. An item's output is
+// the tile's base plus the selected items before it in index order
+// (rows j' < j, then earlier waves of row j, then earlier lanes).
 template <class P, class W>
 __global__ __launch_bounds__(kBlock) void k_write(u32 cap, const u32* d_n, P pred, W write,
                                                   const u32* tile_sums, const u32* gate) {
-    __shared__ u32 scratch[kBlock / kWave + 1];
+    constexpr int kWaves = kBlock / kWave;
+    __shared__ u32 row_wave[kItems * kWaves];  // selected per (row, wave), then its exclusive prefix
     if (gate && *gate == 0) return;
     const u32 n = count_of(cap, d_n);
     const u32 need = (n + kTile - 1) / kTile;
+    const u32 tid = threadIdx.x, wave = tid / kWave;
     for (u32 tile = blockIdx.x; tile < need; tile += gridDim.x) {
         const u32 base = tile * kTile;
-        // Thread t owns the contiguous run [base + t*16, base + t*16 + 16).
-        const u32 first = base + threadIdx.x * kItems;
-        u32 c = 0;
+        u32 sel = 0;  // bit j: item j selected
+        u32 rank[kItems];
+#pragma unroll
         for (int j = 0; j < kItems; ++j) {
-            const u32 i = first + j;
-            if (i < n && pred(i)) ++c;
+            const u32 i = base + j * kBlock + tid;
+            const bool f = i < n && pred(i);
+            const u64 m = __ballot(f);
+            rank[j] = lane_rank(m);
+            sel |= f ? (1u << j) : 0u;
+            if (lane_id() == 0) row_wave[j * kWaves + wave] = static_cast<u32>(__popcll(m));
         }
-        u32 total;
-        u32 out = tile_sums[tile] + block_exclusive_scan<kBlock>(c, scratch, &total);
-        for (int j = 0; j < kItems; ++j) {
-            const u32 i = first + j;
-            if (i < n && pred(i)) write(i, out++);
+        __syncthreads();
+        static_assert(kItems * kWaves == kWave, "one wave scans the (row, wave) counts");
+        if (tid < kWave) {
+            const u32 v = row_wave[tid];
+            row_wave[tid] = wave_inclusive_scan(v) - v;
         }
+        __syncthreads();
+        const u32 out0 = tile_sums[tile];
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (sel & (1u << j)) write(base + j * kBlock + tid, out0 + row_wave[j * kWaves + wave] + rank[j]);
+        __syncthreads();
     }
 }
 
