@@ -91,6 +91,41 @@ __device__ inline bool coarse_overlap(const uint4& a, const uint4& b) {
     return ov;
 }
 
+// The coarse bounds of a sorted entry's box on the two axes other than its
+// array's axis (axis a keeps axes (a+1)%3, (a+2)%3), gathered into sorted
+// order so that a crossing walk reads them in sequence with the keys and
+// payloads. The walked axis itself is left out: two endpoints that cross
+// on it are close there anyway.
+__device__ inline bool coarse_axis_overlap(u32 a, u32 b) { return ((a >> 16) <= (b & 0xffffu)) & ((b >> 16) <= (a & 0xffffu)); }
+// (bitwise, not short-circuit: no branches in the walk loop)
+__device__ inline bool coarse_overlap2(const uint2& a, const uint2& b) {
+    return coarse_axis_overlap(a.x, b.x) & coarse_axis_overlap(a.y, b.y);
+}
+// The two components for axis `axis` of bounds already loaded as a value
+// (scalar selects; a conditional copy of the vector type compiled (ROCm
+// 7.2) into a selection of field addresses that gave the z axis the wrong
+// second component).
+__device__ inline uint2 coarse_other(const uint4 c, u32 axis) {
+    const u32 f = axis == 0u ? c.y : (axis == 1u ? c.z : c.x);
+    const u32 s = axis == 0u ? c.z : (axis == 1u ? c.x : c.y);
+    return make_uint2(f, s);
+}
+// new_cs / old_cs[axis * n2 + j] for the current and previous sorted arrays
+// (both with this frame's coarse bounds); blockIdx.y = array * 3 + axis.
+// The gather reads every entry once, so it pays only when the walks visit
+// many entries: it runs when every box is recomputed or at least half of
+// them are listed (a settling pile); otherwise the walks load each
+// candidate's bounds by slot.
+__global__ __launch_bounds__(256) void k_gather_coarse(u32 n2, const u32* new_pay, const u32* old_pay,
+                                                       const uint4* coarse, uint2* new_cs, uint2* old_cs) {
+    const u32 axis = blockIdx.y % 3u;
+    const bool old = blockIdx.y >= 3u;
+    const u32* P = (old ? old_pay : new_pay) + static_cast<std::size_t>(axis) * n2;
+    uint2* C = (old ? old_cs : new_cs) + static_cast<std::size_t>(axis) * n2;
+    for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n2; j += gridDim.x * blockDim.x)
+        C[j] = coarse_other(coarse[P[j] >> 1], axis);
+}
+
 // Full (re)build: every box gets its keys; no moved list.
 __global__ __launch_bounds__(256) void k_bounds_full(u32 n, const float* pos, const float* quat,
                                                      const float* half, const float* off,
@@ -231,6 +266,8 @@ struct CrossArgs {
     u32 cap;
     u32* long_walks;   // walks still inside after kShortWalk entries
     u32 long_cap;
+    const uint2* new_cs;  // gathered coarse bounds of the other two axes, new array
+    const uint2* old_cs;  // the same for the old array
 };
 
 // Entries a thread walks before handing the rest to a wave.
@@ -347,6 +384,101 @@ __global__ __launch_bounds__(256) void k_cross_long(CrossArgs A) {
             u32 j = 0, pj = 0;
             const bool inside = walk_inside(A, g, base + lane + 1, j, pj);
             if (inside) walk_visit(A, w, pj, an, ao, ca);
+            if (__ballot(inside) != ~0ull) break;
+        }
+    }
+}
+
+// The same walks when the coarse bounds were gathered this frame (a
+// settling pile: most boxes move and the walks are long). Each step reads
+// the entry's key, payload and gathered bounds of the two other axes in
+// sequence; only candidates that pass them touch per-slot data.
+__device__ inline const uint2* walk_cs(const CrossArgs& A, const Walk& w) {
+    return (w.arr ? A.old_cs : A.new_cs) + static_cast<std::size_t>(w.axis) * A.n2;
+}
+__device__ inline bool walk_inside_g(const CrossArgs& A, const WalkGeom& g, const uint2* C, u32 step, u32& pj,
+                                     uint2& cj) {
+    if (g.up) {
+        const u32 j = g.own + step;
+        if (j >= A.n2) return false;
+        const u32 kj = g.K[j];
+        pj = g.P[j];
+        cj = C[j];
+        return kj < g.bound_v || (kj == g.bound_v && pj < g.pe);
+    }
+    if (step > g.own) return false;
+    const u32 j = g.own - step;
+    const u32 kj = g.K[j];
+    pj = g.P[j];
+    cj = C[j];
+    return kj > g.bound_v || (kj == g.bound_v && pj > g.pe);
+}
+// ca: the walker's own gathered entry (its old and new keys on the same two
+// axes); a candidate that misses it on one of them cannot change status.
+__device__ inline void walk_visit_g(const CrossArgs& A, const Walk& w, u32 pf, const uint2& cb, const uint2& ca,
+                                    const Rec& an, const Rec& ao) {
+    const u32 b = pf >> 1;
+    if ((pf & 1u) == w.kind || b == w.a || !coarse_overlap2(ca, cb)) return;
+    const bool b_moved = (A.moved_bits[b >> 5] >> (b & 31u)) & 1u;
+    // an unmoved endpoint sits at the same value in both arrays: the
+    // new-array walk already saw it
+    if (b_moved || !w.arr) {
+        const Rec bn = A.cur[b];
+        if (pair_allowed(an, bn)) {
+            const Rec bo = b_moved ? A.prev[b] : bn;
+            const bool ov_new = overlap3(an, bn);
+            const bool ov_old = overlap3(ao, bo);
+            if (ov_new != ov_old) emit(pair_key(w.a, b) | (ov_new ? kFoundBit : 0ull), A.out, A.cnt, kDelta, A.cap);
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_cross_g(CrossArgs A) {
+    const u32 n_work = min(*A.cnt_moved, A.n2 >> 1) * 12u;
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < n_work; t += gridDim.x * blockDim.x) {
+        const u32 wa = t >> 2;
+        const Walk w{A.moved[wa / 3u], wa % 3u, (t >> 1) & 1u, t & 1u};
+        const Rec an = A.cur[w.a];
+        const Rec ao = A.prev[w.a];
+        const u32 v_old = ao.k[w.axis + 3 * w.kind];
+        const u32 v_new = an.k[w.axis + 3 * w.kind];
+        if (v_old == v_new) continue;
+        const WalkGeom g = walk_geom(A, w, v_old, v_new);
+        const uint2* C = walk_cs(A, w);
+        const uint2 ca = C[g.own];
+        for (u32 step = 1;; ++step) {
+            u32 pj = 0;
+            uint2 cj;
+            if (!walk_inside_g(A, g, C, step, pj, cj)) break;
+            walk_visit_g(A, w, pj, cj, ca, an, ao);
+            if (step == kShortWalk) {
+                const u32 slot = atomicAdd(&A.cnt[kLongWalks], 1u);
+                if (slot < A.long_cap)
+                    A.long_walks[slot] = t;
+                else
+                    atomicOr(&A.cnt[kOverflow], 4u);
+                break;
+            }
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_cross_long_g(CrossArgs A) {
+    const u32 lane = lane_id();
+    const u32 n_long = min(A.cnt[kLongWalks], A.long_cap);
+    const u32 n_waves = gridDim.x * (blockDim.x / kWave);
+    for (u32 i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < n_long; i += n_waves) {
+        const u32 t = A.long_walks[i];
+        const u32 wa = t >> 2;
+        const Walk w{A.moved[wa / 3u], wa % 3u, (t >> 1) & 1u, t & 1u};
+        const Rec an = A.cur[w.a];
+        const Rec ao = A.prev[w.a];
+        const WalkGeom g = walk_geom(A, w, ao.k[w.axis + 3 * w.kind], an.k[w.axis + 3 * w.kind]);
+        const uint2* C = walk_cs(A, w);
+        const uint2 ca = C[g.own];
+        for (u32 base = kShortWalk;; base += kWave) {
+            u32 pj = 0;
+            uint2 cj = make_uint2(0, 0);
+            const bool inside = walk_inside_g(A, g, C, base + lane + 1, pj, cj);
+            if (inside) walk_visit_g(A, w, pj, cj, ca, an, ao);
             if (__ballot(inside) != ~0ull) break;
         }
     }
@@ -516,6 +648,7 @@ struct Sap {
         radix_counts, radix_totals, sel_tiles, dummy_a, dummy_b;
     DevBuf<Rec> rec_cur, rec_prev;
     DevBuf<uint4> coarse;
+    DevBuf<uint2> cs_new, cs_old;  // coarse bounds gathered into sorted order
     DevBuf<u64> pairs[2], work_a, work_b, found, lost, kept;
     std::vector<u32> host_counters;
 
@@ -625,14 +758,27 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
                     Copy{s.pairs[s.cur].p, s.lost.p}, s.sel_tiles.p, cnt + kLost);
     } else {
         long pi = prof->begin("sap.pairs", st);
+        // The gather reads every entry once, so it pays only when the walks
+        // visit many entries: when every box is recomputed or at least half
+        // of them are listed (a settling pile). Otherwise the walks load
+        // each candidate's bounds by slot.
+        const bool gather = n && (!dirty || 2ull * n_dirty >= n);
+        if (gather)
+            hipLaunchKernelGGL(k_gather_coarse, dim3(std::min<u32>(grid_for(n2, 256), 2048u), 6), dim3(256), 0, st,
+                               n2, s.sorted_val[nxt].p, s.sorted_val[s.cur].p, s.coarse.p, s.cs_new.p, s.cs_old.p);
         CrossArgs A{n2, cnt + kMoved, s.moved.p, s.sorted_key[nxt].p, s.sorted_val[nxt].p, s.sorted_key[s.cur].p,
                     s.sorted_val[s.cur].p, s.rank[nxt].p, s.rank[s.cur].p, s.moved_bits.p, s.rec_cur.p,
-                    s.rec_prev.p, s.coarse.p, s.work_a.p, cnt, s.delta_cap, s.long_walks.p, 12u * n};
-        hipLaunchKernelGGL(k_cross, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12, 256), 8192u)),
-                           dim3(256), 0, st, A);
-        hipLaunchKernelGGL(k_cross_long, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12 * kWave, 256),
-                                                            4096u)),
-                           dim3(256), 0, st, A);
+                    s.rec_prev.p, s.coarse.p, s.work_a.p, cnt, s.delta_cap, s.long_walks.p, 12u * n,
+                    s.cs_new.p, s.cs_old.p};
+        const dim3 short_grid(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12, 256), 8192u));
+        const dim3 long_grid(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12 * kWave, 256), 4096u));
+        if (gather) {
+            hipLaunchKernelGGL(k_cross_g, short_grid, dim3(256), 0, st, A);
+            hipLaunchKernelGGL(k_cross_long_g, long_grid, dim3(256), 0, st, A);
+        } else {
+            hipLaunchKernelGGL(k_cross, short_grid, dim3(256), 0, st, A);
+            hipLaunchKernelGGL(k_cross_long, long_grid, dim3(256), 0, st, A);
+        }
         FYZZ_HIP_CHECK(hipGetLastError());
         prof->end("sap.pairs", pi, st);
         ProfScope ps(prof, "sap.lists", st);
@@ -765,6 +911,8 @@ int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_
         s.rec_cur.reserve(n);
         s.rec_prev.reserve(n);
         s.coarse.reserve(n);
+        s.cs_new.reserve(3ull * n2);
+        s.cs_old.reserve(3ull * n2);
         s.work_a.reserve(lcap);
         s.work_b.reserve(lcap);
         s.found.reserve(lcap);
