@@ -55,7 +55,7 @@ enum Counter : u32 {
     kError,     // non-finite bounds seen
     kOverflow,  // a capacity was exceeded
     kSweep,
-    kLarge,     // this frame's candidate list exceeds the one-block path
+    kLarge,     // this frame's candidate list took the multi-workgroup sort
     kLongWalks, // crossing walks handed to k_cross_long
     kCounters
 };
@@ -523,27 +523,57 @@ struct Copy {
     __device__ void operator()(u32 i, u32 o) const { dst[o] = src[i]; }
 };
 
-// One-workgroup path for this frame's candidate changes when they fit in
-// LDS: bitonic sort, unique, split into lost (no kFoundBit) then found.
-// Larger lists raise kLarge and leave the work to the multi-block path.
+// This frame's candidate changes (pair keys with kFoundBit for "found",
+// duplicates possible) -> sorted unique lost list, then found list, in one
+// launch of kListBlocks workgroups (plain launch; every workgroup of the
+// small grid is resident, and the grid barrier's spin is bounded).
+//   n <= kSmall: workgroup 0 sorts the list in LDS (bitonic) and splits it;
+//                the others return.
+//   larger:      each workgroup sorts kSmall-entry chunks in LDS, then
+//                rounds of pairwise merges (merge path, 16 outputs per
+//                thread) double the sorted runs, then a grid-wide scan of
+//                unique entries splits the sorted list.
 constexpr u32 kSmall = 8192;
-__global__ __launch_bounds__(1024) void k_lists_small(const u64* in, const u32* d_n, u32 cap, u64* found,
-                                                      u64* lost, u32* cnt) {
-    __shared__ u64 s[kSmall];
-    __shared__ u32 scratch[1024 / kWave + 1];
-    const u32 n = min(*d_n, cap);
-    const u32 tid = threadIdx.x;
-    if (n > kSmall) {
-        if (tid == 0) cnt[kLarge] = 1;
-        return;
-    }
-    u32 p2 = 64;
-    while (p2 < n) p2 <<= 1;
-    for (u32 i = tid; i < p2; i += 1024) s[i] = i < n ? in[i] : ~0ull;
+constexpr u32 kListThreads = 1024;
+constexpr u32 kListBlocks = 64;
+constexpr u32 kMergeRun = 16;  // outputs per thread in a merge round
+constexpr u32 kListBarWords = 2 + 2 * kListBlocks;  // count, generation, per-workgroup totals
+constexpr u32 kListSpinLimit = 1u << 22;
+
+// bar[0] arrivals, bar[1] generation. Relaxed agent-scope atomics with a
+// release fence before arriving and an acquire fence after leaving, so
+// global writes before the barrier are visible after it.
+__device__ inline void list_grid_sync(u32* bar, u32& gen, u32* cnt) {
     __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const u32 g0 = gen;
+        if (__hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            __hip_atomic_store(&bar[1], g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            u32 spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                if (++spins > kListSpinLimit) {
+                    atomicOr(&cnt[kError], 4u);  // workgroups not co-resident
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        gen = g0 + 1u;
+    }
+    __syncthreads();
+}
+
+// Bitonic sort of s[0, p2) in LDS (p2 a power of two >= 64) by the workgroup.
+__device__ inline void lds_bitonic(u64* s, u32 p2) {
+    const u32 tid = threadIdx.x;
     for (u32 k = 2; k <= p2; k <<= 1) {
         for (u32 j = k >> 1; j > 0; j >>= 1) {
-            for (u32 i = tid; i < p2; i += 1024) {
+            for (u32 i = tid; i < p2; i += kListThreads) {
                 const u32 ixj = i ^ j;
                 if (ixj > i) {
                     const u64 a = s[i], b = s[ixj];
@@ -557,23 +587,139 @@ __global__ __launch_bounds__(1024) void k_lists_small(const u64* in, const u32* 
             __syncthreads();
         }
     }
-    constexpr u32 per = kSmall / 1024;
-    const u32 first = tid * per;
+}
+
+// Merge-path split of output diagonal d of runs A (la) and B (lb): i entries
+// come from A (A first on equal keys).
+__device__ inline u32 merge_split(const u64* A, u32 la, const u64* B, u32 lb, u32 d) {
+    u32 lo = d > lb ? d - lb : 0u, hi = d < la ? d : la;
+    while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        if (A[mid] <= B[d - mid - 1])
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kListThreads) void k_lists(const u64* d_in, u64* tmp, const u32* d_n, u32 cap,
+                                                        u64* found, u64* lost, u32* cnt, u32* bar) {
+    __shared__ u64 s[kSmall];
+    __shared__ u32 scratch[kListThreads / kWave + 1];
+    const u32 n = min(*d_n, cap);
+    const u32 tid = threadIdx.x;
+    if (n <= kSmall) {
+        if (blockIdx.x != 0) return;
+        u32 p2 = 64;
+        while (p2 < n) p2 <<= 1;
+        for (u32 i = tid; i < p2; i += kListThreads) s[i] = i < n ? d_in[i] : ~0ull;
+        __syncthreads();
+        lds_bitonic(s, p2);
+        constexpr u32 per = kSmall / kListThreads;
+        const u32 first = tid * per;
+        u32 cu = 0, cl = 0;
+        for (u32 q = 0; q < per; ++q) {
+            const u32 i = first + q;
+            if (i < n && (i == 0 || s[i] != s[i - 1])) {
+                ++cu;
+                cl += (s[i] & kFoundBit) ? 0u : 1u;
+            }
+        }
+        u32 total_u, total_l;
+        u32 u = block_exclusive_scan<kListThreads>(cu, scratch, &total_u);
+        (void)block_exclusive_scan<kListThreads>(cl, scratch, &total_l);
+        for (u32 q = 0; q < per; ++q) {
+            const u32 i = first + q;
+            if (i < n && (i == 0 || s[i] != s[i - 1])) {
+                const u64 x = s[i];
+                if (u < total_l)
+                    lost[u] = x;
+                else
+                    found[u - total_l] = x & kPairMask;
+                ++u;
+            }
+        }
+        if (tid == 0) {
+            cnt[kUnique] = total_u;
+            cnt[kLost] = total_l;
+            cnt[kFound] = total_u - total_l;
+        }
+        return;
+    }
+    if (tid == 0) cnt[kLarge] = 1u;
+    u32 gen = tid == 0 ? __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    // chunks sorted in LDS: d_in -> tmp
+    const u32 chunks = (n + kSmall - 1) / kSmall;
+    for (u32 c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const u32 base = c * kSmall, len = min(kSmall, n - base);
+        for (u32 i = tid; i < kSmall; i += kListThreads) s[i] = i < len ? d_in[base + i] : ~0ull;
+        __syncthreads();
+        lds_bitonic(s, kSmall);
+        for (u32 i = tid; i < len; i += kListThreads) tmp[base + i] = s[i];
+        __syncthreads();
+    }
+    list_grid_sync(bar, gen, cnt);
+    // merge rounds: tmp -> d_in scratch -> ... (d_in is no longer needed)
+    u64* src = tmp;
+    u64* dst = const_cast<u64*>(d_in);
+    const u32 nthreads = gridDim.x * kListThreads;
+    const u32 gt = blockIdx.x * kListThreads + tid;
+    const u32 tiles = (n + kMergeRun - 1) / kMergeRun;
+    for (u32 L = kSmall; L < n; L <<= 1) {
+        for (u32 t = gt; t < tiles; t += nthreads) {
+            const u32 o0 = t * kMergeRun;
+            const u32 base = o0 / (2 * L) * (2 * L);
+            const u32 la = min(L, n - base);
+            const u32 lb = min(L, n - base - la);
+            const u64* A = src + base;
+            const u64* B = A + la;
+            const u32 d = o0 - base;
+            const u32 end = min(o0 + kMergeRun, n) - base;
+            u32 i = merge_split(A, la, B, lb, d);
+            u32 j = d - i;
+            for (u32 o = d; o < end; ++o) {
+                const bool take_a = j >= lb || (i < la && A[i] <= B[j]);
+                dst[base + o] = take_a ? A[i++] : B[j++];
+            }
+        }
+        list_grid_sync(bar, gen, cnt);
+        u64* t2 = src;
+        src = dst;
+        dst = t2;
+    }
+    // unique entries -> lost (no kFoundBit, sorted first) then found
+    const u32 per_block = (n + gridDim.x - 1) / gridDim.x;
+    const u32 b0 = min(n, blockIdx.x * per_block), b1 = min(n, b0 + per_block);
+    const u32 per_thread = (per_block + kListThreads - 1) / kListThreads;
+    const u32 t0 = min(b1, b0 + tid * per_thread), t1 = min(b1, t0 + per_thread);
     u32 cu = 0, cl = 0;
-    for (u32 q = 0; q < per; ++q) {
-        const u32 i = first + q;
-        if (i < n && (i == 0 || s[i] != s[i - 1])) {
+    for (u32 i = t0; i < t1; ++i) {
+        const u64 x = src[i];
+        if (i == 0 || x != src[i - 1]) {
             ++cu;
-            cl += (s[i] & kFoundBit) ? 0u : 1u;
+            cl += (x & kFoundBit) ? 0u : 1u;
         }
     }
-    u32 total_u, total_l;
-    u32 u = block_exclusive_scan<1024>(cu, scratch, &total_u);
-    (void)block_exclusive_scan<1024>(cl, scratch, &total_l);
-    for (u32 q = 0; q < per; ++q) {
-        const u32 i = first + q;
-        if (i < n && (i == 0 || s[i] != s[i - 1])) {
-            const u64 x = s[i];
+    u32 block_u, block_l;
+    u32 u = block_exclusive_scan<kListThreads>(cu, scratch, &block_u);
+    (void)block_exclusive_scan<kListThreads>(cl, scratch, &block_l);
+    if (tid == 0) {
+        bar[2 + 2 * blockIdx.x] = block_u;
+        bar[3 + 2 * blockIdx.x] = block_l;
+    }
+    list_grid_sync(bar, gen, cnt);
+    u32 before_u = 0, total_u = 0, total_l = 0;
+    for (u32 b = 0; b < gridDim.x; ++b) {
+        const u32 bu = bar[2 + 2 * b], bl = bar[3 + 2 * b];
+        if (b < blockIdx.x) before_u += bu;
+        total_u += bu;
+        total_l += bl;
+    }
+    u += before_u;
+    for (u32 i = t0; i < t1; ++i) {
+        const u64 x = src[i];
+        if (i == 0 || x != src[i - 1]) {
             if (u < total_l)
                 lost[u] = x;
             else
@@ -581,29 +727,10 @@ __global__ __launch_bounds__(1024) void k_lists_small(const u64* in, const u32* 
             ++u;
         }
     }
-    if (tid == 0) {
+    if (blockIdx.x == 0 && tid == 0) {
         cnt[kUnique] = total_u;
         cnt[kLost] = total_l;
         cnt[kFound] = total_u - total_l;
-    }
-}
-
-// Unique entries of the sorted large list -> lost / found (gated on kLarge).
-__global__ void k_split_large(u32 cap, const u64* delta, const u32* d_n, u64* found, u64* lost, u32* cnt) {
-    if (cnt[kLarge] == 0) return;
-    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-    const u32 n = min(*d_n, cap);
-    const u32 boundary = lower_bound64(delta, n, kFoundBit);
-    if (t == 0) {
-        cnt[kLost] = boundary;
-        cnt[kFound] = n - boundary;
-    }
-    for (u32 i = t; i < n; i += gridDim.x * blockDim.x) {
-        const u64 x = delta[i];
-        if (i < boundary)
-            lost[i] = x;
-        else
-            found[i - boundary] = x & kPairMask;
     }
 }
 
@@ -649,6 +776,7 @@ struct Sap {
     DevBuf<Rec> rec_cur, rec_prev;
     DevBuf<uint4> coarse;
     DevBuf<uint2> cs_new, cs_old;  // coarse bounds gathered into sorted order
+    DevBuf<u32> list_bar;          // grid barrier and totals of k_lists
     DevBuf<u64> pairs[2], work_a, work_b, found, lost, kept;
     std::vector<u32> host_counters;
 
@@ -782,14 +910,8 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
         FYZZ_HIP_CHECK(hipGetLastError());
         prof->end("sap.pairs", pi, st);
         ProfScope ps(prof, "sap.lists", st);
-        hipLaunchKernelGGL(k_lists_small, dim3(1), dim3(1024), 0, st, s.work_a.p, cnt + kDelta, s.delta_cap,
-                           s.found.p, s.lost.p, cnt);
-        // multi-block path, each launch gated on kLarge
-        sort_u64(s, st, cnt + kDelta, s.work_a.p, s.work_b.p, cnt + kLarge);
-        sel::select(st, s.delta_cap, cnt + kDelta, Unique{s.work_a.p}, Copy{s.work_a.p, s.work_b.p},
-                    s.sel_tiles.p, cnt + kUnique, cnt + kLarge);
-        hipLaunchKernelGGL(k_split_large, dim3(std::min<u32>(grid_for(s.delta_cap, 256), 1024u)), dim3(256), 0, st,
-                           s.delta_cap, s.work_b.p, cnt + kUnique, s.found.p, s.lost.p, cnt);
+        hipLaunchKernelGGL(k_lists, dim3(kListBlocks), dim3(kListThreads), 0, st, s.work_a.p, s.work_b.p,
+                           cnt + kDelta, s.delta_cap, s.found.p, s.lost.p, cnt, s.list_bar.p);
         hipLaunchKernelGGL(k_apply_delta, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(s.pair_cap) + s.delta_cap, 256), 2048u)), dim3(256), 0, st,
                            s.pair_cap, s.pairs[s.cur].p, cnt + pairs_cur, s.found.p, cnt + kFound, s.lost.p,
                            cnt + kLost, s.pairs[nxt].p, cnt + pairs_nxt, cnt);
@@ -913,6 +1035,8 @@ int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_
         s.coarse.reserve(n);
         s.cs_new.reserve(3ull * n2);
         s.cs_old.reserve(3ull * n2);
+        s.list_bar.reserve(sap::kListBarWords);
+        FYZZ_HIP_CHECK(hipMemset(s.list_bar.p, 0, sizeof(u32) * sap::kListBarWords));
         s.work_a.reserve(lcap);
         s.work_b.reserve(lcap);
         s.found.reserve(lcap);
