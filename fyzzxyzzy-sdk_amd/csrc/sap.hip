@@ -529,15 +529,20 @@ struct Copy {
 // small grid is resident, and the grid barrier's spin is bounded).
 //   n <= kSmall: workgroup 0 sorts the list in LDS (bitonic) and splits it;
 //                the others return.
-//   larger:      each workgroup sorts kSmall-entry chunks in LDS, then
-//                rounds of pairwise merges (merge path, 16 outputs per
-//                thread) double the sorted runs, then a grid-wide scan of
-//                unique entries splits the sorted list.
+//   larger:      the keys are distributed into kBuckets key ranges (found
+//                bit, then the lower slot's range: ranges in key order),
+//                each bucket is sorted in LDS by one workgroup, then a
+//                grid-wide scan of unique entries splits the sorted list.
+//                If a bucket exceeds kSmall entries, the list is instead
+//                sorted as kSmall-entry LDS chunks followed by rounds of
+//                pairwise merges (merge path, 16 outputs per thread).
 constexpr u32 kSmall = 8192;
 constexpr u32 kListThreads = 1024;
 constexpr u32 kListBlocks = 64;
+constexpr u32 kBuckets = 128;  // 2 (found bit) x 64 lower-slot ranges
 constexpr u32 kMergeRun = 16;  // outputs per thread in a merge round
-constexpr u32 kListBarWords = 2 + 2 * kListBlocks;  // count, generation, per-workgroup totals
+// count, generation, per-workgroup totals, per-workgroup bucket histograms
+constexpr u32 kListBarWords = 2 + 2 * kListBlocks + kListBlocks * kBuckets;
 constexpr u32 kListSpinLimit = 1u << 22;
 
 // bar[0] arrivals, bar[1] generation. Relaxed agent-scope atomics with a
@@ -603,10 +608,20 @@ __device__ inline u32 merge_split(const u64* A, u32 la, const u64* B, u32 lb, u3
     return lo;
 }
 
+// Bucket of a pair key: found bit, then the lower slot >> shift (slots below
+// 2^(shift + 6)), so buckets are key ranges in increasing order.
+__device__ inline u32 key_bucket(u64 x, u32 shift) {
+    return static_cast<u32>(x >> 63) * (kBuckets / 2) + min(static_cast<u32>((x >> 32) & 0x7fffffffu) >> shift,
+                                                             kBuckets / 2 - 1);
+}
+
 __global__ __launch_bounds__(kListThreads) void k_lists(const u64* d_in, u64* tmp, const u32* d_n, u32 cap,
-                                                        u64* found, u64* lost, u32* cnt, u32* bar) {
+                                                        u64* found, u64* lost, u32* cnt, u32* bar,
+                                                        u32 slot_shift) {
     __shared__ u64 s[kSmall];
     __shared__ u32 scratch[kListThreads / kWave + 1];
+    __shared__ u32 lhist[kBuckets], loff[kBuckets], ltot[kBuckets];
+    __shared__ u32 s_overflow;
     const u32 n = min(*d_n, cap);
     const u32 tid = threadIdx.x;
     if (n <= kSmall) {
@@ -649,6 +664,66 @@ __global__ __launch_bounds__(kListThreads) void k_lists(const u64* d_in, u64* tm
     }
     if (tid == 0) cnt[kLarge] = 1u;
     u32 gen = tid == 0 ? __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    u32* ghist = bar + 2 + 2 * kListBlocks;  // [workgroup][bucket]
+    const u32 per_block = (n + gridDim.x - 1) / gridDim.x;
+    const u32 b0 = min(n, blockIdx.x * per_block), b1 = min(n, b0 + per_block);
+    // bucket histogram of this workgroup's contiguous range
+    for (u32 k = tid; k < kBuckets; k += kListThreads) lhist[k] = 0;
+    __syncthreads();
+    for (u32 i = b0 + tid; i < b1; i += kListThreads) atomicAdd(&lhist[key_bucket(d_in[i], slot_shift)], 1u);
+    __syncthreads();
+    for (u32 k = tid; k < kBuckets; k += kListThreads) ghist[blockIdx.x * kBuckets + k] = lhist[k];
+    list_grid_sync(bar, gen, cnt);
+    // bucket starts and this workgroup's offsets; any bucket above kSmall
+    // sends every workgroup to the merge path (all see the same totals)
+    if (tid == 0) s_overflow = 0;
+    __syncthreads();
+    for (u32 k = tid; k < kBuckets; k += kListThreads) {
+        u32 total = 0, mine = 0;
+        for (u32 b = 0; b < gridDim.x; ++b) {
+            const u32 h = ghist[b * kBuckets + k];
+            if (b < blockIdx.x) mine += h;
+            total += h;
+        }
+        ltot[k] = total;
+        loff[k] = mine;
+        if (total > kSmall) s_overflow = 1;
+    }
+    __syncthreads();
+    const bool overflow = s_overflow != 0;
+    if (!overflow) {
+        if (tid == 0) {  // bucket starts (exclusive scan of the totals)
+            u32 run = 0;
+            for (u32 k = 0; k < kBuckets; ++k) {
+                lhist[k] = run;
+                run += ltot[k];
+            }
+        }
+        __syncthreads();
+        for (u32 k = tid; k < kBuckets; k += kListThreads) loff[k] += lhist[k];
+        __syncthreads();
+        // scatter into bucket order (order inside a bucket is arbitrary: the
+        // bucket is sorted next)
+        for (u32 i = b0 + tid; i < b1; i += kListThreads) {
+            const u64 x = d_in[i];
+            tmp[atomicAdd(&loff[key_bucket(x, slot_shift)], 1u)] = x;
+        }
+        list_grid_sync(bar, gen, cnt);
+        for (u32 k = blockIdx.x; k < kBuckets; k += gridDim.x) {
+            const u32 start = lhist[k], len = ltot[k];
+            if (len == 0) continue;
+            u32 p2 = 64;
+            while (p2 < len) p2 <<= 1;
+            for (u32 i = tid; i < p2; i += kListThreads) s[i] = i < len ? tmp[start + i] : ~0ull;
+            __syncthreads();
+            lds_bitonic(s, p2);
+            for (u32 i = tid; i < len; i += kListThreads) tmp[start + i] = s[i];
+            __syncthreads();
+        }
+        list_grid_sync(bar, gen, cnt);
+    }
+    const u64* src = tmp;
+    if (overflow) {
     // chunks sorted in LDS: d_in -> tmp
     const u32 chunks = (n + kSmall - 1) / kSmall;
     for (u32 c = blockIdx.x; c < chunks; c += gridDim.x) {
@@ -661,7 +736,7 @@ __global__ __launch_bounds__(kListThreads) void k_lists(const u64* d_in, u64* tm
     }
     list_grid_sync(bar, gen, cnt);
     // merge rounds: tmp -> d_in scratch -> ... (d_in is no longer needed)
-    u64* src = tmp;
+    u64* msrc = tmp;
     u64* dst = const_cast<u64*>(d_in);
     const u32 nthreads = gridDim.x * kListThreads;
     const u32 gt = blockIdx.x * kListThreads + tid;
@@ -672,7 +747,7 @@ __global__ __launch_bounds__(kListThreads) void k_lists(const u64* d_in, u64* tm
             const u32 base = o0 / (2 * L) * (2 * L);
             const u32 la = min(L, n - base);
             const u32 lb = min(L, n - base - la);
-            const u64* A = src + base;
+            const u64* A = msrc + base;
             const u64* B = A + la;
             const u32 d = o0 - base;
             const u32 end = min(o0 + kMergeRun, n) - base;
@@ -684,13 +759,13 @@ __global__ __launch_bounds__(kListThreads) void k_lists(const u64* d_in, u64* tm
             }
         }
         list_grid_sync(bar, gen, cnt);
-        u64* t2 = src;
-        src = dst;
+        u64* t2 = msrc;
+        msrc = dst;
         dst = t2;
     }
+    src = msrc;
+    }
     // unique entries -> lost (no kFoundBit, sorted first) then found
-    const u32 per_block = (n + gridDim.x - 1) / gridDim.x;
-    const u32 b0 = min(n, blockIdx.x * per_block), b1 = min(n, b0 + per_block);
     const u32 per_thread = (per_block + kListThreads - 1) / kListThreads;
     const u32 t0 = min(b1, b0 + tid * per_thread), t1 = min(b1, t0 + per_thread);
     u32 cu = 0, cl = 0;
@@ -910,8 +985,10 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
         FYZZ_HIP_CHECK(hipGetLastError());
         prof->end("sap.pairs", pi, st);
         ProfScope ps(prof, "sap.lists", st);
+        u32 slot_bits = 6;
+        while (slot_bits < 31 && (1ull << slot_bits) < n) ++slot_bits;  // slots < 2^slot_bits
         hipLaunchKernelGGL(k_lists, dim3(kListBlocks), dim3(kListThreads), 0, st, s.work_a.p, s.work_b.p,
-                           cnt + kDelta, s.delta_cap, s.found.p, s.lost.p, cnt, s.list_bar.p);
+                           cnt + kDelta, s.delta_cap, s.found.p, s.lost.p, cnt, s.list_bar.p, slot_bits - 6);
         hipLaunchKernelGGL(k_apply_delta, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(s.pair_cap) + s.delta_cap, 256), 2048u)), dim3(256), 0, st,
                            s.pair_cap, s.pairs[s.cur].p, cnt + pairs_cur, s.found.p, cnt + kFound, s.lost.p,
                            cnt + kLost, s.pairs[nxt].p, cnt + pairs_nxt, cnt);

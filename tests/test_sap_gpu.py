@@ -139,6 +139,41 @@ def _run_frames(ctx, orc, torch, n, frames, seed, scale, move_scale, fraction, r
     bp.close()
 
 
+def test_incremental_sap_concentrated_changes(ctx, orc, torch_cuda):
+    """Four huge low-slot boxes jump across the scene each frame, so tens of
+    thousands of pair changes share one lower slot range: the change list
+    is larger than the one-workgroup sort and one key-range bucket holds
+    more than it can sort in LDS, so the chunk-sort + merge rounds run."""
+    from fyzz_amd import sap
+    torch = torch_cuda
+    n = 12000
+    half, pos, quat, off = random_boxes(21, n, extent_scale=1.5, rotated=False)
+    lo, hi = pos.min(0), pos.max(0)
+    span = hi - lo
+    half[:4] = (0.45 * span).astype(np.float32)
+    corners = [lo + 0.3 * span, hi - 0.3 * span]
+    body = np.arange(n, dtype=np.uint32)
+    dyn = np.ones(n, np.uint8)
+    bp = sap.SapBroadphase(ctx, body, dyn, delta_capacity=2_000_000)
+    dq, dh, do = dev(torch, quat), dev(torch, half), dev(torch, off)
+    prev = np.zeros((0, 2), np.uint32)
+    for f in range(4):
+        pos[:4] = (corners[f % 2] + np.arange(4)[:, None] * 0.01).astype(np.float32)
+        dirty = dev(torch, np.arange(4, dtype=np.int32)) if f else None
+        bp.advance(dev(torch, pos), dq, dh, do, dirty=dirty)
+        b = orc.box_bounds(pos, quat, half, off)
+        want = orc.sap_pairs(b, body_id=body, dynamic=dyn)
+        wf, wl = orc.found_lost(prev, want)
+        if f:
+            assert len(wf) > 8192 and len(wl) > 8192  # one key range over the LDS sort
+        np.testing.assert_array_equal(bp.fetch(0), want, err_msg=f"pairs frame {f}")
+        np.testing.assert_array_equal(bp.fetch(1), wf, err_msg=f"found frame {f}")
+        np.testing.assert_array_equal(bp.fetch(2), wl, err_msg=f"lost frame {f}")
+        prev = want
+    np.testing.assert_equal(bp.counts()[3], 0)
+    bp.close()
+
+
 @pytest.mark.parametrize("n,seed,move", [(3000, 7, 20.0), (20000, 8, 1.0), (20000, 9, 40.0)])
 def test_incremental_sap_matches_oracle(ctx, orc, torch_cuda, n, seed, move):
     _run_frames(ctx, orc, torch_cuda, n, 5, seed, 2.0, move, 0.1, True)
