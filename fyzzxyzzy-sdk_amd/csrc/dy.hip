@@ -98,7 +98,7 @@ struct Ws {
     u32* pos_of;    // [nm] partition-sorted position of each manager in a partition
     uint4* srank;   // [nm] per position: rank of the manager among body0's / body1's (by partition), their degrees
     u32* erank;     // [2 nm] per incidence entry: rank among the body's managers by partition
-    u32* ver;       // [nb] solver updates applied to the body in this sweep
+    u64* vst;       // [6 nb] sweep state: velocity-change component bits | updates applied << 32
     CRow* crow;
     FRow* frow;
     u32 *part_count, *part_off, *part_fill;
@@ -179,11 +179,6 @@ __device__ inline void grid_sync(const Ws& W, u32& gen) {
     }
     __syncthreads();
 }
-// Waits until this wave's outstanding vector memory operations have
-// completed (s_waitcnt vmcnt(0); gfx9 encoding with expcnt and lgkmcnt
-// left at their maxima), so the body stores above are performed at agent
-// scope before the version that publishes them.
-__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0f70); }
 // A wave whose lanes all wait backs off (64, 256, then 1024 cycles per
 // poll), so polling leaves the memory system to the waves doing work.
 __device__ inline void wait_backoff(u32 idle) {
@@ -375,7 +370,8 @@ __global__ __launch_bounds__(kBlock) void k_part_clear(Ws W) {
         W.deg[i] = 0;
         W.fill[i] = 0;
         W.max_dyn[i] = 0;
-        W.ver[i] = 0;
+#pragma unroll
+        for (u32 k = 0; k < 6; ++k) W.vst[6 * i + k] = 0ull;  // 0.0f, no updates yet
     }
     if (i < kMaxParts) {
         W.part_count[i] = 0;
@@ -1085,34 +1081,51 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
 // ------------------------------------------------------------------ solve
 
 // scene.cpp:8985-9010 (normal rows), 9050-9120 (friction rows)
-// Coherent body state: the sweep's body rows move between workgroups on
-// different XCDs through agent-scope relaxed atomics (64-bit halves), so
-// no L2 write-back or invalidation is needed around them.
-__device__ inline float4 load_body(const float4* p) {
-    const u64* q = reinterpret_cast<const u64*>(p);
-    const u64 a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const u64 b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return make_float4(__uint_as_float(static_cast<u32>(a)), __uint_as_float(static_cast<u32>(a >> 32)),
-                       __uint_as_float(static_cast<u32>(b)), __uint_as_float(static_cast<u32>(b >> 32)));
+// Coherent body state: during the sweep a body's velocity change lives in
+// six 64-bit words, each a component's bits with the number of updates
+// applied to the body in the upper half, moved between workgroups on
+// different XCDs by agent-scope relaxed atomics. A reader takes the state
+// when all six words carry the count it waits for, so a writer needs no
+// wait between its state stores and a separate counter.
+struct BodyState {
+    V v, w;
+};
+// The first word is polled alone (waiting costs one load per poll); once
+// it carries the count, the other five are loaded and checked.
+__device__ inline bool poll_state(const u64* q, u32 need, BodyState& s) {
+    u64 x[6];
+    x[0] = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (static_cast<u32>(x[0] >> 32) != need) return false;
+#pragma unroll
+    for (int k = 1; k < 6; ++k) x[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+#pragma unroll
+    for (int k = 1; k < 6; ++k) ok &= static_cast<u32>(x[k] >> 32) == need;
+    s.v = {__uint_as_float(static_cast<u32>(x[0])), __uint_as_float(static_cast<u32>(x[1])),
+           __uint_as_float(static_cast<u32>(x[2]))};
+    s.w = {__uint_as_float(static_cast<u32>(x[3])), __uint_as_float(static_cast<u32>(x[4])),
+           __uint_as_float(static_cast<u32>(x[5]))};
+    return ok;
 }
-__device__ inline void store_body(float4* p, float4 v) {
-    u64* q = reinterpret_cast<u64*>(p);
-    const u64 a = static_cast<u64>(__float_as_uint(v.x)) | (static_cast<u64>(__float_as_uint(v.y)) << 32);
-    const u64 b = static_cast<u64>(__float_as_uint(v.z)) | (static_cast<u64>(__float_as_uint(v.w)) << 32);
-    __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline void publish_state(u64* q, u32 count, const V& v, const V& w) {
+    const u64 hi = static_cast<u64>(count) << 32;
+    const float c[6] = {v.x, v.y, v.z, w.x, w.y, w.z};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) __hip_atomic_store(q + k, hi | __float_as_uint(c[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // One manager's rows of one iteration (scene.cpp:8853-9148). info: (body0,
 // body1, manager) of a partition-sorted position. Rows and patches are read
 // with plain loads: the same thread solves the same manager in every
 // iteration, and nothing else writes them during the sweep.
-__device__ void solve_manager(const Ws& W, int4 info, bool snap0, bool snap1, u32 iter, u32 pos_iters,
-                              u32 total_iters) {
+// s0 / s1: the bodies' states as taken by the sweep (zero for a static
+// side); ima / imb: inverse masses. The updated states are published with
+// counts c0 / c1; fin0 / fin1: the body's last update of the sweep, whose
+// state also goes to the plain rows read after the sweep.
+__device__ void solve_manager(const Ws& W, int4 info, const BodyState& s0, const BodyState& s1, float ima,
+                              float imb, u32 c0, u32 c1, bool snap0, bool snap1, bool fin0, bool fin1, u32 iter,
+                              u32 pos_iters, u32 total_iters) {
     const int i0 = info.x, i1 = info.y;
     const u32 m = static_cast<u32>(info.z);
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 la = i0 >= 0 ? load_body(&W.lin[i0]) : z4, lb = i1 >= 0 ? load_body(&W.lin[i1]) : z4;
-    const float4 aa4 = i0 >= 0 ? load_body(&W.ang[i0]) : z4, ab4 = i1 >= 0 ? load_body(&W.ang[i1]) : z4;
     // row counts come with the position record, so only the rows in use
     // are loaded, all at once
     const u32 no = static_cast<u32>(info.w) & 0xffu;
@@ -1124,8 +1137,7 @@ __device__ void solve_manager(const Ws& W, int4 info, bool snap0, bool snap1, u3
         if (q < no) rows[q] = W.crow[4 * m + q];
         if (q < 2 * anchors) frows[q] = W.frow[4 * m + q];
     }
-    V va = f3(la), vb = f3(lb), wa = f3(aa4), wb = f3(ab4);
-    const float ima = la.w, imb = lb.w;
+    V va = s0.v, vb = s1.v, wa = s0.w, wb = s1.w;
     bool bad = false;
     float normal_impulse = 0.0f;
 #pragma unroll
@@ -1191,21 +1203,27 @@ __device__ void solve_manager(const Ws& W, int4 info, bool snap0, bool snap1, u3
     }
     if (bad) raise_flag(W, FYZZ_DY_NONFINITE);
     if (i0 >= 0) {
+        publish_state(W.vst + 6ull * i0, c0, va, wa);
         const float4 l = make_float4(va.x, va.y, va.z, ima), a = make_float4(wa.x, wa.y, wa.z, 0.f);
-        store_body(&W.lin[i0], l);
-        store_body(&W.ang[i0], a);
         if (snap0) {
             W.mlin[i0] = l;
             W.mang[i0] = a;
         }
+        if (fin0) {
+            W.lin[i0] = l;
+            W.ang[i0] = a;
+        }
     }
     if (i1 >= 0) {
+        publish_state(W.vst + 6ull * i1, c1, vb, wb);
         const float4 l = make_float4(vb.x, vb.y, vb.z, imb), a = make_float4(wb.x, wb.y, wb.z, 0.f);
-        store_body(&W.lin[i1], l);
-        store_body(&W.ang[i1], a);
         if (snap1) {
             W.mlin[i1] = l;
             W.mang[i1] = a;
+        }
+        if (fin1) {
+            W.lin[i1] = l;
+            W.ang[i1] = a;
         }
     }
 }
@@ -1215,10 +1233,10 @@ __device__ void solve_manager(const Ws& W, int4 info, bool snap0, bool snap1, u3
 // without grid barriers. The sweep is the sequence (iteration, sorted
 // position); within a partition no two managers share a dynamic body, so
 // the Gauss-Seidel result depends only on each body seeing its managers'
-// updates in that order. A body's version counts the updates applied to
-// it; the manager at rank r among the body's d managers (by partition)
-// runs in iteration t once the version reads t*d + r, and bumps it after
-// its body stores have completed. Thread k solves positions k, k + T, ...
+// updates in that order. A body's state words count the updates applied
+// to it; the manager at rank r among the body's d managers (by partition)
+// runs in iteration t once all six words carry t*d + r, and publishes its
+// result with t*d + r + 1. Thread k solves positions k, k + T, ...
 // in every iteration (T threads), in sequence order, each as soon as it is
 // ready (a lane waits without holding back the rest of its wave). The
 // earliest unsolved (iteration, position) always has its predecessors done
@@ -1236,6 +1254,8 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
     bool have = false, ready0 = false, ready1 = false;
     int4 info = make_int4(-1, -1, 0, 0);
     uint4 rk = make_uint4(0u, 0u, 0u, 0u);
+    BodyState s0{}, s1{};
+    float ima = 0.f, imb = 0.f;
     for (;;) {
         bool progress = false;
         while (iter < total && i >= n) {
@@ -1249,20 +1269,22 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 have = true;
                 ready0 = info.x < 0;
                 ready1 = info.y < 0;
+                s0 = BodyState{};
+                s1 = BodyState{};
+                // inverse masses: constant through the sweep
+                ima = info.x >= 0 ? W.lin[info.x].w : 0.f;
+                imb = info.y >= 0 ? W.lin[info.y].w : 0.f;
             }
-            // a body whose version reached the awaited value stays there
-            // until this manager runs: it is not polled again
+            // a body whose count reached the awaited value stays there
+            // until this manager runs: its state is kept, not polled again
             const u32 need0 = iter * rk.z + rk.x, need1 = iter * rk.w + rk.y;
-            if (!ready0)
-                ready0 = __hip_atomic_load(&W.ver[info.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == need0;
-            if (!ready1)
-                ready1 = __hip_atomic_load(&W.ver[info.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == need1;
+            if (!ready0) ready0 = poll_state(W.vst + 6ull * info.x, need0, s0);
+            if (!ready1) ready1 = poll_state(W.vst + 6ull * info.y, need1, s1);
             if (ready0 && ready1) {
-                const bool snap = iter + 1 == pos_iters;
-                solve_manager(W, info, snap && rk.x + 1 == rk.z, snap && rk.y + 1 == rk.w, iter, pos_iters, total);
-                wait_stores();
-                if (info.x >= 0) __hip_atomic_store(&W.ver[info.x], need0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (info.y >= 0) __hip_atomic_store(&W.ver[info.y], need1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const bool snap = iter + 1 == pos_iters, last = iter + 1 == total;
+                const bool end0 = rk.x + 1 == rk.z, end1 = rk.y + 1 == rk.w;
+                solve_manager(W, info, s0, s1, ima, imb, need0 + 1u, need1 + 1u, snap && end0, snap && end1,
+                              last && end0, last && end1, iter, pos_iters, total);
                 have = false;
                 i += nthreads;
                 progress = true;
@@ -1660,7 +1682,8 @@ struct fyzz_hip_dy {
     DevBuf<u64> entry;
     DevBuf<int4> sinfo;
     DevBuf<uint4> srank;
-    DevBuf<u32> erank, ver, pos_of;
+    DevBuf<u32> erank, pos_of;
+    DevBuf<u64> vst;
     DevBuf<dy::CRow> crow;
     DevBuf<dy::FRow> frow;
     DevBuf<u32> part_count, part_off, part_fill, blocksums, ctr, flags, bar;
@@ -1898,7 +1921,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->sinfo.reserve(nm);
         d->srank.reserve(nm);
         d->erank.reserve(2 * nm);
-        d->ver.reserve(nb);
+        d->vst.reserve(6ull * nb);
         d->pos_of.reserve(nm);
         for (auto* b : {&d->part, &d->pend, &d->sorted, &d->order}) b->reserve(nm);
         d->crow.reserve(4 * nm);
@@ -1942,7 +1965,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         w.sinfo = d->sinfo.get();
         w.srank = d->srank.get();
         w.erank = d->erank.get();
-        w.ver = d->ver.get();
+        w.vst = d->vst.get();
         w.pos_of = d->pos_of.get();
         w.pend = d->pend.get();
         w.sorted = d->sorted.get();
@@ -1959,7 +1982,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         // every workspace array the step kernels touch must exist
         const void* need[] = {w.pv, w.pw, w.lin, w.ang, w.mlin, w.mang, w.rpos, w.rq, w.sii, w.has_row,
                               w.deg, w.off, w.fill, w.max_dyn, w.body_mask, w.inc, w.pred, w.part, w.pend,
-                              w.sorted, w.order, w.entry, w.sinfo, w.srank, w.erank, w.ver, w.pos_of, w.crow,
+                              w.sorted, w.order, w.entry, w.sinfo, w.srank, w.erank, w.vst, w.pos_of, w.crow,
                               w.frow,
                               w.part_count, w.part_off, w.part_fill, w.blocksums, w.ctr, w.flags, w.bar};
         for (const void* q : need) FYZZ_REQUIRE(q != nullptr, kDeviceError, "dy_create: workspace array missing");
