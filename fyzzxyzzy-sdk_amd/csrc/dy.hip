@@ -1121,22 +1121,26 @@ __device__ inline void publish_state(u64* q, u32 count, const V& v, const V& w) 
 // side); ima / imb: inverse masses. The updated states are published with
 // counts c0 / c1; fin0 / fin1: the body's last update of the sweep, whose
 // state also goes to the plain rows read after the sweep.
-__device__ void solve_manager(const Ws& W, int4 info, const BodyState& s0, const BodyState& s1, float ima,
-                              float imb, u32 c0, u32 c1, bool snap0, bool snap1, bool fin0, bool fin1, u32 iter,
-                              u32 pos_iters, u32 total_iters) {
-    const int i0 = info.x, i1 = info.y;
+// Loads the contact and friction rows in use (row counts come with the
+// position record), all at once.
+__device__ inline void load_rows(const Ws& W, int4 info, CRow (&rows)[4], FRow (&frows)[4]) {
     const u32 m = static_cast<u32>(info.z);
-    // row counts come with the position record, so only the rows in use
-    // are loaded, all at once
     const u32 no = static_cast<u32>(info.w) & 0xffu;
     const u32 anchors = (static_cast<u32>(info.w) >> 8) & 0xffu;
-    CRow rows[4];
-    FRow frows[4];
 #pragma unroll
     for (u32 q = 0; q < 4; ++q) {
         if (q < no) rows[q] = W.crow[4 * m + q];
         if (q < 2 * anchors) frows[q] = W.frow[4 * m + q];
     }
+}
+// rows / frows: the manager's rows as loaded by load_rows.
+__device__ void solve_manager(const Ws& W, int4 info, CRow (&rows)[4], FRow (&frows)[4], const BodyState& s0,
+                              const BodyState& s1, float ima, float imb, u32 c0, u32 c1, bool snap0, bool snap1,
+                              bool fin0, bool fin1, u32 iter, u32 pos_iters, u32 total_iters) {
+    const int i0 = info.x, i1 = info.y;
+    const u32 m = static_cast<u32>(info.z);
+    const u32 no = static_cast<u32>(info.w) & 0xffu;
+    const u32 anchors = (static_cast<u32>(info.w) >> 8) & 0xffu;
     V va = s0.v, vb = s1.v, wa = s0.w, wb = s1.w;
     bool bad = false;
     float normal_impulse = 0.0f;
@@ -1256,6 +1260,8 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
     uint4 rk = make_uint4(0u, 0u, 0u, 0u);
     BodyState s0{}, s1{};
     float ima = 0.f, imb = 0.f;
+    CRow rows[4];
+    FRow frows[4];
     for (;;) {
         bool progress = false;
         while (iter < total && i >= n) {
@@ -1274,6 +1280,9 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 // inverse masses: constant through the sweep
                 ima = info.x >= 0 ? W.lin[info.x].w : 0.f;
                 imb = info.y >= 0 ? W.lin[info.y].w : 0.f;
+                // the rows are loaded while the bodies are awaited (only
+                // this thread writes them during the sweep)
+                load_rows(W, info, rows, frows);
             }
             // a body whose count reached the awaited value stays there
             // until this manager runs: its state is kept, not polled again
@@ -1283,8 +1292,8 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
             if (ready0 && ready1) {
                 const bool snap = iter + 1 == pos_iters, last = iter + 1 == total;
                 const bool end0 = rk.x + 1 == rk.z, end1 = rk.y + 1 == rk.w;
-                solve_manager(W, info, s0, s1, ima, imb, need0 + 1u, need1 + 1u, snap && end0, snap && end1,
-                              last && end0, last && end1, iter, pos_iters, total);
+                solve_manager(W, info, rows, frows, s0, s1, ima, imb, need0 + 1u, need1 + 1u, snap && end0,
+                              snap && end1, last && end0, last && end1, iter, pos_iters, total);
                 have = false;
                 i += nthreads;
                 progress = true;
