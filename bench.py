@@ -60,7 +60,7 @@ SAP_SORT_TRAFFIC_BYTES = 834_500_000
 # at step 610 is loaded: 224k managers, 601k contact rows, 789k friction
 # rows, 5 iterations), same passes, profiles/pile_100k_full_step.md; the
 # algorithmic bytes of those launches by the byte model below.
-PILE_PGS_TRAFFIC = {"bytes": 2_694_000_000, "algorithmic_bytes": 488_600_000,
+PILE_PGS_TRAFFIC = {"bytes": 2_319_000_000, "algorithmic_bytes": 488_600_000,
                     "measured_on": "formed pile: 3 steps after loading the state at step 610 (224k managers)"}
 
 
