@@ -432,19 +432,46 @@ __device__ inline void walk_visit_g(const CrossArgs& A, const Walk& w, u32 pf, c
         }
     }
 }
+// Gathered walks are issued in sorted order: work item t is position j of
+// array (arr, axis), t = (arr * 3 + axis) * n2 + j, and walks when its
+// endpoint belongs to a moved box. Neighbouring lanes then walk
+// neighbouring, overlapping windows and share their cache lines, and the
+// walker's rank is its position.
+__device__ inline bool walker_at(const CrossArgs& A, u32 t, Walk& w, u32& own) {
+    const u32 seg = t / A.n2;
+    own = t - seg * A.n2;
+    w.arr = seg / 3u;
+    w.axis = seg - 3u * w.arr;
+    const u32 pe = (w.arr ? A.old_pay : A.new_pay)[static_cast<std::size_t>(w.axis) * A.n2 + own];
+    w.a = pe >> 1;
+    w.kind = pe & 1u;
+    return (A.moved_bits[w.a >> 5] >> (w.a & 31u)) & 1u;
+}
+__device__ inline WalkGeom walk_geom_at(const CrossArgs& A, const Walk& w, u32 own, u32 v_old, u32 v_new) {
+    WalkGeom g;
+    const std::size_t ax = static_cast<std::size_t>(w.axis) * A.n2;
+    g.K = (w.arr ? A.old_key : A.new_key) + ax;
+    g.P = (w.arr ? A.old_pay : A.new_pay) + ax;
+    g.pe = (w.a << 1) | w.kind;
+    g.own = own;
+    g.bound_v = w.arr ? v_new : v_old;
+    g.up = (w.arr != 0) == (v_new > v_old);
+    return g;
+}
 __global__ __launch_bounds__(256) void k_cross_g(CrossArgs A) {
-    const u32 n_work = min(*A.cnt_moved, A.n2 >> 1) * 12u;
+    const u32 n_work = 6u * A.n2;
     for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < n_work; t += gridDim.x * blockDim.x) {
-        const u32 wa = t >> 2;
-        const Walk w{A.moved[wa / 3u], wa % 3u, (t >> 1) & 1u, t & 1u};
+        Walk w;
+        u32 own;
+        if (!walker_at(A, t, w, own)) continue;
         const Rec an = A.cur[w.a];
         const Rec ao = A.prev[w.a];
         const u32 v_old = ao.k[w.axis + 3 * w.kind];
         const u32 v_new = an.k[w.axis + 3 * w.kind];
         if (v_old == v_new) continue;
-        const WalkGeom g = walk_geom(A, w, v_old, v_new);
+        const WalkGeom g = walk_geom_at(A, w, own, v_old, v_new);
         const uint2* C = walk_cs(A, w);
-        const uint2 ca = C[g.own];
+        const uint2 ca = C[own];
         for (u32 step = 1;; ++step) {
             u32 pj = 0;
             uint2 cj;
@@ -467,13 +494,14 @@ __global__ __launch_bounds__(256) void k_cross_long_g(CrossArgs A) {
     const u32 n_waves = gridDim.x * (blockDim.x / kWave);
     for (u32 i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < n_long; i += n_waves) {
         const u32 t = A.long_walks[i];
-        const u32 wa = t >> 2;
-        const Walk w{A.moved[wa / 3u], wa % 3u, (t >> 1) & 1u, t & 1u};
+        Walk w;
+        u32 own;
+        (void)walker_at(A, t, w, own);
         const Rec an = A.cur[w.a];
         const Rec ao = A.prev[w.a];
-        const WalkGeom g = walk_geom(A, w, ao.k[w.axis + 3 * w.kind], an.k[w.axis + 3 * w.kind]);
+        const WalkGeom g = walk_geom_at(A, w, own, ao.k[w.axis + 3 * w.kind], an.k[w.axis + 3 * w.kind]);
         const uint2* C = walk_cs(A, w);
-        const uint2 ca = C[g.own];
+        const uint2 ca = C[own];
         for (u32 base = kShortWalk;; base += kWave) {
             u32 pj = 0;
             uint2 cj = make_uint2(0, 0);
@@ -974,9 +1002,10 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
                     s.rec_prev.p, s.coarse.p, s.work_a.p, cnt, s.delta_cap, s.long_walks.p, 12u * n,
                     s.cs_new.p, s.cs_old.p};
         const dim3 short_grid(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12, 256), 8192u));
+        const dim3 sorted_grid(std::min<u32>(grid_for(6ull * n2, 256), 8192u));
         const dim3 long_grid(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12 * kWave, 256), 4096u));
         if (gather) {
-            hipLaunchKernelGGL(k_cross_g, short_grid, dim3(256), 0, st, A);
+            hipLaunchKernelGGL(k_cross_g, sorted_grid, dim3(256), 0, st, A);
             hipLaunchKernelGGL(k_cross_long_g, long_grid, dim3(256), 0, st, A);
         } else {
             hipLaunchKernelGGL(k_cross, short_grid, dim3(256), 0, st, A);
