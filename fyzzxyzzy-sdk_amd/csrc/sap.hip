@@ -432,6 +432,8 @@ __device__ inline void walk_visit_g(const CrossArgs& A, const Walk& w, u32 pf, c
         }
     }
 }
+// Entries a gathered walk's thread takes before handing the rest to a wave.
+constexpr u32 kShortWalkG = 256;
 // Gathered walks are issued in sorted order: work item t is position j of
 // array (arr, axis), t = (arr * 3 + axis) * n2 + j, and walks when its
 // endpoint belongs to a moved box. Neighbouring lanes then walk
@@ -477,7 +479,7 @@ __global__ __launch_bounds__(256) void k_cross_g(CrossArgs A) {
             uint2 cj;
             if (!walk_inside_g(A, g, C, step, pj, cj)) break;
             walk_visit_g(A, w, pj, cj, ca, an, ao);
-            if (step == kShortWalk) {
+            if (step == kShortWalkG) {
                 const u32 slot = atomicAdd(&A.cnt[kLongWalks], 1u);
                 if (slot < A.long_cap)
                     A.long_walks[slot] = t;
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(256) void k_cross_long_g(CrossArgs A) {
         const WalkGeom g = walk_geom_at(A, w, own, ao.k[w.axis + 3 * w.kind], an.k[w.axis + 3 * w.kind]);
         const uint2* C = walk_cs(A, w);
         const uint2 ca = C[own];
-        for (u32 base = kShortWalk;; base += kWave) {
+        for (u32 base = kShortWalkG;; base += kWave) {
             u32 pj = 0;
             uint2 cj = make_uint2(0, 0);
             const bool inside = walk_inside_g(A, g, C, base + lane + 1, pj, cj);
