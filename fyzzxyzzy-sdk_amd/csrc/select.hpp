@@ -10,7 +10,7 @@
 namespace fyzz_hip::sel {
 
 constexpr int kBlock = 256;
-constexpr int kItems = 16;
+constexpr int kItems = 4;
 constexpr int kTile = kBlock * kItems;
 
 __device__ inline u32 count_of(u32 cap, const u32* d_n) { return min(*d_n, cap); }
@@ -82,10 +82,12 @@ __global__ __launch_bounds__(kBlock) void k_write(u32 cap, const u32* d_n, P pre
             if (lane_id() == 0) row_wave[j * kWaves + wave] = static_cast<u32>(__popcll(m));
         }
         __syncthreads();
-        static_assert(kItems * kWaves == kWave, "one wave scans the (row, wave) counts");
+        static_assert(kItems * kWaves <= kWave, "one wave scans the (row, wave) counts");
         if (tid < kWave) {
-            const u32 v = row_wave[tid];
-            row_wave[tid] = wave_inclusive_scan(v) - v;
+            const bool mine = tid < static_cast<u32>(kItems * kWaves);
+            const u32 v = mine ? row_wave[tid] : 0u;
+            const u32 ex = wave_inclusive_scan(v) - v;
+            if (mine) row_wave[tid] = ex;
         }
         __syncthreads();
         const u32 out0 = tile_sums[tile];
