@@ -743,6 +743,29 @@ struct Patch {
     bool broken, disable_strong;
 };
 
+// Field-by-field copy (a whole-struct copy from a runtime-indexed slot
+// would keep the destination in memory as well).
+__device__ inline void copy_patch(Patch& d, const Patch& s) {
+    d.normal = V{s.normal.x, s.normal.y, s.normal.z};
+    d.n0 = V{s.n0.x, s.n0.y, s.n0.z};
+    d.n1 = V{s.n1.x, s.n1.y, s.n1.z};
+    d.sf = s.sf;
+    d.df = s.df;
+    d.rest = s.rest;
+    d.damping = s.damping;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        d.a0[k] = V{s.a0[k].x, s.a0[k].y, s.a0[k].z};
+        d.a1[k] = V{s.a1[k].x, s.a1[k].y, s.a1[k].z};
+    }
+    d.bmin = V{s.bmin.x, s.bmin.y, s.bmin.z};
+    d.bmax = V{s.bmax.x, s.bmax.y, s.bmax.z};
+    d.contacts = s.contacts;
+    d.anchors = s.anchors;
+    d.broken = s.broken;
+    d.disable_strong = s.disable_strong;
+}
+
 // scene.cpp:3981-4010
 __device__ bool history_valid(const Patch& old, const BodyView& a0, const BodyView& a1) {
     if (old.broken || old.disable_strong || old.anchors == 0u) return false;
@@ -813,27 +836,12 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
     const float static_friction = smax(dynamic_friction, 0.5f * (sf0 + sf1));
     const float max_pen_bias = -1.0e32f;
     const float rest_distance = 0.0f;
-    // contiguous same-normal runs (scene.cpp:7964-7997)
-    Run runs[4];
-    u32 nr = 0;
-    for (u32 j = 0; j < M.count; ++j) {
-        const V pn = v3(M.contacts[j].normal);
-        const V pp = v3(M.contacts[j].position);
-        if (nr == 0 || vdot(runs[nr - 1].normal, pn) < same_normal) {
-            runs[nr++] = Run{j, 1, pn, pp, pp, -1};
-        } else {
-            Run& r = runs[nr - 1];
-            ++r.count;
-            r.bmin = {smin(r.bmin.x, pp.x), smin(r.bmin.y, pp.y), smin(r.bmin.z, pp.z)};
-            r.bmax = {smax(r.bmax.x, pp.x), smax(r.bmax.y, pp.y), smax(r.bmax.z, pp.z)};
-        }
-    }
-    // correlation against the previous step's patch (scene.cpp:8007-8099)
-    Patch local[5];
-    u32 nl = 0;
-    if (M.history != kNone && M.history < W.nh && W.history[M.history].anchor_count > 2u)
+    // the previous step's patch when it correlates (scene.cpp:8007-8030)
+    bool have_old = false;
+    Patch c{};
+    if (M.history != kNone && M.history < W.nh && W.history[M.history].anchor_count > 2u) {
         raise_flag(W, FYZZ_DY_BAD_INPUT);
-    else if (M.history != kNone && M.history < W.nh) {
+    } else if (M.history != kNone && M.history < W.nh) {
         const fyzz_dy_patch& h = W.history[M.history];
         Patch old{};
         old.n0 = v3(h.body0_normal);
@@ -850,68 +858,136 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
         old.broken = h.flags & 1u;
         old.disable_strong = (h.flags >> 1) & 1u;
         if (history_valid(old, a0, a1)) {
-            Patch c = old;
+            c = old;
             c.normal = srot(mass_orientation(a0.q), old.n0);
             c.contacts = 0;
             c.bmin = {kFltMax, kFltMax, kFltMax};
             c.bmax = {-kFltMax, -kFltMax, -kFltMax};
-            local[nl++] = c;
+            have_old = true;
         }
     }
-    int heads[5] = {-1, -1, -1, -1, -1};
-    for (u32 ri = 0; ri < nr; ++ri) {
-        Run& cp = runs[ri];
-        u32 lp = 0;
-        for (; lp < nl; ++lp) {
-            const Patch& p = local[lp];
-            if (vdot(p.normal, cp.normal) >= same_normal && p.rest == restitution && p.damping == damping &&
-                p.sf == static_friction && p.df == dynamic_friction)
-                break;
-        }
-        if (lp == nl) {
-            Patch p{};
-            p.normal = cp.normal;
-            p.sf = static_friction;
-            p.df = dynamic_friction;
-            p.rest = restitution;
-            p.damping = damping;
-            p.n0 = local_normal(a0, cp.normal);
-            p.n1 = local_normal(a1, cp.normal);
-            local[nl++] = p;
-        }
-        Patch& p = local[lp];
-        if (p.contacts == 0) {
-            p.bmin = cp.bmin;
-            p.bmax = cp.bmax;
+    // The kept patch, its header normal and the contact order (2-bit fields,
+    // correlation-list order). A manager whose contacts all continue the
+    // first contact's normal (every PCM manager: one published normal)
+    // forms one run: the previous patch if it correlates, else one new
+    // patch, keeps all contacts in their own order. Only other managers
+    // take the general path over runs and candidate patches.
+    Patch P;
+    V header;
+    u32 order_bits = 0, no = 0;
+    bool one_run = M.count > 0 && M.count <= 4;
+    {
+        const V n0c = v3(M.contacts[0].normal);
+#pragma unroll
+        for (u32 jj = 1; jj < 4; ++jj)
+            if (jj < M.count && vdot(n0c, v3(M.contacts[jj].normal)) < same_normal) one_run = false;
+    }
+    if (one_run) {
+        // the run (scene.cpp:7964-7997) and its correlation (8007-8099)
+        const V rn = v3(M.contacts[0].normal);
+        V rbmin = v3(M.contacts[0].position), rbmax = rbmin;
+#pragma unroll
+        for (u32 jj = 1; jj < 4; ++jj)
+            if (jj < M.count) {
+                const V pp = v3(M.contacts[jj].position);
+                rbmin = {smin(rbmin.x, pp.x), smin(rbmin.y, pp.y), smin(rbmin.z, pp.z)};
+                rbmax = {smax(rbmax.x, pp.x), smax(rbmax.y, pp.y), smax(rbmax.z, pp.z)};
+            }
+        if (have_old && vdot(c.normal, rn) >= same_normal && c.rest == restitution && c.damping == damping &&
+            c.sf == static_friction && c.df == dynamic_friction) {
+            P = c;
         } else {
-            p.bmin = {smin(p.bmin.x, cp.bmin.x), smin(p.bmin.y, cp.bmin.y), smin(p.bmin.z, cp.bmin.z)};
-            p.bmax = {smax(p.bmax.x, cp.bmax.x), smax(p.bmax.y, cp.bmax.y), smax(p.bmax.z, cp.bmax.z)};
+            P = Patch{};
+            P.normal = rn;
+            P.sf = static_friction;
+            P.df = dynamic_friction;
+            P.rest = restitution;
+            P.damping = damping;
+            P.n0 = local_normal(a0, rn);
+            P.n1 = local_normal(a1, rn);
         }
-        p.contacts += cp.count;
-        cp.next = heads[lp];
-        heads[lp] = int(ri);
-    }
-    // exactly one patch keeps contacts: PCM managers publish one normal
-    int kept = -1, nkept = 0;
-    for (u32 j = 0; j < nl; ++j)
-        if (local[j].contacts) {
-            if (kept < 0) kept = int(j);
-            ++nkept;
+        P.bmin = rbmin;
+        P.bmax = rbmax;
+        P.contacts = M.count;
+        header = rn;
+        no = M.count;
+        order_bits = 0u | (1u << 2) | (2u << 4) | (3u << 6);
+    } else {
+        // contiguous same-normal runs (scene.cpp:7964-7997)
+        Run runs[4];
+        u32 nr = 0;
+        for (u32 j = 0; j < M.count; ++j) {
+            const V pn = v3(M.contacts[j].normal);
+            const V pp = v3(M.contacts[j].position);
+            if (nr == 0 || vdot(runs[nr - 1].normal, pn) < same_normal) {
+                runs[nr++] = Run{j, 1, pn, pp, pp, -1};
+            } else {
+                Run& r = runs[nr - 1];
+                ++r.count;
+                r.bmin = {smin(r.bmin.x, pp.x), smin(r.bmin.y, pp.y), smin(r.bmin.z, pp.z)};
+                r.bmax = {smax(r.bmax.x, pp.x), smax(r.bmax.y, pp.y), smax(r.bmax.z, pp.z)};
+            }
         }
-    if (nkept != 1) {
-        raise_flag(W, FYZZ_DY_LOGIC);
-        return;
+        // correlation against the previous step's patch (scene.cpp:8007-8099)
+        Patch local[5];
+        u32 nl = 0;
+        if (have_old) local[nl++] = c;
+        int heads[5] = {-1, -1, -1, -1, -1};
+        for (u32 ri = 0; ri < nr; ++ri) {
+            Run& cp = runs[ri];
+            u32 lp = 0;
+            for (; lp < nl; ++lp) {
+                const Patch& p = local[lp];
+                if (vdot(p.normal, cp.normal) >= same_normal && p.rest == restitution && p.damping == damping &&
+                    p.sf == static_friction && p.df == dynamic_friction)
+                    break;
+            }
+            if (lp == nl) {
+                Patch p{};
+                p.normal = cp.normal;
+                p.sf = static_friction;
+                p.df = dynamic_friction;
+                p.rest = restitution;
+                p.damping = damping;
+                p.n0 = local_normal(a0, cp.normal);
+                p.n1 = local_normal(a1, cp.normal);
+                local[nl++] = p;
+            }
+            Patch& p = local[lp];
+            if (p.contacts == 0) {
+                p.bmin = cp.bmin;
+                p.bmax = cp.bmax;
+            } else {
+                p.bmin = {smin(p.bmin.x, cp.bmin.x), smin(p.bmin.y, cp.bmin.y), smin(p.bmin.z, cp.bmin.z)};
+                p.bmax = {smax(p.bmax.x, cp.bmax.x), smax(p.bmax.y, cp.bmax.y), smax(p.bmax.z, cp.bmax.z)};
+            }
+            p.contacts += cp.count;
+            cp.next = heads[lp];
+            heads[lp] = int(ri);
+        }
+        // exactly one patch keeps contacts: PCM managers publish one normal
+        int kept = -1, nkept = 0;
+        for (u32 j = 0; j < nl; ++j)
+            if (local[j].contacts) {
+                if (kept < 0) kept = int(j);
+                ++nkept;
+            }
+        if (nkept != 1) {
+            raise_flag(W, FYZZ_DY_LOGIC);
+            return;
+        }
+        copy_patch(P, local[kept]);
+        header = v3(M.contacts[runs[heads[kept]].start].normal);
+        for (int cp = heads[kept]; cp >= 0; cp = runs[cp].next)
+            for (u32 c = runs[cp].start; c < runs[cp].start + runs[cp].count; ++c) {
+                order_bits |= c << (2 * no);
+                ++no;
+            }
     }
-    Patch P = local[kept];
-    const V header = v3(M.contacts[runs[heads[kept]].start].normal);
-    u32 order[4];
-    u32 no = 0;
-    for (int cp = heads[kept]; cp >= 0; cp = runs[cp].next)
-        for (u32 c = runs[cp].start; c < runs[cp].start + runs[cp].count; ++c) order[no++] = c;
     // contact rows in the correlation-list order (scene.cpp:8204-8225, 8237-8330)
     bool bad = false;
     for (u32 q = 0; q < no; ++q) {
-        const fyzz_dy_contact& C = M.contacts[order[q]];
+        const fyzz_dy_contact& C = M.contacts[(order_bits >> (2 * q)) & 3u];
         const V n = header;
         const V cpos = v3(C.position);
         const V ra_x_n = vcross(vsub(cpos, a.pos), n);
@@ -973,8 +1049,10 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
         if (!keep_pair) {
             if (count == 1) anchors[0] = world_point(a0, P.a0[0]);
             float pd2 = 0.0f;
-            for (u32 q = 0; q < no; ++q) {
-                const u32 j = order[q];
+#pragma unroll
+            for (u32 q = 0; q < 4; ++q) {
+                if (q >= no) continue;
+                const u32 j = (order_bits >> (2 * q)) & 3u;
                 if (M.contacts[j].separation >= friction_offset_threshold + rest_distance) continue;
                 const V wp = v3(M.contacts[j].position);
                 if (count == 0) {
@@ -1003,8 +1081,9 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
                     }
                 }
             }
-            for (u32 k = 0; k < count; ++k) {
-                if (!changed[k]) continue;
+#pragma unroll
+            for (u32 k = 0; k < 2; ++k) {
+                if (k >= count || !changed[k]) continue;
                 P.a0[k] = local_point(a0, anchors[k]);
                 P.a1[k] = local_point(a1, anchors[k]);
             }
@@ -1022,7 +1101,9 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
             const float il = 1.0f / sqrtf(len2(t0));
             t0 = {t0.x * il, t0.y * il, t0.z * il};
             const V t1 = vcross(n, t0);
-            for (u32 k = 0; k < count; ++k) {
+#pragma unroll
+            for (u32 k = 0; k < 2; ++k) {
+                if (k >= count) continue;
                 const V ra = srot(mass_orientation(a0.q), P.a0[k]);
                 const V rb = srot(mass_orientation(a1.q), P.a1[k]);
                 const V c0 = mass_position(a0.pos, a0.q), c1 = mass_position(a1.pos, a1.q);
@@ -1072,8 +1153,8 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
     o.anchor_count = P.anchors;
     o.flags = (P.broken ? 1u : 0u) | (P.disable_strong ? 2u : 0u);
     W.patches[m] = o;
-    W.order[m] = no | (order[0] << 8) | ((no > 1 ? order[1] : 0u) << 12) | ((no > 2 ? order[2] : 0u) << 16) |
-                 ((no > 3 ? order[3] : 0u) << 20);
+    W.order[m] = no | ((order_bits & 3u) << 8) | ((no > 1 ? (order_bits >> 2) & 3u : 0u) << 12) |
+                 ((no > 2 ? (order_bits >> 4) & 3u : 0u) << 16) | ((no > 3 ? (order_bits >> 6) & 3u : 0u) << 20);
     // the sweep reads the row counts with the position record
     W.sinfo[W.pos_of[m]].w = static_cast<int>(no | (P.anchors << 8));
 }
