@@ -1731,7 +1731,13 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Ws W) {
 
 // impulses per manifold contact and the partition of each manager; the
 // step's contact and friction row counts
+// (the impulses land in their contact slots through selects, and the row
+// totals are summed per workgroup in LDS: one global atomic per workgroup
+// instead of one per wave on the same two words)
 __global__ __launch_bounds__(kBlock) void k_outputs(Ws W) {
+    __shared__ u32 s_rows, s_frows;
+    if (threadIdx.x == 0) s_rows = s_frows = 0;
+    __syncthreads();
     const u32 m = blockIdx.x * kBlock + threadIdx.x;
     u32 rows = 0, frows = 0;
     if (m < mcount(W)) {
@@ -1740,7 +1746,14 @@ __global__ __launch_bounds__(kBlock) void k_outputs(Ws W) {
         if (W.with_rows && W.part[m] < kMaxParts) {
             const u32 o = W.order[m];
             const u32 no = min(o & 0xffu, 4u);
-            for (u32 q = 0; q < no; ++q) out[(o >> (8 + 4 * q)) & 0xfu] = W.crow[4 * m + q].applied;
+#pragma unroll
+            for (u32 q = 0; q < 4; ++q) {
+                if (q >= no) continue;
+                const u32 slot = (o >> (8 + 4 * q)) & 0xfu;
+                const float v = W.crow[4 * m + q].applied;
+#pragma unroll
+                for (u32 k = 0; k < 4; ++k) out[k] = slot == k ? v : out[k];
+            }
             rows = no;
             frows = 2u * W.patches[m].anchor_count;
         }
@@ -1752,8 +1765,13 @@ __global__ __launch_bounds__(kBlock) void k_outputs(Ws W) {
         frows += __shfl_xor(frows, d, kWave);
     }
     if (lane_id() == 0 && (rows || frows)) {
-        atomicAdd(&W.ctr[kContacts], rows);
-        atomicAdd(&W.ctr[kFrictionRows], frows);
+        atomicAdd(&s_rows, rows);
+        atomicAdd(&s_frows, frows);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (s_rows || s_frows)) {
+        atomicAdd(&W.ctr[kContacts], s_rows);
+        atomicAdd(&W.ctr[kFrictionRows], s_frows);
     }
 }
 
