@@ -491,7 +491,9 @@ __global__ __launch_bounds__(kBlock) void k_part_color(Ws W) {
         idle = __ballot(progress) ? 0u : idle + 1u;
         wait_backoff(idle);
     }
-    if (depth) atomicMax(&W.ctr[kRounds], depth);
+    // wave maximum first: one atomic per wave on the shared word
+    for (int d = kWave / 2; d > 0; d >>= 1) depth = max(depth, static_cast<u32>(__shfl_xor(static_cast<int>(depth), d, kWave)));
+    if (lane_id() == 0 && depth) atomicMax(&W.ctr[kRounds], depth);
     grid_sync(W, gen);
     // overflow waves: the unassigned managers, in order, with fresh masks
     if (blockIdx.x == 0 && threadIdx.x == 0) {
