@@ -1345,6 +1345,11 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
     float ima = 0.f, imb = 0.f;
     CRow rows[4];
     FRow frows[4];
+    // the position record of the thread's next position, loaded while the
+    // current manager waits (positions repeat in every iteration)
+    u32 next_pos = kNone;
+    int4 next_info = info;
+    uint4 next_rk = rk;
     for (;;) {
         bool progress = false;
         while (iter < total && i >= n) {
@@ -1353,8 +1358,13 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
         }
         if (iter < total) {
             if (!have) {
-                info = W.sinfo[i];
-                rk = W.srank[i];
+                if (next_pos == i) {
+                    info = next_info;
+                    rk = next_rk;
+                } else {
+                    info = W.sinfo[i];
+                    rk = W.srank[i];
+                }
                 have = true;
                 ready0 = info.x < 0;
                 ready1 = info.y < 0;
@@ -1366,6 +1376,12 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 // the rows are loaded while the bodies are awaited (only
                 // this thread writes them during the sweep)
                 load_rows(W, info, rows, frows);
+                const u32 nx = i + nthreads < n ? i + nthreads : tid;
+                if (nx < n) {
+                    next_pos = nx;
+                    next_info = W.sinfo[nx];
+                    next_rk = W.srank[nx];
+                }
             }
             // a body whose count reached the awaited value stays there
             // until this manager runs: its state is kept, not polled again
