@@ -83,8 +83,14 @@ __device__ inline K load_key(const Batch& b, const K* keys, u32 seg, u32 i) {
 }
 
 // Lanes of this wave whose `d` equals mine (only among `valid` lanes).
+// A wave whose valid lanes all hold the first one's digit (the common case
+// in the upper passes of nearly sorted input) is answered by one ballot.
 __device__ inline u64 match_digit(u32 d, bool valid) {
-    u64 peers = __ballot(valid);
+    const u64 vmask = __ballot(valid);
+    if (vmask == 0) return 0;
+    const u32 dl = static_cast<u32>(__shfl(static_cast<int>(d), static_cast<int>(__builtin_ctzll(vmask))));
+    if (__ballot(valid && d == dl) == vmask) return vmask;
+    u64 peers = vmask;
 #pragma unroll
     for (int bit = 0; bit < 8; ++bit) {
         const bool set = (d >> bit) & 1u;
@@ -114,12 +120,21 @@ __global__ __launch_bounds__(kBlock) void k_upsweep(Batch b, const K* keys, cons
             const u32 i = base + j * kBlock + tid;
             k[j] = i < n ? load_key<K, S>(b, keys, seg, i) : K(0);
         }
+        // counts only (no ranks): a wave with one digit adds once, others
+        // add per lane in LDS
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const bool valid = base + j * kBlock + tid < n;
             const u32 d = digit_of(k[j], shift);
-            const u64 peers = match_digit(d, valid);
-            if (valid && lane_rank(peers) == 0) atomicAdd(&hist[d], static_cast<u32>(__popcll(peers)));
+            const u64 vmask = __ballot(valid);
+            if (vmask == 0) continue;
+            const u32 leader = static_cast<u32>(__builtin_ctzll(vmask));
+            const u32 dl = static_cast<u32>(__shfl(static_cast<int>(d), static_cast<int>(leader)));
+            if (__ballot(valid && d == dl) == vmask) {
+                if (lane_id() == leader) atomicAdd(&hist[d], static_cast<u32>(__popcll(vmask)));
+            } else if (valid) {
+                atomicAdd(&hist[d], 1u);
+            }
         }
         __syncthreads();
         counts[(static_cast<std::size_t>(seg) * tiles + tile) * kBins + tid] = hist[tid];
