@@ -51,8 +51,8 @@ sys.path.insert(0, str(ROOT / "tests"))
 METRIC = "body-steps/s at 100k-box pile + 4096-scene batch vs plain; SAP/radix HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes moved by one frame's endpoint radix sort (4 passes of tile
-# histogram + tile scan + scatter over 3 x 2n keys, n = 1e6, ranks from the
-# last scatter), from the two rocprofv3 PMC passes (FETCH_SIZE x 2.0,
+# histogram + tile scan + scatter over 3 x 2n keys, n = 1e6), from the two
+# rocprofv3 PMC passes (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0 after the copy calibration) summarised in
 # profiles/sap_1m_incremental.md.
 SAP_SORT_TRAFFIC_BYTES = 829_600_000
@@ -211,10 +211,10 @@ def bench_sap(args):
     # algorithmic bytes of the endpoint radix sort per frame: 3 axes x 2n
     # keys; per pass the tile histogram reads the 4-B key; pass 0's scatter
     # reads the plane key and writes key+payload (12 B); passes 1-3 read and
-    # write key+payload (16 B); the last pass also writes the endpoint's
-    # rank (4 B).
+    # write key+payload (16 B). (The moved boxes' endpoint positions are a
+    # separate pass of the walks.)
     keys = 3 * 2 * n
-    sort_bytes = keys * (4 * 4 + 12 + 3 * 16 + 4)
+    sort_bytes = keys * (4 * 4 + 12 + 3 * 16)
     sort_ms = stages["sap.sort"]["ms_avg"]
     achieved = sort_bytes / (sort_ms * 1e-3) / 1e9
     result = {
@@ -235,8 +235,7 @@ def bench_sap(args):
                    "boxes": n, "moved_per_frame": len(moves[0][0]), "live_pairs": int(n_pairs),
                    "radix_digit_bits": 8, "radix_passes": 4,
                    "endpoint_keys_per_s": keys * args.steps / (ms * 1e-3)},
-        "roofline": {"kernel": "endpoint radix sort (3 axes, 4 passes x tile histogram + tile scan + scatter; "
-                               "ranks from the last scatter)",
+        "roofline": {"kernel": "endpoint radix sort (3 axes, 4 passes x tile histogram + tile scan + scatter)",
                      "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": SAP_SORT_TRAFFIC_BYTES,
                      "algorithmic_bytes_per_launch": sort_bytes},

@@ -126,6 +126,23 @@ __global__ __launch_bounds__(256) void k_gather_coarse(u32 n2, const u32* new_pa
         C[j] = coarse_other(coarse[P[j] >> 1], axis);
 }
 
+// rank_new / rank_old[axis * n2 + endpoint] = the endpoint's position in
+// the current / previous sorted array of that axis, written for the
+// endpoints of moved boxes (the only walkers); blockIdx.y = array * 3 + axis.
+__global__ __launch_bounds__(256) void k_moved_ranks(u32 n2, const u32* new_pay, const u32* old_pay,
+                                                     const u32* moved_bits, u32* rank_new, u32* rank_old) {
+    const u32 axis = blockIdx.y % 3u;
+    const bool old = blockIdx.y >= 3u;
+    const std::size_t ax = static_cast<std::size_t>(axis) * n2;
+    const u32* P = (old ? old_pay : new_pay) + ax;
+    u32* R = (old ? rank_old : rank_new) + ax;
+    for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n2; j += gridDim.x * blockDim.x) {
+        const u32 pe = P[j];
+        const u32 b = pe >> 1;
+        if ((moved_bits[b >> 5] >> (b & 31u)) & 1u) R[pe] = j;
+    }
+}
+
 // Full (re)build: every box gets its keys; no moved list.
 __global__ __launch_bounds__(256) void k_bounds_full(u32 n, const float* pos, const float* quat,
                                                      const float* half, const float* off,
@@ -963,13 +980,10 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
     if (n) {
         ProfScope ps(prof, "sap.sort", st);
         radix::Batch b{n2, 3, n2, n2, nullptr, s.planes.p, n};
-        // the last pass also writes rank[axis][pe] = sorted position (the
-        // SapBox1D min/max indices of the reference GPU path, here for every
-        // endpoint of every axis)
         const bool in_a = radix::sort<u32>(st, b, radix::Source::SapPlanes, nullptr, nullptr,
                                            s.sorted_key[nxt].p, s.sorted_val[nxt].p, s.tmp_key.p,
                                            s.tmp_val.p, 32, s.radix_counts.p, s.radix_totals.p,
-                                           prof, "sap.sort.scatter", s.rank[nxt].p);
+                                           prof, "sap.sort.scatter");
         if (!in_a) {
             FYZZ_HIP_CHECK(hipMemcpyAsync(s.sorted_key[nxt].p, s.tmp_key.p, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, st));
             FYZZ_HIP_CHECK(hipMemcpyAsync(s.sorted_val[nxt].p, s.tmp_val.p, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, st));
@@ -996,11 +1010,18 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
         // of them are listed (a settling pile). Otherwise the walks load
         // each candidate's bounds by slot.
         const bool gather = n && (!dirty || 2ull * n_dirty >= n);
+        // per-slot walks start from their endpoint's position in each array
+        // (the SapBox1D min/max indices of the reference GPU path), for the
+        // moved boxes only; the sorted-order walks start at their position
+        if (n && !gather)
+            hipLaunchKernelGGL(k_moved_ranks, dim3(std::min<u32>(grid_for(n2, 256), 2048u), 6), dim3(256), 0, st,
+                               n2, s.sorted_val[nxt].p, s.sorted_val[s.cur].p, s.moved_bits.p, s.rank[0].p,
+                               s.rank[1].p);
         if (gather)
             hipLaunchKernelGGL(k_gather_coarse, dim3(std::min<u32>(grid_for(n2, 256), 2048u), 6), dim3(256), 0, st,
                                n2, s.sorted_val[nxt].p, s.sorted_val[s.cur].p, s.coarse.p, s.cs_new.p, s.cs_old.p);
         CrossArgs A{n2, cnt + kMoved, s.moved.p, s.sorted_key[nxt].p, s.sorted_val[nxt].p, s.sorted_key[s.cur].p,
-                    s.sorted_val[s.cur].p, s.rank[nxt].p, s.rank[s.cur].p, s.moved_bits.p, s.rec_cur.p,
+                    s.sorted_val[s.cur].p, s.rank[0].p, s.rank[1].p, s.moved_bits.p, s.rec_cur.p,
                     s.rec_prev.p, s.coarse.p, s.work_a.p, cnt, s.delta_cap, s.long_walks.p, 12u * n,
                     s.cs_new.p, s.cs_old.p};
         const dim3 short_grid(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12, 256), 8192u));
