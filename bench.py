@@ -55,7 +55,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 PMC passes (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0 after the copy calibration) summarised in
 # profiles/sap_1m_incremental.md.
-SAP_SORT_TRAFFIC_BYTES = 829_600_000
+SAP_SORT_TRAFFIC_BYTES = 611_000_000
 # HBM bytes of one k_pgs launch on the formed pile (3 steps after the state
 # at step 610 is loaded: 224k managers, 601k contact rows, 789k friction
 # rows, 5 iterations), same passes, profiles/pile_100k_full_step.md; the
