@@ -432,6 +432,41 @@ __global__ __launch_bounds__(kBlock) void k_part_chain(Ws W) {
     }
 }
 
+// Ascending sort of a[0, p) by one workgroup (every thread calls it): a
+// bitonic network over the next power of two, in the form whose every
+// compare-exchange puts the smaller key at the lower index (each merge
+// starts with a flipped half-cleaner), so the virtual padding (+inf keys
+// beyond p) never moves and its exchanges are skipped. The pending list
+// holds distinct manager indices.
+__device__ void sort_pending(u32* a, u32 p) {
+    u32 p2 = 1;
+    while (p2 < p) p2 <<= 1;
+    for (u32 k = 2; k <= p2; k <<= 1) {
+        for (u32 j = k >> 1; j > 0; j >>= 1) {
+            for (u32 t = threadIdx.x; t < (p2 >> 1); t += blockDim.x) {
+                u32 l, r;
+                if (j == (k >> 1)) {  // flipped half-cleaner: l pairs with its mirror in the block of k
+                    const u32 blk = t / j, o = t % j;
+                    l = blk * k + o;
+                    r = blk * k + k - 1 - o;
+                } else {
+                    l = (t / j) * 2 * j + t % j;
+                    r = l + j;
+                }
+                if (r < p) {
+                    const u32 x = a[l], y = a[r];
+                    if (x > y) {
+                        a[l] = y;
+                        a[r] = x;
+                    }
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+    }
+}
+
 // The colouring (cooperative) and the overflow waves.
 __global__ __launch_bounds__(kBlock) void k_part_color(Ws W) {
     u32 gen = barrier_start(W);
@@ -481,6 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_part_color(Ws W) {
                 have = false;
                 m += nthreads;
                 progress = true;
+                spins = 0;  // the budget bounds one wait
             } else if (++spins > kSpinLimit) {
                 raise_flag(W, FYZZ_DY_LOGIC);
                 m = nm;  // give up on this thread's managers
@@ -496,21 +532,20 @@ __global__ __launch_bounds__(kBlock) void k_part_color(Ws W) {
     if (lane_id() == 0 && depth) atomicMax(&W.ctr[kRounds], depth);
     grid_sync(W, gen);
     // overflow waves: the unassigned managers, in order, with fresh masks
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x != 0) return;
+    __shared__ u32 s_pending;
+    if (threadIdx.x == 0) {
         u32 p = ld(&W.ctr[kPend]);
         if (p > kMaxPending) {
             raise_flag(W, FYZZ_DY_OVERFLOW);
             p = 0;
         }
-        for (u32 i = 1; i < p; ++i) {
-            const u32 e = W.pend[i];
-            u32 j = i;
-            while (j > 0 && W.pend[j - 1] > e) {
-                W.pend[j] = W.pend[j - 1];
-                --j;
-            }
-            W.pend[j] = e;
-        }
+        s_pending = p;
+    }
+    __syncthreads();
+    sort_pending(W.pend, s_pending);
+    if (threadIdx.x == 0) {
+        u32 p = s_pending;
         for (u32 wave = 32; p; wave += 32) {
             for (u32 k = 0; k < p; ++k) {
                 const fyzz_dy_manager& M = W.mg[W.pend[k]];
@@ -1396,6 +1431,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 have = false;
                 i += nthreads;
                 progress = true;
+                spins = 0;  // the budget bounds one wait, not the thread's whole sweep
             } else if (++spins > kSpinLimit) {
                 raise_flag(W, FYZZ_DY_LOGIC);
                 iter = total;  // give up on this thread's work

@@ -1069,7 +1069,8 @@ Lists lists(fyzz_hip_sap* h) {
     const u32 cur = s.cur == 0 ? kPairs0 : kPairs1;
     const u32 prev = s.cur == 0 ? kPairs1 : kPairs0;
     return {s.pairs[s.cur].p, s.counters.p + cur, s.pairs[s.cur ^ 1].p, s.counters.p + prev,
-            s.pair_cap, s.counters.p + kError, s.counters.p + kOverflow, s.lost.p, s.counters.p + kLost};
+            s.pair_cap, s.counters.p + kError, s.counters.p + kOverflow, s.lost.p, s.counters.p + kLost,
+            s.counters.p + kFound};
 }
 }  // namespace fyzz_hip::sap
 

@@ -22,6 +22,7 @@ struct Lists {
     const u32* d_overflow;  // non-zero: pair or delta capacity exceeded
     const u64* lost;        // pairs that left the list this frame (sorted)
     const u32* d_lost;
+    const u32* d_found;     // pairs that entered the list this frame (count; may be null)
 };
 Lists lists(fyzz_hip_sap* h);
 // Whether a frame has been advanced (the next one is incremental).

@@ -39,6 +39,7 @@ constexpr float kContactOffset = 0.02f;    // per shape
 constexpr float kContactDistance = 0.04f;  // sum of the pair's offsets
 constexpr u32 kNone = 0xffffffffu;
 constexpr u32 kCacheWords = 56;            // 224-byte manifold cache record
+constexpr u32 kTotals = 9;                 // running totals (fyzz_hip_world_totals)
 
 struct alignas(16) CacheRec {
     u32 w[kCacheWords];
@@ -486,8 +487,9 @@ __global__ void k_touch(u32 cap, const u32* d_nbb, const u32* counts_bb, u32* to
 // running totals over steps (algorithmic byte accounting of the solve):
 // [0] steps, [1] broadphase pairs, [2] managers, [3] contact rows,
 // [4] friction rows, [5] partitions, [6] position+velocity iterations,
-// [7] partition dependency depth
-__global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const u32* dy_ctr, u64* totals) {
+// [7] partition dependency depth, [8] broadphase found + lost pairs
+__global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const u32* dy_ctr, const u32* d_found,
+                             const u32* d_lost, u64* totals) {
     totals[0] += 1;
     totals[1] += min(*d_pairs, cap);
     totals[2] += *d_nm;
@@ -496,6 +498,8 @@ __global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const
     totals[5] += dy_ctr[dy::kCtrParts];
     totals[6] += dy_ctr[dy::kCtrPos] + max(dy_ctr[dy::kCtrVel], 1u);
     totals[7] += dy_ctr[dy::kCtrRounds];
+    if (d_found) totals[8] += *d_found;
+    if (d_lost) totals[8] += *d_lost;
 }
 
 // Scene::add_dynamic_box mass properties (scene.cpp:408-475): mass in
@@ -644,8 +648,8 @@ void upload(fyzz_hip_world* w) {
                                           st));
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     }
-    w->totals.reserve(8);
-    FYZZ_HIP_CHECK(hipMemsetAsync(w->totals.p, 0, 8 * sizeof(u64), st));
+    w->totals.reserve(world::kTotals);
+    FYZZ_HIP_CHECK(hipMemsetAsync(w->totals.p, 0, world::kTotals * sizeof(u64), st));
     w->d_step.reserve(1);
     FYZZ_HIP_CHECK(hipMemsetAsync(w->d_step.p, 0, sizeof(u32), st));
     w->parts.reserve(w->mgr_cap);
@@ -731,7 +735,7 @@ sap::Lists scene_broadphase(fyzz_hip_world* w) {
     FYZZ_HIP_CHECK(hipGetLastError());
     w->scur = nxt;
     return {w->scene_pairs[nxt].p, w->scene_n.p + nxt, w->scene_pairs[cur].p, w->scene_n.p + cur, cap,
-            w->scene_status.p, w->scene_status.p + 1, nullptr, nullptr};
+            w->scene_status.p, w->scene_status.p + 1, nullptr, nullptr, nullptr};
 }
 
 void publish_events(fyzz_hip_world* w, const sap::Lists& L, int c_prev, int c_cur) {
@@ -826,7 +830,7 @@ void step(fyzz_hip_world* w, float dt) {
                        w->d_nm.p, w->mgr_cap, w->patches.p, w->mgr_slot.p, w->hist[h ^ 1].p, w->stamp[h ^ 1].p,
                        w->d_step.p);
     hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(1), 0, st, L.d_cur, cap, w->d_nm.p, dy::counters(w->dy),
-                       w->totals.p);
+                       L.d_found, L.d_lost, w->totals.p);
     if (w->notify) publish_events(w, L, c, c2);
     hipLaunchKernelGGL(k_touch, dim3(std::min<u32>(grid_for(std::max(cap, n), 256), 2048u)), dim3(256), 0, st, cap,
                        L.d_cur, w->counts_bb.p, w->touch_bb[c2].p, n, w->has_floor ? w->counts_pl.p : nullptr,
@@ -905,6 +909,16 @@ void step_graph(fyzz_hip_world* w, float dt) {
     FYZZ_HIP_CHECK(hipGraphLaunch(w->gexec[parity], w->gstream));
     FYZZ_HIP_CHECK(hipEventRecord(w->gev_out, w->gstream));
     FYZZ_HIP_CHECK(hipStreamWaitEvent(outer, w->gev_out, 0));
+}
+
+// The pair list the next step starts from (the last step's current list).
+sap::Lists current_lists(fyzz_hip_world* w) {
+    if (!w->n_scenes) return sap::lists(w->sap);
+    sap::Lists L{};
+    L.cur = w->scene_pairs[w->scur].p;
+    L.d_cur = w->scene_n.p + w->scur;
+    L.cap = w->pair_cap;
+    return L;
 }
 
 }  // namespace
@@ -1208,12 +1222,12 @@ int fyzz_hip_world_totals(fyzz_hip_world* w, uint64_t* out, int reset) {
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
         FYZZ_REQUIRE(out, kInvalidArgument, "world: null output");
-        for (int k = 0; k < 8; ++k) out[k] = 0;
+        for (u32 k = 0; k < world::kTotals; ++k) out[k] = 0;
         if (!w->uploaded) return;
         hipStream_t st = w->ctx->stream;
-        FYZZ_HIP_CHECK(hipMemcpyAsync(out, w->totals.p, 8 * sizeof(u64), hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipMemcpyAsync(out, w->totals.p, world::kTotals * sizeof(u64), hipMemcpyDeviceToHost, st));
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
-        if (reset) FYZZ_HIP_CHECK(hipMemsetAsync(w->totals.p, 0, 8 * sizeof(u64), st));
+        if (reset) FYZZ_HIP_CHECK(hipMemsetAsync(w->totals.p, 0, world::kTotals * sizeof(u64), st));
     });
 }
 
@@ -1309,6 +1323,81 @@ int fyzz_hip_world_copy_bodies(fyzz_hip_world* w, fyzz_dy_body* dst) {
         if (w->n)
             FYZZ_HIP_CHECK(hipMemcpyAsync(dst, w->bodies.p, sizeof(fyzz_dy_body) * w->n, hipMemcpyDeviceToDevice,
                                           w->ctx->stream));
+    });
+}
+
+int fyzz_hip_world_broadphase_reports(fyzz_hip_world* w, int which, uint32_t* out_pairs, uint64_t capacity,
+                                      uint64_t* written) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(which >= 0 && which <= 2, kInvalidArgument, "world: which must be 0, 1 or 2");
+        FYZZ_REQUIRE(which == 0 || !w->n_scenes, kLogicError,
+                     "world: found/lost reports belong to the incremental broadphase (no scenes)");
+        if (written) *written = 0;
+        if (!w->uploaded || !w->n) return;
+        if (!w->n_scenes) {
+            const int rc = fyzz_hip_sap_fetch(w->sap, which, out_pairs, capacity, written);
+            if (rc) throw Error(rc, w->ctx->last_error);
+            return;
+        }
+        hipStream_t st = w->ctx->stream;
+        const sap::Lists L = current_lists(w);
+        u32 np = 0;
+        FYZZ_HIP_CHECK(hipMemcpyAsync(&np, L.d_cur, 4, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        np = std::min(np, w->pair_cap);
+        if (written) *written = np;
+        const u64 m = std::min<u64>(np, capacity);
+        if (!m) return;
+        FYZZ_REQUIRE(out_pairs, kInvalidArgument, "world: null output");
+        std::vector<u64> keys(m);
+        FYZZ_HIP_CHECK(hipMemcpyAsync(keys.data(), L.cur, 8 * m, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        for (u64 i = 0; i < m; ++i) {
+            out_pairs[2 * i] = static_cast<u32>(keys[i] >> 32);
+            out_pairs[2 * i + 1] = static_cast<u32>(keys[i] & 0xffffffffu);
+        }
+    });
+}
+
+int fyzz_hip_world_fetch_persistent(fyzz_hip_world* w, void* caches_bb, uint32_t pair_capacity, uint32_t* n_pairs,
+                                    void* caches_pl, fyzz_dy_patch* history, uint32_t* stamps, uint32_t* step) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        if (n_pairs) *n_pairs = 0;
+        if (step) *step = 0;
+        if (!w->uploaded || !w->n) return;
+        hipStream_t st = w->ctx->stream;
+        const sap::Lists L = current_lists(w);
+        u32 np = 0, dstep = 0;
+        FYZZ_HIP_CHECK(hipMemcpyAsync(&np, L.d_cur, 4, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipMemcpyAsync(&dstep, w->d_step.p, 4, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        np = std::min(np, w->pair_cap);
+        if (n_pairs) *n_pairs = np;
+        if (step) *step = dstep;
+        FYZZ_REQUIRE(!(caches_bb || history || stamps) || pair_capacity >= np, kOverflow,
+                     "world: pair capacity below the pair count");
+        const u32 n = w->n, cap = w->pair_cap;
+        if (caches_bb && np)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(caches_bb, w->cache_bb[w->ccur].p, sizeof(world::CacheRec) * np,
+                                          hipMemcpyDeviceToHost, st));
+        if (caches_pl)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(caches_pl, w->cache_pl.p, sizeof(world::CacheRec) * n,
+                                          hipMemcpyDeviceToHost, st));
+        const int h = w->hcur;
+        if (history) {
+            if (np)
+                FYZZ_HIP_CHECK(hipMemcpyAsync(history, w->hist[h].p, sizeof(fyzz_dy_patch) * np,
+                                              hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(history + np, w->hist[h].p + cap, sizeof(fyzz_dy_patch) * n,
+                                          hipMemcpyDeviceToHost, st));
+        }
+        if (stamps) {
+            if (np) FYZZ_HIP_CHECK(hipMemcpyAsync(stamps, w->stamp[h].p, 4ull * np, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(stamps + np, w->stamp[h].p + cap, 4ull * n, hipMemcpyDeviceToHost, st));
+        }
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     });
 }
 

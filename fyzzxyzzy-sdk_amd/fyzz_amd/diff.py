@@ -70,6 +70,12 @@ class Rollout:
             raise RuntimeError(f"rollout status flags {st['flags']:#x}")
         return final[:, :12].contiguous().view(torch.float32).reshape(self.nb, 3)
 
+    def close(self):
+        """Releases the adjoint solver's device workspace."""
+        if self.solver is not None:
+            self.solver.close()
+            self.solver = None
+
     def _dy_step(self, bodies, s, tape):
         self.solver.set_tape(tape)
         mg, hs = self.records[s]

@@ -183,12 +183,43 @@ class World:
 
     def totals(self, reset=False):
         """Running totals: steps, pairs, managers, contact rows, friction
-        rows, partitions, solver iterations, partition rounds (summed over
-        steps)."""
-        out = (C.c_uint64 * 8)()
+        rows, partitions, solver iterations, partition rounds, broadphase
+        found + lost pairs (summed over steps)."""
+        out = (C.c_uint64 * 9)()
         self._call("fyzz_hip_world_totals", out, C.c_int(1 if reset else 0))
-        keys = ("steps", "pairs", "managers", "contact_rows", "friction_rows", "partitions", "iterations", "rounds")
+        keys = ("steps", "pairs", "managers", "contact_rows", "friction_rows", "partitions", "iterations", "rounds",
+                "found_lost")
         return {k: int(out[i]) for i, k in enumerate(keys)}
+
+    def broadphase_reports(self, which=0):
+        """Sorted (lo, hi) slot pairs of the last step: 0 live pairs, 1
+        found, 2 lost (host copy)."""
+        written = C.c_uint64()
+        self._call("fyzz_hip_world_broadphase_reports", C.c_int(which), None, C.c_uint64(0), C.byref(written))
+        out = np.zeros((max(written.value, 1), 2), np.uint32)
+        self._call("fyzz_hip_world_broadphase_reports", C.c_int(which), out.ctypes.data_as(vp),
+                   C.c_uint64(len(out)), C.byref(written))
+        return out[:written.value]
+
+    def persistent_state(self):
+        """Host copies of what the next step reads besides the body records:
+        {"caches_bb": [P, 56] f32 aligned with the pair list, "caches_pl":
+        [n, 56] f32, "history": DY_PATCH[P + n], "stamps": u32[P + n],
+        "step": steps taken}."""
+        from .dy import DY_PATCH
+        np_ = C.c_uint32()
+        step = C.c_uint32()
+        self._call("fyzz_hip_world_fetch_persistent", None, C.c_uint32(0), C.byref(np_), None, None, None,
+                   C.byref(step))
+        p, n = np_.value, self.n_dynamic
+        cbb = np.zeros((max(p, 1), 56), np.float32)
+        cpl = np.zeros((max(n, 1), 56), np.float32)
+        hist = np.zeros(p + n, DY_PATCH)
+        stamps = np.zeros(p + n, np.uint32)
+        self._call("fyzz_hip_world_fetch_persistent", cbb.ctypes.data_as(vp), C.c_uint32(max(p, 1)), C.byref(np_),
+                   cpl.ctypes.data_as(vp), hist.ctypes.data_as(vp), stamps.ctypes.data_as(vp), C.byref(step))
+        assert np_.value == p
+        return {"caches_bb": cbb[:p], "caches_pl": cpl[:n], "history": hist, "stamps": stamps, "step": step.value}
 
     def managers(self):
         """(managers, impulses[n, 4], partitions[n]) of the last step, host copies."""

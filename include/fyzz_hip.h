@@ -396,7 +396,7 @@ int fyzz_hip_world_fetch_managers(fyzz_hip_world* world, fyzz_dy_manager* manage
 /* Synchronizes; running totals since creation or the last reset: [0]
  * steps, [1] broadphase pairs, [2] touching managers, [3] contact rows,
  * [4] friction rows, [5] partitions, [6] solver iterations, [7] partition
- * dependency depth (out[8], HOST).
+ * dependency depth, [8] broadphase found + lost pairs (out[9], HOST).
  * reset != 0 zeroes them afterwards. */
 int fyzz_hip_world_totals(fyzz_hip_world* world, uint64_t* out, int reset);
 /* Scene::set_contact_notifications (scene.h:654): when enabled, every
@@ -417,6 +417,27 @@ int fyzz_hip_world_contact_events(fyzz_hip_world* world, fyzz_contact_event* eve
  * Synchronizes to read the count. */
 int fyzz_hip_world_record_dy_inputs(fyzz_hip_world* world, fyzz_dy_manager* managers, fyzz_dy_patch* history,
                                     uint32_t capacity, uint32_t* n_managers);
+/* Synchronizes; the last step's broadphase reports as sorted (lo, hi) box
+ * SLOT pairs into a HOST array of 2*capacity u32 (*written = the total
+ * count): which 0 the live pairs (Scene::pairs), 1 found, 2 lost
+ * (GpuBroadphaseReports::found/lost, scene.h:78-91; worlds with scenes
+ * have no found/lost reports and reject 1 and 2). */
+int fyzz_hip_world_broadphase_reports(fyzz_hip_world* world, int which, uint32_t* out_pairs, uint64_t capacity,
+                                      uint64_t* written);
+/* Synchronizes; copies the narrowphase and friction state that the next
+ * step reads (the state Scene::StepCheckpoint holds for these stages,
+ * scene.h:1802-1900, here for checkpoints and parity checks) to HOST
+ * arrays, any of which may be NULL: caches_bb[pair_capacity] the box/box
+ * manifold caches aligned with the live pair list
+ * (fyzz_hip_pcm_cache_bytes() each), caches_pl[n] the plane manifold cache
+ * of each box, history[n_pairs + n] and stamps[n_pairs + n] the friction
+ * patch history by manager slot (box/box managers aligned with the pair
+ * list, then the plane manager of each box) with the step number that
+ * wrote each entry; *step = the number of steps taken. A manager of the
+ * next step reads its slot's patch when the slot's stamp equals *step. */
+int fyzz_hip_world_fetch_persistent(fyzz_hip_world* world, void* caches_bb, uint32_t pair_capacity,
+                                    uint32_t* n_pairs, void* caches_pl, fyzz_dy_patch* history, uint32_t* stamps,
+                                    uint32_t* step);
 /* Device-to-device copy of the body records (creation order). */
 int fyzz_hip_world_copy_bodies(fyzz_hip_world* world, fyzz_dy_body* dst);
 /* Device pointers of the last step's manager list, contact impulses

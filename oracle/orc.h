@@ -49,3 +49,24 @@ void orc_found_lost(std::size_t n_old, const std::uint32_t* old_pairs,
                     std::uint32_t* lost, std::size_t* n_lost);
 
 }  // extern "C"
+
+#ifdef __cplusplus
+#include <thread>
+#include <vector>
+namespace orc {
+// Runs fn(begin, end) over [0, n) split into `threads` contiguous ranges
+// on host threads (inline when threads <= 1 or n is small). The work items
+// must be independent.
+template <class F>
+inline void parallel_ranges(std::uint32_t n, std::uint32_t threads, F&& fn) {
+    if (threads <= 1 || n < 1024) {
+        fn(0u, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const std::uint32_t step = (n + threads - 1) / threads;
+    for (std::uint32_t b = 0; b < n; b += step) pool.emplace_back(fn, b, b + step < n ? b + step : n);
+    for (auto& t : pool) t.join();
+}
+}  // namespace orc
+#endif

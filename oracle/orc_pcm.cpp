@@ -9,6 +9,7 @@
 // translation units (GuPCMContactBoxBox.cpp:33-37, GuPCMGjkEpa.cpp:34-38)
 // compute v + (t w + u x t). Both are restated as written.
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -22,6 +23,7 @@ using V = orc::V3<float>;
 using Q = orc::Q4<float>;
 using orc::cross;
 using orc::dot;
+using orc::parallel_ranges;
 
 V add(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 V sub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -1194,6 +1196,73 @@ int orc_pcm_plane_box(const float* plane_pos, const float* box, float cd, float*
     const int r = plane_box(load3(plane_pos), load3(box), load3(box + 3), load4(box + 6), cd, c, out_pts, flags);
     cache_out(c, cache);
     return r;
+}
+
+// Many pairs at once (a whole step's pair list): pair t is orc_pcm_box_box
+// of box slots first[t] (Gu first) and second[t] read from the slot arrays
+// pos[3n], quat[4n], half[3n]; caches[t] (56 floats) in and out,
+// out_pts[t] 4 x 7 floats, counts[t], flags[t]. With threads > 1 the pairs
+// are split over that many host threads (pairs are independent). Returns
+// 0, or -1 - t for the first pair whose manifold is invalid.
+int orc_pcm_box_box_many(std::uint32_t n_pairs, const std::uint32_t* first, const std::uint32_t* second,
+                         const float* pos, const float* quat, const float* half, float cd, float* caches,
+                         float* out_pts, std::uint32_t* counts, std::uint32_t* flags, std::uint32_t threads) {
+    std::atomic<std::int64_t> bad{-1};
+    auto run = [&](std::uint32_t begin, std::uint32_t end) {
+        for (std::uint32_t t = begin; t < end; ++t) {
+            float a[10], b[10];
+            for (int k = 0; k < 3; ++k) {
+                a[k] = half[3ull * first[t] + k];
+                a[3 + k] = pos[3ull * first[t] + k];
+                b[k] = half[3ull * second[t] + k];
+                b[3 + k] = pos[3ull * second[t] + k];
+            }
+            for (int k = 0; k < 4; ++k) {
+                a[6 + k] = quat[4ull * first[t] + k];
+                b[6 + k] = quat[4ull * second[t] + k];
+            }
+            const int r = orc_pcm_box_box(a, b, cd, caches + 56ull * t, out_pts + 28ull * t, flags + t);
+            if (r < 0) {
+                std::int64_t cur = bad.load();
+                while ((cur < 0 || cur > t) && !bad.compare_exchange_weak(cur, t)) {
+                }
+                counts[t] = 0;
+            } else {
+                counts[t] = static_cast<std::uint32_t>(r);
+            }
+        }
+    };
+    parallel_ranges(n_pairs, threads, run);
+    return bad.load() < 0 ? 0 : static_cast<int>(-1 - bad.load());
+}
+
+// Every box slot t < n against the plane at plane_pos: orc_pcm_plane_box
+// with caches[t] in place; outputs as orc_pcm_box_box_many.
+int orc_pcm_plane_box_many(std::uint32_t n, const float* plane_pos, const float* pos, const float* quat,
+                           const float* half, float cd, float* caches, float* out_pts, std::uint32_t* counts,
+                           std::uint32_t* flags, std::uint32_t threads) {
+    std::atomic<std::int64_t> bad{-1};
+    auto run = [&](std::uint32_t begin, std::uint32_t end) {
+        for (std::uint32_t t = begin; t < end; ++t) {
+            float b[10];
+            for (int k = 0; k < 3; ++k) {
+                b[k] = half[3ull * t + k];
+                b[3 + k] = pos[3ull * t + k];
+            }
+            for (int k = 0; k < 4; ++k) b[6 + k] = quat[4ull * t + k];
+            const int r = orc_pcm_plane_box(plane_pos, b, cd, caches + 56ull * t, out_pts + 28ull * t, flags + t);
+            if (r < 0) {
+                std::int64_t cur = bad.load();
+                while ((cur < 0 || cur > t) && !bad.compare_exchange_weak(cur, t)) {
+                }
+                counts[t] = 0;
+            } else {
+                counts[t] = static_cast<std::uint32_t>(r);
+            }
+        }
+    };
+    parallel_ranges(n, threads, run);
+    return bad.load() < 0 ? 0 : static_cast<int>(-1 - bad.load());
 }
 
 }  // extern "C"

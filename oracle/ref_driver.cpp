@@ -144,6 +144,19 @@ int ref_add_dynamic_box(void* s, const float* half, const float* pos,
             {half[0], half[1], half[2]}, {pos[0], pos[1], pos[2]}, density);
     });
 }
+// n axis-aligned dynamic boxes (the broadphase workload's load); ids[n]
+// receives the body ids.
+int ref_add_aabbs(void* s, std::uint32_t n, const float* half, const float* pos, float density,
+                  std::uint32_t* ids) {
+    return guarded([&] {
+        auto* sc = static_cast<Scene*>(s);
+        for (std::uint32_t i = 0; i < n; ++i) {
+            const float* h = half + 3ull * i;
+            const float* p = pos + 3ull * i;
+            ids[i] = sc->add_dynamic_box({h[0], h[1], h[2]}, {p[0], p[1], p[2]}, density);
+        }
+    });
+}
 // n dynamic boxes with pose, velocities and material (the bench's scene
 // load); ids[n] receives the body ids.
 int ref_add_boxes(void* s, std::uint32_t n, const float* half, const float* pos, const float* quat,

@@ -551,3 +551,126 @@ class OracleDiff:
                                P(np.ascontiguousarray(mu, np.float64)), P(np.ascontiguousarray(adj_p, np.float64)),
                                P(gv), P(gw), P(gm), P(pf))
         return gv, gw, gm, pf
+
+
+def _keys64(pairs):
+    p = np.asarray(pairs, np.uint64).reshape(-1, 2)
+    return (p[:, 0] << np.uint64(32)) | p[:, 1]
+
+
+class StepOracle:
+    """One whole world step composed from the oracle's SAP, PCM (batched
+    C++ entries) and Dy, started from a device world's synchronized state:
+    its body records, its live pair list with the box/box manifold caches
+    aligned to it, its plane caches and its friction history (the world's
+    persistent_state()). The manager order is the world's (touching
+    box/box managers in pair order, then touching plane managers by slot).
+    scene_first (optional) splits the boxes into independent scenes:
+    pairs of boxes in different scenes are dropped, as the batched world's
+    per-scene broadphase never forms them. Sized for whole piles: the pair
+    work runs in C++ on `threads` host threads."""
+
+    def __init__(self, orc, half, floor_height=0.0, floor_friction=0.5, floor_restitution=0.0, scene_first=None,
+                 threads=8):
+        self.orc = orc
+        self.lib = orc.lib
+        self.half = np.ascontiguousarray(half, np.float32)
+        self.n = len(self.half)
+        st = np.zeros(1, DY_STATIC)
+        st["orientation"][0, 3] = 1.0
+        st["position"][0, 1] = floor_height
+        st["static_friction"] = floor_friction
+        st["dynamic_friction"] = floor_friction
+        st["restitution"] = floor_restitution
+        self.statics = st
+        self.scene_first = None if scene_first is None else np.asarray(scene_first, np.int64)
+        self.threads = threads
+        self.dy = OracleDy(orc.lib)
+
+    def pairs(self, bodies):
+        pos = np.ascontiguousarray(bodies["position"])
+        quat = np.ascontiguousarray(bodies["orientation"])
+        off = np.full(self.n, 0.02, np.float32)
+        if self.scene_first is None:
+            pairs = self.orc.sap_pairs(self.orc.box_bounds(pos, quat, self.half, off))
+        else:
+            # one sweep per scene (scenes overlap in space but never pair)
+            parts = []
+            for a, b in zip(self.scene_first[:-1], self.scene_first[1:]):
+                if b - a < 2:
+                    continue
+                bounds = self.orc.box_bounds(pos[a:b], quat[a:b], self.half[a:b], off[a:b])
+                parts.append(self.orc.sap_pairs(bounds).astype(np.uint32) + np.uint32(a))
+            pairs = np.concatenate(parts) if parts else np.zeros((0, 2), np.uint32)
+        return pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+
+    def step(self, bodies, prev_pairs, state, dt):
+        """Returns a dict: pairs, found, lost, counts_bb, pts_bb, counts_pl,
+        pts_pl, managers, bodies (after the step), impulses, partitions."""
+        n = self.n
+        pairs = self.pairs(bodies)
+        found, lost = self.orc.found_lost(prev_pairs, pairs)
+        k_cur, k_prev = _keys64(pairs), _keys64(prev_pairs)
+        j = np.searchsorted(k_prev, k_cur)
+        jc = np.minimum(j, max(len(k_prev) - 1, 0))
+        kept = (j < len(k_prev)) & (k_prev[jc] == k_cur) if len(k_prev) else np.zeros(len(k_cur), bool)
+        src = np.where(kept, jc, -1)
+        caches = np.zeros((max(len(pairs), 1), 56), np.float32)
+        if kept.any():
+            caches[: len(pairs)][kept] = state["caches_bb"][jc[kept]]
+        pos = np.ascontiguousarray(bodies["position"])
+        quat = np.ascontiguousarray(bodies["orientation"])
+        first = np.ascontiguousarray(pairs[:, 1], np.uint32)  # Gu order: the higher slot first
+        second = np.ascontiguousarray(pairs[:, 0], np.uint32)
+        npairs = len(pairs)
+        pts_bb = np.zeros((max(npairs, 1), 4, 7), np.float32)
+        cnt_bb = np.zeros(max(npairs, 1), np.uint32)
+        fl_bb = np.zeros(max(npairs, 1), np.uint32)
+        rc = self.lib.orc_pcm_box_box_many(C.c_uint32(npairs), P(first), P(second), P(pos), P(quat), P(self.half),
+                                           C.c_float(0.04), P(caches), P(pts_bb), P(cnt_bb), P(fl_bb),
+                                           C.c_uint32(self.threads))
+        if rc != 0:
+            raise OverflowError(f"oracle PCM box/box manifold invalid at pair {-1 - rc}")
+        cpl = np.ascontiguousarray(state["caches_pl"].copy())
+        pts_pl = np.zeros((n, 4, 7), np.float32)
+        cnt_pl = np.zeros(n, np.uint32)
+        fl_pl = np.zeros(n, np.uint32)
+        fp = np.ascontiguousarray(self.statics["position"][0])
+        rc = self.lib.orc_pcm_plane_box_many(C.c_uint32(n), P(fp), P(pos), P(quat), P(self.half), C.c_float(0.04),
+                                             P(cpl), P(pts_pl), P(cnt_pl), P(fl_pl), C.c_uint32(self.threads))
+        if rc != 0:
+            raise OverflowError(f"oracle PCM plane/box manifold invalid at box {-1 - rc}")
+        pts_pl[:, :, 3:6] *= -1.0  # the scene publishes the plane's +y normal
+        # touching managers in the world's order, with their history slots
+        tb = np.nonzero(cnt_bb[:npairs])[0]
+        tp = np.nonzero(cnt_pl)[0]
+        nm = len(tb) + len(tp)
+        mg = np.zeros(nm, DY_MANAGER)
+        mg["body0"][: len(tb)] = first[tb]
+        mg["body1"][: len(tb)] = second[tb]
+        mg["count"][: len(tb)] = cnt_bb[tb]
+        mg["body0"][len(tb):] = tp
+        mg["body1"][len(tb):] = -1
+        mg["count"][len(tb):] = cnt_pl[tp]
+        pts = np.concatenate([pts_bb[tb], pts_pl[tp]]) if nm else np.zeros((0, 4, 7), np.float32)
+        live = np.arange(4)[None, :] < mg["count"][:, None]
+        pts = np.where(live[:, :, None], pts, 0.0).astype(np.float32)
+        mg["contacts"]["position"] = pts[:, :, 0:3]
+        mg["contacts"]["normal"] = pts[:, :, 3:6]
+        mg["contacts"]["separation"] = pts[:, :, 6]
+        # history: box/box slot = previous pair index, plane slot = P_prev + box
+        hslot = np.concatenate([src[tb], len(k_prev) + tp]).astype(np.int64)
+        step_no = state["step"]
+        ok = (hslot >= 0) & (step_no > 0)
+        ok[ok] &= state["stamps"][hslot[ok]] == step_no
+        hist_idx = np.full(nm, 0xFFFFFFFF, np.uint32)
+        hist_idx[ok] = np.arange(int(ok.sum()), dtype=np.uint32)
+        mg["history"] = hist_idx
+        hist = state["history"][hslot[ok]] if ok.any() else np.zeros(0, DY_PATCH)
+        b = bodies.copy()
+        counted = np.bincount(pairs.ravel(), minlength=n).astype(np.uint32) + 1
+        b["counted_interactions"] = counted
+        out, patches, imp, parts = self.dy.step(dt, b, self.statics, mg, hist)
+        return {"pairs": pairs, "found": found, "lost": lost, "counts_bb": cnt_bb[:npairs],
+                "pts_bb": pts_bb[:npairs], "counts_pl": cnt_pl, "pts_pl": pts_pl, "managers": mg, "bodies": out,
+                "impulses": imp, "partitions": parts, "patches": patches}

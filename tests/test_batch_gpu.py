@@ -90,3 +90,28 @@ def test_autotune_keeps_results(ctx, torch_cuda):
         tuned.step_graph(1 / 120)
         plain.step(1 / 120)
     np.testing.assert_array_equal(tuned.body_states(), plain.body_states())
+
+
+def test_batched_4096_scenes_spot_steps_match_oracle(ctx, orc, torch_cuda):
+    """The batched workload at full size: 4,096 scenes of U{8..512} boxes
+    (about 1.07M) in one world, graph replays between spot steps; at each
+    spot the whole world's step is compared with the oracle pipeline run
+    scene by scene from the device's synchronized state (pairs and
+    partitions bit-exact, no cross-scene pair, contacts 1e-5, impulses and
+    velocities 1e-4; world_parity.compare_step)."""
+    from oracle_bind import StepOracle
+    from world_parity import compare_step
+    counts, half, pos, quat = scenes.batch_scenes(4096)
+    first = np.concatenate([[0], np.cumsum(counts.astype(np.int64))]).astype(np.int64)
+    w = make(ctx, half, pos, quat, counts)
+    try:
+        so = StepOracle(orc, half, scene_first=first, threads=16)
+        spots = (1, 200)
+        for s in range(spots[-1] + 1):
+            if s in spots:
+                managers, _, _ = compare_step(w, so, 1 / 120, f"4096 scenes, step {s}", reports=False)
+            else:
+                w.step_graph(1 / 120)
+        assert managers > 100_000
+    finally:
+        w.close()

@@ -200,11 +200,14 @@ def test_sap_empty_and_single(ctx, torch_cuda):
     bp0.close()
 
 
-def test_sap_million_boxes_properties(ctx, torch_cuda):
+def test_sap_million_boxes_properties(ctx, orc, torch_cuda):
     """1,000,000 AABBs (the broadphase-only workload at full size), 3 frames
     of 10% moves: endpoints sorted with unique payloads, and the pair list
     evolves exactly as (old - lost) + found, with every found pair
-    overlapping and every lost pair not (checked on the host)."""
+    overlapping and every lost pair not (checked on the host); and every
+    frame's pairs, found and lost are bit-identical to the oracle's full
+    sweep and set difference (so no newly overlapping pair goes
+    unreported)."""
     from fyzz_amd import sap
     torch = torch_cuda
     n = 1_000_000
@@ -240,6 +243,14 @@ def test_sap_million_boxes_properties(ctx, torch_cuda):
                 ov = ~((b[0, a] > b[3, c]) | (b[0, c] > b[3, a]) | (b[1, a] > b[4, c]) | (b[1, c] > b[4, a]) |
                        (b[2, a] > b[5, c]) | (b[2, c] > b[5, a]))
                 assert np.all(ov == want)
+        # bit-exact against the oracle's sweep of the same poses
+        want_pairs = orc.sap_pairs(orc.box_bounds(pos, quat, half, off))
+        np.testing.assert_array_equal(pairs, want_pairs, err_msg=f"frame {f} pairs")
+        if prev is not None:
+            wf, wl = orc.found_lost(prev, want_pairs)
+            np.testing.assert_array_equal(found, wf, err_msg=f"frame {f} found")
+            np.testing.assert_array_equal(lost, wl, err_msg=f"frame {f} lost")
+            print(f"frame {f}: {len(pairs)} pairs, {len(found)} found, {len(lost)} lost, bit-exact", flush=True)
         prev = pairs
     assert len(prev) > 1000
     bp.close()
