@@ -502,6 +502,55 @@ __global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const
     if (d_lost) totals[8] += *d_lost;
 }
 
+// ---- authored edits after the first step
+
+// Scene::set_pose's wake of the body itself (scene.cpp:637-647) and
+// set_velocity / set_angular_velocity's (730-781): the wake counter is
+// raised to 0.4 s and the sleep flags cleared; a sleeping body also drops
+// its sleep filter.
+__host__ __device__ inline void wake_record(fyzz_dy_body& b) {
+    if (b.flags & 1u)
+        for (int k = 0; k < 3; ++k) b.sleep_linear_acc[k] = b.sleep_angular_acc[k] = 0.0f;
+    b.wake_counter = b.wake_counter > 0.4f ? b.wake_counter : 0.4f;
+    b.flags &= ~3u;
+}
+
+// Scene::set_pose's invalidations (scene.cpp:651-700) for the box in
+// `slot`: every current broadphase pair holding it and its plane pair
+// lose their PCM cache (ShapeInteraction::resetManagerCachedState). When
+// the pose changed, their friction history is erased as well and every
+// touching manager's other dynamic body that sleeps or is ready to sleep
+// is woken. `pairs` is the list the next step starts from, `cache_bb` and
+// `stamp` the caches and history stamps aligned with it.
+__global__ void k_author_pose(u32 slot, int changed, u32 cap, const u64* pairs, const u32* d_pairs,
+                              CacheRec* cache_bb, CacheRec* cache_pl, int has_plane, u32* stamp,
+                              const fyzz_dy_manager* managers, const u32* d_nm, u32 cap_m, fyzz_dy_body* bodies) {
+    const u32 np = d_pairs ? min(*d_pairs, cap) : 0u, nm = min(*d_nm, cap_m);
+    const u32 stride = gridDim.x * blockDim.x;
+    const u32 t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (u32 i = t0; i < np; i += stride) {
+        const u64 x = pairs[i];
+        if (static_cast<u32>(x >> 32) != slot && static_cast<u32>(x & 0xffffffffu) != slot) continue;
+        for (u32 k = 0; k < kCacheWords; ++k) cache_bb[i].w[k] = 0u;
+        if (changed) stamp[i] = kNone;
+    }
+    if (t0 == 0 && has_plane) {
+        for (u32 k = 0; k < kCacheWords; ++k) cache_pl[slot].w[k] = 0u;
+        if (changed) stamp[cap + slot] = kNone;
+    }
+    if (!changed) return;
+    for (u32 m = t0; m < nm; m += stride) {
+        const fyzz_dy_manager& M = managers[m];
+        int other = -1;
+        if (M.body0 == static_cast<int>(slot)) other = M.body1;
+        else if (M.body1 == static_cast<int>(slot)) other = M.body0;
+        if (other < 0) continue;
+        fyzz_dy_body& b = bodies[other];
+        // every writer of one peer stores the same values
+        if (b.flags & 3u) wake_record(b);
+    }
+}
+
 // Scene::add_dynamic_box mass properties (scene.cpp:408-475): mass in
 // double, the unit-density inertia tensor scaled by density in float.
 void box_mass(const float h[3], float density, float& inv_mass, float inv_inertia[3]) {
@@ -601,6 +650,17 @@ fyzz_dy_body default_body() {
     b.restitution = 0.2f;
     b.iterations = 4u | (1u << 8);
     return b;
+}
+
+void reserve_events(fyzz_hip_world* w);
+
+// Captured step graphs hold launch shapes and whether events publish.
+void drop_graphs(fyzz_hip_world* w) {
+    for (auto& g : w->gexec)
+        if (g) {
+            FYZZ_HIP_CHECK(hipGraphExecDestroy(g));
+            g = nullptr;
+        }
 }
 
 void upload(fyzz_hip_world* w) {
@@ -705,6 +765,7 @@ void upload(fyzz_hip_world* w) {
     }
     rc = fyzz_hip_dy_create(w->ctx, n, w->mgr_cap, &w->dy);
     if (rc) throw Error(rc, w->ctx->last_error);
+    if (w->notify) reserve_events(w);
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     w->uploaded = true;
 }
@@ -738,28 +799,34 @@ sap::Lists scene_broadphase(fyzz_hip_world* w) {
             w->scene_status.p, w->scene_status.p + 1, nullptr, nullptr, nullptr};
 }
 
+// The contact-event buffers, sized from the world's capacities once, before
+// any step that publishes (never inside a captured step).
+void reserve_events(fyzz_hip_world* w) {
+    if (w->ev_cap) return;
+    const u32 cap = w->pair_cap, n = w->n;
+    w->ev_cap = 2 * cap + n;
+    w->pt_cap = 4 * (cap + n);
+    w->slot_mgr.reserve(w->mgr_cap);
+    w->s1.reserve(cap);
+    w->k1.reserve(cap);
+    w->s2.reserve(std::max<u32>(n, 1));
+    w->k2.reserve(std::max<u32>(n, 1));
+    w->s3.reserve(cap);
+    w->k3.reserve(cap);
+    w->ev_counts.reserve(4);
+    w->ev_src.reserve(w->ev_cap);
+    w->ev_pts.reserve(w->ev_cap);
+    w->ev_off.reserve(w->ev_cap);
+    w->d_npts.reserve(1);
+    w->events.reserve(w->ev_cap);
+    w->points.reserve(w->pt_cap);
+}
+
 void publish_events(fyzz_hip_world* w, const sap::Lists& L, int c_prev, int c_cur) {
     using namespace fyzz_hip::world;
     hipStream_t st = w->ctx->stream;
     const u32 cap = w->pair_cap, n = w->n;
-    if (!w->ev_cap) {
-        w->ev_cap = 2 * cap + n;
-        w->pt_cap = 4 * (cap + n);
-        w->slot_mgr.reserve(w->mgr_cap);
-        w->s1.reserve(cap);
-        w->k1.reserve(cap);
-        w->s2.reserve(std::max<u32>(n, 1));
-        w->k2.reserve(std::max<u32>(n, 1));
-        w->s3.reserve(cap);
-        w->k3.reserve(cap);
-        w->ev_counts.reserve(4);
-        w->ev_src.reserve(w->ev_cap);
-        w->ev_pts.reserve(w->ev_cap);
-        w->ev_off.reserve(w->ev_cap);
-        w->d_npts.reserve(1);
-        w->events.reserve(w->ev_cap);
-        w->points.reserve(w->pt_cap);
-    }
+    reserve_events(w);
     ProfScope ps(&w->ctx->prof, "world.events", st);
     EvArgs A{cap,          L.d_cur,         L.prev,         L.d_prev,        L.lost,       L.d_lost,
              w->src.p,     w->first.p,      w->second.p,    w->counts_bb.p,  w->counts_pl.p,
@@ -921,6 +988,53 @@ sap::Lists current_lists(fyzz_hip_world* w) {
     return L;
 }
 
+// One body record of a stepped world, read and written synchronously on
+// the world's stream (authoring calls are host API calls, not step work).
+fyzz_dy_body read_body(fyzz_hip_world* w, u32 slot) {
+    fyzz_dy_body b{};
+    hipStream_t st = w->ctx->stream;
+    FYZZ_HIP_CHECK(hipMemcpyAsync(&b, w->bodies.p + slot, sizeof b, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    return b;
+}
+
+void write_body(fyzz_hip_world* w, u32 slot, const fyzz_dy_body& b) {
+    hipStream_t st = w->ctx->stream;
+    FYZZ_HIP_CHECK(hipMemcpyAsync(w->bodies.p + slot, &b, sizeof b, hipMemcpyHostToDevice, st));
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// The record a setter edits: the host copy before the first step, the
+// device's afterwards (written back by commit_body).
+fyzz_dy_body& edit_body(fyzz_hip_world* w, u32 id, fyzz_dy_body& scratch_copy, u32* slot_out) {
+    fyzz_dy_body& host = w->body(id);
+    const u32 slot = static_cast<u32>(w->slot_of_id[id]);
+    if (slot_out) *slot_out = slot;
+    if (!w->uploaded) return host;
+    scratch_copy = read_body(w, slot);
+    return scratch_copy;
+}
+
+void commit_body(fyzz_hip_world* w, u32 slot, const fyzz_dy_body& b) {
+    if (!w->uploaded) return;
+    write_body(w, slot, b);
+}
+
+// Scene::set_pose after the first step: the record, then the cache,
+// history and peer-wake pass (scene.cpp:589-700).
+void author_pose(fyzz_hip_world* w, u32 slot, bool changed) {
+    using namespace fyzz_hip::world;
+    hipStream_t st = w->ctx->stream;
+    const sap::Lists L = current_lists(w);
+    const u32 cap = w->pair_cap;
+    hipLaunchKernelGGL(k_author_pose, dim3(std::min<u32>(grid_for(std::max(cap, w->mgr_cap), 256), 1024u)), dim3(256),
+                       0, st, slot, changed ? 1 : 0, cap, L.cur, L.d_cur, w->cache_bb[w->ccur].p, w->cache_pl.p,
+                       w->has_floor ? 1 : 0, w->stamp[w->hcur].p, w->managers.p, w->d_nm.p, w->mgr_cap,
+                       w->bodies.p);
+    FYZZ_HIP_CHECK(hipGetLastError());
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+}
+
 }  // namespace
 
 extern "C" {
@@ -949,16 +1063,9 @@ int fyzz_hip_world_autotune(fyzz_hip_world* w, float dt, uint32_t trial_steps, i
         hipEvent_t a, b;
         FYZZ_HIP_CHECK(hipEventCreate(&a));
         FYZZ_HIP_CHECK(hipEventCreate(&b));
-        auto drop_graphs = [&] {
-            for (auto& g : w->gexec)
-                if (g) {
-                    FYZZ_HIP_CHECK(hipGraphExecDestroy(g));
-                    g = nullptr;
-                }
-        };
         auto trial = [&](int pb, int sb) {
             dy::set_coop_blocks(w->dy, pb, sb);
-            drop_graphs();  // graphs hold their launch shapes
+            drop_graphs(w);  // graphs hold their launch shapes
             for (u32 k = 0; k < 2; ++k) graph ? step_graph(w, dt) : step(w, dt);  // settle both graphs
             FYZZ_HIP_CHECK(hipEventRecord(a, st));
             for (u32 k = 0; k < trial_steps; ++k) graph ? step_graph(w, dt) : step(w, dt);
@@ -981,7 +1088,7 @@ int fyzz_hip_world_autotune(fyzz_hip_world* w, float dt, uint32_t trial_steps, i
             if (t < best) best = t, best_s = c;
         }
         dy::set_coop_blocks(w->dy, best_p, best_s);
-        drop_graphs();
+        drop_graphs(w);
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         int pb = 0, sb = 0;
@@ -1090,51 +1197,137 @@ int fyzz_hip_world_add_dynamic_boxes(fyzz_hip_world* w, uint32_t n, const float*
     });
 }
 
+// Scene::set_pose (scene.cpp:589-700). Before the first step the host
+// record is edited; afterwards the device record, the current pairs' PCM
+// caches, their friction history and the touching peers' wake state.
 int fyzz_hip_world_set_body_pose(fyzz_hip_world* w, uint32_t body_id, const float* position,
                                  const float* orientation) {
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
-        w->require_authoring();
         FYZZ_REQUIRE(position && orientation, kInvalidArgument, "world: null pose");
-        fyzz_dy_body& b = w->body(body_id);
+        if (w->has_floor && body_id == w->floor_id) {
+            // the admitted plane's normal is fixed to +Y and it stays put
+            FYZZ_REQUIRE(false, kLogicError, "world: the floor plane cannot be moved");
+        }
         const float* q = orientation;
-        const float len = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-        FYZZ_REQUIRE(std::isfinite(len) && len > 0.0f, kInvalidArgument, "world: orientation must be non-zero");
+        const float mag = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        bool finite = std::isfinite(mag);
+        for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(position[k]);
+        for (int k = 0; k < 4; ++k) finite = finite && std::isfinite(q[k]);
+        FYZZ_REQUIRE(finite && std::fabs(mag - 1.0f) < 1.0e-2f, kInvalidArgument, "world: rigid pose is not sane");
+        fyzz_dy_body tmp;
+        u32 slot = 0;
+        fyzz_dy_body& b = edit_body(w, body_id, tmp, &slot);
+        float nq[4] = {q[0] / mag, q[1] / mag, q[2] / mag, q[3] / mag};
+        // a translation-only edit passes the stored orientation unchanged
+        if (q[0] == b.orientation[0] && q[1] == b.orientation[1] && q[2] == b.orientation[2] &&
+            q[3] == b.orientation[3])
+            for (int k = 0; k < 4; ++k) nq[k] = b.orientation[k];
+        bool changed = false;
+        for (int k = 0; k < 3; ++k) changed = changed || b.position[k] != position[k];
+        for (int k = 0; k < 4; ++k) changed = changed || b.orientation[k] != nq[k];
+        world::wake_record(b);
         for (int k = 0; k < 3; ++k) b.position[k] = position[k];
-        for (int k = 0; k < 4; ++k) b.orientation[k] = q[k] / len;
+        for (int k = 0; k < 4; ++k) b.orientation[k] = nq[k];
+        if (!w->uploaded) return;
+        commit_body(w, slot, b);
+        author_pose(w, slot, changed);
     });
 }
 
+// Scene::set_velocity / set_angular_velocity (scene.cpp:730-781): the
+// velocity and the body's own wake; no cache is touched.
 int fyzz_hip_world_set_body_velocity(fyzz_hip_world* w, uint32_t body_id, const float* linear,
                                      const float* angular) {
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
-        w->require_authoring();
-        fyzz_dy_body& b = w->body(body_id);
+        FYZZ_REQUIRE(!(w->has_floor && body_id == w->floor_id), kLogicError, "world: static actors have no velocity");
+        for (int k = 0; k < 3; ++k) {
+            FYZZ_REQUIRE(!linear || std::isfinite(linear[k]), kInvalidArgument, "world: velocity must be finite");
+            FYZZ_REQUIRE(!angular || std::isfinite(angular[k]), kInvalidArgument,
+                         "world: angular velocity must be finite");
+        }
+        fyzz_dy_body tmp;
+        u32 slot = 0;
+        fyzz_dy_body& b = edit_body(w, body_id, tmp, &slot);
         for (int k = 0; k < 3; ++k) {
             if (linear) b.linear_velocity[k] = linear[k];
             if (angular) b.angular_velocity[k] = angular[k];
         }
+        if (linear || angular) world::wake_record(b);
+        commit_body(w, slot, b);
     });
 }
 
+// Scene::set_material -> set_attached_shape_material (scene.cpp:1022,
+// 1099-1118): finite, non-negative frictions, restitution at most 1. The
+// next step's contact prep reads the new values.
 int fyzz_hip_world_set_body_material(fyzz_hip_world* w, uint32_t body_id, float static_friction,
                                      float dynamic_friction, float restitution) {
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
-        w->require_authoring();
-        FYZZ_REQUIRE(static_friction >= 0.0f && dynamic_friction >= 0.0f && restitution >= 0.0f, kInvalidArgument,
-                     "world: material values must be non-negative");
+        FYZZ_REQUIRE(std::isfinite(static_friction) && std::isfinite(dynamic_friction) && std::isfinite(restitution) &&
+                         static_friction >= 0.0f && dynamic_friction >= 0.0f && restitution >= 0.0f &&
+                         restitution <= 1.0f,
+                     kInvalidArgument, "world: material values must be finite, non-negative, restitution <= 1");
         if (w->has_floor && body_id == w->floor_id) {
             w->floor.static_friction = static_friction;
             w->floor.dynamic_friction = dynamic_friction;
             w->floor.restitution = restitution;
+            if (w->uploaded) {
+                hipStream_t st = w->ctx->stream;
+                FYZZ_HIP_CHECK(hipMemcpyAsync(w->statics.p, &w->floor, sizeof(fyzz_dy_static), hipMemcpyHostToDevice,
+                                              st));
+                FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+            }
             return;
         }
-        fyzz_dy_body& b = w->body(body_id);
+        fyzz_dy_body tmp;
+        u32 slot = 0;
+        fyzz_dy_body& b = edit_body(w, body_id, tmp, &slot);
         b.static_friction = static_friction;
         b.dynamic_friction = dynamic_friction;
         b.restitution = restitution;
+        commit_body(w, slot, b);
+    });
+}
+
+// Scene::set_dynamics (scene.cpp:1119-1147): the reference's validation,
+// then the fields the admitted class steps with.
+int fyzz_hip_world_set_body_dynamics(fyzz_hip_world* w, uint32_t body_id, const fyzz_body_dynamics* d) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(d, kInvalidArgument, "world: null dynamics");
+        FYZZ_REQUIRE(!(w->has_floor && body_id == w->floor_id), kLogicError,
+                     "world: static actors have no dynamic solver state");
+        const bool valid = std::isfinite(d->linear_damping) && std::isfinite(d->angular_damping) &&
+                           std::isfinite(d->max_linear_velocity) && std::isfinite(d->max_angular_velocity) &&
+                           std::isfinite(d->max_depenetration_velocity) && std::isfinite(d->max_contact_impulse) &&
+                           std::isfinite(d->contact_slop_coefficient) && d->max_depenetration_velocity > 0.0f &&
+                           d->max_contact_impulse >= 0.0f && d->contact_slop_coefficient >= 0.0f &&
+                           d->linear_damping >= 0.0f && d->angular_damping >= 0.0f &&
+                           d->max_linear_velocity >= 0.0f && d->max_angular_velocity >= 0.0f &&
+                           std::isfinite(d->max_linear_velocity * d->max_linear_velocity) &&
+                           std::isfinite(d->max_angular_velocity * d->max_angular_velocity) &&
+                           d->position_iterations != 0 && (d->lock_flags & ~0x3fu) == 0;
+        FYZZ_REQUIRE(valid, kInvalidArgument, "world: rigid body dynamics invalid");
+        // outside the admitted class: the kernels step without these terms
+        FYZZ_REQUIRE(d->max_depenetration_velocity == 1.0e32f && d->max_contact_impulse == 3.40282347e38f &&
+                         d->contact_slop_coefficient == 0.0f && d->lock_flags == 0 && !d->gyroscopic_forces_enabled &&
+                         !d->retain_accelerations,
+                     kLogicError,
+                     "world: depenetration/impulse caps, slop, locks, gyroscopic forces and retained accelerations "
+                     "are outside the admitted class");
+        fyzz_dy_body tmp;
+        u32 slot = 0;
+        fyzz_dy_body& b = edit_body(w, body_id, tmp, &slot);
+        b.linear_damping = d->linear_damping;
+        b.angular_damping = d->angular_damping;
+        b.max_linear_velocity = d->max_linear_velocity;
+        b.max_angular_velocity = d->max_angular_velocity;
+        b.flags = d->gravity_enabled ? (b.flags | 4u) : (b.flags & ~4u);
+        b.iterations = static_cast<u32>(d->position_iterations) | (static_cast<u32>(d->velocity_iterations) << 8);
+        commit_body(w, slot, b);
     });
 }
 
@@ -1236,7 +1429,17 @@ int fyzz_hip_world_set_contact_notifications(fyzz_hip_world* w, int enabled) {
     return guarded(w->ctx, [&] {
         FYZZ_REQUIRE(!(enabled && w->n_scenes), kLogicError,
                      "world: contact notifications are published for single-scene worlds");
-        w->notify = enabled != 0;
+        const bool on = enabled != 0;
+        if (on == w->notify) return;
+        w->notify = on;
+        if (!w->uploaded) return;
+        // Captured steps keep the publication state they were recorded
+        // with; the buffers are reserved here, never inside a step.
+        FYZZ_HIP_CHECK(hipStreamSynchronize(w->ctx->stream));
+        drop_graphs(w);
+        // touch flags are kept every step, so the first published step
+        // still tells found from persists
+        if (on) reserve_events(w);
     });
 }
 
@@ -1281,7 +1484,7 @@ int fyzz_hip_world_set_materials(fyzz_hip_world* w, uint32_t first_id, uint32_t 
                                  const float* dynamic_friction, const float* restitution) {
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
-        w->require_authoring();
+        w->require_authoring();  // bulk form: before the first step (set_body_material works at any time)
         FYZZ_REQUIRE(n == 0 || (static_friction && dynamic_friction && restitution), kInvalidArgument,
                      "world: null material arrays");
         for (u32 k = 0; k < n; ++k) {

@@ -63,6 +63,7 @@ SIGNATURES = {
     "fyzz_hip_world_set_body_pose": (C.c_int, [vp, C.c_uint32, vp, vp]),
     "fyzz_hip_world_set_body_velocity": (C.c_int, [vp, C.c_uint32, vp, vp]),
     "fyzz_hip_world_set_body_material": (C.c_int, [vp, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+    "fyzz_hip_world_set_body_dynamics": (C.c_int, [vp, C.c_uint32, vp]),
     "fyzz_hip_world_set_scenes": (C.c_int, [vp, C.c_uint32, vp]),
     "fyzz_hip_world_set_materials": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp]),
     "fyzz_hip_world_record_dy_inputs": (C.c_int, [vp, vp, vp, C.c_uint32, u32p]),

@@ -3,7 +3,8 @@
 Mirrors the authoring and stepping surface of the reference's fyzz::World
 (include/fyzzxyzzy/world.h:400-470) for a floor plane and dynamic boxes:
 add_static_floor, add_dynamic_box, set_body_pose, set_body_velocity,
-set_body_angular_velocity, set_body_material, step, body_state.
+set_body_angular_velocity, set_body_material, set_body_dynamics, step,
+body_state (the setters work before and after the first step).
 """
 from __future__ import annotations
 
@@ -22,6 +23,19 @@ CONTACT_EVENT = np.dtype([("body0", "<u4"), ("body1", "<u4"), ("kind", "u1"), ("
 CONTACT_EVENT_POINT = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("separation", "<f4"),
                                 ("pad", "<f4"), ("normal_impulse", "<f8")])
 assert CONTACT_EVENT.itemsize == 40 and CONTACT_EVENT_POINT.itemsize == 40
+# fyzz_body_dynamics, the layout of fyzz::RigidBodyDynamics
+BODY_DYNAMICS = np.dtype([("linear_damping", "<f4"), ("angular_damping", "<f4"), ("max_linear_velocity", "<f4"),
+                          ("max_angular_velocity", "<f4"), ("max_depenetration_velocity", "<f4"),
+                          ("max_contact_impulse", "<f4"), ("contact_slop_coefficient", "<f4"),
+                          ("gravity_enabled", "u1"), ("pad0", "u1", 3), ("lock_flags", "<u4"),
+                          ("gyroscopic_forces_enabled", "u1"), ("retain_accelerations", "u1"),
+                          ("position_iterations", "u1"), ("velocity_iterations", "u1")])
+assert BODY_DYNAMICS.itemsize == 40
+BODY_DYNAMICS_DEFAULTS = {"linear_damping": 0.0, "angular_damping": 0.05, "max_linear_velocity": 1.0e16,
+                          "max_angular_velocity": 100.0, "max_depenetration_velocity": 1.0e32,
+                          "max_contact_impulse": np.finfo(np.float32).max, "contact_slop_coefficient": 0.0,
+                          "gravity_enabled": 1, "lock_flags": 0, "gyroscopic_forces_enabled": 0,
+                          "retain_accelerations": 0, "position_iterations": 4, "velocity_iterations": 1}
 
 vp = C.c_void_p
 
@@ -131,6 +145,18 @@ class World:
     def set_body_material(self, body, static_friction, dynamic_friction, restitution):
         self._call("fyzz_hip_world_set_body_material", C.c_uint32(body), C.c_float(static_friction),
                    C.c_float(dynamic_friction), C.c_float(restitution))
+
+    def set_body_dynamics(self, body, **fields):
+        """World::set_body_dynamics: RigidBodyDynamics defaults
+        (world.h:115-140) overridden by keyword."""
+        d = np.zeros(1, BODY_DYNAMICS)
+        for k, v in BODY_DYNAMICS_DEFAULTS.items():
+            d[k] = v
+        for k, v in fields.items():
+            if k not in BODY_DYNAMICS_DEFAULTS:
+                raise TypeError(f"unknown dynamics field {k}")
+            d[k] = v
+        self._call("fyzz_hip_world_set_body_dynamics", C.c_uint32(body), d.ctypes.data_as(vp))
 
     def set_scenes(self, box_counts):
         """Independent scenes: scene k owns the next box_counts[k] boxes."""

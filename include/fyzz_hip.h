@@ -346,14 +346,40 @@ int fyzz_hip_world_add_dynamic_boxes(fyzz_hip_world* world, uint32_t n, const fl
                                      const float* positions, const float* orientations, const float* linear,
                                      const float* angular, float static_friction, float dynamic_friction,
                                      float restitution, uint32_t* first_id);
-/* World::set_body_pose / set_body_velocity + set_body_angular_velocity /
- * set_body_material (world.h:475-478, 491); HOST arrays, NULL leaves a part. */
+/* World::set_body_pose / set_body_velocity + set_body_angular_velocity
+ * (world.h:475-478; Scene::set_pose, set_velocity, set_angular_velocity,
+ * scene.cpp:589-781); HOST arrays, NULL leaves a part. Allowed before and
+ * after the first step. After it, a pose edit wakes the body, resets the
+ * PCM cache of every current broadphase pair holding it and, when the
+ * pose changed, erases those pairs' friction history and wakes touching
+ * peers that sleep or are ready to sleep; a velocity edit wakes the body.
+ * Poses must be finite with |q| within 1e-2 of 1 (invalid_argument);
+ * the floor cannot move (logic_error). Synchronizes. */
 int fyzz_hip_world_set_body_pose(fyzz_hip_world* world, uint32_t body_id, const float* position,
                                  const float* orientation);
 int fyzz_hip_world_set_body_velocity(fyzz_hip_world* world, uint32_t body_id, const float* linear,
                                      const float* angular);
+/* World::set_body_material (world.h:491; scene.cpp:1022, 1099-1118):
+ * frictions finite and >= 0, restitution in [0, 1]; any time. */
 int fyzz_hip_world_set_body_material(fyzz_hip_world* world, uint32_t body_id, float static_friction,
                                      float dynamic_friction, float restitution);
+/* fyzz::RigidBodyDynamics (world.h:115-140), same fields, order and
+ * layout (40 bytes). */
+typedef struct fyzz_body_dynamics {
+    float linear_damping, angular_damping, max_linear_velocity, max_angular_velocity;
+    float max_depenetration_velocity, max_contact_impulse, contact_slop_coefficient;
+    uint8_t gravity_enabled;
+    uint8_t pad0[3];
+    uint32_t lock_flags;
+    uint8_t gyroscopic_forces_enabled, retain_accelerations, position_iterations, velocity_iterations;
+} fyzz_body_dynamics;
+/* World::set_body_dynamics (world.h:506; Scene::set_dynamics,
+ * scene.cpp:1119-1147): the reference's validation (invalid_argument),
+ * then damping, speed limits, gravity and iteration counts, any time.
+ * Non-default depenetration or impulse caps, slop, locks, gyroscopic
+ * forces or retained accelerations are outside the admitted class
+ * (logic_error). */
+int fyzz_hip_world_set_body_dynamics(fyzz_hip_world* world, uint32_t body_id, const fyzz_body_dynamics* dynamics);
 /* Independent scenes in one world (the batched form): scene k owns the
  * next box_counts[k] dynamic boxes in creation order (at most 1024 each);
  * boxes of different scenes never pair. The scenes share the floor plane

@@ -263,6 +263,23 @@ int ref_set_gravity_enabled(void* s, std::uint32_t id, int enabled) {
         sc->set_dynamics(id, d);
     });
 }
+// RigidBodyDynamics with damping, speed limits, gravity and iteration
+// counts set (f = linear damping, angular damping, max linear velocity,
+// max angular velocity); the other fields keep their defaults.
+int ref_set_dynamics(void* s, std::uint32_t id, const float* f, int gravity, int position_iterations,
+                     int velocity_iterations) {
+    return guarded([&] {
+        fyzz::RigidBodyDynamics d;
+        d.linear_damping = f[0];
+        d.angular_damping = f[1];
+        d.max_linear_velocity = f[2];
+        d.max_angular_velocity = f[3];
+        d.gravity_enabled = gravity != 0;
+        d.position_iterations = static_cast<std::uint8_t>(position_iterations);
+        d.velocity_iterations = static_cast<std::uint8_t>(velocity_iterations);
+        static_cast<Scene*>(s)->set_dynamics(id, d);
+    });
+}
 int ref_advance_collision(void* s, int force) {
     return guarded([&] { static_cast<Scene*>(s)->advance_collision(force != 0); });
 }

@@ -8,11 +8,12 @@ from fyzz_amd import scenes
 from fyzz_amd.dy import DY_MANAGER, DY_STATIC, default_bodies
 
 
-def random_case(seed: int, nb: int = 2048, n_dd: int = 4000, hub: int = 70, n_static: int = 1500):
+def random_case(seed: int, nb: int = 2048, n_dd: int = 4000, hub: int = 70, n_static: int = 1500,
+                hubs: int = 1):
     """Bodies with random boxes, poses and velocities; n_dd dynamic/dynamic
-    managers (body pairs drawn at random), `hub` extra managers on body 0
-    (more than 32: forces an overflow wave) and n_static dynamic/static
-    managers against two statics. Each manager has 1-4 contacts sharing
+    managers (body pairs drawn at random), `hub` extra managers on each of
+    bodies 0..hubs-1 (more than 32: forces overflow waves) and n_static
+    dynamic/static managers against two statics. Each manager has 1-4 contacts sharing
     one normal; separations in [-0.03, 0.03]."""
     rng = np.random.default_rng(seed)
     half = (0.2 + 0.8 * scenes.uniform(seed, 0, 3 * nb).reshape(nb, 3)).astype(np.float32)
@@ -35,10 +36,11 @@ def random_case(seed: int, nb: int = 2048, n_dd: int = 4000, hub: int = 70, n_st
     st["position"][1] = [5.0, -1.0, 5.0]
     pairs = []
     while len(pairs) < n_dd:
-        a, c = rng.integers(1, nb, 2)
+        a, c = rng.integers(hubs, nb, 2)
         if a != c:
             pairs.append((int(a), int(c)))
-    pairs += [(0, int(k)) for k in rng.choice(np.arange(1, nb), hub, replace=False)]
+    for h in range(hubs):
+        pairs += [(h, int(k)) for k in rng.choice(np.arange(hubs, nb), hub, replace=False)]
     pairs += [(int(rng.integers(0, nb)), -1 - int(rng.integers(0, 2))) for _ in range(n_static)]
     order = rng.permutation(len(pairs))
     mg = np.zeros(len(pairs), DY_MANAGER)

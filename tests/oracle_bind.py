@@ -457,6 +457,75 @@ class OracleWorld:
         self.last_keys = keys
         return out
 
+    # ---- authored edits between steps, as the reference's Scene setters
+    # make them (scene.cpp:589-781, 1022-1147); i is a box slot
+    def _wake(self, i):
+        """The body's own wake: wake counter raised to 0.4 s, sleep flags
+        cleared, a sleeping body's sleep filter dropped."""
+        b = self.bodies
+        if b["flags"][i] & 1:
+            b["sleep_linear_acc"][i] = 0.0
+            b["sleep_angular_acc"][i] = 0.0
+        b["wake_counter"][i] = max(np.float32(b["wake_counter"][i]), np.float32(0.4))
+        b["flags"][i] = np.uint32(b["flags"][i]) & np.uint32(~3 & 0xFFFFFFFF)
+
+    def set_pose(self, i, position, orientation):
+        """Scene::set_pose: normalized orientation (a translation-only edit
+        passes the stored one back unchanged), the body's wake, every
+        current pair's PCM cache reset; when the pose changed, the pairs'
+        friction history erased and touching peers that sleep or are ready
+        to sleep woken (scene.cpp:589-700)."""
+        b = self.bodies
+        p = np.asarray(position, np.float32)
+        q = np.asarray(orientation, np.float32)
+        mag = np.sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3])
+        nq = (q / mag).astype(np.float32)
+        if np.array_equal(q, b["orientation"][i]):
+            nq = b["orientation"][i].copy()
+        changed = not (np.array_equal(p, b["position"][i]) and np.array_equal(nq, b["orientation"][i]))
+        self._wake(i)
+        for key in list(self.bb_caches):
+            if i in key[1:]:
+                self.bb_caches[key] = self.pcm.new_cache()
+        self.pl_caches[i] = self.pcm.new_cache()
+        if changed:
+            for key in list(self.history):
+                if (key[0] == "bb" and i in key[1:]) or key == ("pl", i):
+                    del self.history[key]
+            if self.last is not None:
+                for m in self.last[1]:
+                    other = m["body1"] if m["body0"] == i else m["body0"] if m["body1"] == i else -1
+                    if other >= 0 and (b["flags"][other] & 3):
+                        self._wake(other)
+        b["position"][i] = p
+        b["orientation"][i] = nq
+
+    def set_velocity(self, i, linear=None, angular=None):
+        """Scene::set_velocity / set_angular_velocity: the value and the
+        body's wake (scene.cpp:730-781)."""
+        if linear is not None:
+            self.bodies["linear_velocity"][i] = np.asarray(linear, np.float32)
+        if angular is not None:
+            self.bodies["angular_velocity"][i] = np.asarray(angular, np.float32)
+        self._wake(i)
+
+    def set_material(self, i, static_friction, dynamic_friction, restitution):
+        b = self.bodies
+        b["static_friction"][i] = static_friction
+        b["dynamic_friction"][i] = dynamic_friction
+        b["restitution"][i] = restitution
+
+    def set_dynamics(self, i, linear_damping, angular_damping, max_linear, max_angular, gravity,
+                     position_iterations=4, velocity_iterations=1):
+        b = self.bodies
+        b["linear_damping"][i] = linear_damping
+        b["angular_damping"][i] = angular_damping
+        b["max_linear_velocity"][i] = max_linear
+        b["max_angular_velocity"][i] = max_angular
+        f = int(b["flags"][i])
+        b["flags"][i] = (f | 4) if gravity else (f & ~4)
+        b["iterations"][i] = position_iterations | (velocity_iterations << 8)
+
     def shape_pair(self, key):
         """Canonical shape-id pair of a manager key (floor = body 0, box
         slot s = body s + 1, as the world tests author them)."""
