@@ -3,6 +3,7 @@
 
 #include "../../include/fyzz_hip.h"
 #include "context.hpp"
+#include "stages.hpp"
 
 using namespace fyzz_hip;
 
@@ -18,7 +19,7 @@ int fyzz_hip_context_create(int device, fyzz_hip_context** out) {
     if (device < 0 || device >= count) return kInvalidArgument;
     auto ctx = std::make_unique<fyzz_hip_context>();
     ctx->device = device;
-    const int rc = guarded(ctx.get(), [] {});
+    const int rc = guarded(ctx.get(), [&] { sap::bind_fixed_workspace(device); });
     if (rc) return rc;
     *out = ctx.release();
     return kOk;

@@ -21,6 +21,7 @@
 // order is shape-id order and every tie-break matches the reference.
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1268,5 +1269,85 @@ int fyzz_hip_sap_endpoints(fyzz_hip_sap* sap, uint32_t* values, uint32_t* payloa
         FYZZ_HIP_CHECK(hipMemcpyAsync(payloads, s.sorted_val[s.cur].p, bytes, hipMemcpyDeviceToDevice, s.ctx->stream));
     });
 }
+
+}  // extern "C"
+
+// ---- endpoint sort with a library-held workspace (no allocation) -------
+//
+// fyzz_hip_sap_sort_endpoints_fixed runs the same radix sort as
+// fyzz_hip_sap_sort_endpoints with its workspace in `scratch`, a
+// file-scope device array: every device the library runs on holds one
+// copy until the process ends. Host code never passes `scratch` itself
+// (on the host it names a stand-in, not device memory): the device
+// address of each device's copy is looked up once per device, with the
+// device current, when a context is created on it, and kept in
+// g_fixed_scratch.
+namespace fyzz_hip::sap {
+constexpr u32 kFixedSortMaxBoxes = 131072;
+constexpr std::size_t kFixedSortKeys = 2ull * kFixedSortMaxBoxes;  // endpoints per axis
+constexpr std::size_t kFixedSortWords =
+    2ull * 3 * kFixedSortKeys + 3ull * radix::kBins * ((kFixedSortKeys + radix::kTile - 1) / radix::kTile) +
+    3ull * radix::kBins;
+constexpr int kMaxDevices = 64;
+}  // namespace fyzz_hip::sap
+
+static __device__ __align__(256) fyzz_hip::u32 scratch[fyzz_hip::sap::kFixedSortWords];
+
+namespace fyzz_hip::sap {
+namespace {
+std::mutex g_fixed_mu;
+u32* g_fixed_scratch[kMaxDevices] = {};
+}  // namespace
+
+// Called with `device` current (context creation). Idempotent.
+void bind_fixed_workspace(int device) {
+    FYZZ_REQUIRE(device >= 0 && device < kMaxDevices, kInvalidArgument, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_fixed_mu);
+    if (g_fixed_scratch[device]) return;
+    void* p = nullptr;
+    FYZZ_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FYZZ_REQUIRE(p != nullptr, kDeviceError, "workspace symbol has no device address");
+    g_fixed_scratch[device] = static_cast<u32*>(p);
+}
+
+u32* fixed_workspace(int device) {
+    FYZZ_REQUIRE(device >= 0 && device < kMaxDevices, kInvalidArgument, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_fixed_mu);
+    u32* p = g_fixed_scratch[device];
+    FYZZ_REQUIRE(p != nullptr, kLogicError, "workspace not bound on this device (create a context first)");
+    return p;
+}
+}  // namespace fyzz_hip::sap
+
+extern "C" {
+
+int fyzz_hip_sap_sort_endpoints_fixed(fyzz_hip_context* ctx, uint32_t n, const uint32_t* bounds, uint32_t* values,
+                                      uint32_t* payloads) {
+    return guarded(ctx, [&] {
+        FYZZ_REQUIRE(n <= sap::kFixedSortMaxBoxes, kInvalidArgument,
+                     "sap_sort_endpoints_fixed: n exceeds the fixed workspace (" +
+                         std::to_string(sap::kFixedSortMaxBoxes) + " boxes)");
+        FYZZ_REQUIRE(n == 0 || (bounds && values && payloads), kInvalidArgument,
+                     "sap_sort_endpoints_fixed: null device pointer");
+        if (!n) return;
+        u32* ws = sap::fixed_workspace(ctx->device);
+        const u32 n2 = 2 * n;
+        radix::Batch b{n2, 3, n2, n2, nullptr, bounds, n};
+        const std::size_t need = 2ull * 3 * n2 + radix::count_words(b) + 3 * radix::kBins;
+        FYZZ_REQUIRE(need <= sap::kFixedSortWords, kLogicError, "sap_sort_endpoints_fixed: workspace sizing");
+        u32* tk = ws;
+        u32* tv = tk + 3ull * n2;
+        u32* counts = tv + 3ull * n2;
+        u32* totals = counts + radix::count_words(b);
+        const bool in_a = radix::sort<u32>(ctx->stream, b, radix::Source::SapPlanes, nullptr, nullptr, values,
+                                           payloads, tk, tv, 32, counts, totals);
+        if (!in_a) {
+            FYZZ_HIP_CHECK(hipMemcpyAsync(values, tk, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, ctx->stream));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(payloads, tv, sizeof(u32) * 3 * n2, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    });
+}
+
+uint32_t fyzz_hip_sap_sort_fixed_max_boxes(void) { return fyzz_hip::sap::kFixedSortMaxBoxes; }
 
 }  // extern "C"

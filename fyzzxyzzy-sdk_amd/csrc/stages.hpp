@@ -30,6 +30,9 @@ bool built(fyzz_hip_sap* h);
 // The host-side bookkeeping of one incremental advance (frame stamp,
 // current buffer), for a replayed graph that recorded such an advance.
 void replay_advance(fyzz_hip_sap* h);
+// Looks up this device's copy of the fixed sort workspace (call with the
+// device current; idempotent).
+void bind_fixed_workspace(int device);
 }  // namespace sap
 
 namespace pcm {

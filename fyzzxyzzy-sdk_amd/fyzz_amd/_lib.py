@@ -30,6 +30,8 @@ SIGNATURES = {
     "fyzz_hip_profile_read": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_double), u64p]),
     "fyzz_hip_sap_box_bounds": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp]),
     "fyzz_hip_sap_sort_endpoints": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, C.c_uint64]),
+    "fyzz_hip_sap_sort_endpoints_fixed": (C.c_int, [vp, C.c_uint32, vp, vp, vp]),
+    "fyzz_hip_sap_sort_fixed_max_boxes": (C.c_uint32, []),
     "fyzz_hip_sap_sort_scratch_words": (C.c_uint64, [C.c_uint32]),
     "fyzz_hip_sap_create": (C.c_int, [vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
     "fyzz_hip_sap_destroy": (C.c_int, [vp]),

@@ -43,6 +43,17 @@ def sort_endpoints(ctx: Context, bounds):
     return vals, pays
 
 
+def sort_endpoints_fixed(ctx: Context, bounds):
+    """As sort_endpoints, through the entry point that allocates nothing
+    (its workspace is the library's fixed per-device array)."""
+    import torch
+    n = bounds.shape[1]
+    vals = torch.empty((3, 2 * n), dtype=torch.int32, device=bounds.device)
+    pays = torch.empty((3, 2 * n), dtype=torch.int32, device=bounds.device)
+    ctx.call("fyzz_hip_sap_sort_endpoints_fixed", C.c_uint32(n), _ptr(bounds), _ptr(vals), _ptr(pays))
+    return vals, pays
+
+
 class SapBroadphase:
     """Incremental SAP over n boxes in slot order (increasing shape id)."""
 

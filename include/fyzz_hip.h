@@ -171,6 +171,20 @@ int fyzz_hip_sap_sort_endpoints(fyzz_hip_context* ctx, uint32_t n, const uint32_
                                 uint64_t scratch_words);
 uint64_t fyzz_hip_sap_sort_scratch_words(uint32_t n);
 
+/* No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same inputs, outputs and order as
+ * fyzz_hip_sap_sort_endpoints, with the workspace held by the library:
+ * one fixed array per device (6.5 MB, kept until the process ends),
+ * looked up when a context is created on that device. Allocates nothing.
+ * n above fyzz_hip_sap_sort_fixed_max_boxes() (131,072 boxes) returns
+ * FYZZ_HIP_E_INVALID_ARGUMENT. Every call on one device shares that
+ * workspace: calls that overlap on one device (other streams, other
+ * threads) race. */
+int fyzz_hip_sap_sort_endpoints_fixed(fyzz_hip_context* ctx, uint32_t n, const uint32_t* bounds,
+                                      uint32_t* values, uint32_t* payloads);
+uint32_t fyzz_hip_sap_sort_fixed_max_boxes(void);
+
 /* ---- incremental broadphase (GpuBroadphaseProducer::advance) -------- */
 /* n boxes in slot order (increasing shape id); body_ids[n] and dynamic[n]
  * are HOST arrays (Bp filter: pairs need different bodies and at least one
