@@ -1209,11 +1209,13 @@ struct BodyState {
     V v, w;
 };
 // The first word is polled alone (waiting costs one load per poll); once
-// it carries the count, the other five are loaded and checked.
-__device__ inline bool poll_state(const u64* q, u32 need, BodyState& s) {
+// it carries the count, the other five are loaded and checked. seen: the
+// count the first word carried (the updates the body has had so far).
+__device__ inline bool poll_state(const u64* q, u32 need, BodyState& s, u32& seen) {
     u64 x[6];
     x[0] = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (static_cast<u32>(x[0] >> 32) != need) return false;
+    seen = static_cast<u32>(x[0] >> 32);
+    if (seen != need) return false;
 #pragma unroll
     for (int k = 1; k < 6; ++k) x[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     bool ok = true;
@@ -1231,6 +1233,16 @@ __device__ inline void publish_state(u64* q, u32 count, const V& v, const V& w) 
 #pragma unroll
     for (int k = 0; k < 6; ++k) __hip_atomic_store(q + k, hi | __float_as_uint(c[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Clock cycles before a body still (need - seen) updates short of a
+// waiting manager is polled again: about one solve hop per missing update
+// (kPollHopCycles), at most 32 hops; a body one update short (or a torn
+// read, seen == need) is polled on the next pass.
+constexpr u32 kPollHopCycles = 1024;
+__device__ inline u64 poll_delay(u32 need, u32 seen) {
+    const u32 d = need - seen;  // counts only grow up to need
+    return d <= 1u ? 0ull : static_cast<u64>(min(d - 1u, 32u)) * kPollHopCycles;
+}
+
 // One manager's rows of one iteration (scene.cpp:8853-9148). info: (body0,
 // body1, manager) of a partition-sorted position. Rows and patches are read
 // with plain loads: the same thread solves the same manager in every
@@ -1373,6 +1385,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
     const u32 total = pos_iters + vel_iters;
     const u32 n = np ? W.part_off[np] : 0u;
     u32 iter = 0, i = tid, spins = 0, idle = 0;
+    u64 wake0 = 0, wake1 = 0;  // earliest clock at which each body is polled again
     bool have = false, ready0 = false, ready1 = false;
     int4 info = make_int4(-1, -1, 0, 0);
     uint4 rk = make_uint4(0u, 0u, 0u, 0u);
@@ -1403,6 +1416,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 have = true;
                 ready0 = info.x < 0;
                 ready1 = info.y < 0;
+                wake0 = wake1 = 0;
                 s0 = BodyState{};
                 s1 = BodyState{};
                 // inverse masses: constant through the sweep
@@ -1419,10 +1433,26 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 }
             }
             // a body whose count reached the awaited value stays there
-            // until this manager runs: its state is kept, not polled again
+            // until this manager runs: its state is kept, not polled again.
+            // A body still `need - seen` updates short is polled again only
+            // after about that many solve hops: waiting lanes would
+            // otherwise keep the memory system busy with polls that cannot
+            // succeed, slowing the hops they wait for.
             const u32 need0 = iter * rk.z + rk.x, need1 = iter * rk.w + rk.y;
-            if (!ready0) ready0 = poll_state(W.vst + 6ull * info.x, need0, s0);
-            if (!ready1) ready1 = poll_state(W.vst + 6ull * info.y, need1, s1);
+            const u64 now = clock64();
+            bool polled = false;
+            if (!ready0 && now >= wake0) {
+                u32 seen = 0;
+                ready0 = poll_state(W.vst + 6ull * info.x, need0, s0, seen);
+                if (!ready0) wake0 = now + poll_delay(need0, seen);
+                polled = true;
+            }
+            if (!ready1 && now >= wake1) {
+                u32 seen = 0;
+                ready1 = poll_state(W.vst + 6ull * info.y, need1, s1, seen);
+                if (!ready1) wake1 = now + poll_delay(need1, seen);
+                polled = true;
+            }
             if (ready0 && ready1) {
                 const bool snap = iter + 1 == pos_iters, last = iter + 1 == total;
                 const bool end0 = rk.x + 1 == rk.z, end1 = rk.y + 1 == rk.w;
@@ -1432,7 +1462,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 i += nthreads;
                 progress = true;
                 spins = 0;  // the budget bounds one wait, not the thread's whole sweep
-            } else if (++spins > kSpinLimit) {
+            } else if (polled && ++spins > kSpinLimit) {
                 raise_flag(W, FYZZ_DY_LOGIC);
                 iter = total;  // give up on this thread's work
             }
@@ -1829,6 +1859,41 @@ __global__ __launch_bounds__(kBlock) void k_outputs(Ws W) {
     }
 }
 
+// Scene::write_back_contacts' friction part (scene.cpp:9339-9388) for
+// the last step's managers: each anchor's world position on body0 (the
+// manager's first body, whose frame holds the patch's body0 anchors: the
+// mass pose the rows were prepared with, centre + rotated local anchor)
+// and its world impulse, the two tangent rows' tangent x applied force
+// summed in double.
+__global__ __launch_bounds__(kBlock) void k_anchors(Ws W, fyzz_friction_anchors* out) {
+    const u32 m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= mcount(W)) return;
+    fyzz_friction_anchors r{};
+    if (W.with_rows && W.part[m] < kMaxParts) {
+        const fyzz_dy_patch& P = W.patches[m];
+        const u32 na = min(P.anchor_count, 2u);
+        r.anchor_count = na;
+        r.broken = P.flags & 1u;
+        const int b0 = W.mg[m].body0;
+        const float4 cp = W.rpos[b0 >= 0 ? b0 : 0];
+        const float4 cq = W.rq[b0 >= 0 ? b0 : 0];
+        const Q q{cq.x, cq.y, cq.z, cq.w};
+        for (u32 k = 0; k < na; ++k) {
+            const V local{P.body0_anchors[k][0], P.body0_anchors[k][1], P.body0_anchors[k][2]};
+            const V rot = srot(q, local);
+            r.position[k][0] = cp.x + rot.x;
+            r.position[k][1] = cp.y + rot.y;
+            r.position[k][2] = cp.z + rot.z;
+            const FRow& x = W.frow[4 * m + 2 * k];
+            const FRow& y = W.frow[4 * m + 2 * k + 1];
+            r.impulse[k][0] = static_cast<double>(x.tx) * x.applied + static_cast<double>(y.tx) * y.applied;
+            r.impulse[k][1] = static_cast<double>(x.ty) * x.applied + static_cast<double>(y.ty) * y.applied;
+            r.impulse[k][2] = static_cast<double>(x.tz) * x.applied + static_cast<double>(y.tz) * y.applied;
+        }
+    }
+    out[m] = r;
+}
+
 }  // namespace fyzz_hip::dy
 
 using namespace fyzz_hip;
@@ -2216,6 +2281,17 @@ int fyzz_hip_dy_adjoint(fyzz_hip_dy* d, const float* adj_v_out, const float* adj
                         float* adj_v_in, float* adj_w_in, float* adj_friction) {
     if (!d) return kInvalidArgument;
     return guarded(d->ctx, [&] { dy_adjoint(d, adj_v_out, adj_w_out, adj_p_out, adj_v_in, adj_w_in, adj_friction); });
+}
+
+int fyzz_hip_dy_friction_anchors(fyzz_hip_dy* d, fyzz_friction_anchors* out) {
+    if (!d) return kInvalidArgument;
+    return guarded(d->ctx, [&] {
+        FYZZ_REQUIRE(out || d->ws.nm == 0, kInvalidArgument, "dy: null anchor output");
+        if (!d->ws.nm) return;
+        hipLaunchKernelGGL(dy::k_anchors, dim3(grid_for(d->ws.nm, dy::kBlock)), dim3(dy::kBlock), 0, d->ctx->stream,
+                           d->ws, out);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
 }
 
 int fyzz_hip_dy_status(fyzz_hip_dy* d, uint32_t* flags, uint32_t* n_partitions, uint32_t* rounds) {

@@ -596,6 +596,7 @@ struct fyzz_hip_world {
     DevBuf<world::CacheRec> cache_bb[2], cache_pl;
     DevBuf<fyzz_dy_manager> managers;
     DevBuf<fyzz_dy_patch> patches, hist[2];
+    DevBuf<fyzz_friction_anchors> anchors;  // friction write-back, filled on demand
     DevBuf<u64> totals;
     DevBuf<u32> d_step;
     // graph stepping: one graph per parity of the double-buffer toggles
@@ -1407,6 +1408,29 @@ int fyzz_hip_world_fetch_managers(fyzz_hip_world* w, fyzz_dy_manager* managers, 
             FYZZ_HIP_CHECK(hipMemcpyAsync(impulses, w->impulses.p, sizeof(float) * 4 * m, hipMemcpyDeviceToHost, st));
         if (partitions)
             FYZZ_HIP_CHECK(hipMemcpyAsync(partitions, w->parts.p, sizeof(u32) * m, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int fyzz_hip_world_friction_anchors(fyzz_hip_world* w, fyzz_friction_anchors* out, uint32_t capacity,
+                                    uint32_t* count) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        u32 nm = 0;
+        hipStream_t st = w->ctx->stream;
+        if (w->uploaded && w->n) {
+            FYZZ_HIP_CHECK(hipMemcpyAsync(&nm, w->d_nm.p, 4, hipMemcpyDeviceToHost, st));
+            FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        if (count) *count = nm;
+        const u32 m = std::min(nm, capacity);
+        if (!m) return;
+        FYZZ_REQUIRE(out, kInvalidArgument, "world: null anchor output");
+        w->anchors.reserve(w->mgr_cap);  // first use only: outside any step
+        const int rc = fyzz_hip_dy_friction_anchors(w->dy, w->anchors.p);
+        if (rc) throw Error(rc, w->ctx->last_error);
+        FYZZ_HIP_CHECK(hipMemcpyAsync(out, w->anchors.p, sizeof(fyzz_friction_anchors) * m, hipMemcpyDeviceToHost,
+                                      st));
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     });
 }

@@ -54,6 +54,7 @@ SIGNATURES = {
     "fyzz_hip_dy_step": (C.c_int, [vp, C.c_float, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_uint32,
                                    vp, vp, vp]),
     "fyzz_hip_dy_status": (C.c_int, [vp, u32p, u32p, u32p]),
+    "fyzz_hip_dy_friction_anchors": (C.c_int, [vp, vp]),
     "fyzz_hip_dy_set_tape": (C.c_int, [vp, C.c_int]),
     "fyzz_hip_dy_adjoint": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "fyzz_hip_world_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
@@ -76,6 +77,7 @@ SIGNATURES = {
     "fyzz_hip_world_body_records": (C.c_int, [vp, vp, C.c_uint32, u32p]),
     "fyzz_hip_world_stats": (C.c_int, [vp, u32p, u32p, u32p, u32p]),
     "fyzz_hip_world_fetch_managers": (C.c_int, [vp, vp, vp, vp, C.c_uint32, u32p]),
+    "fyzz_hip_world_friction_anchors": (C.c_int, [vp, vp, C.c_uint32, vp]),
     "fyzz_hip_world_totals": (C.c_int, [vp, u64p, C.c_int]),
     "fyzz_hip_world_set_contact_notifications": (C.c_int, [vp, C.c_int]),
     "fyzz_hip_world_contact_events": (C.c_int, [vp, vp, C.c_uint32, u32p, vp, C.c_uint32, u32p]),

@@ -30,6 +30,10 @@ DY_MANAGER = np.dtype([("body0", "<i4"), ("body1", "<i4"), ("count", "<u4"), ("h
 DY_PATCH = np.dtype([("body0_normal", "<f4", 3), ("body1_normal", "<f4", 3), ("body0_anchors", "<f4", (2, 3)),
                      ("body1_anchors", "<f4", (2, 3)), ("static_friction", "<f4"), ("dynamic_friction", "<f4"),
                      ("restitution", "<f4"), ("damping", "<f4"), ("anchor_count", "<u4"), ("flags", "<u4")])
+# fyzz_friction_anchors: Scene::FrictionPatchWriteback per manager
+FRICTION_ANCHORS = np.dtype([("anchor_count", "<u4"), ("broken", "<u4"), ("position", "<f4", (2, 3)),
+                             ("impulse", "<f8", (2, 3))])
+assert FRICTION_ANCHORS.itemsize == 80
 assert DY_BODY.itemsize == 144 and DY_STATIC.itemsize == 40
 assert DY_MANAGER.itemsize == 128 and DY_PATCH.itemsize == 96
 NO_HISTORY = 0xFFFFFFFF
@@ -115,6 +119,16 @@ class DySolver:
                    C.c_uint32(statics.shape[0]), _p(statics), C.c_uint32(nm), _p(managers), _p(history),
                    C.c_uint32(nh), _p(patches), _p(impulses), _p(partitions))
         return patches[:nm], impulses[:nm], partitions[:nm]
+
+    def friction_anchors(self, n_managers):
+        """The last step's friction write-back (one FRICTION_ANCHORS record
+        per manager), computed from its solved rows; the patches tensor that
+        step wrote must still be alive. Host copy."""
+        import torch
+        out = torch.zeros((max(n_managers, 1), FRICTION_ANCHORS.itemsize), dtype=torch.uint8,
+                          device=torch.device("cuda", 0))
+        self._call("fyzz_hip_dy_friction_anchors", _p(out))
+        return from_device(out, FRICTION_ANCHORS)[:n_managers]
 
     def set_tape(self, enabled=True):
         self._call("fyzz_hip_dy_set_tape", C.c_int(1 if enabled else 0))

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import Context, FyzzError, lib
-from .dy import DY_BODY, DY_MANAGER
+from .dy import DY_BODY, DY_MANAGER, FRICTION_ANCHORS
 
 # fyzz_contact_event / fyzz_contact_event_point (include/fyzz_hip.h), the
 # layouts of fyzz::ContactEvent / ContactEventPoint
@@ -246,6 +246,15 @@ class World:
                    cpl.ctypes.data_as(vp), hist.ctypes.data_as(vp), stamps.ctypes.data_as(vp), C.byref(step))
         assert np_.value == p
         return {"caches_bb": cbb[:p], "caches_pl": cpl[:n], "history": hist, "stamps": stamps, "step": step.value}
+
+    def friction_anchors(self):
+        """The last step's friction write-back (Scene::solved_friction_patches),
+        one FRICTION_ANCHORS record per manager in manager order."""
+        cnt = C.c_uint32()
+        self._call("fyzz_hip_world_friction_anchors", None, C.c_uint32(0), C.byref(cnt))
+        out = np.zeros(max(cnt.value, 1), FRICTION_ANCHORS)
+        self._call("fyzz_hip_world_friction_anchors", out.ctypes.data_as(vp), C.c_uint32(len(out)), C.byref(cnt))
+        return out[:cnt.value]
 
     def managers(self):
         """(managers, impulses[n, 4], partitions[n]) of the last step, host copies."""

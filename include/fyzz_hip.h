@@ -119,6 +119,18 @@ typedef struct fyzz_contact_event_point {
     double normal_impulse;
 } fyzz_contact_event_point; /* 40 bytes */
 
+/* Scene::FrictionPatchWriteback (scene.h:522-528), per touching manager:
+ * its patch's anchor count and broken flag, each anchor's world position
+ * on body0 (the manager's first body) and world friction impulse (the two
+ * tangent rows' tangent x applied force, summed in double), as
+ * Scene::write_back_contacts publishes them (scene.cpp:9339-9388). */
+typedef struct fyzz_friction_anchors {
+    uint32_t anchor_count;
+    uint32_t broken;
+    float position[2][3];
+    double impulse[2][3];
+} fyzz_friction_anchors; /* 80 bytes */
+
 /* ---- context -------------------------------------------------------- */
 /* Stands in for the producer's owned compute::Context + stream
  * (source_sc_owned_gpu_broadphase_native.h:66-162). */
@@ -331,6 +343,11 @@ int fyzz_hip_dy_adjoint(fyzz_hip_dy* dy, const float* adj_v_out, const float* ad
  * round-by-round colouring needs this many rounds). Returns the status matching the
  * first raised flag (the exception the reference would throw). */
 int fyzz_hip_dy_status(fyzz_hip_dy* dy, uint32_t* flags, uint32_t* n_partitions, uint32_t* rounds);
+/* The friction write-back of the last step's managers (Scene::
+ * write_back_contacts, scene.cpp:9339-9388) into DEVICE out[n_managers],
+ * computed on demand from the step's solved rows and the patches array
+ * the step wrote (which must still hold them). Asynchronous. */
+int fyzz_hip_dy_friction_anchors(fyzz_hip_dy* dy, fyzz_friction_anchors* out);
 
 /* ---- Step pipeline (fyzz::World over the native GPU backend) --------- */
 /* A world of one static floor plane and dynamic boxes whose whole step
@@ -433,6 +450,12 @@ int fyzz_hip_world_stats(fyzz_hip_world* world, uint32_t* n_pairs, uint32_t* n_m
  * entries (any may be NULL); *count = number of managers. */
 int fyzz_hip_world_fetch_managers(fyzz_hip_world* world, fyzz_dy_manager* managers, float* impulses,
                                   uint32_t* partitions, uint32_t capacity, uint32_t* count);
+/* Scene::solved_friction_patches (scene.h:846-848): the last step's
+ * friction write-back, one record per manager in the manager list's
+ * order, into a HOST array of `capacity`; *count = number of managers.
+ * Computed on demand (nothing is written while stepping); synchronizes. */
+int fyzz_hip_world_friction_anchors(fyzz_hip_world* world, fyzz_friction_anchors* out, uint32_t capacity,
+                                    uint32_t* count);
 /* Synchronizes; running totals since creation or the last reset: [0]
  * steps, [1] broadphase pairs, [2] touching managers, [3] contact rows,
  * [4] friction rows, [5] partitions, [6] solver iterations, [7] partition

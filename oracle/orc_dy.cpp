@@ -725,6 +725,29 @@ struct Dy {
             for (std::uint32_t q = 0; q < mgr[m].count; ++q)
                 impulses_out[4 * m + patch_order_[m][q]] = float(crow[m][q].applied);
             if (parts_out) parts_out[m] = static_cast<std::uint32_t>(part[m]);
+            // friction write-back (Scene::write_back_contacts, scene.cpp:
+            // 9339-9388): anchors on body0's mass pose at the step's start,
+            // impulses as double sums of tangent x applied force
+            if (anchors_out_) {
+                fyzz_friction_anchors& A = anchors_out_[m];
+                std::memset(&A, 0, sizeof A);
+                A.anchor_count = P.anchors;
+                A.broken = P.broken ? 1u : 0u;
+                const BodyView a0 = view(mgr[m].body0);
+                const Q mq = mass_orientation(a0.q);
+                const V c0 = mass_position(a0.pos, a0.q);
+                for (std::uint32_t k = 0; k < P.anchors && k < 2; ++k) {
+                    const V r = orc::rotate(mq, P.a0[k]);
+                    A.position[k][0] = float(c0.x + r.x);
+                    A.position[k][1] = float(c0.y + r.y);
+                    A.position[k][2] = float(c0.z + r.z);
+                    const FrictionRow& x = frows[P.row_offset + 2 * k];
+                    const FrictionRow& y = frows[P.row_offset + 2 * k + 1];
+                    A.impulse[k][0] = double(x.t.x) * double(x.applied) + double(y.t.x) * double(y.applied);
+                    A.impulse[k][1] = double(x.t.y) * double(x.applied) + double(y.t.y) * double(y.applied);
+                    A.impulse[k][2] = double(x.t.z) * double(x.applied) + double(y.t.z) * double(y.applied);
+                }
+            }
         }
 
         // ---- publish + integrate + sleep check (scene.cpp:9509-9549, 10508-10707)
@@ -823,6 +846,8 @@ struct Dy {
         return 0;
     }
     static inline thread_local std::vector<std::array<std::uint32_t, 4>> patch_order_;
+    // optional friction write-back output (one record per manager)
+    static inline thread_local fyzz_friction_anchors* anchors_out_ = nullptr;
 
     // ---- frozen-row rollout (the differentiable form's reference)
     struct FrozenBody {
@@ -1307,6 +1332,20 @@ int orc_dy_step(float dt, std::uint32_t nb, fyzz_dy_body* bodies, std::uint32_t 
     Dy<float>::patch_order_.assign(nm, {});
     return Dy<float>::step(dt, nb, bodies, ns, statics, nm, managers, history, nh, patches_out, impulses_out,
                            parts_out, nullptr);
+}
+
+// orc_dy_step plus the friction write-back of the step's managers into
+// anchors_out[nm] (Scene::write_back_contacts, scene.cpp:9339-9388).
+int orc_dy_step_anchors(float dt, std::uint32_t nb, fyzz_dy_body* bodies, std::uint32_t ns,
+                        const fyzz_dy_static* statics, std::uint32_t nm, const fyzz_dy_manager* managers,
+                        const fyzz_dy_patch* history, std::uint32_t nh, fyzz_dy_patch* patches_out,
+                        float* impulses_out, std::uint32_t* parts_out, fyzz_friction_anchors* anchors_out) {
+    Dy<float>::patch_order_.assign(nm, {});
+    Dy<float>::anchors_out_ = anchors_out;
+    const int rc = Dy<float>::step(dt, nb, bodies, ns, statics, nm, managers, history, nh, patches_out, impulses_out,
+                                   parts_out, nullptr);
+    Dy<float>::anchors_out_ = nullptr;
+    return rc;
 }
 
 // The FP64 twin: same inputs; state64[nb][13] receives position, linear

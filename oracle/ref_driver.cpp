@@ -354,6 +354,30 @@ std::size_t ref_manifolds(void* s, std::uint32_t* pairs_out, std::uint8_t* count
         }
     return m.size();
 }
+// Scene::solved_friction_patches (scene.h:846-848): per patch the shape
+// pair (2 u32), anchor count and broken flag (2 u32), anchor positions
+// f32[2][3] and world impulses f64[2][3]. Returns the patch count.
+std::size_t ref_friction_patches(void* s, std::uint32_t* pair_count_broken, float* positions, double* impulses,
+                                 std::size_t cap) {
+    const auto w = static_cast<Scene*>(s)->solved_friction_patches();
+    if (pair_count_broken)
+        for (std::size_t i = 0; i < w.size() && i < cap; ++i) {
+            const auto& p = w[i];
+            pair_count_broken[4 * i] = p.pair.a;
+            pair_count_broken[4 * i + 1] = p.pair.b;
+            pair_count_broken[4 * i + 2] = p.anchor_count;
+            pair_count_broken[4 * i + 3] = p.broken ? 1u : 0u;
+            for (int k = 0; k < 2; ++k) {
+                positions[6 * i + 3 * k] = p.anchor_positions[k].x;
+                positions[6 * i + 3 * k + 1] = p.anchor_positions[k].y;
+                positions[6 * i + 3 * k + 2] = p.anchor_positions[k].z;
+                impulses[6 * i + 3 * k] = p.anchor_impulses[k].x;
+                impulses[6 * i + 3 * k + 1] = p.anchor_impulses[k].y;
+                impulses[6 * i + 3 * k + 2] = p.anchor_impulses[k].z;
+            }
+        }
+    return w.size();
+}
 std::size_t ref_indexed_order(void* s, std::uint64_t* out, std::size_t cap) {
     const auto o = static_cast<Scene*>(s)->indexed_contact_manager_order();
     if (out)

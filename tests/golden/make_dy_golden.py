@@ -10,7 +10,11 @@ by the reference NativeCpu scene. Per step s:
                 shape pair of each descriptor's manifold (Gu order: the
                 pair's canonical (lo, hi) shape ids),
   part_{s}      each descriptor's conflict partition (global),
-  task_{s}      the tasks' cumulative partition ends.
+  task_{s}      the tasks' cumulative partition ends,
+  fr_{s}, frpos_{s}, frimp_{s}  the friction write-back of the step
+                (Scene::solved_friction_patches): per patch the shape pair,
+                anchor count and broken flag, anchor positions and world
+                impulses (f64).
 Contacts are not stored: the oracle's narrowphase reproduces the
 reference's manifolds bit-exactly (tests/test_pcm_oracle.py).
 
@@ -41,7 +45,7 @@ from make_sap_golden import RefScene, load_ref, P  # noqa: E402
 def dy_scene(path: Path, side=3, steps=72, seed=scenes.SEED_BASE + 23, spacing=1.15, base=0.62,
              events_path: Path | None = None):
     ref = load_ref()
-    for f in ("ref_dy_descriptors", "ref_dy_tasks", "ref_manifolds", "ref_contact_events"):
+    for f in ("ref_dy_descriptors", "ref_dy_tasks", "ref_manifolds", "ref_contact_events", "ref_friction_patches"):
         getattr(ref, f).restype = C.c_size_t
     sc = RefScene(ref)
     if events_path is not None:
@@ -116,6 +120,14 @@ def dy_scene(path: Path, side=3, steps=72, seed=scenes.SEED_BASE + 23, spacing=1
             events[f"evimp_{s}"] = imp[:ne]
             events[f"pt_{s}"] = pt[:npts.value]
             events[f"ptimp_{s}"] = pimp[:npts.value]
+        nf = ref.ref_friction_patches(sc.s, None, None, None, 0)
+        fp = np.zeros((max(nf, 1), 4), np.uint32)
+        fpos = np.zeros((max(nf, 1), 2, 3), np.float32)
+        fimp = np.zeros((max(nf, 1), 2, 3), np.float64)
+        ref.ref_friction_patches(sc.s, P(fp), P(fpos), P(fimp), nf)
+        rec[f"fr_{s}"] = fp[:nf]
+        rec[f"frpos_{s}"] = fpos[:nf]
+        rec[f"frimp_{s}"] = fimp[:nf]
         rec[f"desc_{s}"] = mp[di]
         rec[f"body_{s}"] = np.stack([b0, b1], 1)
         rec[f"part_{s}"] = pt

@@ -197,9 +197,11 @@ class OracleDy:
     def __init__(self, lib):
         self.lib = lib
 
-    def step(self, dt, bodies, statics, managers, history, fp64=False):
+    def step(self, dt, bodies, statics, managers, history, fp64=False, anchors=None):
         """Runs one step; returns (bodies', patches[nm], impulses[nm,4],
-        partitions[nm] or state64[nb,13])."""
+        partitions[nm] or state64[nb,13]). anchors: an optional
+        FRICTION_ANCHORS array of at least nm records that receives the
+        step's friction write-back (FP32 only)."""
         b = np.ascontiguousarray(bodies.copy())
         nm = len(managers)
         patches = np.zeros(max(nm, 1), DY_PATCH)
@@ -214,9 +216,15 @@ class OracleDy:
                                         P(imp), P(out))
         else:
             out = np.zeros(max(nm, 1), np.uint32)
-            rc = self.lib.orc_dy_step(C.c_float(dt), C.c_uint32(len(b)), P(b), C.c_uint32(len(st)), P(st),
-                                      C.c_uint32(nm), P(mg), P(hist), C.c_uint32(len(history)), P(patches), P(imp),
-                                      P(out))
+            if anchors is not None:
+                assert anchors.flags["C_CONTIGUOUS"] and len(anchors) >= nm
+                rc = self.lib.orc_dy_step_anchors(C.c_float(dt), C.c_uint32(len(b)), P(b), C.c_uint32(len(st)),
+                                                  P(st), C.c_uint32(nm), P(mg), P(hist), C.c_uint32(len(history)),
+                                                  P(patches), P(imp), P(out), P(anchors))
+            else:
+                rc = self.lib.orc_dy_step(C.c_float(dt), C.c_uint32(len(b)), P(b), C.c_uint32(len(st)), P(st),
+                                          C.c_uint32(nm), P(mg), P(hist), C.c_uint32(len(history)), P(patches),
+                                          P(imp), P(out))
         if rc != 0:
             raise RuntimeError(f"oracle Dy step failed ({rc})")
         return b, patches[:nm], imp[:nm], out[:nm] if not fp64 else out
@@ -343,9 +351,11 @@ class DyScene:
         return mg, (np.array(hist, DY_PATCH) if hist else np.zeros(0, DY_PATCH))
 
 
-def replay_dy_scene(orc, g, on_step, fp64=False):
+def replay_dy_scene(orc, g, on_step, fp64=False, anchors=False):
     """Oracle Dy over the golden pile from synchronized start states; calls
-    on_step(s, bodies_out, partitions or FP64 state, managers)."""
+    on_step(s, bodies_out, partitions or FP64 state, managers), with the
+    step's friction write-back and manager keys appended when anchors."""
+    from fyzz_amd.dy import FRICTION_ANCHORS
     sc = DyScene(orc, g)
     dy = OracleDy(orc.lib)
     history = {}
@@ -353,12 +363,16 @@ def replay_dy_scene(orc, g, on_step, fp64=False):
     acc_ang = np.zeros((sc.n, 3), np.float32)
     for s in range(sc.steps):
         mg, hist = sc.step_managers(s, history)
+        an = np.zeros(max(len(mg), 1), FRICTION_ANCHORS) if anchors else None
         out, patches, imp, extra = dy.step(1.0 / 120.0, sc.bodies(s, acc_lin, acc_ang), sc.statics, mg, hist,
-                                           fp64=fp64)
+                                           fp64=fp64, anchors=an)
         history = {k: patches[i] for i, k in enumerate(sc.keys[s])}
         acc_lin = out["sleep_linear_acc"].copy()
         acc_ang = out["sleep_angular_acc"].copy()
-        on_step(s, out, extra, mg)
+        if anchors:
+            on_step(s, out, extra, mg, an[:len(mg)], sc.keys[s])
+        else:
+            on_step(s, out, extra, mg)
 
 
 class OracleWorld:
@@ -739,7 +753,9 @@ class StepOracle:
         b = bodies.copy()
         counted = np.bincount(pairs.ravel(), minlength=n).astype(np.uint32) + 1
         b["counted_interactions"] = counted
-        out, patches, imp, parts = self.dy.step(dt, b, self.statics, mg, hist)
+        from fyzz_amd.dy import FRICTION_ANCHORS
+        an = np.zeros(max(nm, 1), FRICTION_ANCHORS)
+        out, patches, imp, parts = self.dy.step(dt, b, self.statics, mg, hist, anchors=an)
         return {"pairs": pairs, "found": found, "lost": lost, "counts_bb": cnt_bb[:npairs],
                 "pts_bb": pts_bb[:npairs], "counts_pl": cnt_pl, "pts_pl": pts_pl, "managers": mg, "bodies": out,
-                "impulses": imp, "partitions": parts, "patches": patches}
+                "impulses": imp, "partitions": parts, "patches": patches, "anchors": an[:nm]}

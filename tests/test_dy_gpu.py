@@ -26,7 +26,7 @@ def _solver(ctx, nb, nm):
     return DySolver(ctx, nb, nm)
 
 
-def _gpu_step(solver, dt, bodies, statics, mg, hist):
+def _gpu_step(solver, dt, bodies, statics, mg, hist, anchors=False):
     from fyzz_amd.dy import DY_BODY, DY_PATCH, from_device, to_device
     db = to_device(bodies)
     ds = to_device(statics)
@@ -34,8 +34,23 @@ def _gpu_step(solver, dt, bodies, statics, mg, hist):
     dh = to_device(hist) if len(hist) else None
     patches, imp, parts = solver.step(dt, db, ds, dm, dh)
     flags, nparts, rounds = solver.status()
-    return (from_device(db, DY_BODY), from_device(patches, DY_PATCH), imp.cpu().numpy(),
-            parts.cpu().numpy().astype(np.uint32), (flags, nparts, rounds))
+    out = (from_device(db, DY_BODY), from_device(patches, DY_PATCH), imp.cpu().numpy(),
+           parts.cpu().numpy().astype(np.uint32), (flags, nparts, rounds))
+    if anchors:
+        out = out + (solver.friction_anchors(len(mg)),)
+    return out
+
+
+def _compare_anchors(got, want, label):
+    """Friction write-back: counts and broken flags exact, positions at the
+    contact bar (1e-5), impulses at the impulse bar (1e-4)."""
+    np.testing.assert_array_equal(got["anchor_count"], want["anchor_count"], err_msg=f"{label} anchor counts")
+    np.testing.assert_array_equal(got["broken"], want["broken"], err_msg=f"{label} broken")
+    live = np.arange(2)[None, :] < want["anchor_count"][:, None]
+    np.testing.assert_allclose(got["position"][live], want["position"][live], rtol=1e-5, atol=1e-6,
+                               err_msg=f"{label} anchor positions")
+    np.testing.assert_allclose(got["impulse"][live], want["impulse"][live], rtol=1e-4, atol=1e-5,
+                               err_msg=f"{label} anchor impulses")
 
 
 def test_partitions_match_reference(ctx, orc, torch_cuda):
@@ -58,7 +73,8 @@ def test_partitions_match_reference(ctx, orc, torch_cuda):
 def test_scene_steps_match_oracle_and_reference(ctx, orc, torch_cuda):
     """The golden pile's 72 steps from synchronized start states: GPU ==
     oracle on the same inputs (oracle history carried), GPU == reference
-    next state."""
+    next state; the friction write-back == the oracle's (which equals the
+    reference's bit for bit)."""
     g = np.load(GOLDEN / "dy_scene.npz")
     sc = DyScene(orc, g)
     odd = OracleDy(orc.lib)
@@ -67,11 +83,14 @@ def test_scene_steps_match_oracle_and_reference(ctx, orc, torch_cuda):
     acc_lin = np.zeros((sc.n, 3), np.float32)
     acc_ang = np.zeros((sc.n, 3), np.float32)
     same = total = 0
+    from fyzz_amd.dy import FRICTION_ANCHORS
     for s in range(sc.steps):
         mg, hist = sc.step_managers(s, history)
         bodies = sc.bodies(s, acc_lin, acc_ang)
-        want, wp, wi, wparts = odd.step(1 / 120, bodies, sc.statics, mg, hist)
-        got, gp, gi, gparts, st = _gpu_step(solver, 1 / 120, bodies, sc.statics, mg, hist)
+        wa = np.zeros(max(len(mg), 1), FRICTION_ANCHORS)
+        want, wp, wi, wparts = odd.step(1 / 120, bodies, sc.statics, mg, hist, anchors=wa)
+        got, gp, gi, gparts, st, ga = _gpu_step(solver, 1 / 120, bodies, sc.statics, mg, hist, anchors=True)
+        _compare_anchors(ga, wa[:len(mg)], f"step {s}")
         np.testing.assert_array_equal(gparts, wparts, err_msg=f"step {s} partitions")
         gs, ws = body_state_from(got), body_state_from(want)
         np.testing.assert_allclose(gs, ws, rtol=RTOL, atol=ATOL, err_msg=f"step {s} vs oracle")

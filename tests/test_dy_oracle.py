@@ -44,6 +44,29 @@ def test_dy_matches_reference_per_step(orc):
     print(f"steps bit-identical: {exact[0]}/{exact[1]}, worst relative state error {worst['state']:.3g}")
 
 
+def test_friction_writeback_matches_reference(orc):
+    """The friction write-back (Scene::solved_friction_patches: anchor
+    counts, broken flags, world anchor positions, f64 world impulses) of
+    every step of the reference pile, bit for bit, matched by shape pair."""
+    g = np.load(GOLDEN / "dy_scene.npz")
+    seen = [0, 0]
+
+    def on_step(s, out, parts, mg, an, keys):
+        fr, fpos, fimp = g[f"fr_{s}"], g[f"frpos_{s}"], g[f"frimp_{s}"]
+        want = {(int(a), int(b)): i for i, (a, b, _, _) in enumerate(fr)}
+        assert len(want) == len(fr) == len(keys), f"step {s}: one patch per manager"
+        for m, key in enumerate(keys):
+            i = want[key]
+            na = int(fr[i, 2])
+            assert an["anchor_count"][m] == na and an["broken"][m] == fr[i, 3], f"step {s} {key}"
+            np.testing.assert_array_equal(an["position"][m][:na], fpos[i][:na], err_msg=f"step {s} {key} positions")
+            np.testing.assert_array_equal(an["impulse"][m][:na], fimp[i][:na], err_msg=f"step {s} {key} impulses")
+            seen[0] += na
+            seen[1] += int(np.any(fimp[i][:na] != 0.0))
+    replay_dy_scene(orc, g, on_step, anchors=True)
+    assert seen[0] > 1000 and seen[1] > 500, seen
+
+
 def test_fp64_twin_close_to_fp32(orc):
     g = np.load(GOLDEN / "dy_scene.npz")
     errs = []

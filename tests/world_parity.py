@@ -6,6 +6,8 @@ of the hot path:
   manager list (bodies, counts)       exact; contacts 1e-5 relative (1e-6 absolute)
   partitions                          exact
   contact impulses                    1e-4 relative (1e-5 absolute)
+  friction write-back                 counts exact; anchor positions 1e-5,
+                                      anchor impulses 1e-4 relative
   body state after the step           1e-4 relative (1e-6 absolute)
 Test helper, not a test module."""
 import time
@@ -44,6 +46,16 @@ def compare_step(w, so, dt, label, reports=True):
                                    err_msg=f"{label} contact {f}")
     np.testing.assert_array_equal(parts, want["partitions"], err_msg=f"{label} partitions")
     np.testing.assert_allclose(imp, want["impulses"], rtol=1e-4, atol=1e-5, err_msg=f"{label} impulses")
+    # friction write-back: anchor counts and broken flags exact, positions
+    # at the contact bar, impulses at the impulse bar
+    an, wa = w.friction_anchors(), want["anchors"]
+    np.testing.assert_array_equal(an["anchor_count"], wa["anchor_count"], err_msg=f"{label} anchor counts")
+    np.testing.assert_array_equal(an["broken"], wa["broken"], err_msg=f"{label} broken patches")
+    live = np.arange(2)[None, :] < wa["anchor_count"][:, None]
+    np.testing.assert_allclose(an["position"][live], wa["position"][live], rtol=1e-5, atol=1e-6,
+                               err_msg=f"{label} anchor positions")
+    np.testing.assert_allclose(an["impulse"][live], wa["impulse"][live], rtol=1e-4, atol=1e-5,
+                               err_msg=f"{label} anchor impulses")
     got_b, want_b = w.body_states(), body_state_from(want["bodies"])
     np.testing.assert_allclose(got_b, want_b, rtol=1e-4, atol=1e-6, err_msg=f"{label} bodies")
     exact = int(np.sum(np.all(got_b == want_b, axis=1)))
