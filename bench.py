@@ -126,7 +126,10 @@ def stage_bytes(stage, tot, n):
       dy.pgs       per iteration: 68 B per contact row (64-B row read +
                    applied impulse write), 60 B per friction row, 96 B per
                    body (velocity deltas read + written)
-      dy.integrate 236 B per body (152-B input, 84-B pose/sleep output)"""
+      dy.integrate 236 B per body (152-B input, 84-B pose/sleep output)
+      world.islands 24 B per live pair (pair, counts, source, touch flag
+                   read once) + 32 B per body (flags, two parent words and
+                   three root flags written, flags read)"""
     steps = max(tot["steps"], 1)
     pairs, managers, fl = tot["pairs"], tot["managers"], tot.get("found_lost", 0)
     body_steps = n * steps
@@ -157,6 +160,8 @@ def stage_bytes(stage, tot, n):
         return iters_per_step * (tot["contact_rows"] * 68 + tot["friction_rows"] * 60) + tot["iterations"] * n * 96
     if stage == "dy.integrate":
         return 236 * body_steps
+    if stage == "world.islands":
+        return 24 * pairs + 32 * body_steps
     return None
 
 
@@ -384,7 +389,8 @@ class Clock:
 
 
 PILE_STAGES = ("sap.bounds", "sap.sort", "sap.pairs", "sap.lists", "pcm.box_box", "pcm.plane_box",
-               "world.managers", "dy.bodies", "dy.partition", "dy.prep", "dy.pgs", "dy.integrate")
+               "world.islands", "world.managers", "dy.bodies", "dy.partition", "dy.prep", "dy.pgs",
+               "dy.integrate")
 
 
 def profiled_steps(ctx, w, steps, names, step):

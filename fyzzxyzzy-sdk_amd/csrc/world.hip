@@ -103,14 +103,22 @@ __global__ void k_total(const u32* d_nbb, u32 cap, u32 extra, u32* d_total) {
     *d_total = min(*d_nbb, cap) + extra;
 }
 
+// A touching manager enters the solver list unless its island sleeps
+// (body0 is dynamic in both kinds; a box/box manager's bodies share an
+// island, so they sleep together).
 struct MgrPred {
     const u32* d_nbb;
     u32 cap;
     const u32* counts_bb;
     const u32* counts_pl;
+    const u32* first;
+    const fyzz_dy_body* bodies;
     __device__ bool operator()(u32 i) const {
         const u32 nbb = min(*d_nbb, cap);
-        return i < nbb ? counts_bb[i] != 0u : counts_pl[i - nbb] != 0u;
+        const bool touching = i < nbb ? counts_bb[i] != 0u : counts_pl[i - nbb] != 0u;
+        if (!touching) return false;
+        const u32 b0 = i < nbb ? first[i] : i - nbb;
+        return !(bodies[b0].flags & 1u);
     }
 };
 
@@ -317,6 +325,7 @@ struct EvArgs {
     const u64* k3;
     const u32* s3;
     const u32* d_counts;      // n1, n2, n3, total
+    const fyzz_dy_body* bodies;  // sleeping flags after the island pass
 };
 
 __device__ inline bool prev_touch_bb(const EvArgs& A, u32 i) {
@@ -327,9 +336,16 @@ __device__ inline u32 lost_prev_index(const EvArgs& A, u32 j) {
     return lower_bound64(A.prev, np, A.lost[j]);
 }
 
+// A touch that persists inside a sleeping island is not reported
+// (processPersistentContactEvents needs an active body, scene.cpp:
+// 9199-9215); found and lost touches always are.
 struct EvPred1 {
     EvArgs A;
-    __device__ bool operator()(u32 i) const { return A.counts_bb[i] != 0u || prev_touch_bb(A, i); }
+    __device__ bool operator()(u32 i) const {
+        const bool before = prev_touch_bb(A, i);
+        if (A.counts_bb[i] == 0u) return before;
+        return !(before && (A.bodies[A.first[i]].flags & 1u));
+    }
 };
 struct EvWrite1 {
     EvArgs A;
@@ -342,7 +358,11 @@ struct EvWrite1 {
 };
 struct EvPred2 {
     EvArgs A;
-    __device__ bool operator()(u32 b) const { return A.counts_pl[b] != 0u || A.touch_pl[b] != 0u; }
+    __device__ bool operator()(u32 b) const {
+        const bool before = A.touch_pl[b] != 0u;
+        if (A.counts_pl[b] == 0u) return before;
+        return !(before && (A.bodies[b].flags & 1u));
+    }
 };
 struct EvWrite2 {
     EvArgs A;
@@ -502,30 +522,369 @@ __global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const
     if (d_lost) totals[8] += *d_lost;
 }
 
+// ---- islands: sleep and wake (Scene::activate_published_islands,
+// scene.cpp:6148-6262; Scene::deactivate_ready_accurate_islands,
+// 6465-6536)
+//
+// Body flags: bit0 sleeping, bit1 ready for sleep (the last sleep check),
+// bit3 activating (a wake queued for this step's island pass: an authored
+// wake of a sleeping body, or a peer woken by an authored pose).
+// Island membership is computed each step, not stored: connected
+// components of the dynamic bodies over an edge set, by union-find (a
+// larger root is hooked under a smaller one, so a component's root is its
+// smallest slot whatever the order of the races).
+//   speculative islands (wake): this step's broadphase pairs and the pairs
+//     it lost (those are split only by this step's third pass, after the
+//     wake: scene.cpp:10747-10793)
+//   accurate islands (sleep): this step's touching box/box pairs. Only
+//     ready bodies can sleep, so the union takes the touches between two
+//     ready bodies; a ready body touching one that is not ready poisons
+//     its component (that body is in the same island).
+// The union kernels read and write parents with agent-scope atomics (the
+// workgroups of one launch run on different XCDs, whose L2 caches are not
+// coherent with each other); the kernels after them flatten the parents
+// into labels and read them with plain loads (kernel boundaries make the
+// union's results visible). Every kernel reads its gate from device
+// counters, so the pass costs a few empty launches on steps where nothing
+// can sleep or wake, and it can be captured in a graph.
+constexpr u32 kSleeping = 1u, kReady = 2u, kActivating = 8u;
+enum IslandCounter : u32 {
+    kIslSleeping = 0,
+    kIslActivating,
+    kIslReadyAwake,
+    kIslSleepingActivating,  // sleeping bodies carrying an activation (authored wakes)
+    kIslMixed,               // speculative edges between a sleeping and an awake body
+    kIslGiant,               // the largest speculative component after the link rounds (its root)
+    kIslCounters
+};
+
+__device__ inline u32 ld_parent(const u32* parent, u32 x) {
+    return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline u32 find_root(u32* parent, u32 x) {
+    for (;;) {
+        const u32 p = ld_parent(parent, x);
+        if (p == x) return x;
+        const u32 g = ld_parent(parent, p);
+        if (g == p) return p;
+        // path halving: a non-root's parent only moves toward its root
+        __hip_atomic_store(parent + x, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = g;
+    }
+}
+__device__ inline void unite(u32* parent, u32 a, u32 b) {
+    // most edges join bodies already hanging from the same node
+    if (ld_parent(parent, a) == ld_parent(parent, b)) return;
+    for (;;) {
+        a = find_root(parent, a);
+        b = find_root(parent, b);
+        if (a == b) return;
+        const u32 hi = a > b ? a : b, lo = a > b ? b : a;
+        u32 expect = hi;
+        if (__hip_atomic_compare_exchange_strong(parent + hi, &expect, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+}
+// After the union kernel: the root by plain loads (parents only point down
+// to smaller slots, so the walk ends).
+__device__ inline u32 root_of(const u32* parent, u32 x) {
+    u32 p = parent[x];
+    while (p != x) {
+        x = p;
+        p = parent[x];
+    }
+    return x;
+}
+
+struct IslandArgs {
+    u32 n, cap;
+    fyzz_dy_body* bodies;
+    const u64* cur;        // this step's broadphase pairs (lo, hi)
+    const u32* d_cur;
+    const u64* gone;       // pairs the speculative graph still holds: this step's lost pairs, or the
+    const u32* d_gone;     // whole previous list where the broadphase lists no losses
+    const u32* src;        // previous index of each current pair (kNone: found this step)
+    const u32* first;      // Gu order of each current pair
+    const u32* second;
+    const u32* counts_bb;  // this step's contact counts (touching: non-zero)
+    const u32* touch_prev; // the last step's touching flags, by previous index
+    const u32* counts_pl;  // plane contact counts per box (null: no floor)
+    const u32* touch_pl;   // the last step's plane touching flags
+    u32* spec;             // union-find parents, then labels: speculative islands
+    u32* acc;              // union-find parents, then labels: accurate islands (ready bodies)
+    u32* root_wake;        // per speculative root: a member is activating
+    u32* root_poison;      // per accurate root: a member touches a body that is not ready
+    u32* root_awake;       // per accurate root: a member is awake
+    u32* ctr;              // kIslCounters
+    uint8_t* asleep;       // per body: sleeping, as the pass found it (a compact copy for edge scans)
+};
+
+__device__ inline void set_bits(fyzz_dy_body& b, u32 on, u32 off) {
+    if (on) atomicOr(&b.flags, on);
+    if (off) atomicAnd(&b.flags, ~off);
+}
+
+// Wake marks of this step's broadphase and touch events
+// (activate_published_islands' first two loops): a pair found by the
+// broadphase activates both bodies unless both sleep; a touch found by
+// the narrowphase activates both and clears their readiness. Also resets
+// the union-find parents, the root flags and the counters for the pass.
+__global__ void k_isl_mark(IslandArgs A) {
+    const u32 nbb = min(*A.d_cur, A.cap);
+    const u32 stride = gridDim.x * blockDim.x;
+    const u32 t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 < kIslCounters) A.ctr[t0] = 0u;
+    for (u32 i = t0; i < A.n; i += stride) {
+        A.spec[i] = i;
+        A.acc[i] = i;
+        A.root_wake[i] = 0u;
+        A.root_poison[i] = 0u;
+        A.root_awake[i] = 0u;
+    }
+    for (u32 i = t0; i < nbb; i += stride) {
+        const u32 a = A.first[i], b = A.second[i];
+        const u32 s = A.src[i];
+        if (s == kNone && !((A.bodies[a].flags & A.bodies[b].flags) & kSleeping)) {
+            set_bits(A.bodies[a], kActivating, 0u);
+            set_bits(A.bodies[b], kActivating, 0u);
+        }
+        if (A.counts_bb[i] != 0u && !(s != kNone && A.touch_prev[s] != 0u)) {
+            set_bits(A.bodies[a], kActivating, kReady);
+            set_bits(A.bodies[b], kActivating, kReady);
+        }
+    }
+    if (A.counts_pl)
+        for (u32 i = t0; i < A.n; i += stride)
+            if (A.counts_pl[i] != 0u && A.touch_pl[i] == 0u) set_bits(A.bodies[i], kActivating, kReady);
+}
+
+// Counters of the pass: sleeping, activating, awake-and-ready bodies.
+__global__ void k_isl_count(IslandArgs A) {
+    __shared__ u32 s[kIslCounters];
+    if (threadIdx.x < kIslCounters) s[threadIdx.x] = 0u;
+    __syncthreads();
+    constexpr u32 kCounted = 4;  // the counters filled here
+    u32 c[kCounted] = {0u, 0u, 0u, 0u};
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        const u32 f = A.bodies[i].flags;
+        A.asleep[i] = static_cast<uint8_t>(f & kSleeping);
+        c[kIslSleeping] += (f & kSleeping) ? 1u : 0u;
+        c[kIslActivating] += (f & kActivating) ? 1u : 0u;
+        c[kIslReadyAwake] += ((f & (kReady | kSleeping)) == kReady) ? 1u : 0u;
+        c[kIslSleepingActivating] += ((f & (kActivating | kSleeping)) == (kActivating | kSleeping)) ? 1u : 0u;
+    }
+    for (u32 k = 0; k < kCounted; ++k)
+        for (int d = kWave / 2; d > 0; d >>= 1) c[k] += __shfl_xor(c[k], d, kWave);
+    if (lane_id() == 0)
+        for (u32 k = 0; k < kCounted; ++k)
+            if (c[k]) atomicAdd(&s[k], c[k]);
+    __syncthreads();
+    if (threadIdx.x < kCounted && s[threadIdx.x]) atomicAdd(&A.ctr[threadIdx.x], s[threadIdx.x]);
+}
+
+__device__ inline bool wake_pass(const IslandArgs& A) {
+    return A.ctr[kIslSleeping] != 0u && A.ctr[kIslActivating] != 0u;
+}
+// The components matter only when a sleeping body could reach an
+// activation: one carries an activation itself, or an edge joins a
+// sleeping body to an awake one. Otherwise every sleeping body's island
+// is made of sleeping bodies without activations, and nothing wakes.
+__device__ inline bool spec_needed(const IslandArgs& A) {
+    return wake_pass(A) && (A.ctr[kIslSleepingActivating] != 0u || A.ctr[kIslMixed] != 0u);
+}
+
+// Speculative edges: this step's pairs, then the pairs it lost.
+__device__ inline u64 spec_edge(const IslandArgs& A, u32 i, u32 nbb) { return i < nbb ? A.cur[i] : A.gone[i - nbb]; }
+__device__ inline u32 edge_lo(u64 x) { return static_cast<u32>(x >> 32); }
+__device__ inline u32 edge_hi(u64 x) { return static_cast<u32>(x & 0xffffffffu); }
+// The link edges of the speculative union's first round: each body's
+// first two pairs as the lower slot (the list is sorted by lower slot).
+__device__ inline bool link_edge(const IslandArgs& A, u32 i, u32 nbb) {
+    if (i >= nbb) return false;
+    const u32 lo = edge_lo(A.cur[i]);
+    return i == 0 || edge_lo(A.cur[i - 1]) != lo || i == 1 || edge_lo(A.cur[i - 2]) != lo;
+}
+
+// Counts the edges between a sleeping and an awake body (sleep bits from
+// the compact per-body copy k_isl_count wrote).
+__global__ void k_isl_spec_mixed(IslandArgs A) {
+    if (!wake_pass(A)) return;
+    const u32 nbb = min(*A.d_cur, A.cap), ng = min(*A.d_gone, A.cap);
+    u32 mixed = 0;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb + ng; i += gridDim.x * blockDim.x) {
+        const u64 x = spec_edge(A, i, nbb);
+        mixed += A.asleep[edge_lo(x)] != A.asleep[edge_hi(x)] ? 1u : 0u;
+    }
+    for (int d = kWave / 2; d > 0; d >>= 1) mixed += __shfl_xor(mixed, d, kWave);
+    if (lane_id() == 0 && mixed) atomicAdd(&A.ctr[kIslMixed], mixed);
+}
+
+// When the components are needed: links each body to its first two pairs
+// (Afforest sampling: most bodies join one giant component through a
+// small share of the edges).
+__global__ void k_isl_spec_link(IslandArgs A) {
+    if (!spec_needed(A)) return;
+    const u32 nbb = min(*A.d_cur, A.cap);
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb; i += gridDim.x * blockDim.x)
+        if (link_edge(A, i, nbb)) unite(A.spec, edge_lo(A.cur[i]), edge_hi(A.cur[i]));
+}
+
+// Flattens the linked parents into labels.
+__global__ void k_isl_spec_compress(IslandArgs A) {
+    if (!spec_needed(A)) return;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x)
+        A.spec[i] = root_of(A.spec, i);
+}
+
+// The most frequent label among 1,024 sampled bodies (one workgroup): the
+// giant component, whose internal edges the full union skips.
+__global__ void k_isl_spec_giant(IslandArgs A) {
+    if (!spec_needed(A)) return;
+    constexpr u32 kSamples = 1024;
+    __shared__ u32 lab[kSamples];
+    for (u32 k = threadIdx.x; k < kSamples; k += blockDim.x)
+        lab[k] = A.n ? A.spec[static_cast<u32>((static_cast<u64>(k) * A.n) / kSamples)] : 0u;
+    __syncthreads();
+    // count, for each sampled label, how often it occurs; keep the best
+    u32 best = 0u, best_count = 0u;
+    for (u32 k = threadIdx.x; k < kSamples; k += blockDim.x) {
+        u32 c = 0;
+        for (u32 j = 0; j < kSamples; ++j) c += lab[j] == lab[k] ? 1u : 0u;
+        if (c > best_count || (c == best_count && lab[k] < best)) {
+            best = lab[k];
+            best_count = c;
+        }
+    }
+    __shared__ u64 s_best;
+    if (threadIdx.x == 0) s_best = ~0ull;
+    __syncthreads();
+    // largest count first, then smallest label
+    atomicMin(&s_best, (static_cast<u64>(kSamples - best_count) << 32) | best);
+    __syncthreads();
+    if (threadIdx.x == 0) A.ctr[kIslGiant] = static_cast<u32>(s_best & 0xffffffffu);
+}
+
+// The full speculative union: every edge not already inside the giant
+// component (both labels equal the giant root: their bodies stay joined).
+__global__ void k_isl_union_spec(IslandArgs A) {
+    if (!spec_needed(A)) return;
+    const u32 nbb = min(*A.d_cur, A.cap), ng = min(*A.d_gone, A.cap);
+    const u32 giant = A.ctr[kIslGiant];
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb + ng; i += gridDim.x * blockDim.x) {
+        if (link_edge(A, i, nbb)) continue;
+        const u64 x = spec_edge(A, i, nbb);
+        const u32 a = edge_lo(x), b = edge_hi(x);
+        if (A.spec[a] == giant && A.spec[b] == giant) continue;
+        unite(A.spec, a, b);
+    }
+}
+
+// Labels (flattened roots) and the roots of the speculative islands that
+// hold an activating body.
+__global__ void k_isl_wake_roots(IslandArgs A) {
+    if (!spec_needed(A)) return;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        const u32 r = root_of(A.spec, i);
+        if (A.bodies[i].flags & kActivating) A.root_wake[r] = 1u;
+        A.spec[i] = r;
+    }
+}
+
+// Wakes every sleeping member of those islands (wake counter 0.4 s at
+// least, readiness and sleep filter cleared) and consumes the activation
+// marks.
+__global__ void k_isl_wake(IslandArgs A) {
+    if (A.ctr[kIslActivating] == 0u) return;
+    const bool wake = spec_needed(A);
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        fyzz_dy_body& b = A.bodies[i];
+        u32 f = b.flags & ~kActivating;
+        if (wake && (f & kSleeping) && A.root_wake[root_of(A.spec, i)]) {
+            f &= ~(kSleeping | kReady);
+            b.wake_counter = b.wake_counter > 0.4f ? b.wake_counter : 0.4f;
+            for (int k = 0; k < 3; ++k) b.sleep_linear_acc[k] = b.sleep_angular_acc[k] = 0.0f;
+        }
+        b.flags = f;
+    }
+}
+
+// Accurate union over touches between two ready bodies; a touch between a
+// ready body and one that is not poisons the ready one.
+__global__ void k_isl_union_acc(IslandArgs A) {
+    if (A.ctr[kIslReadyAwake] == 0u) return;
+    const u32 nbb = min(*A.d_cur, A.cap);
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb; i += gridDim.x * blockDim.x) {
+        if (A.counts_bb[i] == 0u) continue;
+        const u32 a = A.first[i], b = A.second[i];
+        const bool ra = A.bodies[a].flags & kReady, rb = A.bodies[b].flags & kReady;
+        if (ra && rb)
+            unite(A.acc, a, b);
+        else if (ra || rb)
+            A.root_poison[ra ? a : b] = 1u;  // per body for now; folded into its root below
+    }
+}
+
+// Per accurate component of ready bodies: poisoned, awake.
+__global__ void k_isl_sleep_roots(IslandArgs A) {
+    if (A.ctr[kIslReadyAwake] == 0u) return;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        const u32 f = A.bodies[i].flags;
+        if (!(f & kReady)) continue;
+        const u32 r = root_of(A.acc, i);
+        if (r != i && A.root_poison[i]) A.root_poison[r] = 1u;
+        if (!(f & kSleeping)) A.root_awake[r] = 1u;
+    }
+}
+
+// A component of ready bodies that is a whole island (unpoisoned), with
+// one awake, goes to sleep: sleeping, wake counter 0, velocities and
+// sleep filter cleared.
+__global__ void k_isl_sleep(IslandArgs A) {
+    if (A.ctr[kIslReadyAwake] == 0u) return;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        fyzz_dy_body& b = A.bodies[i];
+        const u32 f = b.flags;
+        if ((f & (kReady | kSleeping)) != kReady) continue;
+        const u32 r = root_of(A.acc, i);
+        if (A.root_poison[r] || !A.root_awake[r]) continue;
+        b.flags = f | kSleeping;
+        b.wake_counter = 0.0f;
+        for (int k = 0; k < 3; ++k)
+            b.linear_velocity[k] = b.angular_velocity[k] = b.sleep_linear_acc[k] = b.sleep_angular_acc[k] = 0.0f;
+    }
+}
+
 // ---- authored edits after the first step
 
 // Scene::set_pose's wake of the body itself (scene.cpp:637-647) and
 // set_velocity / set_angular_velocity's (730-781): the wake counter is
 // raised to 0.4 s and the sleep flags cleared; a sleeping body also drops
 // its sleep filter.
-__host__ __device__ inline void wake_record(fyzz_dy_body& b) {
-    if (b.flags & 1u)
+// A sleeping body's wake is also queued for the next step's island pass
+// (queue_node_activation: its island wakes with it); `queue` queues it
+// regardless (a peer woken by an authored pose).
+__host__ __device__ inline void wake_record(fyzz_dy_body& b, bool queue = false) {
+    if (b.flags & 1u) {
         for (int k = 0; k < 3; ++k) b.sleep_linear_acc[k] = b.sleep_angular_acc[k] = 0.0f;
+        queue = true;
+    }
     b.wake_counter = b.wake_counter > 0.4f ? b.wake_counter : 0.4f;
-    b.flags &= ~3u;
+    b.flags = (b.flags & ~3u) | (queue ? 8u : 0u);
 }
 
 // Scene::set_pose's invalidations (scene.cpp:651-700) for the box in
 // `slot`: every current broadphase pair holding it and its plane pair
 // lose their PCM cache (ShapeInteraction::resetManagerCachedState). When
-// the pose changed, their friction history is erased as well and every
-// touching manager's other dynamic body that sleeps or is ready to sleep
-// is woken. `pairs` is the list the next step starts from, `cache_bb` and
-// `stamp` the caches and history stamps aligned with it.
+// the pose changed, their friction history is erased as well and the
+// other body of every touching pair (sleeping islands' included) that
+// sleeps or is ready to sleep is woken and queued for the next island
+// pass. `pairs` is the list the next step starts from; `cache_bb`,
+// `stamp` and `counts_bb` (touching: non-zero) are aligned with it.
 __global__ void k_author_pose(u32 slot, int changed, u32 cap, const u64* pairs, const u32* d_pairs,
                               CacheRec* cache_bb, CacheRec* cache_pl, int has_plane, u32* stamp,
-                              const fyzz_dy_manager* managers, const u32* d_nm, u32 cap_m, fyzz_dy_body* bodies) {
-    const u32 np = d_pairs ? min(*d_pairs, cap) : 0u, nm = min(*d_nm, cap_m);
+                              const u32* counts_bb, fyzz_dy_body* bodies) {
+    const u32 np = d_pairs ? min(*d_pairs, cap) : 0u;
     const u32 stride = gridDim.x * blockDim.x;
     const u32 t0 = blockIdx.x * blockDim.x + threadIdx.x;
     for (u32 i = t0; i < np; i += stride) {
@@ -539,15 +898,15 @@ __global__ void k_author_pose(u32 slot, int changed, u32 cap, const u64* pairs, 
         if (changed) stamp[cap + slot] = kNone;
     }
     if (!changed) return;
-    for (u32 m = t0; m < nm; m += stride) {
-        const fyzz_dy_manager& M = managers[m];
-        int other = -1;
-        if (M.body0 == static_cast<int>(slot)) other = M.body1;
-        else if (M.body1 == static_cast<int>(slot)) other = M.body0;
-        if (other < 0) continue;
-        fyzz_dy_body& b = bodies[other];
+    // the other body of every touching pair (sleeping islands' included)
+    for (u32 i = t0; i < np; i += stride) {
+        if (counts_bb[i] == 0u) continue;
+        const u64 x = pairs[i];
+        const u32 lo = static_cast<u32>(x >> 32), hi = static_cast<u32>(x & 0xffffffffu);
+        if (lo != slot && hi != slot) continue;
+        fyzz_dy_body& b = bodies[lo == slot ? hi : lo];
         // every writer of one peer stores the same values
-        if (b.flags & 3u) wake_record(b);
+        if (b.flags & 3u) wake_record(b, true);
     }
 }
 
@@ -597,6 +956,9 @@ struct fyzz_hip_world {
     DevBuf<fyzz_dy_manager> managers;
     DevBuf<fyzz_dy_patch> patches, hist[2];
     DevBuf<fyzz_friction_anchors> anchors;  // friction write-back, filled on demand
+    // island pass: union-find parents (speculative, accurate), per-root flags, counters
+    DevBuf<u32> isl_spec, isl_acc, isl_root_wake, isl_root_poison, isl_root_awake, isl_ctr;
+    DevBuf<uint8_t> isl_asleep;
     DevBuf<u64> totals;
     DevBuf<u32> d_step;
     // graph stepping: one graph per parity of the double-buffer toggles
@@ -709,6 +1071,10 @@ void upload(fyzz_hip_world* w) {
                                           st));
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     }
+    for (auto* b : {&w->isl_spec, &w->isl_acc, &w->isl_root_wake, &w->isl_root_poison, &w->isl_root_awake})
+        b->reserve(nn);
+    w->isl_ctr.reserve(world::kIslCounters);
+    w->isl_asleep.reserve(nn);
     w->totals.reserve(world::kTotals);
     FYZZ_HIP_CHECK(hipMemsetAsync(w->totals.p, 0, world::kTotals * sizeof(u64), st));
     w->d_step.reserve(1);
@@ -833,7 +1199,7 @@ void publish_events(fyzz_hip_world* w, const sap::Lists& L, int c_prev, int c_cu
              w->src.p,     w->first.p,      w->second.p,    w->counts_bb.p,  w->counts_pl.p,
              w->touch_bb[c_prev].p, w->touch_pl.p, w->shape.p, w->floor_id,
              w->k1.p,      w->s1.p,         w->k2.p,        w->s2.p,         w->k3.p,      w->s3.p,
-             w->ev_counts.p};
+             w->ev_counts.p, w->bodies.p};
     hipLaunchKernelGGL(k_slot_mgr, dim3(std::min<u32>(grid_for(w->mgr_cap, 256), 2048u)), dim3(256), 0, st,
                        w->d_nm.p, w->mgr_cap, w->mgr_slot.p, w->slot_mgr.p);
     sel::select(st, cap, L.d_cur, EvPred1{A}, EvWrite1{A, w->k1.p, w->s1.p}, w->tiles.p, w->ev_counts.p);
@@ -853,6 +1219,42 @@ void publish_events(fyzz_hip_world* w, const sap::Lists& L, int c_prev, int c_cu
                        w->points.p, w->slot_mgr.p, w->managers.p, w->impulses.p, w->pt_cap);
     FYZZ_HIP_CHECK(hipGetLastError());
     (void)c_cur;
+}
+
+// The island pass of a step, after the narrowphase and before the solver
+// list: wakes (activate_published_islands), then sleeps (the accurate
+// third pass's deactivate_ready_accurate_islands), as Scene::step orders
+// them (scene.cpp:10747-10793). c: the parity of the previous step's
+// pair-aligned buffers (its touching flags).
+void islands(fyzz_hip_world* w, const sap::Lists& L, int c) {
+    using namespace fyzz_hip::world;
+    hipStream_t st = w->ctx->stream;
+    const u32 n = w->n, cap = w->pair_cap;
+    ProfScope ps(&w->ctx->prof, "world.islands", st);
+    // the speculative graph keeps this step's lost pairs until its third
+    // pass; worlds whose broadphase lists no losses (per-scene) keep the
+    // whole previous list, a superset with the same components
+    const u64* gone = L.lost ? L.lost : L.prev;
+    const u32* d_gone = L.lost ? L.d_lost : L.d_prev;
+    IslandArgs A{n,           cap,         w->bodies.p,    L.cur,         L.d_cur,
+                 gone,        d_gone,      w->src.p,       w->first.p,    w->second.p,
+                 w->counts_bb.p, w->touch_bb[c].p, w->has_floor ? w->counts_pl.p : nullptr, w->touch_pl.p,
+                 w->isl_spec.p, w->isl_acc.p, w->isl_root_wake.p, w->isl_root_poison.p, w->isl_root_awake.p,
+                 w->isl_ctr.p, w->isl_asleep.p};
+    const dim3 gp(std::min<u32>(grid_for(std::max(cap, n), 256), 2048u)), gb(std::min<u32>(grid_for(n, 256), 1024u));
+    hipLaunchKernelGGL(k_isl_mark, gp, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_count, gb, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_spec_mixed, gp, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_spec_link, gp, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_spec_compress, gb, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_spec_giant, dim3(1), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_union_spec, gp, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_wake_roots, gb, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_wake, gb, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_union_acc, gp, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_sleep_roots, gb, dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_isl_sleep, gb, dim3(256), 0, st, A);
+    FYZZ_HIP_CHECK(hipGetLastError());
 }
 
 void step(fyzz_hip_world* w, float dt) {
@@ -880,13 +1282,14 @@ void step(fyzz_hip_world* w, float dt) {
         pcm::plane_all(w->ctx, n, w->pos.p, w->quat.p, w->half.p, w->floor.position, kContactDistance, w->cache_pl.p,
                        w->contacts_pl.p, w->counts_pl.p, w->flags_pl.p);
     }
+    islands(w, L, c);
     // touching managers, in order
     const u32 extra = w->has_floor ? n : 0u;
     hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, st, L.d_cur, cap, extra, w->d_total.p);
     const int h = w->hcur;
     {
         ProfScope ps(&w->ctx->prof, "world.managers", st);
-        MgrPred pred{L.d_cur, cap, w->counts_bb.p, w->counts_pl.p};
+        MgrPred pred{L.d_cur, cap, w->counts_bb.p, w->counts_pl.p, w->first.p, w->bodies.p};
         MgrWrite wr{L.d_cur,       cap,          w->first.p,         w->second.p,   w->src.p,
                     w->counts_bb.p, w->contacts_bb.p, w->counts_pl.p, w->contacts_pl.p, w->stamp[h].p,
                     w->d_step.p,   w->managers.p, w->mgr_slot.p};
@@ -1030,8 +1433,7 @@ void author_pose(fyzz_hip_world* w, u32 slot, bool changed) {
     const u32 cap = w->pair_cap;
     hipLaunchKernelGGL(k_author_pose, dim3(std::min<u32>(grid_for(std::max(cap, w->mgr_cap), 256), 1024u)), dim3(256),
                        0, st, slot, changed ? 1 : 0, cap, L.cur, L.d_cur, w->cache_bb[w->ccur].p, w->cache_pl.p,
-                       w->has_floor ? 1 : 0, w->stamp[w->hcur].p, w->managers.p, w->d_nm.p, w->mgr_cap,
-                       w->bodies.p);
+                       w->has_floor ? 1 : 0, w->stamp[w->hcur].p, w->counts_bb.p, w->bodies.p);
     FYZZ_HIP_CHECK(hipGetLastError());
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
 }
@@ -1584,6 +1986,24 @@ int fyzz_hip_world_broadphase_reports(fyzz_hip_world* w, int which, uint32_t* ou
             out_pairs[2 * i] = static_cast<u32>(keys[i] >> 32);
             out_pairs[2 * i + 1] = static_cast<u32>(keys[i] & 0xffffffffu);
         }
+    });
+}
+
+int fyzz_hip_world_fetch_touch(fyzz_hip_world* w, uint32_t* touch_bb, uint32_t pair_capacity, uint32_t* touch_pl) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        if (!w->uploaded || !w->n) return;
+        hipStream_t st = w->ctx->stream;
+        const sap::Lists L = current_lists(w);
+        u32 np = 0;
+        FYZZ_HIP_CHECK(hipMemcpyAsync(&np, L.d_cur, 4, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+        np = std::min(np, w->pair_cap);
+        FYZZ_REQUIRE(!touch_bb || pair_capacity >= np, kOverflow, "world: pair capacity below the pair count");
+        if (touch_bb && np)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(touch_bb, w->touch_bb[w->ccur].p, 4ull * np, hipMemcpyDeviceToHost, st));
+        if (touch_pl) FYZZ_HIP_CHECK(hipMemcpyAsync(touch_pl, w->touch_pl.p, 4ull * w->n, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     });
 }
 

@@ -83,6 +83,7 @@ SIGNATURES = {
     "fyzz_hip_world_contact_events": (C.c_int, [vp, vp, C.c_uint32, u32p, vp, C.c_uint32, u32p]),
     "fyzz_hip_world_broadphase_reports": (C.c_int, [vp, C.c_int, vp, C.c_uint64, u64p]),
     "fyzz_hip_world_fetch_persistent": (C.c_int, [vp, vp, C.c_uint32, u32p, vp, vp, vp, u32p]),
+    "fyzz_hip_world_fetch_touch": (C.c_int, [vp, vp, C.c_uint32, vp]),
     "fyzz_hip_world_device_views": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                               C.POINTER(vp)]),
 }

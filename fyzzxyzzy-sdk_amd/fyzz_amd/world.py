@@ -231,7 +231,8 @@ class World:
         """Host copies of what the next step reads besides the body records:
         {"caches_bb": [P, 56] f32 aligned with the pair list, "caches_pl":
         [n, 56] f32, "history": DY_PATCH[P + n], "stamps": u32[P + n],
-        "step": steps taken}."""
+        "step": steps taken, "touch_bb": u32[P], "touch_pl": u32[n] (the
+        last step's touching flags)}."""
         from .dy import DY_PATCH
         np_ = C.c_uint32()
         step = C.c_uint32()
@@ -245,7 +246,12 @@ class World:
         self._call("fyzz_hip_world_fetch_persistent", cbb.ctypes.data_as(vp), C.c_uint32(max(p, 1)), C.byref(np_),
                    cpl.ctypes.data_as(vp), hist.ctypes.data_as(vp), stamps.ctypes.data_as(vp), C.byref(step))
         assert np_.value == p
-        return {"caches_bb": cbb[:p], "caches_pl": cpl[:n], "history": hist, "stamps": stamps, "step": step.value}
+        tbb = np.zeros(max(p, 1), np.uint32)
+        tpl = np.zeros(max(n, 1), np.uint32)
+        self._call("fyzz_hip_world_fetch_touch", tbb.ctypes.data_as(vp), C.c_uint32(len(tbb)),
+                   tpl.ctypes.data_as(vp))
+        return {"caches_bb": cbb[:p], "caches_pl": cpl[:n], "history": hist, "stamps": stamps, "step": step.value,
+                "touch_bb": tbb[:p], "touch_pl": tpl[:n]}
 
     def friction_anchors(self):
         """The last step's friction write-back (Scene::solved_friction_patches),

@@ -155,8 +155,8 @@ int fyzz_hip_set_cooperative(fyzz_hip_context* ctx, int cooperative);
  * summed milliseconds and span count of one name, and resets it. Names:
  * "sap.bounds", "sap.sort", "sap.sort.scatter", "sap.pairs", "sap.lists",
  * "pcm.box_box", "pcm.plane_box", "dy.bodies", "dy.partition", "dy.prep",
- * "dy.pgs", "dy.integrate", "dy.adjoint", "world.managers", "world.events",
- * "world.scene_pairs". */
+ * "dy.pgs", "dy.integrate", "dy.adjoint", "world.islands", "world.managers",
+ * "world.events", "world.scene_pairs". */
 int fyzz_hip_profile_enable(fyzz_hip_context* ctx, int enable);
 int fyzz_hip_profile_read(fyzz_hip_context* ctx, const char* name, double* total_ms,
                           uint64_t* count);
@@ -501,6 +501,11 @@ int fyzz_hip_world_broadphase_reports(fyzz_hip_world* world, int which, uint32_t
 int fyzz_hip_world_fetch_persistent(fyzz_hip_world* world, void* caches_bb, uint32_t pair_capacity,
                                     uint32_t* n_pairs, void* caches_pl, fyzz_dy_patch* history, uint32_t* stamps,
                                     uint32_t* step);
+/* Synchronizes; the touching flags the next step's island pass and event
+ * transitions read (non-zero: the pair touched on the last step):
+ * touch_bb[pair_capacity] aligned with the live pair list, touch_pl[n]
+ * per box against the floor (HOST arrays, either may be NULL). */
+int fyzz_hip_world_fetch_touch(fyzz_hip_world* world, uint32_t* touch_bb, uint32_t pair_capacity, uint32_t* touch_pl);
 /* Device-to-device copy of the body records (creation order). */
 int fyzz_hip_world_copy_bodies(fyzz_hip_world* world, fyzz_dy_body* dst);
 /* Device pointers of the last step's manager list, contact impulses

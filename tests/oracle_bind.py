@@ -375,6 +375,133 @@ def replay_dy_scene(orc, g, on_step, fp64=False, anchors=False):
             on_step(s, out, extra, mg)
 
 
+def _components(n, edges):
+    """Union-find roots (smallest member) of n nodes over (a, b) edges."""
+    parent = list(range(n))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+    for a, b in edges:
+        ra, rb = find(a), find(b)
+        if ra != rb:
+            parent[max(ra, rb)] = min(ra, rb)
+    return [find(i) for i in range(n)]
+
+
+def island_pass(b, pairs, prev_pairs, touch_bb, prev_touch_bb, touch_pl, prev_touch_pl):
+    """The island pass of a step on body records b (flags: bit0 sleeping,
+    bit1 ready, bit3 activating), in place, for dynamic boxes in slots:
+    Scene::activate_published_islands (scene.cpp:6148-6262) then
+    deactivate_ready_accurate_islands (6465-6536), ordered as Scene::step
+    orders them (10747-10793). pairs / prev_pairs: this and the last
+    step's broadphase pairs (lo, hi); touch_*: touching box/box pairs and
+    plane boxes of this and the last step. Wake propagates through the
+    speculative islands (components over both steps' broadphase pairs: the
+    pairs lost this step are split only after the wake); sleep takes the
+    accurate islands (components over this step's touching pairs) whose
+    members are all ready and one awake."""
+    f = b["flags"]
+    n = len(b)
+    prev_set = set(prev_pairs)
+    act = set(np.nonzero(f & 8)[0].tolist())
+    for lo, hi in pairs:
+        if (lo, hi) not in prev_set and not (f[lo] & f[hi] & 1):
+            act.update((lo, hi))
+    for key in touch_bb:
+        if key not in prev_touch_bb:
+            act.update(key)
+            f[key[0]] &= ~np.uint32(2)
+            f[key[1]] &= ~np.uint32(2)
+    for i in touch_pl:
+        if i not in prev_touch_pl:
+            act.add(i)
+            f[i] &= ~np.uint32(2)
+    if act and (f & 1).any():
+        root = _components(n, list(prev_set | set(pairs)))
+        wake_roots = {root[i] for i in act}
+        for i in range(n):
+            if (f[i] & 1) and root[i] in wake_roots:
+                f[i] &= ~np.uint32(3)
+                b["wake_counter"][i] = max(np.float32(b["wake_counter"][i]), np.float32(0.4))
+                b["sleep_linear_acc"][i] = 0.0
+                b["sleep_angular_acc"][i] = 0.0
+    f &= ~np.uint32(8)
+    if ((f & 3) == 2).any():
+        root = _components(n, list(touch_bb))
+        notready, awake = set(), set()
+        for i in range(n):
+            if not (f[i] & 2):
+                notready.add(root[i])
+            if not (f[i] & 1):
+                awake.add(root[i])
+        for i in range(n):
+            r = root[i]
+            if not (f[i] & 1) and r not in notready and r in awake:
+                f[i] |= np.uint32(1)
+                b["wake_counter"][i] = 0.0
+                for fld in ("linear_velocity", "angular_velocity", "sleep_linear_acc", "sleep_angular_acc"):
+                    b[fld][i] = 0.0
+    b["flags"] = f
+
+
+def island_pass_arrays(b, pairs, prev_pairs, src, touch_bb, prev_touch_bb, touch_pl, prev_touch_pl):
+    """island_pass on arrays, for whole piles: pairs / prev_pairs [k, 2]
+    (lo, hi) slots; src: each current pair's previous index (-1: found this
+    step); touch_bb (aligned with pairs) and prev_touch_bb (aligned with
+    prev_pairs) touching flags; touch_pl / prev_touch_pl per box (None: no
+    floor). Same semantics; the components come from scipy."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    n = len(b)
+    f = b["flags"].astype(np.uint32)
+
+    def labels(edges):
+        if len(edges) == 0:
+            return np.arange(n)
+        m = coo_matrix((np.ones(len(edges), np.int8), (edges[:, 0], edges[:, 1])), shape=(n, n))
+        return connected_components(m, directed=False)[1]
+    pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
+    prev_pairs = np.asarray(prev_pairs, np.int64).reshape(-1, 2)
+    act = (f & 8) != 0
+    a, c = pairs[:, 0], pairs[:, 1]
+    found = (src < 0) & ((f[a] & f[c] & 1) == 0)
+    act[a[found]] = True
+    act[c[found]] = True
+    was = np.zeros(len(pairs), bool)
+    if len(prev_pairs):
+        ok = src >= 0
+        was[ok] = prev_touch_bb[src[ok]] != 0
+    new_touch = (touch_bb != 0) & ~was
+    for col in (a, c):
+        act[col[new_touch]] = True
+        f[col[new_touch]] &= ~np.uint32(2)
+    if touch_pl is not None:
+        nt = (touch_pl != 0) & (prev_touch_pl == 0)
+        act[nt] = True
+        f[nt] &= ~np.uint32(2)
+    if act.any() and (f & 1).any():
+        lab = labels(np.concatenate([prev_pairs, pairs]))
+        wake = ((f & 1) != 0) & np.isin(lab, np.unique(lab[act]))
+        f[wake] &= ~np.uint32(3)
+        b["wake_counter"][wake] = np.maximum(b["wake_counter"][wake], np.float32(0.4))
+        b["sleep_linear_acc"][wake] = 0.0
+        b["sleep_angular_acc"][wake] = 0.0
+    f &= ~np.uint32(8)
+    if ((f & 3) == 2).any():
+        lab = labels(pairs[touch_bb != 0])
+        notready = np.unique(lab[(f & 2) == 0])
+        awake = np.unique(lab[(f & 1) == 0])
+        sleep = ((f & 1) == 0) & ~np.isin(lab, notready) & np.isin(lab, awake)
+        f[sleep] |= np.uint32(1)
+        b["wake_counter"][sleep] = 0.0
+        for fld in ("linear_velocity", "angular_velocity", "sleep_linear_acc", "sleep_angular_acc"):
+            b[fld][sleep] = 0.0
+    b["flags"] = f
+
+
 class OracleWorld:
     """The whole step pipeline composed from the oracle's SAP, PCM and Dy,
     in the device world's manager order (touching box/box managers in pair
@@ -416,6 +543,7 @@ class OracleWorld:
         self.pl_caches = [self.pcm.new_cache() for _ in range(n)]
         self.history = {}
         self.last = None
+        self.prev_pairs, self.prev_touch_bb, self.prev_touch_pl = [], set(), set()
 
     def step(self, dt):
         n, half, b = self.n, self.half, self.bodies
@@ -447,6 +575,18 @@ class OracleWorld:
                     pts[:, 3:6] *= -1.0
                     mgrs.append((i, -1, cnt, pts))
                     keys.append(("pl", i))
+        # the island pass: wakes, then sleeps; sleeping islands leave the
+        # solver list (their manifolds stay as they are)
+        cur_pairs = [(int(lo), int(hi)) for lo, hi in pairs]
+        touch_bb = {(k[1], k[2]) for k in keys if k[0] == "bb"}
+        touch_pl = {k[1] for k in keys if k[0] == "pl"}
+        island_pass(self.bodies, cur_pairs, self.prev_pairs, touch_bb, self.prev_touch_bb, touch_pl,
+                    self.prev_touch_pl)
+        self.prev_pairs, self.prev_touch_bb, self.prev_touch_pl = cur_pairs, touch_bb, touch_pl
+        b = self.bodies
+        keep = [k for k, m in enumerate(mgrs) if not (b["flags"][m[0]] & 1)]
+        mgrs = [mgrs[k] for k in keep]
+        keys = [keys[k] for k in keep]
         mg = np.zeros(len(mgrs), DY_MANAGER)
         hist = []
         for k, (b0, b1, cnt, pts) in enumerate(mgrs):
@@ -473,15 +613,17 @@ class OracleWorld:
 
     # ---- authored edits between steps, as the reference's Scene setters
     # make them (scene.cpp:589-781, 1022-1147); i is a box slot
-    def _wake(self, i):
+    def _wake(self, i, queue=False):
         """The body's own wake: wake counter raised to 0.4 s, sleep flags
-        cleared, a sleeping body's sleep filter dropped."""
+        cleared; a sleeping body also drops its sleep filter and is queued
+        for the next step's island wake (bit3), as is a peer (queue)."""
         b = self.bodies
         if b["flags"][i] & 1:
             b["sleep_linear_acc"][i] = 0.0
             b["sleep_angular_acc"][i] = 0.0
+            queue = True
         b["wake_counter"][i] = max(np.float32(b["wake_counter"][i]), np.float32(0.4))
-        b["flags"][i] = np.uint32(b["flags"][i]) & np.uint32(~3 & 0xFFFFFFFF)
+        b["flags"][i] = (np.uint32(b["flags"][i]) & np.uint32(~3 & 0xFFFFFFFF)) | np.uint32(8 if queue else 0)
 
     def set_pose(self, i, position, orientation):
         """Scene::set_pose: normalized orientation (a translation-only edit
@@ -506,11 +648,11 @@ class OracleWorld:
             for key in list(self.history):
                 if (key[0] == "bb" and i in key[1:]) or key == ("pl", i):
                     del self.history[key]
-            if self.last is not None:
-                for m in self.last[1]:
-                    other = m["body1"] if m["body0"] == i else m["body0"] if m["body1"] == i else -1
-                    if other >= 0 and (b["flags"][other] & 3):
-                        self._wake(other)
+            # peers of every touching manifold (sleeping islands' too)
+            for lo, hi in self.prev_touch_bb:
+                other = hi if lo == i else lo if hi == i else -1
+                if other >= 0 and (b["flags"][other] & 3):
+                    self._wake(other, queue=True)
         b["position"][i] = p
         b["orientation"][i] = nq
 
@@ -549,6 +691,23 @@ class OracleWorld:
 
     def body_states(self):
         return body_state_from(self.bodies)
+
+
+def world_step_events(lib, ow, prev):
+    """The contact events of an OracleWorld's last step: its solver
+    managers (awake islands) and the touches that started or ended. prev:
+    the sorted touching shape pairs of the step before (the value this
+    returns for it). Returns (events, offsets, impulses, points, point
+    impulses, touching shape pairs). A touch that persists inside a
+    sleeping island is not reported, and is not lost either."""
+    pairs, mg, imp, _ = ow.last
+    keys = [ow.shape_pair(k) for k in ow.last_keys]
+    touching = sorted([ow.shape_pair(("bb",) + k) for k in ow.prev_touch_bb] +
+                      [ow.shape_pair(("pl", i)) for i in ow.prev_touch_pl])
+    tset, kset = set(touching), set(keys)
+    prev = [p for p in prev if not (p in tset and p not in kset)]
+    out = oracle_contact_events(lib, mg, imp, keys, [True] * len(keys), prev)
+    return out + (touching,)
 
 
 def oracle_contact_events(lib, managers, impulses, keys, reversed_flags, prev_pairs, body_of_shape=None):
@@ -724,9 +883,17 @@ class StepOracle:
         if rc != 0:
             raise OverflowError(f"oracle PCM plane/box manifold invalid at box {-1 - rc}")
         pts_pl[:, :, 3:6] *= -1.0  # the scene publishes the plane's +y normal
-        # touching managers in the world's order, with their history slots
+        # the island pass (wakes, then sleeps) on the step's bodies
+        b = bodies.copy()
+        island_pass_arrays(b, pairs, prev_pairs, src, cnt_bb[:npairs], state["touch_bb"], cnt_pl,
+                           state["touch_pl"])
+        asleep = (b["flags"] & 1) != 0
+        # touching managers of awake islands in the world's order, with
+        # their history slots
         tb = np.nonzero(cnt_bb[:npairs])[0]
+        tb = tb[~asleep[first[tb]]]
         tp = np.nonzero(cnt_pl)[0]
+        tp = tp[~asleep[tp]]
         nm = len(tb) + len(tp)
         mg = np.zeros(nm, DY_MANAGER)
         mg["body0"][: len(tb)] = first[tb]
@@ -750,7 +917,6 @@ class StepOracle:
         hist_idx[ok] = np.arange(int(ok.sum()), dtype=np.uint32)
         mg["history"] = hist_idx
         hist = state["history"][hslot[ok]] if ok.any() else np.zeros(0, DY_PATCH)
-        b = bodies.copy()
         counted = np.bincount(pairs.ravel(), minlength=n).astype(np.uint32) + 1
         b["counted_interactions"] = counted
         from fyzz_amd.dy import FRICTION_ANCHORS

@@ -37,8 +37,10 @@ def state_of(ow, step_no):
         s = slot[key] if key[0] == "bb" else p + key[1]
         hist[s] = patch
         stamps[s] = step_no
+    touch_bb = np.array([(int(lo), int(hi)) in ow.prev_touch_bb for lo, hi in pairs], np.uint32)
+    touch_pl = np.array([i in ow.prev_touch_pl for i in range(n)], np.uint32)
     return {"caches_bb": cbb, "caches_pl": np.stack(ow.pl_caches).astype(np.float32), "history": hist,
-            "stamps": stamps, "step": step_no}
+            "stamps": stamps, "step": step_no, "touch_bb": touch_bb, "touch_pl": touch_pl}
 
 
 def test_step_oracle_matches_composed_pipeline(orc):

@@ -48,10 +48,8 @@ def small_pile(side=3, seed=scenes.SEED_BASE + 23):
 def compare_events(orc, w, ow, prev, s):
     """Device events == the event oracle over the oracle world's managers
     and impulses; returns this step's touching shape pairs."""
-    from oracle_bind import oracle_contact_events
-    pairs, mg_o, imp_o, _ = ow.last
-    keys = [ow.shape_pair(k) for k in ow.last_keys]
-    ev_o, off_o, eimp_o, pt_o, pimp_o = oracle_contact_events(orc.lib, mg_o, imp_o, keys, [True] * len(keys), prev)
+    from oracle_bind import world_step_events
+    ev_o, off_o, eimp_o, pt_o, pimp_o, touching = world_step_events(orc.lib, ow, prev)
     ev, pt = w.contact_events()
     got = np.stack([ev["body0"], ev["body1"], ev["kind"], ev["contact_count"], ev["shape0"], ev["shape1"]], 1)
     np.testing.assert_array_equal(got.astype(np.uint32), ev_o, err_msg=f"step {s} events")
@@ -60,7 +58,7 @@ def compare_events(orc, w, ow, prev, s):
     got_pt = np.concatenate([pt["position"], pt["normal"], pt["separation"][:, None]], 1)
     np.testing.assert_allclose(got_pt, pt_o, rtol=1e-5, atol=1e-6, err_msg=f"step {s} points")
     np.testing.assert_allclose(pt["normal_impulse"], pimp_o, rtol=1e-4, atol=1e-5, err_msg=f"step {s} point impulses")
-    return sorted(keys), np.bincount(ev_o[:, 2], minlength=3)
+    return touching, np.bincount(ev_o[:, 2], minlength=3)
 
 
 def run_compare(ctx, orc, sc, steps, dt=1 / 120, events=False):

@@ -9,6 +9,7 @@ of the hot path:
   friction write-back                 counts exact; anchor positions 1e-5,
                                       anchor impulses 1e-4 relative
   body state after the step           1e-4 relative (1e-6 absolute)
+  sleeping and ready flags            exact
 Test helper, not a test module."""
 import time
 
@@ -56,6 +57,9 @@ def compare_step(w, so, dt, label, reports=True):
                                err_msg=f"{label} anchor positions")
     np.testing.assert_allclose(an["impulse"][live], wa["impulse"][live], rtol=1e-4, atol=1e-5,
                                err_msg=f"{label} anchor impulses")
+    rec = w.body_records()
+    np.testing.assert_array_equal(rec["flags"] & 1, want["bodies"]["flags"] & 1, err_msg=f"{label} sleeping")
+    np.testing.assert_array_equal(rec["flags"] & 2, want["bodies"]["flags"] & 2, err_msg=f"{label} ready")
     got_b, want_b = w.body_states(), body_state_from(want["bodies"])
     np.testing.assert_allclose(got_b, want_b, rtol=1e-4, atol=1e-6, err_msg=f"{label} bodies")
     exact = int(np.sum(np.all(got_b == want_b, axis=1)))
