@@ -540,11 +540,10 @@ __global__ void k_accumulate(const u32* d_pairs, u32 cap, const u32* d_nm, const
 //     ready bodies can sleep, so the union takes the touches between two
 //     ready bodies; a ready body touching one that is not ready poisons
 //     its component (that body is in the same island).
-// The union kernels read and write parents with agent-scope atomics (the
-// workgroups of one launch run on different XCDs, whose L2 caches are not
-// coherent with each other); the kernels after them flatten the parents
-// into labels and read them with plain loads (kernel boundaries make the
-// union's results visible). Every kernel reads its gate from device
+// The union kernels hook roots with agent-scope compare-and-swap and walk
+// parents with plain loads (see unite); the kernels after them flatten the
+// parents into labels (kernel boundaries make the union's results
+// visible). Every kernel reads its gate from device
 // counters, so the pass costs a few empty launches on steps where nothing
 // can sleep or wake, and it can be captured in a graph.
 constexpr u32 kSleeping = 1u, kReady = 2u, kActivating = 8u;
@@ -558,32 +557,39 @@ enum IslandCounter : u32 {
     kIslCounters
 };
 
-__device__ inline u32 ld_parent(const u32* parent, u32 x) {
-    return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline u32 find_root(u32* parent, u32 x) {
-    for (;;) {
-        const u32 p = ld_parent(parent, x);
-        if (p == x) return x;
-        const u32 g = ld_parent(parent, p);
-        if (g == p) return p;
-        // path halving: a non-root's parent only moves toward its root
-        __hip_atomic_store(parent + x, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = g;
+// Union-find on hardware whose L2 caches are not coherent across XCDs:
+// plain loads may return an older parent, which is still an ancestor (a
+// parent only ever moves up, to a smaller slot), so a walk from it still
+// ends at a node that was a root once. Only hooking is atomic (agent-scope
+// compare-and-swap, performed coherently): a hook on a node that is no
+// longer a root fails and returns its current parent, where the walk goes
+// on. Every failure moves strictly up, so the union ends; plain loads keep
+// the shared upper nodes in each XCD's L2 instead of sending every read of
+// a giant component's root to memory.
+__device__ inline u32 find_plain(const u32* parent, u32 x) {
+    u32 p = parent[x];
+    while (p != x) {
+        x = p;
+        p = parent[x];
     }
+    return x;
 }
 __device__ inline void unite(u32* parent, u32 a, u32 b) {
     // most edges join bodies already hanging from the same node
-    if (ld_parent(parent, a) == ld_parent(parent, b)) return;
-    for (;;) {
-        a = find_root(parent, a);
-        b = find_root(parent, b);
-        if (a == b) return;
+    if (parent[a] == parent[b]) return;
+    a = find_plain(parent, a);
+    b = find_plain(parent, b);
+    while (a != b) {
         const u32 hi = a > b ? a : b, lo = a > b ? b : a;
         u32 expect = hi;
         if (__hip_atomic_compare_exchange_strong(parent + hi, &expect, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT))
             return;
+        const u32 up = find_plain(parent, expect);  // hi had been hooked: go on from its parent
+        if (hi == a)
+            a = up;
+        else
+            b = up;
     }
 }
 // After the union kernel: the root by plain loads (parents only point down
@@ -737,11 +743,11 @@ __global__ void k_isl_spec_compress(IslandArgs A) {
         A.spec[i] = root_of(A.spec, i);
 }
 
-// The most frequent label among 1,024 sampled bodies (one workgroup): the
+// The most frequent label among 256 sampled bodies (one workgroup): the
 // giant component, whose internal edges the full union skips.
 __global__ void k_isl_spec_giant(IslandArgs A) {
     if (!spec_needed(A)) return;
-    constexpr u32 kSamples = 1024;
+    constexpr u32 kSamples = 256;
     __shared__ u32 lab[kSamples];
     for (u32 k = threadIdx.x; k < kSamples; k += blockDim.x)
         lab[k] = A.n ? A.spec[static_cast<u32>((static_cast<u64>(k) * A.n) / kSamples)] : 0u;
@@ -772,7 +778,7 @@ __global__ void k_isl_union_spec(IslandArgs A) {
     const u32 nbb = min(*A.d_cur, A.cap), ng = min(*A.d_gone, A.cap);
     const u32 giant = A.ctr[kIslGiant];
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb + ng; i += gridDim.x * blockDim.x) {
-        if (link_edge(A, i, nbb)) continue;
+        if (link_edge(A, i, nbb)) continue;  // already united
         const u64 x = spec_edge(A, i, nbb);
         const u32 a = edge_lo(x), b = edge_hi(x);
         if (A.spec[a] == giant && A.spec[b] == giant) continue;
