@@ -1,0 +1,250 @@
+"""The compiled reference NativeCpu scene (oracle/_ref/libfyzzref.so, built
+from the reference's own sources by oracle/Makefile) stepped live as a
+checker: each step yields what a Dy replay in the reference's own solver
+order needs and what it must reproduce. Test helper, not a test module;
+tests that use it skip when the library was not built.
+
+Per step s: the start state (13 floats per box: position, linear velocity,
+orientation, angular velocity), wake counters, ready and sleeping flags;
+the step's descriptor stream (the reference's island order) as Dy manager
+records with the reference's published contacts (plane contacts with the
+published +y normal, box/box pairs with the higher body first, as the
+narrowphase hands them to Dy); each descriptor's partition local to its
+task; the broadphase pair counts per box; and the state after the step."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+REF_LIB = ROOT / "oracle" / "_ref" / "libfyzzref.so"
+P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+
+
+def available() -> bool:
+    return REF_LIB.exists()
+
+
+def _load():
+    ref = C.CDLL(str(REF_LIB))
+    ref.ref_scene_create.restype = C.c_void_p
+    ref.ref_last_error.restype = C.c_char_p
+    for f in ("ref_pairs", "ref_manifolds", "ref_dy_descriptors", "ref_dy_tasks"):
+        getattr(ref, f).restype = C.c_size_t
+    return ref
+
+
+class RefRun:
+    """A scenes.Scene record authored into the reference scene (floor id 0,
+    boxes 1..n in order, as the device world authors it)."""
+
+    def __init__(self, scene):
+        self.ref = ref = _load()
+        self.s = C.c_void_p(ref.ref_scene_create())
+        self.n = n = scene.n
+        fid = C.c_uint32()
+        self._ok(ref.ref_add_floor(self.s, C.c_float(scene.floor_height), C.byref(fid)))
+        self._ok(ref.ref_set_material(self.s, fid, C.c_float(scene.friction), C.c_float(scene.friction),
+                                      C.c_float(scene.restitution)))
+        self.ids = np.zeros(n, np.uint32)
+        for i in range(n):
+            bid = C.c_uint32()
+            self._ok(ref.ref_add_dynamic_box(self.s, P(np.ascontiguousarray(scene.half[i], np.float32)),
+                                             P(np.ascontiguousarray(scene.pos[i], np.float32)),
+                                             C.c_float(scene.density), C.byref(bid)))
+            self.ids[i] = bid.value
+        for i in range(n):
+            b = C.c_uint32(int(self.ids[i]))
+            self._ok(ref.ref_set_pose(self.s, b, P(np.ascontiguousarray(scene.pos[i], np.float32)),
+                                      P(np.ascontiguousarray(scene.quat[i], np.float32))))
+            self._ok(ref.ref_set_velocity(self.s, b, P(np.ascontiguousarray(scene.vel[i], np.float32))))
+            if np.any(scene.ang[i]):
+                self._ok(ref.ref_set_angular_velocity(self.s, b, P(np.ascontiguousarray(scene.ang[i], np.float32))))
+            self._ok(ref.ref_set_material(self.s, b, C.c_float(scene.friction), C.c_float(scene.friction),
+                                          C.c_float(scene.restitution)))
+        assert list(self.ids) == list(range(1, n + 1))
+
+    def _ok(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.ref.ref_last_error().decode())
+
+    def close(self):
+        if self.s:
+            self.ref.ref_scene_destroy(self.s)
+            self.s = None
+
+    def state(self):
+        st = np.zeros(14, np.float32)
+        out = np.zeros((self.n, 13), np.float32)
+        sleeping = np.zeros(self.n, bool)
+        wake = np.zeros(self.n, np.float32)
+        ready = np.zeros(self.n, bool)
+        for i in range(self.n):
+            b = C.c_uint32(int(self.ids[i]))
+            self._ok(self.ref.ref_body_state(self.s, b, P(st)))
+            out[i] = st[:13]
+            sleeping[i] = st[13] > 0
+            w, r = C.c_float(), C.c_int()
+            self._ok(self.ref.ref_sleep_state(self.s, b, C.byref(w), C.byref(r)))
+            wake[i], ready[i] = w.value, bool(r.value)
+        return out, wake, ready, sleeping
+
+    def step(self, dt):
+        """Steps once; returns the step's Dy inputs and outputs (a dict)."""
+        from fyzz_amd.dy import DY_MANAGER
+        ref, n = self.ref, self.n
+        state, wake, ready, sleeping = self.state()
+        self._ok(ref.ref_step(self.s, C.c_float(dt)))
+        m = ref.ref_manifolds(self.s, None, None, None, 0)
+        mp = np.zeros(2 * max(m, 1), np.uint32)
+        mc = np.zeros(max(m, 1), np.uint8)
+        pts = np.zeros(max(m, 1) * 28, np.float32)
+        ref.ref_manifolds(self.s, P(mp), P(mc), P(pts), m)
+        mp, pts = mp.reshape(-1, 2), pts.reshape(-1, 4, 7)
+        d = ref.ref_dy_descriptors(self.s, None, None, None, None, 0)
+        di = np.zeros(max(d, 1), np.uint32)
+        b0 = np.zeros(max(d, 1), np.uint32)
+        b1 = np.zeros(max(d, 1), np.uint32)
+        part = np.zeros(max(d, 1), np.uint64)
+        ref.ref_dy_descriptors(self.s, P(di), P(b0), P(b1), P(part), d)
+        t = ref.ref_dy_tasks(self.s, None, None, 0)
+        me = np.zeros(max(t, 1), np.uint64)
+        pe = np.zeros(max(t, 1), np.uint64)
+        ref.ref_dy_tasks(self.s, P(me), P(pe), t)
+        mg = np.zeros(d, DY_MANAGER)
+        keys = []
+        for k in range(d):
+            a, b = (int(x) for x in mp[di[k]])
+            cnt = int(mc[di[k]])
+            if a == 0:  # the floor (body 0): the box first, the static second
+                mg[k]["body0"], mg[k]["body1"] = b - 1, -1
+            else:  # two boxes: the higher body first
+                mg[k]["body0"], mg[k]["body1"] = b - 1, a - 1
+            mg[k]["count"] = cnt
+            mg[k]["history"] = 0xFFFFFFFF
+            for c in range(cnt):
+                mg[k]["contacts"][c]["position"] = pts[di[k], c, 0:3]
+                mg[k]["contacts"][c]["normal"] = pts[di[k], c, 3:6]
+                mg[k]["contacts"][c]["separation"] = pts[di[k], c, 6]
+            keys.append((a, b))
+        # partitions local to each task
+        glob = part[:d].astype(np.int64)
+        ends = pe[:t].astype(np.int64)
+        base = np.concatenate([[0], ends[:-1]]) if t else np.zeros(0, np.int64)
+        task = np.searchsorted(ends, glob, side="right")
+        local = glob - base[task] if d else glob
+        np_ = ref.ref_pairs(self.s, None, 0)
+        pr = np.zeros(2 * max(np_, 1), np.uint32)
+        ref.ref_pairs(self.s, P(pr), np_)
+        pr = pr[:2 * np_].reshape(-1, 2).astype(np.int64)
+        counted = np.bincount(pr[pr > 0] - 1, minlength=n).astype(np.uint32)
+        box = (pr[:, 0] > 0) & (pr[:, 1] > 0)
+        pairs = sorted((int(a) - 1, int(b) - 1) for a, b in pr[box])
+        live = mc[:m] > 0
+        touch_bb = {(int(a) - 1, int(b) - 1) for (a, b) in mp[:m][live] if a > 0}
+        touch_pl = {int(b) - 1 for (a, b) in mp[:m][live] if a == 0}
+        after = self.state()
+        return {"state": state, "wake": wake, "ready": ready, "sleeping": sleeping, "managers": mg, "keys": keys,
+                "partitions": local, "tasks": t, "counted": counted, "next": after, "pairs": pairs,
+                "touch_bb": touch_bb, "touch_pl": touch_pl}
+
+
+def dy_bodies_from(step, prev, half, density=1000.0, restitution=0.0, acc_lin=None, acc_ang=None, friction=0.5):
+    """Dy body records of a RefRun step: its start state (sleep accumulators
+    carried by the caller from its own previous step) after the step's
+    island pass (the oracle's, over the reference's pairs and touches of
+    this step and prev, the previous step's dict or None), which the
+    reference runs inside the step before Dy."""
+    from fyzz_amd import scenes
+    from oracle_bind import dy_bodies, island_pass
+    st = step["state"]
+    n = len(st)
+    inv_mass, inv_inertia = scenes.box_mass_properties(half, density)
+    b = dy_bodies(n)
+    b["position"] = st[:, 0:3]
+    b["linear_velocity"] = st[:, 3:6]
+    b["orientation"] = st[:, 6:10]
+    b["angular_velocity"] = st[:, 10:13]
+    b["inverse_mass"] = inv_mass
+    b["inverse_inertia"] = inv_inertia
+    b["wake_counter"] = step["wake"]
+    b["flags"] = 4 | (step["ready"].astype(np.uint32) << 1) | step["sleeping"].astype(np.uint32)
+    if acc_lin is not None:
+        b["sleep_linear_acc"] = acc_lin
+        b["sleep_angular_acc"] = acc_ang
+    b["counted_interactions"] = step["counted"]
+    b["restitution"] = restitution
+    b["static_friction"] = friction
+    b["dynamic_friction"] = friction
+    pp, pbb, ppl = (prev["pairs"], prev["touch_bb"], prev["touch_pl"]) if prev else ([], set(), set())
+    island_pass(b, step["pairs"], pp, step["touch_bb"], pbb, step["touch_pl"], ppl)
+    return b
+
+
+def statics_of(scene):
+    """The floor as the Dy static record the device world builds."""
+    from fyzz_amd.dy import DY_STATIC
+    st = np.zeros(1, DY_STATIC)
+    st["orientation"][0, 3] = 1.0
+    st["position"][0, 1] = scene.floor_height
+    st["static_friction"] = scene.friction
+    st["dynamic_friction"] = scene.friction
+    st["restitution"] = scene.restitution
+    return st
+
+
+def replay(scene, steps, solve, dt=1.0 / 120.0):
+    """Steps the reference scene and hands each step's Dy inputs, in the
+    reference's descriptor order from its synchronized start state, to
+    solve(s, step, bodies, statics, managers, history) -> (bodies_out,
+    patches); friction history (by shape pair) and the sleep accumulators
+    are carried from solve's results."""
+    from fyzz_amd.dy import DY_PATCH
+    run = RefRun(scene)
+    statics = statics_of(scene)
+    prev, history = None, {}
+    acc_lin = np.zeros((scene.n, 3), np.float32)
+    acc_ang = np.zeros((scene.n, 3), np.float32)
+    try:
+        for s in range(steps):
+            st = run.step(dt)
+            b = dy_bodies_from(st, prev, scene.half, scene.density, scene.restitution, acc_lin, acc_ang,
+                               scene.friction)
+            mg = st["managers"].copy()
+            hist = []
+            for i, k in enumerate(st["keys"]):
+                if k in history:
+                    mg[i]["history"] = len(hist)
+                    hist.append(history[k])
+            hist = np.array(hist, DY_PATCH) if hist else np.zeros(0, DY_PATCH)
+            out, patches = solve(s, st, b, statics, mg, hist)
+            history = {k: patches[i] for i, k in enumerate(st["keys"])}
+            acc_lin = out["sleep_linear_acc"].copy()
+            acc_ang = out["sleep_angular_acc"].copy()
+            prev = st
+    finally:
+        run.close()
+
+
+def hub_scene(side=8, seed=None):
+    """A slab (half extents 4.6 x 0.25 x 4.6) resting on the floor, a
+    side x side layer of boxes (half 0.45, seeded yaw and small velocities)
+    on it, and a (side-1)^2 layer offset by half a cell on top, each upper
+    box across four lower ones: the slab carries side^2 box/box contacts,
+    more partitions than a 32-bit colour mask holds (the overflow waves)."""
+    from fyzz_amd import scenes
+    seed = scenes.SEED_BASE + 31 if seed is None else seed
+    sp = 1.15
+    g = np.arange(side) - (side - 1) / 2
+    u = np.arange(side - 1) - (side - 2) / 2
+    lower = np.stack(np.meshgrid(g * sp, [0.5 + 0.45 + 0.03], g * sp, indexing="ij"), -1).reshape(-1, 3)
+    upper = np.stack(np.meshgrid(u * sp, [0.5 + 0.9 + 0.45 + 0.06], u * sp, indexing="ij"), -1).reshape(-1, 3)
+    n = len(lower) + len(upper)
+    half = np.concatenate([[[4.6, 0.25, 4.6]], np.full((n, 3), 0.45)])
+    pos = np.concatenate([[[0.0, 0.25, 0.0]], lower, upper])
+    yaw = np.concatenate([[0.0], (scenes.uniform(seed, 0, n) - 0.5) * 0.3])
+    quat = scenes.normalize_quat32(scenes.yaw_quat(yaw))
+    vel = np.concatenate([[[0.0, 0.0, 0.0]], np.stack([scenes.normal(seed, 1 + a, n) * 0.2 for a in range(3)], 1)])
+    return scenes.Scene(half, pos, quat, vel)
