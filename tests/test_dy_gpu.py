@@ -184,3 +184,44 @@ def test_overflow_waves_with_ten_thousand_pending(ctx, orc, torch_cuda):
     np.testing.assert_array_equal(gparts, wparts, err_msg="partitions")
     np.testing.assert_allclose(body_state_from(got), body_state_from(want), rtol=RTOL, atol=ATOL)
     print(f"partitions {nparts}, rounds {rounds}")
+
+
+@pytest.mark.parametrize("case", ["grid_1000", "hub_overflow"])
+def test_reference_order_at_scale(ctx, orc, torch_cuda, case):
+    """The HIP Dy stage fed the live compiled reference's descriptor stream
+    (its island order) from its synchronized post-island-pass states: the
+    1,000-box grid over 240 steps and the 65-partition hub slab over 60.
+    Partitions bit-exact against the reference's per-task numbering, next
+    states, wake counters and impulses at the bars above against the
+    reference (states) and the oracle on the same inputs (impulses), sleep
+    and ready flags exact; the oracle's history is carried."""
+    import ref_live
+    if not ref_live.available():
+        pytest.skip("compiled reference (oracle/_ref) not built")
+    from fyzz_amd import scenes
+    scene, steps = (scenes.grid_scene(), 240) if case == "grid_1000" else (ref_live.hub_scene(), 60)
+    odd = OracleDy(orc.lib)
+    solver = _solver(ctx, scene.n, 4096)
+    count = {"same": 0, "total": 0, "parts": 0}
+
+    def solve(s, st, b, statics, mg, hist):
+        want, wp, wi, wparts = odd.step(1 / 120, b, statics, mg, hist)
+        got, gp, gi, gparts, (flags, nparts, rounds) = _gpu_step(solver, 1 / 120, b, statics, mg, hist)
+        assert flags == 0, f"step {s} status {flags}"
+        np.testing.assert_array_equal(gparts.astype(np.int64), st["partitions"], err_msg=f"step {s} partitions")
+        nxt, wake, ready, sleeping = st["next"]
+        gs = body_state_from(got)
+        np.testing.assert_allclose(gs, nxt, rtol=RTOL, atol=ATOL, err_msg=f"step {s} vs reference")
+        np.testing.assert_allclose(got["wake_counter"], wake, rtol=RTOL, atol=ATOL, err_msg=f"step {s} wake")
+        np.testing.assert_array_equal(got["flags"], want["flags"], err_msg=f"step {s} flags")
+        np.testing.assert_array_equal((got["flags"] & 1).astype(bool), sleeping, err_msg=f"step {s} sleep")
+        np.testing.assert_allclose(gi, wi, rtol=RTOL, atol=1e-5, err_msg=f"step {s} impulses")
+        count["same"] += int((gs == nxt).all(axis=1).sum())
+        count["total"] += scene.n
+        count["parts"] = max(count["parts"], int(nparts))
+        return want, wp
+
+    ref_live.replay(scene, steps, solve)
+    assert count["parts"] >= (65 if case == "hub_overflow" else 11)
+    print(f"{case}: bodies bit-identical to the reference {count['same']}/{count['total']}, "
+          f"most partitions {count['parts']}")
