@@ -12,8 +12,10 @@ plane/box PCM with GJK/EPA continuation, touching-manager list, Dy:
 partitions, contact prep, PGS, integration), one replay of the captured
 step graph. The pile is first dropped and settled for --settle steps (600
 by default: the boxes land and form the pile, ~2e5 touching managers),
-untimed for the headline but clocked as a whole ("drop" in the line); then
-W warm-up steps; then K timed steps of the formed pile. `value` is
+untimed for the headline but clocked as a whole ("drop" in the line);
+with --autotune the library's autotuner then picks the partition and PGS
+workgroup counts on the formed pile; then W warm-up steps; then K timed
+steps of the formed pile. `value` is
 body-steps/s (bodies x timed steps / time; every body counts every step).
 K more steps of the same pile run with direct launches and stage timers
 (HIP events on the launch stream): their per-stage times give the
@@ -77,6 +79,8 @@ def parse():
     p.add_argument("--settle", type=int, default=600,
                    help="pile: untimed steps from the drop before the warm-up (the pile forms)")
     p.add_argument("--workload", default="pile", choices=["pile", "sap", "batch", "diff"])
+    p.add_argument("--autotune", action="store_true",
+                   help="pile: let the autotuner pick the partition/PGS workgroup counts after settling")
     p.add_argument("--no-extras", action="store_true",
                    help="pile: skip the attached 1M-box SAP and 4,096-scene batch measurements")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -439,6 +443,11 @@ def bench_pile(args, ctx, stream):
         drop = {"steps": settle, "ms_per_step": c.ms / settle, "value": n * settle / (c.ms * 1e-3),
                 "unit": "body-steps/s", "contact_rows_per_step": t["contact_rows"] / settle,
                 "managers_per_step": t["managers"] / settle}
+    # optionally: launch shapes of the partition and PGS kernels timed on
+    # the formed pile by the library's autotuner (its trial steps advance
+    # the pile; whole-step trials on a changing pile are noisy, so the
+    # default keeps the measured defaults)
+    tuned = w.autotune(DT, trial_steps=2) if args.autotune else None
     for _ in range(args.warmup):
         step(DT)
     w.totals(reset=True)
@@ -486,6 +495,7 @@ def bench_pile(args, ctx, stream):
                    "pairs_at_end": st["pairs"], "managers_at_end": st["managers"],
                    "partitions_at_end": st["partitions"],
                    "drop": drop,
+                   "autotuned_workgroups": ({"partition": tuned[0], "pgs": tuned[1]} if tuned else None),
                    "direct_launch_profiled": {"steps": args.steps,
                                               "ms_per_step": sum(v["ms_total"] for v in stages.values())
                                               / max(args.steps, 1),
