@@ -98,7 +98,7 @@ struct Ws {
     u32* pos_of;    // [nm] partition-sorted position of each manager in a partition
     uint4* srank;   // [nm] per position: rank of the manager among body0's / body1's (by partition), their degrees
     u32* erank;     // [2 nm] per incidence entry: rank among the body's managers by partition
-    u64* vst;       // [6 nb] sweep state: velocity-change component bits | updates applied << 32
+    u64* vst;       // [8 nb] sweep state, a 64-byte line per body: (component bits, updates applied) x 6, pad
     CRow* crow;
     FRow* frow;
     u32 *part_count, *part_off, *part_fill;
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_part_clear(Ws W) {
         W.fill[i] = 0;
         W.max_dyn[i] = 0;
 #pragma unroll
-        for (u32 k = 0; k < 6; ++k) W.vst[6 * i + k] = 0ull;  // 0.0f, no updates yet
+        for (u32 k = 0; k < 8; ++k) W.vst[8 * i + k] = 0ull;  // 0.0f, no updates yet
     }
     if (i < kMaxParts) {
         W.part_count[i] = 0;
@@ -1200,38 +1200,40 @@ __global__ __launch_bounds__(128) void k_prep(Ws W) {
 
 // scene.cpp:8985-9010 (normal rows), 9050-9120 (friction rows)
 // Coherent body state: during the sweep a body's velocity change lives in
-// six 64-bit words, each a component's bits with the number of updates
-// applied to the body in the upper half, moved between workgroups on
-// different XCDs by agent-scope relaxed atomics. A reader takes the state
-// when all six words carry the count it waits for, so a writer needs no
-// wait between its state stores and a separate counter.
+// one 64-byte line of three 16-byte chunks, each chunk two components'
+// bits, each followed by the number of updates applied to the body, moved
+// between workgroups on different XCDs by device-coherent (sc1) 16-byte
+// loads and stores (one request per chunk). A reader takes the state when
+// all six counts carry the count it waits for: a count is written once
+// per body, so matching counts give the values written with it, and a
+// writer needs no wait between its state stores and a separate counter.
 struct BodyState {
     V v, w;
 };
-// The first word is polled alone (waiting costs one load per poll); once
-// it carries the count, the other five are loaded and checked. seen: the
-// count the first word carried (the updates the body has had so far).
-__device__ inline bool poll_state(const u64* q, u32 need, BodyState& s, u32& seen) {
-    u64 x[6];
-    x[0] = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    seen = static_cast<u32>(x[0] >> 32);
-    if (seen != need) return false;
-#pragma unroll
-    for (int k = 1; k < 6; ++k) x[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    bool ok = true;
-#pragma unroll
-    for (int k = 1; k < 6; ++k) ok &= static_cast<u32>(x[k] >> 32) == need;
-    s.v = {__uint_as_float(static_cast<u32>(x[0])), __uint_as_float(static_cast<u32>(x[1])),
-           __uint_as_float(static_cast<u32>(x[2]))};
-    s.w = {__uint_as_float(static_cast<u32>(x[3])), __uint_as_float(static_cast<u32>(x[4])),
-           __uint_as_float(static_cast<u32>(x[5]))};
-    return ok;
+typedef u32 Chunk __attribute__((ext_vector_type(4)));
+constexpr u32 kStateBytes = 64;
+constexpr int kDeviceCoherent = 16;  // cache policy: sc1 (device scope)
+__device__ inline __amdgpu_buffer_rsrc_t state_rsrc(const Ws& W) {
+    // raw buffer over the state lines, range check off (offsets < 2^32)
+    return __builtin_amdgcn_make_buffer_rsrc(W.vst, 0, 0xFFFFFFFF, 0x00020000);
 }
-__device__ inline void publish_state(u64* q, u32 count, const V& v, const V& w) {
-    const u64 hi = static_cast<u64>(count) << 32;
-    const float c[6] = {v.x, v.y, v.z, w.x, w.y, w.z};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) __hip_atomic_store(q + k, hi | __float_as_uint(c[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline Chunk load_chunk(__amdgpu_buffer_rsrc_t R, u32 off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(R, off, 0, kDeviceCoherent);
+}
+// the state of three chunks as loaded, and whether all counts are `need`
+__device__ inline bool take_state(Chunk a, Chunk b, Chunk c, u32 need, BodyState& s) {
+    s.v = {__uint_as_float(a.x), __uint_as_float(a.z), __uint_as_float(b.x)};
+    s.w = {__uint_as_float(b.z), __uint_as_float(c.x), __uint_as_float(c.z)};
+    return a.y == need && a.w == need && b.y == need && b.w == need && c.y == need && c.w == need;
+}
+__device__ inline void publish_state(__amdgpu_buffer_rsrc_t R, u32 body, u32 count, const V& v, const V& w) {
+    const u32 off = body * kStateBytes;
+    const Chunk a = {__float_as_uint(v.x), count, __float_as_uint(v.y), count};
+    const Chunk b = {__float_as_uint(v.z), count, __float_as_uint(w.x), count};
+    const Chunk c = {__float_as_uint(w.y), count, __float_as_uint(w.z), count};
+    __builtin_amdgcn_raw_buffer_store_b128(a, R, off, 0, kDeviceCoherent);
+    __builtin_amdgcn_raw_buffer_store_b128(b, R, off + 16u, 0, kDeviceCoherent);
+    __builtin_amdgcn_raw_buffer_store_b128(c, R, off + 32u, 0, kDeviceCoherent);
 }
 // Clock cycles before a body still (need - seen) updates short of a
 // waiting manager is polled again: about one solve hop per missing update
@@ -1264,7 +1266,7 @@ __device__ inline void load_rows(const Ws& W, int4 info, CRow (&rows)[4], FRow (
     }
 }
 // rows / frows: the manager's rows as loaded by load_rows.
-__device__ void solve_manager(const Ws& W, int4 info, CRow (&rows)[4], FRow (&frows)[4], const BodyState& s0,
+__device__ void solve_manager(const Ws& W, int4 info, CRow (&rows)[4], FRow (&frows)[4], float4 fric, const BodyState& s0,
                               const BodyState& s1, float ima, float imb, u32 c0, u32 c1, bool snap0, bool snap1,
                               bool fin0, bool fin1, u32 iter, u32 pos_iters, u32 total_iters) {
     const int i0 = info.x, i1 = info.y;
@@ -1295,17 +1297,15 @@ __device__ void solve_manager(const Ws& W, int4 info, CRow (&rows)[4], FRow (&fr
         vb = {vb.x - n.x * db, vb.y - n.y * db, vb.z - n.z * db};
         wa = {wa.x + ra.x * delta, wa.y + ra.y * delta, wa.z + ra.z * delta};
         wb = {wb.x - rb.x * delta, wb.y - rb.y * delta, wb.z - rb.z * delta};
-        bad |= !finite3(va) || !finite3(vb) || !finite3(wa) || !finite3(wb);
         r.applied = next;
         W.crow[4 * m + q].applied = next;
         normal_impulse += next;
     }
-    fyzz_dy_patch& P = W.patches[m];
     if (W.tape && iter < W.tape_iters) W.tape_ni[static_cast<std::size_t>(iter) * W.nm + m] = normal_impulse;
     if ((iter >= pos_iters || pos_iters - iter <= 3u) && anchors) {
         const float coef = anchors == 2 ? 0.5f : 1.0f;
-        const float max_static = P.static_friction * coef * normal_impulse;
-        const float max_dynamic = P.dynamic_friction * coef * normal_impulse;
+        const float max_static = fric.x * coef * normal_impulse;
+        const float max_dynamic = fric.y * coef * normal_impulse;
         bool broken = false;
 #pragma unroll
         for (u32 k = 0; k < 4; ++k) {
@@ -1328,16 +1328,19 @@ __device__ void solve_manager(const Ws& W, int4 info, CRow (&rows)[4], FRow (&fr
             vb = {vb.x - t.x * db, vb.y - t.y * db, vb.z - t.z * db};
             wa = {wa.x + ra.x * delta, wa.y + ra.y * delta, wa.z + ra.z * delta};
             wb = {wb.x - rb.x * delta, wb.y - rb.y * delta, wb.z - rb.z * delta};
-            bad |= !finite3(va) || !finite3(vb) || !finite3(wa) || !finite3(wb);
             fr.applied = next;
             W.frow[4 * m + k].applied = next;
             broken |= clamped;
         }
-        if (iter + 1 == total_iters) P.flags = (P.flags & ~1u) | (broken ? 1u : 0u);
+        if (iter + 1 == total_iters) W.patches[m].flags = (__float_as_uint(fric.z) & ~1u) | (broken ? 1u : 0u);
     }
+    // a velocity component once non-finite stays so through later row
+    // updates (sums with finite or non-finite terms), so the final check
+    // equals a check after every row
+    bad = !finite3(va) || !finite3(vb) || !finite3(wa) || !finite3(wb);
     if (bad) raise_flag(W, FYZZ_DY_NONFINITE);
     if (i0 >= 0) {
-        publish_state(W.vst + 6ull * i0, c0, va, wa);
+        publish_state(state_rsrc(W), static_cast<u32>(i0), c0, va, wa);
         const float4 l = make_float4(va.x, va.y, va.z, ima), a = make_float4(wa.x, wa.y, wa.z, 0.f);
         if (snap0) {
             W.mlin[i0] = l;
@@ -1349,7 +1352,7 @@ __device__ void solve_manager(const Ws& W, int4 info, CRow (&rows)[4], FRow (&fr
         }
     }
     if (i1 >= 0) {
-        publish_state(W.vst + 6ull * i1, c1, vb, wb);
+        publish_state(state_rsrc(W), static_cast<u32>(i1), c1, vb, wb);
         const float4 l = make_float4(vb.x, vb.y, vb.z, imb), a = make_float4(wb.x, wb.y, wb.z, 0.f);
         if (snap1) {
             W.mlin[i1] = l;
@@ -1385,6 +1388,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
     const u32 total = pos_iters + vel_iters;
     const u32 n = np ? W.part_off[np] : 0u;
     u32 iter = 0, i = tid, spins = 0, idle = 0;
+    const __amdgpu_buffer_rsrc_t R = state_rsrc(W);
     u64 wake0 = 0, wake1 = 0;  // earliest clock at which each body is polled again
     bool have = false, ready0 = false, ready1 = false;
     int4 info = make_int4(-1, -1, 0, 0);
@@ -1393,6 +1397,7 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
     float ima = 0.f, imb = 0.f;
     CRow rows[4];
     FRow frows[4];
+    float4 fric = make_float4(0.f, 0.f, 0.f, 0.f);  // static and dynamic friction, patch flags
     // the position record of the thread's next position, loaded while the
     // current manager waits (positions repeat in every iteration)
     u32 next_pos = kNone;
@@ -1425,6 +1430,12 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
                 // the rows are loaded while the bodies are awaited (only
                 // this thread writes them during the sweep)
                 load_rows(W, info, rows, frows);
+                {
+                    // the patch's friction coefficients and flags (only
+                    // this thread writes them during the sweep)
+                    const fyzz_dy_patch& P = W.patches[info.z];
+                    fric = make_float4(P.static_friction, P.dynamic_friction, __uint_as_float(P.flags), 0.f);
+                }
                 const u32 nx = i + nthreads < n ? i + nthreads : tid;
                 if (nx < n) {
                     next_pos = nx;
@@ -1440,23 +1451,36 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
             // succeed, slowing the hops they wait for.
             const u32 need0 = iter * rk.z + rk.x, need1 = iter * rk.w + rk.y;
             const u64 now = clock64();
-            bool polled = false;
-            if (!ready0 && now >= wake0) {
-                u32 seen = 0;
-                ready0 = poll_state(W.vst + 6ull * info.x, need0, s0, seen);
-                if (!ready0) wake0 = now + poll_delay(need0, seen);
-                polled = true;
+            // both bodies' first chunks in flight together, then the other
+            // chunks of the bodies whose first chunk carries the awaited
+            // count (the count it carries: the updates the body has had)
+            const bool p0 = !ready0 && now >= wake0, p1 = !ready1 && now >= wake1;
+            const bool polled = p0 || p1;
+            const u32 o0 = static_cast<u32>(info.x) * kStateBytes, o1 = static_cast<u32>(info.y) * kStateBytes;
+            Chunk a0, b0, c0, a1, b1, c1;
+            if (p0) a0 = load_chunk(R, o0);
+            if (p1) a1 = load_chunk(R, o1);
+            const bool m0 = p0 && a0.y == need0, m1 = p1 && a1.y == need1;
+            if (m0) {
+                b0 = load_chunk(R, o0 + 16u);
+                c0 = load_chunk(R, o0 + 32u);
             }
-            if (!ready1 && now >= wake1) {
-                u32 seen = 0;
-                ready1 = poll_state(W.vst + 6ull * info.y, need1, s1, seen);
-                if (!ready1) wake1 = now + poll_delay(need1, seen);
-                polled = true;
+            if (m1) {
+                b1 = load_chunk(R, o1 + 16u);
+                c1 = load_chunk(R, o1 + 32u);
+            }
+            if (p0) {
+                ready0 = m0 && take_state(a0, b0, c0, need0, s0);
+                if (!ready0) wake0 = now + poll_delay(need0, a0.y);
+            }
+            if (p1) {
+                ready1 = m1 && take_state(a1, b1, c1, need1, s1);
+                if (!ready1) wake1 = now + poll_delay(need1, a1.y);
             }
             if (ready0 && ready1) {
                 const bool snap = iter + 1 == pos_iters, last = iter + 1 == total;
                 const bool end0 = rk.x + 1 == rk.z, end1 = rk.y + 1 == rk.w;
-                solve_manager(W, info, rows, frows, s0, s1, ima, imb, need0 + 1u, need1 + 1u, snap && end0,
+                solve_manager(W, info, rows, frows, fric, s0, s1, ima, imb, need0 + 1u, need1 + 1u, snap && end0,
                               snap && end1, last && end0, last && end1, iter, pos_iters, total);
                 have = false;
                 i += nthreads;
@@ -1926,15 +1950,18 @@ struct fyzz_hip_dy {
 
 namespace {
 
-int coop_blocks(const void* kernel, int device, int wanted = dy::kCoopBlocks) {
+int resident_blocks(const void* kernel, int device) {
     int per_cu = 0, cus = 0;
     FYZZ_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, dy::kBlock, 0));
     FYZZ_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     FYZZ_REQUIRE(per_cu > 0 && cus > 0, kDeviceError, "dy: cooperative kernel does not fit a compute unit");
+    return per_cu * cus;
+}
+int coop_blocks(const void* kernel, int device, int wanted = dy::kCoopBlocks) {
     // 64 resident workgroups: a grid barrier costs ~3 us there, against
     // ~17 us at 512 (the barrier is the Dy kernels' critical path)
     // resident capacity, and the grid barrier's group counters
-    return std::min({per_cu * cus, wanted, static_cast<int>(dy::kBarGroup * (dy::kBarWords - 2))});
+    return std::min({resident_blocks(kernel, device), wanted, static_cast<int>(dy::kBarGroup * (dy::kBarWords - 2))});
 }
 
 // Cooperative launch (the runtime checks that every workgroup is resident).
@@ -2105,13 +2132,20 @@ const u32* counters(fyzz_hip_dy* d) { return d->ctr.get(); }
 void set_plain_launch(fyzz_hip_dy* d, bool plain) { d->plain_launch = plain; }
 void set_coop_blocks(fyzz_hip_dy* d, int partition_blocks, int pgs_blocks) {
     const int cap_p = coop_blocks(reinterpret_cast<const void*>(&k_part_color), d->ctx->device, 1 << 30);
-    const int cap_s = coop_blocks(reinterpret_cast<const void*>(&k_pgs), d->ctx->device, 1 << 30);
+    // the sweep has no grid barrier: its cap is the resident capacity
+    const int cap_s = resident_blocks(reinterpret_cast<const void*>(&k_pgs), d->ctx->device);
     FYZZ_REQUIRE(partition_blocks > 0 && pgs_blocks > 0, kInvalidArgument, "dy: workgroup counts must be positive");
     d->coop_blocks_partition = std::min(partition_blocks, cap_p);
     d->coop_blocks_pgs = std::min(pgs_blocks, cap_s);
     FYZZ_REQUIRE(2 * static_cast<std::size_t>(std::max(d->coop_blocks_partition, d->coop_blocks_pgs)) + 2 <=
                      d->blocksums.cap,
                  kInvalidArgument, "dy: workgroup count beyond the workspace");
+}
+void effective_blocks(int device, int partition_blocks, int pgs_blocks, int* partition_out, int* pgs_out) {
+    const void* color = reinterpret_cast<const void*>(&k_part_color);
+    const void* pgs = reinterpret_cast<const void*>(&k_pgs);
+    *partition_out = partition_blocks > 0 ? coop_blocks(color, device, partition_blocks) : coop_blocks(color, device, 128);
+    *pgs_out = pgs_blocks > 0 ? std::min(pgs_blocks, resident_blocks(pgs, device)) : coop_blocks(pgs, device, kPgsBlocks);
 }
 void coop_blocks_of(fyzz_hip_dy* d, int* partition_blocks, int* pgs_blocks) {
     *partition_blocks = d->coop_blocks_partition;
@@ -2134,6 +2168,8 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
     return guarded(ctx, [&] {
         FYZZ_REQUIRE(out, kInvalidArgument, "dy_create: null out");
         FYZZ_REQUIRE(manager_capacity < (1u << 30), kInvalidArgument, "dy_create: manager capacity too large");
+        // the sweep addresses a body's 64-byte state line with a 32-bit offset
+        FYZZ_REQUIRE(body_capacity <= (1u << 26), kInvalidArgument, "dy_create: body capacity too large");
         auto d = std::make_unique<fyzz_hip_dy>();
         d->ctx = ctx;
         d->body_cap = body_capacity;
@@ -2148,7 +2184,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->sinfo.reserve(nm);
         d->srank.reserve(nm);
         d->erank.reserve(2 * nm);
-        d->vst.reserve(6ull * nb);
+        d->vst.reserve(8ull * nb);
         d->pos_of.reserve(nm);
         for (auto* b : {&d->part, &d->pend, &d->sorted, &d->order}) b->reserve(nm);
         d->crow.reserve(4 * nm);
@@ -2167,7 +2203,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->coop_blocks_adjoint = coop_blocks(reinterpret_cast<const void*>(&dy::k_pgs_adjoint), ctx->device);
         // room for the largest grid the autotuner may pick, and for the
         // tile sums of the degree scan
-        d->blocksums.reserve(std::max<std::size_t>(2 * 512 + 2, sel::tiles_for(static_cast<u32>(nb)) + 1));
+        d->blocksums.reserve(std::max<std::size_t>(2 * 1024 + 2, sel::tiles_for(static_cast<u32>(nb)) + 1));
         FYZZ_HIP_CHECK(hipMemset(d->flags.get(), 0, sizeof(u32)));
         FYZZ_HIP_CHECK(hipMemset(d->ctr.get(), 0, dy::kCtrWords * sizeof(u32)));
         dy::Ws& w = d->ws;

@@ -67,6 +67,9 @@ void set_plain_launch(fyzz_hip_dy* d, bool plain);
 // Workgroups of the partition and PGS launches (clamped to residency).
 void set_coop_blocks(fyzz_hip_dy* d, int partition_blocks, int pgs_blocks);
 void coop_blocks_of(fyzz_hip_dy* d, int* partition_blocks, int* pgs_blocks);
+// the counts a Dy workspace on `device` would take for the requested ones
+// (0: the default), clamped as set_coop_blocks clamps
+void effective_blocks(int device, int partition_blocks, int pgs_blocks, int* partition_out, int* pgs_out);
 // fyzz_hip_dy_step with the manager count read from d_nm (at most nm_cap).
 void step_device(fyzz_hip_dy* d, float dt, u32 nb, fyzz_dy_body* bodies, u32 ns, const fyzz_dy_static* statics,
                  u32 nm_cap, const u32* d_nm, const fyzz_dy_manager* managers, const fyzz_dy_patch* history,

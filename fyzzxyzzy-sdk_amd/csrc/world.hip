@@ -947,6 +947,7 @@ struct fyzz_hip_world {
     u32 floor_id = 0;
     fyzz_dy_static floor{};
     bool uploaded = false;
+    int req_partition_blocks = 0, req_pgs_blocks = 0;  // launch shapes set before the Dy workspace exists
     u32 n = 0;
     u32 pair_cap = 0, mgr_cap = 0;
     u32 step_index = 0;
@@ -1138,6 +1139,7 @@ void upload(fyzz_hip_world* w) {
     }
     rc = fyzz_hip_dy_create(w->ctx, n, w->mgr_cap, &w->dy);
     if (rc) throw Error(rc, w->ctx->last_error);
+    if (w->req_partition_blocks) dy::set_coop_blocks(w->dy, w->req_partition_blocks, w->req_pgs_blocks);
     if (w->notify) reserve_events(w);
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     w->uploaded = true;
@@ -1504,6 +1506,36 @@ int fyzz_hip_world_autotune(fyzz_hip_world* w, float dt, uint32_t trial_steps, i
         dy::coop_blocks_of(w->dy, &pb, &sb);
         if (partition_blocks) *partition_blocks = pb;
         if (pgs_blocks) *pgs_blocks = sb;
+    });
+}
+
+int fyzz_hip_world_set_workgroups(fyzz_hip_world* w, int partition_blocks, int pgs_blocks, int* partition_in_effect,
+                                  int* pgs_in_effect) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(partition_blocks >= 0 && pgs_blocks >= 0, kInvalidArgument,
+                     "world: workgroup counts must not be negative");
+        int pb = 0, sb = 0;
+        if (w->uploaded) {
+            dy::coop_blocks_of(w->dy, &pb, &sb);
+            dy::set_coop_blocks(w->dy, partition_blocks ? partition_blocks : pb, pgs_blocks ? pgs_blocks : sb);
+            drop_graphs(w);
+            dy::coop_blocks_of(w->dy, &pb, &sb);
+        } else {
+            // kept until the first step creates the Dy workspace
+            if (w->req_partition_blocks) {
+                pb = w->req_partition_blocks;
+                sb = w->req_pgs_blocks;
+            } else {
+                dy::effective_blocks(w->ctx->device, 0, 0, &pb, &sb);
+            }
+            dy::effective_blocks(w->ctx->device, partition_blocks ? partition_blocks : pb, pgs_blocks ? pgs_blocks : sb,
+                                 &pb, &sb);
+            w->req_partition_blocks = pb;
+            w->req_pgs_blocks = sb;
+        }
+        if (partition_in_effect) *partition_in_effect = pb;
+        if (pgs_in_effect) *pgs_in_effect = sb;
     });
 }
 

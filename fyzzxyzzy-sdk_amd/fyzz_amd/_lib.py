@@ -74,6 +74,7 @@ SIGNATURES = {
     "fyzz_hip_world_step": (C.c_int, [vp, C.c_float]),
     "fyzz_hip_world_step_graph": (C.c_int, [vp, C.c_float]),
     "fyzz_hip_world_autotune": (C.c_int, [vp, C.c_float, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fyzz_hip_world_set_workgroups": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fyzz_hip_world_body_records": (C.c_int, [vp, vp, C.c_uint32, u32p]),
     "fyzz_hip_world_stats": (C.c_int, [vp, u32p, u32p, u32p, u32p]),
     "fyzz_hip_world_fetch_managers": (C.c_int, [vp, vp, vp, vp, C.c_uint32, u32p]),

@@ -177,6 +177,14 @@ class World:
         self._call("fyzz_hip_world_autotune", C.c_float(dt), C.c_uint32(trial_steps), C.byref(pb), C.byref(sb))
         return pb.value, sb.value
 
+    def set_workgroups(self, partition_blocks: int = 0, pgs_blocks: int = 0):
+        """Sets the partition and PGS workgroup counts (0 keeps one), clamped
+        to what stays co-resident; returns the counts in effect."""
+        pb, sb = C.c_int(), C.c_int()
+        self._call("fyzz_hip_world_set_workgroups", C.c_int(partition_blocks), C.c_int(pgs_blocks), C.byref(pb),
+                   C.byref(sb))
+        return pb.value, sb.value
+
     def body_records(self):
         out = np.zeros(max(self.n_dynamic, 1), DY_BODY)
         cnt = C.c_uint32()

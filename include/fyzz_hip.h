@@ -438,6 +438,12 @@ int fyzz_hip_world_step_graph(fyzz_hip_world* world, float dt);
  * Launch shapes do not change results. Writes the chosen counts. */
 int fyzz_hip_world_autotune(fyzz_hip_world* world, float dt, uint32_t trial_steps, int* partition_blocks,
                             int* pgs_blocks);
+/* Launch shapes set by hand (no counterpart in the reference): workgroup
+ * counts of the partition colouring and PGS launches, each clamped to what
+ * stays co-resident on the device; 0 keeps a count. Drops captured graphs.
+ * Launch shapes do not change results. Writes the counts in effect. */
+int fyzz_hip_world_set_workgroups(fyzz_hip_world* world, int partition_blocks, int pgs_blocks,
+                                  int* partition_in_effect, int* pgs_in_effect);
 /* Synchronizes; copies the dynamic bodies' records (creation order) to
  * HOST out[capacity]; *count = number of dynamic bodies. */
 int fyzz_hip_world_body_records(fyzz_hip_world* world, fyzz_dy_body* out, uint32_t capacity, uint32_t* count);

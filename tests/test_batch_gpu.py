@@ -92,6 +92,29 @@ def test_autotune_keeps_results(ctx, torch_cuda):
     np.testing.assert_array_equal(tuned.body_states(), plain.body_states())
 
 
+def test_set_workgroups_keeps_results(ctx, torch_cuda):
+    """Launch shapes set by hand (clamped to residency; 0 keeps a count)
+    do not change results: worlds at 16/32 and 64/512 partition/PGS
+    workgroups step the same as the default shapes."""
+    counts, half, pos, quat = scenes.batch_scenes(4, seed=scenes.SEED_BASE + 53, min_boxes=10, max_boxes=60,
+                                                  height=3.0, footprint_per_100k=1500.0)
+    ref = make(ctx, half, pos, quat, counts)
+    small = make(ctx, half, pos, quat, counts)
+    wide = make(ctx, half, pos, quat, counts)
+    assert small.set_workgroups(16, 32) == (16, 32)
+    pb, sb = wide.set_workgroups(64, 512)
+    assert pb == 64 and 256 <= sb <= 512
+    assert wide.set_workgroups(0, 0) == (pb, sb)
+    with pytest.raises(Exception):
+        wide.set_workgroups(-1, 0)
+    for _ in range(40):
+        ref.step(1 / 120)
+        small.step_graph(1 / 120)
+        wide.step(1 / 120)
+    np.testing.assert_array_equal(small.body_states(), ref.body_states())
+    np.testing.assert_array_equal(wide.body_states(), ref.body_states())
+
+
 def test_batched_4096_scenes_spot_steps_match_oracle(ctx, orc, torch_cuda):
     """The batched workload at full size: 4,096 scenes of U{8..512} boxes
     (about 1.07M) in one world, graph replays between spot steps; at each
