@@ -804,7 +804,7 @@ __device__ inline void copy_patch(Patch& d, const Patch& s) {
 }
 
 // scene.cpp:3981-4010
-__device__ bool history_valid(const Patch& old, const BodyView& a0, const BodyView& a1) {
+__device__ inline bool history_valid(const Patch& old, const BodyView& a0, const BodyView& a1) {
     if (old.broken || old.disable_strong || old.anchors == 0u) return false;
     const float same_normal = 0.999f;
     const float corr = 0.025f;
@@ -812,7 +812,10 @@ __device__ bool history_valid(const Patch& old, const BodyView& a0, const BodyVi
     const V n10 = srot_inv(q0, srot(q1, old.n1));
     if (vdot(old.n0, n10) <= same_normal) return false;
     const V p0 = mass_position(a0.pos, a0.q), p1 = mass_position(a1.pos, a1.q);
-    for (u32 k = 0; k < old.anchors; ++k) {
+    // constant anchor indices (at most two), so the patch stays in registers
+#pragma unroll
+    for (u32 k = 0; k < 2; ++k) {
+        if (k >= old.anchors) break;
         const V l = srot(q1, old.a1[k]);
         const V world = vadd(l, p1);
         const V in0 = srot_inv(q0, vsub(world, p0));

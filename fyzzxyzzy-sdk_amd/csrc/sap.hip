@@ -288,8 +288,13 @@ struct CrossArgs {
     const uint2* old_cs;  // the same for the old array
 };
 
-// Entries a thread walks before handing the rest to a wave.
-constexpr u32 kShortWalk = 16;
+// Entries a thread walks before handing the rest to a wave. Per-slot walks
+// start at random positions, so a thread's sequential steps are separate
+// line fetches, while a wave reads 64 consecutive entries per request
+// group: handing every walk that crosses more than one entry to a wave
+// measured fastest on 1M boxes at 10% moves (sap.pairs 1.40 ms at 16
+// entries, 1.03 at 4, 0.995 at 1).
+constexpr u32 kShortWalk = 1;
 
 // Walk geometry: the endpoint's rank, the bound (its other value) and the
 // direction. e sits at its new value in the new array and at its old value
