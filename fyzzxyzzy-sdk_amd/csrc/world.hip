@@ -1460,7 +1460,7 @@ int fyzz_hip_world_step_graph(fyzz_hip_world* w, float dt) {
 
 // Autotuner: picks the Dy partition and PGS workgroup counts (the grid
 // barrier's cost against the work per sweep) by timing `trial_steps` real
-// steps per candidate; candidates are tried one parameter at a time. The
+// steps per candidate, twice per candidate; one parameter at a time. The
 // steps advance the simulation (use it in place of warm-up steps); the
 // counts do not change results. Writes the chosen counts.
 int fyzz_hip_world_autotune(fyzz_hip_world* w, float dt, uint32_t trial_steps, int* partition_blocks,
@@ -1486,18 +1486,28 @@ int fyzz_hip_world_autotune(fyzz_hip_world* w, float dt, uint32_t trial_steps, i
             FYZZ_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
             return ms;
         };
-        const int cand[] = {16, 32, 64, 128, 256};
-        int best_p = 64, best_s = 64;
-        float best = 3.4e38f;
-        for (int c : cand) {
-            const float t = trial(c, best_s);
-            if (t < best) best = t, best_p = c;
-        }
-        best = 3.4e38f;
-        for (int c : cand) {
-            const float t = trial(best_p, c);
-            if (t < best) best = t, best_s = c;
-        }
+        // The world changes under the trials (a settling scene gains
+        // contacts), so each parameter's candidates run twice, in
+        // increasing then decreasing order, and their two times are summed:
+        // a drift linear in time adds the same to every candidate.
+        constexpr int kCand = 5;
+        const int cand[kCand] = {16, 32, 64, 128, 256};
+        auto pick = [&](auto run) {
+            float t[kCand] = {};
+            for (int pass = 0; pass < 2; ++pass)
+                for (int k = 0; k < kCand; ++k) {
+                    const int i = pass ? kCand - 1 - k : k;
+                    t[i] += run(cand[i]);
+                }
+            int best = 0;
+            for (int i = 1; i < kCand; ++i)
+                if (t[i] < t[best]) best = i;
+            return cand[best];
+        };
+        int pb0 = 0, sb0 = 0;
+        dy::coop_blocks_of(w->dy, &pb0, &sb0);
+        const int best_p = pick([&](int c) { return trial(c, sb0); });
+        const int best_s = pick([&](int c) { return trial(best_p, c); });
         dy::set_coop_blocks(w->dy, best_p, best_s);
         drop_graphs(w);
         (void)hipEventDestroy(a);

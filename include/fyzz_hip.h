@@ -433,9 +433,12 @@ int fyzz_hip_world_step(fyzz_hip_world* world, float dt);
  * fyzz_hip_world_step's. Asynchronous. */
 int fyzz_hip_world_step_graph(fyzz_hip_world* world, float dt);
 /* Autotuner of the launch shapes (no counterpart in the reference): times
- * trial_steps real steps per candidate workgroup count of the partition
- * and PGS launches and keeps the fastest; the steps advance the world.
- * Launch shapes do not change results. Writes the chosen counts. */
+ * trial_steps real steps per candidate workgroup count (16-256) of the
+ * partition and PGS launches, each candidate twice (increasing, then
+ * decreasing order, so a world that changes under the trials biases no
+ * candidate), and keeps the fastest; the 20 trials of 2 + trial_steps steps
+ * advance the world. Launch shapes do not change results. Writes the chosen
+ * counts. */
 int fyzz_hip_world_autotune(fyzz_hip_world* world, float dt, uint32_t trial_steps, int* partition_blocks,
                             int* pgs_blocks);
 /* Launch shapes set by hand (no counterpart in the reference): workgroup
