@@ -210,27 +210,30 @@ __global__ void k_scatter_patches(const u32* d_nm, u32 cap_m, const fyzz_dy_patc
 //
 // Scene s owns box slots [first[s], first[s+1]). One workgroup per scene
 // computes the boxes' keys (the broadphase's encoding) into LDS and tests
-// every pair of the scene with the broadphase's pair filter (different
+// every pair of the scene once with the broadphase's pair filter (different
 // bodies, at least one dynamic, inclusive 3-axis overlap of the keys,
 // scene.cpp:2606-2612), so the pair set equals the SAP's for that scene.
-// Rows are counted, scanned and written in (lo, hi) order; scenes are
-// concatenated in slot order, so the whole list is sorted like the SAP's.
+// The test writes each box's row of partners as a bitmask (bit j of row i:
+// box j > i overlaps box i) and counts the row; after the scan of the
+// scenes' totals, a second pass scans the rows and writes each row's
+// partners in increasing order: rows in (lo, hi) order, scenes concatenated
+// in slot order, so the whole list is sorted like the SAP's.
 constexpr u32 kSceneMax = 1024;  // boxes per scene in this path
 constexpr int kSceneBlock = 256;
+__device__ inline u32 scene_words(u32 n) { return (n + 31u) / 32u; }
 
-__global__ __launch_bounds__(kSceneBlock) void k_scene_pairs(int phase, u32 n_scenes, const u32* first,
+__global__ __launch_bounds__(kSceneBlock) void k_scene_masks(u32 n_scenes, const u32* first, const u32* mask_off,
                                                              const float* pos, const float* quat, const float* half,
-                                                             const float* offset, u32* scene_cnt,
-                                                             const u32* scene_base, u64* out, u32 cap, u32* status) {
+                                                             const float* offset, u32* mask, u32* row_cnt,
+                                                             u32* scene_cnt, u32* status) {
     __shared__ u32 key[6][kSceneMax];
-    __shared__ u32 row[kSceneMax];
     __shared__ u32 scratch[kSceneBlock / kWave + 1];
     for (u32 sc = blockIdx.x; sc < n_scenes; sc += gridDim.x) {
         const u32 b0 = first[sc], n = first[sc + 1] - b0;
         if (n > kSceneMax) {
             if (threadIdx.x == 0) {
                 atomicOr(&status[0], 4u);
-                if (phase == 0) scene_cnt[sc] = 0;
+                scene_cnt[sc] = 0;
             }
             continue;
         }
@@ -240,39 +243,70 @@ __global__ __launch_bounds__(kSceneBlock) void k_scene_pairs(int phase, u32 n_sc
             for (int a = 0; a < 6; ++a) key[a][i] = k[a];
         }
         __syncthreads();
-        for (u32 i = threadIdx.x; i < n; i += kSceneBlock) {
-            u32 c = 0;
-            const u32 a0 = key[0][i], a1 = key[1][i], a2 = key[2][i], a3 = key[3][i], a4 = key[4][i],
-                      a5 = key[5][i];
-            for (u32 j = i + 1; j < n; ++j)
-                c += !(a0 > key[3][j] || key[0][j] > a3 || a1 > key[4][j] || key[1][j] > a4 || a2 > key[5][j] ||
-                       key[2][j] > a5);
-            row[i] = c;
+        const u32 words = scene_words(n);
+        u32* M = mask + mask_off[sc];
+        u32 total = 0;
+        // a thread takes row r and row n-1-r, so every thread tests about n
+        // pairs (row i tests the n-1-i boxes above it)
+        for (u32 r = threadIdx.x; r < (n + 1) / 2; r += kSceneBlock) {
+            for (u32 h = 0; h < 2; ++h) {
+                const u32 i = h ? n - 1 - r : r;
+                if (h && i == r) break;
+                const u32 a0 = key[0][i], a1 = key[1][i], a2 = key[2][i], a3 = key[3][i], a4 = key[4][i],
+                          a5 = key[5][i];
+                u32* row = M + static_cast<std::size_t>(i) * words;
+                u32 c = 0;
+                for (u32 wd = 0; wd < words; ++wd) {
+                    const u32 j0 = wd * 32u;
+                    u32 bits = 0;
+                    for (u32 j = max(j0, i + 1); j < min(j0 + 32u, n); ++j)
+                        bits |= static_cast<u32>(!(a0 > key[3][j] || key[0][j] > a3 || a1 > key[4][j] ||
+                                                   key[1][j] > a4 || a2 > key[5][j] || key[2][j] > a5))
+                                << (j - j0);
+                    row[wd] = bits;
+                    c += __popc(bits);
+                }
+                row_cnt[b0 + i] = c;
+                total += c;
+            }
         }
+        u32 sum;
+        (void)block_exclusive_scan<kSceneBlock>(total, scratch, &sum);
+        if (threadIdx.x == 0) scene_cnt[sc] = sum;
         __syncthreads();
-        // exclusive scan of the row counts in place
+    }
+}
+
+__global__ __launch_bounds__(kSceneBlock) void k_scene_write(u32 n_scenes, const u32* first, const u32* mask_off,
+                                                             const u32* mask, const u32* row_cnt,
+                                                             const u32* scene_base, u64* out, u32 cap, u32* status) {
+    __shared__ u32 row[kSceneMax];
+    __shared__ u32 scratch[kSceneBlock / kWave + 1];
+    for (u32 sc = blockIdx.x; sc < n_scenes; sc += gridDim.x) {
+        const u32 b0 = first[sc], n = first[sc + 1] - b0;
+        if (n > kSceneMax) continue;
         u32 carry = 0;
         for (u32 t0 = 0; t0 < n; t0 += kSceneBlock) {
             const u32 i = t0 + threadIdx.x;
-            const u32 v = i < n ? row[i] : 0u;
+            const u32 v = i < n ? row_cnt[b0 + i] : 0u;
             u32 total;
             const u32 ex = block_exclusive_scan<kSceneBlock>(v, scratch, &total);
             if (i < n) row[i] = carry + ex;
             carry += total;
         }
-        if (phase == 0) {
-            if (threadIdx.x == 0) scene_cnt[sc] = carry;
-        } else {
-            const u32 base = scene_base[sc];
-            if (threadIdx.x == 0 && base + carry > cap) atomicOr(&status[0], 2u);
-            for (u32 i = threadIdx.x; i < n; i += kSceneBlock) {
-                u32 o = base + row[i];
-                const u32 a0 = key[0][i], a1 = key[1][i], a2 = key[2][i], a3 = key[3][i], a4 = key[4][i],
-                          a5 = key[5][i];
-                for (u32 j = i + 1; j < n; ++j) {
-                    const bool ov = !(a0 > key[3][j] || key[0][j] > a3 || a1 > key[4][j] || key[1][j] > a4 ||
-                                      a2 > key[5][j] || key[2][j] > a5);
-                    if (!ov) continue;
+        __syncthreads();
+        const u32 base = scene_base[sc];
+        if (threadIdx.x == 0 && base + carry > cap) atomicOr(&status[0], 2u);
+        const u32 words = scene_words(n);
+        const u32* M = mask + mask_off[sc];
+        for (u32 i = threadIdx.x; i < n; i += kSceneBlock) {
+            u32 o = base + row[i];
+            const u32* rw = M + static_cast<std::size_t>(i) * words;
+            for (u32 wd = (i + 1) / 32u; wd < words; ++wd) {
+                u32 bits = rw[wd];
+                while (bits) {
+                    const u32 j = wd * 32u + static_cast<u32>(__ffs(bits)) - 1u;
+                    bits &= bits - 1u;
                     if (o < cap) out[o] = (static_cast<u64>(b0 + i) << 32) | (b0 + j);
                     ++o;
                 }
@@ -978,6 +1012,7 @@ struct fyzz_hip_world {
     u32 n_scenes = 0;
     int scur = 0;
     DevBuf<u32> scene_first, scene_cnt, scene_base, scene_n, scene_status;
+    DevBuf<u32> scene_mask_off, scene_mask, scene_rows;  // per-scene pair bitmasks (rows of partners) and row counts
     DevBuf<u64> scene_pairs[2];
     // contact events
     bool notify = false;
@@ -1118,6 +1153,20 @@ void upload(fyzz_hip_world* w) {
         }
         first[w->n_scenes] = static_cast<u32>(total);
         FYZZ_REQUIRE(total == n, kInvalidArgument, "world: scene box counts do not add up to the boxes authored");
+        // each scene's partner bitmasks: one row of ceil(n/32) words per box
+        std::vector<u32> mask_off(w->n_scenes + 1);
+        u64 words = 0;
+        for (u32 k = 0; k < w->n_scenes; ++k) {
+            mask_off[k] = static_cast<u32>(words);
+            words += static_cast<u64>(w->scene_counts[k]) * ((w->scene_counts[k] + 31u) / 32u);
+            FYZZ_REQUIRE(words < (1ull << 32), kInvalidArgument, "world: scenes too large for the per-scene broadphase");
+        }
+        mask_off[w->n_scenes] = static_cast<u32>(words);
+        w->scene_mask_off.reserve(w->n_scenes + 1);
+        w->scene_mask.reserve(std::max<u64>(words, 1));
+        w->scene_rows.reserve(std::max<u32>(n, 1));
+        FYZZ_HIP_CHECK(hipMemcpyAsync(w->scene_mask_off.p, mask_off.data(), sizeof(u32) * mask_off.size(),
+                                      hipMemcpyHostToDevice, st));
         w->scene_first.reserve(w->n_scenes + 1);
         w->scene_cnt.reserve(w->n_scenes);
         w->scene_base.reserve(w->n_scenes);
@@ -1158,15 +1207,15 @@ sap::Lists scene_broadphase(fyzz_hip_world* w) {
     const u32 S = w->n_scenes, cap = w->pair_cap;
     ProfScope ps(&w->ctx->prof, "world.scene_pairs", st);
     const u32 grid = std::min<u32>(S, 4096u);
-    hipLaunchKernelGGL(k_scene_pairs, dim3(grid), dim3(kSceneBlock), 0, st, 0, S, w->scene_first.p, w->pos.p,
-                       w->quat.p, w->half.p, w->offset.p, w->scene_cnt.p, w->scene_base.p, w->scene_pairs[nxt].p,
-                       cap, w->scene_status.p);
+    hipLaunchKernelGGL(k_scene_masks, dim3(grid), dim3(kSceneBlock), 0, st, S, w->scene_first.p,
+                       w->scene_mask_off.p, w->pos.p, w->quat.p, w->half.p, w->offset.p, w->scene_mask.p,
+                       w->scene_rows.p, w->scene_cnt.p, w->scene_status.p);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->scene_n.p + 2, 1u, S);
     sel::exclusive_scan(st, S, w->scene_n.p + 2, SceneCount{w->scene_cnt.p}, w->scene_base.p, w->tiles.p,
                         w->scene_n.p + nxt);
-    hipLaunchKernelGGL(k_scene_pairs, dim3(grid), dim3(kSceneBlock), 0, st, 1, S, w->scene_first.p, w->pos.p,
-                       w->quat.p, w->half.p, w->offset.p, w->scene_cnt.p, w->scene_base.p, w->scene_pairs[nxt].p,
-                       cap, w->scene_status.p);
+    hipLaunchKernelGGL(k_scene_write, dim3(grid), dim3(kSceneBlock), 0, st, S, w->scene_first.p,
+                       w->scene_mask_off.p, w->scene_mask.p, w->scene_rows.p, w->scene_base.p,
+                       w->scene_pairs[nxt].p, cap, w->scene_status.p);
     hipLaunchKernelGGL(k_clamp, dim3(1), dim3(1), 0, st, w->scene_n.p + nxt, cap);
     FYZZ_HIP_CHECK(hipGetLastError());
     w->scur = nxt;
