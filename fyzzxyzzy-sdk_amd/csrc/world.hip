@@ -226,7 +226,10 @@ __global__ __launch_bounds__(kSceneBlock) void k_scene_masks(u32 n_scenes, const
                                                              const float* pos, const float* quat, const float* half,
                                                              const float* offset, u32* mask, u32* row_cnt,
                                                              u32* scene_cnt, u32* status) {
-    __shared__ u32 key[6][kSceneMax];
+    // a box's keys as (min x, min y, min z, max x) and (max y, max z): two
+    // LDS reads per tested pair
+    __shared__ uint4 klo[kSceneMax];
+    __shared__ uint2 khi[kSceneMax];
     __shared__ u32 scratch[kSceneBlock / kWave + 1];
     for (u32 sc = blockIdx.x; sc < n_scenes; sc += gridDim.x) {
         const u32 b0 = first[sc], n = first[sc + 1] - b0;
@@ -240,7 +243,8 @@ __global__ __launch_bounds__(kSceneBlock) void k_scene_masks(u32 n_scenes, const
         for (u32 i = threadIdx.x; i < n; i += kSceneBlock) {
             u32 k[6];
             if (!keys::box_keys(pos, quat, half, offset, b0 + i, k)) atomicOr(&status[0], 1u);
-            for (int a = 0; a < 6; ++a) key[a][i] = k[a];
+            klo[i] = make_uint4(k[0], k[1], k[2], k[3]);
+            khi[i] = make_uint2(k[4], k[5]);
         }
         __syncthreads();
         const u32 words = scene_words(n);
@@ -252,17 +256,21 @@ __global__ __launch_bounds__(kSceneBlock) void k_scene_masks(u32 n_scenes, const
             for (u32 h = 0; h < 2; ++h) {
                 const u32 i = h ? n - 1 - r : r;
                 if (h && i == r) break;
-                const u32 a0 = key[0][i], a1 = key[1][i], a2 = key[2][i], a3 = key[3][i], a4 = key[4][i],
-                          a5 = key[5][i];
+                const uint4 al = klo[i];
+                const uint2 ah = khi[i];
                 u32* row = M + static_cast<std::size_t>(i) * words;
                 u32 c = 0;
                 for (u32 wd = 0; wd < words; ++wd) {
                     const u32 j0 = wd * 32u;
                     u32 bits = 0;
-                    for (u32 j = max(j0, i + 1); j < min(j0 + 32u, n); ++j)
-                        bits |= static_cast<u32>(!(a0 > key[3][j] || key[0][j] > a3 || a1 > key[4][j] ||
-                                                   key[1][j] > a4 || a2 > key[5][j] || key[2][j] > a5))
-                                << (j - j0);
+                    for (u32 j = max(j0, i + 1); j < min(j0 + 32u, n); ++j) {
+                        const uint4 bl = klo[j];
+                        const uint2 bh = khi[j];
+                        // inclusive overlap on all three axes (bitwise: no branches)
+                        const bool ov = (al.x <= bl.w) & (bl.x <= al.w) & (al.y <= bh.x) & (bl.y <= ah.x) &
+                                        (al.z <= bh.y) & (bl.z <= ah.y);
+                        bits |= static_cast<u32>(ov) << (j - j0);
+                    }
                     row[wd] = bits;
                     c += __popc(bits);
                 }
