@@ -121,7 +121,8 @@ struct Ws {
     float* adj_v_in;
     float* adj_w_in;
     float* adj_friction;  // per body: d loss / d dynamic friction
-    float4 *a_lin, *a_ang, *a_vpre, *a_wpre;
+    float4 *a_lin, *a_ang;
+    float4* a_side;  // [4 nm] per manager and side: (velocity adjoint, friction adjoint), (angular adjoint, 0)
     float *a_capp, *a_fapp, *a_err, *a_tgt, *a_df;
 };
 
@@ -1512,16 +1513,10 @@ __global__ __launch_bounds__(kBlock) void k_pgs(Ws W) {
 // through the frozen rows).
 
 __device__ inline V ld3(const float* p, u32 i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
-__device__ inline void atomic_add3(float4* p, V v) {
-    atomicAdd(&p->x, v.x);
-    atomicAdd(&p->y, v.y);
-    atomicAdd(&p->z, v.z);
-}
 
 // adjoint state at the end of the sweeps
 __global__ __launch_bounds__(kBlock) void k_adj_init(Ws W) {
     const u32 t = blockIdx.x * kBlock + threadIdx.x;
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     if (t < W.nb) {
         const V av = ld3(W.adj_v_out, t), aw = ld3(W.adj_w_out, t);
         const float4 c0 = W.sii[3 * t], c1 = W.sii[3 * t + 1], c2 = W.sii[3 * t + 2];
@@ -1529,8 +1524,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_init(Ws W) {
         W.a_lin[t] = make_float4(av.x, av.y, av.z, 0.f);
         W.a_ang[t] = make_float4(c0.x * aw.x + c0.y * aw.y + c0.z * aw.z, c1.x * aw.x + c1.y * aw.y + c1.z * aw.z,
                                  c2.x * aw.x + c2.y * aw.y + c2.z * aw.z, 0.f);
-        W.a_vpre[t] = z;
-        W.a_wpre[t] = z;
         W.adj_friction[t] = 0.0f;
     }
     const u32 nm = mcount(W);
@@ -1684,16 +1677,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_rows(Ws W) {
         awb = vadd(awb, vmul(f3(W.fraw[2 * (4 * m + k) + 1]), a_t));
     }
     const float a_df = P.dynamic_friction > 0.0f ? 0.5f * W.a_df[m] : 0.0f;
-    if (i0 >= 0 && W.has_row[i0]) {
-        atomic_add3(&W.a_vpre[i0], ava);
-        atomic_add3(&W.a_wpre[i0], awa);
-        atomicAdd(&W.adj_friction[i0], a_df);
-    }
-    if (i1 >= 0 && W.has_row[i1]) {
-        atomic_add3(&W.a_vpre[i1], avb);
-        atomic_add3(&W.a_wpre[i1], awb);
-        atomicAdd(&W.adj_friction[i1], a_df);
-    }
+    // each side's terms, summed per body by k_adj_bodies in the body's
+    // incidence order (no atomics: the gradients are the same every run)
+    W.a_side[4 * m] = make_float4(ava.x, ava.y, ava.z, a_df);
+    W.a_side[4 * m + 1] = make_float4(awa.x, awa.y, awa.z, 0.f);
+    W.a_side[4 * m + 2] = make_float4(avb.x, avb.y, avb.z, a_df);
+    W.a_side[4 * m + 3] = make_float4(awb.x, awb.y, awb.z, 0.f);
 }
 
 // Pre-integration backwards: v_pre = lm (v_in + g dt), w_pre = am w_in, and
@@ -1710,7 +1699,21 @@ __global__ __launch_bounds__(kBlock) void k_adj_bodies(Ws W) {
         }
         return;
     }
-    const V a_rows_v = f3(W.a_vpre[i]), a_rows_w = f3(W.a_wpre[i]);
+    // the rows' terms of the body's managers, in incidence order (sorted by
+    // manager when the partitions were built)
+    V a_rows_v{0.f, 0.f, 0.f}, a_rows_w{0.f, 0.f, 0.f};
+    float a_fric = 0.0f;
+    if (mcount(W) && W.has_row[i]) {
+        for (u32 k = W.off[i]; k < W.off[i + 1]; ++k) {
+            const u32 e = W.inc[k], m = e >> 1, side = e & 1u;
+            if (W.part[m] >= kMaxParts) continue;
+            const float4 v = W.a_side[4 * m + 2 * side], w = W.a_side[4 * m + 2 * side + 1];
+            a_rows_v = vadd(a_rows_v, f3(v));
+            a_rows_w = vadd(a_rows_w, f3(w));
+            a_fric += v.w;
+        }
+    }
+    W.adj_friction[i] = a_fric;
     const V a_vpre = vadd(vadd(av, vmul(ap, W.dt)), a_rows_v);
     const V a_wpre = vadd(aw, a_rows_w);
     const float lm = smax(1.0f - b.linear_damping * W.dt, 0.0f);
@@ -1946,7 +1949,7 @@ struct fyzz_hip_dy {
     bool taped_step = false;  // the last solved step recorded a tape
     DevBuf<uint8_t> tape_c, tape_f;
     DevBuf<float> tape_ni, a_capp, a_fapp, a_err, a_tgt, a_df;
-    DevBuf<float4> craw, fraw, a_lin, a_ang, a_vpre, a_wpre;
+    DevBuf<float4> craw, fraw, a_lin, a_ang, a_side;
     dy::Ws ws{};
     bool prepared = false;
 };
@@ -2081,7 +2084,8 @@ void dy_set_tape(fyzz_hip_dy* d, bool enabled) {
         d->a_df.reserve(nm);
         d->craw.reserve(8 * nm);
         d->fraw.reserve(8 * nm);
-        for (auto* b : {&d->a_lin, &d->a_ang, &d->a_vpre, &d->a_wpre}) b->reserve(nb);
+        for (auto* b : {&d->a_lin, &d->a_ang}) b->reserve(nb);
+        d->a_side.reserve(4 * nm);
         dy::Ws& w = d->ws;
         w.tape_iters = kTapeIters;
         w.tape_c = d->tape_c.p;
@@ -2096,8 +2100,7 @@ void dy_set_tape(fyzz_hip_dy* d, bool enabled) {
         w.a_df = d->a_df.p;
         w.a_lin = d->a_lin.p;
         w.a_ang = d->a_ang.p;
-        w.a_vpre = d->a_vpre.p;
-        w.a_wpre = d->a_wpre.p;
+        w.a_side = d->a_side.p;
     }
     d->tape = enabled;
 }

@@ -126,15 +126,15 @@ def rollout_function():
             mu_body = mu_scene.detach()[scene]
             out = rollout.forward(v0, w0, mu_body)
             ctx.rollout = rollout
-            ctx.scene = scene
-            ctx.n_scenes = mu_scene.shape[0]
+            # scenes own contiguous body ranges: per-scene sums are segment
+            # sums (a fixed order, unlike atomic index adds)
+            ctx.lengths = torch.bincount(scene, minlength=mu_scene.shape[0])
             return out
 
         @staticmethod
         def backward(ctx, grad_pos):
             gv, gw, gmu_body = ctx.rollout.backward(grad_pos)
-            gmu = torch.zeros(ctx.n_scenes, dtype=gmu_body.dtype, device=gmu_body.device)
-            gmu.index_add_(0, ctx.scene, gmu_body)
+            gmu = torch.segment_reduce(gmu_body, "sum", lengths=ctx.lengths)
             return gv, gw, gmu, None
 
     return RolloutFn
