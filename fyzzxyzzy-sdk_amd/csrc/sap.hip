@@ -16,6 +16,10 @@
 //   coarse[n]          16-B truncated bounds covering old and new keys
 //   sorted[2][3][2n]   (key, payload) per axis, current and previous frame;
 //                      payload = slot << 1 | is_max
+//   split[2][3][2][n]  gathered walks only: each sorted array split by
+//                      endpoint kind (key, payload, coarse bounds of the
+//                      other two axes), with each position's index in its
+//                      kind's array
 //   pairs[2][cap]      sorted u64 (lo slot << 32 | hi slot)
 // Boxes are identified by slot (0..n-1, increasing shape id), so slot
 // order is shape-id order and every tie-break matches the reference.
@@ -111,20 +115,84 @@ __device__ inline uint2 coarse_other(const uint4 c, u32 axis) {
     const u32 s = axis == 0u ? c.z : (axis == 1u ? c.x : c.y);
     return make_uint2(f, s);
 }
-// new_cs / old_cs[axis * n2 + j] for the current and previous sorted arrays
-// (both with this frame's coarse bounds); blockIdx.y = array * 3 + axis.
-// The gather reads every entry once, so it pays only when the walks visit
-// many entries: it runs when every box is recomputed or at least half of
-// them are listed (a settling pile); otherwise the walks load each
-// candidate's bounds by slot.
-__global__ __launch_bounds__(256) void k_gather_coarse(u32 n2, const u32* new_pay, const u32* old_pay,
-                                                       const uint4* coarse, uint2* new_cs, uint2* old_cs) {
-    const u32 axis = blockIdx.y % 3u;
-    const bool old = blockIdx.y >= 3u;
-    const u32* P = (old ? old_pay : new_pay) + static_cast<std::size_t>(axis) * n2;
-    uint2* C = (old ? old_cs : new_cs) + static_cast<std::size_t>(axis) * n2;
-    for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n2; j += gridDim.x * blockDim.x)
-        C[j] = coarse_other(coarse[P[j] >> 1], axis);
+// The gathered form of a settling pile's walks: each sorted array (new and
+// old, 3 axes) split by endpoint kind into two sorted arrays of n entries
+// (a subsequence keeps the sorted order), each entry with its key, payload
+// and the coarse bounds of the other two axes (both arrays with this
+// frame's coarse bounds). A walk only ever acts on endpoints of the other
+// kind, so it scans the other kind's array alone. split_pos[j]: the index
+// of combined position j within its own kind's array. blockIdx.y = array *
+// 3 + axis; tiles of kSplitTile positions.
+constexpr u32 kSplitTile = 1024;  // 256 threads x 4 positions
+struct SplitArrays {
+    u32* key;      // [2 arrays][3 axes][2 kinds][n]
+    u32* pay;
+    uint2* cs;
+    u32* pos;      // [2 arrays][3 axes][2n]
+    u32* tile_max; // [6][tiles] max endpoints per tile
+};
+__device__ inline std::size_t split_off(u32 seg, u32 kind, u32 n) {
+    return (static_cast<std::size_t>(seg) * 2u + kind) * n;
+}
+__global__ __launch_bounds__(256) void k_split_count(u32 n2, const u32* new_pay, const u32* old_pay, SplitArrays S) {
+    const u32 seg = blockIdx.y, axis = seg % 3u;
+    const u32* P = (seg >= 3u ? old_pay : new_pay) + static_cast<std::size_t>(axis) * n2;
+    const u32 tiles = (n2 + kSplitTile - 1) / kSplitTile;
+    __shared__ u32 scratch[256 / kWave + 1];
+    for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        u32 c = 0;
+        for (u32 k = 0; k < 4; ++k) {
+            const u32 j = tile * kSplitTile + k * 256u + threadIdx.x;
+            if (j < n2) c += P[j] & 1u;
+        }
+        u32 total;
+        (void)block_exclusive_scan<256>(c, scratch, &total);
+        if (threadIdx.x == 0) S.tile_max[seg * tiles + tile] = total;
+        __syncthreads();
+    }
+}
+__global__ __launch_bounds__(256) void k_split(u32 n2, const u32* new_key, const u32* new_pay, const u32* old_key,
+                                               const u32* old_pay, const uint4* coarse, SplitArrays S) {
+    const u32 seg = blockIdx.y, axis = seg % 3u, n = n2 >> 1;
+    const std::size_t ax = static_cast<std::size_t>(axis) * n2;
+    const u32* K = (seg >= 3u ? old_key : new_key) + ax;
+    const u32* P = (seg >= 3u ? old_pay : new_pay) + ax;
+    const u32 tiles = (n2 + kSplitTile - 1) / kSplitTile;
+    __shared__ u32 scratch[256 / kWave + 1];
+    for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        // max endpoints before the tile
+        u32 before = 0;
+        for (u32 t = threadIdx.x; t < tile; t += 256u) before += S.tile_max[seg * tiles + t];
+        u32 sum;
+        (void)block_exclusive_scan<256>(before, scratch, &sum);
+        before = sum;
+        // the tile's positions in order: thread t takes 4 consecutive ones
+        const u32 j0 = tile * kSplitTile + 4u * threadIdx.x;
+        u32 pe[4], kk[4], c = 0;
+#pragma unroll
+        for (u32 k = 0; k < 4; ++k) {
+            const u32 j = j0 + k;
+            pe[k] = j < n2 ? P[j] : 0u;
+            kk[k] = j < n2 ? K[j] : 0u;
+            c += j < n2 ? (pe[k] & 1u) : 0u;
+        }
+        u32 total;
+        u32 mx = before + block_exclusive_scan<256>(c, scratch, &total);
+#pragma unroll
+        for (u32 k = 0; k < 4; ++k) {
+            const u32 j = j0 + k;
+            if (j >= n2) break;
+            const u32 kind = pe[k] & 1u;
+            const u32 idx = kind ? mx : j - mx;
+            S.pos[static_cast<std::size_t>(seg) * n2 + j] = idx;
+            const std::size_t o = split_off(seg, kind, n) + idx;
+            S.key[o] = kk[k];
+            S.pay[o] = pe[k];
+            S.cs[o] = coarse_other(coarse[pe[k] >> 1], axis);
+            mx += kind;
+        }
+        __syncthreads();
+    }
 }
 
 // rank_new / rank_old[axis * n2 + endpoint] = the endpoint's position in
@@ -284,8 +352,7 @@ struct CrossArgs {
     u32 cap;
     u32* long_walks;   // walks still inside after kShortWalk entries
     u32 long_cap;
-    const uint2* new_cs;  // gathered coarse bounds of the other two axes, new array
-    const uint2* old_cs;  // the same for the old array
+    SplitArrays split;    // the gathered walks' kind-split arrays
 };
 
 // Entries a thread walks before handing the rest to a wave. Per-slot walks
@@ -412,32 +479,49 @@ __global__ __launch_bounds__(256) void k_cross_long(CrossArgs A) {
     }
 }
 
-// The same walks when the coarse bounds were gathered this frame (a
-// settling pile: most boxes move and the walks are long). Each step reads
-// the entry's key, payload and gathered bounds of the two other axes in
+// The same walks when the arrays were split by kind this frame (a settling
+// pile: most boxes move and the walks are long). A walk scans the other
+// kind's array from the walker's place in the sorted order; each step reads
+// the entry's key, payload and the gathered bounds of the two other axes in
 // sequence; only candidates that pass them touch per-slot data.
-__device__ inline const uint2* walk_cs(const CrossArgs& A, const Walk& w) {
-    return (w.arr ? A.old_cs : A.new_cs) + static_cast<std::size_t>(w.axis) * A.n2;
-}
-__device__ inline bool walk_inside_g(const CrossArgs& A, const WalkGeom& g, const uint2* C, u32 step, u32& pj,
-                                     uint2& cj) {
+struct SplitWalk {
+    const u32* K;
+    const u32* P;
+    const uint2* C;
+    u32 start;  // walking up: the first other-kind index after the walker; down: one past the last before it
+    u32 n;
+};
+__device__ inline bool walk_inside_s(const WalkGeom& g, const SplitWalk& sw, u32 step, u32& pj, uint2& cj) {
     if (g.up) {
-        const u32 j = g.own + step;
-        if (j >= A.n2) return false;
-        const u32 kj = g.K[j];
-        pj = g.P[j];
-        cj = C[j];
+        const u32 j = sw.start + step - 1u;
+        if (j >= sw.n) return false;
+        const u32 kj = sw.K[j];
+        pj = sw.P[j];
+        cj = sw.C[j];
         return kj < g.bound_v || (kj == g.bound_v && pj < g.pe);
     }
-    if (step > g.own) return false;
-    const u32 j = g.own - step;
-    const u32 kj = g.K[j];
-    pj = g.P[j];
-    cj = C[j];
+    if (step > sw.start) return false;
+    const u32 j = sw.start - step;
+    const u32 kj = sw.K[j];
+    pj = sw.P[j];
+    cj = sw.C[j];
     return kj > g.bound_v || (kj == g.bound_v && pj > g.pe);
 }
-// ca: the walker's own gathered entry (its old and new keys on the same two
-// axes); a candidate that misses it on one of them cannot change status.
+__device__ inline SplitWalk split_walk(const CrossArgs& A, const Walk& w, u32 own) {
+    const u32 seg = w.arr * 3u + w.axis, n = A.n2 >> 1;
+    const u32 other = w.kind ^ 1u;
+    const std::size_t o = split_off(seg, other, n);
+    SplitWalk sw;
+    sw.K = A.split.key + o;
+    sw.P = A.split.pay + o;
+    sw.C = A.split.cs + o;
+    sw.n = n;
+    // other-kind entries before the walker = own - (same-kind entries before it)
+    sw.start = own - A.split.pos[static_cast<std::size_t>(seg) * A.n2 + own];
+    return sw;
+}
+// ca: the walker's own coarse bounds on the same two axes (its old and new
+// keys); a candidate that misses them on one of them cannot change status.
 __device__ inline void walk_visit_g(const CrossArgs& A, const Walk& w, u32 pf, const uint2& cb, const uint2& ca,
                                     const Rec& an, const Rec& ao) {
     const u32 b = pf >> 1;
@@ -495,12 +579,12 @@ __global__ __launch_bounds__(256) void k_cross_g(CrossArgs A) {
         const u32 v_new = an.k[w.axis + 3 * w.kind];
         if (v_old == v_new) continue;
         const WalkGeom g = walk_geom_at(A, w, own, v_old, v_new);
-        const uint2* C = walk_cs(A, w);
-        const uint2 ca = C[own];
+        const SplitWalk sw = split_walk(A, w, own);
+        const uint2 ca = coarse_other(A.coarse[w.a], w.axis);
         for (u32 step = 1;; ++step) {
             u32 pj = 0;
             uint2 cj;
-            if (!walk_inside_g(A, g, C, step, pj, cj)) break;
+            if (!walk_inside_s(g, sw, step, pj, cj)) break;
             walk_visit_g(A, w, pj, cj, ca, an, ao);
             if (step == kShortWalkG) {
                 const u32 slot = atomicAdd(&A.cnt[kLongWalks], 1u);
@@ -525,12 +609,12 @@ __global__ __launch_bounds__(256) void k_cross_long_g(CrossArgs A) {
         const Rec an = A.cur[w.a];
         const Rec ao = A.prev[w.a];
         const WalkGeom g = walk_geom_at(A, w, own, ao.k[w.axis + 3 * w.kind], an.k[w.axis + 3 * w.kind]);
-        const uint2* C = walk_cs(A, w);
-        const uint2 ca = C[own];
+        const SplitWalk sw = split_walk(A, w, own);
+        const uint2 ca = coarse_other(A.coarse[w.a], w.axis);
         for (u32 base = kShortWalkG;; base += kWave) {
             u32 pj = 0;
             uint2 cj = make_uint2(0, 0);
-            const bool inside = walk_inside_g(A, g, C, base + lane + 1, pj, cj);
+            const bool inside = walk_inside_s(g, sw, base + lane + 1, pj, cj);
             if (inside) walk_visit_g(A, w, pj, cj, ca, an, ao);
             if (__ballot(inside) != ~0ull) break;
         }
@@ -903,7 +987,8 @@ struct Sap {
         radix_counts, radix_totals, sel_tiles, dummy_a, dummy_b;
     DevBuf<Rec> rec_cur, rec_prev;
     DevBuf<uint4> coarse;
-    DevBuf<uint2> cs_new, cs_old;  // coarse bounds gathered into sorted order
+    DevBuf<u32> split_key, split_pay, split_pos, split_tiles;  // the gathered walks' kind-split arrays
+    DevBuf<uint2> split_cs;
     DevBuf<u32> list_bar;          // grid barrier and totals of k_lists
     DevBuf<u64> pairs[2], work_a, work_b, found, lost, kept;
     std::vector<u32> host_counters;
@@ -1016,6 +1101,7 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
         // of them are listed (a settling pile). Otherwise the walks load
         // each candidate's bounds by slot.
         const bool gather = n && (!dirty || 2ull * n_dirty >= n);
+        const SplitArrays split{s.split_key.p, s.split_pay.p, s.split_cs.p, s.split_pos.p, s.split_tiles.p};
         // per-slot walks start from their endpoint's position in each array
         // (the SapBox1D min/max indices of the reference GPU path), for the
         // moved boxes only; the sorted-order walks start at their position
@@ -1024,12 +1110,18 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
                                n2, s.sorted_val[nxt].p, s.sorted_val[s.cur].p, s.moved_bits.p, s.rank[0].p,
                                s.rank[1].p);
         if (gather)
-            hipLaunchKernelGGL(k_gather_coarse, dim3(std::min<u32>(grid_for(n2, 256), 2048u), 6), dim3(256), 0, st,
-                               n2, s.sorted_val[nxt].p, s.sorted_val[s.cur].p, s.coarse.p, s.cs_new.p, s.cs_old.p);
+        {
+            const u32 tiles = (n2 + kSplitTile - 1) / kSplitTile;
+            const dim3 grid(std::min<u32>(tiles, 1024u), 6);
+            hipLaunchKernelGGL(k_split_count, grid, dim3(256), 0, st, n2, s.sorted_val[nxt].p, s.sorted_val[s.cur].p,
+                               split);
+            hipLaunchKernelGGL(k_split, grid, dim3(256), 0, st, n2, s.sorted_key[nxt].p, s.sorted_val[nxt].p,
+                               s.sorted_key[s.cur].p, s.sorted_val[s.cur].p, s.coarse.p, split);
+        }
         CrossArgs A{n2, cnt + kMoved, s.moved.p, s.sorted_key[nxt].p, s.sorted_val[nxt].p, s.sorted_key[s.cur].p,
                     s.sorted_val[s.cur].p, s.rank[0].p, s.rank[1].p, s.moved_bits.p, s.rec_cur.p,
                     s.rec_prev.p, s.coarse.p, s.work_a.p, cnt, s.delta_cap, s.long_walks.p, 12u * n,
-                    s.cs_new.p, s.cs_old.p};
+                    split};
         const dim3 short_grid(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12, 256), 8192u));
         const dim3 sorted_grid(std::min<u32>(grid_for(6ull * n2, 256), 8192u));
         const dim3 long_grid(std::min<u32>(grid_for(static_cast<std::size_t>(n) * 12 * kWave, 256), 4096u));
@@ -1169,8 +1261,12 @@ int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_
         s.rec_cur.reserve(n);
         s.rec_prev.reserve(n);
         s.coarse.reserve(n);
-        s.cs_new.reserve(3ull * n2);
-        s.cs_old.reserve(3ull * n2);
+        // kind-split arrays of both sorted arrays: 2 x 3 x 2n entries each
+        s.split_key.reserve(6ull * n2);
+        s.split_pay.reserve(6ull * n2);
+        s.split_cs.reserve(6ull * n2);
+        s.split_pos.reserve(6ull * n2);
+        s.split_tiles.reserve(6ull * ((n2 + sap::kSplitTile - 1) / sap::kSplitTile) + 1);
         s.list_bar.reserve(sap::kListBarWords);
         FYZZ_HIP_CHECK(hipMemset(s.list_bar.p, 0, sizeof(u32) * sap::kListBarWords));
         s.work_a.reserve(lcap);
