@@ -93,6 +93,7 @@ struct Ws {
     u32 *deg, *off, *fill, *max_dyn, *body_mask;
     // per manager
     u32 *inc, *pred, *part, *pend, *sorted, *order;
+    uint8_t* dd;  // [nm] the manager touches two dynamic bodies (a compact copy for the per-body passes)
     u64* entry;  // [2 nm] partition colouring: ready | level << 32 | mask after the manager, per side
     int4* sinfo; // [nm] per partition-sorted position: body0, body1, manager, contact rows | anchors << 8
     u32* pos_of;    // [nm] partition-sorted position of each manager in a partition
@@ -389,7 +390,9 @@ __global__ __launch_bounds__(kBlock) void k_part_degrees(Ws W) {
     W.part[m] = kNone;
     W.entry[2 * m] = 0ull;
     W.entry[2 * m + 1] = 0ull;
-    if (!manager_ok(W, M)) return;
+    const bool ok = manager_ok(W, M);
+    W.dd[m] = ok && dyn_dyn(M) ? 1u : 0u;
+    if (!ok) return;
     if (M.body0 >= 0) atomicAdd(&W.deg[M.body0], 1u);
     if (M.body1 >= 0) atomicAdd(&W.deg[M.body1], 1u);
 }
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_part_chain(Ws W) {
     u32 prev = kNone;
     for (u32 i = lo; i < hi; ++i) {
         const u32 e = W.inc[i];
-        if (!dyn_dyn(W.mg[e >> 1])) continue;
+        if (!W.dd[e >> 1]) continue;
         W.pred[e] = prev;
         prev = e;
     }
@@ -613,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_part_static(Ws W) {
                 const u32 e = W.inc[lo + q];
                 const u32 m = e >> 1;
                 ec[q] = e;
-                if (!dyn_dyn(W.mg[m])) {
+                if (!W.dd[m]) {
                     W.part[m] = k;
                     pc[q] = k++;
                 } else {
@@ -633,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_part_static(Ws W) {
     }
     for (u32 i = lo; i < hi; ++i) {
         const u32 m = W.inc[i] >> 1;
-        if (!dyn_dyn(W.mg[m])) W.part[m] = k++;
+        if (!W.dd[m]) W.part[m] = k++;
     }
     for (u32 i = lo; i < hi; ++i) {
         const u32 e = W.inc[i];
@@ -1936,6 +1939,7 @@ struct fyzz_hip_dy {
     DevBuf<float4> pv, pw, lin, ang, mlin, mang, rpos, rq, sii;
     DevBuf<u32> has_row, deg, off, fill, max_dyn, body_mask;
     DevBuf<u32> inc, pred, part, pend, sorted, order;
+    DevBuf<uint8_t> dd;
     DevBuf<u64> entry;
     DevBuf<int4> sinfo;
     DevBuf<uint4> srank;
@@ -2193,6 +2197,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         d->vst.reserve(8ull * nb);
         d->pos_of.reserve(nm);
         for (auto* b : {&d->part, &d->pend, &d->sorted, &d->order}) b->reserve(nm);
+        d->dd.reserve(nm);
         d->crow.reserve(4 * nm);
         d->frow.reserve(4 * nm);
         d->part_count.reserve(dy::kMaxParts + 1);
@@ -2240,6 +2245,7 @@ int fyzz_hip_dy_create(fyzz_hip_context* ctx, uint32_t body_capacity, uint32_t m
         w.pend = d->pend.get();
         w.sorted = d->sorted.get();
         w.order = d->order.get();
+        w.dd = d->dd.get();
         w.crow = d->crow.get();
         w.frow = d->frow.get();
         w.part_count = d->part_count.get();
