@@ -948,8 +948,10 @@ __global__ __launch_bounds__(kListThreads) void k_lists(const u64* d_in, u64* tm
 
 // new list = (old - lost) + found in one pass: lost is a subset of old and
 // found is disjoint from it, so every output position is a rank sum.
+// src[pos]: the new pair's index in the old list (kNone: found this frame),
+// which the caller would otherwise find by a binary search per pair.
 __global__ void k_apply_delta(u32 cap, const u64* old, const u32* d_no, const u64* found, const u32* d_nf,
-                              const u64* lost, const u32* d_nl, u64* out, u32* d_nout, u32* cnt) {
+                              const u64* lost, const u32* d_nl, u64* out, u32* d_nout, u32* cnt, u32* src) {
     const u32 no = *d_no, nf = *d_nf, nl = *d_nl;
     const u32 total = no - nl + nf;
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -958,19 +960,24 @@ __global__ void k_apply_delta(u32 cap, const u64* old, const u32* d_no, const u6
         if (total > cap) atomicOr(&cnt[kOverflow], 2u);
     }
     for (u32 i = t; i < no + nf; i += gridDim.x * blockDim.x) {
-        u32 pos;
+        u32 pos, from;
         u64 x;
         if (i < no) {
             x = old[i];
             const u32 l = lower_bound64(lost, nl, x);
             if (l < nl && lost[l] == x) continue;
             pos = i - l + lower_bound64(found, nf, x);
+            from = i;
         } else {
             const u32 j = i - no;
             x = found[j];
             pos = lower_bound64(old, no, x) - lower_bound64(lost, nl, x) + j;
+            from = 0xffffffffu;
         }
-        if (pos < cap) out[pos] = x;
+        if (pos < cap) {
+            out[pos] = x;
+            src[pos] = from;
+        }
     }
 }
 
@@ -991,6 +998,8 @@ struct Sap {
     DevBuf<uint2> split_cs;
     DevBuf<u32> list_bar;          // grid barrier and totals of k_lists
     DevBuf<u64> pairs[2], work_a, work_b, found, lost, kept;
+    DevBuf<u32> src;         // previous-list index of each current pair (incremental frames)
+    bool src_valid = false;  // the last frame was incremental, so src is current
     std::vector<u32> host_counters;
 
     u32 list_cap() const { return std::max(pair_cap, delta_cap); }
@@ -1141,11 +1150,12 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
                            cnt + kDelta, s.delta_cap, s.found.p, s.lost.p, cnt, s.list_bar.p, slot_bits - 6);
         hipLaunchKernelGGL(k_apply_delta, dim3(std::min<u32>(grid_for(static_cast<std::size_t>(s.pair_cap) + s.delta_cap, 256), 2048u)), dim3(256), 0, st,
                            s.pair_cap, s.pairs[s.cur].p, cnt + pairs_cur, s.found.p, cnt + kFound, s.lost.p,
-                           cnt + kLost, s.pairs[nxt].p, cnt + pairs_nxt, cnt);
+                           cnt + kLost, s.pairs[nxt].p, cnt + pairs_nxt, cnt, s.src.p);
     }
     FYZZ_HIP_CHECK(hipGetLastError());
     s.cur = nxt;
     s.built = true;
+    s.src_valid = !full;
 }
 
 }  // namespace fyzz_hip::sap
@@ -1161,6 +1171,7 @@ void replay_advance(fyzz_hip_sap* h) {
     s.frame = (s.frame + 1) & 0x7fffffffu;
     if (s.frame == 0) s.frame = 1;
     s.cur ^= 1;
+    s.src_valid = true;  // a recorded frame is incremental
 }
 Lists lists(fyzz_hip_sap* h) {
     Sap& s = h->s;
@@ -1168,7 +1179,7 @@ Lists lists(fyzz_hip_sap* h) {
     const u32 prev = s.cur == 0 ? kPairs1 : kPairs0;
     return {s.pairs[s.cur].p, s.counters.p + cur, s.pairs[s.cur ^ 1].p, s.counters.p + prev,
             s.pair_cap, s.counters.p + kError, s.counters.p + kOverflow, s.lost.p, s.counters.p + kLost,
-            s.counters.p + kFound};
+            s.counters.p + kFound, s.src_valid ? s.src.p : nullptr};
 }
 }  // namespace fyzz_hip::sap
 
@@ -1273,6 +1284,7 @@ int fyzz_hip_sap_create(fyzz_hip_context* ctx, uint32_t n, const uint32_t* body_
         s.work_b.reserve(lcap);
         s.found.reserve(lcap);
         s.lost.reserve(lcap);
+        s.src.reserve(std::max<u32>(s.pair_cap, 1));
         s.kept.reserve(s.pair_cap);
         std::vector<sap::Rec> init(n);
         for (u32 i = 0; i < n; ++i) {

@@ -23,6 +23,7 @@ struct Lists {
     const u64* lost;        // pairs that left the list this frame (sorted)
     const u32* d_lost;
     const u32* d_found;     // pairs that entered the list this frame (count; may be null)
+    const u32* src = nullptr;  // previous-list index of each current pair (kNone: new), when known
 };
 Lists lists(fyzz_hip_sap* h);
 // Whether a frame has been advanced (the next one is incremental).

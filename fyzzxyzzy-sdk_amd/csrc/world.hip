@@ -77,16 +77,22 @@ __device__ inline u32 lower_bound64(const u64* a, u32 n, u64 x) {
 // Gu order of a box/box pair (first = higher shape id, normals in the
 // second's frame), the pair's previous-list index (its cache and friction
 // history) and the broadphase pair count per box.
+// known: the broadphase's own previous-list index of each pair (an
+// incremental frame records it), else found by a binary search.
 __global__ void k_pairs(u32 cap, const u64* cur, const u32* d_cur, const u64* prev, const u32* d_prev, u32* first,
-                        u32* second, u32* src, u32* counted) {
+                        u32* second, u32* src, u32* counted, const u32* known) {
     const u32 n = min(*d_cur, cap), np = min(*d_prev, cap);
     for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const u64 x = cur[t];
         const u32 lo = static_cast<u32>(x >> 32), hi = static_cast<u32>(x & 0xffffffffu);
         first[t] = hi;
         second[t] = lo;
-        const u32 j = lower_bound64(prev, np, x);
-        src[t] = (j < np && prev[j] == x) ? j : kNone;
+        if (known) {
+            src[t] = known[t];
+        } else {
+            const u32 j = lower_bound64(prev, np, x);
+            src[t] = (j < np && prev[j] == x) ? j : kNone;
+        }
         atomicAdd(&counted[lo], 1u);
         atomicAdd(&counted[hi], 1u);
     }
@@ -1335,7 +1341,7 @@ void step(fyzz_hip_world* w, float dt) {
     const sap::Lists L = w->n_scenes ? scene_broadphase(w) : sap_broadphase(w);
     hipLaunchKernelGGL(k_fill, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, w->counted.p, n, 0u);
     hipLaunchKernelGGL(k_pairs, dim3(std::min<u32>(grid_for(cap, 256), 4096u)), dim3(256), 0, st, cap, L.cur, L.d_cur,
-                       L.prev, L.d_prev, w->first.p, w->second.p, w->src.p, w->counted.p);
+                       L.prev, L.d_prev, w->first.p, w->second.p, w->src.p, w->counted.p, L.src);
     const u32 planes = w->has_floor ? 1u : 0u;
     hipLaunchKernelGGL(k_counted, dim3(grid_for(n, 256)), dim3(256), 0, st, n, w->bodies.p, w->counted.p, planes);
     const int c = w->ccur, c2 = c ^ 1;  // previous / current pair-aligned buffers
