@@ -482,8 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_part_color(Ws W) {
     for (;;) {
         bool progress = false;
         if (m < nm && !have) {
-            const fyzz_dy_manager& M = W.mg[m];
-            if (!dyn_dyn(M) || !manager_ok(W, M)) {
+            if (!W.dd[m]) {  // not a valid manager between two dynamic bodies
                 m += nthreads;
                 progress = true;
             } else {
