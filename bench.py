@@ -72,6 +72,14 @@ PILE_TRAFFIC = {
 SAP_SORT_TRAFFIC_BYTES = 611_000_000
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """A progress line on stderr (the result is the one JSON line on stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -211,6 +219,7 @@ def cpu_baseline_sap(n, half, pos0, moves):
     """The compiled reference NativeCpu scene (oracle/_ref) when present,
     else the oracle restatement: one incremental SAP frame of the same
     workload on one host thread."""
+    progress("sap: CPU baseline")
     import ctypes as C
     P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
     ref = _ref()
@@ -275,6 +284,7 @@ def cpu_baseline_pile(records, half, budget_s):
     """The compiled reference NativeCpu scene (oracle/_ref) loaded with the
     device world's body states: 1 untimed step, then timed steps until the
     budget is spent (at least one)."""
+    progress("pile: CPU baseline")
     import ctypes as C
     ref = _ref()
     n = len(records)
@@ -308,6 +318,7 @@ def cpu_baseline_scenes(records_list, halves, budget_s, max_rounds):
     scenes split over host threads."""
     import concurrent.futures as cf
     import ctypes as C
+    progress("batch: CPU baseline")
     ref = _ref()
     if ref is None:
         return {"value": None, "unit": "body-steps/s", "cores": 0, "kind": "reference",
@@ -436,6 +447,7 @@ def bench_pile(args, ctx, stream):
     settle = 0 if args.load_state else args.settle
     drop = None
     if settle:
+        progress(f"pile: dropping and settling {n} boxes for {settle} steps")
         w.totals(reset=True)
         with Clock(stream) as c:
             for _ in range(settle):
@@ -449,6 +461,7 @@ def bench_pile(args, ctx, stream):
     # the pile; whole-step trials on a changing pile are noisy, so the
     # default keeps the measured defaults)
     tuned = w.autotune(DT, trial_steps=2) if args.autotune else None
+    progress(f"pile: {args.warmup} warm-up and {args.steps} timed steps")
     for _ in range(args.warmup):
         step(DT)
     w.totals(reset=True)
@@ -461,6 +474,7 @@ def bench_pile(args, ctx, stream):
     if st["flags"]:
         raise RuntimeError(f"world status flags {st['flags']:#x}")
     # the same pile, K more steps with direct launches and stage timers
+    progress(f"pile: {ms / args.steps:.3f} ms per step; {args.steps} steps with stage timers")
     stages, ptot = profiled_steps(ctx, w, args.steps, PILE_STAGES, w.step)
     pst = w.stats()
     if pst["flags"]:
@@ -510,6 +524,7 @@ def bench_pile(args, ctx, stream):
 def bench_sap(args, ctx, stream, frames=60, warmup=9):
     """Broadphase-only: frame 0 the full build, `warmup` incremental frames,
     then frames-1-warmup timed incremental frames."""
+    progress(f"sap: 1,000,000 boxes, {frames} frames")
     import torch
     from fyzz_amd import scenes
     from fyzz_amd.sap import SapBroadphase
@@ -606,9 +621,12 @@ def bench_batch(args, ctx, stream, steps=200, warmup=40):
         w.set_scenes(counts)
         # the warm-up first, so the autotuner times launch shapes on the
         # formed piles the timed steps run (its trials advance the world)
+        progress(f"batch: {n_scenes} scenes, {n} boxes; {warmup} warm-up steps")
         for _ in range(warmup):
             w.step_graph(DT)
+        progress("batch: autotuning launch shapes")
         tuned = w.autotune(DT, trial_steps=3)
+        progress(f"batch: {steps} timed steps")
         warm = w.totals()["steps"]
         w.totals(reset=True)
         with Clock(stream) as c:
@@ -622,6 +640,7 @@ def bench_batch(args, ctx, stream, steps=200, warmup=40):
         stages, ptot = profiled_steps(ctx, w, 5, PILE_STAGES + ("world.scene_pairs",), w.step)
         stages = {k: v for k, v in stages.items() if v["launches"]}
         # plain per-scene launches: a sample of scenes, one world each
+        progress("batch: the same scenes as plain per-scene worlds")
         first = np.concatenate([[0], np.cumsum(counts.astype(np.int64))]).astype(np.int64)
         sample = np.linspace(0, n_scenes - 1, 32).astype(np.int64)
         for k in sample:
