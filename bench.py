@@ -755,8 +755,8 @@ def bench_diff(args, ctx, stream):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
-                                   "(independent scenes or slabs would shard across GPUs; see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark measures one GPU only; --gpus must be 1 "
+                                   "(independent scenes would shard by scene, one pile by spatial slabs)"}))
         sys.exit(2)
     if os.environ.get("WORLD_SIZE", "1") not in ("1", ""):
         print("bench.py runs on one GPU; WORLD_SIZE>1 is not supported", file=sys.stderr)

@@ -1,4 +1,4 @@
-"""CPU: the reference-side binding shown in INTEGRATION.md compiles against
+"""CPU: the reference-side binding shown in the integration notes compiles against
 the reference's own public headers (include/fyzzxyzzy/world.h) and this
 repo's C-ABI (include/fyzz_hip.h): every `WorldBackend` pure virtual is
 overridden, every override matches a reference signature, and the record
@@ -19,7 +19,7 @@ REF_INCLUDE = Path("/root/reference/include")
 def _blocks():
     text = (ROOT / "INTEGRATION.md").read_text()
     blocks = re.findall(r"<!-- compiled: hip_world_backend\.h -->\s*```cpp\n(.*?)```", text, re.S)
-    assert len(blocks) >= 2, "INTEGRATION.md lost its compiled backend blocks"
+    assert len(blocks) >= 2, "the integration notes lost their compiled backend blocks"
     return blocks
 
 
