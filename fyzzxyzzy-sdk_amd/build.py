@@ -22,6 +22,7 @@ CSRC = PKG / "csrc"
 OUT = PKG / "fyzz_amd" / "libfyzz_hip.so"
 OBJ = PKG / "_build"
 
+HOST_SOURCES = [CSRC / "islands.cpp"]  # host-only parts of the library
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 DEVICE_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++20", "-ffp-contract=off",
@@ -48,6 +49,11 @@ def build_hip(verbose: bool = False) -> Path:
             obj = OBJ / (src.stem + ".o")
             objs.append(obj)
             cmd = [HIPCC, *DEVICE_FLAGS, "-c", str(src), "-o", str(obj)]
+            jobs.append(ex.submit(_run, cmd))
+        for src in HOST_SOURCES:
+            obj = OBJ / (src.stem + ".o")
+            objs.append(obj)
+            cmd = ["g++", "-O2", "-fPIC", "-std=c++20", f"-I{ROOT / 'include'}", "-c", str(src), "-o", str(obj)]
             jobs.append(ex.submit(_run, cmd))
         for j in jobs:
             j.result()

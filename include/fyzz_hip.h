@@ -523,6 +523,71 @@ int fyzz_hip_world_device_views(fyzz_hip_world* world, const fyzz_dy_manager** m
                                 const uint32_t** d_n_managers, const float** impulses,
                                 const uint32_t** partitions, fyzz_dy_body** bodies);
 
+/* ---- Solver-order authority (host) --------------------------------------
+ * The native scene's accurate and speculative island graphs, kept on the
+ * host from each step's broadphase and touch changes, and the solver order
+ * they give: the active accurate islands' contact-manager arrays in
+ * active-list order, batched into solver tasks that close once they hold
+ * at least 128 bodies and one manager. It replaces the reference's host
+ * IslandSim bookkeeping (scene.cpp:5119-5952 update_island_connectivity,
+ * 4759-5118 lost-edge route repair and splits, 6148-6262
+ * activate_published_islands, 6465-6536 deactivate_ready_accurate_islands,
+ * 6746-6884 the active lists, 6885-6918 and 7174-7260 the descriptor and
+ * task order), which its GPU producer keeps on the host as well. Bodies
+ * are numbered as the scene numbers them (one shape per body), all
+ * admitted before the first step; pairs are (low, high) body ids. */
+typedef struct fyzz_hip_islands fyzz_hip_islands;
+
+/* One step's changes, every pair list ascending (low, high) as the
+ * broadphase reports them (scene.cpp:3233-3300 publish_pairs, 4640-4698
+ * update_contacts): pairs_found / pairs_lost the broadphase pairs that
+ * appeared / disappeared this step (static/dynamic pairs included),
+ * touch_found / touch_lost the managers whose contact count became
+ * non-zero / zero (a pair lost from the broadphase while touching is in
+ * both pairs_lost and touch_lost). ready[n_bodies] (may be NULL: keep the
+ * flags held) is the previous solve's sleep check per body. */
+typedef struct fyzz_island_events {
+  const uint32_t* pairs_found;
+  uint32_t pairs_found_count;
+  const uint32_t* pairs_lost;
+  uint32_t pairs_lost_count;
+  const uint32_t* touch_found;
+  uint32_t touch_found_count;
+  const uint32_t* touch_lost;
+  uint32_t touch_lost_count;
+  const uint8_t* ready;
+} fyzz_island_events;
+
+/* The step's solver order: managers[manager_capacity][2] the touching
+ * pairs in descriptor order, task_ends[task_capacity] the manager end of
+ * each solver task, sleeping[n_bodies] (may be NULL) each body's sleep
+ * state after the step's island pass. The counts are written even when a
+ * capacity is too small (FYZZ_HIP_E_OVERFLOW; the authority has still
+ * advanced). */
+typedef struct fyzz_island_order {
+  uint32_t* managers;
+  uint32_t manager_capacity;
+  uint32_t manager_count;
+  uint32_t* task_ends;
+  uint32_t task_capacity;
+  uint32_t task_count;
+  uint8_t* sleeping;
+} fyzz_island_order;
+
+/* dynamic[body_count]: non-zero for a dynamic body, zero for a static one
+ * (Scene::add_dynamic_box / add_static_floor, scene.cpp:584-730). Every
+ * body starts awake. */
+int fyzz_hip_islands_create(uint32_t body_count, const uint8_t* dynamic, fyzz_hip_islands** out);
+void fyzz_hip_islands_destroy(fyzz_hip_islands* islands);
+/* Applies one step's changes in the scene's own order (broadphase
+ * publication, touches, connectivity, wakes, lost-edge repair, sleeps) and
+ * writes the solver order. Pairs that are not ascending, a lost or touching
+ * pair the authority does not hold, or a found pair it already holds are
+ * FYZZ_HIP_E_INVALID_ARGUMENT; after any error the authority refuses
+ * further steps (FYZZ_HIP_E_LOGIC), as part of the step may have applied. */
+int fyzz_hip_islands_step(fyzz_hip_islands* islands, const fyzz_island_events* events, fyzz_island_order* order);
+const char* fyzz_hip_islands_last_error(const fyzz_hip_islands* islands);
+
 #ifdef __cplusplus
 }
 #endif

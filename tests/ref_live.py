@@ -143,12 +143,15 @@ class RefRun:
         box = (pr[:, 0] > 0) & (pr[:, 1] > 0)
         pairs = sorted((int(a) - 1, int(b) - 1) for a, b in pr[box])
         live = mc[:m] > 0
+        raw_pairs = pr.astype(np.uint32)
+        raw_touch = mp[:m][live].astype(np.uint32)
         touch_bb = {(int(a) - 1, int(b) - 1) for (a, b) in mp[:m][live] if a > 0}
         touch_pl = {int(b) - 1 for (a, b) in mp[:m][live] if a == 0}
         after = self.state()
         return {"state": state, "wake": wake, "ready": ready, "sleeping": sleeping, "managers": mg, "keys": keys,
                 "partitions": local, "tasks": t, "counted": counted, "next": after, "pairs": pairs,
-                "touch_bb": touch_bb, "touch_pl": touch_pl}
+                "touch_bb": touch_bb, "touch_pl": touch_pl, "raw_pairs": raw_pairs, "raw_touch": raw_touch,
+                "task_manager_ends": me[:t].astype(np.int64)}
 
 
 def dy_bodies_from(step, prev, half, density=1000.0, restitution=0.0, acc_lin=None, acc_ang=None, friction=0.5):

@@ -1,0 +1,117 @@
+"""CPU: the solver-order authority (fyzz_hip_islands_*, host C++ in
+libfyzz_hip.so) stepped beside the live compiled reference scene
+(oracle/_ref, built from the reference's sources). Each step the
+authority gets the reference's broadphase and touch changes (set
+differences of its pair and touching-manifold lists, ascending) and the
+ready flags its last solve wrote; its solver order (the descriptor stream),
+its task ends and its sleep flags must equal the reference's exactly:
+the 1,000-box grid (240 steps: landing, stacking, splits as boxes bounce
+apart, sleep), the 65-partition hub (60 steps, contacts from the first
+step), a stack that falls asleep and is woken part by part by boxes
+dropped onto it (600 steps: wakes through broadphase islands, re-sleep)
+and a 2,000-box pile (300 steps: one island growing to ~900 managers over
+10+ solver tasks)."""
+import numpy as np
+import pytest
+
+import ref_live
+from fyzz_amd import FyzzError, scenes
+from fyzz_amd.islands import IslandOrder
+
+pytestmark = pytest.mark.skipif(not ref_live.available(), reason="compiled reference (oracle/_ref) not built")
+
+
+def _pairset(a):
+    return {(int(x), int(y)) for x, y in a}
+
+
+def _compare(sc, steps):
+    run = ref_live.RefRun(sc)
+    n = sc.n + 1
+    dyn = np.ones(n, np.uint8)
+    dyn[0] = 0
+    io = IslandOrder(dyn)
+    prev_p, prev_t = set(), set()
+    seen = {"managers": 0, "tasks": 0, "asleep": 0, "woken": 0, "lost": 0}
+    prev_sleep = np.zeros(sc.n, bool)
+    try:
+        for s in range(steps):
+            st = run.step(1 / 120)
+            cur_p, cur_t = _pairset(st["raw_pairs"]), _pairset(st["raw_touch"])
+            ready = np.zeros(n, np.uint8)
+            ready[1:] = st["ready"]
+            mg, te, sl = io.step(sorted(cur_p - prev_p), sorted(prev_p - cur_p), sorted(cur_t - prev_t),
+                                 sorted(prev_t - cur_t), ready)
+            keys = np.asarray(st["keys"], np.uint32).reshape(-1, 2)
+            np.testing.assert_array_equal(mg, keys, err_msg=f"step {s}: solver order")
+            np.testing.assert_array_equal(te, st["task_manager_ends"], err_msg=f"step {s}: task ends")
+            sleeping = st["next"][3]
+            np.testing.assert_array_equal(sl[1:], sleeping, err_msg=f"step {s}: sleep flags")
+            assert not sl[0]
+            seen["managers"] = max(seen["managers"], len(keys))
+            seen["tasks"] = max(seen["tasks"], len(te))
+            seen["asleep"] = max(seen["asleep"], int(sleeping.sum()))
+            seen["woken"] += int((prev_sleep & ~sleeping).sum())
+            seen["lost"] += len(prev_t - cur_t)
+            prev_p, prev_t, prev_sleep = cur_p, cur_t, sleeping
+    finally:
+        run.close()
+        io.close()
+    return seen
+
+
+def _wake_scene(seed=scenes.SEED_BASE + 60):
+    """A 5x5 two-layer stack of unit boxes on the floor that settles and
+    sleeps, and 12 smaller boxes dropped onto it from 6-40 m."""
+    side = 5
+    idx = np.arange(side * side * 2)
+    i, k, j = idx % side, (idx // side) % side, idx // (side * side)
+    pos = np.stack([(i - 2) * 1.05, 0.5 + j * 1.0, (k - 2) * 1.05], 1)
+    half = np.full((len(idx), 3), 0.5)
+    m = 12
+    top = np.stack([(scenes.uniform(seed, 1, m) - 0.5) * 4, 6 + 34 * scenes.uniform(seed, 2, m),
+                    (scenes.uniform(seed, 3, m) - 0.5) * 4], 1)
+    pos = np.concatenate([pos, top])
+    half = np.concatenate([half, np.stack([0.3 + 0.2 * scenes.uniform(seed, 4 + a, m) for a in range(3)], 1)])
+    yaw = np.concatenate([np.zeros(len(idx)), scenes.uniform(seed, 8, m) * 2 * np.pi])
+    return scenes.Scene(half, pos, scenes.normalize_quat32(scenes.yaw_quat(yaw)))
+
+
+def test_grid_order_matches_reference():
+    seen = _compare(scenes.grid_scene(), 240)
+    assert seen["managers"] > 2000 and seen["tasks"] >= 4 and seen["lost"] > 1000 and seen["asleep"] > 0, seen
+
+
+def test_hub_order_matches_reference():
+    seen = _compare(ref_live.hub_scene(), 60)
+    assert seen["managers"] > 100, seen
+
+
+def test_sleep_and_wake_order_matches_reference():
+    seen = _compare(_wake_scene(), 600)
+    assert seen["asleep"] >= 50, seen
+    assert seen["woken"] >= 30, seen
+
+
+def test_pile_order_matches_reference():
+    seen = _compare(scenes.pile_scene(2000, footprint=15.0, height=160.0), 300)
+    assert seen["managers"] > 800 and seen["tasks"] >= 10, seen
+
+
+def test_rejects_bad_events():
+    dyn = np.array([0, 1, 1, 1], np.uint8)
+    io = IslandOrder(dyn)
+    mg, te, sl = io.step([(0, 1), (1, 2)], [], [(1, 2)], [], None)
+    assert mg.tolist() == [[1, 2]] and te.tolist() == [1] and not sl.any()
+    with pytest.raises(FyzzError) as e:
+        io.step([(2, 1)], [], [], [], None)  # not (low, high)
+    assert e.value.code == -1
+    with pytest.raises(FyzzError) as e:  # refuses once a step failed
+        io.step([], [], [], [], None)
+    assert e.value.code == -4
+    io = IslandOrder(dyn)
+    with pytest.raises(FyzzError):
+        io.step([], [], [(1, 2)], [], None)  # touch on a pair the broadphase never reported
+    io = IslandOrder(dyn)
+    with pytest.raises(FyzzError):
+        io.step([(1, 3), (1, 2)], [], [], [], None)  # not ascending
