@@ -1730,6 +1730,61 @@ __global__ __launch_bounds__(kBlock) void k_adj_bodies(Ws W) {
 
 // ------------------------------------------------------------- integrate
 
+// sinf and cosf as the C library the native scene links computes them
+// (glibc 2.28 and later ship the ARM optimized-routines sinf/cosf): the
+// float argument widened to double; below 0.75 (a test on the top 12 bits
+// against pi/4) a double polynomial in x^2 rounded once to float, below
+// 2^-12 the argument itself (sin) or 1 (cos); up to 120 a reduction by the
+// nearest multiple of pi/2 first. The published minimax coefficients are
+// restated here; every float in [2^-13, 120) gives the host library's
+// result but 17 arguments above 17 (the nearest-multiple reduction of a
+// value close to a multiple of pi/2). Beyond 120 (angular speeds above
+// 28,800 rad/s at 120 Hz) the device's own functions are used.
+namespace libm {
+constexpr double kS1 = -0x1.555545995a603p-3, kS2 = 0x1.1107605230bc4p-7, kS3 = -0x1.994eb3774cf24p-13;
+constexpr double kC0 = 0x1p0, kC1 = -0x1.ffffffd0c621cp-2, kC2 = 0x1.55553e1068f19p-5,
+                 kC3 = -0x1.6c087e89a359dp-10, kC4 = 0x1.99343027bf8c3p-16;
+constexpr double kHalfPiInv = 0x1.45F306DC9C883p+23, kHalfPi = 0x1.921FB54442D18p0;
+
+__device__ inline uint32_t top12(float f) { return (__float_as_uint(f) >> 20) & 0x7ffu; }
+
+// cos_branch: the even polynomial (cos), else the odd one (sin); negate:
+// the coefficients' sign flipped (the odd quadrants of the reduction)
+__device__ inline double poly(double x, double x2, bool cos_branch, bool negate) {
+    if (!cos_branch) {
+        const double x3 = x * x2;
+        const double s1 = kS2 + x2 * kS3;
+        const double x7 = x3 * x2;
+        const double s = x + x3 * kS1;
+        return s + x7 * s1;
+    }
+    const double g = negate ? -1.0 : 1.0;
+    const double x4 = x2 * x2;
+    const double c2 = g * kC3 + x2 * (g * kC4);
+    const double c1 = g * kC0 + x2 * (g * kC1);
+    const double x6 = x4 * x2;
+    const double c = c1 + x4 * (g * kC2);
+    return c + x6 * c2;
+}
+
+__device__ inline float sin_cos(float y, bool want_cos) {
+    const double x = y;
+    if (top12(y) < top12(0x1.921FB6p-1f)) {
+        if (top12(y) < top12(0x1p-12f)) return want_cos ? 1.0f : y;
+        return static_cast<float>(poly(x, x * x, want_cos, false));
+    }
+    if (top12(y) < top12(120.0f)) {
+        const double r = x * kHalfPiInv;
+        const int n = (static_cast<int>(r) + 0x800000) >> 24;
+        const double xr = x - n * kHalfPi;
+        const double sgn = ((n & 3) == 1 || (n & 3) == 2) ? -1.0 : 1.0;
+        const int q = want_cos ? n + 1 : n;
+        return static_cast<float>(poly(xr * sgn, xr * xr, (q & 1) != 0, (q & 2) != 0));
+    }
+    return want_cos ? static_cast<float>(cos(static_cast<double>(y))) : sinf(y);
+}
+}  // namespace libm
+
 // scene.cpp:111-146
 __device__ bool integrate_quaternion(Q pose, V w, float dt, Q& out) {
     float ww = w.x * w.x + w.y * w.y + w.z * w.z;
@@ -1742,12 +1797,8 @@ __device__ bool integrate_quaternion(Q pose, V w, float dt, Q& out) {
             ww = max_w;
         }
         const float v = dt * ww * 0.5f;
-        // The reference calls glibc's sinf/cosf, which are not correctly
-        // rounded. For the small arguments of a step, the device sinf and
-        // the double-precision cos rounded to float agree with them on all
-        // but ~0.1% of arguments (one ulp apart there).
-        const float s = sinf(v) / ww;
-        const float c = static_cast<float>(cos(static_cast<double>(v)));
+        const float s = libm::sin_cos(v, false) / ww;
+        const float c = libm::sin_cos(v, true);
         const Q dq = qmul(Q{w.x * s, w.y * s, w.z * s, 0.0f}, pose);
         const Q nx{dq.x + pose.x * c, dq.y + pose.y * c, dq.z + pose.z * c, dq.w + pose.w * c};
         const float len = sqrtf(nx.x * nx.x + nx.y * nx.y + nx.z * nx.z + nx.w * nx.w);

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <iterator>
 #include <memory>
 #include <string>
 #include <vector>
@@ -543,6 +544,19 @@ __global__ void k_ev_points(EvArgs A, const u32* ev_src, const u32* ev_off, fyzz
 }
 
 // this step's touch state for the next step's transitions
+// Flag words of the body records as bytes (bit 0 asleep, bit 1 ready).
+__global__ void k_flag_bytes(u32 n, const fyzz_dy_body* bodies, uint8_t* out) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = static_cast<uint8_t>(bodies[i].flags & 0xffu);
+}
+
+// The solver list in a given order: order[o] is the candidate index
+// (box/box pair slot, or pair count + box for a plane manager) of the o-th
+// manager.
+__global__ void k_mgr_gather(const u32* order, u32 count, MgrWrite wr) {
+    for (u32 o = blockIdx.x * blockDim.x + threadIdx.x; o < count; o += gridDim.x * blockDim.x) wr(order[o], o);
+}
+
 __global__ void k_touch(u32 cap, const u32* d_nbb, const u32* counts_bb, u32* touch_cur, u32 n, const u32* counts_pl,
                         u32* touch_pl) {
     const u32 nbb = min(*d_nbb, cap);
@@ -1035,8 +1049,19 @@ struct fyzz_hip_world {
     DevBuf<fyzz_contact_event> events;
     DevBuf<fyzz_contact_event_point> points;
     u32 ev_cap = 0, pt_cap = 0;
+    // the native scene's solver order (host island authority), chosen
+    // before the first step
+    bool native_order = false;
+    fyzz_hip_islands* order_auth = nullptr;
+    std::vector<u32> id_of_slot;
+    std::vector<u64> native_prev_keys;   // last step's broadphase pairs (slot keys, ascending)
+    std::vector<u64> native_prev_touch;  // last step's touching pairs (id keys, ascending)
+    std::vector<uint8_t> native_sleeping;
+    DevBuf<u32> native_list;
+    DevBuf<uint8_t> flag_bytes;
 
     ~fyzz_hip_world() {
+        if (order_auth) fyzz_hip_islands_destroy(order_auth);
         for (auto& g : gexec)
             if (g) (void)hipGraphExecDestroy(g);
         if (gev_in) (void)hipEventDestroy(gev_in);
@@ -1204,6 +1229,20 @@ void upload(fyzz_hip_world* w) {
     if (rc) throw Error(rc, w->ctx->last_error);
     if (w->req_partition_blocks) dy::set_coop_blocks(w->dy, w->req_partition_blocks, w->req_pgs_blocks);
     if (w->notify) reserve_events(w);
+    if (w->native_order) {
+        const u32 ids = static_cast<u32>(w->slot_of_id.size());
+        std::vector<uint8_t> dynamic(ids, 0);
+        w->id_of_slot.assign(n, 0);
+        for (u32 i = 0; i < ids; ++i)
+            if (w->slot_of_id[i] >= 0) {
+                dynamic[i] = 1;
+                w->id_of_slot[static_cast<u32>(w->slot_of_id[i])] = i;
+            }
+        FYZZ_REQUIRE(ids > 0 && fyzz_hip_islands_create(ids, dynamic.data(), &w->order_auth) == 0, kOutOfMemory,
+                     "world: cannot create the island authority");
+        w->native_list.reserve(w->mgr_cap);
+        w->flag_bytes.reserve(nn);
+    }
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     w->uploaded = true;
 }
@@ -1328,6 +1367,124 @@ void islands(fyzz_hip_world* w, const sap::Lists& L, int c) {
     FYZZ_HIP_CHECK(hipGetLastError());
 }
 
+// The native scene's solver order for this step (Scene::step's island
+// bookkeeping, scene.cpp:10747-10793 and 7174-7260): the step's broadphase
+// and touch changes and the last solve's ready flags go to the host island
+// authority, which returns the descriptor order and its sleep flags.
+// Synchronizes. Returns the order as candidate indices (box/box pair slot,
+// or pair count + box for a plane manager) in w->native_list.
+u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
+    using namespace fyzz_hip::world;
+    hipStream_t st = w->ctx->stream;
+    const u32 n = w->n, cap = w->pair_cap;
+    const u32 ids = static_cast<u32>(w->slot_of_id.size());
+    hipLaunchKernelGGL(k_flag_bytes, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, n, w->bodies.p,
+                       w->flag_bytes.p);
+    u32 nbb = 0;
+    FYZZ_HIP_CHECK(hipMemcpyAsync(&nbb, L.d_cur, sizeof nbb, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    nbb = std::min(nbb, cap);
+    std::vector<u64> keys(nbb);
+    std::vector<u32> cbb(nbb), cpl(w->has_floor ? n : 0);
+    std::vector<uint8_t> flags(n);
+    if (nbb) {
+        FYZZ_HIP_CHECK(hipMemcpyAsync(keys.data(), L.cur, sizeof(u64) * nbb, hipMemcpyDeviceToHost, st));
+        FYZZ_HIP_CHECK(hipMemcpyAsync(cbb.data(), w->counts_bb.p, sizeof(u32) * nbb, hipMemcpyDeviceToHost, st));
+    }
+    if (w->has_floor)
+        FYZZ_HIP_CHECK(hipMemcpyAsync(cpl.data(), w->counts_pl.p, sizeof(u32) * n, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipMemcpyAsync(flags.data(), w->flag_bytes.p, n, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+
+    const auto& id = w->id_of_slot;
+    auto id_key = [](u32 a, u32 b) { return a < b ? (u64(a) << 32) | b : (u64(b) << 32) | a; };
+    auto slot_pair = [&](u64 k) { return id_key(id[static_cast<u32>(k >> 32)], id[static_cast<u32>(k)]); };
+    // broadphase changes: the slot keys are ascending, as are their id keys
+    std::vector<u64> found, lost;
+    const auto& prev = w->native_prev_keys;
+    std::size_t a = 0, b = 0;
+    while (a < prev.size() || b < keys.size()) {
+        if (b == keys.size() || (a < prev.size() && prev[a] < keys[b])) lost.push_back(slot_pair(prev[a++]));
+        else if (a == prev.size() || keys[b] < prev[a]) found.push_back(slot_pair(keys[b++]));
+        else ++a, ++b;
+    }
+    if (w->step_index == 1 && w->has_floor)  // the plane pairs every box from the first step on
+        for (u32 k = 0; k < n; ++k) found.push_back(id_key(w->floor_id, id[k]));
+    std::sort(found.begin(), found.end());
+    std::vector<u64> touch;
+    for (u32 t = 0; t < nbb; ++t)
+        if (cbb[t]) touch.push_back(slot_pair(keys[t]));
+    for (u32 k = 0; k < cpl.size(); ++k)
+        if (cpl[k]) touch.push_back(id_key(w->floor_id, id[k]));
+    std::sort(touch.begin(), touch.end());
+    std::vector<u64> tfound, tlost;
+    std::set_difference(touch.begin(), touch.end(), w->native_prev_touch.begin(), w->native_prev_touch.end(),
+                        std::back_inserter(tfound));
+    std::set_difference(w->native_prev_touch.begin(), w->native_prev_touch.end(), touch.begin(), touch.end(),
+                        std::back_inserter(tlost));
+    auto unpack = [](const std::vector<u64>& v) {
+        std::vector<u32> out(2 * v.size());
+        for (std::size_t i = 0; i < v.size(); ++i) {
+            out[2 * i] = static_cast<u32>(v[i] >> 32);
+            out[2 * i + 1] = static_cast<u32>(v[i]);
+        }
+        return out;
+    };
+    const auto pf = unpack(found), pl = unpack(lost), tf = unpack(tfound), tl = unpack(tlost);
+    std::vector<uint8_t> ready(ids, 0);
+    for (u32 k = 0; k < n; ++k) ready[id[k]] = (flags[k] & 2u) ? 1 : 0;
+    fyzz_island_events ev{pf.data(), static_cast<u32>(found.size()), pl.data(), static_cast<u32>(lost.size()),
+                          tf.data(), static_cast<u32>(tfound.size()), tl.data(), static_cast<u32>(tlost.size()),
+                          ready.data()};
+    std::vector<u32> order(2 * std::max<std::size_t>(touch.size(), 1)), tasks(ids + 1);
+    w->native_sleeping.assign(ids, 0);
+    fyzz_island_order out{order.data(), static_cast<u32>(touch.size()), 0, tasks.data(), ids + 1, 0,
+                          w->native_sleeping.data()};
+    const int rc = fyzz_hip_islands_step(w->order_auth, &ev, &out);
+    FYZZ_REQUIRE(rc == 0, kLogicError,
+                 std::string("world: native solver order: ") + fyzz_hip_islands_last_error(w->order_auth));
+    // candidate index of each ordered manager
+    std::vector<u32> list(out.manager_count);
+    for (u32 m = 0; m < out.manager_count; ++m) {
+        const u32 p = order[2 * m], q = order[2 * m + 1];
+        u32 idx;
+        if (w->has_floor && (p == w->floor_id || q == w->floor_id)) {
+            idx = nbb + static_cast<u32>(w->slot_of_id[p == w->floor_id ? q : p]);
+        } else {
+            u32 sp = static_cast<u32>(w->slot_of_id[p]), sq = static_cast<u32>(w->slot_of_id[q]);
+            if (sp > sq) std::swap(sp, sq);
+            const u64 k = (u64(sp) << 32) | sq;
+            const auto it = std::lower_bound(keys.begin(), keys.end(), k);
+            FYZZ_REQUIRE(it != keys.end() && *it == k, kLogicError, "world: native order names an unknown pair");
+            idx = static_cast<u32>(it - keys.begin());
+        }
+        const bool touching = idx < nbb ? cbb[idx] != 0u : cpl[idx - nbb] != 0u;
+        FYZZ_REQUIRE(touching, kLogicError, "world: native order names a manager without contacts");
+        list[m] = idx;
+    }
+    if (!list.empty())
+        FYZZ_HIP_CHECK(hipMemcpy(w->native_list.p, list.data(), sizeof(u32) * list.size(), hipMemcpyHostToDevice));
+    w->native_prev_keys.swap(keys);
+    w->native_prev_touch.swap(touch);
+    return out.manager_count;
+}
+
+// The device island pass and the host authority decide sleep by the same
+// rules; the solver list would be wrong if they disagreed.
+void native_check_sleep(fyzz_hip_world* w) {
+    using namespace fyzz_hip::world;
+    hipStream_t st = w->ctx->stream;
+    const u32 n = w->n;
+    hipLaunchKernelGGL(k_flag_bytes, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, n, w->bodies.p,
+                       w->flag_bytes.p);
+    std::vector<uint8_t> flags(n);
+    FYZZ_HIP_CHECK(hipMemcpyAsync(flags.data(), w->flag_bytes.p, n, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    for (u32 k = 0; k < n; ++k)
+        FYZZ_REQUIRE(((flags[k] & 1u) != 0) == (w->native_sleeping[w->id_of_slot[k]] != 0), kLogicError,
+                     "world: native order: the island pass and the island authority disagree on sleep");
+}
+
 void step(fyzz_hip_world* w, float dt) {
     using namespace fyzz_hip::world;
     if (!w->uploaded) upload(w);
@@ -1353,18 +1510,27 @@ void step(fyzz_hip_world* w, float dt) {
         pcm::plane_all(w->ctx, n, w->pos.p, w->quat.p, w->half.p, w->floor.position, kContactDistance, w->cache_pl.p,
                        w->contacts_pl.p, w->counts_pl.p, w->flags_pl.p);
     }
+    const u32 native_count = w->native_order ? native_order_list(w, L) : 0u;
     islands(w, L, c);
+    if (w->native_order) native_check_sleep(w);
     // touching managers, in order
     const u32 extra = w->has_floor ? n : 0u;
     hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, st, L.d_cur, cap, extra, w->d_total.p);
     const int h = w->hcur;
     {
         ProfScope ps(&w->ctx->prof, "world.managers", st);
-        MgrPred pred{L.d_cur, cap, w->counts_bb.p, w->counts_pl.p, w->first.p, w->bodies.p};
         MgrWrite wr{L.d_cur,       cap,          w->first.p,         w->second.p,   w->src.p,
                     w->counts_bb.p, w->contacts_bb.p, w->counts_pl.p, w->contacts_pl.p, w->stamp[h].p,
                     w->d_step.p,   w->managers.p, w->mgr_slot.p};
-        sel::select(st, w->mgr_cap, w->d_total.p, pred, wr, w->tiles.p, w->d_nm.p);
+        if (w->native_order) {
+            hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->d_nm.p, 1u, native_count);
+            if (native_count)
+                hipLaunchKernelGGL(k_mgr_gather, dim3(std::min<u32>(grid_for(native_count, 256), 2048u)), dim3(256),
+                                   0, st, w->native_list.p, native_count, wr);
+        } else {
+            MgrPred pred{L.d_cur, cap, w->counts_bb.p, w->counts_pl.p, w->first.p, w->bodies.p};
+            sel::select(st, w->mgr_cap, w->d_total.p, pred, wr, w->tiles.p, w->d_nm.p);
+        }
     }
     dy::step_device(w->dy, dt, n, w->bodies.p, 1, w->statics.p, w->mgr_cap, w->d_nm.p, w->managers.p, w->hist[h].p,
                     w->mgr_cap, w->patches.p, w->impulses.p, w->parts.p);
@@ -1482,6 +1648,8 @@ void write_body(fyzz_hip_world* w, u32 slot, const fyzz_dy_body& b) {
 // The record a setter edits: the host copy before the first step, the
 // device's afterwards (written back by commit_body).
 fyzz_dy_body& edit_body(fyzz_hip_world* w, u32 id, fyzz_dy_body& scratch_copy, u32* slot_out) {
+    FYZZ_REQUIRE(!(w->uploaded && w->native_order), kLogicError,
+                 "world: the native solver order takes no body edits after the first step");
     fyzz_dy_body& host = w->body(id);
     const u32 slot = static_cast<u32>(w->slot_of_id[id]);
     if (slot_out) *slot_out = slot;
@@ -1517,6 +1685,7 @@ int fyzz_hip_world_step_graph(fyzz_hip_world* w, float dt) {
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
         FYZZ_REQUIRE(std::isfinite(dt) && dt > 0.0f, kInvalidArgument, "world: dt must be positive");
+        FYZZ_REQUIRE(!w->native_order, kLogicError, "world: the native solver order steps eagerly (host island pass)");
         step_graph(w, dt);
     });
 }
@@ -1531,6 +1700,7 @@ int fyzz_hip_world_autotune(fyzz_hip_world* w, float dt, uint32_t trial_steps, i
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
         FYZZ_REQUIRE(trial_steps > 0, kInvalidArgument, "world: autotune needs at least one step per trial");
+        FYZZ_REQUIRE(!w->native_order, kLogicError, "world: the native solver order steps eagerly (host island pass)");
         if (!w->uploaded) upload(w);
         const bool graph = w->n_scenes != 0;
         hipStream_t st = w->ctx->stream;
@@ -1980,11 +2150,24 @@ int fyzz_hip_world_set_contact_notifications(fyzz_hip_world* w, int enabled) {
     });
 }
 
+int fyzz_hip_world_set_solver_order(fyzz_hip_world* w, int native) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(native == 0 || native == 1, kInvalidArgument, "world: solver order is 0 (device) or 1 (native)");
+        FYZZ_REQUIRE(native == 0 || w->n_scenes == 0, kLogicError,
+                     "world: the native solver order is for single-scene worlds");
+        w->native_order = native == 1;
+    });
+}
+
 int fyzz_hip_world_set_scenes(fyzz_hip_world* w, uint32_t n_scenes, const uint32_t* box_counts) {
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
         w->require_authoring();
         FYZZ_REQUIRE(n_scenes == 0 || box_counts, kInvalidArgument, "world: null scene box counts");
+        FYZZ_REQUIRE(n_scenes == 0 || !w->native_order, kLogicError,
+                     "world: the native solver order is for single-scene worlds");
         FYZZ_REQUIRE(!w->notify, kLogicError, "world: contact notifications are published for single-scene worlds");
         w->scene_counts.assign(box_counts, box_counts + n_scenes);
         w->n_scenes = n_scenes;

@@ -185,6 +185,12 @@ class World:
                    C.byref(sb))
         return pb.value, sb.value
 
+    def set_solver_order(self, native: bool):
+        """Before the first step: False (default) the device order, True the
+        native scene's island order from the host island authority (eager
+        steps only, two synchronizations per step)."""
+        self._call("fyzz_hip_world_set_solver_order", C.c_int(1 if native else 0))
+
     def body_records(self):
         out = np.zeros(max(self.n_dynamic, 1), DY_BODY)
         cnt = C.c_uint32()

@@ -420,6 +420,19 @@ int fyzz_hip_world_set_body_dynamics(fyzz_hip_world* world, uint32_t body_id, co
  * over all scenes. Each scene steps exactly as a world holding only that
  * scene would. Before the first step; no contact notifications. */
 int fyzz_hip_world_set_scenes(fyzz_hip_world* world, uint32_t n_scenes, const uint32_t* box_counts);
+/* The solver order of the world's steps (before the first step):
+ * 0 (default) the device order, touching box/box managers in broadphase
+ *   pair order then plane managers, built on the device with no host
+ *   round trip (graph capture, batched scenes);
+ * 1 the native scene's order (World::step, world.h:325, of the reference's
+ *   NativeCpu backend; descriptor order scene.cpp:7174-7260): each
+ *   step hands its broadphase and touch changes to the host island
+ *   authority (fyzz_hip_islands_*), whose order the solver list takes, and
+ *   checks that the device island pass sleeps the same bodies. The step
+ *   then synchronizes twice, so fyzz_hip_world_step_graph and
+ *   fyzz_hip_world_autotune refuse it (logic_error), as do batched scenes
+ *   and body edits after the first step. */
+int fyzz_hip_world_set_solver_order(fyzz_hip_world* world, int native);
 /* Per-body materials for n consecutive body ids from first_id (HOST
  * arrays): set_body_material on each. */
 int fyzz_hip_world_set_materials(fyzz_hip_world* world, uint32_t first_id, uint32_t n, const float* static_friction,

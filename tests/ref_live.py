@@ -251,3 +251,22 @@ def hub_scene(side=8, seed=None):
     quat = scenes.normalize_quat32(scenes.yaw_quat(yaw))
     vel = np.concatenate([[[0.0, 0.0, 0.0]], np.stack([scenes.normal(seed, 1 + a, n) * 0.2 for a in range(3)], 1)])
     return scenes.Scene(half, pos, quat, vel)
+
+
+def wake_scene(seed=None):
+    """A 5x5 two-layer stack of unit boxes on the floor that settles and
+    sleeps, and 12 smaller boxes dropped onto it from 6-40 m."""
+    from fyzz_amd import scenes
+    seed = scenes.SEED_BASE + 60 if seed is None else seed
+    side = 5
+    idx = np.arange(side * side * 2)
+    i, k, j = idx % side, (idx // side) % side, idx // (side * side)
+    pos = np.stack([(i - 2) * 1.05, 0.5 + j * 1.0, (k - 2) * 1.05], 1)
+    half = np.full((len(idx), 3), 0.5)
+    m = 12
+    top = np.stack([(scenes.uniform(seed, 1, m) - 0.5) * 4, 6 + 34 * scenes.uniform(seed, 2, m),
+                    (scenes.uniform(seed, 3, m) - 0.5) * 4], 1)
+    pos = np.concatenate([pos, top])
+    half = np.concatenate([half, np.stack([0.3 + 0.2 * scenes.uniform(seed, 4 + a, m) for a in range(3)], 1)])
+    yaw = np.concatenate([np.zeros(len(idx)), scenes.uniform(seed, 8, m) * 2 * np.pi])
+    return scenes.Scene(half, pos, scenes.normalize_quat32(scenes.yaw_quat(yaw)))

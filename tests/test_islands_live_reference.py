@@ -60,23 +60,6 @@ def _compare(sc, steps):
     return seen
 
 
-def _wake_scene(seed=scenes.SEED_BASE + 60):
-    """A 5x5 two-layer stack of unit boxes on the floor that settles and
-    sleeps, and 12 smaller boxes dropped onto it from 6-40 m."""
-    side = 5
-    idx = np.arange(side * side * 2)
-    i, k, j = idx % side, (idx // side) % side, idx // (side * side)
-    pos = np.stack([(i - 2) * 1.05, 0.5 + j * 1.0, (k - 2) * 1.05], 1)
-    half = np.full((len(idx), 3), 0.5)
-    m = 12
-    top = np.stack([(scenes.uniform(seed, 1, m) - 0.5) * 4, 6 + 34 * scenes.uniform(seed, 2, m),
-                    (scenes.uniform(seed, 3, m) - 0.5) * 4], 1)
-    pos = np.concatenate([pos, top])
-    half = np.concatenate([half, np.stack([0.3 + 0.2 * scenes.uniform(seed, 4 + a, m) for a in range(3)], 1)])
-    yaw = np.concatenate([np.zeros(len(idx)), scenes.uniform(seed, 8, m) * 2 * np.pi])
-    return scenes.Scene(half, pos, scenes.normalize_quat32(scenes.yaw_quat(yaw)))
-
-
 def test_grid_order_matches_reference():
     seen = _compare(scenes.grid_scene(), 240)
     assert seen["managers"] > 2000 and seen["tasks"] >= 4 and seen["lost"] > 1000 and seen["asleep"] > 0, seen
@@ -88,7 +71,7 @@ def test_hub_order_matches_reference():
 
 
 def test_sleep_and_wake_order_matches_reference():
-    seen = _compare(_wake_scene(), 600)
+    seen = _compare(ref_live.wake_scene(), 600)
     assert seen["asleep"] >= 50, seen
     assert seen["woken"] >= 30, seen
 

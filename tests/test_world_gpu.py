@@ -219,3 +219,75 @@ def test_world_matches_live_reference_when_order_cannot_matter(ctx, torch_cuda):
         w.close()
     print(f"steps bit-identical to NativeCpu: {same}/300; largest difference {worst:.3g}; asleep at the end "
           f"{int(sleeping.sum())}/{sc.n}")
+
+
+def _native_run(ctx, sc, steps):
+    """The device world in the native solver order stepped freely beside the
+    live reference from the same authored scene. Every step: the solver
+    list equals the reference's descriptor stream, the same boxes sleep,
+    body states bit-identical (checked against the 1e-4 relative / 1e-5
+    absolute bar first, so a failure says how far apart they are).
+    Returns (steps whose states are bit-identical, most managers)."""
+    import ref_live
+    w = make_world(ctx, sc)
+    w.set_solver_order(True)
+    run = ref_live.RefRun(sc)
+    same, most = 0, 0
+    try:
+        for s in range(steps):
+            w.step(1 / 120)
+            st = run.step(1 / 120)
+            mg, _, _ = w.managers()
+            b0 = mg["body0"].astype(np.int64) + 1
+            b1 = np.where(mg["body1"] < 0, 0, mg["body1"].astype(np.int64) + 1)
+            got = np.stack([np.minimum(b0, b1), np.maximum(b0, b1)], 1)
+            want = np.asarray(st["keys"], np.int64).reshape(-1, 2)
+            np.testing.assert_array_equal(got, want, err_msg=f"step {s}: solver order")
+            ref, _wake, _ready, sleeping = st["next"]
+            recs = w.body_records()
+            np.testing.assert_array_equal((recs["flags"] & 1).astype(bool), sleeping, err_msg=f"step {s} sleep")
+            states = w.body_states()
+            np.testing.assert_allclose(states, ref, rtol=1e-4, atol=1e-5, err_msg=f"step {s} states")
+            np.testing.assert_array_equal(states, ref, err_msg=f"step {s} states (bit-identical)")
+            same += 1
+            most = max(most, len(mg))
+    finally:
+        run.close()
+        w.close()
+    return same, most
+
+
+@pytest.mark.parametrize("case", ["grid_1000", "hub", "sleep_wake", "pile_2000"])
+def test_native_order_world_matches_live_reference(ctx, torch_cuda, case):
+    """World::step in the native solver order (the host island authority
+    feeding the device solver list) against the live NativeCpu scene,
+    end to end and free running, bit for bit: the 1,000-box grid over 240
+    steps, the 65-partition hub over 60, the stack woken by dropped boxes
+    over 600 (sleep, wakes through broadphase islands, re-sleep) and a
+    2,000-box pile over 300 (one island of ~900 managers in 10+ tasks)."""
+    import ref_live
+    if not ref_live.available():
+        pytest.skip("compiled reference (oracle/_ref) not built")
+    sc, steps = {"grid_1000": (scenes.grid_scene(), 240), "hub": (ref_live.hub_scene(), 60),
+                 "sleep_wake": (ref_live.wake_scene(), 600),
+                 "pile_2000": (scenes.pile_scene(2000, footprint=15.0, height=160.0), 300)}[case]
+    same, most = _native_run(ctx, sc, steps)
+    print(f"{case}: steps bit-identical to NativeCpu {same}/{steps}, most managers {most}")
+    assert most > 40
+
+
+def test_native_order_refuses_graphs_and_edits(ctx, torch_cuda):
+    from fyzz_amd import FyzzError
+    sc = small_pile()
+    w = make_world(ctx, sc)
+    w.set_solver_order(True)
+    w.step(1 / 120)
+    with pytest.raises(FyzzError) as e:
+        w.step_graph(1 / 120)
+    assert e.value.code == -4
+    with pytest.raises(FyzzError):
+        w.set_body_velocity(1, np.zeros(3, np.float32))
+    with pytest.raises(FyzzError):
+        w.set_solver_order(False)  # after the first step
+    w.step(1 / 120)
+    w.close()
