@@ -82,8 +82,8 @@ struct Graph {
   std::vector<u32> confirmed, seen, pred, pred_edge;
   u32 confirm_stamp = 0, visit_stamp = 0;
   std::vector<u32> heap, members, split_edges;
-  std::vector<u32> island_seen;
-  u32 island_stamp = 0;
+  std::vector<u32> island_seen, expand_seen;
+  u32 island_stamp = 0, expand_stamp = 0;
 
   void resize(std::size_t n) {
     node_island.assign(n, kNone);
@@ -597,9 +597,17 @@ void Authority::synchronize(Graph& G) {
     an.pop_back();
     G.active_index[id] = kNone;
   }
+  // an island's chain is appended once: a second pass over it would find
+  // every member active already (or asleep and skipped again)
+  if (G.expand_seen.size() < G.islands.size()) G.expand_seen.resize(G.islands.size(), 0);
+  if (++G.expand_stamp == 0) {
+    std::fill(G.expand_seen.begin(), G.expand_seen.end(), 0);
+    G.expand_stamp = 1;
+  }
   for (u32 w : G.activating) {
     const u32 h = G.node_island[w];
-    if (h == kNone || !G.islands[h].live) continue;
+    if (h == kNone || !G.islands[h].live || G.expand_seen[h] == G.expand_stamp) continue;
+    G.expand_seen[h] = G.expand_stamp;
     for (u32 id = G.islands[h].root; id != kNone; id = G.node_next[id]) {
       if ((G.kind == 1 && sleeping_[id]) || G.active_index[id] != kNone) continue;
       G.active_index[id] = static_cast<u32>(an.size());
