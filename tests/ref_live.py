@@ -75,6 +75,25 @@ class RefRun:
             self.ref.ref_scene_destroy(self.s)
             self.s = None
 
+    def set_contact_notifications(self, enabled=True):
+        self._ok(self.ref.ref_set_contact_notifications(self.s, 1 if enabled else 0))
+
+    def contact_events(self):
+        """The last step's published events: ev[k, 6] (body0, body1, kind,
+        count, shape0, shape1), offsets, f64 impulses; points[m, 7]
+        (position, normal, separation), f64 point impulses."""
+        ref = self.ref
+        ref.ref_contact_events.restype = C.c_size_t
+        npts = C.c_size_t()
+        ne = ref.ref_contact_events(self.s, None, None, None, 0, None, None, 0, C.byref(npts))
+        ev = np.zeros((max(ne, 1), 6), np.uint32)
+        off = np.zeros(max(ne, 1), np.uint64)
+        imp = np.zeros(max(ne, 1), np.float64)
+        pt = np.zeros((max(npts.value, 1), 7), np.float32)
+        pimp = np.zeros(max(npts.value, 1), np.float64)
+        ref.ref_contact_events(self.s, P(ev), P(off), P(imp), ne, P(pt), P(pimp), npts.value, C.byref(npts))
+        return ev[:ne], off[:ne], imp[:ne], pt[:npts.value], pimp[:npts.value]
+
     def state(self):
         st = np.zeros(14, np.float32)
         out = np.zeros((self.n, 13), np.float32)

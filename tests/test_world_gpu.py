@@ -221,22 +221,40 @@ def test_world_matches_live_reference_when_order_cannot_matter(ctx, torch_cuda):
           f"{int(sleeping.sum())}/{sc.n}")
 
 
-def _native_run(ctx, sc, steps):
+def _native_run(ctx, sc, steps, events=False):
     """The device world in the native solver order stepped freely beside the
     live reference from the same authored scene. Every step: the solver
     list equals the reference's descriptor stream, the same boxes sleep,
     body states bit-identical (checked against the 1e-4 relative / 1e-5
-    absolute bar first, so a failure says how far apart they are).
-    Returns (steps whose states are bit-identical, most managers)."""
+    absolute bar first, so a failure says how far apart they are); with
+    events, the published contact events and points (f64 impulses
+    included) equal the reference's too. Returns (steps whose states are
+    bit-identical, most managers)."""
     import ref_live
     w = make_world(ctx, sc)
     w.set_solver_order(True)
     run = ref_live.RefRun(sc)
+    if events:
+        w.set_contact_notifications(True)
+        run.set_contact_notifications(True)
     same, most = 0, 0
+    kinds = np.zeros(3, np.int64)
     try:
         for s in range(steps):
             w.step(1 / 120)
             st = run.step(1 / 120)
+            if events:
+                ev, pt = w.contact_events()
+                rev, roff, rimp, rpt, rpimp = run.contact_events()
+                got = np.stack([ev["body0"], ev["body1"], ev["kind"], ev["contact_count"], ev["shape0"],
+                                ev["shape1"]], 1).astype(np.uint32)
+                np.testing.assert_array_equal(got, rev, err_msg=f"step {s} events")
+                np.testing.assert_array_equal(ev["point_offset"], roff, err_msg=f"step {s} event point offsets")
+                np.testing.assert_array_equal(ev["normal_impulse"], rimp, err_msg=f"step {s} event impulses")
+                gpt = np.concatenate([pt["position"], pt["normal"], pt["separation"][:, None]], 1)
+                np.testing.assert_array_equal(gpt, rpt, err_msg=f"step {s} event points")
+                np.testing.assert_array_equal(pt["normal_impulse"], rpimp, err_msg=f"step {s} point impulses")
+                kinds += np.bincount(rev[:, 2].astype(np.int64), minlength=3)
             mg, _, _ = w.managers()
             b0 = mg["body0"].astype(np.int64) + 1
             b1 = np.where(mg["body1"] < 0, 0, mg["body1"].astype(np.int64) + 1)
@@ -254,6 +272,8 @@ def _native_run(ctx, sc, steps):
     finally:
         run.close()
         w.close()
+    if events:
+        assert kinds.min() > 0, f"found / persists / lost events all seen: {kinds.tolist()}"
     return same, most
 
 
@@ -264,14 +284,15 @@ def test_native_order_world_matches_live_reference(ctx, torch_cuda, case):
     end to end and free running, bit for bit: the 1,000-box grid over 240
     steps, the 65-partition hub over 60, the stack woken by dropped boxes
     over 600 (sleep, wakes through broadphase islands, re-sleep) and a
-    2,000-box pile over 300 (one island of ~900 managers in 10+ tasks)."""
+    2,000-box pile over 300 (one island of ~900 managers in 10+ tasks);
+    on the grid and the sleep/wake stack the contact events too."""
     import ref_live
     if not ref_live.available():
         pytest.skip("compiled reference (oracle/_ref) not built")
     sc, steps = {"grid_1000": (scenes.grid_scene(), 240), "hub": (ref_live.hub_scene(), 60),
                  "sleep_wake": (ref_live.wake_scene(), 600),
                  "pile_2000": (scenes.pile_scene(2000, footprint=15.0, height=160.0), 300)}[case]
-    same, most = _native_run(ctx, sc, steps)
+    same, most = _native_run(ctx, sc, steps, events=case in ("grid_1000", "sleep_wake"))
     print(f"{case}: steps bit-identical to NativeCpu {same}/{steps}, most managers {most}")
     assert most > 40
 
