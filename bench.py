@@ -21,8 +21,11 @@ K more steps of the same pile run with direct launches and stage timers
 (HIP events on the launch stream): their per-stage times give the
 roofline of the dominant stage.
 
-Two more workloads are measured in the same run and attached to the line
+Three more measurements are made in the same run and attached to the line
 (skipped with --no-extras):
+  "pile_native_order"  the pile in the native solver order (World::step
+                with the host island authority, the NativeCpu order), 10
+                eager steps from the timed region's end state.
   "sap_1m"      broadphase only, 1,000,000 axis-aligned boxes (centers
                 uniform in a 1000 m cube, lognormal half extents median
                 0.5 m, sigma 0.5); each frame a seeded 10% moves by
@@ -66,7 +69,7 @@ DT = 1.0 / 120.0
 # passes and the launches they cover are in profiles/pile_100k_counters.md
 # and profiles/sap_1m_incremental.md.
 PILE_TRAFFIC = {
-    "dy.pgs": {"bytes": 2_234_000_000, "algorithmic_bytes": 543_600_000,
+    "dy.pgs": {"bytes": 2_235_000_000, "algorithmic_bytes": 543_700_000,
                "measured_on": "formed pile: 3 steps after loading the state saved at step 645 (248k managers)"},
 }
 SAP_SORT_TRAFFIC_BYTES = 611_000_000
@@ -91,7 +94,7 @@ def parse():
     p.add_argument("--autotune", action="store_true",
                    help="pile: let the autotuner pick the partition/PGS workgroup counts after settling")
     p.add_argument("--no-extras", action="store_true",
-                   help="pile: skip the attached 1M-box SAP and 4,096-scene batch measurements")
+                   help="pile: skip the attached native-order, 1M-box SAP and 4,096-scene batch measurements")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="wall-clock budget of a timed CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -521,6 +524,37 @@ def bench_pile(args, ctx, stream):
     return result, w, sc
 
 
+def bench_pile_native(ctx, stream, recs, half, steps=10, warmup=2):
+    """The pile in the native solver order (World::step with the host island
+    authority, the NativeCpu order): a world loaded with the timed region's
+    end state, one untimed first step (uploads, full broadphase build, the
+    authority's admission), `warmup` more, then `steps` timed eager steps."""
+    from fyzz_amd.world import World
+    progress(f"pile: {steps} steps in the native solver order")
+    n = len(recs)
+    w = World(ctx, n + 1)
+    try:
+        fid = w.add_static_floor(0.0)
+        w.set_body_material(fid, 0.5, 0.5, 0.0)
+        w.add_dynamic_boxes(half, recs["position"], recs["orientation"], linear=recs["linear_velocity"],
+                            angular=recs["angular_velocity"], friction=0.5, restitution=0.0)
+        w.set_solver_order(True)
+        for _ in range(1 + warmup):
+            w.step(DT)
+        with Clock(stream) as c:
+            for _ in range(steps):
+                w.step(DT)
+        st = w.stats()
+        if st["flags"]:
+            raise RuntimeError(f"world status flags {st['flags']:#x}")
+        return {"value": n * steps / (c.ms * 1e-3), "unit": "body-steps/s", "steps": steps,
+                "ms_per_step": c.ms / steps, "managers_at_end": st["managers"],
+                "note": "World::step in the native solver order (the host island authority feeding the device "
+                        "solver list; two synchronizations per step), eager, from the timed region's end state"}
+    finally:
+        w.close()
+
+
 def bench_sap(args, ctx, stream, frames=60, warmup=9):
     """Broadphase-only: frame 0 the full build, `warmup` incremental frames,
     then frames-1-warmup timed incremental frames."""
@@ -801,6 +835,7 @@ def main():
             recs = w.body_records()
             w.close()
             if not args.no_extras:
+                result["pile_native_order"] = bench_pile_native(ctx, stream, recs, sc.half)
                 sub, (n, half, pos0, moves) = bench_sap(args, ctx, stream)
                 sub["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_sap(n, half, pos0, moves)
                 result["sap_1m"] = sub
