@@ -17,12 +17,15 @@ __device__ inline u32 encode_max(float v) { return (((encode_float(v) >> 4) + 1u
 // extent = sum |R e_i| h_i unless the orientation is exactly identity,
 // plus the contact offset. Same operation order; contraction is off.
 __device__ inline bool box_keys(const float* pos, const float* quat, const float* half,
-                                const float* offset, u32 i, u32 out[6]) {
+                                const float* offset, u32 i, u32 out[6], const uint8_t* sphere = nullptr) {
     const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
     const float hx = half[3 * i], hy = half[3 * i + 1], hz = half[3 * i + 2];
     const float qx = quat[4 * i], qy = quat[4 * i + 1], qz = quat[4 * i + 2], qw = quat[4 * i + 3];
     float ex = hx, ey = hy, ez = hz;
-    if (qx != 0.0f || qy != 0.0f || qz != 0.0f || qw != 1.0f) {
+    // a sphere (radius in all three half extents) ignores its orientation
+    // (scene.cpp:1243-1245)
+    const bool round = sphere && sphere[i];
+    if (!round && (qx != 0.0f || qy != 0.0f || qz != 0.0f || qw != 1.0f)) {
         // rotate(q, e_k): t = 2 (u x e_k); r = e_k + w t + u x t.
         auto rot = [&](float vx, float vy, float vz, float& rx, float& ry, float& rz) {
             const float cx = qy * vz - qz * vy, cy = qz * vx - qx * vz, cz = qx * vy - qy * vx;

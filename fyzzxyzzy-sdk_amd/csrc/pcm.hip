@@ -60,6 +60,69 @@ __device__ inline Box3 load_box(const float* pos, const float* quat, const float
             {quat[4 * s], quat[4 * s + 1], quat[4 * s + 2], quat[4 * s + 3]}};
 }
 
+// ---------------- GuPCMContactSphere.cpp ----------------
+// Spheres carry no persistent manifold: one stateless contact per step
+// (position, normal, separation), in the reference's operation order.
+struct SPt {
+    V p, n;
+    float s;
+};
+// source_pcm_sphere_sphere (GuPCMContactSphere.cpp:49-67): the normal
+// points from the second center to the first, the witness on the second
+// sphere's surface.
+__device__ inline u32 sphere_sphere(V c0, float r0, V c1, float r1, float cd, SPt& o) {
+    const V d = vsub(c0, c1);
+    const float d2 = vdot(d, d);
+    const float rs = r0 + r1;
+    const float inflated = rs + cd;
+    if (!(inflated * inflated > d2)) return 0u;
+    const float dist = sqrtf(d2);
+    const V n = dist <= 0.00001f ? V{1.f, 0.f, 0.f} : vmul(d, 1.f / dist);
+    o = {vadd(c1, vmul(n, r1)), n, dist - rs};
+    return 1u;
+}
+// source_pcm_sphere_plane (GuPCMContactSphere.cpp:69-79): the horizontal
+// plane at height py, the +y normal the solver receives.
+__device__ inline u32 sphere_plane(V c, float r, float py, float cd, SPt& o) {
+    const float sep = c.y - py - r;
+    if (!(cd >= sep)) return 0u;
+    o = {{c.x, c.y - r, c.z}, {0.f, 1.f, 0.f}, sep};
+    return 1u;
+}
+// source_pcm_sphere_box (GuPCMContactSphere.cpp:81-129): the closest point
+// of the oriented box, or the nearest exit face when the center is inside;
+// the normal points from the box to the sphere.
+__device__ inline u32 sphere_box(V c, float r, V bc, Q bq, V h, float cd, SPt& o) {
+    const V local = grot_inv(bq, vsub(c, bc));
+    const V cl{sclamp(local.x, -h.x, h.x), sclamp(local.y, -h.y, h.y), sclamp(local.z, -h.z, h.z)};
+    const V gap = vsub(local, cl);
+    const float l2 = vdot(gap, gap);
+    const float inflated = r + cd;
+    if (!(inflated * inflated > l2)) return 0u;
+    const bool inside = fabsf(local.x) <= h.x && fabsf(local.y) <= h.y && fabsf(local.z) <= h.z;
+    if (inside) {
+        const V dist{h.x - fabsf(cl.x), h.y - fabsf(cl.y), h.z - fabsf(cl.z)};
+        V nl;
+        float depth;
+        if (dist.z <= dist.x && dist.z <= dist.y) {
+            nl = {0.f, 0.f, cl.z >= 0.f ? 1.f : -1.f};
+            depth = dist.z;
+        } else if (dist.x <= dist.y && dist.x <= dist.z) {
+            nl = {cl.x >= 0.f ? 1.f : -1.f, 0.f, 0.f};
+            depth = dist.x;
+        } else {
+            nl = {0.f, cl.y >= 0.f ? 1.f : -1.f, 0.f};
+            depth = dist.y;
+        }
+        const V n = grot(bq, nl);
+        o = {vadd(c, vmul(n, depth)), n, -depth - r};
+        return 1u;
+    }
+    const float len = sqrtf(l2);
+    const V n = grot(bq, vmul(gap, 1.f / len));
+    o = {vadd(bc, grot(bq, cl)), n, len - r};
+    return 1u;
+}
 // ---------------- GuPCMContactBoxBox.cpp ----------------
 struct GBox {
     V half, pos;
@@ -1122,6 +1185,33 @@ enum : u32 {
     kInvalid = 4u,      // reference would throw (invalid normal / non-finite contact)
 };
 
+// One sphere pair of the box/box list (scene.cpp:4514-4537): first / second
+// in Gu order; a box first takes the reversed normal, so the normal always
+// points toward the first shape. A non-finite contact is flagged invalid as
+// in the box path (the reference raises overflow_error there).
+__device__ inline u32 sphere_pair(const Box3& f, bool fs, const Box3& s, bool ss, float cd, float* out,
+                                  u32& flags) {
+    SPt o{};
+    u32 e;
+    if (fs && ss) {
+        e = sphere_sphere(f.pos, f.half.x, s.pos, s.half.x, cd, o);
+    } else {
+        const Box3& sph = fs ? f : s;
+        const Box3& box = fs ? s : f;
+        e = sphere_box(sph.pos, sph.half.x, box.pos, box.rot, box.half, cd, o);
+        if (e && !fs) o.n = vneg(o.n);
+    }
+    if (e) {
+        const float v[7] = {o.p.x, o.p.y, o.p.z, o.n.x, o.n.y, o.n.z, o.s};
+        for (int k = 0; k < 7; ++k) {
+            out[k] = v[k];
+            if (!isfinite(v[k])) flags |= kInvalid;
+        }
+    }
+    return e;
+}
+
+
 // Manifold emission (scene.cpp:3598-3653). direct: contacts came from SAT
 // clipping this frame; normal source as in the reference.
 __device__ u32 emit_box_box(const Cache& cache, const Box3& second, float cd, bool direct, bool gjk, V gjk_n,
@@ -1189,6 +1279,7 @@ struct PairArgs {
     const u32* d_n = nullptr;          // device pair count (at most n_pairs), or null
     const Cache* cache_src = nullptr;  // read caches from here (double-buffered), or null
     const u32* src_index = nullptr;    // per pair: cache_src record, kNoSource = empty
+    const uint8_t* sphere = nullptr;   // per slot: non-zero for a sphere (radius in half[3 s]), or null
 };
 
 // scene.cpp:3466-3654 up to the zero-witness continuation.
@@ -1200,6 +1291,14 @@ __global__ __launch_bounds__(kWitnessThreads) void k_pcm_box_box(PairArgs A) {
     const Box3 f = load_box(A.pos, A.quat, A.half, A.first[t]);
     const Box3 s = load_box(A.pos, A.quat, A.half, A.second[t]);
     Cache& cref = A.cache[A.cache_index ? A.cache_index[t] : t];
+    if (A.sphere && (A.sphere[A.first[t]] || A.sphere[A.second[t]])) {
+        u32 sflags = 0;
+        A.counts[t] = sphere_pair(f, A.sphere[A.first[t]] != 0, s, A.sphere[A.second[t]] != 0, A.cd,
+                                  A.contacts + 28ull * t, sflags);
+        A.flags[t] = sflags;
+        cref = Cache{};
+        return;
+    }
     Cache cache{};
     if (!A.cache_src)
         cache = cref;
@@ -1341,6 +1440,7 @@ struct PlaneArgs {
     float* contacts;
     u32* counts;
     u32* flags;
+    const uint8_t* sphere = nullptr;  // per slot: non-zero for a sphere (radius in half[3 s]), or null
 };
 
 // scene.cpp:3774-3955 (box geometry), followed by the scene's flip of the
@@ -1350,6 +1450,24 @@ __global__ __launch_bounds__(128) void k_pcm_plane_box(PlaneArgs A) {
     if (t >= A.n) return;
     const Box3 bx = load_box(A.pos, A.quat, A.half, A.box ? A.box[t] : t);
     Cache& cref = A.cache[A.cache_index ? A.cache_index[t] : t];
+    if (A.sphere && A.sphere[A.box ? A.box[t] : t]) {
+        // a sphere on the plane (scene.cpp:4502-4513): one stateless contact
+        SPt o{};
+        u32 sflags = 0;
+        const u32 e = sphere_plane(bx.pos, bx.half.x, A.plane_pos.y, A.cd, o);
+        if (e) {
+            const float v[7] = {o.p.x, o.p.y, o.p.z, o.n.x, o.n.y, o.n.z, o.s};
+            float* out = A.contacts + 28ull * t;
+            for (int k = 0; k < 7; ++k) {
+                out[k] = v[k];
+                if (!isfinite(v[k])) sflags |= kInvalid;
+            }
+        }
+        A.counts[t] = e;
+        A.flags[t] = sflags;
+        cref = Cache{};
+        return;
+    }
     Cache cache = cref;
     const V rel{bx.pos.x - A.plane_pos.x, bx.pos.y - A.plane_pos.y, bx.pos.z - A.plane_pos.z};
     const V h = bx.half;
@@ -1637,11 +1755,13 @@ int fyzz_hip_pcm_plane_box(fyzz_hip_context* ctx, uint32_t n, const uint32_t* bo
 namespace fyzz_hip::pcm {
 
 void box_box_list(fyzz_hip_context* ctx, const ListPairs& L, const float* pos, const float* quat,
-                  const float* half, float cd, float* contacts, u32* counts, u32* flags, u32* scratch) {
+                  const float* half, float cd, float* contacts, u32* counts, u32* flags, u32* scratch,
+                  const uint8_t* sphere) {
     if (!L.cap) return;
     PairArgs A{L.cap, L.first, L.second, nullptr, pos, quat, half, cd, static_cast<Cache*>(L.cache_dst),
                contacts, counts, flags, scratch + 1, scratch};
     A.d_n = L.d_n;
+    A.sphere = sphere;
     A.cache_src = static_cast<const Cache*>(L.cache_src);
     A.src_index = L.src_index;
     hipStream_t st = ctx->stream;
@@ -1653,10 +1773,12 @@ void box_box_list(fyzz_hip_context* ctx, const ListPairs& L, const float* pos, c
 }
 
 void plane_all(fyzz_hip_context* ctx, u32 n, const float* pos, const float* quat, const float* half,
-               const float plane_position[3], float cd, void* cache, float* contacts, u32* counts, u32* flags) {
+               const float plane_position[3], float cd, void* cache, float* contacts, u32* counts, u32* flags,
+               const uint8_t* sphere) {
     if (!n) return;
     PlaneArgs A{n, nullptr, nullptr, pos, quat, half, {plane_position[0], plane_position[1], plane_position[2]}, cd,
                 static_cast<Cache*>(cache), contacts, counts, flags};
+    A.sphere = sphere;
     ProfScope ps(&ctx->prof, "pcm.plane_box", ctx->stream);
     hipLaunchKernelGGL(k_pcm_plane_box, dim3(grid_for(n, 128)), dim3(128), 0, ctx->stream, A);
     FYZZ_HIP_CHECK(hipGetLastError());

@@ -215,11 +215,12 @@ __global__ __launch_bounds__(256) void k_moved_ranks(u32 n2, const u32* new_pay,
 // Full (re)build: every box gets its keys; no moved list.
 __global__ __launch_bounds__(256) void k_bounds_full(u32 n, const float* pos, const float* quat,
                                                      const float* half, const float* off,
-                                                     u32* planes, Rec* cur, uint4* coarse, u32 stamp, u32* cnt) {
+                                                     u32* planes, Rec* cur, uint4* coarse, u32 stamp, u32* cnt,
+                                                     const uint8_t* sphere) {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     u32 k[6];
-    if (!box_keys(pos, quat, half, off, i, k)) atomicOr(&cnt[kError], 1u);
+    if (!box_keys(pos, quat, half, off, i, k, sphere)) atomicOr(&cnt[kError], 1u);
     Rec r = cur[i];
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
@@ -241,7 +242,7 @@ __global__ __launch_bounds__(256) void k_bounds_update(u32 count, const u32* dir
                                                        const float* half, const float* off,
                                                        u32* planes, Rec* cur, Rec* prev, uint4* coarse,
                                                        u32* seen, u32 stamp, u32* moved, u32* moved_bits,
-                                                       u32* cnt) {
+                                                       u32* cnt, const uint8_t* sphere) {
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const u32 i = dirty ? dirty[t] : t;
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(256) void k_bounds_update(u32 count, const u32* dir
     }
     if (dirty && atomicExch(&seen[i], stamp) == stamp) return;
     u32 k[6];
-    if (!box_keys(pos, quat, half, off, i, k)) atomicOr(&cnt[kError], 1u);
+    if (!box_keys(pos, quat, half, off, i, k, sphere)) atomicOr(&cnt[kError], 1u);
     Rec r = cur[i];
     bool changed = false;
 #pragma unroll
@@ -1000,6 +1001,7 @@ struct Sap {
     DevBuf<u64> pairs[2], work_a, work_b, found, lost, kept;
     DevBuf<u32> src;         // previous-list index of each current pair (incremental frames)
     bool src_valid = false;  // the last frame was incremental, so src is current
+    const uint8_t* sphere = nullptr;  // per slot: a sphere (set by the world), or null
     std::vector<u32> host_counters;
 
     u32 list_cap() const { return std::max(pair_cap, delta_cap); }
@@ -1064,7 +1066,7 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
     ProfScope ps(prof, "sap.bounds", st);
     if (full) {
         if (n) hipLaunchKernelGGL(k_bounds_full, dim3(grid_for(n, 256)), dim3(256), 0, st, n, pos, quat, half, off,
-                                  s.planes.p, s.rec_cur.p, s.coarse.p, stamp, cnt);
+                                  s.planes.p, s.rec_cur.p, s.coarse.p, stamp, cnt, s.sphere);
     } else {
         const u32 words = (n + 31) / 32;
         if (words) hipLaunchKernelGGL(k_fill, dim3(grid_for(words, 256)), dim3(256), 0, st, s.moved_bits.p, words, 0u);
@@ -1072,7 +1074,7 @@ void advance(Sap& s, const float* pos, const float* quat, const float* half, con
         if (count)
             hipLaunchKernelGGL(k_bounds_update, dim3(grid_for(count, 256)), dim3(256), 0, st, count, dirty, n, pos,
                                quat, half, off, s.planes.p, s.rec_cur.p, s.rec_prev.p, s.coarse.p, s.seen.p, stamp,
-                               s.moved.p, s.moved_bits.p, cnt);
+                               s.moved.p, s.moved_bits.p, cnt, s.sphere);
     }
     FYZZ_HIP_CHECK(hipGetLastError());
     }
@@ -1166,6 +1168,7 @@ struct fyzz_hip_sap {
 
 namespace fyzz_hip::sap {
 bool built(fyzz_hip_sap* h) { return h->s.built; }
+void set_spheres(fyzz_hip_sap* h, const uint8_t* sphere) { h->s.sphere = sphere; }
 void replay_advance(fyzz_hip_sap* h) {
     Sap& s = h->s;
     s.frame = (s.frame + 1) & 0x7fffffffu;

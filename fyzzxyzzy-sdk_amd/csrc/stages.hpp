@@ -34,6 +34,10 @@ void replay_advance(fyzz_hip_sap* h);
 // Looks up this device's copy of the fixed sort workspace (call with the
 // device current; idempotent).
 void bind_fixed_workspace(int device);
+// Per-slot sphere flags (device, n entries; null: all boxes): a sphere's
+// bounds are its radius half[3 slot] on every axis, whatever its
+// orientation. The pointer must stay valid while the broadphase advances.
+void set_spheres(fyzz_hip_sap* h, const uint8_t* sphere);
 }  // namespace sap
 
 namespace pcm {
@@ -50,11 +54,15 @@ struct ListPairs {
     const u32* src_index;
     void* cache_dst;
 };
+// sphere: per slot, non-zero for a sphere of radius half[3 slot] (its
+// pairs take the stateless sphere contacts), or null when all are boxes.
 void box_box_list(fyzz_hip_context* ctx, const ListPairs& L, const float* pos, const float* quat,
-                  const float* half, float cd, float* contacts, u32* counts, u32* flags, u32* scratch);
+                  const float* half, float cd, float* contacts, u32* counts, u32* flags, u32* scratch,
+                  const uint8_t* sphere = nullptr);
 // Every box slot t < n against one plane; caches in place, indexed by slot.
 void plane_all(fyzz_hip_context* ctx, u32 n, const float* pos, const float* quat, const float* half,
-               const float plane_position[3], float cd, void* cache, float* contacts, u32* counts, u32* flags);
+               const float plane_position[3], float cd, void* cache, float* contacts, u32* counts, u32* flags,
+               const uint8_t* sphere = nullptr);
 }  // namespace pcm
 
 namespace dy {

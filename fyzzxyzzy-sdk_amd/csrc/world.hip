@@ -1004,6 +1004,8 @@ struct fyzz_hip_world {
     // authoring (host) until the first step
     std::vector<fyzz_dy_body> host_bodies;
     std::vector<float> host_half;
+    std::vector<uint8_t> host_sphere;  // per slot: a sphere (radius in its three half extents)
+    bool has_spheres = false;
     std::vector<int> slot_of_id;  // body id -> box slot, -1 for the floor
     bool has_floor = false;
     u32 floor_id = 0;
@@ -1059,6 +1061,7 @@ struct fyzz_hip_world {
     std::vector<uint8_t> native_sleeping;
     DevBuf<u32> native_list;
     DevBuf<uint8_t> flag_bytes;
+    DevBuf<uint8_t> sphere;  // per slot, when the world holds spheres
 
     ~fyzz_hip_world() {
         if (order_auth) fyzz_hip_islands_destroy(order_auth);
@@ -1177,6 +1180,10 @@ void upload(fyzz_hip_world* w) {
                                       st));
         std::vector<float> off(n, world::kContactOffset);
         FYZZ_HIP_CHECK(hipMemcpyAsync(w->offset.p, off.data(), sizeof(float) * n, hipMemcpyHostToDevice, st));
+        if (w->has_spheres) {
+            w->sphere.reserve(n);
+            FYZZ_HIP_CHECK(hipMemcpyAsync(w->sphere.p, w->host_sphere.data(), n, hipMemcpyHostToDevice, st));
+        }
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     }
     FYZZ_HIP_CHECK(hipMemcpyAsync(w->statics.p, &w->floor, sizeof(fyzz_dy_static), hipMemcpyHostToDevice, st));
@@ -1224,6 +1231,7 @@ void upload(fyzz_hip_world* w) {
         for (u32 i = 0; i < n; ++i) ids[i] = i;
         rc = fyzz_hip_sap_create(w->ctx, n, ids.data(), dyn.data(), w->pair_cap, w->pair_cap, &w->sap);
         if (rc) throw Error(rc, w->ctx->last_error);
+        if (w->has_spheres) sap::set_spheres(w->sap, w->sphere.p);
     }
     rc = fyzz_hip_dy_create(w->ctx, n, w->mgr_cap, &w->dy);
     if (rc) throw Error(rc, w->ctx->last_error);
@@ -1503,12 +1511,13 @@ void step(fyzz_hip_world* w, float dt) {
     hipLaunchKernelGGL(k_counted, dim3(grid_for(n, 256)), dim3(256), 0, st, n, w->bodies.p, w->counted.p, planes);
     const int c = w->ccur, c2 = c ^ 1;  // previous / current pair-aligned buffers
     pcm::ListPairs lp{cap, L.d_cur, w->first.p, w->second.p, w->cache_bb[c].p, w->src.p, w->cache_bb[c2].p};
+    const uint8_t* sphere = w->has_spheres ? w->sphere.p : nullptr;
     pcm::box_box_list(w->ctx, lp, w->pos.p, w->quat.p, w->half.p, kContactDistance, w->contacts_bb.p,
-                      w->counts_bb.p, w->flags_bb.p, w->pcm_scratch.p);
+                      w->counts_bb.p, w->flags_bb.p, w->pcm_scratch.p, sphere);
     w->ccur = c2;
     if (w->has_floor) {
         pcm::plane_all(w->ctx, n, w->pos.p, w->quat.p, w->half.p, w->floor.position, kContactDistance, w->cache_pl.p,
-                       w->contacts_pl.p, w->counts_pl.p, w->flags_pl.p);
+                       w->contacts_pl.p, w->counts_pl.p, w->flags_pl.p, sphere);
     }
     const u32 native_count = w->native_order ? native_order_list(w, L) : 0u;
     islands(w, L, c);
@@ -1837,6 +1846,43 @@ int fyzz_hip_world_add_dynamic_box(fyzz_hip_world* w, const float* half_extents,
         w->slot_of_id.push_back(static_cast<int>(w->host_bodies.size()));
         w->host_bodies.push_back(b);
         w->host_half.insert(w->host_half.end(), half_extents, half_extents + 3);
+        w->host_sphere.push_back(0);
+        if (body_id) *body_id = id;
+    });
+}
+
+// Scene::add_dynamic_sphere (scene.cpp:491-525): mass 4/3 pi r^3 density and
+// inertia 2/5 m r^2 in double, their inverses rounded to float (an inertia
+// whose inverse a float cannot hold leaves the body without rotation).
+int fyzz_hip_world_add_dynamic_sphere(fyzz_hip_world* w, float radius, float density, uint32_t* body_id) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(!w->n_scenes, kLogicError, "world: batched scenes hold boxes only");
+        FYZZ_REQUIRE(w->host_bodies.size() < w->max_bodies, kOverflow, "world: max_bodies reached");
+        FYZZ_REQUIRE(std::isfinite(radius) && radius > 0.0f && std::isfinite(density) && density > 0.0f,
+                     kInvalidArgument, "world: sphere radius and density must be positive and finite");
+        const double r = radius;
+        const double mass = 4.1887902047863909846168578443727 * r * r * r * density;
+        const double inv_mass = 1.0 / mass;
+        FYZZ_REQUIRE(std::isfinite(inv_mass) && inv_mass > 0.0 && inv_mass <= 3.402823466e+38 &&
+                         static_cast<float>(inv_mass) != 0.0f,
+                     kInvalidArgument, "world: sphere mass outside the supported range");
+        fyzz_dy_body b = default_body();
+        b.inverse_mass = static_cast<float>(inv_mass);
+        const double inertia = 0.4 * mass * r * r;
+        const double inv_inertia = 1.0 / inertia;
+        float inv = 0.0f;
+        if (std::isfinite(inv_inertia) && inv_inertia > 0.0 && inv_inertia <= 3.402823466e+38 &&
+            static_cast<float>(inv_inertia) > 0.0f)
+            inv = static_cast<float>(inv_inertia);
+        b.inverse_inertia[0] = b.inverse_inertia[1] = b.inverse_inertia[2] = inv;
+        const u32 id = static_cast<u32>(w->slot_of_id.size());
+        w->slot_of_id.push_back(static_cast<int>(w->host_bodies.size()));
+        w->host_bodies.push_back(b);
+        w->host_half.insert(w->host_half.end(), {radius, radius, radius});
+        w->host_sphere.push_back(1);
+        w->has_spheres = true;
         if (body_id) *body_id = id;
     });
 }
@@ -1877,6 +1923,7 @@ int fyzz_hip_world_add_dynamic_boxes(fyzz_hip_world* w, uint32_t n, const float*
             w->slot_of_id.push_back(static_cast<int>(w->host_bodies.size()));
             w->host_bodies.push_back(b);
             w->host_half.insert(w->host_half.end(), h, h + 3);
+            w->host_sphere.push_back(0);
         }
     });
 }
@@ -2168,6 +2215,8 @@ int fyzz_hip_world_set_scenes(fyzz_hip_world* w, uint32_t n_scenes, const uint32
         FYZZ_REQUIRE(n_scenes == 0 || box_counts, kInvalidArgument, "world: null scene box counts");
         FYZZ_REQUIRE(n_scenes == 0 || !w->native_order, kLogicError,
                      "world: the native solver order is for single-scene worlds");
+        FYZZ_REQUIRE(n_scenes == 0 || !w->has_spheres, kLogicError,
+                     "world: batched scenes hold boxes only");
         FYZZ_REQUIRE(!w->notify, kLogicError, "world: contact notifications are published for single-scene worlds");
         w->scene_counts.assign(box_counts, box_counts + n_scenes);
         w->n_scenes = n_scenes;

@@ -99,9 +99,11 @@ class Scene:
 
     def __init__(self, half, pos, quat, vel=None, ang=None, density=1000.0,
                  floor=True, floor_height=0.0, friction=0.5, restitution=0.0,
-                 gravity=True):
+                 gravity=True, sphere=None):
         n = len(half)
         self.n = n
+        # per body: a sphere of radius half[i, 0] instead of a box
+        self.sphere = np.zeros(n, bool) if sphere is None else np.asarray(sphere, bool).reshape(n)
         self.half = f32(half).reshape(n, 3)
         self.pos = f32(pos).reshape(n, 3)
         self.quat = np.ascontiguousarray(np.asarray(quat, dtype=np.float32).reshape(n, 4))

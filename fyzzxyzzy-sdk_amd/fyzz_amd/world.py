@@ -83,6 +83,12 @@ class World:
         self.n_dynamic += 1
         return out.value
 
+    def add_dynamic_sphere(self, radius: float, density: float = 1000.0) -> int:
+        out = C.c_uint32()
+        self._call("fyzz_hip_world_add_dynamic_sphere", C.c_float(radius), C.c_float(density), C.byref(out))
+        self.n_dynamic += 1
+        return out.value
+
     def add_dynamic_boxes(self, half_extents, positions, orientations, linear=None, angular=None,
                           density=1000.0, friction=0.5, restitution=0.0) -> int:
         """Bulk add_dynamic_box + set_body_pose/velocity/material; returns

@@ -144,6 +144,11 @@ int ref_add_dynamic_box(void* s, const float* half, const float* pos,
             {half[0], half[1], half[2]}, {pos[0], pos[1], pos[2]}, density);
     });
 }
+int ref_add_dynamic_sphere(void* s, float radius, const float* pos, float density, std::uint32_t* id) {
+    return guarded([&] {
+        *id = static_cast<Scene*>(s)->add_dynamic_sphere(radius, {pos[0], pos[1], pos[2]}, density);
+    });
+}
 // n axis-aligned dynamic boxes (the broadphase workload's load); ids[n]
 // receives the body ids.
 int ref_add_aabbs(void* s, std::uint32_t n, const float* half, const float* pos, float density,

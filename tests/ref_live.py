@@ -51,9 +51,14 @@ class RefRun:
         self.ids = np.zeros(n, np.uint32)
         for i in range(n):
             bid = C.c_uint32()
-            self._ok(ref.ref_add_dynamic_box(self.s, P(np.ascontiguousarray(scene.half[i], np.float32)),
-                                             P(np.ascontiguousarray(scene.pos[i], np.float32)),
-                                             C.c_float(scene.density), C.byref(bid)))
+            if scene.sphere[i]:
+                self._ok(ref.ref_add_dynamic_sphere(self.s, C.c_float(float(scene.half[i, 0])),
+                                                    P(np.ascontiguousarray(scene.pos[i], np.float32)),
+                                                    C.c_float(scene.density), C.byref(bid)))
+            else:
+                self._ok(ref.ref_add_dynamic_box(self.s, P(np.ascontiguousarray(scene.half[i], np.float32)),
+                                                 P(np.ascontiguousarray(scene.pos[i], np.float32)),
+                                                 C.c_float(scene.density), C.byref(bid)))
             self.ids[i] = bid.value
         for i in range(n):
             b = C.c_uint32(int(self.ids[i]))
@@ -289,3 +294,22 @@ def wake_scene(seed=None):
     half = np.concatenate([half, np.stack([0.3 + 0.2 * scenes.uniform(seed, 4 + a, m) for a in range(3)], 1)])
     yaw = np.concatenate([np.zeros(len(idx)), scenes.uniform(seed, 8, m) * 2 * np.pi])
     return scenes.Scene(half, pos, scenes.normalize_quat32(scenes.yaw_quat(yaw)))
+
+
+def mixed_scene(side=6, seed=None):
+    """side^3 bodies on a grid 1.1 m apart above the floor, alternately
+    spheres (radius U[0.3, 0.5]) and boxes (half extents U[0.3, 0.5], seeded
+    yaw), with small seeded velocities: they land and pile up, touching as
+    sphere/sphere, sphere/box, box/box, sphere/plane and box/plane pairs."""
+    from fyzz_amd import scenes
+    seed = scenes.SEED_BASE + 61 if seed is None else seed
+    n = side ** 3
+    idx = np.arange(n)
+    i, k, j = idx % side, (idx // side) % side, idx // (side * side)
+    pos = np.stack([(i - (side - 1) / 2) * 1.1, 0.6 + j * 1.1, (k - (side - 1) / 2) * 1.1], 1)
+    sphere = (i + j + k) % 2 == 0
+    half = np.stack([0.3 + 0.2 * scenes.uniform(seed, a, n) for a in range(3)], 1)
+    half[sphere] = half[sphere][:, :1]
+    yaw = np.where(sphere, 0.0, scenes.uniform(seed, 3, n) * 2 * np.pi)
+    vel = np.stack([scenes.normal(seed, 4 + a, n) * 0.3 for a in range(3)], 1)
+    return scenes.Scene(half, pos, scenes.normalize_quat32(scenes.yaw_quat(yaw)), vel, sphere=sphere)

@@ -26,7 +26,8 @@ def make_world(ctx, sc):
         fid = w.add_static_floor(sc.floor_height)
         w.set_body_material(fid, sc.friction, sc.friction, sc.restitution)
     for i in range(sc.n):
-        bid = w.add_dynamic_box(sc.half[i], sc.density)
+        bid = (w.add_dynamic_sphere(float(sc.half[i, 0]), sc.density) if sc.sphere[i]
+               else w.add_dynamic_box(sc.half[i], sc.density))
         w.set_body_pose(bid, sc.pos[i], sc.quat[i])
         w.set_body_velocity(bid, sc.vel[i])
         w.set_body_angular_velocity(bid, sc.ang[i])
@@ -277,22 +278,25 @@ def _native_run(ctx, sc, steps, events=False):
     return same, most
 
 
-@pytest.mark.parametrize("case", ["grid_1000", "hub", "sleep_wake", "pile_2000"])
+@pytest.mark.parametrize("case", ["grid_1000", "hub", "sleep_wake", "pile_2000", "spheres_and_boxes"])
 def test_native_order_world_matches_live_reference(ctx, torch_cuda, case):
     """World::step in the native solver order (the host island authority
     feeding the device solver list) against the live NativeCpu scene,
     end to end and free running, bit for bit: the 1,000-box grid over 240
     steps, the 65-partition hub over 60, the stack woken by dropped boxes
     over 600 (sleep, wakes through broadphase islands, re-sleep) and a
-    2,000-box pile over 300 (one island of ~900 managers in 10+ tasks);
-    on the grid and the sleep/wake stack the contact events too."""
+    2,000-box pile over 300 (one island of ~900 managers in 10+ tasks) and
+    108 spheres with 108 boxes over 240 (sphere/sphere, sphere/box and
+    sphere/plane contacts beside the box ones); on the grid, the sleep/wake
+    stack and the spheres the contact events too."""
     import ref_live
     if not ref_live.available():
         pytest.skip("compiled reference (oracle/_ref) not built")
     sc, steps = {"grid_1000": (scenes.grid_scene(), 240), "hub": (ref_live.hub_scene(), 60),
                  "sleep_wake": (ref_live.wake_scene(), 600),
-                 "pile_2000": (scenes.pile_scene(2000, footprint=15.0, height=160.0), 300)}[case]
-    same, most = _native_run(ctx, sc, steps, events=case in ("grid_1000", "sleep_wake"))
+                 "pile_2000": (scenes.pile_scene(2000, footprint=15.0, height=160.0), 300),
+                 "spheres_and_boxes": (ref_live.mixed_scene(), 240)}[case]
+    same, most = _native_run(ctx, sc, steps, events=case in ("grid_1000", "sleep_wake", "spheres_and_boxes"))
     print(f"{case}: steps bit-identical to NativeCpu {same}/{steps}, most managers {most}")
     assert most > 40
 
