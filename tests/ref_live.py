@@ -313,3 +313,46 @@ def mixed_scene(side=6, seed=None):
     yaw = np.where(sphere, 0.0, scenes.uniform(seed, 3, n) * 2 * np.pi)
     vel = np.stack([scenes.normal(seed, 4 + a, n) * 0.3 for a in range(3)], 1)
     return scenes.Scene(half, pos, scenes.normalize_quat32(scenes.yaw_quat(yaw)), vel, sphere=sphere)
+
+
+def sphere_cases():
+    """Isolated groups 12 m apart, each one sphere contact case at rest,
+    for comparing the first step's manifolds: spheres embedded in a box
+    (center nearest the +y face, the -x face, the exact center of a box
+    and of a cube, where exit faces tie), in a rotated box, touching a rotated box's edge
+    from outside, coincident sphere centers, spheres 0.03 m apart, a sphere
+    on and one through the floor, each box case with the box created both
+    before and after its sphere (both Gu orders)."""
+    from fyzz_amd import scenes
+    half, pos, quat, sphere = [], [], [], []
+    ident = [0.0, 0.0, 0.0, 1.0]
+    tilt = scenes.normalize_quat32(np.array([[0.2, 0.3, 0.1, 0.93]]))[0].tolist()
+    g = [0]
+
+    def body(h, p, q=ident, s=False):
+        half.append([h, h, h] if np.isscalar(h) else list(h))
+        pos.append([p[0] + 12.0 * g[0], p[1], p[2]])
+        quat.append(q)
+        sphere.append(s)
+
+    def group(*bodies):
+        for b in bodies:
+            body(*b)
+        g[0] += 1
+
+    box = (0.5, 0.6, 0.7)
+    for box_first in (True, False):
+        for center, q in (((0.1, 3.5, 0.05), ident), ((-0.4, 3.1, 0.1), ident), ((0.0, 3.0, 0.0), ident),
+                          ((0.15, 2.9, -0.2), tilt), ((0.62, 3.72, 0.0), tilt)):
+            b = (box, (0.0, 3.0, 0.0), q, False)
+            s = (0.3, center, ident, True)
+            group(*((b, s) if box_first else (s, b)))
+    for box_first in (True, False):  # a cube's center: all exit faces tie
+        b = ((0.5, 0.5, 0.5), (0.0, 3.0, 0.0), ident, False)
+        s = (0.3, (0.0, 3.0, 0.0), ident, True)
+        group(*((b, s) if box_first else (s, b)))
+    group((0.4, (0.0, 5.0, 0.0), ident, True), (0.25, (0.0, 5.0, 0.0), ident, True))
+    group((0.4, (0.0, 5.0, 0.0), ident, True), (0.25, (0.0, 5.68, 0.0), ident, True))
+    group((0.4, (0.0, 0.41, 0.0), ident, True))
+    group((0.4, (0.0, 0.3, 0.0), ident, True))
+    return scenes.Scene(np.array(half), np.array(pos), np.array(quat, np.float32), sphere=np.array(sphere))

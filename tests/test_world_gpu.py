@@ -316,3 +316,35 @@ def test_native_order_refuses_graphs_and_edits(ctx, torch_cuda):
         w.set_solver_order(False)  # after the first step
     w.step(1 / 120)
     w.close()
+
+
+def test_sphere_contact_cases_match_reference(ctx, torch_cuda):
+    """First-step manifolds of the sphere contact cases (embedded centers
+    and their exit-face ties, rotated boxes, edge contact, coincident
+    centers, a gap below the contact distance, floor contacts, both Gu
+    orders) against the live reference's, field by field and bit for bit:
+    pairs, counts, contact positions, normals and separations."""
+    import ref_live
+    if not ref_live.available():
+        pytest.skip("compiled reference (oracle/_ref) not built")
+    sc = ref_live.sphere_cases()
+    w = make_world(ctx, sc)
+    w.set_solver_order(True)
+    run = ref_live.RefRun(sc)
+    try:
+        w.step(1 / 120)
+        st = run.step(1 / 120)
+        mg, _, _ = w.managers()
+        want = st["managers"]
+        assert len(mg) == len(want) and len(mg) >= 16, (len(mg), len(want))
+        for f in ("body0", "body1", "count"):
+            np.testing.assert_array_equal(mg[f], want[f], err_msg=f)
+        for f in ("position", "normal", "separation"):
+            for k in range(4):
+                live = mg["count"] > k
+                np.testing.assert_array_equal(mg["contacts"][f][live, k], want["contacts"][f][live, k],
+                                              err_msg=f"contact {k} {f}")
+        np.testing.assert_array_equal(w.body_states(), st["next"][0])
+    finally:
+        run.close()
+        w.close()
