@@ -232,7 +232,7 @@ __device__ inline u32 scene_words(u32 n) { return (n + 31u) / 32u; }
 __global__ __launch_bounds__(kSceneBlock) void k_scene_masks(u32 n_scenes, const u32* first, const u32* mask_off,
                                                              const float* pos, const float* quat, const float* half,
                                                              const float* offset, u32* mask, u32* row_cnt,
-                                                             u32* scene_cnt, u32* status) {
+                                                             u32* scene_cnt, u32* status, const uint8_t* sphere) {
     // a box's keys as (min x, min y, min z, max x) and (max y, max z): two
     // LDS reads per tested pair
     __shared__ uint4 klo[kSceneMax];
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kSceneBlock) void k_scene_masks(u32 n_scenes, const
         }
         for (u32 i = threadIdx.x; i < n; i += kSceneBlock) {
             u32 k[6];
-            if (!keys::box_keys(pos, quat, half, offset, b0 + i, k)) atomicOr(&status[0], 1u);
+            if (!keys::box_keys(pos, quat, half, offset, b0 + i, k, sphere)) atomicOr(&status[0], 1u);
             klo[i] = make_uint4(k[0], k[1], k[2], k[3]);
             khi[i] = make_uint2(k[4], k[5]);
         }
@@ -1270,7 +1270,7 @@ sap::Lists scene_broadphase(fyzz_hip_world* w) {
     const u32 grid = std::min<u32>(S, 4096u);
     hipLaunchKernelGGL(k_scene_masks, dim3(grid), dim3(kSceneBlock), 0, st, S, w->scene_first.p,
                        w->scene_mask_off.p, w->pos.p, w->quat.p, w->half.p, w->offset.p, w->scene_mask.p,
-                       w->scene_rows.p, w->scene_cnt.p, w->scene_status.p);
+                       w->scene_rows.p, w->scene_cnt.p, w->scene_status.p, w->has_spheres ? w->sphere.p : nullptr);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->scene_n.p + 2, 1u, S);
     sel::exclusive_scan(st, S, w->scene_n.p + 2, SceneCount{w->scene_cnt.p}, w->scene_base.p, w->tiles.p,
                         w->scene_n.p + nxt);
@@ -1858,7 +1858,6 @@ int fyzz_hip_world_add_dynamic_sphere(fyzz_hip_world* w, float radius, float den
     if (!w) return kInvalidArgument;
     return guarded(w->ctx, [&] {
         w->require_authoring();
-        FYZZ_REQUIRE(!w->n_scenes, kLogicError, "world: batched scenes hold boxes only");
         FYZZ_REQUIRE(w->host_bodies.size() < w->max_bodies, kOverflow, "world: max_bodies reached");
         FYZZ_REQUIRE(std::isfinite(radius) && radius > 0.0f && std::isfinite(density) && density > 0.0f,
                      kInvalidArgument, "world: sphere radius and density must be positive and finite");
@@ -2215,8 +2214,6 @@ int fyzz_hip_world_set_scenes(fyzz_hip_world* w, uint32_t n_scenes, const uint32
         FYZZ_REQUIRE(n_scenes == 0 || box_counts, kInvalidArgument, "world: null scene box counts");
         FYZZ_REQUIRE(n_scenes == 0 || !w->native_order, kLogicError,
                      "world: the native solver order is for single-scene worlds");
-        FYZZ_REQUIRE(n_scenes == 0 || !w->has_spheres, kLogicError,
-                     "world: batched scenes hold boxes only");
         FYZZ_REQUIRE(!w->notify, kLogicError, "world: contact notifications are published for single-scene worlds");
         w->scene_counts.assign(box_counts, box_counts + n_scenes);
         w->n_scenes = n_scenes;

@@ -368,7 +368,8 @@ int fyzz_hip_world_add_static_floor(fyzz_hip_world* world, float height, uint32_
 /* World::add_dynamic_sphere (world.h:427; Scene::add_dynamic_sphere,
  * scene.cpp:491-525): radius, density. Sphere pairs take the stateless
  * sphere contacts of GuPCMContactSphere.cpp (sphere/sphere, sphere/box,
- * sphere/plane; no persistent manifold); single-scene worlds only. */
+ * sphere/plane; no persistent manifold), in single worlds and in
+ * batched scenes alike. */
 int fyzz_hip_world_add_dynamic_sphere(fyzz_hip_world* world, float radius, float density, uint32_t* body_id);
 /* World::add_dynamic_box (world.h:425): HOST half_extents[3], density */
 int fyzz_hip_world_add_dynamic_box(fyzz_hip_world* world, const float* half_extents, float density,

@@ -138,3 +138,48 @@ def test_batched_4096_scenes_spot_steps_match_oracle(ctx, orc, torch_cuda):
         assert managers > 100_000
     finally:
         w.close()
+
+
+def test_batched_scenes_with_spheres_match_single_scene_worlds(ctx, torch_cuda):
+    """Scenes holding spheres beside boxes: the batched world (per-scene
+    broadphase with sphere bounds) steps every scene bit-identically to a
+    world holding only that scene."""
+    import ref_live
+    from fyzz_amd.world import World
+    parts = [ref_live.mixed_scene(side=s, seed=scenes.SEED_BASE + 70 + s) for s in (2, 3, 4, 3)]
+
+    def author(w, sc, shift):
+        for i in range(sc.n):
+            bid = (w.add_dynamic_sphere(float(sc.half[i, 0]), sc.density) if sc.sphere[i]
+                   else w.add_dynamic_box(sc.half[i], sc.density))
+            w.set_body_pose(bid, sc.pos[i] + np.float32([shift, 0, 0]), sc.quat[i])
+            w.set_body_velocity(bid, sc.vel[i])
+            w.set_body_material(bid, sc.friction, sc.friction, sc.restitution)
+
+    def world(scs, counts=None):
+        w = World(ctx, sum(sc.n for sc in scs) + 1)
+        fid = w.add_static_floor(0.0)
+        w.set_body_material(fid, 0.5, 0.5, 0.0)
+        for k, sc in enumerate(scs):
+            author(w, sc, 0.0)  # scenes overlap in space; only the broadphase keeps them apart
+        if counts is not None:
+            w.set_scenes(counts)
+        return w
+
+    batched = world(parts, [sc.n for sc in parts])
+    singles = [world([sc]) for sc in parts]
+    first = np.concatenate([[0], np.cumsum([sc.n for sc in parts])])
+    try:
+        for s in range(120):
+            batched.step(1 / 120)
+            for w in singles:
+                w.step(1 / 120)
+            assert batched.stats()["flags"] == 0
+            got = batched.body_states()
+            for k, w in enumerate(singles):
+                np.testing.assert_array_equal(got[first[k]:first[k + 1]], w.body_states(), err_msg=f"step {s} scene {k}")
+        assert batched.stats()["managers"] > 0
+    finally:
+        batched.close()
+        for w in singles:
+            w.close()
