@@ -81,13 +81,16 @@ __device__ inline u32 lower_bound64(const u64* a, u32 n, u64 x) {
 // known: the broadphase's own previous-list index of each pair (an
 // incremental frame records it), else found by a binary search.
 __global__ void k_pairs(u32 cap, const u64* cur, const u32* d_cur, const u64* prev, const u32* d_prev, u32* first,
-                        u32* second, u32* src, u32* counted, const u32* known) {
+                        u32* second, u32* src, u32* counted, const u32* known, u32 n_dyn) {
     const u32 n = min(*d_cur, cap), np = min(*d_prev, cap);
     for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const u64 x = cur[t];
         const u32 lo = static_cast<u32>(x >> 32), hi = static_cast<u32>(x & 0xffffffffu);
-        first[t] = hi;
-        second[t] = lo;
+        // Gu order: two dynamic bodies the higher id first; a static body
+        // (slots from n_dyn on) second (scene.cpp:3278-3287)
+        const bool stat = hi >= n_dyn;
+        first[t] = stat ? lo : hi;
+        second[t] = stat ? hi : lo;
         if (known) {
             src[t] = known[t];
         } else {
@@ -95,7 +98,7 @@ __global__ void k_pairs(u32 cap, const u64* cur, const u32* d_cur, const u64* pr
             src[t] = (j < np && prev[j] == x) ? j : kNone;
         }
         atomicAdd(&counted[lo], 1u);
-        atomicAdd(&counted[hi], 1u);
+        if (!stat) atomicAdd(&counted[hi], 1u);
     }
 }
 
@@ -143,6 +146,7 @@ struct MgrWrite {
     const u32* d_step;  // this step's number (1, 2, ...); history must carry the previous one
     fyzz_dy_manager* managers;
     u32* mgr_slot;
+    u32 n_dyn;  // slots from here on are static bodies (solver static 1 + slot - n_dyn)
     __device__ void operator()(u32 i, u32 o) const {
         const u32 nbb = min(*d_nbb, cap);
         const u32 step = *d_step;
@@ -152,7 +156,7 @@ struct MgrWrite {
         u32 slot, hist_slot;
         if (i < nbb) {
             M.body0 = static_cast<int>(first[i]);
-            M.body1 = static_cast<int>(second[i]);
+            M.body1 = second[i] < n_dyn ? static_cast<int>(second[i]) : -2 - static_cast<int>(second[i] - n_dyn);
             M.count = counts_bb[i];
             c = contacts_bb + 28ull * i;
             slot = i;
@@ -518,7 +522,9 @@ __global__ void k_ev_points(EvArgs A, const u32* ev_src, const u32* ev_off, fyzz
         bool flip;
         if (stream == 0) {
             slot = A.s1[i];
-            flip = true;  // both dynamic: Gu order puts the higher body first
+            // Gu first the higher body (two dynamic ones, or a static one
+            // created before its dynamic partner): reversed
+            flip = A.shape[A.first[slot]] > A.shape[A.second[slot]];
         } else {
             const u32 b = A.s2[i];
             slot = A.cap + b;
@@ -711,15 +717,19 @@ __global__ void k_isl_mark(IslandArgs A) {
         A.root_awake[i] = 0u;
     }
     for (u32 i = t0; i < nbb; i += stride) {
+        // `first` is dynamic; `second` may be a static body (slot >= n),
+        // which is no island node: a found pair with it activates nothing,
+        // a found touch only the dynamic side
         const u32 a = A.first[i], b = A.second[i];
+        const bool dyn_b = b < A.n;
         const u32 s = A.src[i];
-        if (s == kNone && !((A.bodies[a].flags & A.bodies[b].flags) & kSleeping)) {
+        if (dyn_b && s == kNone && !((A.bodies[a].flags & A.bodies[b].flags) & kSleeping)) {
             set_bits(A.bodies[a], kActivating, 0u);
             set_bits(A.bodies[b], kActivating, 0u);
         }
         if (A.counts_bb[i] != 0u && !(s != kNone && A.touch_prev[s] != 0u)) {
             set_bits(A.bodies[a], kActivating, kReady);
-            set_bits(A.bodies[b], kActivating, kReady);
+            if (dyn_b) set_bits(A.bodies[b], kActivating, kReady);
         }
     }
     if (A.counts_pl)
@@ -782,6 +792,7 @@ __global__ void k_isl_spec_mixed(IslandArgs A) {
     u32 mixed = 0;
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb + ng; i += gridDim.x * blockDim.x) {
         const u64 x = spec_edge(A, i, nbb);
+        if (edge_hi(x) >= A.n) continue;  // a static body joins no island
         mixed += A.asleep[edge_lo(x)] != A.asleep[edge_hi(x)] ? 1u : 0u;
     }
     for (int d = kWave / 2; d > 0; d >>= 1) mixed += __shfl_xor(mixed, d, kWave);
@@ -795,7 +806,7 @@ __global__ void k_isl_spec_link(IslandArgs A) {
     if (!spec_needed(A)) return;
     const u32 nbb = min(*A.d_cur, A.cap);
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb; i += gridDim.x * blockDim.x)
-        if (link_edge(A, i, nbb)) unite(A.spec, edge_lo(A.cur[i]), edge_hi(A.cur[i]));
+        if (link_edge(A, i, nbb) && edge_hi(A.cur[i]) < A.n) unite(A.spec, edge_lo(A.cur[i]), edge_hi(A.cur[i]));
 }
 
 // Flattens the linked parents into labels.
@@ -843,6 +854,7 @@ __global__ void k_isl_union_spec(IslandArgs A) {
         if (link_edge(A, i, nbb)) continue;  // already united
         const u64 x = spec_edge(A, i, nbb);
         const u32 a = edge_lo(x), b = edge_hi(x);
+        if (b >= A.n) continue;  // a static body joins no island
         if (A.spec[a] == giant && A.spec[b] == giant) continue;
         unite(A.spec, a, b);
     }
@@ -885,6 +897,7 @@ __global__ void k_isl_union_acc(IslandArgs A) {
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < nbb; i += gridDim.x * blockDim.x) {
         if (A.counts_bb[i] == 0u) continue;
         const u32 a = A.first[i], b = A.second[i];
+        if (b >= A.n) continue;  // touching a static body does not hold an island awake
         const bool ra = A.bodies[a].flags & kReady, rb = A.bodies[b].flags & kReady;
         if (ra && rb)
             unite(A.acc, a, b);
@@ -951,7 +964,7 @@ __host__ __device__ inline void wake_record(fyzz_dy_body& b, bool queue = false)
 // `stamp` and `counts_bb` (touching: non-zero) are aligned with it.
 __global__ void k_author_pose(u32 slot, int changed, u32 cap, const u64* pairs, const u32* d_pairs,
                               CacheRec* cache_bb, CacheRec* cache_pl, int has_plane, u32* stamp,
-                              const u32* counts_bb, fyzz_dy_body* bodies) {
+                              const u32* counts_bb, fyzz_dy_body* bodies, u32 n_dyn) {
     const u32 np = d_pairs ? min(*d_pairs, cap) : 0u;
     const u32 stride = gridDim.x * blockDim.x;
     const u32 t0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -972,7 +985,9 @@ __global__ void k_author_pose(u32 slot, int changed, u32 cap, const u64* pairs, 
         const u64 x = pairs[i];
         const u32 lo = static_cast<u32>(x >> 32), hi = static_cast<u32>(x & 0xffffffffu);
         if (lo != slot && hi != slot) continue;
-        fyzz_dy_body& b = bodies[lo == slot ? hi : lo];
+        const u32 peer = lo == slot ? hi : lo;
+        if (peer >= n_dyn) continue;  // a static peer has nothing to wake
+        fyzz_dy_body& b = bodies[peer];
         // every writer of one peer stores the same values
         if (b.flags & 3u) wake_record(b, true);
     }
@@ -1005,6 +1020,14 @@ struct fyzz_hip_world {
     std::vector<fyzz_dy_body> host_bodies;
     std::vector<float> host_half;
     std::vector<uint8_t> host_sphere;  // per slot: a sphere (radius in its three half extents)
+    // static bodies other than the floor (slot_of_id = -2 - index): SAP
+    // slots n.. after the dynamic ones, solver statics 1.. after the floor
+    struct StaticBody {
+        float half[3], pos[3], quat[4];
+        uint8_t sphere;
+        float static_friction, dynamic_friction, restitution;
+    };
+    std::vector<StaticBody> host_statics;
     bool has_spheres = false;
     std::vector<int> slot_of_id;  // body id -> box slot, -1 for the floor
     bool has_floor = false;
@@ -1073,6 +1096,12 @@ struct fyzz_hip_world {
         if (dy) fyzz_hip_dy_destroy(dy);
         if (sap) fyzz_hip_sap_destroy(sap);
     }
+    // broadphase slot of a dynamic or static (non-floor) body id
+    u32 slot(u32 id) const {
+        const int s = slot_of_id[id];
+        return s >= 0 ? static_cast<u32>(s) : n_dynamic_slots() + static_cast<u32>(-2 - s);
+    }
+    u32 n_dynamic_slots() const { return static_cast<u32>(host_bodies.size()); }
     fyzz_dy_body& body(u32 id) {
         FYZZ_REQUIRE(id < slot_of_id.size() && slot_of_id[id] >= 0, kInvalidArgument, "world: not a dynamic body id");
         return host_bodies[static_cast<u32>(slot_of_id[id])];
@@ -1110,19 +1139,41 @@ void drop_graphs(fyzz_hip_world* w) {
         }
 }
 
+// The solver's statics: the floor, then the static bodies (solver static
+// 1 + k is static body k; a manager names it as -2 - k).
+void upload_statics(fyzz_hip_world* w) {
+    std::vector<fyzz_dy_static> rec(1 + w->host_statics.size());
+    rec[0] = w->floor;
+    for (std::size_t k = 0; k < w->host_statics.size(); ++k) {
+        const auto& S = w->host_statics[k];
+        fyzz_dy_static& r = rec[1 + k];
+        for (int a = 0; a < 3; ++a) r.position[a] = S.pos[a];
+        for (int a = 0; a < 4; ++a) r.orientation[a] = S.quat[a];
+        r.static_friction = S.static_friction;
+        r.dynamic_friction = S.dynamic_friction;
+        r.restitution = S.restitution;
+    }
+    hipStream_t st = w->ctx->stream;
+    FYZZ_HIP_CHECK(hipMemcpyAsync(w->statics.p, rec.data(), sizeof(fyzz_dy_static) * rec.size(), hipMemcpyHostToDevice,
+                                  st));
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+}
+
 void upload(fyzz_hip_world* w) {
     const u32 n = static_cast<u32>(w->host_bodies.size());
+    const u32 ns = static_cast<u32>(w->host_statics.size());
+    const u32 slots = n + ns;  // broadphase slots: the dynamic bodies, then the statics
     w->n = n;
-    w->pair_cap = std::max<u32>(8 * n, 4096);
+    w->pair_cap = std::max<u32>(8 * slots, 4096);
     w->mgr_cap = w->pair_cap + n;
-    const std::size_t nn = std::max<u32>(n, 1);
+    const std::size_t nn = std::max<u32>(n, 1), ss = std::max<u32>(slots, 1);
     hipStream_t st = w->ctx->stream;
     w->bodies.reserve(nn);
-    w->statics.reserve(1);
-    w->pos.reserve(3 * nn);
-    w->quat.reserve(4 * nn);
-    w->half.reserve(3 * nn);
-    w->offset.reserve(nn);
+    w->statics.reserve(1 + ns);
+    w->pos.reserve(3 * ss);
+    w->quat.reserve(4 * ss);
+    w->half.reserve(3 * ss);
+    w->offset.reserve(ss);
     w->counted.reserve(nn);
     w->counts_pl.reserve(nn);
     w->flags_pl.reserve(nn);
@@ -1139,7 +1190,7 @@ void upload(fyzz_hip_world* w) {
     w->d_total.reserve(1);
     w->d_nm.reserve(1);
     w->mgr_slot.reserve(w->mgr_cap);
-    w->shape.reserve(nn);
+    w->shape.reserve(ss);
     w->touch_pl.reserve(nn);
     FYZZ_HIP_CHECK(hipMemsetAsync(w->touch_pl.p, 0, sizeof(u32) * nn, st));
     for (int k = 0; k < 2; ++k) {
@@ -1147,12 +1198,12 @@ void upload(fyzz_hip_world* w) {
         FYZZ_HIP_CHECK(hipMemsetAsync(w->touch_bb[k].p, 0, sizeof(u32) * w->pair_cap, st));
     }
     {
-        std::vector<u32> shape_of_slot(n);
+        std::vector<u32> shape_of_slot(slots);
         for (std::size_t id = 0; id < w->slot_of_id.size(); ++id)
-            if (w->slot_of_id[id] >= 0) shape_of_slot[static_cast<u32>(w->slot_of_id[id])] = static_cast<u32>(id);
-        if (n)
-            FYZZ_HIP_CHECK(hipMemcpyAsync(w->shape.p, shape_of_slot.data(), sizeof(u32) * n, hipMemcpyHostToDevice,
-                                          st));
+            if (w->slot_of_id[id] >= 0 || w->slot_of_id[id] <= -2) shape_of_slot[w->slot(static_cast<u32>(id))] = static_cast<u32>(id);
+        if (slots)
+            FYZZ_HIP_CHECK(hipMemcpyAsync(w->shape.p, shape_of_slot.data(), sizeof(u32) * slots,
+                                          hipMemcpyHostToDevice, st));
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     }
     for (auto* b : {&w->isl_spec, &w->isl_acc, &w->isl_root_wake, &w->isl_root_poison, &w->isl_root_awake})
@@ -1178,15 +1229,31 @@ void upload(fyzz_hip_world* w) {
                                       hipMemcpyHostToDevice, st));
         FYZZ_HIP_CHECK(hipMemcpyAsync(w->half.p, w->host_half.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice,
                                       st));
-        std::vector<float> off(n, world::kContactOffset);
-        FYZZ_HIP_CHECK(hipMemcpyAsync(w->offset.p, off.data(), sizeof(float) * n, hipMemcpyHostToDevice, st));
+    }
+    if (slots) {
+        // statics: their poses and extents sit after the dynamic slots and stay
+        std::vector<float> spos(3 * ns), squat(4 * ns), shalf(3 * ns);
+        std::vector<uint8_t> sph(w->host_sphere);
+        for (u32 k = 0; k < ns; ++k) {
+            const auto& S = w->host_statics[k];
+            for (int a = 0; a < 3; ++a) spos[3 * k + a] = S.pos[a], shalf[3 * k + a] = S.half[a];
+            for (int a = 0; a < 4; ++a) squat[4 * k + a] = S.quat[a];
+            sph.push_back(S.sphere);
+        }
+        if (ns) {
+            FYZZ_HIP_CHECK(hipMemcpyAsync(w->pos.p + 3 * n, spos.data(), sizeof(float) * 3 * ns, hipMemcpyHostToDevice, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(w->quat.p + 4 * n, squat.data(), sizeof(float) * 4 * ns, hipMemcpyHostToDevice, st));
+            FYZZ_HIP_CHECK(hipMemcpyAsync(w->half.p + 3 * n, shalf.data(), sizeof(float) * 3 * ns, hipMemcpyHostToDevice, st));
+        }
+        std::vector<float> off(slots, world::kContactOffset);
+        FYZZ_HIP_CHECK(hipMemcpyAsync(w->offset.p, off.data(), sizeof(float) * slots, hipMemcpyHostToDevice, st));
         if (w->has_spheres) {
-            w->sphere.reserve(n);
-            FYZZ_HIP_CHECK(hipMemcpyAsync(w->sphere.p, w->host_sphere.data(), n, hipMemcpyHostToDevice, st));
+            w->sphere.reserve(slots);
+            FYZZ_HIP_CHECK(hipMemcpyAsync(w->sphere.p, sph.data(), slots, hipMemcpyHostToDevice, st));
         }
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     }
-    FYZZ_HIP_CHECK(hipMemcpyAsync(w->statics.p, &w->floor, sizeof(fyzz_dy_static), hipMemcpyHostToDevice, st));
+    upload_statics(w);
     int rc = 0;
     if (w->n_scenes) {
         u64 total = 0;
@@ -1226,10 +1293,10 @@ void upload(fyzz_hip_world* w) {
         FYZZ_HIP_CHECK(hipMemsetAsync(w->scene_status.p, 0, 4 * sizeof(u32), st));
         FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     } else {
-        std::vector<u32> ids(n);
-        std::vector<uint8_t> dyn(n, 1);
-        for (u32 i = 0; i < n; ++i) ids[i] = i;
-        rc = fyzz_hip_sap_create(w->ctx, n, ids.data(), dyn.data(), w->pair_cap, w->pair_cap, &w->sap);
+        std::vector<u32> ids(slots);
+        std::vector<uint8_t> dyn(slots, 0);
+        for (u32 i = 0; i < slots; ++i) ids[i] = i, dyn[i] = i < n ? 1 : 0;
+        rc = fyzz_hip_sap_create(w->ctx, slots, ids.data(), dyn.data(), w->pair_cap, w->pair_cap, &w->sap);
         if (rc) throw Error(rc, w->ctx->last_error);
         if (w->has_spheres) sap::set_spheres(w->sap, w->sphere.p);
     }
@@ -1240,12 +1307,11 @@ void upload(fyzz_hip_world* w) {
     if (w->native_order) {
         const u32 ids = static_cast<u32>(w->slot_of_id.size());
         std::vector<uint8_t> dynamic(ids, 0);
-        w->id_of_slot.assign(n, 0);
-        for (u32 i = 0; i < ids; ++i)
-            if (w->slot_of_id[i] >= 0) {
-                dynamic[i] = 1;
-                w->id_of_slot[static_cast<u32>(w->slot_of_id[i])] = i;
-            }
+        w->id_of_slot.assign(slots, 0);
+        for (u32 i = 0; i < ids; ++i) {
+            if (w->slot_of_id[i] >= 0) dynamic[i] = 1;
+            if (w->slot_of_id[i] >= 0 || w->slot_of_id[i] <= -2) w->id_of_slot[w->slot(i)] = i;
+        }
         FYZZ_REQUIRE(ids > 0 && fyzz_hip_islands_create(ids, dynamic.data(), &w->order_auth) == 0, kOutOfMemory,
                      "world: cannot create the island authority");
         w->native_list.reserve(w->mgr_cap);
@@ -1419,6 +1485,7 @@ u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
     if (w->step_index == 1 && w->has_floor)  // the plane pairs every box from the first step on
         for (u32 k = 0; k < n; ++k) found.push_back(id_key(w->floor_id, id[k]));
     std::sort(found.begin(), found.end());
+    std::sort(lost.begin(), lost.end());  // static slots follow the dynamic ones, not id order
     std::vector<u64> touch;
     for (u32 t = 0; t < nbb; ++t)
         if (cbb[t]) touch.push_back(slot_pair(keys[t]));
@@ -1459,7 +1526,7 @@ u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
         if (w->has_floor && (p == w->floor_id || q == w->floor_id)) {
             idx = nbb + static_cast<u32>(w->slot_of_id[p == w->floor_id ? q : p]);
         } else {
-            u32 sp = static_cast<u32>(w->slot_of_id[p]), sq = static_cast<u32>(w->slot_of_id[q]);
+            u32 sp = w->slot(p), sq = w->slot(q);
             if (sp > sq) std::swap(sp, sq);
             const u64 k = (u64(sp) << 32) | sq;
             const auto it = std::lower_bound(keys.begin(), keys.end(), k);
@@ -1506,7 +1573,7 @@ void step(fyzz_hip_world* w, float dt) {
     const sap::Lists L = w->n_scenes ? scene_broadphase(w) : sap_broadphase(w);
     hipLaunchKernelGGL(k_fill, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, w->counted.p, n, 0u);
     hipLaunchKernelGGL(k_pairs, dim3(std::min<u32>(grid_for(cap, 256), 4096u)), dim3(256), 0, st, cap, L.cur, L.d_cur,
-                       L.prev, L.d_prev, w->first.p, w->second.p, w->src.p, w->counted.p, L.src);
+                       L.prev, L.d_prev, w->first.p, w->second.p, w->src.p, w->counted.p, L.src, n);
     const u32 planes = w->has_floor ? 1u : 0u;
     hipLaunchKernelGGL(k_counted, dim3(grid_for(n, 256)), dim3(256), 0, st, n, w->bodies.p, w->counted.p, planes);
     const int c = w->ccur, c2 = c ^ 1;  // previous / current pair-aligned buffers
@@ -1530,7 +1597,7 @@ void step(fyzz_hip_world* w, float dt) {
         ProfScope ps(&w->ctx->prof, "world.managers", st);
         MgrWrite wr{L.d_cur,       cap,          w->first.p,         w->second.p,   w->src.p,
                     w->counts_bb.p, w->contacts_bb.p, w->counts_pl.p, w->contacts_pl.p, w->stamp[h].p,
-                    w->d_step.p,   w->managers.p, w->mgr_slot.p};
+                    w->d_step.p,   w->managers.p, w->mgr_slot.p, n};
         if (w->native_order) {
             hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->d_nm.p, 1u, native_count);
             if (native_count)
@@ -1541,7 +1608,7 @@ void step(fyzz_hip_world* w, float dt) {
             sel::select(st, w->mgr_cap, w->d_total.p, pred, wr, w->tiles.p, w->d_nm.p);
         }
     }
-    dy::step_device(w->dy, dt, n, w->bodies.p, 1, w->statics.p, w->mgr_cap, w->d_nm.p, w->managers.p, w->hist[h].p,
+    dy::step_device(w->dy, dt, n, w->bodies.p, 1 + static_cast<u32>(w->host_statics.size()), w->statics.p, w->mgr_cap, w->d_nm.p, w->managers.p, w->hist[h].p,
                     w->mgr_cap, w->patches.p, w->impulses.p, w->parts.p);
     hipLaunchKernelGGL(k_scatter_patches, dim3(std::min<u32>(grid_for(w->mgr_cap, 256), 2048u)), dim3(256), 0, st,
                        w->d_nm.p, w->mgr_cap, w->patches.p, w->mgr_slot.p, w->hist[h ^ 1].p, w->stamp[h ^ 1].p,
@@ -1681,7 +1748,7 @@ void author_pose(fyzz_hip_world* w, u32 slot, bool changed) {
     const u32 cap = w->pair_cap;
     hipLaunchKernelGGL(k_author_pose, dim3(std::min<u32>(grid_for(std::max(cap, w->mgr_cap), 256), 1024u)), dim3(256),
                        0, st, slot, changed ? 1 : 0, cap, L.cur, L.d_cur, w->cache_bb[w->ccur].p, w->cache_pl.p,
-                       w->has_floor ? 1 : 0, w->stamp[w->hcur].p, w->counts_bb.p, w->bodies.p);
+                       w->has_floor ? 1 : 0, w->stamp[w->hcur].p, w->counts_bb.p, w->bodies.p, w->n);
     FYZZ_HIP_CHECK(hipGetLastError());
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
 }
@@ -1851,6 +1918,46 @@ int fyzz_hip_world_add_dynamic_box(fyzz_hip_world* w, const float* half_extents,
     });
 }
 
+// Scene::add_static_box / add_static_sphere (scene.cpp:441-488): a static
+// body with one shape at `position`, identity orientation (set_body_pose
+// may turn it before the first step), the default material.
+namespace {
+int add_static(fyzz_hip_world* w, const float half[3], bool sphere, const float* position, uint32_t* body_id) {
+    return guarded(w->ctx, [&] {
+        w->require_authoring();
+        FYZZ_REQUIRE(!w->n_scenes, kLogicError, "world: batched scenes hold no static bodies but the floor");
+        FYZZ_REQUIRE(position, kInvalidArgument, "world: null position");
+        for (int k = 0; k < 3; ++k)
+            FYZZ_REQUIRE(std::isfinite(half[k]) && half[k] > 0.0f && std::isfinite(position[k]), kInvalidArgument,
+                         "world: static extents must be positive and finite, the position finite");
+        fyzz_hip_world::StaticBody S{};
+        for (int k = 0; k < 3; ++k) S.half[k] = half[k], S.pos[k] = position[k];
+        S.quat[3] = 1.0f;
+        S.sphere = sphere ? 1 : 0;
+        S.static_friction = S.dynamic_friction = 0.5f;
+        S.restitution = 0.2f;
+        const u32 id = static_cast<u32>(w->slot_of_id.size());
+        w->slot_of_id.push_back(-2 - static_cast<int>(w->host_statics.size()));
+        w->host_statics.push_back(S);
+        if (sphere) w->has_spheres = true;
+        if (body_id) *body_id = id;
+    });
+}
+}  // namespace
+
+int fyzz_hip_world_add_static_box(fyzz_hip_world* w, const float* half_extents, const float* position,
+                                  uint32_t* body_id) {
+    if (!w) return kInvalidArgument;
+    if (!half_extents) return kInvalidArgument;
+    return add_static(w, half_extents, false, position, body_id);
+}
+
+int fyzz_hip_world_add_static_sphere(fyzz_hip_world* w, float radius, const float* position, uint32_t* body_id) {
+    if (!w) return kInvalidArgument;
+    const float h[3] = {radius, radius, radius};
+    return add_static(w, h, true, position, body_id);
+}
+
 // Scene::add_dynamic_sphere (scene.cpp:491-525): mass 4/3 pi r^3 density and
 // inertia 2/5 m r^2 in double, their inverses rounded to float (an inertia
 // whose inverse a float cannot hold leaves the body without rotation).
@@ -1945,6 +2052,15 @@ int fyzz_hip_world_set_body_pose(fyzz_hip_world* w, uint32_t body_id, const floa
         for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(position[k]);
         for (int k = 0; k < 4; ++k) finite = finite && std::isfinite(q[k]);
         FYZZ_REQUIRE(finite && std::fabs(mag - 1.0f) < 1.0e-2f, kInvalidArgument, "world: rigid pose is not sane");
+        if (body_id < w->slot_of_id.size() && w->slot_of_id[body_id] <= -2) {
+            FYZZ_REQUIRE(!w->uploaded, kLogicError, "world: static bodies are posed before the first step");
+            auto& S = w->host_statics[static_cast<u32>(-2 - w->slot_of_id[body_id])];
+            const bool same = q[0] == S.quat[0] && q[1] == S.quat[1] && q[2] == S.quat[2] && q[3] == S.quat[3];
+            for (int k = 0; k < 3; ++k) S.pos[k] = position[k];
+            if (!same)
+                for (int k = 0; k < 4; ++k) S.quat[k] = q[k] / mag;
+            return;
+        }
         fyzz_dy_body tmp;
         u32 slot = 0;
         fyzz_dy_body& b = edit_body(w, body_id, tmp, &slot);
@@ -2010,6 +2126,14 @@ int fyzz_hip_world_set_body_material(fyzz_hip_world* w, uint32_t body_id, float 
                                               st));
                 FYZZ_HIP_CHECK(hipStreamSynchronize(st));
             }
+            return;
+        }
+        if (body_id < w->slot_of_id.size() && w->slot_of_id[body_id] <= -2) {
+            auto& S = w->host_statics[static_cast<u32>(-2 - w->slot_of_id[body_id])];
+            S.static_friction = static_friction;
+            S.dynamic_friction = dynamic_friction;
+            S.restitution = restitution;
+            if (w->uploaded) upload_statics(w);
             return;
         }
         fyzz_dy_body tmp;
@@ -2214,6 +2338,8 @@ int fyzz_hip_world_set_scenes(fyzz_hip_world* w, uint32_t n_scenes, const uint32
         FYZZ_REQUIRE(n_scenes == 0 || box_counts, kInvalidArgument, "world: null scene box counts");
         FYZZ_REQUIRE(n_scenes == 0 || !w->native_order, kLogicError,
                      "world: the native solver order is for single-scene worlds");
+        FYZZ_REQUIRE(n_scenes == 0 || w->host_statics.empty(), kLogicError,
+                     "world: batched scenes hold no static bodies but the floor");
         FYZZ_REQUIRE(!w->notify, kLogicError, "world: contact notifications are published for single-scene worlds");
         w->scene_counts.assign(box_counts, box_counts + n_scenes);
         w->n_scenes = n_scenes;

@@ -89,6 +89,8 @@ SIGNATURES = {
                                               C.POINTER(vp)]),
     "fyzz_hip_world_set_solver_order": (C.c_int, [vp, C.c_int]),
     "fyzz_hip_world_add_dynamic_sphere": (C.c_int, [vp, C.c_float, C.c_float, u32p]),
+    "fyzz_hip_world_add_static_box": (C.c_int, [vp, vp, vp, u32p]),
+    "fyzz_hip_world_add_static_sphere": (C.c_int, [vp, C.c_float, vp, u32p]),
     "fyzz_hip_islands_create": (C.c_int, [C.c_uint32, vp, C.POINTER(vp)]),
     "fyzz_hip_islands_destroy": (None, [vp]),
     "fyzz_hip_islands_step": (C.c_int, [vp, vp, vp]),

@@ -104,6 +104,9 @@ class Scene:
         self.n = n
         # per body: a sphere of radius half[i, 0] instead of a box
         self.sphere = np.zeros(n, bool) if sphere is None else np.asarray(sphere, bool).reshape(n)
+        # static bodies created after the dynamic ones: (half extents or
+        # radius, position, orientation, is_sphere)
+        self.statics = []
         self.half = f32(half).reshape(n, 3)
         self.pos = f32(pos).reshape(n, 3)
         self.quat = np.ascontiguousarray(np.asarray(quat, dtype=np.float32).reshape(n, 4))

@@ -83,6 +83,19 @@ class World:
         self.n_dynamic += 1
         return out.value
 
+    def add_static_box(self, half_extents, position) -> int:
+        h, hp = _f32(half_extents, 3)
+        p, pp = _f32(position, 3)
+        out = C.c_uint32()
+        self._call("fyzz_hip_world_add_static_box", hp, pp, C.byref(out))
+        return out.value
+
+    def add_static_sphere(self, radius: float, position) -> int:
+        p, pp = _f32(position, 3)
+        out = C.c_uint32()
+        self._call("fyzz_hip_world_add_static_sphere", C.c_float(radius), pp, C.byref(out))
+        return out.value
+
     def add_dynamic_sphere(self, radius: float, density: float = 1000.0) -> int:
         out = C.c_uint32()
         self._call("fyzz_hip_world_add_dynamic_sphere", C.c_float(radius), C.c_float(density), C.byref(out))

@@ -365,6 +365,16 @@ int fyzz_hip_world_create(fyzz_hip_context* ctx, uint32_t max_bodies, fyzz_hip_w
 int fyzz_hip_world_destroy(fyzz_hip_world* world);
 /* World::add_static_floor (world.h:424) */
 int fyzz_hip_world_add_static_floor(fyzz_hip_world* world, float height, uint32_t* body_id);
+/* World::add_static_box / add_static_sphere (world.h:269, 272;
+ * Scene::add_static_box / add_static_sphere, scene.cpp:441-488): a static
+ * body at HOST position[3] (identity orientation; fyzz_hip_world_set_body_pose
+ * may turn it before the first step), the default material
+ * (fyzz_hip_world_set_body_material changes it at any time). Statics pair
+ * with dynamic bodies only; the Gu order puts them second, and the solver
+ * treats them as it treats the floor. Single-scene worlds only. */
+int fyzz_hip_world_add_static_box(fyzz_hip_world* world, const float* half_extents, const float* position,
+                                  uint32_t* body_id);
+int fyzz_hip_world_add_static_sphere(fyzz_hip_world* world, float radius, const float* position, uint32_t* body_id);
 /* World::add_dynamic_sphere (world.h:427; Scene::add_dynamic_sphere,
  * scene.cpp:491-525): radius, density. Sphere pairs take the stateless
  * sphere contacts of GuPCMContactSphere.cpp (sphere/sphere, sphere/box,

@@ -229,6 +229,9 @@ int ref_add_static_box(void* s, const float* half, const float* pos,
             {half[0], half[1], half[2]}, {pos[0], pos[1], pos[2]});
     });
 }
+int ref_add_static_sphere(void* s, float radius, const float* pos, std::uint32_t* id) {
+    return guarded([&] { *id = static_cast<Scene*>(s)->add_static_sphere(radius, {pos[0], pos[1], pos[2]}); });
+}
 int ref_set_pose(void* s, std::uint32_t id, const float* pos, const float* q) {
     return guarded([&] {
         static_cast<Scene*>(s)->set_pose(id, {pos[0], pos[1], pos[2]},

@@ -70,6 +70,16 @@ class RefRun:
             self._ok(ref.ref_set_material(self.s, b, C.c_float(scene.friction), C.c_float(scene.friction),
                                           C.c_float(scene.restitution)))
         assert list(self.ids) == list(range(1, n + 1))
+        for h, pos, quat, is_sphere in scene.statics:
+            sid = C.c_uint32()
+            p = P(np.ascontiguousarray(pos, np.float32))
+            if is_sphere:
+                self._ok(ref.ref_add_static_sphere(self.s, C.c_float(float(h)), p, C.byref(sid)))
+            else:
+                self._ok(ref.ref_add_static_box(self.s, P(np.ascontiguousarray(h, np.float32)), p, C.byref(sid)))
+            self._ok(ref.ref_set_pose(self.s, sid, p, P(np.ascontiguousarray(quat, np.float32))))
+            self._ok(ref.ref_set_material(self.s, sid, C.c_float(scene.friction), C.c_float(scene.friction),
+                                          C.c_float(scene.restitution)))
 
     def _ok(self, rc):
         if rc != 0:
@@ -356,3 +366,30 @@ def sphere_cases():
     group((0.4, (0.0, 0.41, 0.0), ident, True))
     group((0.4, (0.0, 0.3, 0.0), ident, True))
     return scenes.Scene(np.array(half), np.array(pos), np.array(quat, np.float32), sphere=np.array(sphere))
+
+
+def statics_scene(seed=None):
+    """Dynamic spheres and boxes dropped onto static bodies: a tilted
+    static slab on four legs' worth of height (half extents 3 x 0.25 x 3 at
+    y = 2, turned 6 degrees about z) and a static sphere of radius 1.5
+    resting in the floor's plane beside it; 48 bodies above the slab and 16
+    above the sphere, alternately spheres and boxes."""
+    from fyzz_amd import scenes
+    seed = scenes.SEED_BASE + 62 if seed is None else seed
+    a = np.arange(48)
+    i, k, j = a % 4, (a // 4) % 4, a // 16
+    pos1 = np.stack([(i - 1.5) * 1.2, 3.2 + j * 1.1, (k - 1.5) * 1.2], 1)
+    b = np.arange(16)
+    pos2 = np.stack([7.0 + (b % 4 - 1.5) * 0.6, 3.6 + (b // 4) * 1.1, (b % 2 - 0.5) * 0.5], 1)
+    pos = np.concatenate([pos1, pos2])
+    n = len(pos)
+    sphere = np.arange(n) % 2 == 0
+    half = np.stack([0.25 + 0.2 * scenes.uniform(seed, c, n) for c in range(3)], 1)
+    half[sphere] = half[sphere][:, :1]
+    yaw = np.where(sphere, 0.0, scenes.uniform(seed, 3, n) * 2 * np.pi)
+    sc = scenes.Scene(half, pos, scenes.normalize_quat32(scenes.yaw_quat(yaw)), sphere=sphere)
+    t = 6.0 * np.pi / 180.0
+    slab_q = scenes.normalize_quat32(np.array([[0.0, 0.0, np.sin(t / 2), np.cos(t / 2)]]))[0]
+    sc.statics = [((3.0, 0.25, 3.0), (0.0, 2.0, 0.0), slab_q, False),
+                  (1.5, (7.0, 1.5, 0.0), np.float32([0, 0, 0, 1]), True)]
+    return sc

@@ -32,6 +32,10 @@ def make_world(ctx, sc):
         w.set_body_velocity(bid, sc.vel[i])
         w.set_body_angular_velocity(bid, sc.ang[i])
         w.set_body_material(bid, sc.friction, sc.friction, sc.restitution)
+    for h, pos, quat, is_sphere in getattr(sc, "statics", []):
+        sid = w.add_static_sphere(float(h), pos) if is_sphere else w.add_static_box(h, pos)
+        w.set_body_pose(sid, pos, quat)
+        w.set_body_material(sid, sc.friction, sc.friction, sc.restitution)
     return w
 
 
@@ -258,7 +262,9 @@ def _native_run(ctx, sc, steps, events=False):
                 kinds += np.bincount(rev[:, 2].astype(np.int64), minlength=3)
             mg, _, _ = w.managers()
             b0 = mg["body0"].astype(np.int64) + 1
-            b1 = np.where(mg["body1"] < 0, 0, mg["body1"].astype(np.int64) + 1)
+            b1 = mg["body1"].astype(np.int64)
+            # -1 the floor (id 0), -2 - k static body k (created after the n dynamic ones)
+            b1 = np.where(b1 == -1, 0, np.where(b1 < -1, sc.n + 1 + (-2 - b1), b1 + 1))
             got = np.stack([np.minimum(b0, b1), np.maximum(b0, b1)], 1)
             want = np.asarray(st["keys"], np.int64).reshape(-1, 2)
             np.testing.assert_array_equal(got, want, err_msg=f"step {s}: solver order")
@@ -278,7 +284,7 @@ def _native_run(ctx, sc, steps, events=False):
     return same, most
 
 
-@pytest.mark.parametrize("case", ["grid_1000", "hub", "sleep_wake", "pile_2000", "spheres_and_boxes"])
+@pytest.mark.parametrize("case", ["grid_1000", "hub", "sleep_wake", "pile_2000", "spheres_and_boxes", "statics"])
 def test_native_order_world_matches_live_reference(ctx, torch_cuda, case):
     """World::step in the native solver order (the host island authority
     feeding the device solver list) against the live NativeCpu scene,
@@ -295,8 +301,10 @@ def test_native_order_world_matches_live_reference(ctx, torch_cuda, case):
     sc, steps = {"grid_1000": (scenes.grid_scene(), 240), "hub": (ref_live.hub_scene(), 60),
                  "sleep_wake": (ref_live.wake_scene(), 600),
                  "pile_2000": (scenes.pile_scene(2000, footprint=15.0, height=160.0), 300),
-                 "spheres_and_boxes": (ref_live.mixed_scene(), 240)}[case]
-    same, most = _native_run(ctx, sc, steps, events=case in ("grid_1000", "sleep_wake", "spheres_and_boxes"))
+                 "spheres_and_boxes": (ref_live.mixed_scene(), 240),
+                 "statics": (ref_live.statics_scene(), 300)}[case]
+    same, most = _native_run(ctx, sc, steps,
+                             events=case in ("grid_1000", "sleep_wake", "spheres_and_boxes", "statics"))
     print(f"{case}: steps bit-identical to NativeCpu {same}/{steps}, most managers {most}")
     assert most > 40
 
