@@ -1167,6 +1167,19 @@ struct fyzz_hip_sap {
 };
 
 namespace fyzz_hip::sap {
+std::size_t sort_keys_scratch_words(u32 cap) {
+    const radix::Batch b{cap, 1, cap, 0, nullptr, nullptr, 0, nullptr};
+    return radix::count_words(b) + radix::kBins;
+}
+void sort_keys(hipStream_t st, u64* keys, u32* vals, u64* tmp_keys, u32* tmp_vals, u32 cap, const u32* d_count,
+               int bits, u32* scratch) {
+    const radix::Batch b{cap, 1, cap, 0, d_count, nullptr, 0, nullptr};
+    u32* counts = scratch;
+    u32* totals = scratch + radix::count_words(b);
+    (void)radix::sort<u64>(st, b, radix::Source::Arrays, keys, vals, keys, vals, tmp_keys, tmp_vals, bits, counts,
+                           totals);
+    FYZZ_HIP_CHECK(hipGetLastError());
+}
 bool built(fyzz_hip_sap* h) { return h->s.built; }
 void set_spheres(fyzz_hip_sap* h, const uint8_t* sphere) { h->s.sphere = sphere; }
 void replay_advance(fyzz_hip_sap* h) {

@@ -38,6 +38,13 @@ void bind_fixed_workspace(int device);
 // bounds are its radius half[3 slot] on every axis, whatever its
 // orientation. The pointer must stay valid while the broadphase advances.
 void set_spheres(fyzz_hip_sap* h, const uint8_t* sphere);
+// Stable LSD radix sort of (u64 key, u32 value) pairs in place: the low
+// `bits` key bits (a multiple of 16, so the result lands back in `keys`),
+// *d_count (device) entries of at most cap; scratch of
+// sort_keys_scratch_words(cap) words.
+std::size_t sort_keys_scratch_words(u32 cap);
+void sort_keys(hipStream_t st, u64* keys, u32* vals, u64* tmp_keys, u32* tmp_vals, u32 cap, const u32* d_count,
+               int bits, u32* scratch);
 }  // namespace sap
 
 namespace pcm {

@@ -1085,6 +1085,11 @@ struct fyzz_hip_world {
     DevBuf<u32> native_list;
     DevBuf<uint8_t> flag_bytes;
     DevBuf<uint8_t> sphere;  // per slot, when the world holds spheres
+    // a static body created before a dynamic one: slot order is not id
+    // order, so the event streams built in slot order are sorted by id
+    bool ids_out_of_slot_order = false;
+    DevBuf<u64> ev_tmp_keys;
+    DevBuf<u32> ev_tmp_vals, ev_sort_scratch;
 
     ~fyzz_hip_world() {
         if (order_auth) fyzz_hip_islands_destroy(order_auth);
@@ -1303,6 +1308,15 @@ void upload(fyzz_hip_world* w) {
     rc = fyzz_hip_dy_create(w->ctx, n, w->mgr_cap, &w->dy);
     if (rc) throw Error(rc, w->ctx->last_error);
     if (w->req_partition_blocks) dy::set_coop_blocks(w->dy, w->req_partition_blocks, w->req_pgs_blocks);
+    {
+        // a static id below a dynamic one breaks id order = slot order
+        int max_dyn = -1, min_static = 0x7fffffff;
+        for (std::size_t id = 0; id < w->slot_of_id.size(); ++id) {
+            if (w->slot_of_id[id] >= 0) max_dyn = static_cast<int>(id);
+            else if (w->slot_of_id[id] <= -2) min_static = std::min(min_static, static_cast<int>(id));
+        }
+        w->ids_out_of_slot_order = min_static < max_dyn;
+    }
     if (w->notify) reserve_events(w);
     if (w->native_order) {
         const u32 ids = static_cast<u32>(w->slot_of_id.size());
@@ -1364,6 +1378,11 @@ void reserve_events(fyzz_hip_world* w) {
     w->k2.reserve(std::max<u32>(n, 1));
     w->s3.reserve(cap);
     w->k3.reserve(cap);
+    if (w->ids_out_of_slot_order) {
+        w->ev_tmp_keys.reserve(cap);
+        w->ev_tmp_vals.reserve(cap);
+        w->ev_sort_scratch.reserve(sap::sort_keys_scratch_words(cap));
+    }
     w->ev_counts.reserve(4);
     w->ev_src.reserve(w->ev_cap);
     w->ev_pts.reserve(w->ev_cap);
@@ -1394,6 +1413,14 @@ void publish_events(fyzz_hip_world* w, const sap::Lists& L, int c_prev, int c_cu
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->ev_counts.p + 1, 1u, 0u);
     }
     sel::select(st, cap, L.d_lost, EvPred3{A}, EvWrite3{A, w->k3.p, w->s3.p}, w->tiles.p, w->ev_counts.p + 2);
+    if (w->ids_out_of_slot_order) {
+        // streams 1 and 3 were compacted in slot order; the merge needs id order
+        const int bits = w->slot_of_id.size() <= (1u << 16) ? 48 : 64;
+        sap::sort_keys(st, w->k1.p, w->s1.p, w->ev_tmp_keys.p, w->ev_tmp_vals.p, cap, w->ev_counts.p, bits,
+                       w->ev_sort_scratch.p);
+        sap::sort_keys(st, w->k3.p, w->s3.p, w->ev_tmp_keys.p, w->ev_tmp_vals.p, cap, w->ev_counts.p + 2, bits,
+                       w->ev_sort_scratch.p);
+    }
     hipLaunchKernelGGL(k_ev_total, dim3(1), dim3(1), 0, st, w->ev_counts.p);
     const u32 grid = std::min<u32>(grid_for(w->ev_cap, 256), 2048u);
     hipLaunchKernelGGL(k_ev_merge, dim3(grid), dim3(256), 0, st, A, w->events.p, w->ev_src.p, w->ev_pts.p);

@@ -48,6 +48,9 @@ class RefRun:
         self._ok(ref.ref_add_floor(self.s, C.c_float(scene.floor_height), C.byref(fid)))
         self._ok(ref.ref_set_material(self.s, fid, C.c_float(scene.friction), C.c_float(scene.friction),
                                       C.c_float(scene.restitution)))
+        first = getattr(scene, "statics_first", False)
+        if first:
+            self._add_statics(scene)
         self.ids = np.zeros(n, np.uint32)
         for i in range(n):
             bid = C.c_uint32()
@@ -69,7 +72,13 @@ class RefRun:
                 self._ok(ref.ref_set_angular_velocity(self.s, b, P(np.ascontiguousarray(scene.ang[i], np.float32))))
             self._ok(ref.ref_set_material(self.s, b, C.c_float(scene.friction), C.c_float(scene.friction),
                                           C.c_float(scene.restitution)))
-        assert list(self.ids) == list(range(1, n + 1))
+        base = 1 + (len(scene.statics) if first else 0)
+        assert list(self.ids) == list(range(base, base + n))
+        if not first:
+            self._add_statics(scene)
+
+    def _add_statics(self, scene):
+        ref = self.ref
         for h, pos, quat, is_sphere in scene.statics:
             sid = C.c_uint32()
             p = P(np.ascontiguousarray(pos, np.float32))
@@ -368,12 +377,13 @@ def sphere_cases():
     return scenes.Scene(np.array(half), np.array(pos), np.array(quat, np.float32), sphere=np.array(sphere))
 
 
-def statics_scene(seed=None):
+def statics_scene(seed=None, first=False):
     """Dynamic spheres and boxes dropped onto static bodies: a tilted
     static slab on four legs' worth of height (half extents 3 x 0.25 x 3 at
     y = 2, turned 6 degrees about z) and a static sphere of radius 1.5
     resting in the floor's plane beside it; 48 bodies above the slab and 16
-    above the sphere, alternately spheres and boxes."""
+    above the sphere, alternately spheres and boxes. first: the statics are
+    created before the dynamic bodies (lower ids: the other Gu order)."""
     from fyzz_amd import scenes
     seed = scenes.SEED_BASE + 62 if seed is None else seed
     a = np.arange(48)
@@ -392,4 +402,5 @@ def statics_scene(seed=None):
     slab_q = scenes.normalize_quat32(np.array([[0.0, 0.0, np.sin(t / 2), np.cos(t / 2)]]))[0]
     sc.statics = [((3.0, 0.25, 3.0), (0.0, 2.0, 0.0), slab_q, False),
                   (1.5, (7.0, 1.5, 0.0), np.float32([0, 0, 0, 1]), True)]
+    sc.statics_first = first  # created before the dynamic bodies (lower ids) instead of after
     return sc

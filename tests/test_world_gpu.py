@@ -19,12 +19,21 @@ from oracle_bind import OracleWorld
 pytestmark = pytest.mark.gpu
 
 
+def _add_statics(w, sc):
+    for h, pos, quat, is_sphere in getattr(sc, "statics", []):
+        sid = w.add_static_sphere(float(h), pos) if is_sphere else w.add_static_box(h, pos)
+        w.set_body_pose(sid, pos, quat)
+        w.set_body_material(sid, sc.friction, sc.friction, sc.restitution)
+
+
 def make_world(ctx, sc):
     from fyzz_amd.world import World
     w = World(ctx, sc.n + 1)
     if sc.floor:
         fid = w.add_static_floor(sc.floor_height)
         w.set_body_material(fid, sc.friction, sc.friction, sc.restitution)
+    if getattr(sc, "statics_first", False):
+        _add_statics(w, sc)
     for i in range(sc.n):
         bid = (w.add_dynamic_sphere(float(sc.half[i, 0]), sc.density) if sc.sphere[i]
                else w.add_dynamic_box(sc.half[i], sc.density))
@@ -32,10 +41,8 @@ def make_world(ctx, sc):
         w.set_body_velocity(bid, sc.vel[i])
         w.set_body_angular_velocity(bid, sc.ang[i])
         w.set_body_material(bid, sc.friction, sc.friction, sc.restitution)
-    for h, pos, quat, is_sphere in getattr(sc, "statics", []):
-        sid = w.add_static_sphere(float(h), pos) if is_sphere else w.add_static_box(h, pos)
-        w.set_body_pose(sid, pos, quat)
-        w.set_body_material(sid, sc.friction, sc.friction, sc.restitution)
+    if not getattr(sc, "statics_first", False):
+        _add_statics(w, sc)
     return w
 
 
@@ -261,10 +268,15 @@ def _native_run(ctx, sc, steps, events=False):
                 np.testing.assert_array_equal(pt["normal_impulse"], rpimp, err_msg=f"step {s} point impulses")
                 kinds += np.bincount(rev[:, 2].astype(np.int64), minlength=3)
             mg, _, _ = w.managers()
-            b0 = mg["body0"].astype(np.int64) + 1
+            # ids: the floor 0, then the statics and the dynamic bodies in
+            # creation order; a manager names static k as -2 - k, the floor -1
+            ns = len(getattr(sc, "statics", []))
+            first_static = getattr(sc, "statics_first", False)
+            dyn0 = 1 + (ns if first_static else 0)
+            st0 = 1 if first_static else 1 + sc.n
+            b0 = mg["body0"].astype(np.int64) + dyn0
             b1 = mg["body1"].astype(np.int64)
-            # -1 the floor (id 0), -2 - k static body k (created after the n dynamic ones)
-            b1 = np.where(b1 == -1, 0, np.where(b1 < -1, sc.n + 1 + (-2 - b1), b1 + 1))
+            b1 = np.where(b1 == -1, 0, np.where(b1 < -1, st0 + (-2 - b1), b1 + dyn0))
             got = np.stack([np.minimum(b0, b1), np.maximum(b0, b1)], 1)
             want = np.asarray(st["keys"], np.int64).reshape(-1, 2)
             np.testing.assert_array_equal(got, want, err_msg=f"step {s}: solver order")
@@ -284,7 +296,8 @@ def _native_run(ctx, sc, steps, events=False):
     return same, most
 
 
-@pytest.mark.parametrize("case", ["grid_1000", "hub", "sleep_wake", "pile_2000", "spheres_and_boxes", "statics"])
+@pytest.mark.parametrize("case", ["grid_1000", "hub", "sleep_wake", "pile_2000", "spheres_and_boxes", "statics",
+                                  "statics_first"])
 def test_native_order_world_matches_live_reference(ctx, torch_cuda, case):
     """World::step in the native solver order (the host island authority
     feeding the device solver list) against the live NativeCpu scene,
@@ -302,9 +315,11 @@ def test_native_order_world_matches_live_reference(ctx, torch_cuda, case):
                  "sleep_wake": (ref_live.wake_scene(), 600),
                  "pile_2000": (scenes.pile_scene(2000, footprint=15.0, height=160.0), 300),
                  "spheres_and_boxes": (ref_live.mixed_scene(), 240),
-                 "statics": (ref_live.statics_scene(), 300)}[case]
+                 "statics": (ref_live.statics_scene(), 300),
+                 "statics_first": (ref_live.statics_scene(first=True), 200)}[case]
     same, most = _native_run(ctx, sc, steps,
-                             events=case in ("grid_1000", "sleep_wake", "spheres_and_boxes", "statics"))
+                             events=case in ("grid_1000", "sleep_wake", "spheres_and_boxes", "statics",
+                                             "statics_first"))
     print(f"{case}: steps bit-identical to NativeCpu {same}/{steps}, most managers {most}")
     assert most > 40
 
