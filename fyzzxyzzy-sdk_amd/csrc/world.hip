@@ -205,6 +205,32 @@ __global__ void k_record_dy(const u32* d_nm, u32 cap_m, const fyzz_dy_manager* m
     }
 }
 
+// The same records appended at *cursor of caller arenas, without a host
+// round trip: nothing is copied when they do not fit; k_append_advance then
+// poisons the cursor (all ones) so the caller learns it once, afterwards.
+__global__ void k_append_dy(const u32* d_nm, u32 cap_m, const fyzz_dy_manager* managers, const fyzz_dy_patch* hist,
+                            fyzz_dy_manager* out_m, fyzz_dy_patch* out_h, u64 capacity, const u64* cursor) {
+    const u32 n = min(*d_nm, cap_m);
+    const u64 c = *cursor;
+    if (c == ~0ull || c + n > capacity) return;
+    for (u32 m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+        fyzz_dy_manager M = managers[m];
+        if (M.history != kNone) {
+            out_h[c + m] = hist[M.history];
+            M.history = m;  // index within this step's records
+        } else {
+            out_h[c + m] = fyzz_dy_patch{};
+        }
+        out_m[c + m] = M;
+    }
+}
+__global__ void k_append_advance(const u32* d_nm, u32 cap_m, u64 capacity, u64* cursor, u32* count) {
+    const u32 n = min(*d_nm, cap_m);
+    const u64 c = *cursor;
+    *count = n;
+    *cursor = (c == ~0ull || c + n > capacity) ? ~0ull : c + n;
+}
+
 // friction patches solved this step, keyed for the next step's lookup
 __global__ void k_scatter_patches(const u32* d_nm, u32 cap_m, const fyzz_dy_patch* patches, const u32* mgr_slot,
                                   fyzz_dy_patch* hist_next, u32* stamp_next, const u32* d_step) {
@@ -2433,6 +2459,24 @@ int fyzz_hip_world_record_dy_inputs(fyzz_hip_world* w, fyzz_dy_manager* managers
         FYZZ_REQUIRE(capacity >= nm, kOverflow, "world: record capacity below the manager count");
         hipLaunchKernelGGL(world::k_record_dy, dim3(std::min<u32>(grid_for(nm, 256), 2048u)), dim3(256), 0, st,
                            w->d_nm.p, w->mgr_cap, w->managers.p, w->hist[w->hcur ^ 1].p, managers, history);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int fyzz_hip_world_append_dy_inputs(fyzz_hip_world* w, fyzz_dy_manager* managers, fyzz_dy_patch* history,
+                                    uint64_t capacity, uint64_t* cursor, uint32_t* count) {
+    if (!w) return kInvalidArgument;
+    return guarded(w->ctx, [&] {
+        FYZZ_REQUIRE(w->uploaded, kLogicError, "world: no step taken");
+        FYZZ_REQUIRE(cursor && count && (capacity == 0 || (managers && history)), kInvalidArgument,
+                     "world: null arena, cursor or count");
+        hipStream_t st = w->ctx->stream;
+        const u32 grid = std::min<u32>(grid_for(std::max<u32>(w->mgr_cap, 1u), 256), 2048u);
+        if (w->n)
+            hipLaunchKernelGGL(world::k_append_dy, dim3(grid), dim3(256), 0, st, w->d_nm.p, w->mgr_cap,
+                               w->managers.p, w->hist[w->hcur ^ 1].p, managers, history, capacity, cursor);
+        hipLaunchKernelGGL(world::k_append_advance, dim3(1), dim3(1), 0, st, w->n ? w->d_nm.p : w->d_total.p,
+                           w->n ? w->mgr_cap : 0u, capacity, cursor, count);
         FYZZ_HIP_CHECK(hipGetLastError());
     });
 }

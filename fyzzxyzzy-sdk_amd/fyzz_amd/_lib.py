@@ -88,6 +88,7 @@ SIGNATURES = {
     "fyzz_hip_world_device_views": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                               C.POINTER(vp)]),
     "fyzz_hip_world_set_solver_order": (C.c_int, [vp, C.c_int]),
+    "fyzz_hip_world_append_dy_inputs": (C.c_int, [vp, vp, vp, C.c_uint64, vp, vp]),
     "fyzz_hip_world_add_dynamic_sphere": (C.c_int, [vp, C.c_float, C.c_float, u32p]),
     "fyzz_hip_world_add_static_box": (C.c_int, [vp, vp, vp, u32p]),
     "fyzz_hip_world_add_static_sphere": (C.c_int, [vp, C.c_float, vp, u32p]),

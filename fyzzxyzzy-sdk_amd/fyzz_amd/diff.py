@@ -45,11 +45,27 @@ class Rollout:
 
     def forward(self, v0, w0, mu_body):
         """v0, w0: (nb, 3) host or device arrays; mu_body: (nb,) friction.
-        Returns the final positions (nb, 3) as a device float32 tensor."""
+        Returns the final positions (nb, 3) as a device float32 tensor.
+        Each step's contact set is appended on the device to arenas sized
+        for 4 managers per body and step; the counts are read once, after
+        the rollout (a rollout that needs more runs again with exact
+        arenas)."""
+        return self._forward(v0, w0, mu_body, None)
+
+    def _forward(self, v0, w0, mu_body, capacity):
         import torch
+        from .dy import DY_MANAGER, DY_PATCH
+        v0_in, w0_in, mu_in = v0, w0, mu_body
         v0 = np.ascontiguousarray(torch.as_tensor(v0).detach().cpu().numpy(), np.float32)
         w0 = np.ascontiguousarray(torch.as_tensor(w0).detach().cpu().numpy(), np.float32)
         mu = np.ascontiguousarray(torch.as_tensor(mu_body).detach().cpu().numpy(), np.float32)
+        if capacity is None:
+            capacity = self.steps * max(4 * self.nb, 1024)
+        mg_arena = torch.empty((max(capacity, 1), DY_MANAGER.itemsize), dtype=torch.uint8, device="cuda")
+        hs_arena = torch.empty((max(capacity, 1), DY_PATCH.itemsize), dtype=torch.uint8, device="cuda")
+        cursor = torch.zeros(1, dtype=torch.int64, device="cuda")
+        counts = torch.zeros(self.steps, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()  # the zeroed cursor and counts before the world's stream uses them
         w = World(self.ctx, self.nb + 1)
         fid = w.add_static_floor(0.0)
         w.set_body_material(fid, self.floor_friction, self.floor_friction, 0.0)
@@ -62,12 +78,17 @@ class Rollout:
             if s % self.every == 0:
                 self.checkpoints[s] = w.copy_bodies().clone()
             w.step(self.dt)
-            self.records.append(w.record_dy_inputs())
+            w.append_dy_inputs(mg_arena, hs_arena, capacity, cursor, counts[s:s + 1])
         final = w.copy_bodies().clone()
-        st = w.stats()
+        st = w.stats()  # synchronizes the world's stream
         w.close()
         if st["flags"]:
             raise RuntimeError(f"rollout status flags {st['flags']:#x}")
+        n = counts.cpu().numpy().astype(np.int64)
+        if int(cursor.item()) == -1:  # the arenas were too small: once more with the exact size
+            return self._forward(v0_in, w0_in, mu_in, int(n.sum()))
+        off = np.concatenate([[0], np.cumsum(n)])
+        self.records = [(mg_arena[off[k]:off[k + 1]], hs_arena[off[k]:off[k + 1]]) for k in range(self.steps)]
         return final[:, :12].contiguous().view(torch.float32).reshape(self.nb, 3)
 
     def close(self):

@@ -141,6 +141,13 @@ class World:
                    C.byref(cnt))
         return mg[:n], hs[:n]
 
+    def append_dy_inputs(self, managers, history, capacity, cursor, count):
+        """Appends the last step's Dy inputs to device arenas (uint8 tensors of
+        DY_MANAGER / DY_PATCH records) at cursor (int64 device tensor),
+        writing the step's count (int32 device tensor); no host round trip."""
+        self._call("fyzz_hip_world_append_dy_inputs", vp(managers.data_ptr()), vp(history.data_ptr()),
+                   C.c_uint64(capacity), vp(cursor.data_ptr()), vp(count.data_ptr()))
+
     def copy_bodies(self):
         """Device copy of the body records, (n, 144) uint8 tensor."""
         import torch

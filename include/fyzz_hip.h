@@ -518,6 +518,15 @@ int fyzz_hip_world_contact_events(fyzz_hip_world* world, fyzz_contact_event* eve
  * Synchronizes to read the count. */
 int fyzz_hip_world_record_dy_inputs(fyzz_hip_world* world, fyzz_dy_manager* managers, fyzz_dy_patch* history,
                                     uint32_t capacity, uint32_t* n_managers);
+/* The same records appended to DEVICE arenas of `capacity` records at
+ * *cursor (DEVICE u64), with no host round trip (no counterpart in the
+ * reference): *count (DEVICE u32) receives the step's manager count and
+ * the cursor advances by it; when the records do not fit nothing is
+ * copied and the cursor becomes all ones, so later appends copy nothing
+ * either (a caller reads the cursor once, after its rollout). Each
+ * record's history index is its index within the step's records. */
+int fyzz_hip_world_append_dy_inputs(fyzz_hip_world* world, fyzz_dy_manager* managers, fyzz_dy_patch* history,
+                                    uint64_t capacity, uint64_t* cursor, uint32_t* count);
 /* Synchronizes; the last step's broadphase reports as sorted (lo, hi) box
  * SLOT pairs into a HOST array of 2*capacity u32 (*written = the total
  * count): which 0 the live pairs (Scene::pairs), 1 found, 2 lost
