@@ -1105,8 +1105,9 @@ struct fyzz_hip_world {
     bool native_order = false;
     fyzz_hip_islands* order_auth = nullptr;
     std::vector<u32> id_of_slot;
-    std::vector<u64> native_prev_keys;   // last step's broadphase pairs (slot keys, ascending)
     std::vector<u64> native_prev_touch;  // last step's touching pairs (id keys, ascending)
+    DevBuf<u64> native_keys, native_found, native_order_keys;  // compacted pairs; the order as slot keys
+    DevBuf<u32> native_planes, native_counts, native_n;
     std::vector<uint8_t> native_sleeping;
     DevBuf<u32> native_list;
     DevBuf<uint8_t> flag_bytes;
@@ -1356,6 +1357,12 @@ void upload(fyzz_hip_world* w) {
                      "world: cannot create the island authority");
         w->native_list.reserve(w->mgr_cap);
         w->flag_bytes.reserve(nn);
+        w->native_keys.reserve(w->pair_cap);
+        w->native_found.reserve(w->pair_cap);
+        w->native_order_keys.reserve(w->mgr_cap);
+        w->native_planes.reserve(nn);
+        w->native_counts.reserve(4);
+        w->native_n.reserve(1);
     }
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
     w->uploaded = true;
@@ -1494,57 +1501,117 @@ void islands(fyzz_hip_world* w, const sap::Lists& L, int c) {
     FYZZ_HIP_CHECK(hipGetLastError());
 }
 
+// Compactions for the native order: this step's touching box/box pairs,
+// its found pairs (no previous index) and the touching plane managers.
+struct NatTouchPred {
+    const u32* counts_bb;
+    __device__ bool operator()(u32 i) const { return counts_bb[i] != 0u; }
+};
+struct NatFoundPred {
+    const u32* src;
+    __device__ bool operator()(u32 i) const { return src[i] == ::fyzz_hip::world::kNone; }
+};
+struct NatKeyWrite {
+    const u64* cur;
+    u64* out;
+    __device__ void operator()(u32 i, u32 o) const { out[o] = cur[i]; }
+};
+struct NatPlanePred {
+    const u32* counts_pl;
+    __device__ bool operator()(u32 b) const { return counts_pl[b] != 0u; }
+};
+struct NatSlotWrite {
+    u32* out;
+    __device__ void operator()(u32 b, u32 o) const { out[o] = b; }
+};
+// The solver list's candidate index of each ordered manager: a plane
+// manager (key hi bit set, box slot below) at pair count + slot, a box/box
+// pair found by binary search in the current pair list. A key that names
+// no touching manager raises bit 0 of *error.
+__global__ void k_native_index(const u64* order, u32 count, const u64* cur, const u32* d_cur, u32 cap,
+                               const u32* counts_bb, const u32* counts_pl, u32 n, u32* out, u32* error) {
+    const u32 nbb = min(*d_cur, cap);
+    for (u32 o = blockIdx.x * blockDim.x + threadIdx.x; o < count; o += gridDim.x * blockDim.x) {
+        const u64 k = order[o];
+        u32 idx;
+        bool ok;
+        if (k >> 63) {
+            const u32 b = static_cast<u32>(k & 0xffffffffu);
+            ok = b < n && counts_pl && counts_pl[b] != 0u;
+            idx = nbb + (b < n ? b : 0u);
+        } else {
+            const u32 j = ::fyzz_hip::world::lower_bound64(cur, nbb, k);
+            ok = j < nbb && cur[j] == k && counts_bb[j] != 0u;
+            idx = j < nbb ? j : 0u;
+        }
+        out[o] = idx;
+        if (!ok) atomicOr(error, 1u);
+    }
+}
+
 // The native scene's solver order for this step (Scene::step's island
-// bookkeeping, scene.cpp:10747-10793 and 7174-7260): the step's broadphase
-// and touch changes and the last solve's ready flags go to the host island
-// authority, which returns the descriptor order and its sleep flags.
-// Synchronizes. Returns the order as candidate indices (box/box pair slot,
-// or pair count + box for a plane manager) in w->native_list.
+// bookkeeping, scene.cpp:10747-10793 and 7174-7260): the device compacts
+// the step's touching pairs, found pairs and touching plane managers; the
+// host turns them, the broadphase's lost pairs and the last solve's ready
+// flags into the authority's change lists, runs it, and hands the order
+// back as pair keys, which the device maps to solver-list candidates in
+// w->native_list. Synchronizes. Returns the order's length.
 u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
     using namespace fyzz_hip::world;
     hipStream_t st = w->ctx->stream;
     const u32 n = w->n, cap = w->pair_cap;
     const u32 ids = static_cast<u32>(w->slot_of_id.size());
+    u32* cnt = w->native_counts.p;  // [0] touching pairs, [1] found pairs, [2] touching planes, [3] errors
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt, 4u, 0u);
     hipLaunchKernelGGL(k_flag_bytes, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, n, w->bodies.p,
                        w->flag_bytes.p);
-    u32 nbb = 0;
-    FYZZ_HIP_CHECK(hipMemcpyAsync(&nbb, L.d_cur, sizeof nbb, hipMemcpyDeviceToHost, st));
-    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
-    nbb = std::min(nbb, cap);
-    std::vector<u64> keys(nbb);
-    std::vector<u32> cbb(nbb), cpl(w->has_floor ? n : 0);
-    std::vector<uint8_t> flags(n);
-    if (nbb) {
-        FYZZ_HIP_CHECK(hipMemcpyAsync(keys.data(), L.cur, sizeof(u64) * nbb, hipMemcpyDeviceToHost, st));
-        FYZZ_HIP_CHECK(hipMemcpyAsync(cbb.data(), w->counts_bb.p, sizeof(u32) * nbb, hipMemcpyDeviceToHost, st));
+    sel::select(st, cap, L.d_cur, NatTouchPred{w->counts_bb.p}, NatKeyWrite{L.cur, w->native_keys.p}, w->tiles.p,
+                cnt + 0);
+    sel::select(st, cap, L.d_cur, NatFoundPred{w->src.p}, NatKeyWrite{L.cur, w->native_found.p}, w->tiles.p,
+                cnt + 1);
+    if (w->has_floor) {
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->native_n.p, 1u, n);
+        sel::select(st, n, w->native_n.p, NatPlanePred{w->counts_pl.p}, NatSlotWrite{w->native_planes.p},
+                    w->tiles.p, cnt + 2);
     }
-    if (w->has_floor)
-        FYZZ_HIP_CHECK(hipMemcpyAsync(cpl.data(), w->counts_pl.p, sizeof(u32) * n, hipMemcpyDeviceToHost, st));
+    u32 c[4] = {0, 0, 0, 0};
+    u32 nlost = 0;
+    FYZZ_HIP_CHECK(hipMemcpyAsync(c, cnt, sizeof c, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipMemcpyAsync(&nlost, L.d_lost, sizeof nlost, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    c[0] = std::min(c[0], cap);
+    c[1] = std::min(c[1], cap);
+    c[2] = std::min(c[2], n);
+    nlost = std::min(nlost, cap);
+    std::vector<u64> tkeys(c[0]), fkeys(c[1]), lkeys(nlost);
+    std::vector<u32> planes(c[2]);
+    std::vector<uint8_t> flags(n);
+    if (c[0]) FYZZ_HIP_CHECK(hipMemcpyAsync(tkeys.data(), w->native_keys.p, 8ull * c[0], hipMemcpyDeviceToHost, st));
+    if (c[1]) FYZZ_HIP_CHECK(hipMemcpyAsync(fkeys.data(), w->native_found.p, 8ull * c[1], hipMemcpyDeviceToHost, st));
+    if (nlost) FYZZ_HIP_CHECK(hipMemcpyAsync(lkeys.data(), L.lost, 8ull * nlost, hipMemcpyDeviceToHost, st));
+    if (c[2]) FYZZ_HIP_CHECK(hipMemcpyAsync(planes.data(), w->native_planes.p, 4ull * c[2], hipMemcpyDeviceToHost, st));
     FYZZ_HIP_CHECK(hipMemcpyAsync(flags.data(), w->flag_bytes.p, n, hipMemcpyDeviceToHost, st));
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
 
     const auto& id = w->id_of_slot;
     auto id_key = [](u32 a, u32 b) { return a < b ? (u64(a) << 32) | b : (u64(b) << 32) | a; };
     auto slot_pair = [&](u64 k) { return id_key(id[static_cast<u32>(k >> 32)], id[static_cast<u32>(k)]); };
-    // broadphase changes: the slot keys are ascending, as are their id keys
+    // broadphase changes (the plane pairs every box from the first step on)
     std::vector<u64> found, lost;
-    const auto& prev = w->native_prev_keys;
-    std::size_t a = 0, b = 0;
-    while (a < prev.size() || b < keys.size()) {
-        if (b == keys.size() || (a < prev.size() && prev[a] < keys[b])) lost.push_back(slot_pair(prev[a++]));
-        else if (a == prev.size() || keys[b] < prev[a]) found.push_back(slot_pair(keys[b++]));
-        else ++a, ++b;
-    }
-    if (w->step_index == 1 && w->has_floor)  // the plane pairs every box from the first step on
+    found.reserve(fkeys.size() + (w->step_index == 1 && w->has_floor ? n : 0));
+    for (u64 k : fkeys) found.push_back(slot_pair(k));
+    if (w->step_index == 1 && w->has_floor)
         for (u32 k = 0; k < n; ++k) found.push_back(id_key(w->floor_id, id[k]));
+    for (u64 k : lkeys) lost.push_back(slot_pair(k));
     std::sort(found.begin(), found.end());
-    std::sort(lost.begin(), lost.end());  // static slots follow the dynamic ones, not id order
+    std::sort(lost.begin(), lost.end());
+    // this step's touching pairs in id order: already ascending when ids
+    // follow slots and the floor precedes every box
     std::vector<u64> touch;
-    for (u32 t = 0; t < nbb; ++t)
-        if (cbb[t]) touch.push_back(slot_pair(keys[t]));
-    for (u32 k = 0; k < cpl.size(); ++k)
-        if (cpl[k]) touch.push_back(id_key(w->floor_id, id[k]));
-    std::sort(touch.begin(), touch.end());
+    touch.reserve(planes.size() + tkeys.size());
+    for (u32 b : planes) touch.push_back(id_key(w->floor_id, id[b]));
+    for (u64 k : tkeys) touch.push_back(slot_pair(k));
+    if (!std::is_sorted(touch.begin(), touch.end())) std::sort(touch.begin(), touch.end());
     std::vector<u64> tfound, tlost;
     std::set_difference(touch.begin(), touch.end(), w->native_prev_touch.begin(), w->native_prev_touch.end(),
                         std::back_inserter(tfound));
@@ -1571,28 +1638,25 @@ u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
     const int rc = fyzz_hip_islands_step(w->order_auth, &ev, &out);
     FYZZ_REQUIRE(rc == 0, kLogicError,
                  std::string("world: native solver order: ") + fyzz_hip_islands_last_error(w->order_auth));
-    // candidate index of each ordered manager
-    std::vector<u32> list(out.manager_count);
+    // the order as slot keys (a plane manager: high bit | box slot) for the device's lookup
+    std::vector<u64> okeys(out.manager_count);
     for (u32 m = 0; m < out.manager_count; ++m) {
         const u32 p = order[2 * m], q = order[2 * m + 1];
-        u32 idx;
         if (w->has_floor && (p == w->floor_id || q == w->floor_id)) {
-            idx = nbb + static_cast<u32>(w->slot_of_id[p == w->floor_id ? q : p]);
+            okeys[m] = (1ull << 63) | w->slot(p == w->floor_id ? q : p);
         } else {
             u32 sp = w->slot(p), sq = w->slot(q);
             if (sp > sq) std::swap(sp, sq);
-            const u64 k = (u64(sp) << 32) | sq;
-            const auto it = std::lower_bound(keys.begin(), keys.end(), k);
-            FYZZ_REQUIRE(it != keys.end() && *it == k, kLogicError, "world: native order names an unknown pair");
-            idx = static_cast<u32>(it - keys.begin());
+            okeys[m] = (u64(sp) << 32) | sq;
         }
-        const bool touching = idx < nbb ? cbb[idx] != 0u : cpl[idx - nbb] != 0u;
-        FYZZ_REQUIRE(touching, kLogicError, "world: native order names a manager without contacts");
-        list[m] = idx;
     }
-    if (!list.empty())
-        FYZZ_HIP_CHECK(hipMemcpy(w->native_list.p, list.data(), sizeof(u32) * list.size(), hipMemcpyHostToDevice));
-    w->native_prev_keys.swap(keys);
+    if (!okeys.empty()) {
+        FYZZ_HIP_CHECK(hipMemcpy(w->native_order_keys.p, okeys.data(), 8ull * okeys.size(), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_native_index, dim3(std::min<u32>(grid_for(out.manager_count, 256), 2048u)), dim3(256), 0,
+                           st, w->native_order_keys.p, out.manager_count, L.cur, L.d_cur, cap, w->counts_bb.p,
+                           w->has_floor ? w->counts_pl.p : nullptr, n, w->native_list.p, cnt + 3);
+        FYZZ_HIP_CHECK(hipGetLastError());
+    }
     w->native_prev_touch.swap(touch);
     return out.manager_count;
 }
@@ -1606,8 +1670,11 @@ void native_check_sleep(fyzz_hip_world* w) {
     hipLaunchKernelGGL(k_flag_bytes, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, n, w->bodies.p,
                        w->flag_bytes.p);
     std::vector<uint8_t> flags(n);
+    u32 error = 0;
     FYZZ_HIP_CHECK(hipMemcpyAsync(flags.data(), w->flag_bytes.p, n, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipMemcpyAsync(&error, w->native_counts.p + 3, sizeof error, hipMemcpyDeviceToHost, st));
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
+    FYZZ_REQUIRE(!error, kLogicError, "world: native order names a manager without contacts");
     for (u32 k = 0; k < n; ++k)
         FYZZ_REQUIRE(((flags[k] & 1u) != 0) == (w->native_sleeping[w->id_of_slot[k]] != 0), kLogicError,
                      "world: native order: the island pass and the island authority disagree on sleep");
