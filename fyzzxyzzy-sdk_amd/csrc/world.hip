@@ -1105,9 +1105,9 @@ struct fyzz_hip_world {
     bool native_order = false;
     fyzz_hip_islands* order_auth = nullptr;
     std::vector<u32> id_of_slot;
-    std::vector<u64> native_prev_touch;  // last step's touching pairs (id keys, ascending)
-    DevBuf<u64> native_keys, native_found, native_order_keys;  // compacted pairs; the order as slot keys
-    DevBuf<u32> native_planes, native_counts, native_n;
+    DevBuf<u64> native_keys, native_keys2, native_keys3, native_found, native_order_keys;  // compacted pairs;
+                                                                                         // the order as slot keys
+    DevBuf<u32> native_planes, native_planes2, native_counts, native_n;
     std::vector<uint8_t> native_sleeping;
     DevBuf<u32> native_list;
     DevBuf<uint8_t> flag_bytes;
@@ -1358,10 +1358,13 @@ void upload(fyzz_hip_world* w) {
         w->native_list.reserve(w->mgr_cap);
         w->flag_bytes.reserve(nn);
         w->native_keys.reserve(w->pair_cap);
+        w->native_keys2.reserve(w->pair_cap);
+        w->native_keys3.reserve(w->pair_cap);
         w->native_found.reserve(w->pair_cap);
         w->native_order_keys.reserve(w->mgr_cap);
         w->native_planes.reserve(nn);
-        w->native_counts.reserve(4);
+        w->native_planes2.reserve(nn);
+        w->native_counts.reserve(8);
         w->native_n.reserve(1);
     }
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
@@ -1501,12 +1504,8 @@ void islands(fyzz_hip_world* w, const sap::Lists& L, int c) {
     FYZZ_HIP_CHECK(hipGetLastError());
 }
 
-// Compactions for the native order: this step's touching box/box pairs,
-// its found pairs (no previous index) and the touching plane managers.
-struct NatTouchPred {
-    const u32* counts_bb;
-    __device__ bool operator()(u32 i) const { return counts_bb[i] != 0u; }
-};
+// Compactions for the native order: this step's found pairs (no previous
+// index), touch changes of box/box pairs and of plane managers.
 struct NatFoundPred {
     const u32* src;
     __device__ bool operator()(u32 i) const { return src[i] == ::fyzz_hip::world::kNone; }
@@ -1516,9 +1515,40 @@ struct NatKeyWrite {
     u64* out;
     __device__ void operator()(u32 i, u32 o) const { out[o] = cur[i]; }
 };
+// touch changes: a current pair that touches now and did not before
+// (`want` 1), or that touched before and no longer does (`want` 0)
+struct NatTouchChangePred {
+    const u32* counts_bb;
+    const u32* src;
+    const u32* touch_prev;
+    u32 want;
+    __device__ bool operator()(u32 i) const {
+        const bool now = counts_bb[i] != 0u;
+        const bool before = src[i] != ::fyzz_hip::world::kNone && touch_prev[src[i]] != 0u;
+        return want ? (now && !before) : (!now && before);
+    }
+};
+// a pair the broadphase lost that touched on the last step
+struct NatLostTouchPred {
+    const u64* lost;
+    const u64* prev;
+    const u32* d_prev;
+    u32 cap;
+    const u32* touch_prev;
+    __device__ bool operator()(u32 j) const {
+        const u32 np = min(*d_prev, cap);
+        const u32 o = ::fyzz_hip::world::lower_bound64(prev, np, lost[j]);
+        return o < np && prev[o] == lost[j] && touch_prev[o] != 0u;
+    }
+};
 struct NatPlanePred {
     const u32* counts_pl;
-    __device__ bool operator()(u32 b) const { return counts_pl[b] != 0u; }
+    const u32* touch_pl;
+    u32 want;  // 1: began touching, 0: stopped
+    __device__ bool operator()(u32 b) const {
+        const bool now = counts_pl[b] != 0u, before = touch_pl[b] != 0u;
+        return want ? (now && !before) : (!now && before);
+    }
 };
 struct NatSlotWrite {
     u32* out;
@@ -1551,46 +1581,64 @@ __global__ void k_native_index(const u64* order, u32 count, const u64* cur, cons
 
 // The native scene's solver order for this step (Scene::step's island
 // bookkeeping, scene.cpp:10747-10793 and 7174-7260): the device compacts
-// the step's touching pairs, found pairs and touching plane managers; the
-// host turns them, the broadphase's lost pairs and the last solve's ready
-// flags into the authority's change lists, runs it, and hands the order
-// back as pair keys, which the device maps to solver-list candidates in
-// w->native_list. Synchronizes. Returns the order's length.
-u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
+// the step's found pairs and its touch changes (c_prev: the parity of the
+// last step's touching flags); the host turns them, the broadphase's lost
+// pairs and the last solve's ready flags into the authority's change
+// lists, runs it, and hands the order back as pair keys, which the device
+// maps to solver-list candidates in w->native_list. Synchronizes. Returns
+// the order's length.
+u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L, int c_prev) {
     using namespace fyzz_hip::world;
     hipStream_t st = w->ctx->stream;
     const u32 n = w->n, cap = w->pair_cap;
     const u32 ids = static_cast<u32>(w->slot_of_id.size());
-    u32* cnt = w->native_counts.p;  // [0] touching pairs, [1] found pairs, [2] touching planes, [3] errors
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt, 4u, 0u);
+    // [0] touches found, [1] pairs found, [2] plane touches found, [3] errors,
+    // [4] touches lost (pairs kept), [5] touches lost with their pair, [6] plane touches lost
+    u32* cnt = w->native_counts.p;
+    const u32* tprev = w->touch_bb[c_prev].p;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, cnt, 8u, 0u);
     hipLaunchKernelGGL(k_flag_bytes, dim3(std::min<u32>(grid_for(n, 256), 1024u)), dim3(256), 0, st, n, w->bodies.p,
                        w->flag_bytes.p);
-    sel::select(st, cap, L.d_cur, NatTouchPred{w->counts_bb.p}, NatKeyWrite{L.cur, w->native_keys.p}, w->tiles.p,
-                cnt + 0);
+    sel::select(st, cap, L.d_cur, NatTouchChangePred{w->counts_bb.p, w->src.p, tprev, 1u},
+                NatKeyWrite{L.cur, w->native_keys.p}, w->tiles.p, cnt + 0);
     sel::select(st, cap, L.d_cur, NatFoundPred{w->src.p}, NatKeyWrite{L.cur, w->native_found.p}, w->tiles.p,
                 cnt + 1);
+    sel::select(st, cap, L.d_cur, NatTouchChangePred{w->counts_bb.p, w->src.p, tprev, 0u},
+                NatKeyWrite{L.cur, w->native_keys2.p}, w->tiles.p, cnt + 4);
+    sel::select(st, cap, L.d_lost, NatLostTouchPred{L.lost, L.prev, L.d_prev, cap, tprev},
+                NatKeyWrite{L.lost, w->native_keys3.p}, w->tiles.p, cnt + 5);
     if (w->has_floor) {
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, w->native_n.p, 1u, n);
-        sel::select(st, n, w->native_n.p, NatPlanePred{w->counts_pl.p}, NatSlotWrite{w->native_planes.p},
-                    w->tiles.p, cnt + 2);
+        sel::select(st, n, w->native_n.p, NatPlanePred{w->counts_pl.p, w->touch_pl.p, 1u},
+                    NatSlotWrite{w->native_planes.p}, w->tiles.p, cnt + 2);
+        sel::select(st, n, w->native_n.p, NatPlanePred{w->counts_pl.p, w->touch_pl.p, 0u},
+                    NatSlotWrite{w->native_planes2.p}, w->tiles.p, cnt + 6);
     }
-    u32 c[4] = {0, 0, 0, 0};
-    u32 nlost = 0;
+    u32 c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    u32 nlost = 0, nbb = 0;
     FYZZ_HIP_CHECK(hipMemcpyAsync(c, cnt, sizeof c, hipMemcpyDeviceToHost, st));
     FYZZ_HIP_CHECK(hipMemcpyAsync(&nlost, L.d_lost, sizeof nlost, hipMemcpyDeviceToHost, st));
+    FYZZ_HIP_CHECK(hipMemcpyAsync(&nbb, L.d_cur, sizeof nbb, hipMemcpyDeviceToHost, st));
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
-    c[0] = std::min(c[0], cap);
-    c[1] = std::min(c[1], cap);
+    for (int k : {0, 1, 4, 5}) c[k] = std::min(c[k], cap);
     c[2] = std::min(c[2], n);
+    c[6] = std::min(c[6], n);
     nlost = std::min(nlost, cap);
-    std::vector<u64> tkeys(c[0]), fkeys(c[1]), lkeys(nlost);
-    std::vector<u32> planes(c[2]);
+    nbb = std::min(nbb, cap);
+    std::vector<u64> tkeys(c[0]), fkeys(c[1]), lkeys(nlost), tl_keep(c[4]), tl_gone(c[5]);
+    std::vector<u32> planes(c[2]), planes_lost(c[6]);
     std::vector<uint8_t> flags(n);
-    if (c[0]) FYZZ_HIP_CHECK(hipMemcpyAsync(tkeys.data(), w->native_keys.p, 8ull * c[0], hipMemcpyDeviceToHost, st));
-    if (c[1]) FYZZ_HIP_CHECK(hipMemcpyAsync(fkeys.data(), w->native_found.p, 8ull * c[1], hipMemcpyDeviceToHost, st));
-    if (nlost) FYZZ_HIP_CHECK(hipMemcpyAsync(lkeys.data(), L.lost, 8ull * nlost, hipMemcpyDeviceToHost, st));
-    if (c[2]) FYZZ_HIP_CHECK(hipMemcpyAsync(planes.data(), w->native_planes.p, 4ull * c[2], hipMemcpyDeviceToHost, st));
-    FYZZ_HIP_CHECK(hipMemcpyAsync(flags.data(), w->flag_bytes.p, n, hipMemcpyDeviceToHost, st));
+    auto fetch = [&](void* dst, const void* src, std::size_t bytes) {
+        if (bytes) FYZZ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    };
+    fetch(tkeys.data(), w->native_keys.p, 8ull * c[0]);
+    fetch(fkeys.data(), w->native_found.p, 8ull * c[1]);
+    fetch(lkeys.data(), L.lost, 8ull * nlost);
+    fetch(tl_keep.data(), w->native_keys2.p, 8ull * c[4]);
+    fetch(tl_gone.data(), w->native_keys3.p, 8ull * c[5]);
+    fetch(planes.data(), w->native_planes.p, 4ull * c[2]);
+    fetch(planes_lost.data(), w->native_planes2.p, 4ull * c[6]);
+    fetch(flags.data(), w->flag_bytes.p, n);
     FYZZ_HIP_CHECK(hipStreamSynchronize(st));
 
     const auto& id = w->id_of_slot;
@@ -1605,18 +1653,15 @@ u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
     for (u64 k : lkeys) lost.push_back(slot_pair(k));
     std::sort(found.begin(), found.end());
     std::sort(lost.begin(), lost.end());
-    // this step's touching pairs in id order: already ascending when ids
-    // follow slots and the floor precedes every box
-    std::vector<u64> touch;
-    touch.reserve(planes.size() + tkeys.size());
-    for (u32 b : planes) touch.push_back(id_key(w->floor_id, id[b]));
-    for (u64 k : tkeys) touch.push_back(slot_pair(k));
-    if (!std::is_sorted(touch.begin(), touch.end())) std::sort(touch.begin(), touch.end());
+    // touch changes in id order
     std::vector<u64> tfound, tlost;
-    std::set_difference(touch.begin(), touch.end(), w->native_prev_touch.begin(), w->native_prev_touch.end(),
-                        std::back_inserter(tfound));
-    std::set_difference(w->native_prev_touch.begin(), w->native_prev_touch.end(), touch.begin(), touch.end(),
-                        std::back_inserter(tlost));
+    for (u32 b : planes) tfound.push_back(id_key(w->floor_id, id[b]));
+    for (u64 k : tkeys) tfound.push_back(slot_pair(k));
+    for (u32 b : planes_lost) tlost.push_back(id_key(w->floor_id, id[b]));
+    for (u64 k : tl_keep) tlost.push_back(slot_pair(k));
+    for (u64 k : tl_gone) tlost.push_back(slot_pair(k));
+    std::sort(tfound.begin(), tfound.end());
+    std::sort(tlost.begin(), tlost.end());
     auto unpack = [](const std::vector<u64>& v) {
         std::vector<u32> out(2 * v.size());
         for (std::size_t i = 0; i < v.size(); ++i) {
@@ -1631,10 +1676,10 @@ u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
     fyzz_island_events ev{pf.data(), static_cast<u32>(found.size()), pl.data(), static_cast<u32>(lost.size()),
                           tf.data(), static_cast<u32>(tfound.size()), tl.data(), static_cast<u32>(tlost.size()),
                           ready.data()};
-    std::vector<u32> order(2 * std::max<std::size_t>(touch.size(), 1)), tasks(ids + 1);
+    const u32 most = nbb + (w->has_floor ? n : 0u);  // every touching manager at most
+    std::vector<u32> order(2 * std::max<u32>(most, 1u)), tasks(ids + 1);
     w->native_sleeping.assign(ids, 0);
-    fyzz_island_order out{order.data(), static_cast<u32>(touch.size()), 0, tasks.data(), ids + 1, 0,
-                          w->native_sleeping.data()};
+    fyzz_island_order out{order.data(), most, 0, tasks.data(), ids + 1, 0, w->native_sleeping.data()};
     const int rc = fyzz_hip_islands_step(w->order_auth, &ev, &out);
     FYZZ_REQUIRE(rc == 0, kLogicError,
                  std::string("world: native solver order: ") + fyzz_hip_islands_last_error(w->order_auth));
@@ -1657,7 +1702,6 @@ u32 native_order_list(fyzz_hip_world* w, const sap::Lists& L) {
                            w->has_floor ? w->counts_pl.p : nullptr, n, w->native_list.p, cnt + 3);
         FYZZ_HIP_CHECK(hipGetLastError());
     }
-    w->native_prev_touch.swap(touch);
     return out.manager_count;
 }
 
@@ -1706,7 +1750,7 @@ void step(fyzz_hip_world* w, float dt) {
         pcm::plane_all(w->ctx, n, w->pos.p, w->quat.p, w->half.p, w->floor.position, kContactDistance, w->cache_pl.p,
                        w->contacts_pl.p, w->counts_pl.p, w->flags_pl.p, sphere);
     }
-    const u32 native_count = w->native_order ? native_order_list(w, L) : 0u;
+    const u32 native_count = w->native_order ? native_order_list(w, L, c) : 0u;
     islands(w, L, c);
     if (w->native_order) native_check_sleep(w);
     // touching managers, in order
