@@ -53,11 +53,17 @@ struct Edge {
   bool live = false, retiring = false, connected = false;
 };
 
+// One entry of an island's contact array: the manager and its bodies (so
+// writing the solver order reads the array alone).
+struct Contact {
+  u32 id, a, b;
+};
+
 struct Island {
   u32 root = kNone, tail = kNone, count = 0;
   u32 active = kNone;            // index in the active island list
   bool live = false;
-  std::vector<u32> contacts;     // accurate graph only
+  std::vector<Contact> contacts; // accurate graph only
 };
 
 class Authority;
@@ -108,6 +114,7 @@ class Authority {
   Authority(u32 n, const uint8_t* dynamic) : n_(n), dyn_(dynamic, dynamic + n) {
     sleeping_.assign(n, 0);
     ready_.assign(n, 0);
+    by_pair_.reserve(8ull * n);  // broadphase pairs: a few per body
     for (int g = 0; g < 2; ++g) {
       graph_[g].kind = g;
       graph_[g].resize(n);
@@ -205,7 +212,7 @@ void Authority::append_contact(u32 id, u32 owner) {
   if (!isl.live || !e.connected || e.slot != kNone)
     throw std::logic_error("island contact insertion out of order");
   e.slot = static_cast<u32>(isl.contacts.size());
-  isl.contacts.push_back(id);
+  isl.contacts.push_back({id, e.a, e.b});
 }
 
 void Authority::remove_contact(u32 id) {
@@ -214,10 +221,10 @@ void Authority::remove_contact(u32 id) {
   const u32 owner = graph_[1].node_island[owner_node(e)];
   if (owner == kNone) return;
   auto& c = graph_[1].islands[owner].contacts;
-  if (e.slot >= c.size() || c[e.slot] != id) throw std::logic_error("island contact array corrupt");
-  const u32 last = c.back();
+  if (e.slot >= c.size() || c[e.slot].id != id) throw std::logic_error("island contact array corrupt");
+  const Contact last = c.back();
   c[e.slot] = last;
-  edges_[last].slot = e.slot;
+  edges_[last.id].slot = e.slot;
   c.pop_back();
   e.slot = kNone;
 }
@@ -288,9 +295,9 @@ bool Authority::merge(Graph& G, u32 pa, u32 pb, u32 edge) {
   w.tail = l.tail;
   w.count += l.count;
   if (G.kind == 1) {
-    for (u32 id : l.contacts) {
-      edges_[id].slot = static_cast<u32>(w.contacts.size());
-      w.contacts.push_back(id);
+    for (const Contact& c : l.contacts) {
+      edges_[c.id].slot = static_cast<u32>(w.contacts.size());
+      w.contacts.push_back(c);
     }
   }
   if (l.active != kNone) {
@@ -831,10 +838,10 @@ void Authority::publish(const fyzz_island_events& ev, fyzz_island_order& out) {
   std::size_t bodies = 0, rows = 0;
   for (u32 h : A.active_islands) {
     const Island& isl = A.islands[h];
-    for (u32 id : isl.contacts) {
+    for (const Contact& c : isl.contacts) {
       if (m < out.manager_capacity) {
-        out.managers[2 * m] = edges_[id].a;
-        out.managers[2 * m + 1] = edges_[id].b;
+        out.managers[2 * m] = c.a;
+        out.managers[2 * m + 1] = c.b;
       }
       ++m;
     }
