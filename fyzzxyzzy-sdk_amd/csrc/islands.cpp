@@ -24,7 +24,6 @@
 #include <new>
 #include <stdexcept>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "fyzz_hip.h"
@@ -68,6 +67,65 @@ struct Island {
 
 class Authority;
 
+// Broadphase pair -> manager slot: open addressing with linear probing and
+// backward-shift erase, kept at most half full. A key is (a << 32) | b with
+// a < b, so the all-ones key never names a pair and marks an empty cell.
+class PairMap {
+ public:
+  explicit PairMap(std::size_t expected) { rehash(expected * 2); }
+  u32 find(u64 k) const {
+    for (std::size_t i = home(k);; i = (i + 1) & mask_) {
+      if (cells_[i].key == k) return cells_[i].value;
+      if (cells_[i].key == kEmpty) return kNone;
+    }
+  }
+  void insert(u64 k, u32 v) {
+    if (2 * (size_ + 1) > cells_.size()) rehash(cells_.size() * 2);
+    std::size_t i = home(k);
+    while (cells_[i].key != kEmpty && cells_[i].key != k) i = (i + 1) & mask_;
+    if (cells_[i].key == kEmpty) ++size_;
+    cells_[i] = Cell{k, v};
+  }
+  void erase(u64 k) {
+    std::size_t i = home(k);
+    while (cells_[i].key != k) {
+      if (cells_[i].key == kEmpty) return;
+      i = (i + 1) & mask_;
+    }
+    // pull later entries of the probe run back over the hole
+    for (std::size_t j = (i + 1) & mask_;; j = (j + 1) & mask_) {
+      if (cells_[j].key == kEmpty) break;
+      const std::size_t h = home(cells_[j].key);
+      if (((j - h) & mask_) >= ((j - i) & mask_)) {
+        cells_[i] = cells_[j];
+        i = j;
+      }
+    }
+    cells_[i].key = kEmpty;
+    --size_;
+  }
+
+ private:
+  static constexpr u64 kEmpty = ~u64(0);
+  struct Cell {
+    u64 key = kEmpty;
+    u32 value = kNone;
+  };
+  std::size_t home(u64 k) const { return std::size_t((k * 0x9E3779B97F4A7C15ull) >> 20) & mask_; }
+  void rehash(std::size_t want) {
+    std::size_t cap = 1024;
+    while (cap < want) cap *= 2;
+    std::vector<Cell> old(cap);
+    old.swap(cells_);
+    mask_ = cap - 1;
+    size_ = 0;
+    for (const Cell& c : old)
+      if (c.key != kEmpty) insert(c.key, c.value);
+  }
+  std::vector<Cell> cells_;
+  std::size_t mask_ = 0, size_ = 0;
+};
+
 // One island graph (0: speculative, over broadphase pairs; 1: accurate,
 // over touching managers).
 struct Graph {
@@ -84,10 +142,16 @@ struct Graph {
   std::vector<u32> activating;
   std::vector<u32> dirty;
   std::vector<uint8_t> dirty_mark;
-  // route search scratch
-  std::vector<u32> confirmed, seen, pred, pred_edge;
+  // route search scratch: one record per node, so a search reads one
+  // line per node it reaches; heap entries carry their node's hop estimate
+  // (fixed while a search runs) above the node
+  struct Mark {
+    u32 seen = 0, pred = kNone, pred_edge = kNone, confirmed = 0;
+  };
+  std::vector<Mark> mark;
   u32 confirm_stamp = 0, visit_stamp = 0;
-  std::vector<u32> heap, members, split_edges;
+  std::vector<u64> heap;
+  std::vector<u32> members, split_edges;
   std::vector<u32> island_seen, expand_seen;
   u32 island_stamp = 0, expand_stamp = 0;
 
@@ -102,19 +166,15 @@ struct Graph {
     hint.assign(n, 0);
     active_index.assign(n, kNone);
     dirty_mark.assign(n, 0);
-    confirmed.assign(n, 0);
-    seen.assign(n, 0);
-    pred.assign(n, kNone);
-    pred_edge.assign(n, kNone);
+    mark.assign(n, Mark{});
   }
 };
 
 class Authority {
  public:
-  Authority(u32 n, const uint8_t* dynamic) : n_(n), dyn_(dynamic, dynamic + n) {
+  Authority(u32 n, const uint8_t* dynamic) : n_(n), dyn_(dynamic, dynamic + n), by_pair_(4ull * n) {
     sleeping_.assign(n, 0);
     ready_.assign(n, 0);
-    by_pair_.reserve(8ull * n);  // broadphase pairs: a few per body
     for (int g = 0; g < 2; ++g) {
       graph_[g].kind = g;
       graph_[g].resize(n);
@@ -127,8 +187,7 @@ class Authority {
  private:
   bool dynamic(u32 node) const { return node < n_ && dyn_[node]; }
   u32 edge_of(u32 a, u32 b) const {
-    auto it = by_pair_.find(key(a, b));
-    return it == by_pair_.end() ? kNone : it->second;
+    return by_pair_.find(key(a, b));
   }
   u32 owner_node(const Edge& e) const { return dynamic(e.a) ? e.a : e.b; }
 
@@ -154,7 +213,7 @@ class Authority {
   std::vector<uint8_t> dyn_, sleeping_, ready_;
   std::vector<Edge> edges_;
   std::vector<u32> free_edges_, retired_edges_;
-  std::unordered_map<u64, u32> by_pair_;
+  PairMap by_pair_;  // broadphase pairs: a few per body
   Graph graph_[2];
   bool admitted_ = false;
 };
@@ -358,29 +417,30 @@ void Authority::rebuild_chains(Graph& G) {
 // search visits nodes in).
 void Authority::heap_push(Graph& G, u32 node) {
   auto& h = G.heap;
+  const u64 entry = (u64(G.hops[node]) << 32) | node;
   std::size_t i = h.size();
-  h.push_back(node);
+  h.push_back(entry);
   while (i) {
     const std::size_t p = (i - 1) / 2;
-    if (!(G.hops[node] < G.hops[h[p]])) break;
+    if (!((entry >> 32) < (h[p] >> 32))) break;
     h[i] = h[p];
     i = p;
   }
-  h[i] = node;
+  h[i] = entry;
 }
 
 u32 Authority::heap_pop(Graph& G) {
   auto& h = G.heap;
-  const u32 top = h.front();
-  const u32 last = h.back();
+  const u32 top = static_cast<u32>(h.front());
+  const u64 last = h.back();
   h.pop_back();
   const std::size_t n = h.size();
   std::size_t i = 0;
   for (;;) {
     std::size_t c = 2 * i + 1;
     if (c >= n) break;
-    if (c + 1 < n && G.hops[h[c + 1]] < G.hops[h[c]]) ++c;
-    if (G.hops[last] < G.hops[h[c]]) break;
+    if (c + 1 < n && (h[c + 1] >> 32) < (h[c] >> 32)) ++c;
+    if ((last >> 32) < (h[c] >> 32)) break;
     h[i] = h[c];
     i = c;
   }
@@ -404,17 +464,17 @@ bool Authority::repair(Graph& G) {
   if (G.dirty.empty()) return false;
   std::sort(G.dirty.begin(), G.dirty.end());
   if (++G.confirm_stamp == 0) {
-    std::fill(G.confirmed.begin(), G.confirmed.end(), 0);
+    for (auto& m : G.mark) m.confirmed = 0;
     G.confirm_stamp = 1;
   }
   const u32 cs = G.confirm_stamp;
   bool split_any = false;
   for (u32 node : G.dirty) {
-    if (!dynamic(node) || G.confirmed[node] == cs) continue;
+    if (!dynamic(node) || G.mark[node].confirmed == cs) continue;
     const u32 isl = G.node_island[node];
     const u32 root = G.islands[isl].root;
     if (root == node) {
-      G.confirmed[node] = cs;
+      G.mark[node].confirmed = cs;
       continue;
     }
     // cached route: strictly decreasing hops over live edges
@@ -422,7 +482,7 @@ bool Authority::repair(Graph& G) {
     bool ok = false;
     G.members.clear();
     for (u32 depth = 0; depth < n_; ++depth) {
-      if (probe == root || (G.confirmed[probe] == cs && G.node_island[probe] == isl)) {
+      if (probe == root || (G.mark[probe].confirmed == cs && G.node_island[probe] == isl)) {
         ok = true;
         break;
       }
@@ -435,12 +495,12 @@ bool Authority::repair(Graph& G) {
       probe = next;
     }
     if (ok) {
-      for (u32 v : G.members) G.confirmed[v] = cs;
+      for (u32 v : G.members) G.mark[v].confirmed = cs;
       continue;
     }
     // hop-ordered search; members are recorded when first reached
     if (++G.visit_stamp == 0) {
-      std::fill(G.seen.begin(), G.seen.end(), 0);
+      for (auto& m : G.mark) m.seen = 0;
       G.visit_stamp = 1;
     }
     const u32 vs = G.visit_stamp;
@@ -448,9 +508,9 @@ bool Authority::repair(Graph& G) {
     G.members.clear();
     G.members.push_back(node);
     heap_push(G, node);
-    G.seen[node] = vs;
-    G.pred[node] = kNone;
-    G.pred_edge[node] = kNone;
+    G.mark[node].seen = vs;
+    G.mark[node].pred = kNone;
+    G.mark[node].pred_edge = kNone;
     bool found = false;
     u32 witness = root;
     while (!G.heap.empty() && !found) {
@@ -460,11 +520,11 @@ bool Authority::repair(Graph& G) {
         const u32 to = (i & 1) ? e.a : e.b;
         const u32 eid = i / 2;
         i = e.link[G.kind][i & 1].next;
-        if (!dynamic(to) || G.seen[to] == vs) continue;
-        G.seen[to] = vs;
-        G.pred[to] = from;
-        G.pred_edge[to] = eid;
-        if (to == root || (G.confirmed[to] == cs && G.node_island[to] == isl)) {
+        if (!dynamic(to) || G.mark[to].seen == vs) continue;
+        G.mark[to].seen = vs;
+        G.mark[to].pred = from;
+        G.mark[to].pred_edge = eid;
+        if (to == root || (G.mark[to].confirmed == cs && G.node_island[to] == isl)) {
           found = true;
           witness = to;
           break;
@@ -475,7 +535,7 @@ bool Authority::repair(Graph& G) {
     }
     if (!found) {
       split(G, node);
-      for (u32 v : G.members) G.confirmed[v] = cs;
+      for (u32 v : G.members) G.mark[v].confirmed = cs;
       split_any = true;
       continue;
     }
@@ -486,23 +546,23 @@ bool Authority::repair(Graph& G) {
       G.hint[root] = 1;
     }
     for (u32 cur = witness; cur != node;) {
-      const u32 p = G.pred[cur];
+      const u32 p = G.mark[cur].pred;
       G.fast[p] = cur;
-      G.fast_edge[p] = G.pred_edge[cur];
+      G.fast_edge[p] = G.mark[cur].pred_edge;
       G.hops[p] = G.hops[cur] + 1;
       G.hint[p] = 1;
-      G.confirmed[p] = cs;
+      G.mark[p].confirmed = cs;
       cur = p;
     }
     for (u32 v : G.members) {
-      if (G.confirmed[v] == cs) continue;
-      const u32 p = G.pred[v];
+      if (G.mark[v].confirmed == cs) continue;
+      const u32 p = G.mark[v].pred;
       if (p == kNone) throw std::logic_error("route search lost a parent");
       G.fast[v] = p;
-      G.fast_edge[v] = G.pred_edge[v];
+      G.fast_edge[v] = G.mark[v].pred_edge;
       G.hops[v] = G.hops[p] + 1;
       G.hint[v] = 1;
-      G.confirmed[v] = cs;
+      G.mark[v].confirmed = cs;
     }
   }
   for (u32 v : G.dirty) G.dirty_mark[v] = 0;
@@ -565,9 +625,9 @@ void Authority::split(Graph& G, u32 seed) {
     G.node_prev[node] = k ? G.members[k - 1] : kNone;
     G.node_next[node] = k + 1 < G.members.size() ? G.members[k + 1] : kNone;
     G.node_island[node] = created;
-    G.fast[node] = k ? G.pred[node] : kNone;
-    G.fast_edge[node] = k ? G.pred_edge[node] : kNone;
-    G.hops[node] = k ? G.hops[G.pred[node]] + 1 : 0;
+    G.fast[node] = k ? G.mark[node].pred : kNone;
+    G.fast_edge[node] = k ? G.mark[node].pred_edge : kNone;
+    G.hops[node] = k ? G.hops[G.mark[node].pred] + 1 : 0;
     G.hint[node] = 1;
   }
   for (u32 eid : G.split_edges) append_contact(eid, created);
@@ -776,7 +836,7 @@ void Authority::publish(const fyzz_island_events& ev, fyzz_island_order& out) {
     edges_[id].a = a;
     edges_[id].b = b;
     edges_[id].live = true;
-    by_pair_[key(a, b)] = id;
+    by_pair_.insert(key(a, b), id);
     link(0, id);
   }
   // speculative connectivity (scene.cpp:5841-5898)
